@@ -1,0 +1,313 @@
+"""CPU oracle for the SIMPLE pressure-velocity path -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes wrapper over ``liboracle.so`` (built from ``cfd_oracle.cpp`` by
+``oracle/Makefile``), which restates the reference's CPU algorithm with the same
+floating-point operation order. Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg use this module, and only as the checker; the
+product package ``cfdapp_amd`` never imports it.
+
+Parity of the restatement is pinned by ``tests/test_oracle_*.py``: bitwise
+against vectors written by the reference compiled from its own sources
+(``oracle/Makefile.ref``), which is in turn bitwise-equal to the reference's
+committed ``cases/lid_driven_cavity/results`` fields and residual history.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+P_F64 = ctypes.POINTER(ctypes.c_double)
+P_I64 = ctypes.POINTER(ctypes.c_int64)
+
+# Boundary-condition kinds (oracle and C ABI share these codes).
+WALL, MOVING_WALL, INLET, OUTLET = 0, 1, 2, 3
+FIXED_GRADIENT, FIXED_VALUE = 0, 1
+
+# SolverStatus (include/cfd/algebra/LinearSolver.hpp:16-23) order.
+STATUS_NAMES = ["Converged", "MaxIterations", "Breakdown", "NonFiniteInput",
+                "NonFiniteResidual", "InvalidSystem"]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        _LIB = ctypes.CDLL(path)
+        _LIB.orc_case_create.restype = ctypes.c_void_p
+        _LIB.orc_case_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double]
+        _LIB.orc_case_destroy.argtypes = [ctypes.c_void_p]
+        _LIB.orc_csr_build.restype = ctypes.c_int64
+        _LIB.orc_momentum.restype = ctypes.c_int64
+        _LIB.orc_pressure_assemble.restype = ctypes.c_int64
+        _LIB.orc_case_set_bc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double]
+    return _LIB
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(P_F64)
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(P_I64)
+    raise TypeError(a.dtype)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+@dataclass
+class SolveResult:
+    x: np.ndarray
+    status: int
+    iterations: int
+    restarts: int
+    initial: float
+    final: float
+    history: np.ndarray
+
+    @property
+    def converged(self):
+        return self.status == 0
+
+
+class Csr:
+    def __init__(self, rowptr, col, val):
+        self.rowptr = i64(rowptr)
+        self.col = i64(col)
+        self.val = f64(val)
+        self.n = len(self.rowptr) - 1
+
+    def spmv(self, x):
+        y = np.empty(self.n)
+        lib().orc_spmv(ctypes.c_int64(self.n), _p(self.rowptr), _p(self.col), _p(self.val),
+                       _p(f64(x)), _p(y))
+        return y
+
+    def diagonal(self):
+        d = np.zeros(self.n)
+        for r in range(self.n):
+            s, e = self.rowptr[r], self.rowptr[r + 1]
+            hit = np.nonzero(self.col[s:e] == r)[0]
+            if len(hit):
+                d[r] = self.val[s + hit[0]]
+        return d
+
+    def dense(self):
+        a = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            for k in range(self.rowptr[r], self.rowptr[r + 1]):
+                a[r, self.col[k]] = self.val[k]
+        return a
+
+    def _solve(self, fn, b, x0, abs_tol, rel_tol, max_iter, jacobi):
+        n = self.n
+        x = np.empty(n)
+        meta = np.zeros(6)
+        cap = int(max_iter) + 2
+        hist = np.zeros(cap)
+        s = f64([abs_tol, rel_tol, max_iter, 1.0 if jacobi else 0.0])
+        x0 = np.zeros(n) if x0 is None else f64(x0)
+        fn(ctypes.c_int64(n), _p(self.rowptr), _p(self.col), _p(self.val), _p(f64(b)), _p(x0),
+           _p(s), _p(x), _p(meta), _p(hist), ctypes.c_int64(cap))
+        hl = int(meta[5])
+        return SolveResult(x, int(meta[0]), int(meta[1]), int(meta[2]), meta[3], meta[4],
+                           hist[:min(hl, cap)].copy())
+
+    def cg(self, b, x0=None, abs_tol=0.0, rel_tol=1e-8, max_iter=20000, jacobi=True):
+        return self._solve(lib().orc_cg, b, x0, abs_tol, rel_tol, max_iter, jacobi)
+
+    def bicgstab(self, b, x0=None, abs_tol=1e-10, rel_tol=1e-8, max_iter=500, jacobi=False):
+        return self._solve(lib().orc_bicgstab, b, x0, abs_tol, rel_tol, max_iter, jacobi)
+
+
+def csr_from_triplets(n, rows, cols, vals):
+    rows, cols, vals = i64(rows), i64(cols), f64(vals)
+    rp = np.zeros(n + 1, np.int64)
+    col = np.zeros(len(vals), np.int64)
+    val = np.zeros(len(vals))
+    nnz = lib().orc_csr_build(ctypes.c_int64(n), ctypes.c_int64(len(vals)), _p(rows), _p(cols),
+                              _p(vals), _p(rp), _p(col), _p(val))
+    return Csr(rp, col[:nnz], val[:nnz])
+
+
+class Case:
+    """A Cartesian cavity/channel case: mesh + per-patch boundary conditions."""
+
+    def __init__(self, dim, nx, ny, nz=1, lx=1.0, ly=1.0, lz=1.0):
+        self.dim = dim
+        self.nx, self.ny, self.nz = nx, ny, (nz if dim == 3 else 1)
+        self.h = lib().orc_case_create(dim, nx, ny, self.nz, lx, ly, lz)
+        sizes = np.zeros(4, np.int64)
+        lib().orc_case_sizes(ctypes.c_void_p(self.h), _p(sizes))
+        self.n_cells, self.n_faces, self.n_patches, self.n_cf = (int(v) for v in sizes)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_case_destroy(ctypes.c_void_p(self.h))
+            self.h = None
+
+    def set_bc(self, patch, vkind, vvalue=(0.0, 0.0, 0.0), pkind=FIXED_GRADIENT, pvalue=0.0):
+        lib().orc_case_set_bc(ctypes.c_void_p(self.h), patch, vkind, *map(float, vvalue), pkind,
+                              float(pvalue))
+
+    @classmethod
+    def cavity(cls, dim, nx, ny, nz=1, lid=1.0):
+        c = cls(dim, nx, ny, nz)
+        lid_patch = 3  # "top" in 2D, "ymax" in 3D
+        for p in range(c.n_patches):
+            if p == lid_patch:
+                c.set_bc(p, MOVING_WALL, (lid, 0.0, 0.0))
+            else:
+                c.set_bc(p, WALL)
+        return c
+
+    def mesh_arrays(self):
+        nf, nc = self.n_faces, self.n_cells
+        out = dict(owner=np.zeros(nf, np.int64), neighbor=np.zeros(nf, np.int64),
+                   fc=np.zeros(3 * nf), sf=np.zeros(3 * nf), cc=np.zeros(3 * nc),
+                   vol=np.zeros(nc), cf_off=np.zeros(nc + 1, np.int64),
+                   cf=np.zeros(self.n_cf, np.int64), patch=np.zeros(nf, np.int64))
+        lib().orc_mesh_arrays(ctypes.c_void_p(self.h), *(_p(out[k]) for k in
+                              ["owner", "neighbor", "fc", "sf", "cc", "vol", "cf_off", "cf", "patch"]))
+        for k in ("fc", "sf", "cc"):
+            out[k] = out[k].reshape(-1, 3)
+        return out
+
+    def mesh_flags(self):
+        out = np.zeros(3, np.int64)
+        lib().orc_mesh_flags(ctypes.c_void_p(self.h), _p(out))
+        return dict(boundary_transfer=int(out[0]), oblique_faces=int(out[1]),
+                    skewed_faces=int(out[2]))
+
+    def _u(self, u):
+        u = [f64(c) for c in u]
+        while len(u) < 3:
+            u.append(np.zeros(self.n_cells))
+        return u
+
+    def mass_flux(self, u, rho=1.0):
+        ux, uy, uz = self._u(u)
+        F = np.zeros(self.n_faces)
+        lib().orc_mass_flux(ctypes.c_void_p(self.h), _p(ux), _p(uy), _p(uz), ctypes.c_double(rho),
+                            _p(F))
+        return F
+
+    def gradient(self, phi, correction=False):
+        g = [np.zeros(self.n_cells) for _ in range(3)]
+        lib().orc_gradient(ctypes.c_void_p(self.h), _p(f64(phi)), 1 if correction else 0,
+                           *(_p(a) for a in g))
+        return g
+
+    def velocity_gradient(self, u):
+        ux, uy, uz = self._u(u)
+        n = self.n_cells
+        gu, gv, gw = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        lib().orc_velocity_gradient(ctypes.c_void_p(self.h), _p(ux), _p(uy), _p(uz), _p(gu),
+                                    _p(gv), _p(gw))
+        return gu.reshape(3, n), gv.reshape(3, n), gw.reshape(3, n)
+
+    def momentum(self, component, u, p, F, mu, prev, alpha):
+        ux, uy, uz = self._u(u)
+        n = self.n_cells
+        cap = 7 * n + 16
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(cap, np.int64)
+        val = np.zeros(cap)
+        rhs, diag = np.zeros(n), np.zeros(n)
+        nnz = lib().orc_momentum(ctypes.c_void_p(self.h), component, _p(ux), _p(uy), _p(uz),
+                                 _p(f64(p)), _p(f64(F)), ctypes.c_double(mu), _p(f64(prev)),
+                                 ctypes.c_double(alpha), _p(rp), _p(col), _p(val), _p(rhs),
+                                 _p(diag))
+        if nnz < 0:
+            raise FloatingPointError("momentum system is not finite")
+        return Csr(rp, col[:nnz], val[:nnz]), rhs, diag
+
+    def response(self, diag):
+        d = np.zeros(self.n_cells)
+        lib().orc_response(ctypes.c_void_p(self.h), _p(f64(diag)), _p(d))
+        return d
+
+    def rhie_chow(self, u, p, du, dv, dw=None, rho=1.0, alpha=0.7):
+        ux, uy, uz = self._u(u)
+        dw = np.zeros(self.n_cells) if dw is None else f64(dw)
+        F = np.zeros(self.n_faces)
+        lib().orc_rhie_chow(ctypes.c_void_p(self.h), _p(ux), _p(uy), _p(uz), _p(f64(p)),
+                            _p(f64(du)), _p(f64(dv)), _p(dw), ctypes.c_double(rho),
+                            ctypes.c_double(alpha), _p(F))
+        return F
+
+    def pressure_system(self, Fstar, du, dv, dw=None, rho=1.0, ref=0):
+        n = self.n_cells
+        cap = 7 * n + 16
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(cap, np.int64)
+        val = np.zeros(cap)
+        rhs, coeff = np.zeros(n), np.zeros(self.n_faces)
+        dw = np.zeros(n) if dw is None else f64(dw)
+        nnz = lib().orc_pressure_assemble(ctypes.c_void_p(self.h), _p(f64(Fstar)), _p(f64(du)),
+                                          _p(f64(dv)), _p(dw), ctypes.c_double(rho),
+                                          ctypes.c_int64(ref), _p(rp), _p(col), _p(val), _p(rhs),
+                                          _p(coeff))
+        return Csr(rp, col[:nnz], val[:nnz]), rhs, coeff
+
+    def correct(self, ustar, du, dv, dw, pp, Fstar, coeff):
+        usx, usy, usz = self._u(ustar)
+        n = self.n_cells
+        dw = np.zeros(n) if dw is None else f64(dw)
+        ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+        F = np.zeros(self.n_faces)
+        lib().orc_correct(ctypes.c_void_p(self.h), _p(usx), _p(usy), _p(usz), _p(f64(du)),
+                          _p(f64(dv)), _p(dw), _p(f64(pp)), _p(f64(Fstar)), _p(f64(coeff)),
+                          _p(ux), _p(uy), _p(uz), _p(F))
+        return (ux, uy, uz)[:self.dim], F
+
+    def continuity(self, F):
+        imb = np.zeros(self.n_cells)
+        sc = np.zeros(2)
+        lib().orc_continuity(ctypes.c_void_p(self.h), _p(f64(F)), _p(imb), _p(sc))
+        return imb, float(sc[0]), float(sc[1])
+
+    def simple(self, iters, re=100.0, lid=1.0, rhie_chow=True, pressure="cg_jacobi",
+               au=0.7, ap=0.3, vtol=1e-6, ptol=1e-6, ctol=1e-6, mom=(1e-10, 1e-8, 500),
+               pres=None, u0=None, p0=None):
+        n = self.n_cells
+        if pres is None:
+            pres = (0.0, 1e-8, 20000) if pressure == "cg_jacobi" else (1e-10, 1e-8, 2000)
+        s = f64([iters, au, ap, vtol, ptol, ctol, 1.0, lid / re, 0, 1.0 if rhie_chow else 0.0,
+                 0.0 if pressure == "bicgstab" else 1.0, mom[0], mom[1], mom[2], 0.0,
+                 pres[0], pres[1], pres[2], 1.0 if pressure == "cg_jacobi" else 0.0])
+        u0 = self._u(u0 if u0 is not None else [])
+        p0 = np.zeros(n) if p0 is None else f64(p0)
+        ux, uy, uz, p = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        F = np.zeros(self.n_faces)
+        meta = np.zeros(6)
+        cap = int(iters)
+        hist = np.zeros(5 * cap)
+        lib().orc_simple(ctypes.c_void_p(self.h), _p(s), _p(u0[0]), _p(u0[1]), _p(u0[2]), _p(p0),
+                         _p(ux), _p(uy), _p(uz), _p(p), _p(F), _p(meta), _p(hist),
+                         ctypes.c_int64(cap))
+        it = int(meta[1])
+        h = hist.reshape(5, cap)[:, :it]
+        return dict(u=(ux, uy, uz)[:self.dim], p=p, F=F, status=int(meta[0]), iterations=it,
+                    momentum_iterations=int(meta[2]), pressure_iterations=int(meta[3]),
+                    final_continuity=meta[4], global_imbalance=meta[5],
+                    hist=dict(u=h[0], v=h[1], w=h[2] if self.dim == 3 else h[2][:0], p=h[3],
+                              c=h[4]))

@@ -1,0 +1,1615 @@
+// CPU oracle for the SIMPLE pressure-velocity path -- TEST INFRASTRUCTURE ONLY.
+//
+// A restatement, in plain C++ with the same floating-point operation order, of
+// the reference's CPU path on structured Cartesian meshes. It is the checker
+// for the HIP path and the CPU timing baseline; nothing in the product
+// (cfdapp_amd/) includes, links or calls it. Build: oracle/Makefile
+// (-O2 -ffp-contract=off, OpenMP on the SpMV row loop only, as the reference).
+//
+// Every function cites the reference function it follows. Parity of each is
+// pinned by tests/test_oracle_*.py against vectors that the compiled
+// reference (oracle/Makefile.ref -> oracle/_ref) wrote, and the compiled
+// reference is itself pinned against the reference's committed cavity
+// results (cases/lid_driven_cavity/results/*.csv, bitwise).
+//
+// Scope: Cartesian 2D/3D meshes, Wall/MovingWall/Inlet/Outlet velocity and
+// FixedGradient/FixedValue pressure conditions, upwind convection,
+// Green-Gauss gradients, no non-orthogonal corrector passes, laminar model.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+namespace orc {
+
+using Index = int64_t;
+
+// include/cfd/core/Vector3.hpp:20-91 -- component-wise ops; dot skips an exactly
+// zero z product (Vector3.hpp:78-82).
+struct V3 {
+  double x = 0.0, y = 0.0, z = 0.0;
+};
+inline V3 operator+(V3 a, const V3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 v, double s) { return {v.x * s, v.y * s, v.z * s}; }
+inline double dot(const V3& a, const V3& b) {
+  const double inPlane = (a.x * b.x) + (a.y * b.y);
+  const double normal = a.z * b.z;
+  return normal == 0.0 ? inPlane : inPlane + normal;
+}
+inline V3 cross(const V3& a, const V3& b) {
+  return {(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)};
+}
+inline double magnitude(const V3& v) { return std::sqrt(dot(v, v)); }
+inline bool isZero(const V3& v) { return v.x == 0.0 && v.y == 0.0 && v.z == 0.0; }
+
+// ---------------------------------------------------------------------------
+// Mesh: src/mesh/MeshGeometry.cpp:428-553 (2D), :1047-1160 (3D);
+// AxisSpacing::uniform/node/center/width: src/mesh/MeshGrading.cpp:125-172.
+struct Mesh {
+  int dim = 2;
+  Index nx = 0, ny = 0, nz = 1;
+  std::vector<V3> cc;
+  std::vector<double> vol;
+  std::vector<Index> owner, neigh;  // neigh = -1 on a boundary face
+  std::vector<V3> fc, sf;
+  std::vector<Index> cfOff, cf;       // cell face lists in creation order
+  std::vector<int> facePatch;         // patch of a boundary face, -1 internal
+  std::vector<std::vector<Index>> patches;
+  Index nCells() const { return static_cast<Index>(vol.size()); }
+  Index nFaces() const { return static_cast<Index>(owner.size()); }
+  bool boundary(Index f) const { return neigh[f] < 0; }
+  double area(Index f) const { return magnitude(sf[f]); }
+};
+
+struct Axis {
+  Index n;
+  double delta;
+  Axis(Index cells, double length) : n(cells), delta(length / static_cast<double>(cells)) {}
+  double node(Index i) const { return static_cast<double>(i) * delta; }
+  double center(Index i) const { return (static_cast<double>(i) + 0.5) * delta; }
+  double width(Index) const { return delta; }
+};
+
+static void finishCellFaces(Mesh& m, const std::vector<std::vector<Index>>& lists) {
+  m.cfOff.assign(1, 0);
+  m.cf.clear();
+  for (const auto& l : lists) {
+    m.cf.insert(m.cf.end(), l.begin(), l.end());
+    m.cfOff.push_back(static_cast<Index>(m.cf.size()));
+  }
+}
+
+static void addFace(Mesh& m, std::vector<std::vector<Index>>& lists, Index owner, Index neigh,
+                    V3 c, V3 s, int patch) {
+  const Index id = m.nFaces();
+  m.owner.push_back(owner);
+  m.neigh.push_back(neigh);
+  m.fc.push_back(c);
+  m.sf.push_back(s);
+  m.facePatch.push_back(patch);
+  lists[owner].push_back(id);
+  if (neigh >= 0) lists[neigh].push_back(id);
+  if (patch >= 0) m.patches[patch].push_back(id);
+}
+
+// createCartesian2D -> createRectilinear2D -> buildStructuredMesh
+// (src/mesh/MeshGeometry.cpp:428-521): cell(i,j) = j*nx+i; vertical faces
+// (j outer, i = 0..nx) then horizontal faces (j = 0..ny outer, i inner);
+// patches left, right, bottom, top.
+Mesh cartesian2D(Index nx, Index ny, double lx, double ly) {
+  Mesh m;
+  m.dim = 2;
+  m.nx = nx;
+  m.ny = ny;
+  m.nz = 1;
+  const Axis x(nx, lx), y(ny, ly);
+  for (Index j = 0; j < ny; ++j)
+    for (Index i = 0; i < nx; ++i) {
+      m.cc.push_back(V3{x.center(i), y.center(j), 0.0});
+      m.vol.push_back(x.width(i) * y.width(j));
+    }
+  m.patches.assign(4, {});
+  std::vector<std::vector<Index>> lists(nx * ny);
+  const auto cell = [nx](Index i, Index j) { return (j * nx) + i; };
+  for (Index j = 0; j < ny; ++j)
+    for (Index i = 0; i <= nx; ++i) {
+      const double h = y.width(j);
+      const V3 c{x.node(i), y.center(j), 0.0};
+      if (i == 0) addFace(m, lists, cell(0, j), -1, c, V3{-h, 0.0, 0.0}, 0);
+      else if (i == nx) addFace(m, lists, cell(nx - 1, j), -1, c, V3{h, 0.0, 0.0}, 1);
+      else addFace(m, lists, cell(i - 1, j), cell(i, j), c, V3{h, 0.0, 0.0}, -1);
+    }
+  for (Index j = 0; j <= ny; ++j)
+    for (Index i = 0; i < nx; ++i) {
+      const double w = x.width(i);
+      const V3 c{x.center(i), y.node(j), 0.0};
+      if (j == 0) addFace(m, lists, cell(i, 0), -1, c, V3{0.0, -w, 0.0}, 2);
+      else if (j == ny) addFace(m, lists, cell(i, ny - 1), -1, c, V3{0.0, w, 0.0}, 3);
+      else addFace(m, lists, cell(i, j - 1), cell(i, j), c, V3{0.0, w, 0.0}, -1);
+    }
+  finishCellFaces(m, lists);
+  return m;
+}
+
+// createCartesian3D (src/mesh/MeshGeometry.cpp:1047-1160): cell = ((k*ny)+j)*nx+i;
+// x-faces, then y-faces, then z-faces, k outer; a low-side boundary face keeps
+// the negated positive area vector (-A, -0.0, -0.0); patches xmin..zmax.
+Mesh cartesian3D(Index nx, Index ny, Index nz, double lx, double ly, double lz) {
+  Mesh m;
+  m.dim = 3;
+  m.nx = nx;
+  m.ny = ny;
+  m.nz = nz;
+  const Axis x(nx, lx), y(ny, ly), z(nz, lz);
+  const double dx = x.width(0), dy = y.width(0), dz = z.width(0);
+  const double volume = (dx * dy) * dz;
+  const double ax = dy * dz, ay = dx * dz, az = dx * dy;
+  const auto cell = [nx, ny](Index i, Index j, Index k) { return (((k * ny) + j) * nx) + i; };
+  for (Index k = 0; k < nz; ++k)
+    for (Index j = 0; j < ny; ++j)
+      for (Index i = 0; i < nx; ++i) {
+        m.cc.push_back(V3{0.0 + x.center(i), 0.0 + y.center(j), 0.0 + z.center(k)});
+        m.vol.push_back(volume);
+      }
+  m.patches.assign(6, {});
+  std::vector<std::vector<Index>> lists(nx * ny * nz);
+  const auto face = [&](V3 c, V3 pos, bool atLow, bool atHigh, Index low, Index high, int lowPatch) {
+    if (atLow) addFace(m, lists, high, -1, c, V3{-pos.x, -pos.y, -pos.z}, lowPatch);
+    else if (atHigh) addFace(m, lists, low, -1, c, pos, lowPatch + 1);
+    else addFace(m, lists, low, high, c, pos, -1);
+  };
+  for (Index k = 0; k < nz; ++k)
+    for (Index j = 0; j < ny; ++j)
+      for (Index i = 0; i <= nx; ++i)
+        face(V3{0.0 + x.node(i), 0.0 + y.center(j), 0.0 + z.center(k)}, V3{ax, 0.0, 0.0}, i == 0,
+             i == nx, i == 0 ? 0 : cell(i - 1, j, k), i == nx ? 0 : cell(i, j, k), 0);
+  for (Index k = 0; k < nz; ++k)
+    for (Index j = 0; j <= ny; ++j)
+      for (Index i = 0; i < nx; ++i)
+        face(V3{0.0 + x.center(i), 0.0 + y.node(j), 0.0 + z.center(k)}, V3{0.0, ay, 0.0}, j == 0,
+             j == ny, j == 0 ? 0 : cell(i, j - 1, k), j == ny ? 0 : cell(i, j, k), 2);
+  for (Index k = 0; k <= nz; ++k)
+    for (Index j = 0; j < ny; ++j)
+      for (Index i = 0; i < nx; ++i)
+        face(V3{0.0 + x.center(i), 0.0 + y.center(j), 0.0 + z.node(k)}, V3{0.0, 0.0, az}, k == 0,
+             k == nz, k == 0 ? 0 : cell(i, j, k - 1), k == nz ? 0 : cell(i, j, k), 4);
+  finishCellFaces(m, lists);
+  return m;
+}
+
+// Geometry helpers: src/mesh/MeshGeometry.cpp:20-284.
+inline double distance(const V3& a, const V3& b) { return magnitude(b - a); }
+inline V3 unitNormal(const Mesh& m, Index f) { return m.sf[f] * (1.0 / m.area(f)); }
+inline bool wellPosed(const V3& d, const V3& s) {  // :99-107
+  const double dDotSf = dot(d, s);
+  const double dMag = magnitude(d), sMag = magnitude(s);
+  if (!(dMag > 0.0) || !(sMag > 0.0)) return false;
+  return dDotSf > (1e-6 * dMag * sMag);
+}
+
+// MeshGeometry::oppositeInteriorFace (:47-96): the most anti-parallel interior
+// face of `cell`, -1 when none is anti-parallel.
+Index oppositeInteriorFace(const Mesh& m, Index cell, Index ref) {
+  const V3 raw = unitNormal(m, ref);
+  const V3 outRef = (m.owner[ref] == cell) ? raw : (raw * -1.0);
+  Index best = -1;
+  double bestDot = 0.0;
+  for (Index k = m.cfOff[cell]; k < m.cfOff[cell + 1]; ++k) {
+    const Index f = m.cf[k];
+    if (f == ref || m.boundary(f)) continue;
+    const V3 rn = unitNormal(m, f);
+    const V3 out = (m.owner[f] == cell) ? rn : (rn * -1.0);
+    const double a = dot(outRef, out);
+    if (a < 0.0 && (best < 0 || a < bestDot)) {
+      bestDot = a;
+      best = f;
+    }
+  }
+  return best;
+}
+
+struct Crossing {
+  bool valid = false;
+  double t = 0.0;
+  V3 point, skew;
+};
+// MeshGeometry::ownerNeighborCrossing (:169-199).
+Crossing ownerNeighborCrossing(const Mesh& m, Index f) {
+  const V3& co = m.cc[m.owner[f]];
+  const V3 d = m.cc[m.neigh[f]] - co;
+  const V3& s = m.sf[f];
+  if (!wellPosed(d, s)) return {};
+  const V3 toFace = m.fc[f] - co;
+  if (isZero(cross(toFace, d))) return {true, dot(toFace, d) / dot(d, d), m.fc[f], V3{0, 0, 0}};
+  const double t = dot(m.fc[f] - co, s) / dot(d, s);
+  const V3 p = co + (d * t);
+  return {true, t, p, m.fc[f] - p};
+}
+
+struct LineHit {
+  bool valid = false;
+  double distance = 0.0;
+  V3 point;
+};
+// MeshGeometry::boundaryLineIntersection (:201-224).
+LineHit boundaryLineIntersection(const Mesh& m, Index bf, const V3& dir) {
+  const V3& co = m.cc[m.owner[bf]];
+  const V3& s = m.sf[bf];
+  if (!wellPosed(dir * -1.0, s)) return {};
+  const double dist = dot(co - m.fc[bf], s) / dot(dir, s);
+  if (!(dist > 0.0)) return {};
+  return {true, dist, co - (dir * dist)};
+}
+
+struct InwardStencil {
+  bool valid = false;
+  double h1 = 0.0, h2 = 0.0;
+  Index far = 0;
+  V3 deltaP, deltaF;
+};
+// MeshGeometry::boundaryInwardStencil (:226-257).
+InwardStencil boundaryInwardStencil(const Mesh& m, Index bf) {
+  InwardStencil st;
+  const Index ow = m.owner[bf];
+  const Index opp = oppositeInteriorFace(m, ow, bf);
+  if (opp < 0) return st;
+  const Index far = (m.owner[opp] == ow) ? m.neigh[opp] : m.owner[opp];
+  const V3 n = unitNormal(m, bf);
+  const V3& xf = m.fc[bf];
+  const double h1 = dot(xf - m.cc[ow], n);
+  const double h2 = dot(xf - m.cc[far], n);
+  if (!(h1 > 0.0) || !(h2 > h1)) return st;
+  st.valid = true;
+  st.h1 = h1;
+  st.h2 = h2;
+  st.far = far;
+  st.deltaP = m.cc[ow] - (xf - (n * h1));
+  st.deltaF = m.cc[far] - (xf - (n * h2));
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions (src/boundary/*.cpp): velocity Wall/MovingWall/Inlet
+// return a prescribed vector, Outlet the owner value; pressure FixedGradient
+// returns owner + g*distance, FixedValue its value.
+enum VelKind { kWall = 0, kMovingWall = 1, kInlet = 2, kOutlet = 3 };
+enum PresKind { kFixedGradient = 0, kFixedValue = 1 };
+struct Boundaries {
+  std::vector<int> vkind;
+  std::vector<V3> vvalue;
+  std::vector<int> pkind;
+  std::vector<double> pvalue;
+};
+inline bool velocityPrescribed(int k) { return k != kOutlet; }  // NonOrthogonalDiffusion.cpp:14-31
+inline V3 boundaryVelocity(const Boundaries& b, int patch, const V3& owner) {
+  return b.vkind[patch] == kOutlet ? owner : (b.vkind[patch] == kWall ? V3{0, 0, 0} : b.vvalue[patch]);
+}
+inline double scalarBoundaryValue(int kind, double value, double ownerValue, double dist) {
+  return kind == kFixedValue ? value : ownerValue + (value * dist);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse algebra: src/algebra/SparseMatrix.cpp, Vector.cpp, Preconditioner.cpp.
+struct Csr {
+  Index n = 0;
+  std::vector<Index> rp, col;
+  std::vector<double> val;
+};
+struct Triplet {
+  Index r, c;
+  double v;
+};
+struct Builder {
+  Index n;
+  std::vector<Triplet> t;
+  explicit Builder(Index rows) : n(rows) {}
+  void add(Index r, Index c, double v) { t.push_back({r, c, v}); }
+  // SparseMatrixBuilder::build (:143-178): stable sort by (row, col), duplicate
+  // values summed in insertion order, exact-zero sums dropped.
+  Csr build() const {
+    std::vector<Triplet> s = t;
+    std::stable_sort(s.begin(), s.end(), [](const Triplet& a, const Triplet& b) {
+      return a.r != b.r ? a.r < b.r : a.c < b.c;
+    });
+    Csr m;
+    m.n = n;
+    m.rp.assign(n + 1, 0);
+    size_t k = 0;
+    for (Index r = 0; r < n; ++r) {
+      while (k < s.size() && s[k].r == r) {
+        const Index c = s[k].c;
+        double sum = s[k].v;
+        ++k;
+        while (k < s.size() && s[k].r == r && s[k].c == c) sum += s[k++].v;
+        if (sum != 0.0) {
+          m.val.push_back(sum);
+          m.col.push_back(c);
+        }
+      }
+      m.rp[r + 1] = static_cast<Index>(m.val.size());
+    }
+    return m;
+  }
+};
+// SparseMatrix::diagonal (:102-114); -inf signals "no diagonal stored".
+double diagonal(const Csr& a, Index r) {
+  for (Index k = a.rp[r]; k < a.rp[r + 1]; ++k)
+    if (a.col[k] == r) return a.val[k];
+  return -std::numeric_limits<double>::infinity();
+}
+// SparseMatrix::multiply (:72-100): per row, ascending k, from 0.0; OpenMP rows.
+void spmv(const Csr& a, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (Index r = 0; r < a.n; ++r) {
+    double sum = 0.0;
+    for (Index k = a.rp[r]; k < a.rp[r + 1]; ++k) sum += a.val[k] * x[a.col[k]];
+    y[r] = sum;
+  }
+}
+double vdot(const std::vector<double>& a, const std::vector<double>& b) {  // Vector.cpp:117-124
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+double l2(const std::vector<double>& v) {  // Vector.cpp:134-140
+  double s = 0.0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+bool allFinite(const std::vector<double>& v) {
+  for (double x : v)
+    if (!std::isfinite(x)) return false;
+  return true;
+}
+
+enum Status { kConverged = 0, kMaxIterations = 1, kBreakdown = 2, kNonFiniteInput = 3,
+              kNonFiniteResidual = 4, kInvalidSystem = 5 };
+struct Settings {
+  double abs = 1e-12, rel = 1e-10;
+  Index maxIter = 1000;
+  bool jacobi = false;
+};
+struct Result {
+  std::vector<double> x;
+  int status = kMaxIterations;
+  Index iterations = 0, restarts = 0;
+  double initial = 0.0, final = 0.0;
+  std::vector<double> hist;
+};
+
+// computeInverseDiagonal (Preconditioner.cpp:11-32): false on a missing, zero,
+// non-finite or |d| < 1e-12 diagonal.
+bool inverseDiagonal(const Csr& a, std::vector<double>& inv) {
+  inv.resize(a.n);
+  for (Index r = 0; r < a.n; ++r) {
+    const double d = diagonal(a, r);
+    if (!std::isfinite(d) || d == 0.0 || std::abs(d) < 1e-12) return false;
+    inv[r] = 1.0 / d;
+  }
+  return true;
+}
+static bool matrixFinite(const Csr& a) { return allFinite(a.val); }
+static void applyPrec(const std::vector<double>* inv, const std::vector<double>& in,
+                      std::vector<double>& out) {
+  out.resize(in.size());
+  if (inv == nullptr) {
+    out = in;
+    return;
+  }
+  for (size_t i = 0; i < in.size(); ++i) out[i] = (*inv)[i] * in[i];
+}
+
+// CG::solve (src/algebra/CG.cpp:14-148): recurrence residual relative to
+// b0 = |b - A x0|, absolute |pAp| < 1e-30 breakdown.
+Result cg(const Csr& A, const std::vector<double>& b, const std::vector<double>& x0,
+          const Settings& s) {
+  Result res;
+  res.x = x0;
+  const Index n = A.n;
+  if (static_cast<Index>(x0.size()) != n || !matrixFinite(A) || !allFinite(b) || !allFinite(x0)) {
+    res.status = kNonFiniteInput;
+    return res;
+  }
+  std::vector<double> inv;
+  if (s.jacobi && !inverseDiagonal(A, inv)) {
+    res.status = kInvalidSystem;
+    return res;
+  }
+  const std::vector<double>* M = s.jacobi ? &inv : nullptr;
+  std::vector<double> x = x0, r(n), Ap(n), z, p, rNew(n), zNew;
+  spmv(A, x.data(), Ap.data());
+  for (Index i = 0; i < n; ++i) r[i] = b[i] - Ap[i];
+  const double b0 = l2(r);
+  res.initial = b0;
+  res.hist.push_back(b0);
+  const auto converged = [&](double a) {
+    return a <= s.abs || ((b0 > 0.0) && (a / b0 <= s.rel));
+  };
+  if (b0 == 0.0 || converged(b0)) {
+    res.status = kConverged;
+    res.final = b0;
+    return res;
+  }
+  applyPrec(M, r, z);
+  p = z;
+  double rz = vdot(r, z);
+  if (!std::isfinite(rz)) {
+    res.status = kBreakdown;
+    res.final = b0;
+    return res;
+  }
+  for (Index it = 1; it <= s.maxIter; ++it) {
+    spmv(A, p.data(), Ap.data());
+    const double pAp = vdot(p, Ap);
+    if (!std::isfinite(pAp) || std::abs(pAp) < 1e-30) {
+      res.status = kBreakdown;
+      res.x = x;
+      res.final = l2(r);
+      res.iterations = it - 1;
+      return res;
+    }
+    const double alpha = rz / pAp;
+    if (!std::isfinite(alpha)) {
+      res.status = kBreakdown;
+      res.x = x;
+      res.final = l2(r);
+      res.iterations = it - 1;
+      return res;
+    }
+    for (Index i = 0; i < n; ++i) x[i] += p[i] * alpha;
+    for (Index i = 0; i < n; ++i) rNew[i] = r[i] - (Ap[i] * alpha);
+    const double rn = l2(rNew);
+    res.hist.push_back(rn);
+    res.iterations = it;
+    if (!std::isfinite(rn)) {
+      res.status = kNonFiniteResidual;
+      res.x = x;
+      res.final = rn;
+      return res;
+    }
+    if (converged(rn)) {
+      res.status = kConverged;
+      res.x = x;
+      res.final = rn;
+      return res;
+    }
+    applyPrec(M, rNew, zNew);
+    const double rzNew = vdot(rNew, zNew);
+    if (!std::isfinite(rzNew)) {
+      res.status = kBreakdown;
+      res.x = x;
+      res.final = rn;
+      return res;
+    }
+    const double beta = rzNew / rz;
+    if (!std::isfinite(beta)) {
+      res.status = kBreakdown;
+      res.x = x;
+      res.final = rn;
+      return res;
+    }
+    for (Index i = 0; i < n; ++i) p[i] = zNew[i] + (p[i] * beta);
+    r.swap(rNew);
+    rz = rzNew;
+  }
+  res.status = kMaxIterations;
+  res.x = x;
+  res.final = res.hist.back();
+  return res;
+}
+
+// cancelledToRoundingLevel (src/algebra/BiCGSTAB.cpp:34-41).
+static bool cancelled(double ip, const std::vector<double>& x, const std::vector<double>& y,
+                      double nx, double ny) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  if (std::abs(ip) > eps * nx * ny) return false;
+  double t = 0.0;
+  for (size_t i = 0; i < x.size(); ++i) t += std::abs(x[i] * y[i]);
+  return std::abs(ip) <= eps * t;
+}
+
+// BiCGSTAB::solve (src/algebra/BiCGSTAB.cpp:48-275), scale-relative breakdown
+// test and restart from the true residual.
+Result bicgstab(const Csr& A, const std::vector<double>& b, const std::vector<double>& x0,
+                const Settings& s) {
+  Result res;
+  res.x = x0;
+  const Index n = A.n;
+  if (static_cast<Index>(x0.size()) != n || !matrixFinite(A) || !allFinite(b) || !allFinite(x0)) {
+    res.status = kNonFiniteInput;
+    return res;
+  }
+  std::vector<double> inv;
+  if (s.jacobi && !inverseDiagonal(A, inv)) {
+    res.status = kInvalidSystem;
+    return res;
+  }
+  const std::vector<double>* M = s.jacobi ? &inv : nullptr;
+  std::vector<double> x = x0, r(n), tmp(n);
+  spmv(A, x.data(), tmp.data());
+  for (Index i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
+  std::vector<double> rHat = r;
+  const double b0 = l2(r);
+  res.initial = b0;
+  res.hist.push_back(b0);
+  const auto converged = [&](double a) {
+    return a <= s.abs || ((b0 > 0.0) && (a / b0 <= s.rel));
+  };
+  if (b0 == 0.0 || converged(b0)) {
+    res.status = kConverged;
+    res.final = b0;
+    return res;
+  }
+  double rhoOld = 1.0, alpha = 1.0, omega = 1.0;
+  std::vector<double> v(n, 0.0), p(n, 0.0), pHat, sv(n), sHat, t(n);
+  Index completed = 0;
+  double rHatNorm = b0, rNorm = b0, restartResidual = b0;
+  const auto breakdown = [&]() {
+    res.status = kBreakdown;
+    res.x = x;
+    res.final = l2(r);
+    res.iterations = completed;
+    return res;
+  };
+  enum { Restarted, Conv, Stop };
+  const auto restart = [&]() {
+    if (!(rNorm < restartResidual)) return static_cast<int>(Stop);
+    spmv(A, x.data(), tmp.data());
+    for (Index i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
+    rNorm = l2(r);
+    if (!std::isfinite(rNorm)) return static_cast<int>(Stop);
+    if (converged(rNorm)) return static_cast<int>(Conv);
+    rHat = r;
+    rHatNorm = rNorm;
+    restartResidual = rNorm;
+    rhoOld = 1.0;
+    alpha = 1.0;
+    omega = 1.0;
+    std::fill(v.begin(), v.end(), 0.0);
+    std::fill(p.begin(), p.end(), 0.0);
+    ++res.restarts;
+    return static_cast<int>(Restarted);
+  };
+  const auto restartedConvergence = [&]() {
+    res.status = kConverged;
+    res.x = x;
+    res.final = rNorm;
+    res.iterations = completed;
+    return res;
+  };
+  while (completed < s.maxIter) {
+    const Index iter = completed + 1;
+    const double rho = vdot(rHat, r);
+    if (!std::isfinite(rho)) return breakdown();
+    if (cancelled(rho, rHat, r, rHatNorm, rNorm)) {
+      const int o = restart();
+      if (o == Restarted) continue;
+      if (o == Conv) return restartedConvergence();
+      return breakdown();
+    }
+    const double beta = (rho / rhoOld) * (alpha / omega);
+    if (!std::isfinite(beta)) return breakdown();
+    for (Index i = 0; i < n; ++i) p[i] = r[i] + ((p[i] - (v[i] * omega)) * beta);
+    applyPrec(M, p, pHat);
+    spmv(A, pHat.data(), v.data());
+    const double rHatDotV = vdot(rHat, v);
+    if (!std::isfinite(rHatDotV)) return breakdown();
+    if (cancelled(rHatDotV, rHat, v, rHatNorm, l2(v))) {
+      const int o = restart();
+      if (o == Restarted) continue;
+      if (o == Conv) return restartedConvergence();
+      return breakdown();
+    }
+    alpha = rho / rHatDotV;
+    if (!std::isfinite(alpha)) return breakdown();
+    for (Index i = 0; i < n; ++i) sv[i] = r[i] - (v[i] * alpha);
+    const double sNorm = l2(sv);
+    if (!std::isfinite(sNorm)) {
+      res.status = kNonFiniteResidual;
+      res.x = x;
+      res.final = sNorm;
+      res.iterations = iter;
+      res.hist.push_back(sNorm);
+      return res;
+    }
+    if (converged(sNorm)) {
+      for (Index i = 0; i < n; ++i) x[i] += pHat[i] * alpha;
+      res.status = kConverged;
+      res.x = x;
+      res.final = sNorm;
+      res.iterations = iter;
+      res.hist.push_back(sNorm);
+      return res;
+    }
+    applyPrec(M, sv, sHat);
+    spmv(A, sHat.data(), t.data());
+    const double tDotT = vdot(t, t);
+    if (!std::isfinite(tDotT) || tDotT < std::numeric_limits<double>::min()) return breakdown();
+    const double tDotS = vdot(t, sv);
+    if (!std::isfinite(tDotS)) return breakdown();
+    if (cancelled(tDotS, t, sv, std::sqrt(tDotT), sNorm)) {
+      const int o = restart();
+      if (o == Restarted) continue;
+      if (o == Conv) return restartedConvergence();
+      return breakdown();
+    }
+    omega = tDotS / tDotT;
+    if (!std::isfinite(omega)) return breakdown();
+    for (Index i = 0; i < n; ++i) x[i] += pHat[i] * alpha;
+    for (Index i = 0; i < n; ++i) x[i] += sHat[i] * omega;
+    for (Index i = 0; i < n; ++i) r[i] = sv[i] - (t[i] * omega);
+    const double rn = l2(r);
+    rNorm = rn;
+    if (!std::isfinite(rn)) {
+      res.status = kNonFiniteResidual;
+      res.x = x;
+      res.final = rn;
+      res.iterations = iter;
+      res.hist.push_back(rn);
+      return res;
+    }
+    completed = iter;
+    res.hist.push_back(rn);
+    res.iterations = iter;
+    if (converged(rn)) {
+      res.status = kConverged;
+      res.x = x;
+      res.final = rn;
+      return res;
+    }
+    rhoOld = rho;
+  }
+  res.status = kMaxIterations;
+  res.x = x;
+  res.final = res.hist.back();
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// Discretization: src/discretization/Interpolation.cpp:21-41,
+// Gradient.cpp:104-162,478-596, VectorGradient.cpp:19-80.
+double interpScalar(const Mesh& m, Index f, const std::vector<double>& phi) {
+  const double dPf = distance(m.cc[m.owner[f]], m.fc[f]);
+  const double dNf = distance(m.cc[m.neigh[f]], m.fc[f]);
+  return ((dNf * phi[m.owner[f]]) + (dPf * phi[m.neigh[f]])) / (dPf + dNf);
+}
+V3 interpVector(const Mesh& m, Index f, const std::vector<V3>& u) {
+  const double dPf = distance(m.cc[m.owner[f]], m.fc[f]);
+  const double dNf = distance(m.cc[m.neigh[f]], m.fc[f]);
+  return ((u[m.owner[f]] * dNf) + (u[m.neigh[f]] * dPf)) * (1.0 / (dPf + dNf));
+}
+
+struct ScalarBC {  // one scalar condition per patch
+  std::vector<int> kind;
+  std::vector<double> value;
+};
+
+// interpolate() face values (Interpolation.cpp:126-166), boundary via the condition.
+std::vector<double> faceValues(const Mesh& m, const std::vector<double>& phi, const ScalarBC& bc) {
+  std::vector<double> fv(m.nFaces());
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    if (!m.boundary(f)) {
+      fv[f] = interpScalar(m, f, phi);
+      continue;
+    }
+    const int p = m.facePatch[f];
+    const double d = distance(m.cc[m.owner[f]], m.fc[f]);
+    fv[f] = scalarBoundaryValue(bc.kind[p], bc.value[p], phi[m.owner[f]], d);
+  }
+  return fv;
+}
+
+struct Consistent {
+  bool applies = false;
+  Index face = 0;
+  double value = 0.0, align = 0.0;
+};
+// boundaryConsistentFaceValue (Gradient.cpp:104-162).
+Consistent boundaryConsistent(const Mesh& m, Index cell, Index bf, const std::vector<double>& phi,
+                              const std::vector<double>& fv, const V3& ownerGrad) {
+  const Index opp = oppositeInteriorFace(m, cell, bf);
+  if (opp < 0) return {};
+  const Index far = (m.owner[opp] == cell) ? m.neigh[opp] : m.owner[opp];
+  const V3 d = m.cc[far] - m.cc[cell];
+  const double farDistance = magnitude(d);
+  if (!(farDistance > 0.0)) return {};
+  const V3 inward = d * (1.0 / farDistance);
+  const Crossing cr = ownerNeighborCrossing(m, opp);
+  if (!cr.valid) return {};
+  const LineHit hit = boundaryLineIntersection(m, bf, inward);
+  if (!hit.valid) return {};
+  const double phiB = fv[bf] + dot(ownerGrad, hit.point - m.fc[bf]);
+  const double phiP = phi[cell], phiF = phi[far];
+  const double back = hit.distance;
+  const double d2 = 2.0 * (((phiB - phiP) / back) + ((phiF - phiP) / farDistance)) /
+                    (back + farDistance);
+  const double w = cr.t;
+  const double corr = -0.5 * w * (1.0 - w) * farDistance * farDistance * d2;
+  const V3 refN = unitNormal(m, bf);
+  const V3 oppN = (m.owner[opp] == cell) ? unitNormal(m, opp) : (unitNormal(m, opp) * -1.0);
+  return {true, opp, fv[opp] + corr, dot(refN, oppN)};
+}
+
+// greenGaussSweep (Gradient.cpp:511-568).
+std::vector<V3> greenGaussSweep(const Mesh& m, const std::vector<double>& phi,
+                                const std::vector<double>& fv, const std::vector<V3>* prev) {
+  std::vector<V3> g(m.nCells());
+  struct Claim {
+    Index face;
+    double value, align;
+    Index bface;
+  };
+  std::vector<Claim> claims;
+  for (Index c = 0; c < m.nCells(); ++c) {
+    claims.clear();
+    for (Index k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+      const Index f = m.cf[k];
+      if (!m.boundary(f)) continue;
+      const V3 og = prev == nullptr ? V3{0.0, 0.0, 0.0} : (*prev)[c];
+      const Consistent cs = boundaryConsistent(m, c, f, phi, fv, og);
+      if (!cs.applies) continue;
+      auto it = std::find_if(claims.begin(), claims.end(),
+                             [&](const Claim& e) { return e.face == cs.face; });
+      const Claim cl{cs.face, cs.value, cs.align, f};
+      if (it == claims.end()) claims.push_back(cl);
+      else if (std::make_pair(cs.align, f) < std::make_pair(it->align, it->bface)) *it = cl;
+    }
+    V3 sum{0.0, 0.0, 0.0};
+    for (Index k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+      const Index f = m.cf[k];
+      const V3 s = (m.owner[f] == c) ? m.sf[f] : (m.sf[f] * -1.0);
+      auto it = std::find_if(claims.begin(), claims.end(),
+                             [&](const Claim& e) { return e.face == f; });
+      sum = sum + (s * (it == claims.end() ? fv[f] : it->value));
+    }
+    g[c] = sum * (1.0 / m.vol[c]);
+  }
+  return g;
+}
+
+// boundaryTransferNeeded (Gradient.cpp:484-509).
+bool boundaryTransferNeeded(const Mesh& m) {
+  for (Index c = 0; c < m.nCells(); ++c)
+    for (Index k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+      const Index f = m.cf[k];
+      if (!m.boundary(f)) continue;
+      const Index opp = oppositeInteriorFace(m, c, f);
+      if (opp < 0) continue;
+      const Index far = (m.owner[opp] == c) ? m.neigh[opp] : m.owner[opp];
+      const V3 d = m.cc[far] - m.cc[c];
+      const double fd = magnitude(d);
+      if (!(fd > 0.0)) continue;
+      const LineHit h = boundaryLineIntersection(m, f, d * (1.0 / fd));
+      if (h.valid && !isZero(h.point - m.fc[f])) return true;
+    }
+  return false;
+}
+
+struct Oblique {
+  bool applies = false;
+  double dn = 0.0;
+  V3 offset;
+};
+// obliqueNeumannFace (Gradient.cpp:186-216) for a Neumann-type scalar face.
+Oblique obliqueFace(const Mesh& m, Index f, int kind) {
+  if (kind == kFixedValue) return {};
+  const V3 d = m.fc[f] - m.cc[m.owner[f]];
+  const V3 n = unitNormal(m, f);
+  const double dn = dot(d, n);
+  if (!(dn > 0.0)) return {};
+  const V3 t = d - (n * dn);
+  if (isZero(t)) return {};
+  return {true, dn, t};
+}
+
+// greenGaussGradient (Gradient.cpp:534-596) with kGreenGaussSkewCorrectionSweeps = 4.
+std::vector<V3> gradient(const Mesh& m, const std::vector<double>& phi, const ScalarBC& bc) {
+  std::vector<double> fv = faceValues(m, phi, bc);
+  std::vector<V3> g = greenGaussSweep(m, phi, fv, nullptr);
+  std::vector<Index> skewed;
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    if (m.boundary(f)) continue;
+    const Crossing cr = ownerNeighborCrossing(m, f);
+    if (cr.valid && !isZero(cr.skew)) skewed.push_back(f);
+  }
+  std::vector<std::pair<Index, Oblique>> oblique;
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    if (!m.boundary(f)) continue;
+    const Oblique o = obliqueFace(m, f, bc.kind[m.facePatch[f]]);
+    if (o.applies) oblique.emplace_back(f, o);
+  }
+  const bool transfer = boundaryTransferNeeded(m);
+  for (int sweep = 0; sweep < 4 && (!skewed.empty() || !oblique.empty() || transfer); ++sweep) {
+    for (Index f : skewed) {
+      // interpolateInternalFaceSkewCorrected (Interpolation.cpp:43-60).
+      const Crossing cr = ownerNeighborCrossing(m, f);
+      if (!cr.valid || isZero(cr.skew)) {
+        fv[f] = interpScalar(m, f, phi);
+        continue;
+      }
+      const Index o = m.owner[f], nb = m.neigh[f];
+      const double t = cr.t;
+      const double pc = phi[o] + (t * (phi[nb] - phi[o]));
+      const V3 gc = g[o] + ((g[nb] - g[o]) * t);
+      fv[f] = pc + dot(gc, cr.skew);
+    }
+    for (const auto& [f, o] : oblique) {
+      const int p = m.facePatch[f];
+      fv[f] = scalarBoundaryValue(bc.kind[p], bc.value[p], phi[m.owner[f]], o.dn) +
+              dot(g[m.owner[f]], o.offset);
+    }
+    g = greenGaussSweep(m, phi, fv, &g);
+  }
+  return g;
+}
+
+// greenGaussVelocityGradient (VectorGradient.cpp:19-80): plain face sums (no
+// boundary-consistent correction); skewed faces do not occur on Cartesian meshes.
+void velocityGradient(const Mesh& m, const std::vector<V3>& u, const Boundaries& b,
+                      std::vector<V3>& gu, std::vector<V3>& gv, std::vector<V3>& gw) {
+  std::vector<V3> fu(m.nFaces());
+  for (Index f = 0; f < m.nFaces(); ++f)
+    fu[f] = m.boundary(f) ? boundaryVelocity(b, m.facePatch[f], u[m.owner[f]]) : interpVector(m, f, u);
+  const bool three = m.dim == 3;
+  gu.assign(m.nCells(), V3{});
+  gv.assign(m.nCells(), V3{});
+  gw.assign(three ? m.nCells() : 0, V3{});
+  for (Index c = 0; c < m.nCells(); ++c) {
+    V3 su{0, 0, 0}, sv{0, 0, 0}, sw{0, 0, 0};
+    for (Index k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+      const Index f = m.cf[k];
+      const V3 s = (m.owner[f] == c) ? m.sf[f] : (m.sf[f] * -1.0);
+      su = su + (s * fu[f].x);
+      sv = sv + (s * fu[f].y);
+      if (three) sw = sw + (s * fu[f].z);
+    }
+    gu[c] = su * (1.0 / m.vol[c]);
+    gv[c] = sv * (1.0 / m.vol[c]);
+    if (three) gw[c] = sw * (1.0 / m.vol[c]);
+  }
+}
+
+inline double comp(const V3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// ---------------------------------------------------------------------------
+// Physics.
+// calculateMassFlux (src/physics/MassFlux.cpp:15-36).
+std::vector<double> massFlux(const Mesh& m, const std::vector<V3>& u, const Boundaries& b,
+                             double rho) {
+  std::vector<double> F(m.nFaces());
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    const V3 uf = m.boundary(f) ? boundaryVelocity(b, m.facePatch[f], u[m.owner[f]])
+                                : interpVector(m, f, u);
+    F[f] = rho * dot(uf, m.sf[f]);
+  }
+  return F;
+}
+
+struct Momentum {
+  Csr A;
+  std::vector<double> rhs, diag;
+  bool ok = true;
+};
+
+ScalarBC pressureBC(const Boundaries& b) { return ScalarBC{b.pkind, b.pvalue}; }
+
+// assembleRelaxedMomentumComponent (src/pressure_velocity/RelaxedMomentum.cpp:30-104):
+// diffusion (MomentumEquation.cpp:187-262 with NonOrthogonalDiffusion.cpp:51-113),
+// upwind convection (:264-327), pressure source (:415-430), implicit
+// relaxation (UnderRelaxation.cpp:13-36).
+Momentum assembleMomentum(const Mesh& m, const std::vector<V3>& u, const std::vector<double>& p,
+                          const std::vector<double>& F, const std::vector<double>& mu,
+                          const Boundaries& b, int component, const std::vector<double>& prev,
+                          double alpha) {
+  const Index n = m.nCells();
+  Builder B(n);
+  std::vector<double> rhs(n, 0.0);
+  std::vector<V3> gu, gv, gw;
+  velocityGradient(m, u, b, gu, gv, gw);
+  const std::vector<V3>& gphi = component == 0 ? gu : (component == 1 ? gv : gw);
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    if (m.boundary(f)) {
+      const Index o = m.owner[f];
+      const int patch = m.facePatch[f];
+      const double dist = distance(m.cc[o], m.fc[f]);
+      const double gamma = mu[o];
+      double coeff, bvCoeff, farCoeff = 0.0, explicitFlux = 0.0;
+      Index far = 0;
+      const InwardStencil st = velocityPrescribed(b.vkind[patch]) ? boundaryInwardStencil(m, f)
+                                                                  : InwardStencil{};
+      if (velocityPrescribed(b.vkind[patch]) && st.valid) {
+        const double h1 = st.h1, h2 = st.h2;
+        const double gA = gamma * m.area(f);
+        const double cP = h2 / (h1 * (h2 - h1));
+        const double cF = h1 / (h2 * (h2 - h1));
+        const double cB = (1.0 / h1) + (1.0 / h2);
+        coeff = gA * cP;
+        farCoeff = gA * cF;
+        far = st.far;
+        bvCoeff = gA * cB;
+        explicitFlux = gA * ((cP * dot(gphi[o], st.deltaP)) - (cF * dot(gphi[st.far], st.deltaF)));
+      } else {
+        // Non-prescribed (Outlet) faces: the two-point form; the one-cell-thick
+        // fallback decomposition reduces to it on a Cartesian face.
+        coeff = gamma * m.area(f) / dist;
+        bvCoeff = coeff;
+      }
+      const V3 ub = boundaryVelocity(b, patch, u[o]);
+      const double phiB = comp(ub, component);
+      rhs[o] += explicitFlux;
+      B.add(o, o, coeff);
+      rhs[o] += bvCoeff * phiB;
+      if (farCoeff != 0.0) B.add(o, far, -farCoeff);
+      continue;
+    }
+    const Index o = m.owner[f], nb = m.neigh[f];
+    const double dPN = distance(m.cc[o], m.cc[nb]);
+    const double muf = interpScalar(m, f, mu);
+    const double d = muf * m.area(f) / dPN;
+    B.add(o, o, d);
+    B.add(o, nb, -d);
+    B.add(nb, nb, d);
+    B.add(nb, o, -d);
+  }
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    const double flux = F[f];
+    const Index o = m.owner[f];
+    if (m.boundary(f)) {
+      if (flux >= 0.0) {
+        B.add(o, o, flux);
+      } else {
+        const V3 ub = boundaryVelocity(b, m.facePatch[f], u[o]);
+        rhs[o] -= flux * comp(ub, component);
+      }
+      continue;
+    }
+    const Index nb = m.neigh[f];
+    if (flux >= 0.0) {
+      B.add(o, o, flux);
+      B.add(nb, o, -flux);
+    } else {
+      B.add(o, nb, flux);
+      B.add(nb, nb, -flux);
+    }
+  }
+  const std::vector<V3> gp = gradient(m, p, pressureBC(b));
+  for (Index c = 0; c < n; ++c) rhs[c] += -m.vol[c] * comp(gp[c], component);
+  Momentum out;
+  if (alpha != 1.0) {
+    const Csr un = B.build();
+    const double factor = (1.0 / alpha) - 1.0;
+    for (Index i = 0; i < n; ++i) {
+      const double extra = diagonal(un, i) * factor;
+      B.add(i, i, extra);
+      rhs[i] += extra * prev[i];
+    }
+  }
+  out.A = B.build();
+  out.diag.resize(n);
+  for (Index i = 0; i < n; ++i) out.diag[i] = diagonal(out.A, i);
+  out.rhs = rhs;
+  out.ok = matrixFinite(out.A) && allFinite(rhs);
+  for (double d : out.diag)
+    if (!std::isfinite(d)) out.ok = false;
+  return out;
+}
+
+// computeMomentumResponseCoefficient (PressureCorrectionEquation.cpp:103-114).
+std::vector<double> response(const Mesh& m, const std::vector<double>& diag) {
+  std::vector<double> d(m.nCells());
+  for (Index c = 0; c < m.nCells(); ++c) d[c] = m.vol[c] / diag[c];
+  return d;
+}
+
+inline bool axisAligned(const V3& s) {
+  return ((s.x != 0.0 ? 1 : 0) + (s.y != 0.0 ? 1 : 0) + (s.z != 0.0 ? 1 : 0)) == 1;
+}
+
+// pressureCorrectionFaceCoupling / coupling (PressureCorrectionEquation.cpp:74-99,146-179),
+// nonOrthogonal = false.
+double faceCoupling(const Mesh& m, Index f, double rho, const std::vector<double>& dU,
+                    const std::vector<double>& dV, const std::vector<double>* dW) {
+  const Index o = m.owner[f];
+  const V3& s = m.sf[f];
+  const auto couple = [&](const V3& d, double du, double dv, double dw) {
+    const double dist = magnitude(d);
+    if (axisAligned(s) && isZero(cross(d, s))) {
+      const double dc = (s.x != 0.0) ? du : ((s.y != 0.0) ? dv : dw);
+      return rho * m.area(f) * dc / dist;
+    }
+    const V3 rv = (s.z == 0.0) ? V3{du * s.x, dv * s.y, 0.0} : V3{du * s.x, dv * s.y, dw * s.z};
+    return rho * magnitude(rv) / dist;
+  };
+  if (m.boundary(f)) {
+    const V3 d = m.fc[f] - m.cc[o];
+    return couple(d, dU[o], dV[o], dW != nullptr ? (*dW)[o] : 0.0);
+  }
+  const V3 d = m.cc[m.neigh[f]] - m.cc[o];
+  if (axisAligned(s) && isZero(cross(d, s))) {
+    const std::vector<double>& r = (s.x != 0.0) ? dU : ((s.y != 0.0) ? dV : *dW);
+    const double df = interpScalar(m, f, r);
+    return couple(d, df, df, df);
+  }
+  return couple(d, interpScalar(m, f, dU), interpScalar(m, f, dV),
+                dW != nullptr ? interpScalar(m, f, *dW) : 0.0);
+}
+
+// rhieChowFaceCorrection + rhieChowMassFlux (src/pressure_velocity/RhieChow.cpp:17-65).
+std::vector<double> rhieChowFlux(const Mesh& m, const std::vector<V3>& ustar,
+                                 const std::vector<double>& p, const std::vector<V3>& gradP,
+                                 const std::vector<double>& dU, const std::vector<double>& dV,
+                                 const std::vector<double>* dW, const Boundaries& b, double rho,
+                                 double alpha) {
+  std::vector<double> F = massFlux(m, ustar, b, rho);
+  std::vector<double> corr(m.nFaces(), 0.0);
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    if (m.boundary(f)) continue;
+    const Index o = m.owner[f], nb = m.neigh[f];
+    const double c = faceCoupling(m, f, rho, dU, dV, dW);
+    const V3 d = m.cc[nb] - m.cc[o];
+    const V3 gf = interpVector(m, f, gradP);
+    const double cmi = (p[nb] - p[o]) - dot(gf, d);
+    corr[f] = -(c / alpha) * cmi;
+  }
+  for (Index f = 0; f < m.nFaces(); ++f) F[f] += corr[f];
+  return F;
+}
+
+struct Continuity {
+  std::vector<double> imbalance;
+  double global = 0.0;
+};
+// evaluateContinuity (src/physics/ContinuityEquation.cpp:14-55).
+Continuity continuity(const Mesh& m, const std::vector<double>& F) {
+  Continuity c;
+  c.imbalance.assign(m.nCells(), 0.0);
+  for (Index cell = 0; cell < m.nCells(); ++cell) {
+    double s = 0.0;
+    for (Index k = m.cfOff[cell]; k < m.cfOff[cell + 1]; ++k) {
+      const Index f = m.cf[k];
+      s += (m.owner[f] == cell) ? F[f] : -F[f];
+    }
+    c.imbalance[cell] = s;
+  }
+  double g = 0.0;
+  for (const auto& patch : m.patches)
+    for (Index f : patch) g += F[f];
+  c.global = g;
+  return c;
+}
+
+struct PressureSystem {
+  Csr A;
+  std::vector<double> rhs, faceCoeff;
+};
+// assembleGeometricPressureCorrection (PressureCorrectionEquation.cpp:181-339) via
+// assemblePressureCorrection (:341-367): constant density, no additional diagonal,
+// no non-orthogonal pass.
+PressureSystem assemblePressure(const Mesh& m, const std::vector<double>& Fstar,
+                                const std::vector<double>& dU, const std::vector<double>& dV,
+                                const std::vector<double>* dW, double rho, Index ref,
+                                const Boundaries& b) {
+  const Index n = m.nCells();
+  bool open = false;
+  for (size_t p = 0; p < m.patches.size(); ++p)
+    if (b.pkind[p] == kFixedValue) open = true;
+  const bool pin = !open;
+  Builder B(n);
+  PressureSystem out;
+  out.faceCoeff.assign(m.nFaces(), 0.0);
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    const Index o = m.owner[f];
+    if (m.boundary(f)) {
+      if (b.pkind[m.facePatch[f]] != kFixedValue) continue;
+      const double c = faceCoupling(m, f, rho, dU, dV, dW);
+      out.faceCoeff[f] = c;
+      if (!pin || o != ref) B.add(o, o, c);
+      continue;
+    }
+    const double c = faceCoupling(m, f, rho, dU, dV, dW);
+    out.faceCoeff[f] = c;
+    const Index nb = m.neigh[f];
+    if (!pin || o != ref) {
+      B.add(o, o, c);
+      B.add(o, nb, -c);
+    }
+    if (!pin || nb != ref) {
+      B.add(nb, nb, c);
+      B.add(nb, o, -c);
+    }
+  }
+  const Continuity cont = continuity(m, Fstar);
+  out.rhs.resize(n);
+  for (Index c = 0; c < n; ++c) out.rhs[c] = -cont.imbalance[c];
+  if (pin) {
+    B.add(ref, ref, 1.0);
+    out.rhs[ref] = 0.0;
+  }
+  out.A = B.build();
+  return out;
+}
+
+// correctFaceMassFlux (PressureCorrectionEquation.cpp:369-402), no explicit term.
+std::vector<double> correctFlux(const Mesh& m, const std::vector<double>& Fstar,
+                                const std::vector<double>& coeff, const std::vector<double>& pp) {
+  std::vector<double> out(m.nFaces());
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    const double po = pp[m.owner[f]];
+    const double pn = m.boundary(f) ? 0.0 : pp[m.neigh[f]];
+    out[f] = Fstar[f] + (coeff[f] * (po - pn));
+  }
+  return out;
+}
+
+// makeGradientBoundaries (PressureCorrectionEquation.cpp:43-58): FixedValue(0)
+// where p is FixedValue, FixedGradient(0) elsewhere.
+ScalarBC correctionBC(const Boundaries& b) {
+  ScalarBC s;
+  for (int k : b.pkind) {
+    s.kind.push_back(k == kFixedValue ? kFixedValue : kFixedGradient);
+    s.value.push_back(0.0);
+  }
+  return s;
+}
+
+// correctVelocity (PressureCorrectionEquation.cpp:404-438).
+std::vector<V3> correctVelocity(const Mesh& m, const std::vector<V3>& ustar,
+                                const std::vector<double>& dU, const std::vector<double>& dV,
+                                const std::vector<double>* dW, const std::vector<double>& pp,
+                                const Boundaries& b, std::vector<V3>* gradOut) {
+  const std::vector<V3> g = gradient(m, pp, correctionBC(b));
+  if (gradOut) *gradOut = g;
+  std::vector<V3> out(m.nCells());
+  for (Index c = 0; c < m.nCells(); ++c) {
+    if (dW == nullptr)
+      out[c] = V3{ustar[c].x - (dU[c] * g[c].x), ustar[c].y - (dV[c] * g[c].y), 0.0};
+    else
+      out[c] = V3{ustar[c].x - (dU[c] * g[c].x), ustar[c].y - (dV[c] * g[c].y),
+                  ustar[c].z - ((*dW)[c] * g[c].z)};
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// SIMPLE::solve (src/pressure_velocity/SIMPLE.cpp:148-1124), CPU path, laminar,
+// default (absolute) OuterIterationMonitor gate (SolverRobustness.cpp:354-363).
+struct SimpleSettings {
+  Index maxIter = 1;
+  double au = 0.7, ap = 0.3, vtol = 1e-6, ptol = 1e-6, ctol = 1e-6;
+  Settings mom, pres;
+  bool presCG = true;
+  bool rhieChow = true;
+  double rho = 1.0, mu = 0.01;
+  Index ref = 0;
+};
+enum SimpleStatus { sConverged = 0, sMaxIterations = 1, sMomentumFailure = 2,
+                    sPressureFailure = 3, sNonFinite = 4 };
+struct SimpleResult {
+  std::vector<V3> u;
+  std::vector<double> p, F;
+  int status = sMaxIterations;
+  Index iterations = 0, momentumIters = 0, pressureIters = 0;
+  std::vector<double> hu, hv, hw, hp, hc;
+  double finalContinuity = 0.0, global = 0.0;
+};
+
+static bool finiteField(const std::vector<V3>& u) {
+  for (const V3& v : u)
+    if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z)) return false;
+  return true;
+}
+
+SimpleResult simple(const Mesh& m, const Boundaries& b, const SimpleSettings& s,
+                    std::vector<V3> u, std::vector<double> p) {
+  SimpleResult R;
+  const Index n = m.nCells();
+  const bool three = m.dim == 3;
+  const int ncomp = three ? 3 : 2;
+  std::vector<double> F = massFlux(m, u, b, s.rho);
+  const std::vector<double> mu(n, s.mu + 0.0);  // TurbulenceModel::effectiveViscosity
+  int status = sMaxIterations;
+  for (Index it = 0; it < s.maxIter; ++it) {
+    std::vector<std::vector<double>> prev(ncomp, std::vector<double>(n));
+    for (int k = 0; k < ncomp; ++k)
+      for (Index c = 0; c < n; ++c) prev[k][c] = comp(u[c], k);
+    std::vector<Momentum> asmb;
+    bool ok = true;
+    for (int k = 0; k < ncomp; ++k) {
+      asmb.push_back(assembleMomentum(m, u, p, F, mu, b, k, prev[k], s.au));
+      ok = ok && asmb.back().ok;
+    }
+    if (!ok) {
+      status = sNonFinite;
+      break;
+    }
+    std::vector<Result> ms;
+    bool mfail = false;
+    for (int k = 0; k < ncomp; ++k) {
+      ms.push_back(bicgstab(asmb[k].A, asmb[k].rhs, prev[k], s.mom));
+      if (ms.back().status != kConverged) {
+        mfail = true;
+        break;
+      }
+      R.momentumIters += ms.back().iterations;
+    }
+    if (mfail) {
+      status = sMomentumFailure;
+      break;
+    }
+    std::vector<V3> ustar(n);
+    for (Index c = 0; c < n; ++c)
+      ustar[c] = V3{ms[0].x[c], ms[1].x[c], three ? ms[2].x[c] : 0.0};
+    if (!finiteField(ustar)) {
+      status = sNonFinite;
+      break;
+    }
+    std::vector<std::vector<double>> d(ncomp);
+    for (int k = 0; k < ncomp; ++k) d[k] = response(m, asmb[k].diag);
+    const std::vector<double>* dW = three ? &d[2] : nullptr;
+    std::vector<double> Fstar;
+    if (s.rhieChow) {
+      const std::vector<V3> gp = gradient(m, p, pressureBC(b));
+      Fstar = rhieChowFlux(m, ustar, p, gp, d[0], d[1], dW, b, s.rho, s.au);
+    } else {
+      Fstar = massFlux(m, ustar, b, s.rho);
+    }
+    if (!allFinite(Fstar)) {
+      status = sNonFinite;
+      break;
+    }
+    const PressureSystem ps = assemblePressure(m, Fstar, d[0], d[1], dW, s.rho, s.ref, b);
+    if (!matrixFinite(ps.A) || !allFinite(ps.rhs)) {
+      status = sNonFinite;
+      break;
+    }
+    const std::vector<double> zero(n, 0.0);
+    const Result pr = s.presCG ? cg(ps.A, ps.rhs, zero, s.pres) : bicgstab(ps.A, ps.rhs, zero, s.pres);
+    if (pr.status != kConverged) {
+      status = sPressureFailure;
+      break;
+    }
+    R.pressureIters += pr.iterations;
+    if (!allFinite(pr.x)) {
+      status = sNonFinite;
+      break;
+    }
+    std::vector<double> pnew(n);
+    for (Index c = 0; c < n; ++c) pnew[c] = p[c] + (s.ap * pr.x[c]);
+    std::vector<V3> unew = correctVelocity(m, ustar, d[0], d[1], dW, pr.x, b, nullptr);
+    std::vector<double> Fnew = correctFlux(m, Fstar, ps.faceCoeff, pr.x);
+    if (!finiteField(unew) || !allFinite(pnew) || !allFinite(Fnew)) {
+      status = sNonFinite;
+      break;
+    }
+    const Continuity cont = continuity(m, Fnew);
+    double ss = 0.0;
+    for (Index c = 0; c < n; ++c) ss += cont.imbalance[c] * cont.imbalance[c];
+    const double crms = std::sqrt(ss / static_cast<double>(n));
+    const double gimb = std::abs(cont.global);
+    R.hu.push_back(ms[0].initial);
+    R.hv.push_back(ms[1].initial);
+    R.hp.push_back(pr.initial);
+    R.hc.push_back(crms);
+    if (three) R.hw.push_back(ms[2].initial);
+    u = std::move(unew);
+    p = std::move(pnew);
+    F = std::move(Fnew);
+    R.iterations = it + 1;
+    R.finalContinuity = crms;
+    R.global = gimb;
+    const bool wOk = !three || ms[2].initial <= s.vtol;
+    if (ms[0].initial <= s.vtol && ms[1].initial <= s.vtol && pr.initial <= s.ptol &&
+        crms <= s.ctol && gimb <= s.ctol && wOk) {
+      status = sConverged;
+      break;
+    }
+  }
+  R.status = status;
+  R.u = std::move(u);
+  R.p = std::move(p);
+  R.F = std::move(F);
+  return R;
+}
+
+}  // namespace orc
+
+// ===========================================================================
+// C ABI for ctypes (tests/, bench.py cpu_baseline, __graft_entry__.smoke()).
+using namespace orc;
+
+namespace {
+struct Case {
+  Mesh mesh;
+  Boundaries bc;
+};
+void toV3(const Mesh& m, const double* x, const double* y, const double* z, std::vector<V3>& u) {
+  u.resize(m.nCells());
+  for (Index c = 0; c < m.nCells(); ++c) u[c] = V3{x[c], y[c], m.dim == 3 && z ? z[c] : 0.0};
+}
+void fromV3(const std::vector<V3>& u, double* x, double* y, double* z) {
+  for (size_t c = 0; c < u.size(); ++c) {
+    if (x) x[c] = u[c].x;
+    if (y) y[c] = u[c].y;
+    if (z) z[c] = u[c].z;
+  }
+}
+Settings settings(const double* s) {  // {abs, rel, maxIter, jacobi}
+  Settings o;
+  o.abs = s[0];
+  o.rel = s[1];
+  o.maxIter = static_cast<Index>(s[2]);
+  o.jacobi = s[3] != 0.0;
+  return o;
+}
+Csr csrFrom(Index n, const int64_t* rp, const int64_t* col, const double* val) {
+  Csr a;
+  a.n = n;
+  a.rp.assign(rp, rp + n + 1);
+  a.col.assign(col, col + rp[n]);
+  a.val.assign(val, val + rp[n]);
+  return a;
+}
+int64_t csrOut(const Csr& a, int64_t* rp, int64_t* col, double* val) {
+  std::copy(a.rp.begin(), a.rp.end(), rp);
+  std::copy(a.col.begin(), a.col.end(), col);
+  std::copy(a.val.begin(), a.val.end(), val);
+  return static_cast<int64_t>(a.val.size());
+}
+// result: {status, iterations, restarts, initial, final, histLen}; hist capacity histCap.
+void resultOut(const Result& r, double* x, double* meta, double* hist, int64_t histCap) {
+  std::copy(r.x.begin(), r.x.end(), x);
+  meta[0] = r.status;
+  meta[1] = static_cast<double>(r.iterations);
+  meta[2] = static_cast<double>(r.restarts);
+  meta[3] = r.initial;
+  meta[4] = r.final;
+  meta[5] = static_cast<double>(r.hist.size());
+  if (hist)
+    for (int64_t i = 0; i < histCap && i < static_cast<int64_t>(r.hist.size()); ++i) hist[i] = r.hist[i];
+}
+}  // namespace
+
+extern "C" {
+
+void* orc_case_create(int dim, int64_t nx, int64_t ny, int64_t nz, double lx, double ly, double lz) {
+  auto* c = new Case;
+  c->mesh = dim == 3 ? cartesian3D(nx, ny, nz, lx, ly, lz) : cartesian2D(nx, ny, lx, ly);
+  const size_t np = c->mesh.patches.size();
+  c->bc.vkind.assign(np, kWall);
+  c->bc.vvalue.assign(np, V3{});
+  c->bc.pkind.assign(np, kFixedGradient);
+  c->bc.pvalue.assign(np, 0.0);
+  return c;
+}
+void orc_case_destroy(void* h) { delete static_cast<Case*>(h); }
+int64_t orc_case_sizes(void* h, int64_t* out) {  // {cells, faces, patches, cellFaceEntries}
+  const Mesh& m = static_cast<Case*>(h)->mesh;
+  out[0] = m.nCells();
+  out[1] = m.nFaces();
+  out[2] = static_cast<int64_t>(m.patches.size());
+  out[3] = static_cast<int64_t>(m.cf.size());
+  return 0;
+}
+void orc_case_set_bc(void* h, int patch, int vkind, double vx, double vy, double vz, int pkind,
+                     double pvalue) {
+  auto* c = static_cast<Case*>(h);
+  c->bc.vkind[patch] = vkind;
+  c->bc.vvalue[patch] = V3{vx, vy, vz};
+  c->bc.pkind[patch] = pkind;
+  c->bc.pvalue[patch] = pvalue;
+}
+void orc_mesh_arrays(void* h, int64_t* owner, int64_t* neigh, double* fc, double* sf, double* cc,
+                     double* vol, int64_t* cfOff, int64_t* cf, int64_t* patchOf) {
+  const Mesh& m = static_cast<Case*>(h)->mesh;
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    owner[f] = m.owner[f];
+    neigh[f] = m.neigh[f];
+    fc[3 * f] = m.fc[f].x;
+    fc[3 * f + 1] = m.fc[f].y;
+    fc[3 * f + 2] = m.fc[f].z;
+    sf[3 * f] = m.sf[f].x;
+    sf[3 * f + 1] = m.sf[f].y;
+    sf[3 * f + 2] = m.sf[f].z;
+    patchOf[f] = m.facePatch[f];
+  }
+  for (Index c = 0; c < m.nCells(); ++c) {
+    cc[3 * c] = m.cc[c].x;
+    cc[3 * c + 1] = m.cc[c].y;
+    cc[3 * c + 2] = m.cc[c].z;
+    vol[c] = m.vol[c];
+  }
+  std::copy(m.cfOff.begin(), m.cfOff.end(), cfOff);
+  std::copy(m.cf.begin(), m.cf.end(), cf);
+}
+// Geometry flags that decide whether the single Green-Gauss sweep is final.
+void orc_mesh_flags(void* h, int64_t* out) {
+  const Mesh& m = static_cast<Case*>(h)->mesh;
+  out[0] = boundaryTransferNeeded(m) ? 1 : 0;
+  int64_t oblique = 0, skew = 0;
+  for (Index f = 0; f < m.nFaces(); ++f) {
+    if (m.boundary(f)) oblique += obliqueFace(m, f, kFixedGradient).applies ? 1 : 0;
+    else {
+      const Crossing cr = ownerNeighborCrossing(m, f);
+      skew += (cr.valid && !isZero(cr.skew)) ? 1 : 0;
+    }
+  }
+  out[1] = oblique;
+  out[2] = skew;
+}
+
+void orc_spmv(int64_t n, const int64_t* rp, const int64_t* col, const double* val, const double* x,
+              double* y) {
+  const Csr a = csrFrom(n, rp, col, val);
+  spmv(a, x, y);
+}
+void orc_cg(int64_t n, const int64_t* rp, const int64_t* col, const double* val, const double* b,
+            const double* x0, const double* s, double* x, double* meta, double* hist, int64_t cap) {
+  const Csr a = csrFrom(n, rp, col, val);
+  const Result r = cg(a, std::vector<double>(b, b + n), std::vector<double>(x0, x0 + n), settings(s));
+  resultOut(r, x, meta, hist, cap);
+}
+void orc_bicgstab(int64_t n, const int64_t* rp, const int64_t* col, const double* val,
+                  const double* b, const double* x0, const double* s, double* x, double* meta,
+                  double* hist, int64_t cap) {
+  const Csr a = csrFrom(n, rp, col, val);
+  const Result r =
+      bicgstab(a, std::vector<double>(b, b + n), std::vector<double>(x0, x0 + n), settings(s));
+  resultOut(r, x, meta, hist, cap);
+}
+// CSR from triplets (SparseMatrixBuilder): returns nnz.
+int64_t orc_csr_build(int64_t n, int64_t nt, const int64_t* r, const int64_t* c, const double* v,
+                      int64_t* rp, int64_t* col, double* val) {
+  Builder B(n);
+  for (int64_t k = 0; k < nt; ++k) B.add(r[k], c[k], v[k]);
+  return csrOut(B.build(), rp, col, val);
+}
+
+void orc_mass_flux(void* h, const double* ux, const double* uy, const double* uz, double rho,
+                   double* F) {
+  auto* c = static_cast<Case*>(h);
+  std::vector<V3> u;
+  toV3(c->mesh, ux, uy, uz, u);
+  const auto out = massFlux(c->mesh, u, c->bc, rho);
+  std::copy(out.begin(), out.end(), F);
+}
+// kind 0: pressure boundaries, 1: p' boundaries (makeGradientBoundaries).
+void orc_gradient(void* h, const double* phi, int kind, double* gx, double* gy, double* gz) {
+  auto* c = static_cast<Case*>(h);
+  const std::vector<double> f(phi, phi + c->mesh.nCells());
+  const auto g = gradient(c->mesh, f, kind == 0 ? pressureBC(c->bc) : correctionBC(c->bc));
+  fromV3(g, gx, gy, gz);
+}
+void orc_velocity_gradient(void* h, const double* ux, const double* uy, const double* uz,
+                           double* gu, double* gv, double* gw) {
+  auto* c = static_cast<Case*>(h);
+  std::vector<V3> u, a, b2, w;
+  toV3(c->mesh, ux, uy, uz, u);
+  velocityGradient(c->mesh, u, c->bc, a, b2, w);
+  fromV3(a, gu, gu + c->mesh.nCells(), gu + 2 * c->mesh.nCells());
+  fromV3(b2, gv, gv + c->mesh.nCells(), gv + 2 * c->mesh.nCells());
+  if (c->mesh.dim == 3) fromV3(w, gw, gw + c->mesh.nCells(), gw + 2 * c->mesh.nCells());
+}
+// Returns nnz (or -1 when the assembled system is non-finite).
+int64_t orc_momentum(void* h, int component, const double* ux, const double* uy, const double* uz,
+                     const double* p, const double* F, double mu, const double* prev, double alpha,
+                     int64_t* rp, int64_t* col, double* val, double* rhs, double* diag) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  std::vector<V3> u;
+  toV3(m, ux, uy, uz, u);
+  const Momentum M = assembleMomentum(m, u, std::vector<double>(p, p + m.nCells()),
+                                      std::vector<double>(F, F + m.nFaces()),
+                                      std::vector<double>(m.nCells(), mu + 0.0), c->bc, component,
+                                      std::vector<double>(prev, prev + m.nCells()), alpha);
+  std::copy(M.rhs.begin(), M.rhs.end(), rhs);
+  std::copy(M.diag.begin(), M.diag.end(), diag);
+  const int64_t nnz = csrOut(M.A, rp, col, val);
+  return M.ok ? nnz : -1;
+}
+void orc_response(void* h, const double* diag, double* d) {
+  auto* c = static_cast<Case*>(h);
+  const auto out = response(c->mesh, std::vector<double>(diag, diag + c->mesh.nCells()));
+  std::copy(out.begin(), out.end(), d);
+}
+void orc_rhie_chow(void* h, const double* ux, const double* uy, const double* uz, const double* p,
+                   const double* du, const double* dv, const double* dw, double rho, double alpha,
+                   double* F) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  const Index n = m.nCells();
+  std::vector<V3> u;
+  toV3(m, ux, uy, uz, u);
+  const std::vector<double> pv(p, p + n), dU(du, du + n), dV(dv, dv + n);
+  std::vector<double> dW;
+  if (m.dim == 3) dW.assign(dw, dw + n);
+  const auto gp = gradient(m, pv, pressureBC(c->bc));
+  const auto out = rhieChowFlux(m, u, pv, gp, dU, dV, m.dim == 3 ? &dW : nullptr, c->bc, rho, alpha);
+  std::copy(out.begin(), out.end(), F);
+}
+int64_t orc_pressure_assemble(void* h, const double* Fstar, const double* du, const double* dv,
+                              const double* dw, double rho, int64_t ref, int64_t* rp, int64_t* col,
+                              double* val, double* rhs, double* faceCoeff) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  const Index n = m.nCells();
+  const std::vector<double> dU(du, du + n), dV(dv, dv + n);
+  std::vector<double> dW;
+  if (m.dim == 3) dW.assign(dw, dw + n);
+  const PressureSystem ps = assemblePressure(m, std::vector<double>(Fstar, Fstar + m.nFaces()), dU,
+                                             dV, m.dim == 3 ? &dW : nullptr, rho, ref, c->bc);
+  std::copy(ps.rhs.begin(), ps.rhs.end(), rhs);
+  std::copy(ps.faceCoeff.begin(), ps.faceCoeff.end(), faceCoeff);
+  return csrOut(ps.A, rp, col, val);
+}
+void orc_correct(void* h, const double* usx, const double* usy, const double* usz, const double* du,
+                 const double* dv, const double* dw, const double* pp, const double* Fstar,
+                 const double* coeff, double* ux, double* uy, double* uz, double* Fout) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  const Index n = m.nCells();
+  std::vector<V3> us;
+  toV3(m, usx, usy, usz, us);
+  const std::vector<double> dU(du, du + n), dV(dv, dv + n), ppv(pp, pp + n);
+  std::vector<double> dW;
+  if (m.dim == 3) dW.assign(dw, dw + n);
+  const auto u = correctVelocity(m, us, dU, dV, m.dim == 3 ? &dW : nullptr, ppv, c->bc, nullptr);
+  fromV3(u, ux, uy, m.dim == 3 ? uz : nullptr);
+  const auto F = correctFlux(m, std::vector<double>(Fstar, Fstar + m.nFaces()),
+                             std::vector<double>(coeff, coeff + m.nFaces()), ppv);
+  std::copy(F.begin(), F.end(), Fout);
+}
+void orc_continuity(void* h, const double* F, double* imbalance, double* scalars) {
+  auto* c = static_cast<Case*>(h);
+  const Continuity ct = continuity(c->mesh, std::vector<double>(F, F + c->mesh.nFaces()));
+  std::copy(ct.imbalance.begin(), ct.imbalance.end(), imbalance);
+  double ss = 0.0;
+  for (double v : ct.imbalance) ss += v * v;
+  scalars[0] = std::sqrt(ss / static_cast<double>(ct.imbalance.size()));
+  scalars[1] = ct.global;
+}
+// settings: {maxIter, au, ap, vtol, ptol, ctol, rho, mu, ref, rhieChow, presCG,
+//            mom abs, rel, max, jacobi, pres abs, rel, max, jacobi}
+// meta out: {status, iterations, momentumIters, pressureIters, finalContinuity, global}
+void orc_simple(void* h, const double* s, const double* ux0, const double* uy0, const double* uz0,
+                const double* p0, double* ux, double* uy, double* uz, double* p, double* F,
+                double* meta, double* hist, int64_t cap) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  SimpleSettings ss;
+  ss.maxIter = static_cast<Index>(s[0]);
+  ss.au = s[1];
+  ss.ap = s[2];
+  ss.vtol = s[3];
+  ss.ptol = s[4];
+  ss.ctol = s[5];
+  ss.rho = s[6];
+  ss.mu = s[7];
+  ss.ref = static_cast<Index>(s[8]);
+  ss.rhieChow = s[9] != 0.0;
+  ss.presCG = s[10] != 0.0;
+  ss.mom = settings(s + 11);
+  ss.pres = settings(s + 15);
+  std::vector<V3> u0;
+  toV3(m, ux0, uy0, uz0, u0);
+  const SimpleResult R = simple(m, c->bc, ss, u0, std::vector<double>(p0, p0 + m.nCells()));
+  fromV3(R.u, ux, uy, m.dim == 3 ? uz : nullptr);
+  std::copy(R.p.begin(), R.p.end(), p);
+  std::copy(R.F.begin(), R.F.end(), F);
+  meta[0] = R.status;
+  meta[1] = static_cast<double>(R.iterations);
+  meta[2] = static_cast<double>(R.momentumIters);
+  meta[3] = static_cast<double>(R.pressureIters);
+  meta[4] = R.finalContinuity;
+  meta[5] = R.global;
+  // hist rows: u, v, w, p, c; each `cap` long.
+  const std::vector<double>* rows[5] = {&R.hu, &R.hv, &R.hw, &R.hp, &R.hc};
+  for (int r = 0; r < 5; ++r)
+    for (int64_t i = 0; i < cap && i < static_cast<int64_t>(rows[r]->size()); ++i)
+      hist[r * cap + i] = (*rows[r])[i];
+}
+
+}  // extern "C"

@@ -1,0 +1,455 @@
+// Golden-vector driver for the reference solver (checker only).
+//
+// Linked against the reference's own CPU sources (oracle/Makefile.ref builds
+// them from where they lie); never shipped, never run on the GPU path. Each
+// command runs one piece of the SIMPLE pressure-velocity path through the
+// reference's public API and dumps every array as raw little-endian files
+// (<name>.f64 / <name>.i64) into out=DIR, for tests/golden/make_golden.py to
+// pack into fixtures.
+//
+//   mesh    dim nx ny [nz]                     -> owner/neighbour/geometry (a1)
+//   pcorr   dim nx ny [nz] seed                -> p' system with d == 1 (a9) + CG (a12)
+//   simple  dim nx ny [nz] re iters flux psolver -> SIMPLE::solve fields+histories (a15)
+//   stages  dim nx ny [nz] re warm flux psolver  -> one hand-driven SIMPLE iteration,
+//                                                 every stage dumped (a3-a14)
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cfd/algebra/BiCGSTAB.hpp"
+#include "cfd/algebra/CG.hpp"
+#include "cfd/algebra/Preconditioner.hpp"
+#include "cfd/boundary/FixedGradient.hpp"
+#include "cfd/boundary/MovingWall.hpp"
+#include "cfd/boundary/Wall.hpp"
+#include "cfd/discretization/Gradient.hpp"
+#include "cfd/mesh/MeshGeometry.hpp"
+#include "cfd/physics/ContinuityEquation.hpp"
+#include "cfd/physics/FluidProperties.hpp"
+#include "cfd/physics/MassFlux.hpp"
+#include "cfd/pressure_velocity/PressureCorrectionEquation.hpp"
+#include "cfd/pressure_velocity/RelaxedMomentum.hpp"
+#include "cfd/pressure_velocity/RhieChow.hpp"
+#include "cfd/pressure_velocity/SIMPLE.hpp"
+
+using namespace cfd;
+using cfd::algebra::LinearSolverSettings;
+using cfd::algebra::SolverResult;
+using cfd::algebra::SparseMatrix;
+using cfd::algebra::Vector;
+using cfd::fields::ScalarField;
+using cfd::fields::SurfaceField;
+using cfd::fields::VectorField;
+using cfd::mesh::Mesh;
+using cfd::mesh::MeshGeometry;
+
+namespace {
+
+std::map<std::string, std::string> args;
+std::string outDir;
+
+std::string arg(const std::string& k, const std::string& def = "") {
+  auto it = args.find(k);
+  if (it == args.end()) {
+    if (def.empty()) {
+      std::fprintf(stderr, "missing argument %s\n", k.c_str());
+      std::exit(2);
+    }
+    return def;
+  }
+  return it->second;
+}
+long iarg(const std::string& k, const std::string& d = "") { return std::stol(arg(k, d)); }
+double farg(const std::string& k, const std::string& d = "") { return std::stod(arg(k, d)); }
+
+void dumpRaw(const std::string& name, const void* data, size_t bytes) {
+  const std::string path = outDir + "/" + name;
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    std::perror(path.c_str());
+    std::exit(1);
+  }
+  if (bytes) std::fwrite(data, 1, bytes, f);
+  std::fclose(f);
+}
+void dumpF(const std::string& name, const std::vector<double>& v) {
+  dumpRaw(name + ".f64", v.data(), v.size() * sizeof(double));
+}
+void dumpI(const std::string& name, const std::vector<int64_t>& v) {
+  dumpRaw(name + ".i64", v.data(), v.size() * sizeof(int64_t));
+}
+template <class F>
+std::vector<double> toVec(const F& field) {
+  std::vector<double> v(field.size());
+  for (size_t i = 0; i < field.size(); ++i) v[i] = field[i];
+  return v;
+}
+void dumpVecField(const std::string& name, const VectorField& f, bool threeD) {
+  std::vector<double> x(f.size()), y(f.size()), z(f.size());
+  for (size_t i = 0; i < f.size(); ++i) {
+    x[i] = f[i].x;
+    y[i] = f[i].y;
+    z[i] = f[i].z;
+  }
+  dumpF(name + "_x", x);
+  dumpF(name + "_y", y);
+  if (threeD) dumpF(name + "_z", z);
+}
+void dumpMatrix(const std::string& name, const SparseMatrix& m) {
+  std::vector<int64_t> rp(m.rows() + 1), col(m.nonZeros());
+  std::vector<double> val(m.nonZeros());
+  for (size_t i = 0; i <= m.rows(); ++i) rp[i] = static_cast<int64_t>(m.rowOffsetsData()[i]);
+  for (size_t k = 0; k < m.nonZeros(); ++k) {
+    col[k] = static_cast<int64_t>(m.columnIndicesData()[k]);
+    val[k] = m.valuesData()[k];
+  }
+  dumpI(name + "_rowptr", rp);
+  dumpI(name + "_col", col);
+  dumpF(name + "_val", val);
+}
+void dumpResult(const std::string& name, const SolverResult& r) {
+  dumpF(name + "_x", toVec(r.solution));
+  dumpF(name + "_hist", r.residualHistory);
+  dumpI(name + "_meta", {static_cast<int64_t>(r.status), static_cast<int64_t>(r.iterations),
+                         static_cast<int64_t>(r.restarts)});
+  dumpF(name + "_res", {r.initialResidual, r.finalResidual});
+}
+
+// splitmix64 -> uniform [0,1) with 53 bits: (x >> 11) * 2^-53.
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }
+};
+
+struct Case {
+  int dim;
+  Mesh mesh;
+  boundary::BoundaryConditionSet vbc;
+  boundary::BoundaryConditionSet pbc;
+};
+
+Case makeCavity() {
+  const int dim = static_cast<int>(iarg("dim", "2"));
+  const size_t nx = iarg("nx"), ny = iarg("ny");
+  Mesh mesh = dim == 3 ? MeshGeometry::createCartesian3D(nx, ny, iarg("nz"), 1.0, 1.0, 1.0)
+                       : MeshGeometry::createCartesian2D(nx, ny, 1.0, 1.0);
+  boundary::BoundaryConditionSet vbc, pbc;
+  const double lid = farg("lid", "1.0");
+  for (const auto& patch : mesh.boundaryPatches()) {
+    const std::string& name = patch.name();
+    const bool isLid = (dim == 2) ? name == "top" : name == "ymax";
+    if (isLid)
+      vbc.set(mesh, name, std::make_unique<boundary::MovingWall>(Vector3{lid, 0.0, 0.0}));
+    else
+      vbc.set(mesh, name, std::make_unique<boundary::Wall>());
+    pbc.set(mesh, name, std::make_unique<boundary::FixedGradient>(0.0));
+  }
+  return Case{dim, std::move(mesh), std::move(vbc), std::move(pbc)};
+}
+
+LinearSolverSettings momentumSettings() {
+  LinearSolverSettings s;
+  s.type = algebra::LinearSolverType::BiCGSTAB;
+  s.absoluteTolerance = farg("mabs", "1e-10");
+  s.relativeTolerance = farg("mrel", "1e-8");
+  s.maxIterations = iarg("mmax", "500");
+  return s;
+}
+
+LinearSolverSettings pressureSettings() {
+  LinearSolverSettings s;
+  const std::string kind = arg("psolver", "cg_jacobi");
+  s.type = kind == "bicgstab" ? algebra::LinearSolverType::BiCGSTAB : algebra::LinearSolverType::CG;
+  s.preconditioner = kind == "cg_jacobi" ? algebra::PreconditionerType::Jacobi
+                                         : algebra::PreconditionerType::None;
+  s.absoluteTolerance = farg("pabs", kind == "bicgstab" ? "1e-10" : "0");
+  s.relativeTolerance = farg("prel", "1e-8");
+  s.maxIterations = iarg("pmax", "20000");
+  return s;
+}
+
+pressure_velocity::SIMPLESettings simpleSettings() {
+  pressure_velocity::SIMPLESettings s;
+  s.maxIterations = iarg("iters", "1");
+  s.velocityRelaxation = farg("au", "0.7");
+  s.pressureRelaxation = farg("ap", "0.3");
+  s.velocityTolerance = farg("vtol", "1e-6");
+  s.pressureTolerance = farg("ptol", "1e-6");
+  s.continuityTolerance = farg("ctol", "1e-6");
+  s.momentumSolver = momentumSettings();
+  s.pressureSolver = pressureSettings();
+  const std::string flux = arg("flux", "rhie_chow");
+  s.faceFlux = pressure_velocity::parseFaceFluxScheme(flux);
+  return s;
+}
+
+void cmdMesh() {
+  const Case c = makeCavity();
+  const Mesh& m = c.mesh;
+  std::vector<int64_t> owner, neighbor, cellFaces, cellFaceOffsets{0};
+  std::vector<double> fcx, fcy, fcz, sx, sy, sz, ccx, ccy, ccz, vol;
+  for (const auto& f : m.faces()) {
+    owner.push_back(static_cast<int64_t>(f.owner()));
+    neighbor.push_back(f.neighbor().has_value() ? static_cast<int64_t>(*f.neighbor()) : -1);
+    fcx.push_back(f.centroid().x);
+    fcy.push_back(f.centroid().y);
+    fcz.push_back(f.centroid().z);
+    sx.push_back(f.areaVector().x);
+    sy.push_back(f.areaVector().y);
+    sz.push_back(f.areaVector().z);
+  }
+  for (const auto& cell : m.cells()) {
+    ccx.push_back(cell.centroid().x);
+    ccy.push_back(cell.centroid().y);
+    ccz.push_back(cell.centroid().z);
+    vol.push_back(cell.volume());
+    for (Index fid : cell.faceIds()) cellFaces.push_back(static_cast<int64_t>(fid));
+    cellFaceOffsets.push_back(static_cast<int64_t>(cellFaces.size()));
+  }
+  std::vector<int64_t> patchFaces, patchOffsets{0};
+  for (const auto& p : m.boundaryPatches()) {
+    for (Index fid : p.faceIds()) patchFaces.push_back(static_cast<int64_t>(fid));
+    patchOffsets.push_back(static_cast<int64_t>(patchFaces.size()));
+  }
+  dumpI("owner", owner);
+  dumpI("neighbor", neighbor);
+  dumpI("cell_faces", cellFaces);
+  dumpI("cell_face_offsets", cellFaceOffsets);
+  dumpI("patch_faces", patchFaces);
+  dumpI("patch_offsets", patchOffsets);
+  dumpF("face_cx", fcx);
+  dumpF("face_cy", fcy);
+  dumpF("face_cz", fcz);
+  dumpF("face_sx", sx);
+  dumpF("face_sy", sy);
+  dumpF("face_sz", sz);
+  dumpF("cell_cx", ccx);
+  dumpF("cell_cy", ccy);
+  dumpF("cell_cz", ccz);
+  dumpF("cell_vol", vol);
+}
+
+// p' operator with d == 1 (rho = 1, cell 0 pinned) and a seeded, mean-free RHS,
+// solved by CG + Jacobi; the face flux passed in is a seeded field too so the
+// assembled RHS (-continuity imbalance) is non-trivial.
+void cmdPcorr() {
+  const Case c = makeCavity();
+  const Mesh& m = c.mesh;
+  const size_t n = m.numberOfCells(), nf = m.numberOfFaces();
+  SplitMix64 rng{static_cast<uint64_t>(iarg("seed", "20480001"))};
+  SurfaceField flux(nf, 0.0);
+  for (size_t f = 0; f < nf; ++f) flux[f] = 2.0 * rng.uniform() - 1.0;
+  ScalarField ones(n, 1.0);
+  pressure_velocity::PressureCorrectionOptions opts;
+  const auto asmb = pressure_velocity::assemblePressureCorrection(
+      m, flux, ones, ones, 1.0, 0, c.pbc, opts, c.dim == 3 ? &ones : nullptr);
+  dumpF("flux", toVec(flux));
+  dumpMatrix("A", asmb.system.matrix());
+  dumpF("rhs", toVec(asmb.system.rhs()));
+  dumpF("face_coeff", toVec(asmb.faceCoefficient));
+
+  // Standalone solve: b_i = 2u_i - 1 from the seed, serial mean removed, b_0 = 0.
+  Vector b(n);
+  for (size_t i = 0; i < n; ++i) b[i] = 2.0 * rng.uniform() - 1.0;
+  double sum = 0.0;
+  for (size_t i = 0; i < n; ++i) sum += b[i];
+  const double mean = sum / static_cast<double>(n);
+  for (size_t i = 0; i < n; ++i) b[i] -= mean;
+  b[0] = 0.0;
+  dumpF("b", toVec(b));
+  LinearSolverSettings s;
+  s.type = algebra::LinearSolverType::CG;
+  s.absoluteTolerance = 0.0;
+  s.relativeTolerance = farg("prel", "1e-8");
+  s.maxIterations = iarg("pmax", "20000");
+  algebra::CG cg(s, std::make_shared<algebra::JacobiPreconditioner>());
+  const algebra::LinearSystem sys(asmb.system.matrix(), b);
+  const auto r = cg.solve(sys);
+  dumpResult("cg", r);
+  const Vector Ax = asmb.system.matrix().multiply(r.solution);
+  dumpF("Ax", toVec(Ax));
+}
+
+void cmdSimple() {
+  const Case c = makeCavity();
+  const double re = farg("re", "100");
+  const physics::FluidProperties fluid(1.0, farg("lid", "1.0") / re);
+  pressure_velocity::SIMPLE simple(simpleSettings(), 0);
+  const size_t n = c.mesh.numberOfCells();
+  const auto res = simple.solve(c.mesh, fluid, c.vbc, c.pbc, VectorField(n, Vector3{}),
+                                ScalarField(n, 0.0));
+  dumpVecField("u", res.velocity, c.dim == 3);
+  dumpF("p", toVec(res.pressure));
+  dumpF("F", toVec(res.massFlux));
+  dumpF("hist_u", res.uResidualHistory);
+  dumpF("hist_v", res.vResidualHistory);
+  dumpF("hist_w", res.wResidualHistory);
+  dumpF("hist_p", res.pressureResidualHistory);
+  dumpF("hist_c", res.continuityHistory);
+  dumpI("meta", {static_cast<int64_t>(res.status), static_cast<int64_t>(res.iterations),
+                 static_cast<int64_t>(res.momentumLinearIterations),
+                 static_cast<int64_t>(res.pressureLinearIterations)});
+  dumpF("scalars", {res.finalContinuityResidual, res.globalMassImbalance});
+}
+
+// One outer iteration driven by hand through the same calls SIMPLE::solve makes
+// on its CPU path, starting from the state after `warm` SIMPLE iterations.
+void cmdStages() {
+  const Case c = makeCavity();
+  const Mesh& m = c.mesh;
+  const bool threeD = c.dim == 3;
+  const size_t n = m.numberOfCells();
+  const double re = farg("re", "100");
+  const physics::FluidProperties fluid(1.0, farg("lid", "1.0") / re);
+  const auto settings = simpleSettings();
+  const double au = settings.velocityRelaxation, ap = settings.pressureRelaxation;
+
+  VectorField velocity(n, Vector3{});
+  ScalarField pressure(n, 0.0);
+  SurfaceField massFlux;
+  const long warm = iarg("warm", "0");
+  if (warm > 0) {
+    auto s = settings;
+    s.maxIterations = static_cast<Index>(warm);
+    pressure_velocity::SIMPLE simple(s, 0);
+    const auto res = simple.solve(m, fluid, c.vbc, c.pbc, velocity, pressure);
+    velocity = res.velocity;
+    pressure = res.pressure;
+    massFlux = res.massFlux;
+  } else {
+    massFlux = physics::calculateMassFlux(m, velocity, fluid, c.vbc);
+  }
+  dumpVecField("in_u", velocity, threeD);
+  dumpF("in_p", toVec(pressure));
+  dumpF("in_F", toVec(massFlux));
+
+  const ScalarField mu(n, fluid.dynamicViscosity());
+  const auto comp = [&](const VectorField& v, int k) {
+    ScalarField s(n);
+    for (size_t i = 0; i < n; ++i) s[i] = k == 0 ? v[i].x : (k == 1 ? v[i].y : v[i].z);
+    return s;
+  };
+  const int ncomp = threeD ? 3 : 2;
+  const char* names[3] = {"mu", "mv", "mw"};
+  std::vector<Vector> solutions;
+  std::vector<ScalarField> response;
+  algebra::BiCGSTAB momentumSolver(settings.momentumSolver);
+  for (int k = 0; k < ncomp; ++k) {
+    const ScalarField prev = comp(velocity, k);
+    const auto asmb = pressure_velocity::assembleRelaxedMomentumComponent(
+        m, velocity, pressure, massFlux, mu, c.vbc, c.pbc,
+        static_cast<physics::VelocityComponent>(k), prev, au);
+    dumpMatrix(names[k], asmb.system.matrix());
+    dumpF(std::string(names[k]) + "_rhs", toVec(asmb.system.rhs()));
+    dumpF(std::string(names[k]) + "_diag", toVec(asmb.diagonal));
+    Vector x0(n);
+    for (size_t i = 0; i < n; ++i) x0[i] = prev[i];
+    const auto r = momentumSolver.solve(asmb.system, x0);
+    dumpResult(std::string(names[k]) + "_solve", r);
+    solutions.push_back(r.solution);
+    response.push_back(pressure_velocity::computeMomentumResponseCoefficient(m, asmb.diagonal));
+    dumpF(std::string("d") + "uvw"[k], toVec(response.back()));
+  }
+  VectorField vstar(n);
+  for (size_t i = 0; i < n; ++i)
+    vstar[i] = Vector3{solutions[0][i], solutions[1][i], threeD ? solutions[2][i] : 0.0};
+
+  const auto faceFlux = pressure_velocity::resolveFaceFluxScheme(settings.faceFlux, c.dim);
+  const ScalarField* dW = threeD ? &response[2] : nullptr;
+  SurfaceField predictor;
+  const VectorField gradP =
+      discretization::gradient(m, pressure, c.pbc, discretization::GradientScheme::GreenGauss);
+  dumpVecField("gradp", gradP, threeD);
+  if (faceFlux == pressure_velocity::FaceFluxScheme::RhieChow) {
+    predictor = pressure_velocity::rhieChowMassFlux(m, vstar, pressure, gradP, response[0],
+                                                    response[1], dW, fluid, c.vbc, au);
+  } else {
+    predictor = physics::calculateMassFlux(m, vstar, fluid, c.vbc);
+  }
+  dumpF("Fstar", toVec(predictor));
+  dumpF("Flinear", toVec(physics::calculateMassFlux(m, vstar, fluid, c.vbc)));
+
+  pressure_velocity::PressureCorrectionOptions opts;
+  const auto pasm = pressure_velocity::assemblePressureCorrection(
+      m, predictor, response[0], response[1], fluid.density(), 0, c.pbc, opts, dW);
+  dumpMatrix("pc", pasm.system.matrix());
+  dumpF("pc_rhs", toVec(pasm.system.rhs()));
+  dumpF("pc_face_coeff", toVec(pasm.faceCoefficient));
+
+  const auto pset = settings.pressureSolver;
+  std::unique_ptr<algebra::LinearSolver> psolver;
+  if (pset.type == algebra::LinearSolverType::CG)
+    psolver = std::make_unique<algebra::CG>(
+        pset, pset.preconditioner == algebra::PreconditionerType::Jacobi
+                  ? std::make_shared<algebra::JacobiPreconditioner>()
+                  : nullptr);
+  else
+    psolver = std::make_unique<algebra::BiCGSTAB>(pset);
+  const auto pr = psolver->solve(pasm.system);
+  dumpResult("pc_solve", pr);
+  ScalarField pprime(n);
+  for (size_t i = 0; i < n; ++i) pprime[i] = pr.solution[i];
+
+  ScalarField pnew(n);
+  for (size_t i = 0; i < n; ++i) pnew[i] = pressure[i] + (ap * pprime[i]);
+  dumpF("p_new", toVec(pnew));
+  const VectorField gradPP = discretization::gradient(
+      m, pprime,
+      [&] {
+        boundary::BoundaryConditionSet b;
+        for (const auto& patch : m.boundaryPatches())
+          b.set(m, patch.name(), std::make_unique<boundary::FixedGradient>(0.0));
+        return b;
+      }(),
+      discretization::GradientScheme::GreenGauss);
+  dumpVecField("gradpp", gradPP, threeD);
+  const VectorField unew = pressure_velocity::correctVelocity(
+      m, vstar, response[0], response[1], pprime, c.pbc,
+      discretization::GradientScheme::GreenGauss, dW);
+  dumpVecField("u_new", unew, threeD);
+  const SurfaceField fnew =
+      pressure_velocity::correctFaceMassFlux(m, predictor, pasm.faceCoefficient, pprime, nullptr);
+  dumpF("F_new", toVec(fnew));
+  const auto cont = physics::evaluateContinuity(m, fnew);
+  dumpF("imbalance", toVec(cont.cellImbalance));
+  double ss = 0.0;
+  for (size_t i = 0; i < n; ++i) ss += cont.cellImbalance[i] * cont.cellImbalance[i];
+  dumpF("cont_scalars", {std::sqrt(ss / static_cast<double>(n)), cont.globalNetFlux});
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: ref_driver cmd key=value ... out=DIR\n");
+    return 2;
+  }
+  for (int i = 2; i < argc; ++i) {
+    std::string a = argv[i];
+    const auto eq = a.find('=');
+    if (eq == std::string::npos) continue;
+    args[a.substr(0, eq)] = a.substr(eq + 1);
+  }
+  outDir = arg("out");
+  const std::string cmd = argv[1];
+  if (cmd == "mesh") cmdMesh();
+  else if (cmd == "pcorr") cmdPcorr();
+  else if (cmd == "simple") cmdSimple();
+  else if (cmd == "stages") cmdStages();
+  else {
+    std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
+    return 2;
+  }
+  return 0;
+}
