@@ -1,0 +1,203 @@
+"""Benchmark: the 2048x2048 stand-alone p' solve and SIMPLE iterations on one MI355X.
+
+Workload (the plain method at scale, FP64 on one GPU): 2D lid-driven cavity,
+2048x2048 uniform cells. A step is one stand-alone pressure-correction solve on
+the 5-point p' operator (d = 1, rho = 1, cell 0 pinned) with an RHS drawn
+seeded-uniform in [-1, 1] (splitmix64, seed 20480001), its serial mean removed
+and b_0 = 0, solved from x0 = 0 by Jacobi-PCG to |r| <= 1e-8 |b| (FP64). The
+operator and RHS are resident in HBM before the timed region. Also reported:
+SIMPLE outer iterations per second at 2048x2048, Re = 1000 (Rhie-Chow,
+upwind, alpha_u 0.7, alpha_p 0.3, BiCGSTAB momentum, CG+Jacobi p').
+
+Prints one JSON line (see the repository's bench contract).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+N = 2048
+SEED = 20480001
+HBM_PEAK_GBS = 8000.0
+
+
+def splitmix_uniform(seed, count):
+    """splitmix64 -> (x >> 11) * 2^-53, vectorised (same stream as the C driver)."""
+    out = np.empty(count, dtype=np.uint64)
+    s = np.uint64(seed)
+    golden = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, count + 1, dtype=np.uint64)
+        z = s + golden * idx
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    out[:] = z
+    return (out >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def standalone_rhs(n_cells, n_faces=0, offset=0):
+    """b_i = 2 u_i - 1 from draws offset.., serial mean removed, b_0 = 0."""
+    u = splitmix_uniform(SEED, offset + n_cells)[offset:]
+    b = 2.0 * u - 1.0
+    s = 0.0
+    for chunk in np.array_split(b, 64):  # serial mean in index order
+        for v in chunk.tolist():
+            s += v
+    b -= s / n_cells
+    b[0] = 0.0
+    return b
+
+
+def cpu_baseline(iters_gpu):
+    """The reference's own CG + Jacobi (compiled from its sources, OpenMP SpMV),
+    bounded sample of 40 iterations at 2048x2048, scaled to the GPU's iteration
+    count; falls back to the oracle restatement when the reference build is absent."""
+    driver = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+    cores = min(os.cpu_count() or 1, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    k = 40
+    if os.path.exists(driver):
+        out = subprocess.run([driver, "cgtime", "dim=2", f"nx={N}", f"ny={N}", f"seed={SEED}",
+                              f"pmax={k}"], capture_output=True, text=True, env=env,
+                             timeout=600, check=True).stdout
+        rec = json.loads(out.strip().splitlines()[-1])
+        per_iter = rec["seconds"] / max(rec["iterations"], 1)
+        kind = "reference"
+    else:
+        import oracle
+        c = oracle.Case.cavity(2, N, N)
+        ones = np.ones(c.n_cells)
+        A, _, _ = c.pressure_system(np.zeros(c.n_faces), ones, ones)
+        b = standalone_rhs(c.n_cells)
+        t0 = time.perf_counter()
+        r = A.cg(b, max_iter=k)
+        per_iter = (time.perf_counter() - t0) / max(r.iterations, 1)
+        kind = "port"
+    return {"value": per_iter * iters_gpu, "unit": "s", "cores": cores, "kind": kind,
+            "sample": f"{k} CG+Jacobi iterations on the 2048x2048 p' system "
+                      f"({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU solve's "
+                      f"{iters_gpu} iterations"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--simple-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU decomposition is a design, not built", file=sys.stderr)
+        sys.exit(2)
+
+    import cfdapp_amd as cfd
+    if not cfd.device_available():
+        sys.exit("bench.py: no gfx950 device")
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    # --- resident operator + RHS ------------------------------------------------
+    d = cfd.GpuSimpleDiscretization.cavity(2, N, N)
+    n, nf = d.n_cells, d.n_faces
+    d.upload("RESPONSE", np.ones(n))
+    d.upload("FLUX_STAR", np.zeros(nf))
+    d.assemble_pressure_correction_resident(0)
+    b = standalone_rhs(n)
+    d.upload("PC_RHS", b)
+    pcg = cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi", check_interval=64)
+    log("setup done")
+
+    for w in range(args.warmup):
+        r = d.solve_pressure_rhs(pcg, None, download=False)
+        log(f"warmup {w}: {r.status_name} {r.iterations} it {r.device_ms:.1f} ms")
+
+    d._ctx.call("cfdhip_synchronize")
+    times, iters, results = [], [], []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        r = d.solve_pressure_rhs(pcg, None, download=False)
+        results.append(r)
+        iters.append(r.iterations)
+    d._ctx.call("cfdhip_synchronize")
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    r = results[-1]
+    log(f"timed: {args.steps} solves, {ms_per_step:.1f} ms/solve, {r.iterations} it, "
+        f"true res {r.true_residual:.3e} (|b| {np.linalg.norm(b):.3e})")
+
+    # --- dominant kernel timing (p' SpMV fused with p.Ap), HIP events ------------
+    d.set_timing(True)
+    rt = d.solve_pressure_rhs(pcg, None, download=False)
+    kt = d.kernel_timing()
+    d.set_timing(False)
+    spmv_us = kt["mean_us"]
+    # algorithmic bytes of one SpMV in this operator format: diagonal (n),
+    # face couplings (nf), x (n) read, y (n) written -- all FP64
+    spmv_bytes = 8.0 * (n + nf + n + n)
+    achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9 if spmv_us > 0 else 0.0
+    log(f"SpMV kernel: {kt['launches']:.0f} launches, mean {spmv_us:.2f} us, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    # --- SIMPLE outer iterations at Re = 1000 --------------------------------------
+    simple = {}
+    if args.simple_iters > 0:
+        ds = cfd.GpuSimpleDiscretization.cavity(2, N, N, viscosity=1e-3)
+        t1 = time.perf_counter()
+        res = ds.simple_run(args.simple_iters,
+                            pressure=cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi",
+                                                              check_interval=64))
+        dt = time.perf_counter() - t1
+        simple = {"simple_iter_per_s": res["iterations"] / dt,
+                  "simple_iterations": res["iterations"],
+                  "simple_status": res["status"],
+                  "simple_pressure_iterations": res["pressure_iterations"]}
+        log(f"SIMPLE: {res['iterations']} outer it in {dt:.2f} s, status {res['status']}, "
+            f"p' iterations {res['pressure_iterations']}")
+        ds.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(int(np.median(iters)))
+        log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['kind']}, {cpu['cores']} cores)")
+
+    line = {
+        "metric": "p' solve time to 1e-8 rel.res + SIMPLE iter/s: learned FCG vs Jacobi-PCG",
+        "value": ms_per_step / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D cavity 2048x2048: stand-alone p' solve, Jacobi-PCG FP64 to "
+                               "1e-8 relative residual (seeded mean-free RHS)",
+                   "cells": n, "solver": "CG+Jacobi", "precision": "fp64",
+                   "iterations": int(np.median(iters)),
+                   "true_relative_residual": r.true_residual / np.linalg.norm(b),
+                   **simple},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "cgSpmvKernel<PressureOp> (p' SpMV fused with p.Ap)",
+                     "kernel_mean_us": spmv_us, "bytes_per_launch": spmv_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,411 @@
+"""MI355X (gfx950) SIMPLE pressure-velocity path: host-side mirror of the
+reference's linear-solver and GPU-discretization interfaces over libcfdhip.
+
+Names follow the reference's C++ API:
+
+* ``device_available()``                  -- ``cfd::gpu::cudaAvailable``
+  (include/cfd/gpu/GPUBackend.hpp:15)
+* ``LinearSolverSettings`` / ``SolverResult`` -- ``cfd::algebra`` types of the same
+  name (include/cfd/algebra/LinearSolver.hpp:103-166)
+* ``make_gpu_cg`` / ``make_gpu_bicgstab``  -- ``cfd::gpu::makeGpuCG/makeGpuBiCGSTAB``
+  (include/cfd/gpu/GpuLinearSolver.hpp:48,52): ``None`` when no device
+* ``GpuSimpleDiscretization``              -- the resident SIMPLE stage interface
+  (include/cfd/gpu/GpuSimpleDiscretization.hpp:56-262) on a Cartesian case
+
+All arithmetic happens in libcfdhip.so on the device; this module only moves
+host arrays across the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BREAKDOWN, CONVERGED, FIELDS, FIXED_GRADIENT, FIXED_VALUE, FP64, INLET,
+                   INVALID_SYSTEM, MAX_ITERATIONS, MIXED, MOVING_WALL, NONFINITE_INPUT,
+                   NONFINITE_RESIDUAL, OUTLET, PREC_JACOBI, PREC_LEARNED, PREC_NONE, SOLVER_BICGSTAB,
+                   SOLVER_CG, SOLVER_FCG, STATUS_NAMES, WALL)
+
+__all__ = ["device_available", "LinearSolverSettings", "SolverResult", "make_gpu_cg",
+           "make_gpu_bicgstab", "GpuSimpleDiscretization", "CfdHipError"]
+
+
+class CfdHipError(RuntimeError):
+    """A failing libcfdhip call (code + the context's last error message)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"libcfdhip error {code}: {message}")
+        self.code = code
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _pf(a):
+    return None if a is None else a.ctypes.data_as(_lib.P_F64)
+
+
+def _pi(a):
+    return None if a is None else a.ctypes.data_as(_lib.P_I64)
+
+
+def device_available() -> bool:
+    """True when a gfx950 device is usable (never raises)."""
+    try:
+        return bool(_lib.load().cfdhip_device_available())
+    except OSError:
+        return False
+
+
+@dataclass
+class LinearSolverSettings:
+    """cfd::algebra::LinearSolverSettings (LinearSolver.hpp:136-166) + device knobs."""
+    absolute_tolerance: float = 1e-12
+    relative_tolerance: float = 1e-10
+    max_iterations: int = 1000
+    type: str = "BiCGSTAB"          # "CG", "BiCGSTAB", "FCG"
+    preconditioner: str = "None"    # "None", "Jacobi", "Learned"
+    precision: str = "fp64"         # "fp64" or "mixed"
+    check_interval: int = 0
+    stagnation_window: int = 0
+
+    def __post_init__(self):
+        # LinearSolver's constructor validation (src/algebra/LinearSolver.cpp:9-32)
+        if self.absolute_tolerance < 0.0:
+            raise ValueError("LinearSolverSettings: absoluteTolerance must be >= 0")
+        if self.relative_tolerance < 0.0:
+            raise ValueError("LinearSolverSettings: relativeTolerance must be >= 0")
+        if self.absolute_tolerance == 0.0 and self.relative_tolerance == 0.0:
+            raise ValueError("LinearSolverSettings: at least one of absoluteTolerance/"
+                             "relativeTolerance must be > 0")
+        if self.max_iterations < 1:
+            raise ValueError("LinearSolverSettings: maxIterations must be >= 1")
+        if self.type not in ("CG", "BiCGSTAB", "FCG"):
+            raise ValueError(f"unknown solver type {self.type}")
+        if self.preconditioner not in ("None", "Jacobi", "Learned"):
+            raise ValueError(f"unknown preconditioner {self.preconditioner}")
+
+    def c(self) -> _lib.SolverSettings:
+        s = _lib.SolverSettings()
+        s.absolute_tolerance = self.absolute_tolerance
+        s.relative_tolerance = self.relative_tolerance
+        s.max_iterations = int(self.max_iterations)
+        s.type = {"CG": SOLVER_CG, "BiCGSTAB": SOLVER_BICGSTAB, "FCG": SOLVER_FCG}[self.type]
+        s.preconditioner = {"None": PREC_NONE, "Jacobi": PREC_JACOBI,
+                            "Learned": PREC_LEARNED}[self.preconditioner]
+        s.precision = MIXED if self.precision == "mixed" else FP64
+        s.check_interval = int(self.check_interval)
+        s.stagnation_window = int(self.stagnation_window)
+        return s
+
+
+@dataclass
+class SolverResult:
+    """cfd::algebra::SolverResult (LinearSolver.hpp:103-134); backend is always GPU."""
+    solution: np.ndarray | None
+    status: int
+    iterations: int
+    initial_residual: float
+    final_residual: float
+    true_residual: float
+    restarts: int = 0
+    device_ms: float = 0.0
+    fallback_used: bool = False
+    residual_history: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    backend_used: str = "GPU"
+
+    @property
+    def converged(self) -> bool:
+        return self.status == CONVERGED
+
+    @property
+    def status_name(self) -> str:
+        return STATUS_NAMES[self.status]
+
+
+class _Context:
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        self.h = self.lib.cfdhip_create(device)
+        if not self.h:
+            raise CfdHipError(_lib.ERR_NO_DEVICE, "no usable gfx950 device")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cfdhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def call(self, name, *args):
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != _lib.OK:
+            msg = self.lib.cfdhip_last_error(self.h)
+            raise CfdHipError(rc, msg.decode() if msg else "")
+        return rc
+
+    def history(self):
+        n = ctypes.c_int64(0)
+        self.call("cfdhip_solve_history", None, 0, ctypes.byref(n))
+        h = np.zeros(n.value)
+        self.call("cfdhip_solve_history", _pf(h), n.value, ctypes.byref(n))
+        return h
+
+
+def _result(r: _lib.SolveResult, x, hist) -> SolverResult:
+    return SolverResult(solution=x, status=r.status, iterations=r.iterations,
+                        initial_residual=r.initial_residual, final_residual=r.final_residual,
+                        true_residual=r.true_residual, restarts=r.restarts,
+                        device_ms=r.device_ms, fallback_used=bool(r.fallback_used),
+                        residual_history=hist)
+
+
+class GpuLinearSolver:
+    """LinearSolver::solve(system, x0) on the device for a host CSR system."""
+
+    def __init__(self, settings: LinearSolverSettings, ctx: _Context):
+        self.settings = settings
+        self._ctx = ctx
+
+    def solve(self, rowptr, col, val, b, x0=None) -> SolverResult:
+        rowptr, col, val, b = _i64(rowptr), _i64(col), _f64(val), _f64(b)
+        n = len(rowptr) - 1
+        x = np.empty(n)
+        r = _lib.SolveResult()
+        s = self.settings.c()
+        self._ctx.call("cfdhip_csr_solve", ctypes.byref(s), n, _pi(rowptr), _pi(col), _pf(val),
+                       _pf(b), _pf(None if x0 is None else _f64(x0)), _pf(x), ctypes.byref(r))
+        return _result(r, x, self._ctx.history())
+
+    def spmv(self, rowptr, col, val, x):
+        rowptr, col, val, x = _i64(rowptr), _i64(col), _f64(val), _f64(x)
+        y = np.empty(len(rowptr) - 1)
+        self._ctx.call("cfdhip_csr_spmv", len(rowptr) - 1, _pi(rowptr), _pi(col), _pf(val),
+                       _pf(x), _pf(y))
+        return y
+
+
+def _make(settings, kind):
+    if not device_available():
+        return None
+    s = LinearSolverSettings(**{**settings.__dict__, "type": kind})
+    return GpuLinearSolver(s, _Context())
+
+
+def make_gpu_cg(settings: LinearSolverSettings):
+    """cfd::gpu::makeGpuCG: a device CG solver, or None when no device."""
+    return _make(settings, "CG")
+
+
+def make_gpu_bicgstab(settings: LinearSolverSettings):
+    """cfd::gpu::makeGpuBiCGSTAB: a device BiCGSTAB solver, or None when no device."""
+    return _make(settings, "BiCGSTAB")
+
+
+class GpuSimpleDiscretization:
+    """The resident SIMPLE stages on a uniform Cartesian case.
+
+    ``prepare`` mirrors GpuSimpleDiscretization::prepare (all or nothing):
+    it returns (False, reason) for configurations outside the structured path.
+    Patches: 2D left, right, bottom, top; 3D xmin, xmax, ymin, ymax, zmin, zmax.
+    """
+
+    def __init__(self, device=0):
+        self._ctx = _Context(device)
+        self.dim = 0
+        self.n_cells = self.n_faces = self.n_patches = 0
+
+    def close(self):
+        self._ctx.close()
+
+    # -- plan ---------------------------------------------------------------
+    def prepare(self, dim, nx, ny, nz=1, lx=1.0, ly=1.0, lz=1.0, velocity=None, pressure=None,
+                density=1.0, viscosity=0.01):
+        """velocity: per patch (kind, (vx, vy, vz)); pressure: per patch (kind, value)."""
+        try:
+            self._ctx.call("cfdhip_mesh_cartesian", dim, nx, ny, nz, lx, ly, lz)
+        except CfdHipError as e:
+            if e.code == _lib.ERR_UNSUPPORTED:
+                return False, str(e)
+            raise
+        sizes = np.zeros(4, np.int64)
+        self._ctx.call("cfdhip_mesh_sizes", _pi(sizes))
+        self.n_cells, self.n_faces, self.n_patches, self.dim = (int(v) for v in sizes)
+        self.shape = (nx, ny, nz if dim == 3 else 1)
+        for p in range(self.n_patches):
+            vk, vv = (velocity[p] if velocity else (WALL, (0.0, 0.0, 0.0)))
+            pk, pv = (pressure[p] if pressure else (FIXED_GRADIENT, 0.0))
+            self.set_boundary(p, vk, vv, pk, pv)
+        self._ctx.call("cfdhip_set_fluid", density, viscosity)
+        return True, ""
+
+    @classmethod
+    def cavity(cls, dim, nx, ny, nz=1, lid=1.0, viscosity=0.01, device=0):
+        """Lid-driven cavity: lid on 'top' (2D) / 'ymax' (3D), other walls no-slip,
+        zero-gradient pressure everywhere (cases/lid_driven_cavity/boundaries.json)."""
+        d = cls(device)
+        npatch = 2 * dim
+        vel = [(MOVING_WALL, (lid, 0.0, 0.0)) if p == 3 else (WALL, (0.0, 0.0, 0.0))
+               for p in range(npatch)]
+        ok, why = d.prepare(dim, nx, ny, nz, velocity=vel, viscosity=viscosity)
+        if not ok:
+            raise CfdHipError(_lib.ERR_UNSUPPORTED, why)
+        return d
+
+    def set_boundary(self, patch, vkind, vvalue, pkind, pvalue):
+        v = _f64(list(vvalue) + [0.0] * (3 - len(vvalue)))
+        self._ctx.call("cfdhip_set_boundary", patch, vkind, _pf(v), pkind, float(pvalue))
+
+    def mesh_faces(self):
+        o = np.zeros(self.n_faces, np.int64)
+        nb = np.zeros(self.n_faces, np.int64)
+        self._ctx.call("cfdhip_mesh_faces", _pi(o), _pi(nb))
+        return o, nb
+
+    # -- fields -------------------------------------------------------------
+    def _count(self, name):
+        return self.n_faces if name in FACE_FIELDS else self.n_cells
+
+    def upload(self, name, values):
+        a = _f64(values)
+        self._ctx.call("cfdhip_field_upload", FIELDS[name], _pf(a), len(a))
+
+    def download(self, name):
+        a = np.empty(self._count(name))
+        self._ctx.call("cfdhip_field_download", FIELDS[name], _pf(a), len(a))
+        return a
+
+    def upload_initial_state(self, velocity, pressure, mass_flux=None, viscosity=None):
+        for k, name in enumerate("UVW"[:self.dim]):
+            self.upload(name, velocity[k])
+        self.upload("P", pressure)
+        if mass_flux is not None:
+            self.upload("FLUX", mass_flux)
+        if viscosity is not None:
+            self.upload("VISCOSITY", viscosity)
+
+    def download_velocity(self):
+        return [self.download(n) for n in "UVW"[:self.dim]]
+
+    def download_pressure(self):
+        return self.download("P")
+
+    def download_mass_flux(self):
+        return self.download("FLUX")
+
+    # -- stages -------------------------------------------------------------
+    def gradient(self, name="P", correction=False):
+        self._ctx.call("cfdhip_gradient", FIELDS[name], 1 if correction else 0)
+        return [self.download(n) for n in ("GRAD_X", "GRAD_Y", "GRAD_Z")[:self.dim]]
+
+    def mass_flux(self):
+        self._ctx.call("cfdhip_mass_flux")
+        return self.download("FLUX")
+
+    def assemble_momentum_resident(self, alpha):
+        self._ctx.call("cfdhip_assemble_momentum", float(alpha))
+
+    def solve_momentum_resident(self, component, settings: LinearSolverSettings) -> SolverResult:
+        r = _lib.SolveResult()
+        s = settings.c()
+        self._ctx.call("cfdhip_solve_momentum", component, ctypes.byref(s), ctypes.byref(r))
+        return _result(r, None, self._ctx.history())
+
+    def compute_response_coefficients(self):
+        self._ctx.call("cfdhip_response")
+
+    def compute_predicted_face_flux(self, rhie_chow, alpha):
+        self._ctx.call("cfdhip_predicted_flux", 1 if rhie_chow else 0, float(alpha))
+
+    def assemble_pressure_correction_resident(self, reference_cell=0):
+        self._ctx.call("cfdhip_assemble_pressure", int(reference_cell))
+
+    def solve_pressure_correction_resident(self, settings: LinearSolverSettings) -> SolverResult:
+        r = _lib.SolveResult()
+        s = settings.c()
+        self._ctx.call("cfdhip_solve_pressure", ctypes.byref(s), ctypes.byref(r))
+        return _result(r, None, self._ctx.history())
+
+    def solve_pressure_rhs(self, settings: LinearSolverSettings, b=None, download=True):
+        """Stand-alone p' solve on the resident operator with a caller RHS (x0 = 0)."""
+        r = _lib.SolveResult()
+        s = settings.c()
+        x = np.empty(self.n_cells) if download else None
+        self._ctx.call("cfdhip_pressure_solve_rhs", ctypes.byref(s),
+                       _pf(None if b is None else _f64(b)), _pf(x), ctypes.byref(r))
+        return _result(r, x, self._ctx.history())
+
+    def update_pressure(self, alpha):
+        self._ctx.call("cfdhip_update_pressure", float(alpha))
+
+    def correct(self):
+        self._ctx.call("cfdhip_correct")
+
+    def continuity(self):
+        out = np.zeros(2)
+        self._ctx.call("cfdhip_continuity", _pf(out))
+        return self.download("IMBALANCE"), float(out[0]), float(out[1])
+
+    def pressure_spmv(self, x):
+        x = _f64(x)
+        y = np.empty(self.n_cells)
+        self._ctx.call("cfdhip_pressure_spmv", _pf(x), _pf(y))
+        return y
+
+    def _csr(self, name):
+        n = self.n_cells
+        cap = (2 * self.dim + 1) * n
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(cap, np.int64)
+        val = np.zeros(cap)
+        nnz = ctypes.c_int64(0)
+        self._ctx.call(name, _pi(rp), _pi(col), _pf(val), cap, ctypes.byref(nnz))
+        return rp, col[:nnz.value], val[:nnz.value]
+
+    def pressure_csr(self):
+        return self._csr("cfdhip_pressure_csr")
+
+    def momentum_csr(self):
+        return self._csr("cfdhip_momentum_csr")
+
+    def set_timing(self, enabled):
+        self._ctx.call("cfdhip_set_timing", 1 if enabled else 0)
+
+    def kernel_timing(self):
+        out = np.zeros(3)
+        self._ctx.call("cfdhip_kernel_timing", _pf(out))
+        return dict(launches=out[0], mean_us=out[1], iterations=out[2])
+
+    def simple_run(self, iterations, alpha_u=0.7, alpha_p=0.3, velocity_tol=1e-6,
+                   pressure_tol=1e-6, continuity_tol=1e-6, rhie_chow=True, reference_cell=0,
+                   momentum: LinearSolverSettings | None = None,
+                   pressure: LinearSolverSettings | None = None, use_graph=False):
+        """SIMPLE::solve on the device from the uploaded state (u, p)."""
+        momentum = momentum or LinearSolverSettings(1e-10, 1e-8, 500, "BiCGSTAB")
+        pressure = pressure or LinearSolverSettings(0.0, 1e-8, 20000, "CG", "Jacobi")
+        s = _f64([iterations, alpha_u, alpha_p, velocity_tol, pressure_tol, continuity_tol,
+                  1.0 if rhie_chow else 0.0, reference_cell, 1.0 if use_graph else 0.0, 0, 0])
+        meta = np.zeros(6)
+        cap = int(iterations)
+        hist = np.zeros(5 * cap)
+        ms, ps = momentum.c(), pressure.c()
+        self._ctx.call("cfdhip_simple_run", _pf(s), ctypes.byref(ms), ctypes.byref(ps),
+                       _pf(meta), _pf(hist), cap)
+        it = int(meta[1])
+        h = hist.reshape(5, cap)[:, :it]
+        return dict(status=int(meta[0]), iterations=it, momentum_iterations=int(meta[2]),
+                    pressure_iterations=int(meta[3]), final_continuity=meta[4],
+                    global_imbalance=meta[5],
+                    hist=dict(u=h[0], v=h[1], w=h[2] if self.dim == 3 else h[2][:0], p=h[3],
+                              c=h[4]))
+
+
+FACE_FIELDS = _lib.FACE_FIELDS

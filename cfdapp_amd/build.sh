@@ -1,0 +1,10 @@
+#!/bin/sh
+# Builds libcfdhip.so (HIP for gfx950) in-tree next to this script.
+set -e
+HERE="$(cd "$(dirname "$0")" && pwd)"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+"$HIPCC" -std=c++20 -O3 --offload-arch=gfx950 -fPIC -shared \
+  -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops \
+  -Wall -Wno-unused-function \
+  "$HERE"/csrc/*.hip -o "$HERE/libcfdhip.so.tmp"
+mv "$HERE/libcfdhip.so.tmp" "$HERE/libcfdhip.so"

@@ -1,0 +1,434 @@
+// extern "C" entry points of libcfdhip (declared in include/cfdhip.h).
+// Each catches every library exception, records the message on the context
+// and returns the error code, so no exception crosses the C boundary.
+#include <cmath>
+#include <cstdio>
+
+#include "context.hpp"
+
+using namespace cfdhip;
+
+struct cfdhip_ctx {
+  Context c;
+};
+
+namespace cfdhip {
+
+Context::~Context() {
+  if (kw.hostState) (void)hipHostFree(kw.hostState);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::ensureFields() {
+  const int64_t n = view.n, nf = view.nf;
+  for (int id = 0; id < CFDHIP_FIELD_COUNT; ++id) {
+    const bool face = id == CFDHIP_FIELD_FLUX || id == CFDHIP_FIELD_FLUX_STAR ||
+                      id == CFDHIP_FIELD_PC_FACE_COEFF;
+    field[id].ensure(face ? nf : n);
+    CFDHIP_CHECK(hipMemsetAsync(field[id].get(), 0, sizeof(double) * (face ? nf : n), stream));
+  }
+  CFDHIP_CHECK(hipStreamSynchronize(stream));
+}
+
+}  // namespace cfdhip
+
+namespace {
+
+const int kGfx950 = 950;
+
+bool isGfx950(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return false;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+template <class F>
+int guarded(cfdhip_ctx* ctx, F&& fn) {
+  if (ctx == nullptr) return CFDHIP_ERR_INVALID;
+  try {
+    CFDHIP_CHECK(hipSetDevice(ctx->c.device));
+    fn(ctx->c);
+    ctx->c.lastError.clear();
+    return CFDHIP_OK;
+  } catch (const Error& e) {
+    ctx->c.lastError = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->c.lastError = e.what();
+    return CFDHIP_ERR_RUNTIME;
+  }
+}
+
+int fieldCount(const Context& c, int id) {
+  const bool face = id == CFDHIP_FIELD_FLUX || id == CFDHIP_FIELD_FLUX_STAR ||
+                    id == CFDHIP_FIELD_PC_FACE_COEFF;
+  return static_cast<int>(face ? c.view.nf : c.view.n);
+}
+
+cfdhip_solver_settings checkedSettings(const cfdhip_solver_settings* s) {
+  require(s != nullptr, "solver settings are required");
+  return *s;
+}
+
+void uploadCsr(Context& c, CsrDevice& M, int64_t n, const int64_t* rp, const int64_t* col,
+               const double* val) {
+  require(n > 0 && rp && col && val, "CSR arrays are required");
+  require(rp[0] == 0, "SparseMatrix: rowOffsets[0] must be 0");
+  for (int64_t r = 0; r < n; ++r) {
+    require(rp[r] <= rp[r + 1], "SparseMatrix: rowOffsets must be non-decreasing");
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      require(col[k] >= 0 && col[k] < n, "SparseMatrix: column index out of range");
+      require(k == rp[r] || col[k] > col[k - 1],
+              "SparseMatrix: column indices within a row must be strictly ascending");
+    }
+  }
+  const int64_t nnz = rp[n];
+  M.n = n;
+  M.rp.ensure(n + 1);
+  M.col.ensure(nnz > 0 ? nnz : 1);
+  M.val.ensure(nnz > 0 ? nnz : 1);
+  M.val.count = nnz;
+  CFDHIP_CHECK(hipMemcpyAsync(M.rp.get(), rp, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                              c.stream));
+  if (nnz > 0) {
+    CFDHIP_CHECK(hipMemcpyAsync(M.col.get(), col, nnz * sizeof(int64_t), hipMemcpyHostToDevice,
+                                c.stream));
+    CFDHIP_CHECK(hipMemcpyAsync(M.val.get(), val, nnz * sizeof(double), hipMemcpyHostToDevice,
+                                c.stream));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfdhip_device_available(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return 0;
+  for (int d = 0; d < count; ++d)
+    if (isGfx950(d)) return 1;
+  (void)kGfx950;
+  return 0;
+}
+
+cfdhip_ctx* cfdhip_create(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
+  if (!isGfx950(device)) return nullptr;
+  auto* ctx = new (std::nothrow) cfdhip_ctx;
+  if (!ctx) return nullptr;
+  try {
+    ctx->c.device = device;
+    CFDHIP_CHECK(hipSetDevice(device));
+    CFDHIP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    ctx->c.flags.ensure(4);
+    CFDHIP_CHECK(hipMemsetAsync(ctx->c.flags.get(), 0, 4 * sizeof(unsigned), ctx->c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  } catch (...) {
+    delete ctx;
+    return nullptr;
+  }
+  return ctx;
+}
+
+void cfdhip_destroy(cfdhip_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  delete ctx;
+}
+
+const char* cfdhip_last_error(const cfdhip_ctx* ctx) {
+  return ctx ? ctx->c.lastError.c_str() : "null context";
+}
+
+int cfdhip_synchronize(cfdhip_ctx* ctx) {
+  return guarded(ctx, [](Context& c) { CFDHIP_CHECK(hipStreamSynchronize(c.stream)); });
+}
+
+int cfdhip_mesh_cartesian(cfdhip_ctx* ctx, int dim, int64_t nx, int64_t ny, int64_t nz, double lx,
+                          double ly, double lz) {
+  return guarded(ctx, [&](Context& c) { buildCartesianPlan(c, dim, nx, ny, nz, lx, ly, lz); });
+}
+
+int cfdhip_mesh_sizes(cfdhip_ctx* ctx, int64_t* out) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(out != nullptr, "out is required");
+    out[0] = c.view.n;
+    out[1] = c.view.nf;
+    out[2] = c.view.npatch;
+    out[3] = c.view.dim;
+  });
+}
+
+int cfdhip_mesh_faces(cfdhip_ctx* ctx, int64_t* owner, int64_t* neighbor) {
+  return guarded(ctx, [&](Context& c) {
+    require(owner && neighbor, "owner/neighbor buffers are required");
+    exportFaces(c, owner, neighbor);
+  });
+}
+
+int cfdhip_set_boundary(cfdhip_ctx* ctx, int patch, int velocity_kind, const double* velocity3,
+                        int pressure_kind, double pressure_value) {
+  return guarded(ctx, [&](Context& c) {
+    setBoundary(c, patch, velocity_kind, velocity3, pressure_kind, pressure_value);
+  });
+}
+
+int cfdhip_set_fluid(cfdhip_ctx* ctx, double density, double viscosity) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(std::isfinite(density) && density > 0.0, "density must be finite and > 0");
+    require(std::isfinite(viscosity) && viscosity > 0.0,
+            "effectiveViscosity must be finite and > 0");
+    c.rho = density;
+    c.mu = viscosity;
+    c.view.rho = density;
+    std::vector<double> mu(c.view.n, viscosity + 0.0);
+    CFDHIP_CHECK(hipMemcpyAsync(c.f(CFDHIP_FIELD_VISCOSITY), mu.data(), mu.size() * sizeof(double),
+                                hipMemcpyHostToDevice, c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+    c.fluidSet = true;
+  });
+}
+
+int cfdhip_field_upload(cfdhip_ctx* ctx, int field, const double* host, int64_t count) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(field >= 0 && field < CFDHIP_FIELD_COUNT, "unknown field");
+    require(host != nullptr && count == fieldCount(c, field), "field size mismatch");
+    CFDHIP_CHECK(hipMemcpyAsync(c.f(field), host, count * sizeof(double), hipMemcpyHostToDevice,
+                                c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+    if (field == CFDHIP_FIELD_VISCOSITY) c.fluidSet = true;
+  });
+}
+
+int cfdhip_field_download(cfdhip_ctx* ctx, int field, double* host, int64_t count) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(field >= 0 && field < CFDHIP_FIELD_COUNT, "unknown field");
+    require(host != nullptr && count == fieldCount(c, field), "field size mismatch");
+    CFDHIP_CHECK(hipMemcpyAsync(host, c.f(field), count * sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_gradient(cfdhip_ctx* ctx, int field, int bc) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(field >= 0 && field < CFDHIP_FIELD_COUNT && fieldCount(c, field) == c.view.n,
+            "gradient needs a cell field");
+    require(bc == 0 || bc == 1, "bc must be 0 (pressure) or 1 (p')");
+    launchGradient(c, c.f(field), bc, c.f(CFDHIP_FIELD_GRAD_X), c.f(CFDHIP_FIELD_GRAD_Y),
+                   c.f(CFDHIP_FIELD_GRAD_Z));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_mass_flux(cfdhip_ctx* ctx) {
+  return guarded(ctx, [&](Context& c) {
+    launchMassFlux(c, c.f(CFDHIP_FIELD_U), c.f(CFDHIP_FIELD_V), c.f(CFDHIP_FIELD_W),
+                   c.f(CFDHIP_FIELD_FLUX));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_assemble_momentum(cfdhip_ctx* ctx, double alpha) {
+  return guarded(ctx, [&](Context& c) { launchMomentum(c, alpha); });
+}
+
+int cfdhip_solve_momentum(cfdhip_ctx* ctx, int component, const cfdhip_solver_settings* s,
+                          cfdhip_solve_result* r) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(component >= 0 && component < c.view.dim, "component out of range");
+    const cfdhip_solver_settings set = checkedSettings(s);
+    double* x = c.f(CFDHIP_FIELD_USTAR + component);
+    CFDHIP_CHECK(hipMemcpyAsync(x, c.f(CFDHIP_FIELD_U + component), c.view.n * sizeof(double),
+                                hipMemcpyDeviceToDevice, c.stream));
+    const cfdhip_solve_result res =
+        solveStencil(c, set, c.f(CFDHIP_FIELD_MOM_RHS_U + component), x);
+    if (r) *r = res;
+  });
+}
+
+int cfdhip_response(cfdhip_ctx* ctx) {
+  return guarded(ctx, [&](Context& c) {
+    launchResponse(c);
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_predicted_flux(cfdhip_ctx* ctx, int rhie_chow, double alpha) {
+  return guarded(ctx, [&](Context& c) {
+    launchPredictedFlux(c, rhie_chow != 0, alpha);
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_assemble_pressure(cfdhip_ctx* ctx, int64_t reference_cell) {
+  return guarded(ctx, [&](Context& c) { launchPressureAssembly(c, reference_cell); });
+}
+
+int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r) {
+  return guarded(ctx, [&](Context& c) {
+    const cfdhip_solver_settings set = checkedSettings(s);
+    const cfdhip_solve_result res =
+        solvePressure(c, set, c.f(CFDHIP_FIELD_PC_RHS), c.f(CFDHIP_FIELD_PPRIME), true);
+    if (r) *r = res;
+  });
+}
+
+int cfdhip_update_pressure(cfdhip_ctx* ctx, double alpha) {
+  return guarded(ctx, [&](Context& c) {
+    launchUpdatePressure(c, alpha);
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_correct(cfdhip_ctx* ctx) {
+  return guarded(ctx, [&](Context& c) {
+    launchCorrect(c);
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_continuity(cfdhip_ctx* ctx, double* out) {
+  return guarded(ctx, [&](Context& c) {
+    double tmp[2];
+    launchContinuity(c, tmp);
+    if (out) {
+      out[0] = tmp[0];
+      out[1] = tmp[1];
+    }
+  });
+}
+
+int cfdhip_simple_run(cfdhip_ctx* ctx, const double* settings,
+                      const cfdhip_solver_settings* momentum,
+                      const cfdhip_solver_settings* pressure, double* meta, double* history,
+                      int64_t cap) {
+  return guarded(ctx, [&](Context& c) {
+    require(settings && momentum && pressure && meta, "settings, solvers and meta are required");
+    simpleRun(c, settings, *momentum, *pressure, meta, history, cap);
+  });
+}
+
+int cfdhip_pressure_csr(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
+                        int64_t capacity, int64_t* nnz) {
+  return guarded(ctx, [&](Context& c) {
+    require(rowptr && col && val && nnz, "CSR buffers are required");
+    *nnz = exportPressureCsr(c, rowptr, col, val, capacity);
+  });
+}
+
+int cfdhip_momentum_csr(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
+                        int64_t capacity, int64_t* nnz) {
+  return guarded(ctx, [&](Context& c) {
+    require(rowptr && col && val && nnz, "CSR buffers are required");
+    *nnz = exportMomentumCsr(c, rowptr, col, val, capacity);
+  });
+}
+
+int cfdhip_csr_spmv(cfdhip_ctx* ctx, int64_t n, const int64_t* rowptr, const int64_t* col,
+                    const double* val, const double* x, double* y) {
+  return guarded(ctx, [&](Context& c) {
+    require(x && y, "x and y are required");
+    CsrDevice M;
+    uploadCsr(c, M, n, rowptr, col, val);
+    DeviceArray<double> dx, dy;
+    dx.ensure(n);
+    dy.ensure(n);
+    CFDHIP_CHECK(hipMemcpyAsync(dx.get(), x, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    csrSpmv(c, M, dx.get(), dy.get());
+    CFDHIP_CHECK(hipMemcpyAsync(y, dy.get(), n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_csr_solve(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, int64_t n,
+                     const int64_t* rowptr, const int64_t* col, const double* val,
+                     const double* b, const double* x0, double* x, cfdhip_solve_result* r) {
+  return guarded(ctx, [&](Context& c) {
+    const cfdhip_solver_settings set = checkedSettings(s);
+    require(b && x, "b and x are required");
+    CsrDevice M;
+    uploadCsr(c, M, n, rowptr, col, val);
+    DeviceArray<double> db, dx;
+    db.ensure(n);
+    dx.ensure(n);
+    CFDHIP_CHECK(hipMemcpyAsync(db.get(), b, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (x0) {
+      CFDHIP_CHECK(
+          hipMemcpyAsync(dx.get(), x0, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    } else {
+      CFDHIP_CHECK(hipMemsetAsync(dx.get(), 0, n * sizeof(double), c.stream));
+    }
+    const cfdhip_solve_result res = solveCsr(c, set, M, db.get(), dx.get());
+    CFDHIP_CHECK(hipMemcpyAsync(x, dx.get(), n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+    if (r) *r = res;
+  });
+}
+
+int cfdhip_pressure_spmv(cfdhip_ctx* ctx, const double* x, double* y) {
+  return guarded(ctx, [&](Context& c) {
+    require(x && y, "x and y are required");
+    const int64_t n = c.view.n;
+    DeviceArray<double> dx, dy;
+    dx.ensure(n);
+    dy.ensure(n);
+    CFDHIP_CHECK(hipMemcpyAsync(dx.get(), x, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    pressureSpmv(c, dx.get(), dy.get());
+    CFDHIP_CHECK(hipMemcpyAsync(y, dy.get(), n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_pressure_solve_rhs(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
+                              double* x, cfdhip_solve_result* r) {
+  return guarded(ctx, [&](Context& c) {
+    const cfdhip_solver_settings set = checkedSettings(s);
+    c.requireMesh();
+    const int64_t n = c.view.n;
+    if (b) {
+      CFDHIP_CHECK(hipMemcpyAsync(c.f(CFDHIP_FIELD_PC_RHS), b, n * sizeof(double),
+                                  hipMemcpyHostToDevice, c.stream));
+    }
+    const cfdhip_solve_result res =
+        solvePressure(c, set, c.f(CFDHIP_FIELD_PC_RHS), c.f(CFDHIP_FIELD_PPRIME), true);
+    if (x) {
+      CFDHIP_CHECK(hipMemcpyAsync(x, c.f(CFDHIP_FIELD_PPRIME), n * sizeof(double),
+                                  hipMemcpyDeviceToHost, c.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    if (r) *r = res;
+  });
+}
+
+int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t* length) {
+  return guarded(ctx, [&](Context& c) {
+    const int64_t len = static_cast<int64_t>(c.lastHistory.size());
+    if (length) *length = len;
+    if (history)
+      for (int64_t i = 0; i < len && i < cap; ++i) history[i] = c.lastHistory[i];
+  });
+}
+
+int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled) {
+  return guarded(ctx, [&](Context& c) { c.timing = enabled != 0; });
+}
+
+int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out) {
+  return guarded(ctx, [&](Context& c) {
+    require(out != nullptr, "out is required");
+    out[0] = c.spmvCount;
+    out[1] = c.spmvMeanUs;
+    out[2] = c.timedIters;
+  });
+}
+
+}  // extern "C"

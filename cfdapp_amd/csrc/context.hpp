@@ -1,0 +1,111 @@
+// The context: device, stream and the resident state of one case.
+#pragma once
+
+#include <memory>
+
+#include "common.hpp"
+
+namespace cfdhip {
+
+// Device-resident state of one Krylov solve. Written by the finalising block
+// of each reduction kernel; the host only reads it every check_interval
+// iterations (and at the end) through a pinned mirror.
+struct KState {
+  double b0, absTol, relTol;
+  double rz, pAp, alpha, beta, rnorm;
+  // BiCGSTAB recurrence (src/algebra/BiCGSTAB.cpp:48-275)
+  double rho, rhoOld, omega, rHatNorm, restartResidual, vnorm, snorm;
+  double tDotT;
+  int64_t iter, maxIter, completed, restarts, histLen;
+  int32_t status;  // CFDHIP_CONVERGED ... ; -1 while running
+  int32_t done;    // 1 once status is final
+  int32_t restart; // 1: the host must run the true-residual restart
+  int32_t sConverged;
+};
+
+// Work vectors (allocated once per size, reused across solves).
+struct KrylovWork {
+  DeviceArray<double> r, p, Ap, z, x, b, dinv, rhat, v, s, t, phat, shat;
+  DeviceArray<float> r32, p32, Ap32, z32, x32, dinv32, w32;
+  DeviceArray<double> partial;
+  DeviceArray<unsigned> counter;
+  DeviceArray<KState> state;
+  DeviceArray<double> hist;
+  KState* hostState = nullptr;  // pinned mirror
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string lastError;
+
+  bool hasMesh = false;
+  View view{};
+  DeviceArray<double> tables[3];
+  double rho = 1.0, mu = 0.0;
+  bool fluidSet = false;
+  bool bcSet[6] = {false, false, false, false, false, false};
+  int64_t pin = -1;         // p' reference cell of the last assembly (-1: none)
+  bool pressureAssembled = false, momentumAssembled = false;
+
+  DeviceArray<double> field[CFDHIP_FIELD_COUNT];
+  DeviceArray<double> momCoef;  // (2*dim+1) * n, CSR column order
+  DeviceArray<unsigned> flags;  // [0] non-finite counter
+
+  KrylovWork kw;
+  std::vector<double> lastHistory;
+
+  // timing of the pressure SpMV kernel
+  bool timing = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double spmvCount = 0, spmvMeanUs = 0, timedIters = 0;
+
+  ~Context();
+  int64_t cellCount() const { return view.n; }
+  int64_t faceCount() const { return view.nf; }
+  double* f(int id) const { return field[id].get(); }
+  void requireMesh() const { require(hasMesh, "no mesh: call cfdhip_mesh_cartesian first"); }
+  void ensureFields();
+};
+
+// plan.hip
+void buildCartesianPlan(Context& ctx, int dim, int64_t nx, int64_t ny, int64_t nz, double lx,
+                        double ly, double lz);
+void setBoundary(Context& ctx, int patch, int vkind, const double* v3, int pkind, double pvalue);
+void exportFaces(Context& ctx, int64_t* owner, int64_t* neighbor);
+
+// assembly.hip
+void launchGradient(Context& ctx, const double* phi, int bcKind, double* gx, double* gy, double* gz);
+void launchMassFlux(Context& ctx, const double* u, const double* v, const double* w, double* F);
+void launchMomentum(Context& ctx, double alpha);
+void launchResponse(Context& ctx);
+void launchPredictedFlux(Context& ctx, bool rhieChow, double alpha);
+void launchPressureAssembly(Context& ctx, int64_t ref);
+void launchUpdatePressure(Context& ctx, double alpha);
+void launchCorrect(Context& ctx);
+void launchContinuity(Context& ctx, double* out);
+int64_t exportPressureCsr(Context& ctx, int64_t* rp, int64_t* col, double* val, int64_t cap);
+int64_t exportMomentumCsr(Context& ctx, int64_t* rp, int64_t* col, double* val, int64_t cap);
+unsigned countNonFinite(Context& ctx, const double* const* arrays, const int64_t* counts, int k);
+
+// krylov.hip
+enum class OpKind { Pressure, Stencil, Csr };
+struct CsrDevice {
+  DeviceArray<int64_t> rp, col;
+  DeviceArray<double> val;
+  int64_t n = 0;
+};
+cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s, const double* b,
+                                  double* x, bool zeroGuess);
+cfdhip_solve_result solveStencil(Context& ctx, const cfdhip_solver_settings& s, const double* b,
+                                 double* x);
+cfdhip_solve_result solveCsr(Context& ctx, const cfdhip_solver_settings& s, const CsrDevice& A,
+                             const double* b, double* x);
+void pressureSpmv(Context& ctx, const double* x, double* y);
+void csrSpmv(Context& ctx, const CsrDevice& A, const double* x, double* y);
+
+// simple.hip
+void simpleRun(Context& ctx, const double* settings, const cfdhip_solver_settings& mom,
+               const cfdhip_solver_settings& pres, double* meta, double* history, int64_t cap);
+
+}  // namespace cfdhip

@@ -1,0 +1,984 @@
+// Krylov solvers on the device: CG (src/algebra/CG.cpp:14-148) and BiCGSTAB
+// (src/algebra/BiCGSTAB.cpp:48-275) with Jacobi (Preconditioner.cpp:11-45),
+// over three operators: the resident p' operator (diagonal + face couplings),
+// a 5-/7-point stencil (the momentum matrix) and a general CSR matrix.
+//
+// Every inner product is a two-stage deterministic reduction: wave shuffles,
+// block partials, and the block that arrives last sums all partials in a
+// fixed order and evaluates the solver's scalar recurrence and its
+// convergence / breakdown tests on the device. The host reads the verdict
+// only every check_interval iterations, so an iteration costs no host round
+// trip (the reference pays 5 host syncs per BiCGSTAB iteration,
+// cuda/kernels/DeviceVectorOpsKernel.cu:226-260).
+#include <cfloat>
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "context.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+constexpr int kMaxBlocks = 2048;
+constexpr double kEps = DBL_EPSILON;
+
+inline dim3 gridStride(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b > kMaxBlocks) b = kMaxBlocks;
+  return dim3(static_cast<unsigned>(b < 1 ? 1 : b));
+}
+
+// ---------------------------------------------------------------------------
+// Operators. apply(c, x) = row c of A times x, summed in ascending column
+// order from 0.0 (SparseMatrix::multiply); a stored zero contributes a signed
+// zero, which leaves a sum that started at +0.0 unchanged, so the stencil
+// forms equal the zero-dropped CSR bit for bit on finite x.
+
+struct PressureOp {
+  View v;
+  const double* diag;
+  const double* coeff;
+  int64_t pin;
+  __device__ double apply(int64_t c, const double* __restrict__ x) const {
+    if (c == pin) return 0.0 + diag[c] * x[c];
+    int64_t idx[3];
+    cellIndex(v, c, idx);
+    const int D = v.dim;
+    double y = 0.0;
+    // low neighbours, highest axis first (ascending column)
+    for (int a = D - 1; a >= 0; --a) {
+      if (idx[a] == 0) continue;
+      int64_t f[3] = {idx[0], idx[1], idx[2]};
+      y += (-coeff[faceId(v, a, f)]) * x[c - v.stride[a]];
+    }
+    y += diag[c] * x[c];
+    for (int a = 0; a < D; ++a) {
+      if (idx[a] == v.ncell[a] - 1) continue;
+      int64_t f[3] = {idx[0], idx[1], idx[2]};
+      f[a] += 1;
+      y += (-coeff[faceId(v, a, f)]) * x[c + v.stride[a]];
+    }
+    return y;
+  }
+  __device__ double diagonal(int64_t c) const { return diag[c]; }
+};
+
+struct StencilOp {
+  View v;
+  const double* coef;  // (2D+1) * n in column order
+  __device__ double apply(int64_t c, const double* __restrict__ x) const {
+    int64_t idx[3];
+    cellIndex(v, c, idx);
+    const int D = v.dim;
+    double y = 0.0;
+    for (int a = D - 1; a >= 0; --a) {
+      if (idx[a] == 0) continue;
+      y += coef[(D - 1 - a) * v.n + c] * x[c - v.stride[a]];
+    }
+    y += coef[D * v.n + c] * x[c];
+    for (int a = 0; a < D; ++a) {
+      if (idx[a] == v.ncell[a] - 1) continue;
+      y += coef[(D + 1 + a) * v.n + c] * x[c + v.stride[a]];
+    }
+    return y;
+  }
+  __device__ double diagonal(int64_t c) const { return coef[v.dim * v.n + c]; }
+};
+
+struct CsrOp {
+  int64_t n;
+  const int64_t* rp;
+  const int64_t* col;
+  const double* val;
+  __device__ double apply(int64_t r, const double* __restrict__ x) const {
+    double sum = 0.0;
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) sum += val[k] * x[col[k]];
+    return sum;
+  }
+  // SparseMatrix::diagonal: the stored diagonal, 0 when absent (rejected).
+  __device__ double diagonal(int64_t r) const {
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+      if (col[k] == r) return val[k];
+    return 0.0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic reduction of W sums; returns true in the last-arriving block,
+// where tot[] holds the grid totals in every thread.
+template <int W>
+__device__ bool reduceLast(double (&val)[W], double* partial, unsigned* counter, double (&tot)[W]) {
+  __shared__ double lds[W][kBlock / 64];
+  __shared__ int amLast;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int w = 0; w < W; ++w) {
+    double x = val[w];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) lds[w][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < W; ++w) {
+      double t = 0.0;
+      for (int i = 0; i < kBlock / 64; ++i) t += lds[w][i];
+      partial[blockIdx.x * W + w] = t;
+    }
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    amLast = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!amLast) return false;
+  __threadfence();
+  for (int w = 0; w < W; ++w) {
+    double x = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) x += partial[b * W + w];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    __syncthreads();
+    if (lane == 0) lds[w][wid] = x;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) t += lds[w][i];
+    tot[w] = t;
+  }
+  if (threadIdx.x == 0) *counter = 0;
+  return true;
+}
+
+__device__ inline bool convergedTest(const KState& s, double a) {
+  return a <= s.absTol || ((s.b0 > 0.0) && (a / s.b0 <= s.relTol));
+}
+__device__ inline void finish(KState& s, int status) {
+  s.status = status;
+  s.done = 1;
+}
+__device__ inline void pushHist(KState& s, double* hist, double value) {
+  hist[s.histLen] = value;
+  s.histLen += 1;
+}
+// cancelledToRoundingLevel (BiCGSTAB.cpp:34-41) with the |x_i y_i| sum at hand.
+__device__ inline bool cancelled(double ip, double absSum, double nx, double ny) {
+  if (fabs(ip) > kEps * nx * ny) return false;
+  return fabs(ip) <= kEps * absSum;
+}
+
+struct Red {
+  double* partial;
+  unsigned* counter;
+  KState* st;
+  double* hist;
+};
+
+// ---------------------------------------------------------------------------
+// Shared kernels.
+
+template <class Op>
+__global__ void residualKernel(Op A, int64_t n, const double* __restrict__ b,
+                               const double* __restrict__ x, double* r, double* copy, Red R,
+                               int mode) {
+  // mode 0: CG / BiCGSTAB start, 1: BiCGSTAB restart
+  double rr[1] = {0.0};
+  KState& s = *R.st;
+  const bool skip = s.done;
+  if (!skip)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const double ri = b[i] - A.apply(i, x);
+      r[i] = ri;
+      if (copy) copy[i] = ri;
+      rr[0] += ri * ri;
+    }
+  double tot[1];
+  if (!reduceLast<1>(rr, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double norm = sqrt(tot[0]);
+  if (mode == 0) {
+    s.b0 = norm;
+    s.rnorm = norm;
+    s.rHatNorm = norm;
+    s.restartResidual = norm;
+    pushHist(s, R.hist, norm);
+    if (norm == 0.0 || convergedTest(s, norm)) {
+      s.rnorm = norm;
+      finish(s, CFDHIP_CONVERGED);
+    }
+    return;
+  }
+  // BiCGSTAB restart (BiCGSTAB.cpp:128-144), after the rNorm < restartResidual
+  // test the host already passed.
+  s.rnorm = norm;
+  s.restart = 0;
+  if (!isfinite(norm)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  if (convergedTest(s, norm)) {
+    finish(s, CFDHIP_CONVERGED);
+    return;
+  }
+  s.rHatNorm = norm;
+  s.restartResidual = norm;
+  s.rhoOld = 1.0;
+  s.alpha = 1.0;
+  s.omega = 1.0;
+  s.restarts += 1;
+  s.restart = 2;  // the host zeroes v and p
+}
+
+// Jacobi inverse diagonal with the reference's rejection rules; counts rejects.
+template <class Op>
+__global__ void jacobiKernel(Op A, int64_t n, double* dinv, unsigned* bad) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double d = A.diagonal(i);
+    if (!isfinite(d) || d == 0.0 || fabs(d) < 1e-12) {
+      atomicAdd(bad, 1u);
+      dinv[i] = 0.0;
+    } else {
+      dinv[i] = 1.0 / d;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CG (CG.cpp:59-147).
+
+// z = M r; p = z; rz = r . z
+__global__ void cgStartKernel(int64_t n, const double* __restrict__ r,
+                              const double* __restrict__ dinv, double* p, Red R) {
+  double acc[1] = {0.0};
+  KState& s = *R.st;
+  const bool skip = s.done;
+  if (!skip)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const double z = dinv ? dinv[i] * r[i] : r[i];
+      p[i] = z;
+      acc[0] += r[i] * z;
+    }
+  double tot[1];
+  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  s.rz = tot[0];
+  if (!isfinite(s.rz)) {
+    s.rnorm = s.b0;
+    finish(s, CFDHIP_BREAKDOWN);
+  }
+}
+
+// Ap = A p; pAp -> alpha
+template <class Op>
+__global__ void cgSpmvKernel(Op A, int64_t n, const double* __restrict__ p, double* Ap, Red R) {
+  double acc[1] = {0.0};
+  KState& s = *R.st;
+  const bool skip = s.done;
+  if (!skip)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const double y = A.apply(i, p);
+      Ap[i] = y;
+      acc[0] += p[i] * y;
+    }
+  double tot[1];
+  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double pAp = tot[0];
+  s.pAp = pAp;
+  if (!isfinite(pAp) || fabs(pAp) < 1e-30) {
+    finish(s, CFDHIP_BREAKDOWN);  // iterations = it - 1, final = |r|
+    return;
+  }
+  const double alpha = s.rz / pAp;
+  if (!isfinite(alpha)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  s.alpha = alpha;
+}
+
+// x += p alpha; r = r - Ap alpha; |r|, r . z -> convergence, beta
+__global__ void cgUpdateKernel(int64_t n, double* x, double* r, const double* __restrict__ p,
+                               const double* __restrict__ Ap, const double* __restrict__ dinv,
+                               Red R) {
+  double acc[2] = {0.0, 0.0};
+  KState& s = *R.st;
+  const bool skip = s.done;
+  if (!skip) {
+    const double alpha = s.alpha;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      x[i] += p[i] * alpha;
+      const double rn = r[i] - (Ap[i] * alpha);
+      r[i] = rn;
+      const double z = dinv ? dinv[i] * rn : rn;
+      acc[0] += rn * rn;
+      acc[1] += rn * z;
+    }
+  }
+  double tot[2];
+  if (!reduceLast<2>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double rnorm = sqrt(tot[0]);
+  s.iter += 1;
+  pushHist(s, R.hist, rnorm);
+  s.rnorm = rnorm;
+  if (!isfinite(rnorm)) {
+    finish(s, CFDHIP_NONFINITE_RESIDUAL);
+    return;
+  }
+  if (convergedTest(s, rnorm)) {
+    finish(s, CFDHIP_CONVERGED);
+    return;
+  }
+  const double rzNew = tot[1];
+  if (!isfinite(rzNew)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  const double beta = rzNew / s.rz;
+  if (!isfinite(beta)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  s.beta = beta;
+  s.rz = rzNew;
+  if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+}
+
+// p = z + p beta
+__global__ void cgDirectionKernel(int64_t n, const double* __restrict__ r,
+                                  const double* __restrict__ dinv, double* p, const KState* st) {
+  if (st->done) return;
+  const double beta = st->beta;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double z = dinv ? dinv[i] * r[i] : r[i];
+    p[i] = z + (p[i] * beta);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB (BiCGSTAB.cpp:111-274).
+
+// rho = rHat . r -> beta
+__global__ void bcgRhoKernel(int64_t n, const double* __restrict__ rhat,
+                             const double* __restrict__ r, Red R) {
+  double acc[2] = {0.0, 0.0};
+  KState& s = *R.st;
+  const bool skip = s.done || s.restart;
+  if (!skip)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const double t = rhat[i] * r[i];
+      acc[0] += t;
+      acc[1] += fabs(t);
+    }
+  double tot[2];
+  if (!reduceLast<2>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double rho = tot[0];
+  if (!isfinite(rho)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  if (cancelled(rho, tot[1], s.rHatNorm, s.rnorm)) {
+    s.restart = 1;
+    return;
+  }
+  const double beta = (rho / s.rhoOld) * (s.alpha / s.omega);
+  if (!isfinite(beta)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  s.rho = rho;
+  s.beta = beta;
+}
+
+// p = r + (p - v omega) beta; pHat = M p
+__global__ void bcgDirectionKernel(int64_t n, const double* __restrict__ r,
+                                   const double* __restrict__ v, const double* __restrict__ dinv,
+                                   double* p, double* phat, const KState* st) {
+  if (st->done || st->restart) return;
+  const double beta = st->beta, omega = st->omega;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double pn = r[i] + ((p[i] - (v[i] * omega)) * beta);
+    p[i] = pn;
+    phat[i] = dinv ? dinv[i] * pn : pn;
+  }
+}
+
+// v = A pHat; rHat . v -> alpha
+template <class Op>
+__global__ void bcgVKernel(Op A, int64_t n, const double* __restrict__ phat,
+                           const double* __restrict__ rhat, double* v, Red R) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  KState& s = *R.st;
+  const bool skip = s.done || s.restart;
+  if (!skip)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const double y = A.apply(i, phat);
+      v[i] = y;
+      const double t = rhat[i] * y;
+      acc[0] += t;
+      acc[1] += fabs(t);
+      acc[2] += y * y;
+    }
+  double tot[3];
+  if (!reduceLast<3>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double rHatDotV = tot[0];
+  if (!isfinite(rHatDotV)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  if (cancelled(rHatDotV, tot[1], s.rHatNorm, sqrt(tot[2]))) {
+    s.restart = 1;
+    return;
+  }
+  const double alpha = s.rho / rHatDotV;
+  if (!isfinite(alpha)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  s.alpha = alpha;
+}
+
+// s = r - v alpha; |s| -> early convergence; sHat = M s
+__global__ void bcgSKernel(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
+                           const double* __restrict__ dinv, double* sv, double* shat, Red R) {
+  double acc[1] = {0.0};
+  KState& s = *R.st;
+  const bool skip = s.done || s.restart;
+  if (!skip) {
+    const double alpha = s.alpha;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const double si = r[i] - (v[i] * alpha);
+      sv[i] = si;
+      shat[i] = dinv ? dinv[i] * si : si;
+      acc[0] += si * si;
+    }
+  }
+  double tot[1];
+  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double sNorm = sqrt(tot[0]);
+  s.snorm = sNorm;
+  const int64_t iter = s.completed + 1;
+  if (!isfinite(sNorm)) {
+    s.iter = iter;
+    pushHist(s, R.hist, sNorm);
+    s.rnorm = sNorm;
+    finish(s, CFDHIP_NONFINITE_RESIDUAL);
+    return;
+  }
+  if (convergedTest(s, sNorm)) {
+    s.iter = iter;
+    pushHist(s, R.hist, sNorm);
+    s.rnorm = sNorm;
+    s.sConverged = 1;  // x += pHat alpha happens in the next kernel
+  }
+}
+
+// t = A sHat; (t,t), (t,s) -> omega   (or the early-exit x update)
+template <class Op>
+__global__ void bcgTKernel(Op A, int64_t n, const double* __restrict__ shat,
+                           const double* __restrict__ sv, const double* __restrict__ phat,
+                           double* t, double* x, Red R) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  KState& s = *R.st;
+  const bool skip = s.done || s.restart;
+  const bool early = s.sConverged;
+  if (!skip) {
+    const double alpha = s.alpha;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      if (early) {
+        x[i] += phat[i] * alpha;
+        continue;
+      }
+      const double y = A.apply(i, shat);
+      t[i] = y;
+      const double ts = y * sv[i];
+      acc[0] += y * y;
+      acc[1] += ts;
+      acc[2] += fabs(ts);
+    }
+  }
+  double tot[3];
+  if (!reduceLast<3>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  if (early) {
+    finish(s, CFDHIP_CONVERGED);
+    return;
+  }
+  const double tDotT = tot[0];
+  if (!isfinite(tDotT) || tDotT < DBL_MIN) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  const double tDotS = tot[1];
+  if (!isfinite(tDotS)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  if (cancelled(tDotS, tot[2], sqrt(tDotT), s.snorm)) {
+    s.restart = 1;
+    return;
+  }
+  const double omega = tDotS / tDotT;
+  if (!isfinite(omega)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  s.omega = omega;
+}
+
+// x += pHat alpha; x += sHat omega; r = s - t omega; |r|
+__global__ void bcgXKernel(int64_t n, double* x, double* r, const double* __restrict__ phat,
+                           const double* __restrict__ shat, const double* __restrict__ sv,
+                           const double* __restrict__ t, Red R) {
+  double acc[1] = {0.0};
+  KState& s = *R.st;
+  const bool skip = s.done || s.restart;
+  if (!skip) {
+    const double alpha = s.alpha, omega = s.omega;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      x[i] += phat[i] * alpha;
+      x[i] += shat[i] * omega;
+      const double ri = sv[i] - (t[i] * omega);
+      r[i] = ri;
+      acc[0] += ri * ri;
+    }
+  }
+  double tot[1];
+  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
+  const double rn = sqrt(tot[0]);
+  const int64_t iter = s.completed + 1;
+  s.rnorm = rn;
+  s.iter = iter;
+  pushHist(s, R.hist, rn);
+  if (!isfinite(rn)) {
+    finish(s, CFDHIP_NONFINITE_RESIDUAL);
+    return;
+  }
+  s.completed = iter;
+  if (convergedTest(s, rn)) {
+    finish(s, CFDHIP_CONVERGED);
+    return;
+  }
+  s.rhoOld = s.rho;
+  if (s.completed >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+}
+
+__global__ void zeroKernel(int64_t n, double* a, double* b) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    a[i] = 0.0;
+    if (b) b[i] = 0.0;
+  }
+}
+
+__global__ void copyKernel(int64_t n, const double* __restrict__ src, double* dst) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void nonFiniteKernel(int64_t n, const double* __restrict__ a, unsigned* out) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    if (!isfinite(a[i])) atomicAdd(out, 1u);
+}
+
+template <class Op>
+__global__ void spmvKernel(Op A, int64_t n, const double* __restrict__ x, double* y) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    y[i] = A.apply(i, x);
+}
+
+// True residual |b - A x| (FP64), written to state.tDotT slot via a reduction.
+template <class Op>
+__global__ void trueResidualKernel(Op A, int64_t n, const double* __restrict__ b,
+                                   const double* __restrict__ x, double* partial,
+                                   unsigned* counter, double* out) {
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double ri = b[i] - A.apply(i, x);
+    acc[0] += ri * ri;
+  }
+  double tot[1];
+  if (!reduceLast<1>(acc, partial, counter, tot) || threadIdx.x != 0) return;
+  *out = sqrt(tot[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers.
+
+struct Solve {
+  Context& ctx;
+  int64_t n;
+  Red R;
+  KState* hs;
+  explicit Solve(Context& c, int64_t nn) : ctx(c), n(nn) {
+    KrylovWork& w = ctx.kw;
+    w.partial.ensure(kMaxBlocks * 4);
+    if (!w.counter.get()) {
+      w.counter.ensure(2);
+      CFDHIP_CHECK(hipMemsetAsync(w.counter.get(), 0, 2 * sizeof(unsigned), ctx.stream));
+    }
+    w.state.ensure(1);
+    if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+    R = Red{w.partial.get(), w.counter.get(), w.state.get(), nullptr};
+    hs = w.hostState;
+  }
+  void begin(const cfdhip_solver_settings& s) {
+    require(s.absolute_tolerance >= 0.0 && s.relative_tolerance >= 0.0,
+            "LinearSolverSettings: tolerances must be >= 0");
+    require(!(s.absolute_tolerance == 0.0 && s.relative_tolerance == 0.0),
+            "LinearSolverSettings: at least one of absoluteTolerance/relativeTolerance must be > 0");
+    require(s.max_iterations >= 1, "LinearSolverSettings: maxIterations must be >= 1");
+    KState k{};
+    k.absTol = s.absolute_tolerance;
+    k.relTol = s.relative_tolerance;
+    k.maxIter = s.max_iterations;
+    k.status = -1;
+    k.rhoOld = 1.0;
+    k.alpha = 1.0;
+    k.omega = 1.0;
+    *hs = k;
+    CFDHIP_CHECK(hipMemcpyAsync(R.st, hs, sizeof(KState), hipMemcpyHostToDevice, ctx.stream));
+    ctx.kw.hist.ensure(s.max_iterations + 2);
+    R.hist = ctx.kw.hist.get();
+  }
+  void pull() {
+    CFDHIP_CHECK(hipMemcpyAsync(hs, R.st, sizeof(KState), hipMemcpyDeviceToHost, ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  }
+  void push() {
+    CFDHIP_CHECK(hipMemcpyAsync(R.st, hs, sizeof(KState), hipMemcpyHostToDevice, ctx.stream));
+  }
+  dim3 grid() const { return gridStride(n); }
+  cfdhip_solve_result result(double trueRes, double ms) {
+    cfdhip_solve_result r{};
+    r.status = hs->status < 0 ? CFDHIP_MAX_ITERATIONS : hs->status;
+    r.iterations = hs->iter;
+    r.restarts = hs->restarts;
+    r.initial_residual = hs->b0;
+    r.final_residual = hs->rnorm;
+    r.true_residual = trueRes;
+    r.device_ms = ms;
+    r.history_length = hs->histLen;
+    ctx.lastHistory.resize(hs->histLen);
+    if (hs->histLen > 0)
+      CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), R.hist, hs->histLen * sizeof(double),
+                             hipMemcpyDeviceToHost));
+    return r;
+  }
+};
+
+template <class Op>
+bool inputsFinite(Context& ctx, Op&, const std::vector<std::pair<const double*, int64_t>>& arrays) {
+  for (const auto& a : arrays) {
+    if (a.second <= 0 || a.first == nullptr) continue;
+    hipLaunchKernelGGL(nonFiniteKernel, gridStride(a.second), dim3(kBlock), 0, ctx.stream,
+                       a.second, a.first, ctx.flags.get() + 1);
+  }
+  unsigned h = 0;
+  CFDHIP_CHECK(hipMemcpyAsync(&h, ctx.flags.get() + 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  CFDHIP_CHECK(hipMemsetAsync(ctx.flags.get() + 1, 0, sizeof(unsigned), ctx.stream));
+  return h == 0;
+}
+
+template <class Op>
+bool buildJacobi(Context& ctx, Op A, int64_t n, double* dinv) {
+  hipLaunchKernelGGL((jacobiKernel<Op>), gridStride(n), dim3(kBlock), 0, ctx.stream, A, n, dinv,
+                     ctx.flags.get() + 1);
+  CFDHIP_CHECK(hipGetLastError());
+  unsigned h = 0;
+  CFDHIP_CHECK(hipMemcpyAsync(&h, ctx.flags.get() + 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  CFDHIP_CHECK(hipMemsetAsync(ctx.flags.get() + 1, 0, sizeof(unsigned), ctx.stream));
+  return h == 0;
+}
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    CFDHIP_CHECK(hipEventCreate(&a));
+    CFDHIP_CHECK(hipEventCreate(&b));
+    CFDHIP_CHECK(hipEventRecord(a, s));
+  }
+  double stop() {
+    CFDHIP_CHECK(hipEventRecord(b, s));
+    CFDHIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    CFDHIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+  ~Timer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+template <class Op>
+double trueResidual(Context& ctx, Op A, int64_t n, const double* b, const double* x) {
+  KrylovWork& w = ctx.kw;
+  double* out = w.partial.get() + kMaxBlocks * 4 - 1;
+  hipLaunchKernelGGL((trueResidualKernel<Op>), gridStride(n), dim3(kBlock), 0, ctx.stream, A, n, b,
+                     x, w.partial.get(), w.counter.get() + 1, out);
+  CFDHIP_CHECK(hipGetLastError());
+  double h = 0.0;
+  CFDHIP_CHECK(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  return h;
+}
+
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  ~EventPool() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+EventPool& eventPool(Context& ctx, int chunk) {
+  static thread_local std::vector<std::pair<Context*, std::unique_ptr<EventPool>>> pools;
+  for (auto& p : pools)
+    if (p.first == &ctx && static_cast<int>(p.second->ev.size()) >= 2 * chunk) return *p.second;
+  auto pool = std::make_unique<EventPool>();
+  pool->ev.resize(2 * chunk);
+  for (hipEvent_t& e : pool->ev) CFDHIP_CHECK(hipEventCreate(&e));
+  pools.emplace_back(&ctx, std::move(pool));
+  return *pools.back().second;
+}
+
+// CG with optional Jacobi; b, x device pointers (x holds x0 on entry).
+template <class Op>
+cfdhip_solve_result runCG(Context& ctx, Op A, int64_t n, const cfdhip_solver_settings& set,
+                          const double* b, double* x, const std::vector<std::pair<const double*, int64_t>>& opArrays) {
+  KrylovWork& w = ctx.kw;
+  w.r.ensure(n);
+  w.p.ensure(n);
+  w.Ap.ensure(n);
+  w.dinv.ensure(n);
+  Solve S(ctx, n);
+  S.begin(set);
+  auto arrays = opArrays;
+  arrays.push_back({b, n});
+  arrays.push_back({x, n});
+  if (!inputsFinite(ctx, A, arrays)) {
+    S.hs->status = CFDHIP_NONFINITE_INPUT;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  const bool jacobi = set.preconditioner == CFDHIP_PREC_JACOBI;
+  if (jacobi && !buildJacobi(ctx, A, n, w.dinv.get())) {
+    S.hs->status = CFDHIP_INVALID_SYSTEM;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  const double* dinv = jacobi ? w.dinv.get() : nullptr;
+  Timer timer(ctx.stream);
+  const dim3 g = S.grid();
+  hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x, w.r.get(),
+                     nullptr, S.R, 0);
+  hipLaunchKernelGGL(cgStartKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), dinv, w.p.get(),
+                     S.R);
+  CFDHIP_CHECK(hipGetLastError());
+  const int chunk = set.check_interval > 0 ? set.check_interval : 32;
+  int64_t launched = 0;
+  // Kernel timing: one event pair per SpMV launch of a chunk, read after the
+  // chunk's verdict sync, so timing never stalls the launch pipeline.
+  EventPool* pool = ctx.timing ? &eventPool(ctx, chunk) : nullptr;
+  double spmvMs = 0.0, spmvLaunches = 0.0;
+  while (true) {
+    int k = 0;
+    for (; k < chunk && launched < set.max_iterations; ++k, ++launched) {
+      if (pool) CFDHIP_CHECK(hipEventRecord(pool->ev[2 * k], ctx.stream));
+      hipLaunchKernelGGL((cgSpmvKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.p.get(),
+                         w.Ap.get(), S.R);
+      if (pool) CFDHIP_CHECK(hipEventRecord(pool->ev[2 * k + 1], ctx.stream));
+      hipLaunchKernelGGL(cgUpdateKernel, g, dim3(kBlock), 0, ctx.stream, n, x, w.r.get(),
+                         w.p.get(), w.Ap.get(), dinv, S.R);
+      hipLaunchKernelGGL(cgDirectionKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), dinv,
+                         w.p.get(), S.R.st);
+    }
+    CFDHIP_CHECK(hipGetLastError());
+    const int64_t doneBefore = S.hs->iter;
+    S.pull();
+    if (pool) {
+      // count only launches that did work (iterations completed in this chunk,
+      // plus the one that ended it)
+      const int64_t worked = std::min<int64_t>(k, S.hs->iter - doneBefore + (S.hs->done ? 1 : 0));
+      for (int q = 0; q < worked; ++q) {
+        float ms = 0.f;
+        CFDHIP_CHECK(hipEventElapsedTime(&ms, pool->ev[2 * q], pool->ev[2 * q + 1]));
+        spmvMs += ms;
+        spmvLaunches += 1.0;
+      }
+    }
+    if (S.hs->done || launched >= set.max_iterations) break;
+  }
+  const double ms = timer.stop();
+  if (pool) {
+    ctx.spmvCount = spmvLaunches;
+    ctx.spmvMeanUs = spmvLaunches > 0 ? 1000.0 * spmvMs / spmvLaunches : 0.0;
+    ctx.timedIters = static_cast<double>(S.hs->iter);
+  }
+  if (!S.hs->done) {
+    S.hs->status = CFDHIP_MAX_ITERATIONS;
+    S.hs->done = 1;
+  }
+  const double tr = trueResidual(ctx, A, n, b, x);
+  return S.result(tr, ms);
+}
+
+template <class Op>
+cfdhip_solve_result runBiCGSTAB(Context& ctx, Op A, int64_t n, const cfdhip_solver_settings& set,
+                                const double* b, double* x,
+                                const std::vector<std::pair<const double*, int64_t>>& opArrays) {
+  KrylovWork& w = ctx.kw;
+  for (DeviceArray<double>* a : {&w.r, &w.p, &w.rhat, &w.v, &w.s, &w.t, &w.phat, &w.shat, &w.dinv})
+    a->ensure(n);
+  Solve S(ctx, n);
+  S.begin(set);
+  auto arrays = opArrays;
+  arrays.push_back({b, n});
+  arrays.push_back({x, n});
+  if (!inputsFinite(ctx, A, arrays)) {
+    S.hs->status = CFDHIP_NONFINITE_INPUT;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  const bool jacobi = set.preconditioner == CFDHIP_PREC_JACOBI;
+  if (jacobi && !buildJacobi(ctx, A, n, w.dinv.get())) {
+    S.hs->status = CFDHIP_INVALID_SYSTEM;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  const double* dinv = jacobi ? w.dinv.get() : nullptr;
+  Timer timer(ctx.stream);
+  const dim3 g = S.grid();
+  hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x, w.r.get(),
+                     w.rhat.get(), S.R, 0);
+  hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, w.v.get(), w.p.get());
+  CFDHIP_CHECK(hipGetLastError());
+  const int chunk = set.check_interval > 0 ? set.check_interval : 8;
+  int64_t launched = 0;
+  // A restart does not complete an iteration but still consumes launch
+  // budget; bound the total launches generously.
+  const int64_t launchCap = 4 * set.max_iterations + 64;
+  while (true) {
+    for (int k = 0; k < chunk && launched < launchCap; ++k, ++launched) {
+      hipLaunchKernelGGL(bcgRhoKernel, g, dim3(kBlock), 0, ctx.stream, n, w.rhat.get(), w.r.get(),
+                         S.R);
+      hipLaunchKernelGGL(bcgDirectionKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(),
+                         w.v.get(), dinv, w.p.get(), w.phat.get(), S.R.st);
+      hipLaunchKernelGGL((bcgVKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.phat.get(),
+                         w.rhat.get(), w.v.get(), S.R);
+      hipLaunchKernelGGL(bcgSKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), w.v.get(), dinv,
+                         w.s.get(), w.shat.get(), S.R);
+      hipLaunchKernelGGL((bcgTKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.shat.get(),
+                         w.s.get(), w.phat.get(), w.t.get(), x, S.R);
+      hipLaunchKernelGGL(bcgXKernel, g, dim3(kBlock), 0, ctx.stream, n, x, w.r.get(), w.phat.get(),
+                         w.shat.get(), w.s.get(), w.t.get(), S.R);
+    }
+    CFDHIP_CHECK(hipGetLastError());
+    S.pull();
+    if (S.hs->done) break;
+    if (S.hs->restart == 1) {
+      // BiCGSTAB.cpp:128-130: restart only after progress in this sequence.
+      if (!(S.hs->rnorm < S.hs->restartResidual)) {
+        S.hs->status = CFDHIP_BREAKDOWN;
+        S.hs->done = 1;
+        S.hs->iter = S.hs->completed;
+        break;
+      }
+      hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x,
+                         w.r.get(), w.rhat.get(), S.R, 1);
+      hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, w.v.get(), w.p.get());
+      CFDHIP_CHECK(hipGetLastError());
+      S.pull();
+      if (S.hs->done) {
+        S.hs->iter = S.hs->completed;
+        break;
+      }
+      S.hs->restart = 0;
+      S.push();
+    }
+    if (launched >= launchCap) break;
+  }
+  const double ms = timer.stop();
+  if (!S.hs->done) {
+    S.hs->status = CFDHIP_MAX_ITERATIONS;
+    S.hs->done = 1;
+  }
+  if (S.hs->status == CFDHIP_BREAKDOWN) S.hs->iter = S.hs->completed;
+  const double tr = trueResidual(ctx, A, n, b, x);
+  return S.result(tr, ms);
+}
+
+template <class Op>
+cfdhip_solve_result dispatch(Context& ctx, Op A, int64_t n, const cfdhip_solver_settings& s,
+                             const double* b, double* x,
+                             const std::vector<std::pair<const double*, int64_t>>& arrays) {
+  if (s.type == CFDHIP_SOLVER_CG) return runCG(ctx, A, n, s, b, x, arrays);
+  if (s.type == CFDHIP_SOLVER_BICGSTAB) return runBiCGSTAB(ctx, A, n, s, b, x, arrays);
+  throw Error(CFDHIP_ERR_INVALID, "unsupported linear solver type for this operator");
+}
+
+PressureOp pressureOp(Context& ctx) {
+  return PressureOp{ctx.view, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
+                    ctx.pin};
+}
+
+}  // namespace
+
+cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s, const double* b,
+                                  double* x, bool zeroGuess) {
+  ctx.requireMesh();
+  require(ctx.pressureAssembled, "no p' system assembled");
+  const int64_t n = ctx.view.n;
+  if (zeroGuess) {
+    hipLaunchKernelGGL(zeroKernel, gridStride(n), dim3(kBlock), 0, ctx.stream, n, x, nullptr);
+    CFDHIP_CHECK(hipGetLastError());
+  }
+  return dispatch(ctx, pressureOp(ctx), n, s, b, x,
+                  {{ctx.f(CFDHIP_FIELD_PC_DIAG), n}, {ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.view.nf}});
+}
+
+cfdhip_solve_result solveStencil(Context& ctx, const cfdhip_solver_settings& s, const double* b,
+                                 double* x) {
+  ctx.requireMesh();
+  require(ctx.momentumAssembled, "no momentum system assembled");
+  const int64_t n = ctx.view.n;
+  StencilOp A{ctx.view, ctx.momCoef.get()};
+  return dispatch(ctx, A, n, s, b, x, {{ctx.momCoef.get(), (2 * ctx.view.dim + 1) * n}});
+}
+
+cfdhip_solve_result solveCsr(Context& ctx, const cfdhip_solver_settings& s, const CsrDevice& M,
+                             const double* b, double* x) {
+  CsrOp A{M.n, M.rp.get(), M.col.get(), M.val.get()};
+  return dispatch(ctx, A, M.n, s, b, x, {{M.val.get(), M.val.count}});
+}
+
+void pressureSpmv(Context& ctx, const double* x, double* y) {
+  ctx.requireMesh();
+  require(ctx.pressureAssembled, "no p' system assembled");
+  hipLaunchKernelGGL((spmvKernel<PressureOp>), gridStride(ctx.view.n), dim3(kBlock), 0, ctx.stream,
+                     pressureOp(ctx), ctx.view.n, x, y);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+void csrSpmv(Context& ctx, const CsrDevice& M, const double* x, double* y) {
+  CsrOp A{M.n, M.rp.get(), M.col.get(), M.val.get()};
+  hipLaunchKernelGGL((spmvKernel<CsrOp>), gridStride(M.n), dim3(kBlock), 0, ctx.stream, A, M.n, x,
+                     y);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace cfdhip

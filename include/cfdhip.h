@@ -1,0 +1,250 @@
+/*
+ * cfdhip.h -- C ABI of libcfdhip, the MI355X (gfx950) pressure-velocity path.
+ *
+ * Plain C: opaque context, caller-owned host buffers, int status codes
+ * (0 = OK, see cfdhip_last_error), no exceptions and no C++ types crossing the
+ * boundary. Device memory belongs to the context; one HIP stream per context.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository). The reference reaches these through
+ *   cfd::gpu::makeGpuCG / makeGpuBiCGSTAB    (include/cfd/gpu/GpuLinearSolver.hpp:48,52)
+ *   cfd::gpu::GpuSimpleDiscretization        (include/cfd/gpu/GpuSimpleDiscretization.hpp:56-262)
+ *   cfd::gpu::cudaAvailable                   (include/cfd/gpu/GPUBackend.hpp:15)
+ * and a maintainer binds them as shown in INTEGRATION.md.
+ *
+ * Arithmetic: everything on the reference path is FP64. Assembly, gradient,
+ * flux, correction and SpMV results are bitwise equal to the reference CPU
+ * path on the same inputs; Krylov iterates differ only by reduction order.
+ * The mixed-precision (CFDHIP_MIXED) and learned-preconditioner solves are
+ * additions with no reference counterpart; they are judged by the same FP64
+ * true-residual tolerance.
+ *
+ * Threading: a context is not re-entrant; calls on one context must not
+ * overlap. Distinct contexts are independent.
+ */
+#ifndef CFDHIP_H
+#define CFDHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cfdhip_ctx cfdhip_ctx;
+
+/* Return codes. */
+enum {
+  CFDHIP_OK = 0,
+  CFDHIP_ERR_INVALID = 1,     /* bad argument or state (reference: InvalidArgumentError) */
+  CFDHIP_ERR_RUNTIME = 2,     /* HIP runtime failure (reference: NumericalError via checkCuda) */
+  CFDHIP_ERR_NO_DEVICE = 3,   /* no usable gfx950 device */
+  CFDHIP_ERR_UNSUPPORTED = 4, /* configuration outside the structured path (prepare() == false) */
+  CFDHIP_ERR_NONFINITE = 5    /* assembled system or state not finite (SIMPLE NonFiniteState) */
+};
+
+/* cfd::algebra::SolverStatus (include/cfd/algebra/LinearSolver.hpp:16-23), same order. */
+enum {
+  CFDHIP_CONVERGED = 0,
+  CFDHIP_MAX_ITERATIONS = 1,
+  CFDHIP_BREAKDOWN = 2,
+  CFDHIP_NONFINITE_INPUT = 3,
+  CFDHIP_NONFINITE_RESIDUAL = 4,
+  CFDHIP_INVALID_SYSTEM = 5
+};
+
+/* cfd::algebra::LinearSolverType (LinearSolver.hpp:31-39) plus flexible CG. */
+enum { CFDHIP_SOLVER_CG = 0, CFDHIP_SOLVER_BICGSTAB = 1, CFDHIP_SOLVER_FCG = 3 };
+/* cfd::algebra::PreconditionerType (LinearSolver.hpp:62-65) plus the learned one. */
+enum { CFDHIP_PREC_NONE = 0, CFDHIP_PREC_JACOBI = 1, CFDHIP_PREC_LEARNED = 2 };
+/* Arithmetic of the inner iteration. */
+enum { CFDHIP_FP64 = 0, CFDHIP_MIXED = 1 };
+
+/* Velocity / pressure boundary-condition kinds (src/boundary/Wall.cpp etc.). */
+enum { CFDHIP_WALL = 0, CFDHIP_MOVING_WALL = 1, CFDHIP_INLET = 2, CFDHIP_OUTLET = 3 };
+enum { CFDHIP_FIXED_GRADIENT = 0, CFDHIP_FIXED_VALUE = 1 };
+
+/* cfd::algebra::LinearSolverSettings (LinearSolver.hpp:136-166) as POD. */
+typedef struct {
+  double absolute_tolerance;
+  double relative_tolerance;
+  int64_t max_iterations;
+  int32_t type;           /* CFDHIP_SOLVER_* */
+  int32_t preconditioner; /* CFDHIP_PREC_* */
+  int32_t precision;      /* CFDHIP_FP64 or CFDHIP_MIXED */
+  int32_t check_interval; /* iterations between host reads of the device verdict (0 = 32) */
+  int32_t stagnation_window; /* FCG: fall back to Jacobi-PCG after this many non-improving iterations (0 = 200) */
+  int32_t reserved;
+} cfdhip_solver_settings;
+
+/* cfd::algebra::SolverResult (LinearSolver.hpp:103-134) without the solution
+ * vector (copied separately into caller memory). */
+typedef struct {
+  int32_t status;          /* CFDHIP_CONVERGED ... */
+  int32_t fallback_used;   /* FCG: 1 if the Jacobi-PCG fallback finished the solve */
+  int64_t iterations;
+  int64_t restarts;        /* BiCGSTAB restarts / refinement cycles (mixed) */
+  double initial_residual; /* |b - A x0| */
+  double final_residual;   /* recurrence residual at exit (reference semantics) */
+  double true_residual;    /* |b - A x| recomputed in FP64 at exit */
+  double device_ms;        /* device time of the solve (HIP events on the context stream) */
+  int64_t history_length;  /* residual history entries available (cfdhip_solve_history) */
+} cfdhip_solve_result;
+
+/* Persistent device fields of a case (GpuSimpleDiscretization::PersistentField
+ * and the intermediate stages). Cell fields hold n doubles, face fields nf. */
+enum {
+  CFDHIP_FIELD_U = 0, CFDHIP_FIELD_V = 1, CFDHIP_FIELD_W = 2,
+  CFDHIP_FIELD_P = 3, CFDHIP_FIELD_PPRIME = 4,
+  CFDHIP_FIELD_FLUX = 5,       /* face mass flux F (faces) */
+  CFDHIP_FIELD_FLUX_STAR = 6,  /* predicted face flux F* (faces) */
+  CFDHIP_FIELD_VISCOSITY = 7,  /* effective viscosity (cells) */
+  CFDHIP_FIELD_RESPONSE = 8,   /* d = V / a_P (cells; equal for u, v, w) */
+  CFDHIP_FIELD_USTAR = 9, CFDHIP_FIELD_VSTAR = 10, CFDHIP_FIELD_WSTAR = 11,
+  CFDHIP_FIELD_GRAD_X = 12, CFDHIP_FIELD_GRAD_Y = 13, CFDHIP_FIELD_GRAD_Z = 14,
+  CFDHIP_FIELD_IMBALANCE = 15, /* continuity imbalance per cell */
+  CFDHIP_FIELD_MOM_RHS_U = 16, CFDHIP_FIELD_MOM_RHS_V = 17, CFDHIP_FIELD_MOM_RHS_W = 18,
+  CFDHIP_FIELD_MOM_DIAG = 19,  /* relaxed momentum diagonal (cells) */
+  CFDHIP_FIELD_PC_RHS = 20, CFDHIP_FIELD_PC_DIAG = 21,
+  CFDHIP_FIELD_PC_FACE_COEFF = 22, /* p' face coupling D_f (faces) */
+  CFDHIP_FIELD_COUNT = 23
+};
+
+/* ---- device ---------------------------------------------------------------- */
+
+/* cfd::gpu::cudaAvailable (include/cfd/gpu/GPUBackend.hpp:15; src/gpu/GPUBackend.cpp:10):
+ * 1 if a gfx950 device is usable, else 0. Never fails. */
+int cfdhip_device_available(void);
+
+/* Context = device + stream + resident case state. NULL when no device
+ * (the reference's makeGpu* then returns nullptr and the factory falls back,
+ * src/algebra/LinearSolverFactory.cpp:12-31). */
+cfdhip_ctx* cfdhip_create(int device);
+void cfdhip_destroy(cfdhip_ctx* ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char* cfdhip_last_error(const cfdhip_ctx* ctx);
+/* Blocks until all work queued on the context stream has finished. */
+int cfdhip_synchronize(cfdhip_ctx* ctx);
+
+/* ---- mesh plan (GpuSimpleDiscretization::prepare,
+ *      include/cfd/gpu/GpuSimpleDiscretization.hpp:74) ------------------------- */
+
+/* Uniform Cartesian mesh in the order of MeshGeometry::createCartesian2D/3D
+ * (src/mesh/MeshGeometry.cpp:428-553, 1047-1160). dim 2 ignores nz/lz.
+ * Returns CFDHIP_ERR_UNSUPPORTED (with the reason) for geometry the structured
+ * kernels do not cover. */
+int cfdhip_mesh_cartesian(cfdhip_ctx* ctx, int dim, int64_t nx, int64_t ny, int64_t nz,
+                          double lx, double ly, double lz);
+/* out[0..3] = {cells, faces, patches, dim}. */
+int cfdhip_mesh_sizes(cfdhip_ctx* ctx, int64_t* out);
+/* Face owner/neighbour (neighbour = -1 on a boundary face), generated on the
+ * device in the reference face order (Mesh face list / DeviceMeshCuda upload). */
+int cfdhip_mesh_faces(cfdhip_ctx* ctx, int64_t* owner, int64_t* neighbor);
+/* Boundary conditions of patch p (2D: left, right, bottom, top; 3D: xmin..zmax),
+ * as the BoundaryConditionSet the reference passes to prepare(). */
+int cfdhip_set_boundary(cfdhip_ctx* ctx, int patch, int velocity_kind, const double* velocity3,
+                        int pressure_kind, double pressure_value);
+/* FluidProperties density + uniform laminar viscosity (LaminarModel:
+ * mu_eff = mu + 0, src/turbulence/TurbulenceModel.cpp:13-24). */
+int cfdhip_set_fluid(cfdhip_ctx* ctx, double density, double viscosity);
+
+/* ---- fields (uploadInitialState / download*, GpuSimpleDiscretization.hpp) ---- */
+int cfdhip_field_upload(cfdhip_ctx* ctx, int field, const double* host, int64_t count);
+int cfdhip_field_download(cfdhip_ctx* ctx, int field, double* host, int64_t count);
+
+/* ---- SIMPLE stages (GpuSimpleDiscretization resident methods) ---------------- */
+
+/* Green-Gauss gradient of a cell field into GRAD_X/Y/Z; bc = 0 pressure
+ * conditions, 1 p' conditions (makeGradientBoundaries). gradient(),
+ * src/discretization/Gradient.cpp:534-605. */
+int cfdhip_gradient(cfdhip_ctx* ctx, int field, int bc);
+/* calculateMassFlux (src/physics/MassFlux.cpp:15) of U/V/W into FLUX. */
+int cfdhip_mass_flux(cfdhip_ctx* ctx);
+/* assembleRelaxedMomentumComponent for all components at once
+ * (src/pressure_velocity/RelaxedMomentum.cpp:30; the u, v, w matrices are
+ * identical, the right-hand sides differ). assembleMomentumResident. */
+int cfdhip_assemble_momentum(cfdhip_ctx* ctx, double alpha);
+/* solveMomentumResident: BiCGSTAB (or the type in settings) for one
+ * component, warm-started from that component; solution into USTAR+comp. */
+int cfdhip_solve_momentum(cfdhip_ctx* ctx, int component, const cfdhip_solver_settings* s,
+                          cfdhip_solve_result* r);
+/* computeResponseCoefficients: RESPONSE = V / a_P
+ * (PressureCorrectionEquation.cpp:103-114). */
+int cfdhip_response(cfdhip_ctx* ctx);
+/* computePredictedFaceFlux: FLUX_STAR from USTAR/VSTAR/WSTAR, linear or
+ * Rhie-Chow (src/pressure_velocity/RhieChow.cpp:53-65). */
+int cfdhip_predicted_flux(cfdhip_ctx* ctx, int rhie_chow, double alpha);
+/* assemblePressureCorrectionResident (PressureCorrectionEquation.cpp:181-367):
+ * PC_DIAG, PC_FACE_COEFF, PC_RHS; reference_cell pinned when no patch is FixedValue. */
+int cfdhip_assemble_pressure(cfdhip_ctx* ctx, int64_t reference_cell);
+/* solvePressureCorrectionResident: p' from x0 = 0 into PPRIME. */
+int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r);
+/* updatePressure: P += alpha * PPRIME (SIMPLE.cpp:939-942). */
+int cfdhip_update_pressure(cfdhip_ctx* ctx, double alpha);
+/* correctVelocityResident + correctFaceMassFluxResident
+ * (PressureCorrectionEquation.cpp:369-438): U/V/W and FLUX. */
+int cfdhip_correct(cfdhip_ctx* ctx);
+/* evaluateContinuity (src/physics/ContinuityEquation.cpp:14-55): IMBALANCE per
+ * cell; out[0] = RMS imbalance, out[1] = global net boundary flux. */
+int cfdhip_continuity(cfdhip_ctx* ctx, double* out);
+
+/* One whole SIMPLE run on the device (SIMPLE::solve, SIMPLE.cpp:148-1124,
+ * resident loop, laminar, default absolute convergence gate).
+ * settings[0..10] = {max_iterations, alpha_u, alpha_p, velocity_tol,
+ *   pressure_tol, continuity_tol, rhie_chow (0/1), reference_cell,
+ *   use_graph (0/1), 0, 0}; momentum/pressure are the two linear solvers.
+ * meta[0..5] out = {status (SIMPLEStatus: 0 converged, 1 max iterations,
+ *   2 momentum failure, 3 pressure failure, 4 non-finite), iterations,
+ *   momentum linear iterations, pressure linear iterations, last continuity
+ *   RMS, last |global imbalance|}; history (5 x cap, may be NULL) =
+ *   u, v, w, p initial residuals and continuity RMS per outer iteration. */
+int cfdhip_simple_run(cfdhip_ctx* ctx, const double* settings,
+                      const cfdhip_solver_settings* momentum,
+                      const cfdhip_solver_settings* pressure, double* meta, double* history,
+                      int64_t cap);
+
+/* ---- assembled systems as CSR (the host-round-trip assemble* variants,
+ *      GpuSimpleDiscretizationCuda.cpp:44-57: exact zeros dropped) ------------ */
+/* Returns nnz through *nnz; buffers sized n+1 / capacity. */
+int cfdhip_pressure_csr(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
+                        int64_t capacity, int64_t* nnz);
+int cfdhip_momentum_csr(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
+                        int64_t capacity, int64_t* nnz);
+
+/* ---- stand-alone linear algebra (GpuCG / GpuBiCGSTAB,
+ *      cuda/kernels/GpuLinearSolverCuda.cpp:164-650) ----------------------------- */
+
+/* y = A x for a host CSR matrix (SparseMatrix::multiply,
+ * src/algebra/SparseMatrix.cpp:72-100; csrSpmvKernel). Row sums in ascending
+ * column order from 0.0: bitwise equal to the reference. */
+int cfdhip_csr_spmv(cfdhip_ctx* ctx, int64_t n, const int64_t* rowptr, const int64_t* col,
+                    const double* val, const double* x, double* y);
+/* LinearSolver::solve(system, x0) for a host CSR system
+ * (include/cfd/algebra/LinearSolver.hpp:171-186): CG / BiCGSTAB with the
+ * reference's convergence and breakdown rules. A 5-/7-point structured
+ * pattern on the current mesh runs on the stencil kernels; anything else
+ * on the CSR kernels. x receives the solution. */
+int cfdhip_csr_solve(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, int64_t n,
+                     const int64_t* rowptr, const int64_t* col, const double* val,
+                     const double* b, const double* x0, double* x, cfdhip_solve_result* r);
+/* y = A x with the resident p' operator (PC_DIAG / PC_FACE_COEFF). */
+int cfdhip_pressure_spmv(cfdhip_ctx* ctx, const double* x, double* y);
+/* Solve the resident p' operator for a caller RHS (the stand-alone p' solve):
+ * x0 = 0, solution into x (may be NULL to keep it on the device in PPRIME). */
+int cfdhip_pressure_solve_rhs(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
+                              double* x, cfdhip_solve_result* r);
+/* Residual history of the last solve on this context (up to cap entries). */
+int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t* length);
+
+/* ---- timing -------------------------------------------------------------------- */
+/* Kernel timing of the last solve: out[0] = SpMV kernel launches,
+ * out[1] = their mean duration in microseconds (HIP events on the context
+ * stream), out[2] = iterations timed. Filled when timing is enabled. */
+int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled);
+int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFDHIP_H */

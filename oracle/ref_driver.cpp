@@ -12,6 +12,10 @@
 //   simple  dim nx ny [nz] re iters flux psolver -> SIMPLE::solve fields+histories (a15)
 //   stages  dim nx ny [nz] re warm flux psolver  -> one hand-driven SIMPLE iteration,
 //                                                 every stage dumped (a3-a14)
+//   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
+//                                                 iterations on the seeded p' system
+//                                                 (CPU baseline; prints one JSON line)
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -428,6 +432,41 @@ void cmdStages() {
   dumpF("cont_scalars", {std::sqrt(ss / static_cast<double>(n)), cont.globalNetFlux});
 }
 
+// CPU baseline: the reference's CG + Jacobi on the stand-alone p' system
+// (d == 1, cell 0 pinned, seeded mean-free RHS), bounded to pmax iterations.
+void cmdCgTime() {
+  const Case c = makeCavity();
+  const Mesh& m = c.mesh;
+  const size_t n = m.numberOfCells();
+  ScalarField ones(n, 1.0);
+  const SurfaceField zeroFlux(m.numberOfFaces(), 0.0);
+  pressure_velocity::PressureCorrectionOptions opts;
+  const auto asmb = pressure_velocity::assemblePressureCorrection(m, zeroFlux, ones, ones, 1.0, 0,
+                                                                  c.pbc, opts, nullptr);
+  SplitMix64 rng{static_cast<uint64_t>(iarg("seed", "20480001"))};
+  Vector b(n);
+  for (size_t i = 0; i < n; ++i) b[i] = 2.0 * rng.uniform() - 1.0;
+  double sum = 0.0;
+  for (size_t i = 0; i < n; ++i) sum += b[i];
+  const double mean = sum / static_cast<double>(n);
+  for (size_t i = 0; i < n; ++i) b[i] -= mean;
+  b[0] = 0.0;
+  LinearSolverSettings s;
+  s.type = algebra::LinearSolverType::CG;
+  s.absoluteTolerance = 0.0;
+  s.relativeTolerance = 1e-8;
+  s.maxIterations = iarg("pmax", "200");
+  algebra::CG cg(s, std::make_shared<algebra::JacobiPreconditioner>());
+  const algebra::LinearSystem sys(asmb.system.matrix(), b);
+  const auto t0 = std::chrono::steady_clock::now();
+  const auto r = cg.solve(sys);
+  const double sec =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::printf("{\"seconds\": %.9g, \"iterations\": %lld, \"status\": %d, \"final\": %.17g}\n",
+              sec, static_cast<long long>(r.iterations), static_cast<int>(r.status),
+              r.finalResidual);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -441,12 +480,13 @@ int main(int argc, char** argv) {
     if (eq == std::string::npos) continue;
     args[a.substr(0, eq)] = a.substr(eq + 1);
   }
-  outDir = arg("out");
+  outDir = arg("out", ".");
   const std::string cmd = argv[1];
   if (cmd == "mesh") cmdMesh();
   else if (cmd == "pcorr") cmdPcorr();
   else if (cmd == "simple") cmdSimple();
   else if (cmd == "stages") cmdStages();
+  else if (cmd == "cgtime") cmdCgTime();
   else {
     std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
     return 2;
