@@ -26,7 +26,7 @@ struct KState {
 // Work vectors (allocated once per size, reused across solves).
 struct KrylovWork {
   DeviceArray<double> r, p, Ap, z, x, b, dinv, rhat, v, s, t, phat, shat;
-  DeviceArray<float> r32, p32, Ap32, z32, x32, dinv32, w32;
+  DeviceArray<float> r32, p32, q32, Ap32, z32, x32, dinv32, w32;
   DeviceArray<double> partial;
   DeviceArray<unsigned> counter;
   DeviceArray<KState> state;
@@ -103,6 +103,15 @@ cfdhip_solve_result solveCsr(Context& ctx, const cfdhip_solver_settings& s, cons
                              const double* b, double* x);
 void pressureSpmv(Context& ctx, const double* x, double* y);
 void csrSpmv(Context& ctx, const CsrDevice& A, const double* x, double* y);
+
+// pcg2d.hip: fused 2D Jacobi-PCG (T = double / float) and the mixed-precision solve
+extern int pcg2dStripHeight;
+template <class T>
+cfdhip_solve_result fusedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const T* diag,
+                               const T* coeff, const T* b, T* x, T* r, T* p0, T* p1, T* Ap);
+cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
+                               double* x);
+void castToFloat(Context& ctx, int64_t n, const double* src, float* dst);
 
 // simple.hip
 void simpleRun(Context& ctx, const double* settings, const cfdhip_solver_settings& mom,

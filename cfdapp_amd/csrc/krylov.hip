@@ -947,8 +947,42 @@ cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s,
     hipLaunchKernelGGL(zeroKernel, gridStride(n), dim3(kBlock), 0, ctx.stream, n, x, nullptr);
     CFDHIP_CHECK(hipGetLastError());
   }
-  return dispatch(ctx, pressureOp(ctx), n, s, b, x,
-                  {{ctx.f(CFDHIP_FIELD_PC_DIAG), n}, {ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.view.nf}});
+  const std::vector<std::pair<const double*, int64_t>> opArrays = {
+      {ctx.f(CFDHIP_FIELD_PC_DIAG), n}, {ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.view.nf}};
+  const bool fused = ctx.view.dim == 2 && s.type == CFDHIP_SOLVER_CG &&
+                     s.preconditioner == CFDHIP_PREC_JACOBI;
+  if (s.precision == CFDHIP_MIXED)
+    require(fused && zeroGuess,
+            "mixed precision is implemented for the 2D p' operator with CG + Jacobi from x0 = 0",
+            CFDHIP_ERR_UNSUPPORTED);
+  if (!fused) return dispatch(ctx, pressureOp(ctx), n, s, b, x, opArrays);
+  // fused 2D Jacobi-PCG: same input checks and Jacobi rejection as runCG
+  PressureOp A = pressureOp(ctx);
+  KrylovWork& w = ctx.kw;
+  auto arrays = opArrays;
+  arrays.push_back({b, n});
+  arrays.push_back({x, n});
+  cfdhip_solve_result res{};
+  w.dinv.ensure(n);
+  if (!inputsFinite(ctx, A, arrays)) {
+    res.status = CFDHIP_NONFINITE_INPUT;
+    ctx.lastHistory.clear();
+    return res;
+  }
+  if (!buildJacobi(ctx, A, n, w.dinv.get())) {
+    res.status = CFDHIP_INVALID_SYSTEM;
+    ctx.lastHistory.clear();
+    return res;
+  }
+  if (s.precision == CFDHIP_MIXED) return mixedPcg2d(ctx, s, b, x);
+  w.r.ensure(n);
+  w.p.ensure(n);
+  w.z.ensure(n);
+  w.Ap.ensure(n);
+  res = fusedPcg2d<double>(ctx, s, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
+                           b, x, w.r.get(), w.p.get(), w.z.get(), w.Ap.get());
+  res.true_residual = trueResidual(ctx, A, n, b, x);
+  return res;
 }
 
 cfdhip_solve_result solveStencil(Context& ctx, const cfdhip_solver_settings& s, const double* b,

@@ -95,6 +95,35 @@ def test_pressure_cg_solution(name):
     assert r.final_residual <= 1e-8 * r.initial_residual
 
 
+@pytest.mark.parametrize("shape", [(129, 129), (257, 96)])
+def test_mixed_precision_pressure_solve(shape):
+    """FP32 inner PCG + FP64 refinement reaches the FP64 true-residual target
+    and the oracle's FP64 CG solution (within what that target guarantees)."""
+    print_case("mixed", shape)
+    nx, ny = shape
+    d = cfd.GpuSimpleDiscretization.cavity(2, nx, ny)
+    c = oracle.Case.cavity(2, nx, ny)
+    rng = np.random.default_rng(nx + ny)
+    D = rng.uniform(0.5, 2.0, d.n_cells)  # variable response coefficients
+    Fs = rng.uniform(-1, 1, d.n_faces)
+    d.upload("RESPONSE", D)
+    d.upload("FLUX_STAR", Fs)
+    d.assemble_pressure_correction_resident(0)
+    A, rhs, _ = c.pressure_system(Fs, D, D)
+    ref = A.cg(rhs, rel_tol=1e-12, max_iter=50000)
+    s = cfd.LinearSolverSettings(0.0, 1e-8, 50000, "CG", "Jacobi", precision="mixed")
+    r = d.solve_pressure_rhs(s, rhs)
+    assert r.converged, r.status_name
+    true = np.linalg.norm(rhs - A.spmv(r.solution))
+    assert true <= 1e-8 * np.linalg.norm(rhs)
+    assert r.true_residual == pytest.approx(true, rel=1e-6)
+    assert r.restarts >= 2  # refinement cycles
+    assert np.abs(r.solution - ref.x).max() <= 1e-5 * np.abs(ref.x).max()
+    # FP64 fused path on the same system, for the iteration comparison
+    r64 = d.solve_pressure_rhs(cfd.LinearSolverSettings(0.0, 1e-8, 50000, "CG", "Jacobi"), rhs)
+    assert r64.converged and r.iterations <= 2 * r64.iterations
+
+
 def test_csr_spmv_and_solvers_bitwise_spmv():
     print_case("csr spmv/solve")
     g = load_golden("stages2d_33_rc")
@@ -168,7 +197,13 @@ def test_simple_stages_bitwise(name):
     pr = d.solve_pressure_correction_resident(ps)
     pref = g["pc_solve_x"]
     assert pr.converged
-    assert np.abs(d.download("PPRIME") - pref).max() <= 1e-6 * np.abs(pref).max()
+    # same tolerance test as the reference run; the iterate it stops at can
+    # differ in slowly converging modes (unpreconditioned BiCGSTAB), so the
+    # solutions are compared at the accuracy the tolerance guarantees
+    assert pr.true_residual <= 2.0 * max(ps.absolute_tolerance,
+                                         ps.relative_tolerance * pr.initial_residual)
+    tol = 1e-6 if p["psolver"] == "cg_jacobi" else 1e-4
+    assert np.abs(d.download("PPRIME") - pref).max() <= tol * np.abs(pref).max()
     d.upload("PPRIME", pref)
     gpp = d.gradient("PPRIME", correction=True)
     for k in range(dim):
