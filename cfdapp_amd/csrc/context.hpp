@@ -26,11 +26,13 @@ struct KState {
 // Work vectors (allocated once per size, reused across solves).
 struct KrylovWork {
   DeviceArray<double> r, p, Ap, z, x, b, dinv, rhat, v, s, t, phat, shat;
-  DeviceArray<float> r32, p32, q32, Ap32, z32, x32, dinv32, w32;
+  DeviceArray<float> r32, x32, dinv32, w32;
+  DeviceArray<double> d8[8];  // single-kernel PCG buffers (FP64)
+  DeviceArray<float> f8[8];   // single-kernel PCG buffers (FP32)
   DeviceArray<double> partial;
-  DeviceArray<unsigned> counter;
   DeviceArray<KState> state;
   DeviceArray<double> hist;
+  DeviceArray<double> scal;  // scalar results of one-off reductions
   KState* hostState = nullptr;  // pinned mirror
 };
 
@@ -104,11 +106,13 @@ cfdhip_solve_result solveCsr(Context& ctx, const cfdhip_solver_settings& s, cons
 void pressureSpmv(Context& ctx, const double* x, double* y);
 void csrSpmv(Context& ctx, const CsrDevice& A, const double* x, double* y);
 
-// pcg2d.hip: fused 2D Jacobi-PCG (T = double / float) and the mixed-precision solve
-extern int pcg2dStripHeight;
+// cg1.hip: single-kernel 2D Jacobi-PCG iteration (T = double / float);
+// work = r0 r1 w0 w1 p0 p1 s0 s1
 template <class T>
-cfdhip_solve_result fusedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const T* diag,
-                               const T* coeff, const T* b, T* x, T* r, T* p0, T* p1, T* Ap);
+cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings& set,
+                                      const T* diag, const T* coeff, const T* b, T* x,
+                                      T* const work[8]);
+// pcg2d.hip: the mixed-precision solve
 cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                                double* x);
 void castToFloat(Context& ctx, int64_t n, const double* src, float* dst);

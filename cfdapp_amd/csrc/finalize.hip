@@ -1,0 +1,302 @@
+// The finalize kernel: reduces block partials in a fixed order and runs one
+// scalar step of a Krylov recurrence on the device (CG.cpp:45-147,
+// BiCGSTAB.cpp:88-274), so the host never waits on an inner product.
+#include <cfloat>
+
+#include "reduce.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+__device__ inline bool convergedTest(const KState& s, double a) {
+  return a <= s.absTol || ((s.b0 > 0.0) && (a / s.b0 <= s.relTol));
+}
+__device__ inline void finish(KState& s, int status) {
+  s.status = status;
+  s.done = 1;
+}
+__device__ inline void pushHist(KState& s, double* hist, double value) {
+  hist[s.histLen] = value;
+  s.histLen += 1;
+}
+// cancelledToRoundingLevel (BiCGSTAB.cpp:34-41) with sum_i |x_i y_i| at hand.
+__device__ inline bool cancelled(double ip, double absSum, double nx, double ny) {
+  if (fabs(ip) > DBL_EPSILON * nx * ny) return false;
+  return fabs(ip) <= DBL_EPSILON * absSum;
+}
+
+__device__ void step(KState& s, double* hist, double* out, int stepId, const double* t) {
+  switch (stepId) {
+    case kStepStartNorm: {  // r0 = b - A x0 (CG.cpp:45-57, BiCGSTAB.cpp:82-93)
+      const double norm = sqrt(t[0]);
+      s.b0 = norm;
+      s.rnorm = norm;
+      s.rHatNorm = norm;
+      s.restartResidual = norm;
+      pushHist(s, hist, norm);
+      if (norm == 0.0 || convergedTest(s, norm)) finish(s, CFDHIP_CONVERGED);
+      return;
+    }
+    case kStepCgStartRz: {  // rz = (r, z) (CG.cpp:59-69)
+      s.rz = t[0];
+      if (!isfinite(s.rz)) finish(s, CFDHIP_BREAKDOWN);
+      return;
+    }
+    case kStepCgPAp: {  // CG.cpp:77-95
+      const double pAp = t[0];
+      s.pAp = pAp;
+      if (!isfinite(pAp) || fabs(pAp) < 1e-30) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double alpha = s.rz / pAp;
+      if (!isfinite(alpha)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.alpha = alpha;
+      return;
+    }
+    case kStepCgUpdate: {  // CG.cpp:97-140
+      const double rnorm = sqrt(t[0]);
+      s.iter += 1;
+      pushHist(s, hist, rnorm);
+      s.rnorm = rnorm;
+      if (!isfinite(rnorm)) {
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      if (convergedTest(s, rnorm)) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      const double rzNew = t[1];
+      if (!isfinite(rzNew)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double beta = rzNew / s.rz;
+      if (!isfinite(beta)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.beta = beta;
+      s.rz = rzNew;
+      if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+      return;
+    }
+    case kStepCg1Start: {  // gamma0 = (r0,u0), delta0 = (w0,u0) = (p, Ap) of iteration 1
+      const double gamma = t[0], delta = t[1];
+      s.rz = gamma;
+      if (!isfinite(gamma)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (!isfinite(delta) || fabs(delta) < 1e-30) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double a = gamma / delta;
+      if (!isfinite(a)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.alpha = a;
+      s.beta = 0.0;
+      return;
+    }
+    case kStepCg1Iter: {
+      const double gamma = t[0], delta = t[1];
+      const double rnorm = sqrt(t[2]);
+      s.iter += 1;
+      pushHist(s, hist, rnorm);
+      s.rnorm = rnorm;
+      if (!isfinite(rnorm)) {
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      if (convergedTest(s, rnorm)) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      if (!isfinite(gamma)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double beta = gamma / s.rz;
+      const double pAp = delta - beta * gamma / s.alpha;  // (p', A p') of the next step
+      if (!isfinite(beta) || !isfinite(pAp) || fabs(pAp) < 1e-30) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double alpha = gamma / pAp;
+      if (!isfinite(alpha)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.beta = beta;
+      s.alpha = alpha;
+      s.rz = gamma;
+      if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+      return;
+    }
+    case kStepBcgRho: {  // BiCGSTAB.cpp:149-163
+      const double rho = t[0];
+      if (!isfinite(rho)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (cancelled(rho, t[1], s.rHatNorm, s.rnorm)) {
+        s.restart = 1;
+        return;
+      }
+      const double beta = (rho / s.rhoOld) * (s.alpha / s.omega);
+      if (!isfinite(beta)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.rho = rho;
+      s.beta = beta;
+      return;
+    }
+    case kStepBcgV: {  // BiCGSTAB.cpp:171-185
+      const double rHatDotV = t[0];
+      if (!isfinite(rHatDotV)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (cancelled(rHatDotV, t[1], s.rHatNorm, sqrt(t[2]))) {
+        s.restart = 1;
+        return;
+      }
+      const double alpha = s.rho / rHatDotV;
+      if (!isfinite(alpha)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.alpha = alpha;
+      return;
+    }
+    case kStepBcgS: {  // BiCGSTAB.cpp:187-207
+      const double sNorm = sqrt(t[0]);
+      s.snorm = sNorm;
+      const int64_t iter = s.completed + 1;
+      if (!isfinite(sNorm)) {
+        s.iter = iter;
+        pushHist(s, hist, sNorm);
+        s.rnorm = sNorm;
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      if (convergedTest(s, sNorm)) {
+        s.iter = iter;
+        pushHist(s, hist, sNorm);
+        s.rnorm = sNorm;
+        s.sConverged = 1;  // x += pHat alpha in the next worker
+      }
+      return;
+    }
+    case kStepBcgT: {  // BiCGSTAB.cpp:209-238
+      if (s.sConverged) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      const double tDotT = t[0];
+      if (!isfinite(tDotT) || tDotT < DBL_MIN) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double tDotS = t[1];
+      if (!isfinite(tDotS)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (cancelled(tDotS, t[2], sqrt(tDotT), s.snorm)) {
+        s.restart = 1;
+        return;
+      }
+      const double omega = tDotS / tDotT;
+      if (!isfinite(omega)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.omega = omega;
+      return;
+    }
+    case kStepBcgX: {  // BiCGSTAB.cpp:240-268
+      const double rn = sqrt(t[0]);
+      const int64_t iter = s.completed + 1;
+      s.rnorm = rn;
+      s.iter = iter;
+      pushHist(s, hist, rn);
+      if (!isfinite(rn)) {
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      s.completed = iter;
+      if (convergedTest(s, rn)) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      s.rhoOld = s.rho;
+      if (s.completed >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+      return;
+    }
+    case kStepBcgRestart: {  // BiCGSTAB.cpp:128-144, after the rNorm < restartResidual test
+      const double norm = sqrt(t[0]);
+      s.rnorm = norm;
+      s.restart = 0;
+      if (!isfinite(norm)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (convergedTest(s, norm)) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      s.rHatNorm = norm;
+      s.restartResidual = norm;
+      s.rhoOld = 1.0;
+      s.alpha = 1.0;
+      s.omega = 1.0;
+      s.restarts += 1;
+      s.restart = 2;  // the host zeroes v and p
+      return;
+    }
+    case kStepNorm:
+      out[0] = sqrt(t[0]);
+      return;
+    default:
+      return;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void finalizeKernel(FinalArgs a) {
+  KState& s = *a.st;
+  const bool bcg = a.step >= kStepBcgRho && a.step <= kStepBcgX;
+  const bool skip = a.step != kStepNorm && (s.done || (bcg && s.restart));
+  if (skip) return;
+  double t[3] = {0.0, 0.0, 0.0};
+  if (a.step == kStepCgUpdate || a.step == kStepBcgRho) {
+    double v[2];
+    sumPartials<2>(a.partial, a.nParts, v);
+    t[0] = v[0], t[1] = v[1];
+  } else if (a.step == kStepCg1Start || a.step == kStepCg1Iter || a.step == kStepBcgV ||
+             a.step == kStepBcgT) {
+    sumPartials<3>(a.partial, a.nParts, t);
+  } else {
+    double v[1];
+    sumPartials<1>(a.partial, a.nParts, v);
+    t[0] = v[0];
+  }
+  if (threadIdx.x == 0) step(s, a.hist, a.out, a.step, t);
+}
+
+}  // namespace
+
+void launchFinalize(Context& ctx, const FinalArgs& a) {
+  hipLaunchKernelGGL(finalizeKernel, dim3(1), dim3(kRedThreads), 0, ctx.stream, a);
+}
+
+}  // namespace cfdhip

@@ -15,14 +15,13 @@
 #include <cmath>
 #include <memory>
 
-#include "context.hpp"
+#include "reduce.hpp"
 
 namespace cfdhip {
 
 namespace {
 
 constexpr int kMaxBlocks = 2048;
-constexpr double kEps = DBL_EPSILON;
 
 inline dim3 gridStride(int64_t n) {
   int64_t b = (n + kBlock - 1) / kBlock;
@@ -105,70 +104,9 @@ struct CsrOp {
   }
 };
 
-// ---------------------------------------------------------------------------
-// Deterministic reduction of W sums; returns true in the last-arriving block,
-// where tot[] holds the grid totals in every thread.
-template <int W>
-__device__ bool reduceLast(double (&val)[W], double* partial, unsigned* counter, double (&tot)[W]) {
-  __shared__ double lds[W][kBlock / 64];
-  __shared__ int amLast;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int w = 0; w < W; ++w) {
-    double x = val[w];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) lds[w][wid] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 0; w < W; ++w) {
-      double t = 0.0;
-      for (int i = 0; i < kBlock / 64; ++i) t += lds[w][i];
-      partial[blockIdx.x * W + w] = t;
-    }
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    amLast = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!amLast) return false;
-  __threadfence();
-  for (int w = 0; w < W; ++w) {
-    double x = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) x += partial[b * W + w];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    __syncthreads();
-    if (lane == 0) lds[w][wid] = x;
-    __syncthreads();
-    double t = 0.0;
-    for (int i = 0; i < kBlock / 64; ++i) t += lds[w][i];
-    tot[w] = t;
-  }
-  if (threadIdx.x == 0) *counter = 0;
-  return true;
-}
-
-__device__ inline bool convergedTest(const KState& s, double a) {
-  return a <= s.absTol || ((s.b0 > 0.0) && (a / s.b0 <= s.relTol));
-}
-__device__ inline void finish(KState& s, int status) {
-  s.status = status;
-  s.done = 1;
-}
-__device__ inline void pushHist(KState& s, double* hist, double value) {
-  hist[s.histLen] = value;
-  s.histLen += 1;
-}
-// cancelledToRoundingLevel (BiCGSTAB.cpp:34-41) with the |x_i y_i| sum at hand.
-__device__ inline bool cancelled(double ip, double absSum, double nx, double ny) {
-  if (fabs(ip) > kEps * nx * ny) return false;
-  return fabs(ip) <= kEps * absSum;
-}
-
 struct Red {
   double* partial;
-  unsigned* counter;
   KState* st;
-  double* hist;
 };
 
 // ---------------------------------------------------------------------------
@@ -178,7 +116,8 @@ template <class Op>
 __global__ void residualKernel(Op A, int64_t n, const double* __restrict__ b,
                                const double* __restrict__ x, double* r, double* copy, Red R,
                                int mode) {
-  // mode 0: CG / BiCGSTAB start, 1: BiCGSTAB restart
+  // mode 0: CG / BiCGSTAB start, 1: BiCGSTAB restart (finalized by the host's step)
+  (void)mode;
   double rr[1] = {0.0};
   KState& s = *R.st;
   const bool skip = s.done;
@@ -190,40 +129,7 @@ __global__ void residualKernel(Op A, int64_t n, const double* __restrict__ b,
       if (copy) copy[i] = ri;
       rr[0] += ri * ri;
     }
-  double tot[1];
-  if (!reduceLast<1>(rr, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double norm = sqrt(tot[0]);
-  if (mode == 0) {
-    s.b0 = norm;
-    s.rnorm = norm;
-    s.rHatNorm = norm;
-    s.restartResidual = norm;
-    pushHist(s, R.hist, norm);
-    if (norm == 0.0 || convergedTest(s, norm)) {
-      s.rnorm = norm;
-      finish(s, CFDHIP_CONVERGED);
-    }
-    return;
-  }
-  // BiCGSTAB restart (BiCGSTAB.cpp:128-144), after the rNorm < restartResidual
-  // test the host already passed.
-  s.rnorm = norm;
-  s.restart = 0;
-  if (!isfinite(norm)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  if (convergedTest(s, norm)) {
-    finish(s, CFDHIP_CONVERGED);
-    return;
-  }
-  s.rHatNorm = norm;
-  s.restartResidual = norm;
-  s.rhoOld = 1.0;
-  s.alpha = 1.0;
-  s.omega = 1.0;
-  s.restarts += 1;
-  s.restart = 2;  // the host zeroes v and p
+  blockPartials<1>(rr, R.partial);
 }
 
 // Jacobi inverse diagonal with the reference's rejection rules; counts rejects.
@@ -257,13 +163,7 @@ __global__ void cgStartKernel(int64_t n, const double* __restrict__ r,
       p[i] = z;
       acc[0] += r[i] * z;
     }
-  double tot[1];
-  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  s.rz = tot[0];
-  if (!isfinite(s.rz)) {
-    s.rnorm = s.b0;
-    finish(s, CFDHIP_BREAKDOWN);
-  }
+  blockPartials<1>(acc, R.partial);
 }
 
 // Ap = A p; pAp -> alpha
@@ -279,20 +179,7 @@ __global__ void cgSpmvKernel(Op A, int64_t n, const double* __restrict__ p, doub
       Ap[i] = y;
       acc[0] += p[i] * y;
     }
-  double tot[1];
-  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double pAp = tot[0];
-  s.pAp = pAp;
-  if (!isfinite(pAp) || fabs(pAp) < 1e-30) {
-    finish(s, CFDHIP_BREAKDOWN);  // iterations = it - 1, final = |r|
-    return;
-  }
-  const double alpha = s.rz / pAp;
-  if (!isfinite(alpha)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  s.alpha = alpha;
+  blockPartials<1>(acc, R.partial);
 }
 
 // x += p alpha; r = r - Ap alpha; |r|, r . z -> convergence, beta
@@ -314,33 +201,7 @@ __global__ void cgUpdateKernel(int64_t n, double* x, double* r, const double* __
       acc[1] += rn * z;
     }
   }
-  double tot[2];
-  if (!reduceLast<2>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double rnorm = sqrt(tot[0]);
-  s.iter += 1;
-  pushHist(s, R.hist, rnorm);
-  s.rnorm = rnorm;
-  if (!isfinite(rnorm)) {
-    finish(s, CFDHIP_NONFINITE_RESIDUAL);
-    return;
-  }
-  if (convergedTest(s, rnorm)) {
-    finish(s, CFDHIP_CONVERGED);
-    return;
-  }
-  const double rzNew = tot[1];
-  if (!isfinite(rzNew)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  const double beta = rzNew / s.rz;
-  if (!isfinite(beta)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  s.beta = beta;
-  s.rz = rzNew;
-  if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+  blockPartials<2>(acc, R.partial);
 }
 
 // p = z + p beta
@@ -371,24 +232,7 @@ __global__ void bcgRhoKernel(int64_t n, const double* __restrict__ rhat,
       acc[0] += t;
       acc[1] += fabs(t);
     }
-  double tot[2];
-  if (!reduceLast<2>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double rho = tot[0];
-  if (!isfinite(rho)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  if (cancelled(rho, tot[1], s.rHatNorm, s.rnorm)) {
-    s.restart = 1;
-    return;
-  }
-  const double beta = (rho / s.rhoOld) * (s.alpha / s.omega);
-  if (!isfinite(beta)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  s.rho = rho;
-  s.beta = beta;
+  blockPartials<2>(acc, R.partial);
 }
 
 // p = r + (p - v omega) beta; pHat = M p
@@ -422,23 +266,7 @@ __global__ void bcgVKernel(Op A, int64_t n, const double* __restrict__ phat,
       acc[1] += fabs(t);
       acc[2] += y * y;
     }
-  double tot[3];
-  if (!reduceLast<3>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double rHatDotV = tot[0];
-  if (!isfinite(rHatDotV)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  if (cancelled(rHatDotV, tot[1], s.rHatNorm, sqrt(tot[2]))) {
-    s.restart = 1;
-    return;
-  }
-  const double alpha = s.rho / rHatDotV;
-  if (!isfinite(alpha)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  s.alpha = alpha;
+  blockPartials<3>(acc, R.partial);
 }
 
 // s = r - v alpha; |s| -> early convergence; sHat = M s
@@ -457,24 +285,7 @@ __global__ void bcgSKernel(int64_t n, const double* __restrict__ r, const double
       acc[0] += si * si;
     }
   }
-  double tot[1];
-  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double sNorm = sqrt(tot[0]);
-  s.snorm = sNorm;
-  const int64_t iter = s.completed + 1;
-  if (!isfinite(sNorm)) {
-    s.iter = iter;
-    pushHist(s, R.hist, sNorm);
-    s.rnorm = sNorm;
-    finish(s, CFDHIP_NONFINITE_RESIDUAL);
-    return;
-  }
-  if (convergedTest(s, sNorm)) {
-    s.iter = iter;
-    pushHist(s, R.hist, sNorm);
-    s.rnorm = sNorm;
-    s.sConverged = 1;  // x += pHat alpha happens in the next kernel
-  }
+  blockPartials<1>(acc, R.partial);
 }
 
 // t = A sHat; (t,t), (t,s) -> omega   (or the early-exit x update)
@@ -502,32 +313,7 @@ __global__ void bcgTKernel(Op A, int64_t n, const double* __restrict__ shat,
       acc[2] += fabs(ts);
     }
   }
-  double tot[3];
-  if (!reduceLast<3>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  if (early) {
-    finish(s, CFDHIP_CONVERGED);
-    return;
-  }
-  const double tDotT = tot[0];
-  if (!isfinite(tDotT) || tDotT < DBL_MIN) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  const double tDotS = tot[1];
-  if (!isfinite(tDotS)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  if (cancelled(tDotS, tot[2], sqrt(tDotT), s.snorm)) {
-    s.restart = 1;
-    return;
-  }
-  const double omega = tDotS / tDotT;
-  if (!isfinite(omega)) {
-    finish(s, CFDHIP_BREAKDOWN);
-    return;
-  }
-  s.omega = omega;
+  blockPartials<3>(acc, R.partial);
 }
 
 // x += pHat alpha; x += sHat omega; r = s - t omega; |r|
@@ -548,24 +334,7 @@ __global__ void bcgXKernel(int64_t n, double* x, double* r, const double* __rest
       acc[0] += ri * ri;
     }
   }
-  double tot[1];
-  if (!reduceLast<1>(acc, R.partial, R.counter, tot) || threadIdx.x != 0 || skip) return;
-  const double rn = sqrt(tot[0]);
-  const int64_t iter = s.completed + 1;
-  s.rnorm = rn;
-  s.iter = iter;
-  pushHist(s, R.hist, rn);
-  if (!isfinite(rn)) {
-    finish(s, CFDHIP_NONFINITE_RESIDUAL);
-    return;
-  }
-  s.completed = iter;
-  if (convergedTest(s, rn)) {
-    finish(s, CFDHIP_CONVERGED);
-    return;
-  }
-  s.rhoOld = s.rho;
-  if (s.completed >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+  blockPartials<1>(acc, R.partial);
 }
 
 __global__ void zeroKernel(int64_t n, double* a, double* b) {
@@ -598,17 +367,14 @@ __global__ void spmvKernel(Op A, int64_t n, const double* __restrict__ x, double
 // True residual |b - A x| (FP64), written to state.tDotT slot via a reduction.
 template <class Op>
 __global__ void trueResidualKernel(Op A, int64_t n, const double* __restrict__ b,
-                                   const double* __restrict__ x, double* partial,
-                                   unsigned* counter, double* out) {
+                                   const double* __restrict__ x, double* partial) {
   double acc[1] = {0.0};
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const double ri = b[i] - A.apply(i, x);
     acc[0] += ri * ri;
   }
-  double tot[1];
-  if (!reduceLast<1>(acc, partial, counter, tot) || threadIdx.x != 0) return;
-  *out = sqrt(tot[0]);
+  blockPartials<1>(acc, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,13 +388,9 @@ struct Solve {
   explicit Solve(Context& c, int64_t nn) : ctx(c), n(nn) {
     KrylovWork& w = ctx.kw;
     w.partial.ensure(kMaxBlocks * 4);
-    if (!w.counter.get()) {
-      w.counter.ensure(2);
-      CFDHIP_CHECK(hipMemsetAsync(w.counter.get(), 0, 2 * sizeof(unsigned), ctx.stream));
-    }
     w.state.ensure(1);
     if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
-    R = Red{w.partial.get(), w.counter.get(), w.state.get(), nullptr};
+    R = Red{w.partial.get(), w.state.get()};
     hs = w.hostState;
   }
   void begin(const cfdhip_solver_settings& s) {
@@ -648,7 +410,11 @@ struct Solve {
     *hs = k;
     CFDHIP_CHECK(hipMemcpyAsync(R.st, hs, sizeof(KState), hipMemcpyHostToDevice, ctx.stream));
     ctx.kw.hist.ensure(s.max_iterations + 2);
-    R.hist = ctx.kw.hist.get();
+  }
+  // the finalize step after a worker kernel of `grid` blocks
+  void fin(int step, dim3 grid) {
+    launchFinalize(ctx, FinalArgs{R.partial, static_cast<int64_t>(grid.x), R.st,
+                                  ctx.kw.hist.get(), nullptr, step});
   }
   void pull() {
     CFDHIP_CHECK(hipMemcpyAsync(hs, R.st, sizeof(KState), hipMemcpyDeviceToHost, ctx.stream));
@@ -670,8 +436,8 @@ struct Solve {
     r.history_length = hs->histLen;
     ctx.lastHistory.resize(hs->histLen);
     if (hs->histLen > 0)
-      CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), R.hist, hs->histLen * sizeof(double),
-                             hipMemcpyDeviceToHost));
+      CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), ctx.kw.hist.get(),
+                             hs->histLen * sizeof(double), hipMemcpyDeviceToHost));
     return r;
   }
 };
@@ -728,38 +494,27 @@ struct Timer {
 template <class Op>
 double trueResidual(Context& ctx, Op A, int64_t n, const double* b, const double* x) {
   KrylovWork& w = ctx.kw;
-  double* out = w.partial.get() + kMaxBlocks * 4 - 1;
-  hipLaunchKernelGGL((trueResidualKernel<Op>), gridStride(n), dim3(kBlock), 0, ctx.stream, A, n, b,
-                     x, w.partial.get(), w.counter.get() + 1, out);
+  w.scal.ensure(4);
+  w.partial.ensure(kMaxBlocks * 4);
+  w.state.ensure(1);
+  double* out = w.scal.get() + 1;
+  const dim3 g = gridStride(n);
+  hipLaunchKernelGGL((trueResidualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x,
+                     w.partial.get());
   CFDHIP_CHECK(hipGetLastError());
+  launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                nullptr, out, kStepNorm});
   double h = 0.0;
   CFDHIP_CHECK(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
   CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
   return h;
 }
 
-struct EventPool {
-  std::vector<hipEvent_t> ev;
-  ~EventPool() {
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  }
-};
-
-EventPool& eventPool(Context& ctx, int chunk) {
-  static thread_local std::vector<std::pair<Context*, std::unique_ptr<EventPool>>> pools;
-  for (auto& p : pools)
-    if (p.first == &ctx && static_cast<int>(p.second->ev.size()) >= 2 * chunk) return *p.second;
-  auto pool = std::make_unique<EventPool>();
-  pool->ev.resize(2 * chunk);
-  for (hipEvent_t& e : pool->ev) CFDHIP_CHECK(hipEventCreate(&e));
-  pools.emplace_back(&ctx, std::move(pool));
-  return *pools.back().second;
-}
-
 // CG with optional Jacobi; b, x device pointers (x holds x0 on entry).
 template <class Op>
 cfdhip_solve_result runCG(Context& ctx, Op A, int64_t n, const cfdhip_solver_settings& set,
-                          const double* b, double* x, const std::vector<std::pair<const double*, int64_t>>& opArrays) {
+                          const double* b, double* x,
+                          const std::vector<std::pair<const double*, int64_t>>& opArrays) {
   KrylovWork& w = ctx.kw;
   w.r.ensure(n);
   w.p.ensure(n);
@@ -786,49 +541,29 @@ cfdhip_solve_result runCG(Context& ctx, Op A, int64_t n, const cfdhip_solver_set
   const dim3 g = S.grid();
   hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x, w.r.get(),
                      nullptr, S.R, 0);
+  S.fin(kStepStartNorm, g);
   hipLaunchKernelGGL(cgStartKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), dinv, w.p.get(),
                      S.R);
+  S.fin(kStepCgStartRz, g);
   CFDHIP_CHECK(hipGetLastError());
   const int chunk = set.check_interval > 0 ? set.check_interval : 32;
   int64_t launched = 0;
-  // Kernel timing: one event pair per SpMV launch of a chunk, read after the
-  // chunk's verdict sync, so timing never stalls the launch pipeline.
-  EventPool* pool = ctx.timing ? &eventPool(ctx, chunk) : nullptr;
-  double spmvMs = 0.0, spmvLaunches = 0.0;
   while (true) {
-    int k = 0;
-    for (; k < chunk && launched < set.max_iterations; ++k, ++launched) {
-      if (pool) CFDHIP_CHECK(hipEventRecord(pool->ev[2 * k], ctx.stream));
+    for (int k = 0; k < chunk && launched < set.max_iterations; ++k, ++launched) {
       hipLaunchKernelGGL((cgSpmvKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.p.get(),
                          w.Ap.get(), S.R);
-      if (pool) CFDHIP_CHECK(hipEventRecord(pool->ev[2 * k + 1], ctx.stream));
+      S.fin(kStepCgPAp, g);
       hipLaunchKernelGGL(cgUpdateKernel, g, dim3(kBlock), 0, ctx.stream, n, x, w.r.get(),
                          w.p.get(), w.Ap.get(), dinv, S.R);
+      S.fin(kStepCgUpdate, g);
       hipLaunchKernelGGL(cgDirectionKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), dinv,
                          w.p.get(), S.R.st);
     }
     CFDHIP_CHECK(hipGetLastError());
-    const int64_t doneBefore = S.hs->iter;
     S.pull();
-    if (pool) {
-      // count only launches that did work (iterations completed in this chunk,
-      // plus the one that ended it)
-      const int64_t worked = std::min<int64_t>(k, S.hs->iter - doneBefore + (S.hs->done ? 1 : 0));
-      for (int q = 0; q < worked; ++q) {
-        float ms = 0.f;
-        CFDHIP_CHECK(hipEventElapsedTime(&ms, pool->ev[2 * q], pool->ev[2 * q + 1]));
-        spmvMs += ms;
-        spmvLaunches += 1.0;
-      }
-    }
     if (S.hs->done || launched >= set.max_iterations) break;
   }
   const double ms = timer.stop();
-  if (pool) {
-    ctx.spmvCount = spmvLaunches;
-    ctx.spmvMeanUs = spmvLaunches > 0 ? 1000.0 * spmvMs / spmvLaunches : 0.0;
-    ctx.timedIters = static_cast<double>(S.hs->iter);
-  }
   if (!S.hs->done) {
     S.hs->status = CFDHIP_MAX_ITERATIONS;
     S.hs->done = 1;
@@ -865,6 +600,7 @@ cfdhip_solve_result runBiCGSTAB(Context& ctx, Op A, int64_t n, const cfdhip_solv
   const dim3 g = S.grid();
   hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x, w.r.get(),
                      w.rhat.get(), S.R, 0);
+  S.fin(kStepStartNorm, g);
   hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, w.v.get(), w.p.get());
   CFDHIP_CHECK(hipGetLastError());
   const int chunk = set.check_interval > 0 ? set.check_interval : 8;
@@ -876,16 +612,21 @@ cfdhip_solve_result runBiCGSTAB(Context& ctx, Op A, int64_t n, const cfdhip_solv
     for (int k = 0; k < chunk && launched < launchCap; ++k, ++launched) {
       hipLaunchKernelGGL(bcgRhoKernel, g, dim3(kBlock), 0, ctx.stream, n, w.rhat.get(), w.r.get(),
                          S.R);
+      S.fin(kStepBcgRho, g);
       hipLaunchKernelGGL(bcgDirectionKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(),
                          w.v.get(), dinv, w.p.get(), w.phat.get(), S.R.st);
       hipLaunchKernelGGL((bcgVKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.phat.get(),
                          w.rhat.get(), w.v.get(), S.R);
+      S.fin(kStepBcgV, g);
       hipLaunchKernelGGL(bcgSKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), w.v.get(), dinv,
                          w.s.get(), w.shat.get(), S.R);
+      S.fin(kStepBcgS, g);
       hipLaunchKernelGGL((bcgTKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.shat.get(),
                          w.s.get(), w.phat.get(), w.t.get(), x, S.R);
+      S.fin(kStepBcgT, g);
       hipLaunchKernelGGL(bcgXKernel, g, dim3(kBlock), 0, ctx.stream, n, x, w.r.get(), w.phat.get(),
                          w.shat.get(), w.s.get(), w.t.get(), S.R);
+      S.fin(kStepBcgX, g);
     }
     CFDHIP_CHECK(hipGetLastError());
     S.pull();
@@ -900,6 +641,7 @@ cfdhip_solve_result runBiCGSTAB(Context& ctx, Op A, int64_t n, const cfdhip_solv
       }
       hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x,
                          w.r.get(), w.rhat.get(), S.R, 1);
+      S.fin(kStepBcgRestart, g);
       hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, w.v.get(), w.p.get());
       CFDHIP_CHECK(hipGetLastError());
       S.pull();
@@ -975,12 +717,13 @@ cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s,
     return res;
   }
   if (s.precision == CFDHIP_MIXED) return mixedPcg2d(ctx, s, b, x);
-  w.r.ensure(n);
-  w.p.ensure(n);
-  w.z.ensure(n);
-  w.Ap.ensure(n);
-  res = fusedPcg2d<double>(ctx, s, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
-                           b, x, w.r.get(), w.p.get(), w.z.get(), w.Ap.get());
+  double* work[8];
+  for (int q = 0; q < 8; ++q) {
+    w.d8[q].ensure(n);
+    work[q] = w.d8[q].get();
+  }
+  res = singleKernelPcg2d<double>(ctx, s, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                                  ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, work);
   res.true_residual = trueResidual(ctx, A, n, b, x);
   return res;
 }

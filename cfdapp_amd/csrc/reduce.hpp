@@ -1,0 +1,83 @@
+// Deterministic two-stage reductions without in-kernel fences.
+//
+// Stage 1 (inside a worker kernel): each block reduces its values with wave
+// shuffles and LDS and writes one partial per sum. Stage 2 (finalizeKernel,
+// one block, launched right after on the same stream): sums the partials in a
+// fixed order and runs the solver's scalar step on the device. The kernel
+// boundary makes the partials visible; no device-scope fence is needed
+// inside the worker, which on gfx950 would write back the L2 of the XCD once
+// per block while the kernel is streaming its stores.
+#pragma once
+
+#include "context.hpp"
+
+namespace cfdhip {
+
+constexpr int kRedThreads = 256;
+
+template <int W>
+__device__ inline void blockPartials(double (&val)[W], double* partial) {
+  __shared__ double lds[W][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < W; ++w) {
+    double x = val[w];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) lds[w][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < W) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += lds[threadIdx.x][i];
+    partial[static_cast<int64_t>(blockIdx.x) * W + threadIdx.x] = t;
+  }
+}
+
+// Sums W partial streams of nParts blocks in a fixed order (one block of 256).
+template <int W>
+__device__ inline void sumPartials(const double* partial, int64_t nParts, double (&tot)[W]) {
+  __shared__ double lds[W][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int w = 0; w < W; ++w) {
+    double x = 0.0;
+    for (int64_t b = threadIdx.x; b < nParts; b += blockDim.x) x += partial[b * W + w];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) lds[w][wid] = x;
+  }
+  __syncthreads();
+  for (int w = 0; w < W; ++w) {
+    double t = 0.0;
+    for (int i = 0; i < kRedThreads / 64; ++i) t += lds[w][i];
+    tot[w] = t;
+  }
+}
+
+// Scalar steps run by the finalize kernel (thread 0) on the reduced sums.
+enum FinalStep : int {
+  kStepStartNorm = 0,   // b0 = |r0|, converged at start?          (1 sum)
+  kStepCgStartRz,       // rz = (r, z)                             (1 sum)
+  kStepCgPAp,           // pAp -> alpha                            (1 sum)
+  kStepCgUpdate,        // |r|, (r, z) -> convergence, beta        (2 sums)
+  kStepCg1Start,        // gamma, delta -> alpha (single-kernel CG) (3 sums)
+  kStepCg1Iter,         // gamma, delta, |r| -> alpha, beta        (3 sums)
+  kStepBcgRho,          // rho, sum|.| -> beta                     (2 sums)
+  kStepBcgV,            // (rHat, v), sum|.|, |v|^2 -> alpha       (3 sums)
+  kStepBcgS,            // |s| -> early convergence                (1 sum)
+  kStepBcgT,            // (t,t), (t,s), sum|.| -> omega           (3 sums)
+  kStepBcgX,            // |r| -> convergence                      (1 sum)
+  kStepBcgRestart,      // |r| after the true-residual restart     (1 sum)
+  kStepNorm             // sqrt of the sum into out                (1 sum)
+};
+
+struct FinalArgs {
+  const double* partial;
+  int64_t nParts;
+  KState* st;
+  double* hist;
+  double* out;  // kStepNorm target
+  int step;
+};
+
+void launchFinalize(Context& ctx, const FinalArgs& a);
+
+}  // namespace cfdhip
