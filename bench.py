@@ -142,12 +142,11 @@ def main():
     kt = d.kernel_timing()
     d.set_timing(False)
     spmv_us = kt["mean_us"]
-    # algorithmic bytes of the fused kernel (direction update + SpMV + p.Ap):
-    # reads r, p_old, diag (n each) and the face couplings (nf); writes the new
-    # direction and A p (n each) -- all FP64
-    spmv_bytes = 8.0 * (5 * n + nf)
+    # algorithmic bytes of one single-kernel PCG iteration (FP64): reads p, s,
+    # r, w, x, diag (n each) and the face couplings (nf); writes p, s, r, w, x
+    spmv_bytes = 8.0 * (11 * n + nf)
     achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9 if spmv_us > 0 else 0.0
-    log(f"SpMV kernel: {kt['launches']:.0f} launches, mean {spmv_us:.2f} us, "
+    log(f"PCG iteration kernel: {kt['launches']:.0f} launches, mean {spmv_us:.2f} us, "
         f"{achieved:.0f} GB/s algorithmic")
 
     # --- mixed precision (FP32 inner PCG, FP64 refinement) on the same system ------
@@ -209,7 +208,7 @@ def main():
                    **mixed, **simple},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fusedSpmvKernel<double> (p' direction update + SpMV + p.Ap)",
+                     "kernel": "cgIterKernel<double> (one Jacobi-PCG iteration: updates + 5-point SpMV + dots)",
                      "kernel_mean_us": spmv_us, "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,
     }
