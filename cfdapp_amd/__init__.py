@@ -29,7 +29,8 @@ from ._lib import (BREAKDOWN, CONVERGED, FIELDS, FIXED_GRADIENT, FIXED_VALUE, FP
                    SOLVER_CG, SOLVER_FCG, STATUS_NAMES, WALL)
 
 __all__ = ["device_available", "LinearSolverSettings", "SolverResult", "make_gpu_cg",
-           "make_gpu_bicgstab", "GpuSimpleDiscretization", "CfdHipError"]
+           "make_gpu_bicgstab", "GpuSimpleDiscretization", "CfdHipError", "UnetNet",
+           "unet_param_count", "load_learned_weights", "WEIGHTS_PATH"]
 
 
 class CfdHipError(RuntimeError):
@@ -384,6 +385,29 @@ class GpuSimpleDiscretization:
         self._ctx.call("cfdhip_kernel_timing", _pf(out))
         return dict(launches=out[0], mean_us=out[1], iterations=out[2])
 
+    def set_learned_weights(self, weights=None):
+        """Loads the learned p' preconditioner (FP32 master weights; the
+        trained set shipped with the package by default)."""
+        w = np.ascontiguousarray(load_learned_weights() if weights is None else weights,
+                                 dtype=np.float32)
+        self._ctx.call("cfdhip_set_learned_weights", w.ctypes.data_as(_lib.P_F32), w.size)
+
+    def harvest(self, settings: LinearSolverSettings, stride, max_pairs, h, w, r_out, e_out,
+                ce_out=None, cn_out=None, b=None):
+        """Training pairs from the assembled p' system (see cfdhip_pressure_harvest).
+        r_out / e_out / ce_out / cn_out are device buffers (torch tensors or raw
+        addresses) of max_pairs*h*w and h*w floats. Returns (pairs, den, result)."""
+        den = np.zeros(max(1, int(max_pairs)), dtype=np.float32)
+        pairs = ctypes.c_int64(0)
+        _sync_producer()
+        r = _lib.SolveResult()
+        cs = settings.c()
+        self._ctx.call("cfdhip_pressure_harvest", ctypes.byref(cs),
+                       _pf(None if b is None else _f64(b)), int(stride), int(max_pairs), int(h),
+                       int(w), _dev(r_out), _dev(e_out), _dev(ce_out), _dev(cn_out),
+                       den.ctypes.data_as(_lib.P_F32), ctypes.byref(pairs), ctypes.byref(r))
+        return pairs.value, den[:pairs.value].copy(), _result(r, None, self._ctx.history())
+
     def simple_run(self, iterations, alpha_u=0.7, alpha_p=0.3, velocity_tol=1e-6,
                    pressure_tol=1e-6, continuity_tol=1e-6, rhie_chow=True, reference_cell=0,
                    momentum: LinearSolverSettings | None = None,
@@ -409,3 +433,111 @@ class GpuSimpleDiscretization:
 
 
 FACE_FIELDS = _lib.FACE_FIELDS
+
+
+# ---------------------------------------------------------------------------
+# Learned preconditioner
+
+WEIGHTS_PATH = __import__("os").path.join(_lib.HERE, "weights", "pprime_unet.f32")
+
+
+def unet_param_count() -> int:
+    return int(_lib.load().cfdhip_unet_param_count())
+
+
+def load_learned_weights(path: str | None = None) -> np.ndarray:
+    """The trained network's FP32 master weights (raw little-endian float32,
+    layer order e0a .. out, each W[ky][kx][cin][cout])."""
+    w = np.fromfile(path or WEIGHTS_PATH, dtype="<f4")
+    if w.size != unet_param_count():
+        raise ValueError(f"{path or WEIGHTS_PATH}: {w.size} weights, expected "
+                         f"{unet_param_count()}")
+    return w
+
+
+def _sync_producer():
+    """Device buffers handed to the library are usually filled by torch on its
+    own stream; the library works on its own stream, so wait for torch first."""
+    t = __import__("sys").modules.get("torch")
+    if t is not None and t.cuda.is_initialized():
+        t.cuda.synchronize()
+
+
+def _dev(t):
+    """Device address of a torch tensor (or an int address, or None)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class UnetNet:
+    """The learned preconditioner's network on the device (cfdhip_unet_*).
+    Inputs and outputs are caller-owned FP32 device buffers (torch tensors);
+    every call is synchronous with respect to the library's own stream, so
+    the caller synchronises its producer stream before calling."""
+
+    def __init__(self, weights=None):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.cfdhip_unet_create(ctypes.byref(h))
+        if rc != _lib.OK:
+            raise CfdHipError(rc, "cfdhip_unet_create failed")
+        self.h = h
+        if weights is not None:
+            self.set_weights(weights)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cfdhip_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _call(self, name, *args):
+        _sync_producer()
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != _lib.OK:
+            msg = self.lib.cfdhip_unet_last_error(self.h)
+            raise CfdHipError(rc, msg.decode() if msg else "")
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._call("cfdhip_unet_set_weights", w.ctypes.data_as(_lib.P_F32), w.size)
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(unet_param_count(), dtype=np.float32)
+        self._call("cfdhip_unet_get_weights", w.ctypes.data_as(_lib.P_F32), w.size)
+        return w
+
+    def forward(self, x, y, batch, h, w):
+        self._call("cfdhip_unet_forward", _dev(x), _dev(y), int(batch), int(h), int(w))
+
+    def backward(self, x, gy, batch, h, w) -> np.ndarray:
+        g = np.zeros(unet_param_count(), dtype=np.float32)
+        self._call("cfdhip_unet_backward", _dev(x), _dev(gy), int(batch), int(h), int(w),
+                   g.ctypes.data_as(_lib.P_F32))
+        return g
+
+    TENSORS = ["e0a", "e0", "p0", "e1a", "e1", "p1", "e2a", "e2", "p2", "b3a", "b3", "d2a", "d2",
+               "d1a", "d1", "d0a", "d0", "gd0", "gd0a", "gd1", "ge0s", "gd1a", "gd2", "ge1s",
+               "gd2a", "gb3", "ge2s", "gb3a", "gp2", "ge2a", "gp1", "ge1a", "gp0", "ge0a"]
+
+    def tensor(self, name) -> np.ndarray:
+        """An activation (or gradient) tensor of the last pass, flat NHWC."""
+        t = self.TENSORS.index(name)
+        m = int(self.lib.cfdhip_unet_tensor_size(self.h, t))
+        if m < 0:
+            raise CfdHipError(_lib.ERR_INVALID, f"tensor {name} is not available")
+        out = np.zeros(m, dtype=np.float32)
+        self._call("cfdhip_unet_tensor", t, out.ctypes.data_as(_lib.P_F32), m)
+        return out
+
+    def train_step(self, x, e, ce, cn, sys_offset, den, batch, h, w, lr, update=True) -> float:
+        loss = ctypes.c_double(0.0)
+        self._call("cfdhip_unet_train_step", _dev(x), _dev(e), _dev(ce), _dev(cn),
+                   _dev(sys_offset), _dev(den), int(batch), int(h), int(w), float(lr),
+                   1 if update else 0, ctypes.byref(loss))
+        return loss.value

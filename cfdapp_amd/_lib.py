@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libcfdhip.so")
 
 P_F64 = ctypes.POINTER(ctypes.c_double)
 P_I64 = ctypes.POINTER(ctypes.c_int64)
+P_F32 = ctypes.POINTER(ctypes.c_float)
+VP = ctypes.c_void_p
 
 # include/cfdhip.h enums
 OK, ERR_INVALID, ERR_RUNTIME, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_NONFINITE = range(6)
@@ -109,6 +111,27 @@ SIGNATURES = {
     "cfdhip_solve_history": (ctypes.c_int, [ctypes.c_void_p, P_F64, ctypes.c_int64, P_I64]),
     "cfdhip_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "cfdhip_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
+    # learned preconditioner (device buffers are passed as void*)
+    "cfdhip_unet_param_count": (ctypes.c_int64, []),
+    "cfdhip_set_learned_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
+    "cfdhip_pressure_harvest": (ctypes.c_int, [VP, ctypes.POINTER(SolverSettings), P_F64,
+                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.c_int64, VP, VP, VP, VP, P_F32, P_I64,
+                                               ctypes.POINTER(SolveResult)]),
+    "cfdhip_unet_create": (ctypes.c_int, [ctypes.POINTER(VP)]),
+    "cfdhip_unet_destroy": (None, [VP]),
+    "cfdhip_unet_last_error": (ctypes.c_char_p, [VP]),
+    "cfdhip_unet_set_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
+    "cfdhip_unet_get_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
+    "cfdhip_unet_forward": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_int64]),
+    "cfdhip_unet_backward": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, P_F32]),
+    "cfdhip_unet_tensor_size": (ctypes.c_int64, [VP, ctypes.c_int]),
+    "cfdhip_unet_tensor": (ctypes.c_int, [VP, ctypes.c_int, P_F32, ctypes.c_int64]),
+    "cfdhip_unet_train_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
+                                              ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                              ctypes.c_int, P_F64]),
 }
 
 _lib = None

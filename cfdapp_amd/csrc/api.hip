@@ -5,6 +5,7 @@
 #include <cstdio>
 
 #include "context.hpp"
+#include "unet_net.hpp"
 
 using namespace cfdhip;
 

@@ -22,6 +22,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
+#include <vector>
 
 #include "reduce.hpp"
 
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void cgStartKernel(Grid2 g, const T* __restric
 template <class T>
 cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings& set,
                                       const T* diag, const T* coeff, const T* b, T* x,
-                                      T* const work[8]) {
+                                      T* const work[8], Snapshots* snap) {
   const View& v = ctx.view;
   require(v.dim == 2, "the single-kernel PCG needs a 2D mesh");
   KrylovWork& w = ctx.kw;
@@ -243,6 +245,13 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
       if (ctx.timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q + 1], ctx.stream));
       finalize(kStepCg1Iter, grid.x);
       cur = 1 - cur;
+      if constexpr (std::is_same_v<T, double>) {
+        if (snap != nullptr && (launched + 1) % snap->stride == 0 && snap->taken < snap->cap) {
+          CFDHIP_CHECK(hipMemcpyAsync(snap->buf + snap->taken * v.n, x, sizeof(T) * v.n,
+                                      hipMemcpyDeviceToDevice, ctx.stream));
+          ++snap->taken;
+        }
+      }
     }
     CFDHIP_CHECK(hipGetLastError());
     const int64_t before = w.hostState->iter;
@@ -292,9 +301,9 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
 template cfdhip_solve_result singleKernelPcg2d<double>(Context&, const cfdhip_solver_settings&,
                                                        const double*, const double*,
                                                        const double*, double*,
-                                                       double* const[8]);
+                                                       double* const[8], Snapshots*);
 template cfdhip_solve_result singleKernelPcg2d<float>(Context&, const cfdhip_solver_settings&,
                                                       const float*, const float*, const float*,
-                                                      float*, float* const[8]);
+                                                      float*, float* const[8], Snapshots*);
 
 }  // namespace cfdhip

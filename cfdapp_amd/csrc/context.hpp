@@ -7,6 +7,8 @@
 
 namespace cfdhip {
 
+class Unet;
+
 // Device-resident state of one Krylov solve. Written by the finalising block
 // of each reduction kernel; the host only reads it every check_interval
 // iterations (and at the end) through a pinned mirror.
@@ -21,12 +23,18 @@ struct KState {
   int32_t done;    // 1 once status is final
   int32_t restart; // 1: the host must run the true-residual restart
   int32_t sConverged;
+  // flexible CG with the learned preconditioner: stagnation watch
+  double best;         // smallest residual norm so far
+  int64_t stagnant;    // iterations since it last decreased
+  int64_t window;      // stagnant iterations that end the solve (0: never)
+  int32_t stagnated;   // 1 when the window ran out
+  int32_t pad0;
 };
 
 // Work vectors (allocated once per size, reused across solves).
 struct KrylovWork {
   DeviceArray<double> r, p, Ap, z, x, b, dinv, rhat, v, s, t, phat, shat;
-  DeviceArray<float> r32, x32, dinv32, w32;
+  DeviceArray<float> r32, x32, dinv32, w32, dis32;
   DeviceArray<double> d8[8];  // single-kernel PCG buffers (FP64)
   DeviceArray<float> f8[8];   // single-kernel PCG buffers (FP32)
   DeviceArray<double> partial;
@@ -56,6 +64,8 @@ struct Context {
 
   KrylovWork kw;
   std::vector<double> lastHistory;
+  std::unique_ptr<Unet> net;  // learned p' preconditioner (weights loaded by the caller)
+  int64_t lastFallback = 0;
 
   // timing of the pressure SpMV kernel
   bool timing = false;
@@ -108,14 +118,24 @@ void csrSpmv(Context& ctx, const CsrDevice& A, const double* x, double* y);
 
 // cg1.hip: single-kernel 2D Jacobi-PCG iteration (T = double / float);
 // work = r0 r1 w0 w1 p0 p1 s0 s1
+// Optional copies of the iterate x every `stride` iterations (training data).
+struct Snapshots {
+  double* buf = nullptr;  // cap x n
+  int64_t stride = 0, cap = 0, taken = 0;
+};
 template <class T>
 cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings& set,
                                       const T* diag, const T* coeff, const T* b, T* x,
-                                      T* const work[8]);
+                                      T* const work[8], Snapshots* snap = nullptr);
 // pcg2d.hip: the mixed-precision solve
 cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                                double* x);
 void castToFloat(Context& ctx, int64_t n, const double* src, float* dst);
+// fcg.hip: flexible CG with the learned preconditioner (FP32 inner, FP64
+// refinement, Jacobi-PCG fallback on stagnation)
+cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
+                                 double* x);
+void loadLearnedWeights(Context& ctx, const float* weights);
 
 // simple.hip
 void simpleRun(Context& ctx, const double* settings, const cfdhip_solver_settings& mom,

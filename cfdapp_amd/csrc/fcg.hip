@@ -1,0 +1,376 @@
+// Flexible CG with the learned preconditioner on the 2D p' operator.
+//
+// Inner solve (FP32 vectors, FP64 reductions), truncated flexible CG with one
+// stored direction (Notay's FCG(1)):
+//   z = M r;  beta = -(z, q_old) / (p_old, q_old);  p = z + beta p_old
+//   q = A p;  alpha = (p, r) / (p, q);  d += alpha p;  r -= alpha q
+// with M r = D^-1/2 N(D^-1/2 r), N the network, and z = r / a_P at the
+// pinned reference cell. Outer loop (FP64): r = b - A x, inner solve for the
+// correction, x += d, until the FP64 true residual meets the tolerance
+// relative to |b - A x0|. An inner solve whose residual has not reached a new
+// minimum for `stagnation_window` iterations hands the solve to FP64
+// Jacobi-PCG from the current x (the fallback, reported in fallback_used).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "reduce.hpp"
+#include "unet_net.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+constexpr int kMaxParts = 2048;
+
+struct Op32 {
+  int64_t nx, ny, pin, offY;
+  const float* diag;
+  const float* coeff;
+};
+
+inline dim3 stride(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > kMaxParts) b = kMaxParts;
+  return dim3(static_cast<unsigned>(b < 1 ? 1 : b));
+}
+
+__device__ inline float pAt(const float* __restrict__ z, const float* __restrict__ pOld, float beta,
+                            bool first, int64_t c) {
+  return first ? z[c] : z[c] + beta * pOld[c];
+}
+
+// p = z + beta pOld; q = A p; partials (p, q), (p, r).
+__global__ __launch_bounds__(256) void fcgDirSpmvKernel(Op32 op, const float* __restrict__ z,
+                                                        const float* __restrict__ pOld,
+                                                        float* __restrict__ pNew,
+                                                        float* __restrict__ q,
+                                                        const float* __restrict__ r,
+                                                        const KState* __restrict__ st, int first,
+                                                        double* partial) {
+  double acc[2] = {0.0, 0.0};
+  if (!st->done) {
+    const float beta = static_cast<float>(st->beta);
+    const bool f = first != 0;
+    const int64_t n = op.nx * op.ny;
+    for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const int64_t i = c % op.nx, j = c / op.nx;
+      const float pc = pAt(z, pOld, beta, f, c);
+      float y;
+      if (c == op.pin) {
+        y = 0.f + op.diag[c] * pc;
+      } else {
+        y = 0.f;
+        if (j > 0) y += (-op.coeff[op.offY + j * op.nx + i]) * pAt(z, pOld, beta, f, c - op.nx);
+        if (i > 0) y += (-op.coeff[j * (op.nx + 1) + i]) * pAt(z, pOld, beta, f, c - 1);
+        y += op.diag[c] * pc;
+        if (i < op.nx - 1) y += (-op.coeff[j * (op.nx + 1) + i + 1]) * pAt(z, pOld, beta, f, c + 1);
+        if (j < op.ny - 1)
+          y += (-op.coeff[op.offY + (j + 1) * op.nx + i]) * pAt(z, pOld, beta, f, c + op.nx);
+      }
+      pNew[c] = pc;
+      q[c] = y;
+      acc[0] += static_cast<double>(pc) * y;
+      acc[1] += static_cast<double>(pc) * r[c];
+    }
+  }
+  blockPartials<2>(acc, partial);
+}
+
+// d += alpha p; r -= alpha q; partial |r|^2.
+__global__ __launch_bounds__(256) void fcgUpdateKernel(int64_t n, const float* __restrict__ p,
+                                                       const float* __restrict__ q,
+                                                       float* __restrict__ d, float* __restrict__ r,
+                                                       const KState* __restrict__ st,
+                                                       double* partial) {
+  double acc[1] = {0.0};
+  if (!st->done) {
+    const float alpha = static_cast<float>(st->alpha);
+    for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      d[c] += alpha * p[c];
+      const float rc = r[c] - alpha * q[c];
+      r[c] = rc;
+      acc[0] += static_cast<double>(rc) * rc;
+    }
+  }
+  blockPartials<1>(acc, partial);
+}
+
+__global__ void normKernel32(int64_t n, const float* __restrict__ r, double* partial) {
+  double acc[1] = {0.0};
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    acc[0] += static_cast<double>(r[c]) * r[c];
+  blockPartials<1>(acc, partial);
+}
+
+// D^-1/2 in FP32 (diagonal entries are positive on a valid p' system).
+__global__ void invSqrtKernel(int64_t n, const double* __restrict__ diag, float* __restrict__ dis) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dis[c] = static_cast<float>(1.0 / std::sqrt(diag[c]));
+}
+
+__global__ void zeroKernelD64(int64_t n, double* a) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[c] = 0.0;
+}
+
+__global__ void zeroF(int64_t n, float* a) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[c] = 0.f;
+}
+
+__global__ void addCorrection(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[c] += static_cast<double>(d[c]);
+}
+
+// r64 = b - A x (FP64) -> r32, |r64|^2 partials.
+__global__ __launch_bounds__(256) void outerResidualKernel(int64_t nx, int64_t ny, int64_t pin,
+                                                           int64_t offY,
+                                                           const double* __restrict__ diag,
+                                                           const double* __restrict__ coeff,
+                                                           const double* __restrict__ b,
+                                                           const double* __restrict__ x,
+                                                           float* __restrict__ r32,
+                                                           double* partial) {
+  const int64_t n = nx * ny;
+  double acc[1] = {0.0};
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % nx, j = c / nx;
+    double y;
+    if (c == pin) {
+      y = 0.0 + diag[c] * x[c];
+    } else {
+      y = 0.0;
+      if (j > 0) y += (-coeff[offY + j * nx + i]) * x[c - nx];
+      if (i > 0) y += (-coeff[j * (nx + 1) + i]) * x[c - 1];
+      y += diag[c] * x[c];
+      if (i < nx - 1) y += (-coeff[j * (nx + 1) + i + 1]) * x[c + 1];
+      if (j < ny - 1) y += (-coeff[offY + (j + 1) * nx + i]) * x[c + nx];
+    }
+    const double rc = b[c] - y;
+    r32[c] = static_cast<float>(rc);
+    acc[0] += rc * rc;
+  }
+  blockPartials<1>(acc, partial);
+}
+
+struct InnerResult {
+  int64_t iterations = 0;
+  int status = CFDHIP_MAX_ITERATIONS;
+  bool stagnated = false;
+};
+
+// The preconditioner: z = M r on the solver grid (ny x nx), network grid h x w.
+void applyM(Context& ctx, const float* r, const float* q, float* z, double* partial, int parts,
+            int h, int w) {
+  const View& v = ctx.view;
+  Unet& net = *ctx.net;
+  net.forward(r, ctx.kw.dis32.get(), static_cast<int>(v.ny), static_cast<int>(v.nx), 0, 1, h, w);
+  net.outputApply(ctx.kw.dis32.get(), r, q, z, static_cast<int>(v.ny), static_cast<int>(v.nx),
+                  ctx.pin, h, w, partial, parts);
+}
+
+InnerResult innerFcg(Context& ctx, double relTol, int64_t maxIter, int64_t window, float* r,
+                     float* d, std::vector<double>& history) {
+  const View& v = ctx.view;
+  const int64_t n = v.n;
+  KrylovWork& w = ctx.kw;
+  const int h = static_cast<int>((v.ny + 7) / 8 * 8), wd = static_cast<int>((v.nx + 7) / 8 * 8);
+  ctx.net->ensureShape(1, h, wd, false);
+  float* z = w.f8[0].get();
+  float* q = w.f8[1].get();
+  float* p[2] = {w.f8[2].get(), w.f8[3].get()};
+  const Op32 op{v.nx, v.ny, ctx.pin, v.offY, w.dinv32.get(), w.w32.get()};
+  const dim3 g = stride(n);
+  const int parts = static_cast<int>(g.x);
+  KState k{};
+  k.absTol = 0.0;
+  k.relTol = relTol;
+  k.maxIter = maxIter;
+  k.window = window;
+  k.status = -1;
+  *w.hostState = k;
+  CFDHIP_CHECK(hipMemcpyAsync(w.state.get(), w.hostState, sizeof(KState), hipMemcpyHostToDevice,
+                              ctx.stream));
+  auto fin = [&](int step, int64_t nParts) {
+    launchFinalize(ctx, FinalArgs{w.partial.get(), nParts, w.state.get(), w.hist.get(), nullptr,
+                                  step});
+  };
+  hipLaunchKernelGGL(zeroF, g, dim3(256), 0, ctx.stream, n, d);
+  hipLaunchKernelGGL(normKernel32, g, dim3(256), 0, ctx.stream, n, r, w.partial.get());
+  fin(kStepStartNorm, parts);
+  CFDHIP_CHECK(hipGetLastError());
+  const int chunk = 4;
+  int64_t launched = 0;
+  int cur = 0;
+  InnerResult res;
+  while (true) {
+    for (int c = 0; c < chunk && launched < maxIter; ++c, ++launched) {
+      const bool first = launched == 0;
+      // z = M r and (z, q_old) -> beta
+      applyM(ctx, r, first ? nullptr : q, z, first ? nullptr : w.partial.get(), parts, h, wd);
+      if (!first) fin(kStepFcgBeta, parts);
+      hipLaunchKernelGGL(fcgDirSpmvKernel, g, dim3(256), 0, ctx.stream, op, z, p[1 - cur], p[cur], q,
+                         r, w.state.get(), first ? 1 : 0, w.partial.get());
+      fin(kStepFcgAlpha, parts);
+      hipLaunchKernelGGL(fcgUpdateKernel, g, dim3(256), 0, ctx.stream, n, p[cur], q, d, r,
+                         w.state.get(), w.partial.get());
+      fin(kStepFcgUpdate, parts);
+      cur = 1 - cur;
+    }
+    CFDHIP_CHECK(hipGetLastError());
+    CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState), hipMemcpyDeviceToHost,
+                                ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    if (w.hostState->done || launched >= maxIter) break;
+  }
+  const KState& s = *w.hostState;
+  res.iterations = s.iter;
+  res.status = s.done ? s.status : CFDHIP_MAX_ITERATIONS;
+  res.stagnated = s.stagnated != 0;
+  std::vector<double> hist(static_cast<size_t>(s.histLen));
+  if (s.histLen > 0)
+    CFDHIP_CHECK(hipMemcpy(hist.data(), w.hist.get(), sizeof(double) * hist.size(),
+                           hipMemcpyDeviceToHost));
+  history.insert(history.end(), hist.begin(), hist.end());
+  return res;
+}
+
+}  // namespace
+
+void loadLearnedWeights(Context& ctx, const float* weights) {
+  if (!ctx.net) ctx.net = std::make_unique<Unet>(ctx.stream);
+  ctx.net->setWeights(weights);
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+}
+
+cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
+                                 double* x) {
+  require(ctx.net && ctx.net->ready(),
+          "the learned preconditioner has no weights: call cfdhip_set_learned_weights first");
+  const View& v = ctx.view;
+  const int64_t n = v.n, nf = v.nf;
+  KrylovWork& w = ctx.kw;
+  w.r32.ensure(n);
+  w.x32.ensure(n);
+  for (int q = 0; q < 4; ++q) w.f8[q].ensure(n);
+  w.dinv32.ensure(n);
+  w.w32.ensure(nf);
+  w.dis32.ensure(n);
+  castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
+  castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
+  const dim3 g = stride(n);
+  hipLaunchKernelGGL(invSqrtKernel, g, dim3(256), 0, ctx.stream, n, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                     w.dis32.get());
+  w.partial.ensure(kMaxParts * 4 + 8);
+  w.state.ensure(1);
+  w.scal.ensure(4);
+  w.hist.ensure(set.max_iterations + 8);
+  if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+  double* normOut = w.scal.get();
+  hipEvent_t t0, t1;
+  CFDHIP_CHECK(hipEventCreate(&t0));
+  CFDHIP_CHECK(hipEventCreate(&t1));
+  CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
+  hipLaunchKernelGGL(zeroKernelD64, g, dim3(256), 0, ctx.stream, n, x);
+  static const double innerTol = [] {
+    const char* e = std::getenv("CFDHIP_FCG_INNER_TOL");
+    const double t = e ? std::atof(e) : 1e-4;
+    return t > 0.0 && t < 1.0 ? t : 1e-4;
+  }();
+  std::vector<double> history;
+  double b0 = 0.0, rTrue = 0.0;
+  int64_t total = 0, cycles = 0;
+  int status = CFDHIP_MAX_ITERATIONS;
+  bool fallback = false;
+  while (true) {
+    hipLaunchKernelGGL(outerResidualKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin,
+                       v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b,
+                       x, w.r32.get(), w.partial.get());
+    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                  nullptr, normOut, kStepNorm});
+    CFDHIP_CHECK(hipGetLastError());
+    CFDHIP_CHECK(hipMemcpyAsync(&rTrue, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    if (cycles == 0) b0 = rTrue;
+    if (!std::isfinite(rTrue)) {
+      status = CFDHIP_NONFINITE_RESIDUAL;
+      break;
+    }
+    if (rTrue == 0.0 || rTrue <= set.absolute_tolerance ||
+        (b0 > 0.0 && rTrue / b0 <= set.relative_tolerance)) {
+      status = CFDHIP_CONVERGED;
+      break;
+    }
+    if (total >= set.max_iterations) break;
+    const double need = std::max(set.relative_tolerance * b0, set.absolute_tolerance) / rTrue;
+    const int64_t window = set.stagnation_window == 0 ? 200 : std::max(0, set.stagnation_window);
+    const InnerResult ir = innerFcg(ctx, std::max(innerTol, need), set.max_iterations - total,
+                                    window, w.r32.get(), w.x32.get(), history);
+    total += ir.iterations;
+    ++cycles;
+    if (ir.iterations > 0) {
+      hipLaunchKernelGGL(addCorrection, g, dim3(256), 0, ctx.stream, n, w.x32.get(), x);
+      CFDHIP_CHECK(hipGetLastError());
+    }
+    if (ir.stagnated || (ir.iterations == 0 && ir.status != CFDHIP_CONVERGED)) {
+      // FP64 Jacobi-PCG from the current x, to the same FP64 target
+      fallback = true;
+      cfdhip_solver_settings fb = set;
+      fb.type = CFDHIP_SOLVER_CG;
+      fb.preconditioner = CFDHIP_PREC_JACOBI;
+      fb.precision = CFDHIP_FP64;
+      fb.max_iterations = std::max<int64_t>(1, set.max_iterations - total);
+      // its relative test is against |b - A x| at its start; rescale to |b - A x0|
+      hipLaunchKernelGGL(outerResidualKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin,
+                         v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b,
+                         x, w.r32.get(), w.partial.get());
+      launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                    nullptr, normOut, kStepNorm});
+      double rNow = 0.0;
+      CFDHIP_CHECK(hipMemcpyAsync(&rNow, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+      fb.relative_tolerance = rNow > 0.0 ? std::min(1.0, set.relative_tolerance * b0 / rNow) : 0.0;
+      double* work[8];
+      for (int q = 0; q < 8; ++q) {
+        w.d8[q].ensure(n);
+        work[q] = w.d8[q].get();
+      }
+      const cfdhip_solve_result fr = singleKernelPcg2d<double>(
+          ctx, fb, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, work);
+      total += fr.iterations;
+      history.insert(history.end(), ctx.lastHistory.begin(), ctx.lastHistory.end());
+      continue;  // the outer test decides, in FP64
+    }
+  }
+  CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
+  CFDHIP_CHECK(hipEventSynchronize(t1));
+  float ms = 0.f;
+  CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  cfdhip_solve_result res{};
+  res.status = status;
+  res.fallback_used = fallback ? 1 : 0;
+  res.iterations = total;
+  res.restarts = cycles;
+  res.initial_residual = b0;
+  res.final_residual = rTrue;
+  res.true_residual = rTrue;
+  res.device_ms = ms;
+  ctx.lastHistory = history;
+  ctx.lastFallback = fallback ? 1 : 0;
+  res.history_length = static_cast<int64_t>(history.size());
+  return res;
+}
+
+}  // namespace cfdhip

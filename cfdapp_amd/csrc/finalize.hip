@@ -34,6 +34,8 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
       s.rnorm = norm;
       s.rHatNorm = norm;
       s.restartResidual = norm;
+      s.best = norm;
+      s.stagnant = 0;
       pushHist(s, hist, norm);
       if (norm == 0.0 || convergedTest(s, norm)) finish(s, CFDHIP_CONVERGED);
       return;
@@ -267,18 +269,66 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
     case kStepNorm:
       out[0] = sqrt(t[0]);
       return;
+    case kStepFcgAlpha: {  // flexible CG: alpha = (p, r) / (p, q)
+      const double pq = t[0], pr = t[1];
+      s.pAp = pq;
+      if (!isfinite(pq) || !isfinite(pr) || fabs(pq) < 1e-30) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double a = pr / pq;
+      if (!isfinite(a)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.alpha = a;
+      return;
+    }
+    case kStepFcgUpdate: {  // |r| -> convergence; stagnation window
+      const double rnorm = sqrt(t[0]);
+      s.iter += 1;
+      pushHist(s, hist, rnorm);
+      s.rnorm = rnorm;
+      if (!isfinite(rnorm)) {
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      if (convergedTest(s, rnorm)) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      if (rnorm < s.best) {
+        s.best = rnorm;
+        s.stagnant = 0;
+      } else if (++s.stagnant >= s.window && s.window > 0) {
+        s.stagnated = 1;
+        finish(s, CFDHIP_MAX_ITERATIONS);
+        return;
+      }
+      if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+      return;
+    }
+    case kStepFcgBeta: {  // beta = -(z, q_old) / (p_old, q_old)
+      const double b = -t[0] / s.pAp;
+      if (!isfinite(b)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.beta = b;
+      return;
+    }
     default:
       return;
   }
 }
 
-__global__ __launch_bounds__(kRedThreads) void finalizeKernel(FinalArgs a) {
+__global__ __launch_bounds__(kFinThreads) void finalizeKernel(FinalArgs a) {
   KState& s = *a.st;
   const bool bcg = a.step >= kStepBcgRho && a.step <= kStepBcgX;
   const bool skip = a.step != kStepNorm && (s.done || (bcg && s.restart));
   if (skip) return;
   double t[3] = {0.0, 0.0, 0.0};
-  if (a.step == kStepCgUpdate || a.step == kStepBcgRho) {
+  if (a.step == kStepCgUpdate || a.step == kStepBcgRho || a.step == kStepFcgAlpha) {
     double v[2];
     sumPartials<2>(a.partial, a.nParts, v);
     t[0] = v[0], t[1] = v[1];
@@ -296,7 +346,7 @@ __global__ __launch_bounds__(kRedThreads) void finalizeKernel(FinalArgs a) {
 }  // namespace
 
 void launchFinalize(Context& ctx, const FinalArgs& a) {
-  hipLaunchKernelGGL(finalizeKernel, dim3(1), dim3(kRedThreads), 0, ctx.stream, a);
+  hipLaunchKernelGGL(finalizeKernel, dim3(1), dim3(kFinThreads), 0, ctx.stream, a);
 }
 
 }  // namespace cfdhip

@@ -691,6 +691,29 @@ cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s,
   }
   const std::vector<std::pair<const double*, int64_t>> opArrays = {
       {ctx.f(CFDHIP_FIELD_PC_DIAG), n}, {ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.view.nf}};
+  if (s.preconditioner == CFDHIP_PREC_LEARNED) {
+    require(ctx.view.dim == 2 && zeroGuess &&
+                (s.type == CFDHIP_SOLVER_FCG || s.type == CFDHIP_SOLVER_CG),
+            "the learned preconditioner runs flexible CG on the 2D p' operator from x0 = 0",
+            CFDHIP_ERR_UNSUPPORTED);
+    PressureOp A = pressureOp(ctx);
+    KrylovWork& w = ctx.kw;
+    auto arrays = opArrays;
+    arrays.push_back({b, n});
+    cfdhip_solve_result res{};
+    w.dinv.ensure(n);
+    if (!inputsFinite(ctx, A, arrays)) {
+      res.status = CFDHIP_NONFINITE_INPUT;
+      ctx.lastHistory.clear();
+      return res;
+    }
+    if (!buildJacobi(ctx, A, n, w.dinv.get())) {  // the scaling needs a positive diagonal
+      res.status = CFDHIP_INVALID_SYSTEM;
+      ctx.lastHistory.clear();
+      return res;
+    }
+    return learnedFcg2d(ctx, s, b, x);
+  }
   const bool fused = ctx.view.dim == 2 && s.type == CFDHIP_SOLVER_CG &&
                      s.preconditioner == CFDHIP_PREC_JACOBI;
   if (s.precision == CFDHIP_MIXED)

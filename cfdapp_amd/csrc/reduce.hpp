@@ -33,21 +33,43 @@ __device__ inline void blockPartials(double (&val)[W], double* partial) {
   }
 }
 
-// Sums W partial streams of nParts blocks in a fixed order (one block of 256).
+// The finalize kernel is one block: it is latency-bound, so it uses the
+// widest block and issues four independent loads per stream before adding.
+constexpr int kFinThreads = 1024;
+
+// Sums W partial streams of nParts blocks in a fixed order (one block of
+// kFinThreads).
 template <int W>
-__device__ inline void sumPartials(const double* partial, int64_t nParts, double (&tot)[W]) {
-  __shared__ double lds[W][kRedThreads / 64];
+__device__ inline void sumPartials(const double* __restrict__ partial, int64_t nParts,
+                                   double (&tot)[W]) {
+  __shared__ double lds[W][kFinThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double x[W];
+  for (int w = 0; w < W; ++w) x[w] = 0.0;
+  int64_t b = threadIdx.x;
+  for (; b + 3 * kFinThreads < nParts; b += 4 * kFinThreads) {
+    double v[4][W];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int w = 0; w < W; ++w) v[k][w] = partial[(b + k * kFinThreads) * W + w];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int w = 0; w < W; ++w) x[w] += v[k][w];
+  }
+  for (; b < nParts; b += kFinThreads)
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] += partial[b * W + w];
   for (int w = 0; w < W; ++w) {
-    double x = 0.0;
-    for (int64_t b = threadIdx.x; b < nParts; b += blockDim.x) x += partial[b * W + w];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) lds[w][wid] = x;
+    double y = x[w];
+    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+    if (lane == 0) lds[w][wid] = y;
   }
   __syncthreads();
   for (int w = 0; w < W; ++w) {
     double t = 0.0;
-    for (int i = 0; i < kRedThreads / 64; ++i) t += lds[w][i];
+    for (int i = 0; i < kFinThreads / 64; ++i) t += lds[w][i];
     tot[w] = t;
   }
 }
@@ -66,7 +88,10 @@ enum FinalStep : int {
   kStepBcgT,            // (t,t), (t,s), sum|.| -> omega           (3 sums)
   kStepBcgX,            // |r| -> convergence                      (1 sum)
   kStepBcgRestart,      // |r| after the true-residual restart     (1 sum)
-  kStepNorm             // sqrt of the sum into out                (1 sum)
+  kStepNorm,            // sqrt of the sum into out                (1 sum)
+  kStepFcgAlpha,        // (p, q), (p, r) -> alpha                 (2 sums)
+  kStepFcgUpdate,       // |r| -> convergence, stagnation          (1 sum)
+  kStepFcgBeta          // (z, q) -> beta                          (1 sum)
 };
 
 struct FinalArgs {

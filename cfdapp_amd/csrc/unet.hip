@@ -1,0 +1,667 @@
+// The learned preconditioner's network on the device: weight packing, the
+// single-channel first and last layers (VALU), forward and backward passes
+// over the MFMA convolutions, the A-norm training loss and Adam.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "reduce.hpp"
+#include "unet.hpp"
+#include "unet_net.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+__device__ inline float bf2f(bf16 h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+__device__ inline bf16 f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u)
+    return static_cast<bf16>((u >> 16) | ((u & 0xffffu) != 0 ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<bf16>(u >> 16);
+}
+__device__ inline float roundBf(float f) { return bf2f(f2bf(f)); }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+inline dim3 grid1(int64_t n, int64_t cap = 8192) {
+  int64_t b = (n + 255) / 256;
+  if (b > cap) b = cap;
+  return dim3(static_cast<unsigned>(b < 1 ? 1 : b));
+}
+
+// Forward fragments: element ((m NB + nb) 64 + lane) 8 + e holds
+// W[k = 32 m + 8 (lane / 16) + e][co = 16 nb + lane % 16], k = tap cin + ci.
+__global__ void packFwdKernel(const float* __restrict__ master, bf16* __restrict__ packed, int cin,
+                              int cout, int64_t total) {
+  const int nbt = cout / 16;
+  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int e = static_cast<int>(idx & 7);
+    const int lane = static_cast<int>((idx >> 3) & 63);
+    const int64_t blk = idx >> 9;
+    const int nb = static_cast<int>(blk % nbt);
+    const int64_t m = blk / nbt;
+    const int64_t k = 32 * m + 8 * (lane >> 4) + e;
+    const int co = 16 * nb + (lane & 15);
+    packed[idx] = k < 9LL * cin ? f2bf(master[k * cout + co]) : bf16(0);
+  }
+}
+
+// Transposed-conv fragments: cin' = cout, cout' = cin, W'[tap'][c'][o'] =
+// W[8 - tap'][o'][c'] (spatial flip, channel transpose).
+__global__ void packBwdKernel(const float* __restrict__ master, bf16* __restrict__ packed, int cin,
+                              int cout, int64_t total) {
+  const int cinT = cout, coutT = cin;
+  const int nbt = coutT / 16;
+  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int e = static_cast<int>(idx & 7);
+    const int lane = static_cast<int>((idx >> 3) & 63);
+    const int64_t blk = idx >> 9;
+    const int nb = static_cast<int>(blk % nbt);
+    const int64_t m = blk / nbt;
+    const int64_t k = 32 * m + 8 * (lane >> 4) + e;
+    const int o = 16 * nb + (lane & 15);  // forward input channel
+    bf16 v = 0;
+    if (k < 9LL * cinT) {
+      const int tap = static_cast<int>(k / cinT);
+      const int c = static_cast<int>(k - static_cast<int64_t>(tap) * cinT);  // forward output channel
+      v = f2bf(master[(static_cast<int64_t>(8 - tap) * cin + o) * cout + c]);
+    }
+    packed[idx] = v;
+  }
+}
+
+// e0a = conv(x, 1 -> 16). x(y, x) = src[y srcW + x] * scale (inside srcH x srcW,
+// zero elsewhere); one pixel per thread.
+__global__ __launch_bounds__(256) void firstConvKernel(const float* __restrict__ src,
+                                                       const float* __restrict__ scale, int srcH,
+                                                       int srcW, int64_t srcBatch,
+                                                       const float* __restrict__ w,
+                                                       bf16* __restrict__ out, int n, int h,
+                                                       int wd) {
+  __shared__ float ws[144];
+  if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
+  __syncthreads();
+  const int64_t total = static_cast<int64_t>(n) * h * wd;
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(p % wd);
+    const int64_t t = p / wd;
+    const int y = static_cast<int>(t % h);
+    const int b = static_cast<int>(t / h);
+    float acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < 3; ++kx) {
+        const int yy = y + ky - 1, xx = x + kx - 1;
+        float v = 0.f;
+        if (yy >= 0 && yy < srcH && xx >= 0 && xx < srcW) {
+          const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
+          v = src[si];
+          if (scale != nullptr) v *= scale[si];
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] += v * ws[(ky * 3 + kx) * 16 + c];
+      }
+    u32x4 lo, hi;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lo[k] = static_cast<uint32_t>(f2bf(acc[2 * k])) | (static_cast<uint32_t>(f2bf(acc[2 * k + 1])) << 16);
+      hi[k] = static_cast<uint32_t>(f2bf(acc[8 + 2 * k])) |
+              (static_cast<uint32_t>(f2bf(acc[8 + 2 * k + 1])) << 16);
+    }
+    u32x4* o = reinterpret_cast<u32x4*>(out + p * 16);
+    o[0] = lo;
+    o[1] = hi;
+  }
+}
+
+// y = conv(d0, 16 -> 1) at pixel (b, y, x).
+__device__ inline float lastConvAt(const bf16* __restrict__ d0, const float* ws, int b, int y, int x,
+                                   int h, int wd) {
+  float acc = 0.f;
+#pragma unroll
+  for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx) {
+      const int yy = y + ky - 1, xx = x + kx - 1;
+      if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+      const u32x4* src =
+          reinterpret_cast<const u32x4*>(d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
+      const u32x4 v0 = src[0], v1 = src[1];
+      const float* wt = ws + (ky * 3 + kx) * 16;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        acc += __uint_as_float(v0[k] << 16) * wt[2 * k];
+        acc += __uint_as_float(v0[k] & 0xffff0000u) * wt[2 * k + 1];
+        acc += __uint_as_float(v1[k] << 16) * wt[8 + 2 * k];
+        acc += __uint_as_float(v1[k] & 0xffff0000u) * wt[8 + 2 * k + 1];
+      }
+    }
+  return acc;
+}
+
+// Training output: y (FP32, n x h x w).
+__global__ __launch_bounds__(256) void lastConvKernel(const bf16* __restrict__ d0,
+                                                      const float* __restrict__ w,
+                                                      float* __restrict__ y, int n, int h, int wd) {
+  __shared__ float ws[144];
+  if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
+  __syncthreads();
+  const int64_t total = static_cast<int64_t>(n) * h * wd;
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(p % wd);
+    const int64_t t = p / wd;
+    y[p] = lastConvAt(d0, ws, static_cast<int>(t / h), static_cast<int>(t % h), x, h, wd);
+  }
+}
+
+// Preconditioner output on the solver grid (ny x nx inside the network's
+// h x w): z = scale * y, z[pin] = r[pin] scale[pin]^2, and block partials of
+// (z, q).
+__global__ __launch_bounds__(256) void lastConvApplyKernel(const bf16* __restrict__ d0,
+                                                           const float* __restrict__ w,
+                                                           const float* __restrict__ scale,
+                                                           const float* __restrict__ r,
+                                                           const float* __restrict__ q,
+                                                           float* __restrict__ z, int ny, int nx,
+                                                           int64_t pin, int h, int wd,
+                                                           double* partial) {
+  __shared__ float ws[144];
+  if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
+  __syncthreads();
+  const int64_t total = static_cast<int64_t>(ny) * nx;
+  double acc[1] = {0.0};
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < total;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
+    float v = lastConvAt(d0, ws, 0, y, x, h, wd) * scale[c];
+    if (c == pin) v = r[c] * scale[c] * scale[c];
+    z[c] = v;
+    if (q != nullptr) acc[0] += static_cast<double>(v) * static_cast<double>(q[c]);
+  }
+  if (partial != nullptr) blockPartials<1>(acc, partial);
+}
+
+// gd0 = conv^T(gy, 1 -> 16): gd0[p][c] = sum_tap gy[p - off(tap)] W[tap][c].
+__global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict__ gy,
+                                                         const float* __restrict__ w,
+                                                         bf16* __restrict__ gd0, int n, int h,
+                                                         int wd) {
+  __shared__ float ws[144];
+  if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
+  __syncthreads();
+  const int64_t total = static_cast<int64_t>(n) * h * wd;
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(p % wd);
+    const int64_t t = p / wd;
+    const int y = static_cast<int>(t % h);
+    const int b = static_cast<int>(t / h);
+    float acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < 3; ++kx) {
+        const int yy = y - ky + 1, xx = x - kx + 1;
+        if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+        const float g = gy[(static_cast<int64_t>(b) * h + yy) * wd + xx];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] += g * ws[(ky * 3 + kx) * 16 + c];
+      }
+    u32x4 lo, hi;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lo[k] = static_cast<uint32_t>(f2bf(acc[2 * k])) | (static_cast<uint32_t>(f2bf(acc[2 * k + 1])) << 16);
+      hi[k] = static_cast<uint32_t>(f2bf(acc[8 + 2 * k])) |
+              (static_cast<uint32_t>(f2bf(acc[8 + 2 * k + 1])) << 16);
+    }
+    u32x4* o = reinterpret_cast<u32x4*>(gd0 + p * 16);
+    o[0] = lo;
+    o[1] = hi;
+  }
+}
+
+// Weight gradient of a single-channel layer, 144 sums per block:
+//   first (mode 0): dW[tap][c] = sum_p x[p + off(tap)] g16[p][c]
+//   last  (mode 1): dW[tap][c] = sum_p a16[p + off(tap)][c] gy[p]
+// x / gy are FP32 scalar images, g16 / a16 BF16 16-channel images.
+__global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ s1,
+                                                      const bf16* __restrict__ m16, int mode,
+                                                      int n, int h, int wd, float* partial) {
+  float acc[144];
+#pragma unroll
+  for (int k = 0; k < 144; ++k) acc[k] = 0.f;
+  const int64_t total = static_cast<int64_t>(n) * h * wd;
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(p % wd);
+    const int64_t t = p / wd;
+    const int y = static_cast<int>(t % h);
+    const int b = static_cast<int>(t / h);
+    if (mode == 0) {
+      const u32x4* gp = reinterpret_cast<const u32x4*>(m16 + p * 16);
+      const u32x4 g0 = gp[0], g1 = gp[1];
+      float gv[16];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        gv[2 * k] = __uint_as_float(g0[k] << 16);
+        gv[2 * k + 1] = __uint_as_float(g0[k] & 0xffff0000u);
+        gv[8 + 2 * k] = __uint_as_float(g1[k] << 16);
+        gv[8 + 2 * k + 1] = __uint_as_float(g1[k] & 0xffff0000u);
+      }
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+        const float xv = s1[(static_cast<int64_t>(b) * h + yy) * wd + xx];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += xv * gv[c];
+      }
+    } else {
+      const float g = s1[p];
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+        const u32x4* ap = reinterpret_cast<const u32x4*>(
+            m16 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
+        const u32x4 a0 = ap[0], a1 = ap[1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          acc[tap * 16 + 2 * k] += __uint_as_float(a0[k] << 16) * g;
+          acc[tap * 16 + 2 * k + 1] += __uint_as_float(a0[k] & 0xffff0000u) * g;
+          acc[tap * 16 + 8 + 2 * k] += __uint_as_float(a1[k] << 16) * g;
+          acc[tap * 16 + 8 + 2 * k + 1] += __uint_as_float(a1[k] & 0xffff0000u) * g;
+        }
+      }
+    }
+  }
+  __shared__ float lds[4][144];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 144; ++k) {
+    float v = acc[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) lds[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 144)
+    partial[static_cast<int64_t>(blockIdx.x) * 144 + threadIdx.x] =
+        lds[0][threadIdx.x] + lds[1][threadIdx.x] + lds[2][threadIdx.x] + lds[3][threadIdx.x];
+}
+
+__global__ void singleDwReduceKernel(const float* __restrict__ partial, int parts,
+                                     float* __restrict__ grad) {
+  const int k = threadIdx.x;
+  if (k >= 144) return;
+  float s = 0.f;
+  for (int b = 0; b < parts; ++b) s += partial[static_cast<int64_t>(b) * 144 + k];
+  grad[k] = s;
+}
+
+// dW[tap][ci][co] (master layout) = sum over splits, fixed order.
+__global__ void dwReduceKernel(const float* __restrict__ partial, int splits, int cin, int cout,
+                               float* __restrict__ grad) {
+  const int ncb = cin / 16, nob = cout / 16, npair = 9 * ncb;
+  const int64_t total = 9LL * cin * cout;
+  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int co = static_cast<int>(idx % cout);
+    const int64_t t = idx / cout;
+    const int ci = static_cast<int>(t % cin);
+    const int tap = static_cast<int>(t / cin);
+    const int pair = tap * ncb + ci / 16, ob = co / 16;
+    const int e = (ci & 15) * 16 + (co & 15);
+    float s = 0.f;
+    for (int sp = 0; sp < splits; ++sp)
+      s += partial[((static_cast<int64_t>(sp) * npair + pair) * nob + ob) * 256 + e];
+    grad[idx] = s;
+  }
+}
+
+// A-norm loss of the scaled system: d = y - e, A d with unit diagonal and
+// couplings cE (to x + 1) and cN (to y + 1) of the sample's system;
+// L = (1 / n) sum_s (d, A d)_s / den_s, gy = 2 A d / (den_s n).
+__global__ __launch_bounds__(256) void anormLossKernel(
+    const float* __restrict__ y, const float* __restrict__ e, const float* __restrict__ cE,
+    const float* __restrict__ cN, const int64_t* __restrict__ sysOff,
+    const float* __restrict__ den, int n, int h, int wd, float* __restrict__ gy, double* partial) {
+  const int64_t hw = static_cast<int64_t>(h) * wd, total = n * hw;
+  double acc[1] = {0.0};
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int s = static_cast<int>(p / hw);
+    const int64_t c = p - s * hw;
+    const int x = static_cast<int>(c % wd), yy = static_cast<int>(c / wd);
+    const float* ce = cE + sysOff[s];
+    const float* cn = cN + sysOff[s];
+    const float d = y[p] - e[p];
+    float ad = d;
+    if (x + 1 < wd) ad -= ce[c] * (y[p + 1] - e[p + 1]);
+    if (x > 0) ad -= ce[c - 1] * (y[p - 1] - e[p - 1]);
+    if (yy + 1 < h) ad -= cn[c] * (y[p + wd] - e[p + wd]);
+    if (yy > 0) ad -= cn[c - wd] * (y[p - wd] - e[p - wd]);
+    const float inv = 1.f / (den[s] * static_cast<float>(n));
+    gy[p] = 2.f * ad * inv;
+    acc[0] += static_cast<double>(d) * ad * inv;
+  }
+  blockPartials<1>(acc, partial);
+}
+
+__global__ void sumDoubleKernel(const double* __restrict__ partial, int parts, double* out) {
+  double v[1];
+  sumPartials<1>(partial, parts, v);
+  if (threadIdx.x == 0) out[0] = v[0];
+}
+
+__global__ void adamKernel(int64_t count, float* __restrict__ w, float* __restrict__ m,
+                           float* __restrict__ v, const float* __restrict__ g, float lr, float b1,
+                           float b2, float eps, float bc1, float bc2) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+Unet::Unet(hipStream_t s) : stream(s) {
+  master.ensure(kUnetParams);
+  grad.ensure(kUnetParams);
+  int64_t fo = 0, bo = 0;
+  for (int l = 0; l < kNumLayers; ++l) {
+    fwdOff[l] = fo;
+    bwdOff[l] = bo;
+    if (l != kE0a && l != kOut) {
+      fo += packedElems(kLayers[l].cin, kLayers[l].cout);
+      bo += packedElems(kLayers[l].cout, kLayers[l].cin);
+    }
+  }
+  fwdPack.ensure(fo);
+  bwdPack.ensure(bo);
+}
+
+void Unet::setWeights(const float* host) {
+  CFDHIP_CHECK(hipMemcpyAsync(master.get(), host, kUnetParams * sizeof(float),
+                              hipMemcpyHostToDevice, stream));
+  repack();
+  loaded = true;
+}
+
+void Unet::getWeights(float* host) {
+  CFDHIP_CHECK(hipMemcpyAsync(host, master.get(), kUnetParams * sizeof(float),
+                              hipMemcpyDeviceToHost, stream));
+  CFDHIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Unet::repack() {
+  for (int l = 0; l < kNumLayers; ++l) {
+    if (l == kE0a || l == kOut) continue;
+    const int ci = kLayers[l].cin, co = kLayers[l].cout;
+    const int64_t nf = packedElems(ci, co), nb = packedElems(co, ci);
+    hipLaunchKernelGGL(packFwdKernel, grid1(nf), dim3(256), 0, stream, master.get() + layerOffset(l),
+                       fwdPack.get() + fwdOff[l], ci, co, nf);
+    hipLaunchKernelGGL(packBwdKernel, grid1(nb), dim3(256), 0, stream, master.get() + layerOffset(l),
+                       bwdPack.get() + bwdOff[l], ci, co, nb);
+  }
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+void Unet::ensureShape(int n, int h, int w, bool training) {
+  if (h % 8 != 0 || w % 8 != 0 || h < 8 || w < 8 || n < 1)
+    throw Error(CFDHIP_ERR_INVALID, "network grid must be a positive multiple of 8");
+  if (n == shapeN && h == shapeH && w == shapeW && (training <= shapeTraining)) return;
+  const int64_t p0 = static_cast<int64_t>(n) * h * w, p1 = p0 / 4, p2 = p0 / 16, p3 = p0 / 64;
+  // forward tensors (elements), then gradient tensors
+  const int64_t sizes[kNumTensors] = {
+      p0 * 16, p0 * 16, p1 * 16,            // e0a e0 p0
+      p1 * 32, p1 * 32, p2 * 32,            // e1a e1 p1
+      p2 * 64, p2 * 64, p3 * 64,            // e2a e2 p2
+      p3 * 64, p3 * 64,                     // b3a b3
+      p2 * 64, p2 * 64, p1 * 32, p1 * 32,   // d2a d2 d1a d1
+      p0 * 16, p0 * 16,                     // d0a d0
+      // gradients
+      p0 * 16, p0 * 16, p1 * 32, p0 * 16,   // gd0 gd0a gd1 ge0s
+      p1 * 32, p2 * 64, p1 * 32,            // gd1a gd2 ge1s
+      p2 * 64, p3 * 64, p2 * 64,            // gd2a gb3 ge2s
+      p3 * 64, p3 * 64,                     // gb3a gp2
+      p2 * 64, p2 * 32,                     // ge2a gp1
+      p1 * 32, p1 * 16,                     // ge1a gp0
+      p0 * 16};                             // ge0a
+  const int count = training ? kNumTensors : kNumForward;
+  int64_t off = 0;
+  for (int t = 0; t < kNumTensors; ++t) {
+    tensorOff[t] = off;
+    if (t < count) off += (sizes[t] + 127) / 128 * 128;
+  }
+  act.ensure(off);
+  y.ensure(p0);
+  gy.ensure(p0);
+  shapeN = n;
+  shapeH = h;
+  shapeW = w;
+  shapeTraining = training;
+}
+
+int64_t Unet::tensorElems(int t) const {
+  require(t >= 0 && t < kNumTensors && (t < kNumForward || shapeTraining),
+          "tensor id out of range for the current shape");
+  const int64_t p0 = static_cast<int64_t>(shapeN) * shapeH * shapeW;
+  static const int kLevel[kNumTensors] = {0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 2, 2, 1, 1, 0, 0,
+                                          0, 0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 3, 2, 2, 1, 1, 0};
+  static const int kChan[kNumTensors] = {16, 16, 16, 32, 32, 32, 64, 64, 64, 64, 64, 64, 64, 32,
+                                         32, 16, 16, 16, 16, 32, 16, 32, 64, 32, 64, 64, 64, 64,
+                                         64, 64, 32, 32, 16, 16};
+  return (p0 >> (2 * kLevel[t])) * kChan[t];
+}
+
+void Unet::readTensor(int t, float* host, int64_t count) {
+  const int64_t m = tensorElems(t);
+  require(count == m, "tensor size mismatch");
+  std::vector<bf16> tmp(static_cast<size_t>(m));
+  CFDHIP_CHECK(hipMemcpyAsync(tmp.data(), act.get() + tensorOff[t], m * sizeof(bf16),
+                              hipMemcpyDeviceToHost, stream));
+  CFDHIP_CHECK(hipStreamSynchronize(stream));
+  for (int64_t i = 0; i < m; ++i) {
+    const uint32_t u = static_cast<uint32_t>(tmp[i]) << 16;
+    std::memcpy(host + i, &u, sizeof(float));
+  }
+}
+
+void Unet::conv(int layer, bool transposed, int gather, int epi, int split, const bf16* a,
+                const bf16* b, bf16* out, bf16* aux, int n, int h, int w) {
+  ConvArgs args{};
+  args.a = a;
+  args.b = b;
+  args.out = out;
+  args.aux = aux;
+  args.n = n;
+  args.h = h;
+  args.w_ = w;
+  args.tilesX = (w + 31) / 32;
+  args.tilesY = (h + 7) / 8;
+  const int ci = transposed ? kLayers[layer].cout : kLayers[layer].cin;
+  const int co = transposed ? kLayers[layer].cin : kLayers[layer].cout;
+  args.w = transposed ? static_cast<const void*>(bwdPack.get() + bwdOff[layer])
+                      : static_cast<const void*>(fwdPack.get() + fwdOff[layer]);
+  args.nbTotal = co / 16;
+  int ca = ci;
+  if (gather == kGatherConcatUp) ca = ci - (layer == kD2a ? 64 : layer == kD1a ? 32 : 16);
+  launchConv(stream, ci, ca, co, gather, epi, split, args);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+void Unet::convDw(int layer, int gx, int gg, const bf16* xa, const bf16* xb, const bf16* ga,
+                  const bf16* gb, int n, int h, int w) {
+  DwArgs a{};
+  a.xa = xa;
+  a.xb = xb;
+  a.ga = ga;
+  a.gb = gb;
+  a.n = n;
+  a.h = h;
+  a.w_ = w;
+  a.tilesX = (w + 31) / 32;
+  a.tilesY = (h + 7) / 8;
+  a.tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * n;
+  const int ci = kLayers[layer].cin, co = kLayers[layer].cout;
+  const int npair = 9 * (ci / 16);
+  const int groups = (npair + 7) / 8;
+  int splits = static_cast<int>(std::min<int64_t>(a.tiles, std::max(1, 2048 / groups)));
+  const int64_t need = static_cast<int64_t>(splits) * npair * (co / 16) * 256;
+  dwPartial.ensure(need);
+  a.partial = dwPartial.get();
+  int ca = ci;
+  if (gx == kGatherConcatUp) ca = ci - (layer == kD2a ? 64 : layer == kD1a ? 32 : 16);
+  launchConvDw(stream, ci, ca, co, gx, gg, a, splits);
+  hipLaunchKernelGGL(dwReduceKernel, grid1(9LL * ci * co), dim3(256), 0, stream, dwPartial.get(),
+                     splits, ci, co, grad.get() + layerOffset(layer));
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+void Unet::forward(const float* src, const float* scale, int srcH, int srcW, int64_t srcBatch,
+                   int n, int h, int w) {
+  if (!loaded) throw Error(CFDHIP_ERR_INVALID, "network weights are not loaded");
+  const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+  const int64_t p0 = static_cast<int64_t>(n) * h * w;
+  hipLaunchKernelGGL(firstConvKernel, grid1(p0), dim3(256), 0, stream, src, scale, srcH, srcW,
+                     srcBatch, master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+  CFDHIP_CHECK(hipGetLastError());
+  conv(kE0b, false, kGatherPlain, kEpiStorePool, 0, T(kTe0a), nullptr, T(kTe0), T(kTp0), n, h, w);
+  conv(kE1a, false, kGatherPlain, kEpiStore, 0, T(kTp0), nullptr, T(kTe1a), nullptr, n, h1, w1);
+  conv(kE1b, false, kGatherPlain, kEpiStorePool, 0, T(kTe1a), nullptr, T(kTe1), T(kTp1), n, h1, w1);
+  conv(kE2a, false, kGatherPlain, kEpiStore, 0, T(kTp1), nullptr, T(kTe2a), nullptr, n, h2, w2);
+  conv(kE2b, false, kGatherPlain, kEpiStorePool, 0, T(kTe2a), nullptr, T(kTe2), T(kTp2), n, h2, w2);
+  conv(kB3a, false, kGatherPlain, kEpiStore, 0, T(kTp2), nullptr, T(kTb3a), nullptr, n, h3, w3);
+  conv(kB3b, false, kGatherPlain, kEpiStore, 0, T(kTb3a), nullptr, T(kTb3), nullptr, n, h3, w3);
+  conv(kD2a, false, kGatherConcatUp, kEpiStore, 0, T(kTb3), T(kTe2), T(kTd2a), nullptr, n, h2, w2);
+  conv(kD2b, false, kGatherPlain, kEpiStore, 0, T(kTd2a), nullptr, T(kTd2), nullptr, n, h2, w2);
+  conv(kD1a, false, kGatherConcatUp, kEpiStore, 0, T(kTd2), T(kTe1), T(kTd1a), nullptr, n, h1, w1);
+  conv(kD1b, false, kGatherPlain, kEpiStore, 0, T(kTd1a), nullptr, T(kTd1), nullptr, n, h1, w1);
+  conv(kD0a, false, kGatherConcatUp, kEpiStore, 0, T(kTd1), T(kTe0), T(kTd0a), nullptr, n, h, w);
+  conv(kD0b, false, kGatherPlain, kEpiStore, 0, T(kTd0a), nullptr, T(kTd0), nullptr, n, h, w);
+}
+
+void Unet::outputTrain(int n, int h, int w) {
+  const int64_t p0 = static_cast<int64_t>(n) * h * w;
+  hipLaunchKernelGGL(lastConvKernel, grid1(p0), dim3(256), 0, stream, T(kTd0),
+                     master.get() + layerOffset(kOut), y.get(), n, h, w);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+void Unet::outputApply(const float* scale, const float* r, const float* q, float* z, int ny,
+                       int nx, int64_t pin, int h, int w, double* partial, int parts) {
+  hipLaunchKernelGGL(lastConvApplyKernel, dim3(static_cast<unsigned>(parts)), dim3(256), 0, stream,
+                     T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin, h, w,
+                     partial);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
+  const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+  const int64_t p0 = static_cast<int64_t>(n) * h * w;
+  // output layer
+  hipLaunchKernelGGL(lastConvBwdKernel, grid1(p0), dim3(256), 0, stream, gyIn,
+                     master.get() + layerOffset(kOut), T(kTgd0), n, h, w);
+  singleDw(1, gyIn, T(kTd0), n, h, w, grad.get() + layerOffset(kOut));
+  // decoder, level 0
+  conv(kD0b, true, kGatherPlain, kEpiStore, 0, T(kTgd0), nullptr, T(kTgd0a), nullptr, n, h, w);
+  convDw(kD0b, kGatherPlain, kGatherPlain, T(kTd0a), nullptr, T(kTgd0), nullptr, n, h, w);
+  conv(kD0a, true, kGatherPlain, kEpiSplitUp, 32, T(kTgd0a), nullptr, T(kTge0s), T(kTgd1), n, h, w);
+  convDw(kD0a, kGatherConcatUp, kGatherPlain, T(kTd1), T(kTe0), T(kTgd0a), nullptr, n, h, w);
+  // level 1
+  conv(kD1b, true, kGatherPlain, kEpiStore, 0, T(kTgd1), nullptr, T(kTgd1a), nullptr, n, h1, w1);
+  convDw(kD1b, kGatherPlain, kGatherPlain, T(kTd1a), nullptr, T(kTgd1), nullptr, n, h1, w1);
+  conv(kD1a, true, kGatherPlain, kEpiSplitUp, 64, T(kTgd1a), nullptr, T(kTge1s), T(kTgd2), n, h1, w1);
+  convDw(kD1a, kGatherConcatUp, kGatherPlain, T(kTd2), T(kTe1), T(kTgd1a), nullptr, n, h1, w1);
+  // level 2
+  conv(kD2b, true, kGatherPlain, kEpiStore, 0, T(kTgd2), nullptr, T(kTgd2a), nullptr, n, h2, w2);
+  convDw(kD2b, kGatherPlain, kGatherPlain, T(kTd2a), nullptr, T(kTgd2), nullptr, n, h2, w2);
+  conv(kD2a, true, kGatherPlain, kEpiSplitUp, 64, T(kTgd2a), nullptr, T(kTge2s), T(kTgb3), n, h2, w2);
+  convDw(kD2a, kGatherConcatUp, kGatherPlain, T(kTb3), T(kTe2), T(kTgd2a), nullptr, n, h2, w2);
+  // bottom
+  conv(kB3b, true, kGatherPlain, kEpiStore, 0, T(kTgb3), nullptr, T(kTgb3a), nullptr, n, h3, w3);
+  convDw(kB3b, kGatherPlain, kGatherPlain, T(kTb3a), nullptr, T(kTgb3), nullptr, n, h3, w3);
+  conv(kB3a, true, kGatherPlain, kEpiStore, 0, T(kTgb3a), nullptr, T(kTgp2), nullptr, n, h3, w3);
+  convDw(kB3a, kGatherPlain, kGatherPlain, T(kTp2), nullptr, T(kTgb3a), nullptr, n, h3, w3);
+  // encoder, level 2: d e2 = skip + pool gradient
+  conv(kE2b, true, kGatherSumUp, kEpiStore, 0, T(kTge2s), T(kTgp2), T(kTge2a), nullptr, n, h2, w2);
+  convDw(kE2b, kGatherPlain, kGatherSumUp, T(kTe2a), nullptr, T(kTge2s), T(kTgp2), n, h2, w2);
+  conv(kE2a, true, kGatherPlain, kEpiStore, 0, T(kTge2a), nullptr, T(kTgp1), nullptr, n, h2, w2);
+  convDw(kE2a, kGatherPlain, kGatherPlain, T(kTp1), nullptr, T(kTge2a), nullptr, n, h2, w2);
+  // level 1
+  conv(kE1b, true, kGatherSumUp, kEpiStore, 0, T(kTge1s), T(kTgp1), T(kTge1a), nullptr, n, h1, w1);
+  convDw(kE1b, kGatherPlain, kGatherSumUp, T(kTe1a), nullptr, T(kTge1s), T(kTgp1), n, h1, w1);
+  conv(kE1a, true, kGatherPlain, kEpiStore, 0, T(kTge1a), nullptr, T(kTgp0), nullptr, n, h1, w1);
+  convDw(kE1a, kGatherPlain, kGatherPlain, T(kTp0), nullptr, T(kTge1a), nullptr, n, h1, w1);
+  // level 0
+  conv(kE0b, true, kGatherSumUp, kEpiStore, 0, T(kTge0s), T(kTgp0), T(kTge0a), nullptr, n, h, w);
+  convDw(kE0b, kGatherPlain, kGatherSumUp, T(kTe0a), nullptr, T(kTge0s), T(kTgp0), n, h, w);
+  singleDw(0, x, T(kTge0a), n, h, w, grad.get() + layerOffset(kE0a));
+}
+
+void Unet::singleDw(int mode, const float* s1, const bf16* m16, int n, int h, int w, float* out) {
+  const int64_t p0 = static_cast<int64_t>(n) * h * w;
+  const dim3 g = grid1(p0, 1024);
+  dwPartial.ensure(static_cast<int64_t>(g.x) * 144);
+  hipLaunchKernelGGL(singleDwKernel, g, dim3(256), 0, stream, s1, m16, mode, n, h, w,
+                     dwPartial.get());
+  hipLaunchKernelGGL(singleDwReduceKernel, dim3(1), dim3(256), 0, stream, dwPartial.get(),
+                     static_cast<int>(g.x), out);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+double Unet::loss(const float* e, const float* cE, const float* cN, const int64_t* sysOffDev,
+                  const float* denDev, int n, int h, int w) {
+  const int64_t p0 = static_cast<int64_t>(n) * h * w;
+  const dim3 g = grid1(p0, 2048);
+  lossPartial.ensure(static_cast<int64_t>(g.x) + 8);
+  hipLaunchKernelGGL(anormLossKernel, g, dim3(256), 0, stream, y.get(), e, cE, cN, sysOffDev,
+                     denDev, n, h, w, gy.get(), lossPartial.get());
+  double* out = lossPartial.get() + g.x;
+  hipLaunchKernelGGL(sumDoubleKernel, dim3(1), dim3(kFinThreads), 0, stream, lossPartial.get(),
+                     static_cast<int>(g.x), out);
+  CFDHIP_CHECK(hipGetLastError());
+  double v = 0.0;
+  CFDHIP_CHECK(hipMemcpyAsync(&v, out, sizeof(double), hipMemcpyDeviceToHost, stream));
+  CFDHIP_CHECK(hipStreamSynchronize(stream));
+  return v;
+}
+
+void Unet::adam(float lr, float b1, float b2, float eps) {
+  if (adamM.count < kUnetParams) {
+    adamM.ensure(kUnetParams);
+    adamV.ensure(kUnetParams);
+    CFDHIP_CHECK(hipMemsetAsync(adamM.get(), 0, kUnetParams * sizeof(float), stream));
+    CFDHIP_CHECK(hipMemsetAsync(adamV.get(), 0, kUnetParams * sizeof(float), stream));
+    adamStep = 0;
+  }
+  ++adamStep;
+  const float bc1 = 1.f - std::pow(b1, static_cast<float>(adamStep));
+  const float bc2 = 1.f - std::pow(b2, static_cast<float>(adamStep));
+  hipLaunchKernelGGL(adamKernel, grid1(kUnetParams), dim3(256), 0, stream,
+                     static_cast<int64_t>(kUnetParams), master.get(), adamM.get(), adamV.get(),
+                     grad.get(), lr, b1, b2, eps, bc1, bc2);
+  CFDHIP_CHECK(hipGetLastError());
+  repack();
+}
+
+}  // namespace cfdhip

@@ -1,0 +1,99 @@
+// Learned p' preconditioner: a linear 4-level convolutional U-Net.
+//
+// All convolutions are 3x3 with zero padding and no bias; there are no
+// activations, so the network is a fixed linear map r -> z and flexible CG
+// sees a fixed preconditioner up to BF16 rounding.
+//
+//   e0a = conv(x, 1->16)            e0  = conv(e0a, 16->16)     (H x W)
+//   e1a = conv(pool e0, 16->32)     e1  = conv(e1a, 32->32)     (H/2)
+//   e2a = conv(pool e1, 32->64)     e2  = conv(e2a, 64->64)     (H/4)
+//   b3a = conv(pool e2, 64->64)     b3  = conv(b3a, 64->64)     (H/8)
+//   d2a = conv([up b3, e2], 128->64) d2 = conv(d2a, 64->64)
+//   d1a = conv([up d2, e1], 96->32)  d1 = conv(d1a, 32->32)
+//   d0a = conv([up d1, e0], 48->16)  d0 = conv(d0a, 16->16)
+//   y   = conv(d0, 16->1)
+//
+// pool = 2x2 average, up = nearest x2, [a, b] = channel concatenation.
+// Activations: BF16, NHWC, batch-major. Master weights: FP32,
+// W[ky][kx][cin][cout] per layer, layers concatenated in the order above.
+// The MFMA layers run from BF16 copies packed in the operand order of
+// v_mfma_f32_16x16x32_bf16 (one 16-byte fragment per lane).
+#pragma once
+
+#include <cstdint>
+
+#include "context.hpp"
+
+namespace cfdhip {
+
+using bf16 = uint16_t;
+
+enum UnetLayer : int {
+  kE0a = 0, kE0b, kE1a, kE1b, kE2a, kE2b, kB3a, kB3b,
+  kD2a, kD2b, kD1a, kD1b, kD0a, kD0b, kOut, kNumLayers
+};
+
+struct LayerShape {
+  int cin, cout;
+};
+
+constexpr LayerShape kLayers[kNumLayers] = {
+    {1, 16},  {16, 16}, {16, 32}, {32, 32}, {32, 64}, {64, 64}, {64, 64}, {64, 64},
+    {128, 64}, {64, 64}, {96, 32}, {32, 32}, {48, 16}, {16, 16}, {16, 1}};
+
+constexpr int64_t layerWeights(int l) { return 9LL * kLayers[l].cin * kLayers[l].cout; }
+constexpr int64_t layerOffset(int l) {
+  int64_t o = 0;
+  for (int i = 0; i < l; ++i) o += layerWeights(i);
+  return o;
+}
+constexpr int64_t kUnetParams = layerOffset(kNumLayers);
+
+// Packed MFMA fragments: ceil(9 cin / 32) K-steps x (cout / 16) N-blocks x
+// 64 lanes x 8 BF16.
+constexpr int64_t packedSteps(int cin) { return (9LL * cin + 31) / 32; }
+constexpr int64_t packedElems(int cin, int cout) { return packedSteps(cin) * (cout / 16) * 512; }
+
+// Gathers applied while a tile of the conv input is staged in LDS.
+enum Gather : int {
+  kGatherPlain = 0,   // channels of a at (y, x)
+  kGatherConcatUp,    // [a at (y/2, x/2) (ca channels), b at (y, x)]
+  kGatherSumUp        // a at (y, x) + 0.25 b at (y/2, x/2)  (gradient of pool + skip)
+};
+// What the conv does with its FP32 accumulators.
+enum Epilogue : int {
+  kEpiStore = 0,   // BF16 to out
+  kEpiStorePool,   // BF16 to out, 2x2 average to aux (H/2 x W/2)
+  kEpiSplitUp      // channels < split: 2x2 sum to aux (coarse); the rest to out
+};
+
+struct ConvArgs {
+  const bf16* a;
+  const bf16* b;
+  const void* w;  // packed fragments of the layer
+  bf16* out;
+  bf16* aux;
+  int n, h, w_;   // batch, image size of the output (= input) grid
+  int tilesX, tilesY;
+  int nbTotal;    // N-blocks of the layer (cout / 16)
+};
+
+struct DwArgs {
+  const bf16* xa;
+  const bf16* xb;
+  const bf16* ga;
+  const bf16* gb;
+  float* partial;  // [split][pair][ob][16 x 16]
+  int n, h, w_;
+  int tilesX, tilesY;
+  int64_t tiles;
+};
+
+// One convolution (forward or transposed) on the device.
+void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, int split,
+                const ConvArgs& a);
+// Weight gradient of an MFMA layer: partial sums over pixel splits.
+int launchConvDw(hipStream_t s, int cin, int ca, int cout, int gx, int gg, const DwArgs& a,
+                 int splits);
+
+}  // namespace cfdhip

@@ -1,0 +1,351 @@
+// BF16 MFMA convolutions of the learned preconditioner (gfx950).
+//
+// Forward and transposed 3x3 convolutions are one implicit GEMM: M = pixels
+// of an 8 x 32 output tile, N = output channels, K = 9 taps x input
+// channels, on v_mfma_f32_16x16x32_bf16 with FP32 accumulators. The input
+// tile (with its one-pixel halo, gathered through upsampling, concatenation or
+// the pool-gradient sum) is staged once in LDS; each wave owns two tile rows
+// (four 16-pixel M-blocks, each a 2 x 8 patch ordered so that every lane's
+// four accumulator rows form one 2 x 2 pixel block, which makes the fused
+// 2x2 pool and the nearest-upsample gradient lane-local). The packed weight
+// fragments stream from L2 (16 bytes per lane per K-step).
+//
+// The weight gradient is a GEMM over pixels: D[ci][co] += X[p + tap][ci] G[p][co]
+// per (tap, 16-channel block), X and G tiles in LDS, partial sums per pixel
+// split written to memory and reduced in a fixed order.
+#include "unet.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf8v;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4v;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTH = 8, kTW = 32;
+constexpr int kInRows = kTH + 2, kInCols = kTW + 2;
+
+__device__ inline float bf2f(bf16 h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// Round to nearest even; NaN stays NaN.
+__device__ inline bf16 f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u)
+    return static_cast<bf16>((u >> 16) | ((u & 0xffffu) != 0 ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<bf16>(u >> 16);
+}
+
+__device__ inline uint32_t pack2(float lo, float hi) {
+  return static_cast<uint32_t>(f2bf(lo)) | (static_cast<uint32_t>(f2bf(hi)) << 16);
+}
+
+// 8 channels starting at c of the gathered input at pixel (y, x), inside the image.
+template <int C, int CA, int G>
+__device__ inline u32x4 gather8(const bf16* __restrict__ A, const bf16* __restrict__ B, int n,
+                                int h, int w, int y, int x, int c) {
+  if constexpr (G == kGatherPlain) {
+    return *reinterpret_cast<const u32x4*>(A + ((static_cast<int64_t>(n) * h + y) * w + x) * C + c);
+  } else if constexpr (G == kGatherConcatUp) {
+    if (c < CA) {
+      const int hh = h >> 1, hw = w >> 1;
+      return *reinterpret_cast<const u32x4*>(
+          A + ((static_cast<int64_t>(n) * hh + (y >> 1)) * hw + (x >> 1)) * CA + c);
+    }
+    return *reinterpret_cast<const u32x4*>(
+        B + ((static_cast<int64_t>(n) * h + y) * w + x) * (C - CA) + (c - CA));
+  } else {
+    const int hh = h >> 1, hw = w >> 1;
+    const u32x4 va =
+        *reinterpret_cast<const u32x4*>(A + ((static_cast<int64_t>(n) * h + y) * w + x) * C + c);
+    const u32x4 vb = *reinterpret_cast<const u32x4*>(
+        B + ((static_cast<int64_t>(n) * hh + (y >> 1)) * hw + (x >> 1)) * C + c);
+    u32x4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float lo = __uint_as_float(va[k] << 16) + 0.25f * __uint_as_float(vb[k] << 16);
+      const float hi =
+          __uint_as_float(va[k] & 0xffff0000u) + 0.25f * __uint_as_float(vb[k] & 0xffff0000u);
+      r[k] = pack2(lo, hi);
+    }
+    return r;
+  }
+}
+
+// Stages rows x cols pixels with origin (y0, x0) (zero outside the image) into
+// lds[(ty * cols + tx) * cs + c].
+template <int C, int CA, int G>
+__device__ inline void stageTile(bf16* lds, int cs, int rows, int cols, const bf16* A,
+                                 const bf16* B, int n, int h, int w, int y0, int x0) {
+  constexpr int C8 = C / 8;
+  const int items = rows * cols * C8;
+  for (int it = threadIdx.x; it < items; it += blockDim.x) {
+    const int c8 = it % C8;
+    const int px = it / C8;
+    const int ty = px / cols, tx = px - ty * cols;
+    const int y = y0 + ty, x = x0 + tx;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (y >= 0 && y < h && x >= 0 && x < w) v = gather8<C, CA, G>(A, B, n, h, w, y, x, c8 * 8);
+    *reinterpret_cast<u32x4*>(lds + (ty * cols + tx) * cs + c8 * 8) = v;
+  }
+}
+
+__device__ inline f4v mfma(u32x4 a, u32x4 b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a),
+                                                 __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+
+template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
+__global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
+  constexpr int CS = CIN + 8;
+  constexpr int IN_ELEMS = kInRows * kInCols * CS;
+  constexpr int SS = NBG * 16 + 8;
+  constexpr int OUT_ELEMS = kTH * kTW * SS;
+  constexpr int ELEMS = IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS;
+  constexpr int MS = (9 * CIN + 31) / 32;
+  __shared__ __attribute__((aligned(16))) bf16 lds[ELEMS];
+
+  int tile = blockIdx.x;
+  const int tx = tile % a.tilesX;
+  tile /= a.tilesX;
+  const int ty = tile % a.tilesY;
+  const int n = tile / a.tilesY;
+  const int y0 = ty * kTH, x0 = tx * kTW;
+  const int nb0 = blockIdx.y * NBG;
+
+  stageTile<CIN, CA, G>(lds, CS, kInRows, kInCols, a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1);
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, g = lane >> 4;
+  // A-operand pixel of this lane (row li of each M-block): element li & 3 of
+  // the 2 x 2 block li >> 2; M-block q adds 8 q columns.
+  const int ayl = 2 * wv + ((li & 3) >> 1);
+  const int axl = 2 * (li >> 2) + (li & 1);
+  f4v acc[4][NBG];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int nb = 0; nb < NBG; ++nb) acc[q][nb] = f4v{0.f, 0.f, 0.f, 0.f};
+  const u32x4* __restrict__ wp = reinterpret_cast<const u32x4*>(a.w);
+  for (int m = 0; m < MS; ++m) {
+    const int kk = 32 * m + 8 * g;
+    u32x4 af[4];
+    if (kk < 9 * CIN) {
+      const int tap = kk / CIN;
+      const int c0 = kk - tap * CIN;
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      const bf16* base = lds + ((ayl + ky) * kInCols + axl + kx) * CS + c0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) af[q] = *reinterpret_cast<const u32x4*>(base + 8 * q * CS);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) af[q] = u32x4{0u, 0u, 0u, 0u};
+    }
+    u32x4 bw[NBG];
+#pragma unroll
+    for (int nb = 0; nb < NBG; ++nb)
+      bw[nb] = wp[(static_cast<int64_t>(m) * a.nbTotal + nb0 + nb) * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int nb = 0; nb < NBG; ++nb) acc[q][nb] = mfma(af[q], bw[nb], acc[q][nb]);
+  }
+  __syncthreads();  // the input tile is dead; its LDS becomes the output staging
+
+  // Lane holds D[4 g + r][li]: pixel (2 wv + (r >> 1), 8 q + 2 g + (r & 1)),
+  // channel 16 (nb0 + nb) + li.
+  const bool coarseGroup = (E == kEpiSplitUp) && (nb0 * 16 < SPLIT);
+  if (E == kEpiStorePool || coarseGroup) {
+    const int hh = a.h >> 1, hw = a.w_ >> 1;
+    const int py = (y0 >> 1) + wv;
+    constexpr int AUXC = E == kEpiStorePool ? COUT : SPLIT;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int px = (x0 >> 1) + 4 * q + g;
+#pragma unroll
+      for (int nb = 0; nb < NBG; ++nb) {
+        float s = acc[q][nb][0] + acc[q][nb][1] + acc[q][nb][2] + acc[q][nb][3];
+        if (E == kEpiStorePool) s *= 0.25f;
+        if (py < hh && px < hw)
+          a.aux[((static_cast<int64_t>(n) * hh + py) * hw + px) * AUXC + 16 * (nb0 + nb) + li] =
+              f2bf(s);
+      }
+    }
+  }
+  if (!coarseGroup) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int nb = 0; nb < NBG; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int yl = 2 * wv + (r >> 1), xl = 8 * q + 2 * g + (r & 1);
+          lds[(yl * kTW + xl) * SS + 16 * nb + li] = f2bf(acc[q][nb][r]);
+        }
+    __syncthreads();
+    constexpr int OUTC = E == kEpiSplitUp ? COUT - SPLIT : COUT;
+    const int cOff = E == kEpiSplitUp ? nb0 * 16 - SPLIT : nb0 * 16;
+    constexpr int G8 = NBG * 2;
+    for (int it = threadIdx.x; it < kTH * kTW * G8; it += blockDim.x) {
+      const int c8 = it % G8;
+      const int px = it / G8;
+      const int yl = px / kTW, xl = px - yl * kTW;
+      const int y = y0 + yl, x = x0 + xl;
+      if (y < a.h && x < a.w_)
+        *reinterpret_cast<u32x4*>(a.out + ((static_cast<int64_t>(n) * a.h + y) * a.w_ + x) * OUTC +
+                                  cOff + c8 * 8) =
+            *reinterpret_cast<const u32x4*>(lds + px * SS + c8 * 8);
+    }
+  }
+}
+
+// Eight BF16 values p[0], p[stride], ..., p[7 stride] packed into a fragment.
+__device__ inline u32x4 pack8(const bf16* p, int stride) {
+  u32x4 r;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    r[k] = static_cast<uint32_t>(p[(2 * k) * stride]) |
+           (static_cast<uint32_t>(p[(2 * k + 1) * stride]) << 16);
+  return r;
+}
+
+template <int CIN, int CA, int COUT, int GX, int GG, int PPW>
+__global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
+  constexpr int CS = CIN + 8, GS = COUT + 8;
+  constexpr int NCB = CIN / 16, NOB = COUT / 16, NPAIR = 9 * NCB;
+  __shared__ __attribute__((aligned(16))) bf16 xs[kInRows * kInCols * CS];
+  __shared__ __attribute__((aligned(16))) bf16 gs[kTH * kTW * GS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, g = lane >> 4;
+  const int pair0 = (blockIdx.y * 4 + wv) * PPW;
+  f4v acc[PPW][NOB];
+#pragma unroll
+  for (int p = 0; p < PPW; ++p)
+#pragma unroll
+    for (int ob = 0; ob < NOB; ++ob) acc[p][ob] = f4v{0.f, 0.f, 0.f, 0.f};
+  int tapY[PPW], tapX[PPW], cbk[PPW];
+#pragma unroll
+  for (int p = 0; p < PPW; ++p) {
+    const int pair = pair0 + p < NPAIR ? pair0 + p : 0;
+    const int tap = pair / NCB;
+    cbk[p] = pair - tap * NCB;
+    tapY[p] = tap / 3;
+    tapX[p] = tap - 3 * tapY[p];
+  }
+  for (int64_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+    int64_t tile = t;
+    const int tx = static_cast<int>(tile % a.tilesX);
+    tile /= a.tilesX;
+    const int ty = static_cast<int>(tile % a.tilesY);
+    const int n = static_cast<int>(tile / a.tilesY);
+    const int y0 = ty * kTH, x0 = tx * kTW;
+    __syncthreads();
+    stageTile<CIN, CA, GX>(xs, CS, kInRows, kInCols, a.xa, a.xb, n, a.h, a.w_, y0 - 1, x0 - 1);
+    stageTile<COUT, COUT, GG>(gs, GS, kTH, kTW, a.ga, a.gb, n, a.h, a.w_, y0, x0);
+    __syncthreads();
+    for (int s = 0; s < kTH; ++s) {
+      u32x4 bfr[NOB];
+#pragma unroll
+      for (int ob = 0; ob < NOB; ++ob) bfr[ob] = pack8(gs + (s * kTW + 8 * g) * GS + 16 * ob + li, GS);
+#pragma unroll
+      for (int p = 0; p < PPW; ++p) {
+        if (pair0 + p >= NPAIR) continue;  // uniform across the wave
+        const u32x4 afr =
+            pack8(xs + ((s + tapY[p]) * kInCols + 8 * g + tapX[p]) * CS + 16 * cbk[p] + li, CS);
+#pragma unroll
+        for (int ob = 0; ob < NOB; ++ob) acc[p][ob] = mfma(afr, bfr[ob], acc[p][ob]);
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PPW; ++p) {
+    const int pair = pair0 + p;
+    if (pair >= NPAIR) continue;
+#pragma unroll
+    for (int ob = 0; ob < NOB; ++ob)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        a.partial[((static_cast<int64_t>(blockIdx.x) * NPAIR + pair) * NOB + ob) * 256 +
+                  (4 * g + r) * 16 + li] = acc[p][ob][r];
+  }
+}
+
+template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
+void conv(hipStream_t s, const ConvArgs& a) {
+  const dim3 grid(static_cast<unsigned>(a.tilesX * a.tilesY * a.n),
+                  static_cast<unsigned>(COUT / (16 * NBG)));
+  hipLaunchKernelGGL((convKernel<CIN, CA, COUT, NBG, G, E, SPLIT>), grid, dim3(256), 0, s, a);
+}
+
+constexpr int kPpw = 2;
+
+template <int CIN, int CA, int COUT, int GX, int GG>
+int dw(hipStream_t s, const DwArgs& a, int splits) {
+  constexpr int NPAIR = 9 * (CIN / 16);
+  constexpr int groups = (NPAIR + 4 * kPpw - 1) / (4 * kPpw);
+  const dim3 grid(static_cast<unsigned>(splits), static_cast<unsigned>(groups));
+  hipLaunchKernelGGL((dwKernel<CIN, CA, COUT, GX, GG, kPpw>), grid, dim3(256), 0, s, a);
+  return NPAIR;
+}
+
+}  // namespace
+
+// Every (cin, ca, cout, gather, epilogue, split) the network uses; the N-group
+// width keeps the accumulators at <= 64 registers and never straddles the
+// split of a transposed concat layer.
+void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, int split,
+                const ConvArgs& a) {
+#define CFDHIP_CONV(CI, CAV, CO, NBG, G, E, SP)                                            \
+  if (cin == CI && (G != kGatherConcatUp || ca == CAV) && cout == CO && gather == G &&     \
+      epi == E && (E != kEpiSplitUp || split == SP)) {                                     \
+    conv<CI, CAV, CO, NBG, G, E, SP>(s, a);                                                \
+    return;                                                                                \
+  }
+  // forward
+  CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(16, 16, 32, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(32, 32, 64, 4, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(64, 64, 64, 4, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(64, 64, 64, 4, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(128, 64, 64, 4, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(96, 64, 32, 2, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(48, 32, 16, 1, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStore, 0)
+  // transposed (gradient of the input)
+  CFDHIP_CONV(16, 16, 48, 1, kGatherPlain, kEpiSplitUp, 32)
+  CFDHIP_CONV(32, 32, 96, 2, kGatherPlain, kEpiSplitUp, 64)
+  CFDHIP_CONV(64, 64, 128, 4, kGatherPlain, kEpiSplitUp, 64)
+  CFDHIP_CONV(64, 64, 64, 4, kGatherSumUp, kEpiStore, 0)
+  CFDHIP_CONV(64, 64, 32, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 2, kGatherSumUp, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 16, 1, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(16, 16, 16, 1, kGatherSumUp, kEpiStore, 0)
+#undef CFDHIP_CONV
+  throw Error(CFDHIP_ERR_UNSUPPORTED, "no convolution kernel for this layer shape");
+}
+
+int launchConvDw(hipStream_t s, int cin, int ca, int cout, int gx, int gg, const DwArgs& a,
+                 int splits) {
+#define CFDHIP_DW(CI, CAV, CO, GX, GG)                                                      \
+  if (cin == CI && (GX != kGatherConcatUp || ca == CAV) && cout == CO && gx == GX && gg == GG) \
+    return dw<CI, CAV, CO, GX, GG>(s, a, splits);
+  CFDHIP_DW(16, 16, 16, kGatherPlain, kGatherPlain)     // d0b
+  CFDHIP_DW(48, 32, 16, kGatherConcatUp, kGatherPlain)  // d0a
+  CFDHIP_DW(32, 32, 32, kGatherPlain, kGatherPlain)     // d1b
+  CFDHIP_DW(96, 64, 32, kGatherConcatUp, kGatherPlain)  // d1a
+  CFDHIP_DW(64, 64, 64, kGatherPlain, kGatherPlain)     // d2b, b3b, b3a
+  CFDHIP_DW(128, 64, 64, kGatherConcatUp, kGatherPlain) // d2a
+  CFDHIP_DW(64, 64, 64, kGatherPlain, kGatherSumUp)     // e2b
+  CFDHIP_DW(32, 32, 64, kGatherPlain, kGatherPlain)     // e2a
+  CFDHIP_DW(32, 32, 32, kGatherPlain, kGatherSumUp)     // e1b
+  CFDHIP_DW(16, 16, 32, kGatherPlain, kGatherPlain)     // e1a
+  CFDHIP_DW(16, 16, 16, kGatherPlain, kGatherSumUp)     // e0b
+#undef CFDHIP_DW
+  throw Error(CFDHIP_ERR_UNSUPPORTED, "no weight-gradient kernel for this layer shape");
+}
+
+}  // namespace cfdhip

@@ -1,0 +1,81 @@
+// Device state of the learned preconditioner's network.
+#pragma once
+
+#include "unet.hpp"
+
+namespace cfdhip {
+
+// Activation and gradient tensors (BF16, NHWC), in allocation order: the
+// forward tensors first, so an inference-only shape allocates only those.
+enum UnetTensor : int {
+  kTe0a = 0, kTe0, kTp0, kTe1a, kTe1, kTp1, kTe2a, kTe2, kTp2, kTb3a, kTb3,
+  kTd2a, kTd2, kTd1a, kTd1, kTd0a, kTd0,
+  kNumForward,
+  kTgd0 = kNumForward, kTgd0a, kTgd1, kTge0s, kTgd1a, kTgd2, kTge1s, kTgd2a, kTgb3, kTge2s,
+  kTgb3a, kTgp2, kTge2a, kTgp1, kTge1a, kTgp0, kTge0a,
+  kNumTensors
+};
+
+class Unet {
+ public:
+  explicit Unet(hipStream_t s);
+
+  void setWeights(const float* host);
+  void getWeights(float* host);
+  void getGrads(float* host) {
+    CFDHIP_CHECK(hipMemcpyAsync(host, grad.get(), kUnetParams * sizeof(float),
+                                hipMemcpyDeviceToHost, stream));
+    CFDHIP_CHECK(hipStreamSynchronize(stream));
+  }
+  bool ready() const { return loaded; }
+
+  // Activations for n images of h x w (multiples of 8); training adds the
+  // gradient tensors.
+  void ensureShape(int n, int h, int w, bool training);
+  // Encoder/decoder up to d0 on x = src * scale (src: srcH x srcW per image,
+  // zero-extended to h x w).
+  void forward(const float* src, const float* scale, int srcH, int srcW, int64_t srcBatch, int n,
+               int h, int w);
+  // y = conv(d0) into the FP32 training output.
+  void outputTrain(int n, int h, int w);
+  // z = scale * conv(d0) on the ny x nx solver grid, z[pin] = r[pin] scale^2
+  // (the inverse of the decoupled pinned row), and block partials of (z, q)
+  // (parts blocks; none when partial is null).
+  void outputApply(const float* scale, const float* r, const float* q, float* z, int ny, int nx,
+                   int64_t pin, int h, int w, double* partial, int parts);
+  // Weight gradients for dL/dy = gyIn (x: the FP32 network input).
+  void backward(const float* x, const float* gyIn, int n, int h, int w);
+  // A-norm loss of the training output against e; fills gy, returns the loss.
+  double loss(const float* e, const float* cE, const float* cN, const int64_t* sysOffDev,
+              const float* denDev, int n, int h, int w);
+  void adam(float lr, float b1, float b2, float eps);
+
+  float* output() { return y.get(); }
+  // BF16 tensor t of the current shape, widened to FP32 on the host.
+  int64_t tensorElems(int t) const;
+  void readTensor(int t, float* host, int64_t count);
+  float* outputGrad() { return gy.get(); }
+  hipStream_t stream;
+
+ private:
+  void repack();
+  bf16* T(int t) { return act.get() + tensorOff[t]; }
+  void conv(int layer, bool transposed, int gather, int epi, int split, const bf16* a,
+            const bf16* b, bf16* out, bf16* aux, int n, int h, int w);
+  void convDw(int layer, int gx, int gg, const bf16* xa, const bf16* xb, const bf16* ga,
+              const bf16* gb, int n, int h, int w);
+  void singleDw(int mode, const float* s1, const bf16* m16, int n, int h, int w, float* out);
+
+  DeviceArray<float> master, grad, adamM, adamV, y, gy, dwPartial;
+  DeviceArray<double> lossPartial;
+  DeviceArray<bf16> fwdPack, bwdPack, act;
+  int64_t fwdOff[kNumLayers] = {};
+  int64_t bwdOff[kNumLayers] = {};
+  int64_t tensorOff[kNumTensors] = {};
+  int shapeN = 0, shapeH = 0, shapeW = 0;
+  bool shapeTraining = false;
+  bool loaded = false;
+  int64_t adamStep = 0;
+};
+
+}  // namespace cfdhip

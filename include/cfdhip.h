@@ -243,6 +243,62 @@ int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t*
 int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled);
 int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out);
 
+/* ---- learned preconditioner ------------------------------------------------------ */
+/* No counterpart in the reference: the p' solve with preconditioner
+ * CFDHIP_PREC_LEARNED is flexible CG (type CFDHIP_SOLVER_FCG) preconditioned by
+ * a linear 4-level convolutional U-Net (3x3 convolutions, 16-64 channels,
+ * BF16 weights and activations, FP32 accumulation) applied to the
+ * Jacobi-scaled residual, z = D^-1/2 N(D^-1/2 r). Inner iterations run in
+ * FP32, FP64 refinement reaches the requested FP64 true residual, and an
+ * inner solve that stops improving for stagnation_window iterations hands the
+ * solve to FP64 Jacobi-PCG (fallback_used = 1). 2D p' operator only. */
+
+/* Number of FP32 master weights: layers e0a e0b e1a e1b e2a e2b b3a b3b d2a
+ * d2b d1a d1b d0a d0b out, each W[ky][kx][cin][cout]. */
+int64_t cfdhip_unet_param_count(void);
+/* Loads the network weights used by CFDHIP_PREC_LEARNED on this context. */
+int cfdhip_set_learned_weights(cfdhip_ctx* ctx, const float* weights, int64_t count);
+
+/* Training pairs from the assembled p' system (b: host RHS of n doubles, or
+ * NULL for the assembled one): an FP64 Jacobi-PCG solve (settings s) keeps its
+ * iterate every `stride` iterations; with x* the converged solution (left in
+ * CFDHIP_FIELD_PPRIME) each kept iterate gives e = x* - x_k and r = A e, written
+ * in the Jacobi-scaled system (e^ = D^1/2 e, r^ = D^-1/2 r), scaled to unit RMS
+ * of r^ and zero-padded to h x w. Device buffers: r_out, e_out (max_pairs x h x
+ * w floats), ce_out, cn_out (h x w: scaled couplings to x + 1 and y + 1, may be
+ * NULL); host: den_out (max_pairs floats: (e^, r^) per pair). */
+int cfdhip_pressure_harvest(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
+                            int64_t stride, int64_t max_pairs, int64_t h, int64_t w, float* r_out,
+                            float* e_out, float* ce_out, float* cn_out, float* den_out,
+                            int64_t* pairs, cfdhip_solve_result* result);
+
+/* A network handle for training and evaluation on caller-owned device buffers
+ * (FP32, batch x h x w, h and w multiples of 8). Calls are synchronous. */
+typedef struct cfdhip_unet cfdhip_unet;
+int cfdhip_unet_create(cfdhip_unet** out);
+void cfdhip_unet_destroy(cfdhip_unet* net);
+const char* cfdhip_unet_last_error(const cfdhip_unet* net);
+int cfdhip_unet_set_weights(cfdhip_unet* net, const float* weights, int64_t count);
+int cfdhip_unet_get_weights(cfdhip_unet* net, float* weights, int64_t count);
+/* y = N(x). */
+int cfdhip_unet_forward(cfdhip_unet* net, const float* x, float* y, int64_t batch, int64_t h,
+                        int64_t w);
+/* grad (host, param_count floats) = d/dW of sum(gy * N(x)). */
+int cfdhip_unet_backward(cfdhip_unet* net, const float* x, const float* gy, int64_t batch, int64_t h,
+                         int64_t w, float* grad);
+/* Activation / gradient tensor `tensor` of the last pass (BF16 widened to
+ * FP32, NHWC): forward e0a e0 p0 e1a e1 p1 e2a e2 p2 b3a b3 d2a d2 d1a d1 d0a d0
+ * (ids 0-16), after a backward pass also the gradients (ids 17-33). */
+int64_t cfdhip_unet_tensor_size(cfdhip_unet* net, int tensor);
+int cfdhip_unet_tensor(cfdhip_unet* net, int tensor, float* out, int64_t count);
+/* A-norm loss L = mean_s (N(x_s) - e_s, A_s (N(x_s) - e_s)) / den_s with A_s the
+ * unit-diagonal operator of couplings ce, cn + sys_offset[s] (device int64 per
+ * sample); with update != 0 one Adam step (lr, betas 0.9 / 0.999). */
+int cfdhip_unet_train_step(cfdhip_unet* net, const float* x, const float* e, const float* ce,
+                           const float* cn, const int64_t* sys_offset, const float* den,
+                           int64_t batch, int64_t h, int64_t w, float lr, int update,
+                           double* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,303 @@
+"""CPU oracle of the learned p' preconditioner -- TEST INFRASTRUCTURE ONLY.
+
+The network has no counterpart in the reference (it is one of the additions
+on the hot path), so this restatement is pinned by mathematics instead: its
+backward pass against central finite differences of its own loss
+(tests/test_unet_oracle.py), linearity and the adjoint identity of each layer,
+and the reference-independent Green's-function check of the trained
+preconditioner (FCG iteration counts). The HIP kernels (cfdapp_amd/csrc/unet.hip)
+are checked against this module on the same weights and inputs.
+
+Network (all convolutions 3x3, zero padding, no bias, no activation: the
+preconditioner is a linear operator, z = M r, so flexible CG sees a fixed M):
+  level 0 (H x W):      e0a = conv(x, 1->16);  e0 = conv(e0a, 16->16)
+  level 1 (H/2):        e1a = conv(pool(e0), 16->32); e1 = conv(e1a, 32->32)
+  level 2 (H/4):        e2a = conv(pool(e1), 32->64); e2 = conv(e2a, 64->64)
+  level 3 (H/8):        b3a = conv(pool(e2), 64->64); b3 = conv(b3a, 64->64)
+  decoder:              d2a = conv([up(b3), e2], 128->64); d2 = conv(d2a, 64->64)
+                        d1a = conv([up(d2), e1], 96->32);  d1 = conv(d1a, 32->32)
+                        d0a = conv([up(d1), e0], 48->16);  d0 = conv(d0a, 16->16)
+  output:               y = conv(d0, 16->1)
+pool = 2x2 average, up = nearest-neighbour x2, [a, b] = channel concatenation
+(upsampled tensor first). Activations are NHWC; weights W[ky, kx, cin, cout].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+CHANNELS = (16, 32, 64, 64)
+
+# (name, cin, cout) in execution order
+LAYERS = [
+    ("e0a", 1, 16), ("e0b", 16, 16),
+    ("e1a", 16, 32), ("e1b", 32, 32),
+    ("e2a", 32, 64), ("e2b", 64, 64),
+    ("b3a", 64, 64), ("b3b", 64, 64),
+    ("d2a", 128, 64), ("d2b", 64, 64),
+    ("d1a", 96, 32), ("d1b", 32, 32),
+    ("d0a", 48, 16), ("d0b", 16, 16),
+    ("out", 16, 1),
+]
+
+
+def init_weights(seed=0, scale=1.0):
+    """He-style init, seeded (numpy default_rng)."""
+    rng = np.random.default_rng(seed)
+    w = {}
+    for name, cin, cout in LAYERS:
+        std = scale * np.sqrt(1.0 / (9.0 * cin))
+        w[name] = (rng.standard_normal((3, 3, cin, cout)) * std).astype(np.float32)
+    return w
+
+
+def conv3x3(x, w):
+    """x: (N, H, W, Cin), w: (3, 3, Cin, Cout) -> (N, H, W, Cout); zero padding."""
+    n, h, wd, cin = x.shape
+    xp = np.zeros((n, h + 2, wd + 2, cin), dtype=np.float64)
+    xp[:, 1:-1, 1:-1, :] = x
+    out = np.zeros((n, h, wd, w.shape[3]), dtype=np.float64)
+    for ky in range(3):
+        for kx in range(3):
+            out += xp[:, ky:ky + h, kx:kx + wd, :] @ w[ky, kx].astype(np.float64)
+    return out
+
+
+def conv3x3_input_grad(g, w):
+    """Adjoint of conv3x3 in x: dL/dx given dL/dout = g."""
+    n, h, wd, cout = g.shape
+    gp = np.zeros((n, h + 2, wd + 2, cout), dtype=np.float64)
+    gp[:, 1:-1, 1:-1, :] = g
+    dx = np.zeros((n, h, wd, w.shape[2]), dtype=np.float64)
+    # out[y, x] += in[y + ky - 1, x + kx - 1] w[ky, kx]  =>
+    # din[y', x'] += g[y' - ky + 1, x' - kx + 1] w[ky, kx]^T
+    for ky in range(3):
+        for kx in range(3):
+            dx += gp[:, 2 - ky:2 - ky + h, 2 - kx:2 - kx + wd, :] @ w[ky, kx].T.astype(np.float64)
+    return dx
+
+
+def conv3x3_weight_grad(x, g):
+    """dL/dw given the conv input x and dL/dout = g."""
+    n, h, wd, cin = x.shape
+    xp = np.zeros((n, h + 2, wd + 2, cin), dtype=np.float64)
+    xp[:, 1:-1, 1:-1, :] = x
+    dw = np.zeros((3, 3, cin, g.shape[3]), dtype=np.float64)
+    for ky in range(3):
+        for kx in range(3):
+            a = xp[:, ky:ky + h, kx:kx + wd, :].reshape(-1, cin)
+            dw[ky, kx] = a.T @ g.reshape(-1, g.shape[3])
+    return dw
+
+
+def pool(x):
+    n, h, w, c = x.shape
+    return x.reshape(n, h // 2, 2, w // 2, 2, c).mean(axis=(2, 4))
+
+
+def pool_grad(g):
+    return np.repeat(np.repeat(g, 2, axis=1), 2, axis=2) * 0.25
+
+
+def up(x):
+    return np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)
+
+
+def up_grad(g):
+    n, h, w, c = g.shape
+    return g.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
+
+
+def forward(w, x, keep=False):
+    """x: (N, H, W, 1) with H, W divisible by 8. Returns y (N, H, W, 1) [and the tape]."""
+    t = {"x": x.astype(np.float64)}
+    t["e0a"] = conv3x3(t["x"], w["e0a"])
+    t["e0"] = conv3x3(t["e0a"], w["e0b"])
+    t["p0"] = pool(t["e0"])
+    t["e1a"] = conv3x3(t["p0"], w["e1a"])
+    t["e1"] = conv3x3(t["e1a"], w["e1b"])
+    t["p1"] = pool(t["e1"])
+    t["e2a"] = conv3x3(t["p1"], w["e2a"])
+    t["e2"] = conv3x3(t["e2a"], w["e2b"])
+    t["p2"] = pool(t["e2"])
+    t["b3a"] = conv3x3(t["p2"], w["b3a"])
+    t["b3"] = conv3x3(t["b3a"], w["b3b"])
+    t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
+    t["d2a"] = conv3x3(t["c2"], w["d2a"])
+    t["d2"] = conv3x3(t["d2a"], w["d2b"])
+    t["c1"] = np.concatenate([up(t["d2"]), t["e1"]], axis=3)
+    t["d1a"] = conv3x3(t["c1"], w["d1a"])
+    t["d1"] = conv3x3(t["d1a"], w["d1b"])
+    t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
+    t["d0a"] = conv3x3(t["c0"], w["d0a"])
+    t["d0"] = conv3x3(t["d0a"], w["d0b"])
+    t["y"] = conv3x3(t["d0"], w["out"])
+    return (t["y"], t) if keep else t["y"]
+
+
+def backward(w, t, gy):
+    """Gradients of a scalar loss with dL/dy = gy: returns (dW dict, dL/dx)."""
+    g = {}
+    dw = {}
+
+    def conv_back(name, inp, gout):
+        dw[name] = conv3x3_weight_grad(inp, gout)
+        return conv3x3_input_grad(gout, w[name])
+
+    gd0 = conv_back("out", t["d0"], gy)
+    gd0a = conv_back("d0b", t["d0a"], gd0)
+    gc0 = conv_back("d0a", t["c0"], gd0a)
+    c = t["d1"].shape[3]
+    g["d1"] = up_grad(gc0[..., :c])
+    g["e0"] = gc0[..., c:]
+    gd1a = conv_back("d1b", t["d1a"], g["d1"])
+    gc1 = conv_back("d1a", t["c1"], gd1a)
+    c = t["d2"].shape[3]
+    g["d2"] = up_grad(gc1[..., :c])
+    g["e1"] = gc1[..., c:]
+    gd2a = conv_back("d2b", t["d2a"], g["d2"])
+    gc2 = conv_back("d2a", t["c2"], gd2a)
+    c = t["b3"].shape[3]
+    g["b3"] = up_grad(gc2[..., :c])
+    g["e2"] = gc2[..., c:]
+    gb3a = conv_back("b3b", t["b3a"], g["b3"])
+    gp2 = conv_back("b3a", t["p2"], gb3a)
+    g["e2"] = g["e2"] + pool_grad(gp2)
+    ge2a = conv_back("e2b", t["e2a"], g["e2"])
+    gp1 = conv_back("e2a", t["p1"], ge2a)
+    g["e1"] = g["e1"] + pool_grad(gp1)
+    ge1a = conv_back("e1b", t["e1a"], g["e1"])
+    gp0 = conv_back("e1a", t["p0"], ge1a)
+    g["e0"] = g["e0"] + pool_grad(gp0)
+    ge0a = conv_back("e0b", t["e0a"], g["e0"])
+    gx = conv_back("e0a", t["x"], ge0a)
+    return dw, gx
+
+
+# ---------------------------------------------------------------------------
+# BF16 storage emulation: the device keeps weights and activations in BF16
+# (round to nearest even) and accumulates in FP32. These variants round at
+# exactly the points where the device stores, so the remaining difference is
+# the FP32 accumulation order.
+
+def bf(a):
+    """Round to BF16 (nearest even) and back to float64."""
+    a32 = np.ascontiguousarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(w, x):
+    """Forward pass with device rounding: returns (y, tape)."""
+    wb = {k: bf(v) for k, v in w.items()}
+    t = {"x": np.asarray(x, dtype=np.float64)}
+
+    def conv_pool(name, inp):
+        acc = conv3x3(inp, wb[name])
+        return bf(acc), bf(pool(acc))
+
+    t["e0a"] = bf(conv3x3(t["x"], wb["e0a"]))
+    t["e0"], t["p0"] = conv_pool("e0b", t["e0a"])
+    t["e1a"] = bf(conv3x3(t["p0"], wb["e1a"]))
+    t["e1"], t["p1"] = conv_pool("e1b", t["e1a"])
+    t["e2a"] = bf(conv3x3(t["p1"], wb["e2a"]))
+    t["e2"], t["p2"] = conv_pool("e2b", t["e2a"])
+    t["b3a"] = bf(conv3x3(t["p2"], wb["b3a"]))
+    t["b3"] = bf(conv3x3(t["b3a"], wb["b3b"]))
+    t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
+    t["d2a"] = bf(conv3x3(t["c2"], wb["d2a"]))
+    t["d2"] = bf(conv3x3(t["d2a"], wb["d2b"]))
+    t["c1"] = np.concatenate([up(t["d2"]), t["e1"]], axis=3)
+    t["d1a"] = bf(conv3x3(t["c1"], wb["d1a"]))
+    t["d1"] = bf(conv3x3(t["d1a"], wb["d1b"]))
+    t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
+    t["d0a"] = bf(conv3x3(t["c0"], wb["d0a"]))
+    t["d0"] = bf(conv3x3(t["d0a"], wb["d0b"]))
+    t["y"] = conv3x3(t["d0"], wb["out"])
+    return t["y"], t
+
+
+def backward_bf16(w, t, gy):
+    """Weight gradients with device rounding of the stored gradient tensors."""
+    wb = {k: bf(v) for k, v in w.items()}
+    dw = {}
+
+    def back(name, inp, g):
+        dw[name] = conv3x3_weight_grad(inp, g)
+        return conv3x3_input_grad(g, wb[name])
+
+    def split(gc, c):  # transposed concat layer: coarse part summed 2x2, skip part
+        return bf(up_grad(gc[..., :c])), bf(gc[..., c:])
+
+    gd0 = bf(back("out", t["d0"], gy))
+    gd0a = bf(back("d0b", t["d0a"], gd0))
+    gd1, ge0s = split(back("d0a", t["c0"], gd0a), t["d1"].shape[3])
+    gd1a = bf(back("d1b", t["d1a"], gd1))
+    gd2, ge1s = split(back("d1a", t["c1"], gd1a), t["d2"].shape[3])
+    gd2a = bf(back("d2b", t["d2a"], gd2))
+    gb3, ge2s = split(back("d2a", t["c2"], gd2a), t["b3"].shape[3])
+    gb3a = bf(back("b3b", t["b3a"], gb3))
+    gp2 = bf(back("b3a", t["p2"], gb3a))
+    ge2 = bf(ge2s + pool_grad(gp2))
+    ge2a = bf(back("e2b", t["e2a"], ge2))
+    gp1 = bf(back("e2a", t["p1"], ge2a))
+    ge1 = bf(ge1s + pool_grad(gp1))
+    ge1a = bf(back("e1b", t["e1a"], ge1))
+    gp0 = bf(back("e1a", t["p0"], ge1a))
+    ge0 = bf(ge0s + pool_grad(gp0))
+    ge0a = bf(back("e0b", t["e0a"], ge0))
+    dw["e0a"] = conv3x3_weight_grad(t["x"], ge0a)
+    return dw
+
+
+def flatten(w):
+    """Master-weight vector in layer order (the device layout)."""
+    return np.concatenate([np.asarray(w[name], dtype=np.float32).ravel()
+                           for name, _, _ in LAYERS])
+
+
+def unflatten(v):
+    w, o = {}, 0
+    for name, cin, cout in LAYERS:
+        k = 9 * cin * cout
+        w[name] = np.asarray(v[o:o + k], dtype=np.float32).reshape(3, 3, cin, cout)
+        o += k
+    return w
+
+
+def anorm_loss(y, e, ce, cn, den):
+    """Mean over samples of (d, A d) / den with d = y - e, A the unit-diagonal
+    operator of couplings ce (to x + 1) and cn (to y + 1); returns (L, dL/dy).
+    y, e: (N, H, W); ce, cn: (N, H, W); den: (N,)."""
+    d = np.asarray(y, np.float64) - np.asarray(e, np.float64)
+    ad = d.copy()
+    ad[:, :, :-1] -= ce[:, :, :-1] * d[:, :, 1:]
+    ad[:, :, 1:] -= ce[:, :, :-1] * d[:, :, :-1]
+    ad[:, :-1, :] -= cn[:, :-1, :] * d[:, 1:, :]
+    ad[:, 1:, :] -= cn[:, :-1, :] * d[:, :-1, :]
+    n = d.shape[0]
+    per = (d * ad).sum(axis=(1, 2)) / den
+    return float(per.mean()), 2.0 * ad / (den[:, None, None] * n)
+
+
+def mse_loss(y, target):
+    """L = mean((y - target)^2); returns (L, dL/dy)."""
+    d = y - target
+    return float(np.mean(d * d)), 2.0 * d / d.size
+
+
+def laplacian_apply(z, pin=True):
+    """The d = 1 p' operator on an (H, W) grid (Neumann faces carry no
+    coupling, cell 0 pinned to the identity row), as a dense reference."""
+    h, w = z.shape
+    y = np.zeros_like(z, dtype=np.float64)
+    deg = np.zeros_like(z, dtype=np.float64)
+    for dy, dx in ((0, 1), (0, -1), (1, 0), (-1, 0)):
+        ys, ye = max(0, -dy), h - max(0, dy)
+        xs, xe = max(0, -dx), w - max(0, dx)
+        y[ys:ye, xs:xe] -= z[ys + dy:ye + dy, xs + dx:xe + dx]
+        deg[ys:ye, xs:xe] += 1.0
+    y += deg * z
+    if pin:
+        y[0, 0] = z[0, 0]
+    return y
