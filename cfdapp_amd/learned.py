@@ -1,0 +1,302 @@
+"""Training of the learned p' preconditioner on the device.
+
+The preconditioner is the linear 4-level U-Net of ``csrc/unet.hpp`` applied to
+the Jacobi-scaled residual inside flexible CG. This module builds its training
+set from the solver's own p' solves (``GpuSimpleDiscretization.harvest``),
+initialises it as an additive multilevel operator and trains it with the
+device kernels (``UnetNet.train_step``: BF16 MFMA forward and backward, A-norm
+loss, Adam).
+
+    python -m cfdapp_amd.learned --out cfdapp_amd/weights/pprime_unet.f32
+
+Device memory comes from torch tensors (allocation only); every arithmetic
+step runs in libcfdhip.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import (FIXED_GRADIENT, FIXED_VALUE, INLET, OUTLET, WALL, GpuSimpleDiscretization,
+               LinearSolverSettings, UnetNet, unet_param_count)
+
+LAYERS = [("e0a", 1, 16), ("e0b", 16, 16), ("e1a", 16, 32), ("e1b", 32, 32),
+          ("e2a", 32, 64), ("e2b", 64, 64), ("b3a", 64, 64), ("b3b", 64, 64),
+          ("d2a", 128, 64), ("d2b", 64, 64), ("d1a", 96, 32), ("d1b", 32, 32),
+          ("d0a", 48, 16), ("d0b", 16, 16), ("out", 16, 1)]
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# initialisation
+
+def structured_init(seed=0, ratio=8.0, noise=0.01) -> np.ndarray:
+    """An additive multilevel operator (BPX-like) as the starting point.
+
+    Channel 0 carries each level's residual along the skip path, channel 1 a
+    copy smoothed by the 3x3 binomial kernel K before it is averaged down, so
+    restriction (average of K r) and prolongation (K of the nearest upsample)
+    are transposes and the operator is symmetric positive definite:
+        M r = sum_l ratio^l P_l R_l r.
+    The other channels start as small random weights for training to use.
+    Returns the flat FP32 master vector (layer order, W[ky][kx][cin][cout]).
+    """
+    rng = np.random.default_rng(seed)
+    k1 = np.array([1.0, 2.0, 1.0])
+    K = np.outer(k1, k1) / 16.0
+    w = {}
+    for name, ci, co in LAYERS:
+        w[name] = rng.standard_normal((3, 3, ci, co)) * math.sqrt(1.0 / (9 * ci)) * noise
+
+    def ident(name, o, i, s=1.0):
+        w[name][1, 1, i, o] += s
+
+    def smooth(name, o, i, s=1.0):
+        w[name][:, :, i, o] += s * K
+
+    ident("e0a", 0, 0)
+    ident("e0a", 1, 0)
+    for b in ("e0b", "e1b", "e2b"):
+        ident(b, 0, 0)
+        smooth(b, 1, 1)
+    for a in ("e1a", "e2a"):
+        ident(a, 0, 1)
+        ident(a, 1, 1)
+    ident("b3a", 0, 1)
+    smooth("b3b", 0, 0, ratio ** 3)
+    smooth("d2a", 0, 0)
+    ident("d2a", 0, 64, ratio ** 2)
+    smooth("d1a", 0, 0)
+    ident("d1a", 0, 64, ratio)
+    smooth("d0a", 0, 0)
+    ident("d0a", 0, 32, 1.0)
+    for n in ("d2b", "d1b", "d0b", "out"):
+        ident(n, 0, 0)
+    flat = np.concatenate([w[name].astype(np.float32).ravel() for name, _, _ in LAYERS])
+    assert flat.size == unet_param_count()
+    return flat
+
+
+# ---------------------------------------------------------------------------
+# cases and harvesting
+
+def make_case(kind, n, re, speed):
+    """A square 2D cavity (lid speed) or channel (inlet speed) with Re = speed L / nu."""
+    nu = speed * 1.0 / re
+    if kind == "cavity":
+        return GpuSimpleDiscretization.cavity(2, n, n, lid=speed, viscosity=nu)
+    d = GpuSimpleDiscretization()
+    vel = [(INLET, (speed, 0.0, 0.0)), (OUTLET, (0.0, 0.0, 0.0)),
+           (WALL, (0.0, 0.0, 0.0)), (WALL, (0.0, 0.0, 0.0))]
+    pres = [(FIXED_GRADIENT, 0.0), (FIXED_VALUE, 0.0), (FIXED_GRADIENT, 0.0),
+            (FIXED_GRADIENT, 0.0)]
+    ok, why = d.prepare(2, n, n, velocity=vel, pressure=pres, viscosity=nu)
+    if not ok:
+        raise RuntimeError(why)
+    return d
+
+
+def assemble_pprime(d, warm_iterations, alpha_u=0.7):
+    """Advances SIMPLE, then assembles the next iteration's p' system."""
+    mom = LinearSolverSettings(1e-12, 1e-8, 2000, "BiCGSTAB")
+    pres = LinearSolverSettings(0.0, 1e-8, 200000, "CG", "Jacobi", check_interval=64)
+    if warm_iterations > 0:
+        d.simple_run(warm_iterations, pressure=pres, momentum=mom)
+    d.mass_flux()
+    d.assemble_momentum_resident(alpha_u)
+    for k in range(2):
+        d.solve_momentum_resident(k, mom)
+    d.compute_response_coefficients()
+    d.compute_predicted_face_flux(True, alpha_u)
+    d.assemble_pressure_correction_resident(0)
+
+
+class PairBank:
+    """Harvested pairs on the device, one group per grid size."""
+
+    def __init__(self, torch, capacity):
+        self.torch = torch
+        self.capacity = dict(capacity)  # n -> (max pairs, max systems)
+        self.r, self.e, self.ce, self.cn, self.den, self.sys = {}, {}, {}, {}, {}, {}
+        self.count, self.systems = {}, {}
+        for n, (pairs, systems) in self.capacity.items():
+            dev = dict(device="cuda", dtype=torch.float32)
+            self.r[n] = torch.zeros((pairs, n, n), **dev)
+            self.e[n] = torch.zeros((pairs, n, n), **dev)
+            self.ce[n] = torch.zeros((systems, n, n), **dev)
+            self.cn[n] = torch.zeros((systems, n, n), **dev)
+            self.den[n] = np.zeros(pairs, np.float32)
+            self.sys[n] = np.zeros(pairs, np.int64)
+            self.count[n] = 0
+            self.systems[n] = 0
+
+    def harvest(self, d, n, stride, max_pairs):
+        k0, s = self.count[n], self.systems[n]
+        room = min(max_pairs, self.capacity[n][0] - k0)
+        if room <= 0 or s >= self.capacity[n][1]:
+            return 0, None
+        settings = LinearSolverSettings(0.0, 1e-10, 400000, "CG", "Jacobi", check_interval=64)
+        got, den, res = d.harvest(settings, stride, room, n, n, self.r[n][k0], self.e[n][k0],
+                                  self.ce[n][s], self.cn[n][s])
+        self.den[n][k0:k0 + got] = den
+        self.sys[n][k0:k0 + got] = s
+        self.count[n] += got
+        self.systems[n] += 1
+        return got, res
+
+    def total(self):
+        return sum(self.count.values())
+
+
+def harvest_cases(torch, cases, capacity, seed=0, deadline=None):
+    """Runs SIMPLE cases and harvests p' pairs; cases: list of (kind, n, re, speed, warm)."""
+    bank = PairBank(torch, capacity)
+    t0 = time.time()
+    for i, (kind, n, re, speed, warm) in enumerate(cases):
+        if deadline is not None and time.time() > deadline:
+            _log(f"harvest: time budget reached after {i} cases")
+            break
+        d = make_case(kind, n, re, speed)
+        try:
+            assemble_pprime(d, warm)
+            stride = max(4, n // 16)
+            got, res = bank.harvest(d, n, stride, 200)
+        finally:
+            d.close()
+        it = res.iterations if res is not None else 0
+        _log(f"harvest {i}: {kind} {n}^2 Re {re:.0f} U {speed:.2f} warm {warm}: "
+             f"{it} it, {got} pairs (total {bank.total()}, {time.time() - t0:.0f} s)")
+    return bank
+
+
+def default_cases(seed, n_cases):
+    """Cavities and channels at 256^2-1024^2, Re log-uniform in 100-5000,
+    lid/inlet speed uniform in 0.5-2, 0-4 SIMPLE iterations before the harvest."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_cases):
+        kind = "cavity" if i % 2 == 0 else "channel"
+        n = int(rng.choice([256, 512, 1024], p=[0.55, 0.33, 0.12]))
+        re = float(math.exp(rng.uniform(math.log(100.0), math.log(5000.0))))
+        speed = float(rng.uniform(0.5, 2.0))
+        warm = int(rng.integers(0, 5))
+        out.append((kind, n, re, speed, warm))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# training
+
+BATCH = {256: 16, 512: 4, 1024: 1}
+
+
+def train(torch, bank, weights, steps, lr, seed=0, deadline=None, log_every=50):
+    net = UnetNet(weights)
+    rng = np.random.default_rng(seed)
+    sizes = [n for n in bank.count if bank.count[n] > 0]
+    probs = np.array([bank.count[n] * n * n for n in sizes], np.float64)
+    probs /= probs.sum()
+    losses = []
+    t0 = time.time()
+    warm = max(1, steps // 20)
+    for step in range(steps):
+        if deadline is not None and time.time() > deadline:
+            _log(f"train: time budget reached at step {step}")
+            break
+        n = sizes[int(rng.choice(len(sizes), p=probs))]
+        b = min(BATCH[n], bank.count[n])
+        idx = rng.choice(bank.count[n], size=b, replace=False)
+        it = torch.from_numpy(idx).cuda()
+        x = bank.r[n].index_select(0, it)
+        e = bank.e[n].index_select(0, it)
+        off = torch.from_numpy(bank.sys[n][idx] * n * n).cuda()
+        den = torch.from_numpy(bank.den[n][idx]).cuda()
+        # linear warm-up, cosine decay
+        f = min(1.0, (step + 1) / warm) * 0.5 * (1.0 + math.cos(math.pi * step / steps))
+        loss = net.train_step(x, e, bank.ce[n], bank.cn[n], off, den, b, n, n, lr * f)
+        losses.append(loss)
+        if step % log_every == 0 or step == steps - 1:
+            recent = float(np.mean(losses[-log_every:]))
+            _log(f"step {step}: loss {recent:.4f} (lr {lr * f:.2e}, {n}^2 x{b}, "
+                 f"{time.time() - t0:.0f} s)")
+    return net.get_weights(), losses
+
+
+# ---------------------------------------------------------------------------
+# evaluation
+
+def standalone_system(n, seed=20480001):
+    """The d = 1 p' system of an n x n cavity with a seeded mean-free RHS."""
+    d = GpuSimpleDiscretization.cavity(2, n, n)
+    d.upload("RESPONSE", np.ones(d.n_cells))
+    d.upload("FLUX_STAR", np.zeros(d.n_faces))
+    d.assemble_pressure_correction_resident(0)
+    b = np.random.default_rng(seed).uniform(-1.0, 1.0, d.n_cells)
+    b -= b.mean()
+    b[0] = 0.0
+    return d, b
+
+
+def evaluate(weights, n, tol=1e-8):
+    d, b = standalone_system(n)
+    try:
+        d.upload("PC_RHS", b)
+        jac = LinearSolverSettings(0.0, tol, 400000, "CG", "Jacobi", check_interval=64)
+        fcg = LinearSolverSettings(0.0, tol, 20000, "FCG", "Learned")
+        d.set_learned_weights(weights)
+        rj = d.solve_pressure_rhs(jac, None, download=False)
+        rf = d.solve_pressure_rhs(fcg, None, download=False)
+        return dict(n=n, jacobi_iterations=rj.iterations, jacobi_ms=rj.device_ms,
+                    fcg_iterations=rf.iterations, fcg_ms=rf.device_ms,
+                    fcg_fallback=rf.fallback_used, fcg_status=rf.status_name,
+                    fcg_true_rel=rf.true_residual / rf.initial_residual)
+    finally:
+        d.close()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--cases", type=int, default=120)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--harvest-seconds", type=float, default=300)
+    ap.add_argument("--train-seconds", type=float, default=420)
+    ap.add_argument("--init", default=None, help="start from these weights instead")
+    ap.add_argument("--eval", type=int, nargs="*", default=[256, 1024])
+    ap.add_argument("--summary", default=None)
+    args = ap.parse_args(argv)
+    import torch
+    w0 = (np.fromfile(args.init, dtype="<f4") if args.init else structured_init(args.seed))
+    summary = dict(args=vars(args))
+    summary["eval_init"] = [evaluate(w0, n) for n in args.eval]
+    _log("init:", summary["eval_init"])
+    cap = {256: (13000, 200), 512: (6500, 100), 1024: (1600, 30)}
+    cases = default_cases(args.seed, args.cases)
+    bank = harvest_cases(torch, cases, cap, args.seed, time.time() + args.harvest_seconds)
+    summary["pairs"] = {str(n): int(c) for n, c in bank.count.items()}
+    w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
+                      time.time() + args.train_seconds)
+    w.astype("<f4").tofile(args.out)
+    summary["loss_first"] = float(np.mean(losses[:50])) if losses else None
+    summary["loss_last"] = float(np.mean(losses[-50:])) if losses else None
+    summary["steps"] = len(losses)
+    summary["eval_trained"] = [evaluate(w, n) for n in args.eval]
+    _log("trained:", summary["eval_trained"])
+    if args.summary:
+        with open(args.summary, "w") as f:
+            json.dump(summary, f, indent=1)
+    print(json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()
