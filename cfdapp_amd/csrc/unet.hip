@@ -25,6 +25,7 @@ __device__ inline bf16 f2bf(float f) {
 __device__ inline float roundBf(float f) { return bf2f(f2bf(f)); }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
 
 inline dim3 grid1(int64_t n, int64_t cap = 8192) {
   int64_t b = (n + 255) / 256;
@@ -32,47 +33,38 @@ inline dim3 grid1(int64_t n, int64_t cap = 8192) {
   return dim3(static_cast<unsigned>(b < 1 ? 1 : b));
 }
 
-// Forward fragments: element ((m NB + nb) 64 + lane) 8 + e holds
-// W[k = 32 m + 8 (lane / 16) + e][co = 16 nb + lane % 16], k = tap cin + ci.
-__global__ void packFwdKernel(const float* __restrict__ master, bf16* __restrict__ packed, int cin,
-                              int cout, int64_t total) {
-  const int nbt = cout / 16;
-  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
-       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int e = static_cast<int>(idx & 7);
-    const int lane = static_cast<int>((idx >> 3) & 63);
-    const int64_t blk = idx >> 9;
-    const int nb = static_cast<int>(blk % nbt);
-    const int64_t m = blk / nbt;
-    const int64_t k = 32 * m + 8 * (lane >> 4) + e;
-    const int co = 16 * nb + (lane & 15);
-    packed[idx] = k < 9LL * cin ? f2bf(master[k * cout + co]) : bf16(0);
-  }
+// Forward fragments, K walked in chunks of kChunk(cin) input channels:
+// element (((ch MSC + m) NB + nb) 64 + lane) 8 + e holds W[tap][ci][co] with
+// k = 32 m + 8 (lane / 16) + e = tap * chunk + c, ci = ch * chunk + c,
+// co = 16 nb + lane % 16 (zero for k >= 9 chunk).
+__device__ inline bf16 packedValue(const float* __restrict__ master, int cin, int cout, int64_t idx,
+                                   bool transposed) {
+  const int cinK = transposed ? cout : cin;   // K-side channels of this pass
+  const int coutN = transposed ? cin : cout;  // N-side channels of this pass
+  const int cc = kChunk(cinK);
+  const int msc = (9 * cc + 31) / 32;
+  const int nbt = coutN / 16;
+  const int e = static_cast<int>(idx & 7);
+  const int lane = static_cast<int>((idx >> 3) & 63);
+  const int64_t blk = idx >> 9;
+  const int nb = static_cast<int>(blk % nbt);
+  const int64_t qm = blk / nbt;
+  const int ch = static_cast<int>(qm / msc), m = static_cast<int>(qm % msc);
+  const int k = 32 * m + 8 * (lane >> 4) + e;
+  if (k >= 9 * cc) return 0;
+  const int tap = k / cc;
+  const int c = ch * cc + (k - tap * cc);  // K-side channel
+  const int o = 16 * nb + (lane & 15);     // N-side channel
+  // transposed conv: cin' = cout, cout' = cin, W'[tap'][c'][o'] = W[8 - tap'][o'][c']
+  return transposed ? f2bf(master[(static_cast<int64_t>(8 - tap) * cin + o) * cout + c])
+                    : f2bf(master[(static_cast<int64_t>(tap) * cin + c) * cout + o]);
 }
 
-// Transposed-conv fragments: cin' = cout, cout' = cin, W'[tap'][c'][o'] =
-// W[8 - tap'][o'][c'] (spatial flip, channel transpose).
-__global__ void packBwdKernel(const float* __restrict__ master, bf16* __restrict__ packed, int cin,
-                              int cout, int64_t total) {
-  const int cinT = cout, coutT = cin;
-  const int nbt = coutT / 16;
+__global__ void packKernel(const float* __restrict__ master, bf16* __restrict__ packed, int cin,
+                           int cout, int64_t total, int transposed) {
   for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
-       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int e = static_cast<int>(idx & 7);
-    const int lane = static_cast<int>((idx >> 3) & 63);
-    const int64_t blk = idx >> 9;
-    const int nb = static_cast<int>(blk % nbt);
-    const int64_t m = blk / nbt;
-    const int64_t k = 32 * m + 8 * (lane >> 4) + e;
-    const int o = 16 * nb + (lane & 15);  // forward input channel
-    bf16 v = 0;
-    if (k < 9LL * cinT) {
-      const int tap = static_cast<int>(k / cinT);
-      const int c = static_cast<int>(k - static_cast<int64_t>(tap) * cinT);  // forward output channel
-      v = f2bf(master[(static_cast<int64_t>(8 - tap) * cin + o) * cout + c]);
-    }
-    packed[idx] = v;
-  }
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    packed[idx] = packedValue(master, cin, cout, idx, transposed != 0);
 }
 
 // e0a = conv(x, 1 -> 16). x(y, x) = src[y srcW + x] * scale (inside srcH x srcW,
@@ -81,7 +73,7 @@ __global__ __launch_bounds__(256) void firstConvKernel(const float* __restrict__
                                                        const float* __restrict__ scale, int srcH,
                                                        int srcW, int64_t srcBatch,
                                                        const float* __restrict__ w,
-                                                       bf16* __restrict__ out, int n, int h,
+                                                       float* __restrict__ out, int n, int h,
                                                        int wd) {
   __shared__ float ws[144];
   if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
@@ -93,63 +85,51 @@ __global__ __launch_bounds__(256) void firstConvKernel(const float* __restrict__
     const int64_t t = p / wd;
     const int y = static_cast<int>(t % h);
     const int b = static_cast<int>(t / h);
-    float acc[16];
+    float v[9];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
-#pragma unroll
-    for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-      for (int kx = 0; kx < 3; ++kx) {
-        const int yy = y + ky - 1, xx = x + kx - 1;
-        float v = 0.f;
-        if (yy >= 0 && yy < srcH && xx >= 0 && xx < srcW) {
-          const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
-          v = src[si];
-          if (scale != nullptr) v *= scale[si];
-        }
-#pragma unroll
-        for (int c = 0; c < 16; ++c) acc[c] += v * ws[(ky * 3 + kx) * 16 + c];
+    for (int tap = 0; tap < 9; ++tap) {
+      const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+      v[tap] = 0.f;
+      if (yy >= 0 && yy < srcH && xx >= 0 && xx < srcW) {
+        const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
+        v[tap] = scale != nullptr ? src[si] * scale[si] : src[si];
       }
-    u32x4 lo, hi;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      lo[k] = static_cast<uint32_t>(f2bf(acc[2 * k])) | (static_cast<uint32_t>(f2bf(acc[2 * k + 1])) << 16);
-      hi[k] = static_cast<uint32_t>(f2bf(acc[8 + 2 * k])) |
-              (static_cast<uint32_t>(f2bf(acc[8 + 2 * k + 1])) << 16);
     }
-    u32x4* o = reinterpret_cast<u32x4*>(out + p * 16);
-    o[0] = lo;
-    o[1] = hi;
+    fx4* o = reinterpret_cast<fx4*>(out + p * 16);
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+      fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] += v[tap] * ws[tap * 16 + c4 * 4 + k];
+      o[c4] = acc;
+    }
   }
 }
 
 // y = conv(d0, 16 -> 1) at pixel (b, y, x).
-__device__ inline float lastConvAt(const bf16* __restrict__ d0, const float* ws, int b, int y, int x,
+__device__ inline float lastConvAt(const float* __restrict__ d0, const float* ws, int b, int y, int x,
                                    int h, int wd) {
   float acc = 0.f;
 #pragma unroll
-  for (int ky = 0; ky < 3; ++ky)
+  for (int tap = 0; tap < 9; ++tap) {
+    const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+    if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+    const fx4* src =
+        reinterpret_cast<const fx4*>(d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
 #pragma unroll
-    for (int kx = 0; kx < 3; ++kx) {
-      const int yy = y + ky - 1, xx = x + kx - 1;
-      if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
-      const u32x4* src =
-          reinterpret_cast<const u32x4*>(d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
-      const u32x4 v0 = src[0], v1 = src[1];
-      const float* wt = ws + (ky * 3 + kx) * 16;
+    for (int c4 = 0; c4 < 4; ++c4) {
+      const fx4 v = src[c4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        acc += __uint_as_float(v0[k] << 16) * wt[2 * k];
-        acc += __uint_as_float(v0[k] & 0xffff0000u) * wt[2 * k + 1];
-        acc += __uint_as_float(v1[k] << 16) * wt[8 + 2 * k];
-        acc += __uint_as_float(v1[k] & 0xffff0000u) * wt[8 + 2 * k + 1];
-      }
+      for (int k = 0; k < 4; ++k) acc += v[k] * ws[tap * 16 + c4 * 4 + k];
     }
+  }
   return acc;
 }
 
 // Training output: y (FP32, n x h x w).
-__global__ __launch_bounds__(256) void lastConvKernel(const bf16* __restrict__ d0,
+__global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ d0,
                                                       const float* __restrict__ w,
                                                       float* __restrict__ y, int n, int h, int wd) {
   __shared__ float ws[144];
@@ -167,7 +147,7 @@ __global__ __launch_bounds__(256) void lastConvKernel(const bf16* __restrict__ d
 // Preconditioner output on the solver grid (ny x nx inside the network's
 // h x w): z = scale * y, z[pin] = r[pin] scale[pin]^2, and block partials of
 // (z, q).
-__global__ __launch_bounds__(256) void lastConvApplyKernel(const bf16* __restrict__ d0,
+__global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restrict__ d0,
                                                            const float* __restrict__ w,
                                                            const float* __restrict__ scale,
                                                            const float* __restrict__ r,
@@ -194,7 +174,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const bf16* __restric
 // gd0 = conv^T(gy, 1 -> 16): gd0[p][c] = sum_tap gy[p - off(tap)] W[tap][c].
 __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict__ gy,
                                                          const float* __restrict__ w,
-                                                         bf16* __restrict__ gd0, int n, int h,
+                                                         float* __restrict__ gd0, int n, int h,
                                                          int wd) {
   __shared__ float ws[144];
   if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
@@ -206,38 +186,34 @@ __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict
     const int64_t t = p / wd;
     const int y = static_cast<int>(t % h);
     const int b = static_cast<int>(t / h);
-    float acc[16];
+    float g[9];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
-#pragma unroll
-    for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-      for (int kx = 0; kx < 3; ++kx) {
-        const int yy = y - ky + 1, xx = x - kx + 1;
-        if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
-        const float g = gy[(static_cast<int64_t>(b) * h + yy) * wd + xx];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) acc[c] += g * ws[(ky * 3 + kx) * 16 + c];
-      }
-    u32x4 lo, hi;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      lo[k] = static_cast<uint32_t>(f2bf(acc[2 * k])) | (static_cast<uint32_t>(f2bf(acc[2 * k + 1])) << 16);
-      hi[k] = static_cast<uint32_t>(f2bf(acc[8 + 2 * k])) |
-              (static_cast<uint32_t>(f2bf(acc[8 + 2 * k + 1])) << 16);
+    for (int tap = 0; tap < 9; ++tap) {
+      const int yy = y - tap / 3 + 1, xx = x - tap % 3 + 1;
+      g[tap] = (yy < 0 || yy >= h || xx < 0 || xx >= wd)
+                   ? 0.f
+                   : gy[(static_cast<int64_t>(b) * h + yy) * wd + xx];
     }
-    u32x4* o = reinterpret_cast<u32x4*>(gd0 + p * 16);
-    o[0] = lo;
-    o[1] = hi;
+    fx4* o = reinterpret_cast<fx4*>(gd0 + p * 16);
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+      fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] += g[tap] * ws[tap * 16 + c4 * 4 + k];
+      o[c4] = acc;
+    }
   }
 }
 
-// Weight gradient of a single-channel layer, 144 sums per block:
+// Weight gradient of a single-channel layer, 144 sums per block (BF16
+// operands like the MFMA weight gradients, FP32 sums):
 //   first (mode 0): dW[tap][c] = sum_p x[p + off(tap)] g16[p][c]
 //   last  (mode 1): dW[tap][c] = sum_p a16[p + off(tap)][c] gy[p]
-// x / gy are FP32 scalar images, g16 / a16 BF16 16-channel images.
+// x / gy are FP32 scalar images, g16 / a16 FP32 16-channel images.
 __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ s1,
-                                                      const bf16* __restrict__ m16, int mode,
+                                                      const float* __restrict__ m16, int mode,
                                                       int n, int h, int wd, float* partial) {
   float acc[144];
 #pragma unroll
@@ -250,40 +226,26 @@ __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ 
     const int y = static_cast<int>(t % h);
     const int b = static_cast<int>(t / h);
     if (mode == 0) {
-      const u32x4* gp = reinterpret_cast<const u32x4*>(m16 + p * 16);
-      const u32x4 g0 = gp[0], g1 = gp[1];
       float gv[16];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        gv[2 * k] = __uint_as_float(g0[k] << 16);
-        gv[2 * k + 1] = __uint_as_float(g0[k] & 0xffff0000u);
-        gv[8 + 2 * k] = __uint_as_float(g1[k] << 16);
-        gv[8 + 2 * k + 1] = __uint_as_float(g1[k] & 0xffff0000u);
-      }
+      for (int c = 0; c < 16; ++c) gv[c] = roundBf(m16[p * 16 + c]);
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
         if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
-        const float xv = s1[(static_cast<int64_t>(b) * h + yy) * wd + xx];
+        const float xv = roundBf(s1[(static_cast<int64_t>(b) * h + yy) * wd + xx]);
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += xv * gv[c];
       }
     } else {
-      const float g = s1[p];
+      const float g = roundBf(s1[p]);
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
         if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
-        const u32x4* ap = reinterpret_cast<const u32x4*>(
-            m16 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
-        const u32x4 a0 = ap[0], a1 = ap[1];
+        const float* ap = m16 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          acc[tap * 16 + 2 * k] += __uint_as_float(a0[k] << 16) * g;
-          acc[tap * 16 + 2 * k + 1] += __uint_as_float(a0[k] & 0xffff0000u) * g;
-          acc[tap * 16 + 8 + 2 * k] += __uint_as_float(a1[k] << 16) * g;
-          acc[tap * 16 + 8 + 2 * k + 1] += __uint_as_float(a1[k] & 0xffff0000u) * g;
-        }
+        for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += roundBf(ap[c]) * g;
       }
     }
   }
@@ -417,10 +379,10 @@ void Unet::repack() {
     if (l == kE0a || l == kOut) continue;
     const int ci = kLayers[l].cin, co = kLayers[l].cout;
     const int64_t nf = packedElems(ci, co), nb = packedElems(co, ci);
-    hipLaunchKernelGGL(packFwdKernel, grid1(nf), dim3(256), 0, stream, master.get() + layerOffset(l),
-                       fwdPack.get() + fwdOff[l], ci, co, nf);
-    hipLaunchKernelGGL(packBwdKernel, grid1(nb), dim3(256), 0, stream, master.get() + layerOffset(l),
-                       bwdPack.get() + bwdOff[l], ci, co, nb);
+    hipLaunchKernelGGL(packKernel, grid1(nf), dim3(256), 0, stream, master.get() + layerOffset(l),
+                       fwdPack.get() + fwdOff[l], ci, co, nf, 0);
+    hipLaunchKernelGGL(packKernel, grid1(nb), dim3(256), 0, stream, master.get() + layerOffset(l),
+                       bwdPack.get() + bwdOff[l], ci, co, nb, 1);
   }
   CFDHIP_CHECK(hipGetLastError());
 }
@@ -476,18 +438,13 @@ int64_t Unet::tensorElems(int t) const {
 void Unet::readTensor(int t, float* host, int64_t count) {
   const int64_t m = tensorElems(t);
   require(count == m, "tensor size mismatch");
-  std::vector<bf16> tmp(static_cast<size_t>(m));
-  CFDHIP_CHECK(hipMemcpyAsync(tmp.data(), act.get() + tensorOff[t], m * sizeof(bf16),
+  CFDHIP_CHECK(hipMemcpyAsync(host, act.get() + tensorOff[t], m * sizeof(float),
                               hipMemcpyDeviceToHost, stream));
   CFDHIP_CHECK(hipStreamSynchronize(stream));
-  for (int64_t i = 0; i < m; ++i) {
-    const uint32_t u = static_cast<uint32_t>(tmp[i]) << 16;
-    std::memcpy(host + i, &u, sizeof(float));
-  }
 }
 
-void Unet::conv(int layer, bool transposed, int gather, int epi, int split, const bf16* a,
-                const bf16* b, bf16* out, bf16* aux, int n, int h, int w) {
+void Unet::conv(int layer, bool transposed, int gather, int epi, int split, const float* a,
+                const float* b, float* out, float* aux, int n, int h, int w) {
   ConvArgs args{};
   args.a = a;
   args.b = b;
@@ -509,8 +466,8 @@ void Unet::conv(int layer, bool transposed, int gather, int epi, int split, cons
   CFDHIP_CHECK(hipGetLastError());
 }
 
-void Unet::convDw(int layer, int gx, int gg, const bf16* xa, const bf16* xb, const bf16* ga,
-                  const bf16* gb, int n, int h, int w) {
+void Unet::convDw(int layer, int gx, int gg, const float* xa, const float* xb, const float* ga,
+                  const float* gb, int n, int h, int w) {
   DwArgs a{};
   a.xa = xa;
   a.xb = xb;
@@ -618,7 +575,7 @@ void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
   singleDw(0, x, T(kTge0a), n, h, w, grad.get() + layerOffset(kE0a));
 }
 
-void Unet::singleDw(int mode, const float* s1, const bf16* m16, int n, int h, int w, float* out) {
+void Unet::singleDw(int mode, const float* s1, const float* m16, int n, int h, int w, float* out) {
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   const dim3 g = grid1(p0, 1024);
   dwPartial.ensure(static_cast<int64_t>(g.x) * 144);

@@ -14,10 +14,14 @@
 //   y   = conv(d0, 16->1)
 //
 // pool = 2x2 average, up = nearest x2, [a, b] = channel concatenation.
-// Activations: BF16, NHWC, batch-major. Master weights: FP32,
+// Activations: FP32, NHWC, batch-major. Master weights: FP32,
 // W[ky][kx][cin][cout] per layer, layers concatenated in the order above.
 // The MFMA layers run from BF16 copies packed in the operand order of
-// v_mfma_f32_16x16x32_bf16 (one 16-byte fragment per lane).
+// v_mfma_f32_16x16x32_bf16 (one 16-byte fragment per lane); activations enter
+// the MFMA as a hi + lo pair of BF16 values (two products per weight), which
+// keeps about 16 bits of each activation: the preconditioner's output must be
+// accurate in the A-norm, and BF16-rounded activations are not (their rounding
+// noise at the fine levels outweighs the smooth part of z on large grids).
 #pragma once
 
 #include <cstdint>
@@ -49,9 +53,14 @@ constexpr int64_t layerOffset(int l) {
 }
 constexpr int64_t kUnetParams = layerOffset(kNumLayers);
 
-// Packed MFMA fragments: ceil(9 cin / 32) K-steps x (cout / 16) N-blocks x
-// 64 lanes x 8 BF16.
-constexpr int64_t packedSteps(int cin) { return (9LL * cin + 31) / 32; }
+// The K dimension (9 taps x cin) is walked in chunks of kChunk(cin) input
+// channels (one LDS tile each); within a chunk k = tap * chunk + c in steps of
+// 32. Packed MFMA fragments: chunks x steps x (cout / 16) N-blocks x 64 lanes
+// x 8 BF16.
+constexpr int kChunk(int cin) { return cin == 48 ? 16 : cin > 32 ? 32 : cin; }
+constexpr int64_t packedSteps(int cin) {
+  return (cin / kChunk(cin)) * ((9LL * kChunk(cin) + 31) / 32);
+}
 constexpr int64_t packedElems(int cin, int cout) { return packedSteps(cin) * (cout / 16) * 512; }
 
 // Gathers applied while a tile of the conv input is staged in LDS.
@@ -68,21 +77,21 @@ enum Epilogue : int {
 };
 
 struct ConvArgs {
-  const bf16* a;
-  const bf16* b;
+  const float* a;
+  const float* b;
   const void* w;  // packed fragments of the layer
-  bf16* out;
-  bf16* aux;
+  float* out;
+  float* aux;
   int n, h, w_;   // batch, image size of the output (= input) grid
   int tilesX, tilesY;
   int nbTotal;    // N-blocks of the layer (cout / 16)
 };
 
 struct DwArgs {
-  const bf16* xa;
-  const bf16* xb;
-  const bf16* ga;
-  const bf16* gb;
+  const float* xa;
+  const float* xb;
+  const float* ga;
+  const float* gb;
   float* partial;  // [split][pair][ob][16 x 16]
   int n, h, w_;
   int tilesX, tilesY;

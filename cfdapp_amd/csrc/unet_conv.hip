@@ -1,18 +1,22 @@
-// BF16 MFMA convolutions of the learned preconditioner (gfx950).
+// MFMA convolutions of the learned preconditioner (gfx950).
 //
 // Forward and transposed 3x3 convolutions are one implicit GEMM: M = pixels
 // of an 8 x 32 output tile, N = output channels, K = 9 taps x input
-// channels, on v_mfma_f32_16x16x32_bf16 with FP32 accumulators. The input
-// tile (with its one-pixel halo, gathered through upsampling, concatenation or
-// the pool-gradient sum) is staged once in LDS; each wave owns two tile rows
-// (four 16-pixel M-blocks, each a 2 x 8 patch ordered so that every lane's
-// four accumulator rows form one 2 x 2 pixel block, which makes the fused
-// 2x2 pool and the nearest-upsample gradient lane-local). The packed weight
-// fragments stream from L2 (16 bytes per lane per K-step).
+// channels, on v_mfma_f32_16x16x32_bf16 with FP32 accumulators.
+//  * Activations are FP32 in HBM. The input tile (one-pixel halo, gathered
+//    through upsampling, concatenation or the pool-gradient sum) is staged in
+//    LDS one chunk of input channels at a time, split into hi = bf16(x) and
+//    lo = bf16(x - hi); every weight fragment meets both (two MFMAs).
+//  * The layer's BF16 weight fragments for the workgroup's output channels
+//    are staged in LDS once, beside the tile.
+//  * Each wave owns two tile rows: four 16-pixel M-blocks, each a 2 x 8 patch
+//    ordered so that a lane's four accumulator rows are one 2 x 2 pixel block;
+//    the 2x2 average pool and the nearest-upsample gradient (2x2 sum) are then
+//    lane-local in the epilogue.
 //
-// The weight gradient is a GEMM over pixels: D[ci][co] += X[p + tap][ci] G[p][co]
-// per (tap, 16-channel block), X and G tiles in LDS, partial sums per pixel
-// split written to memory and reduced in a fixed order.
+// The weight gradient is a GEMM over pixels, D[ci][co] += X[p + tap][ci] G[p][co]
+// per (tap, 16-channel block), with BF16 operands (gradients need no more),
+// X and G tiles in LDS, partial sums per pixel split reduced in a fixed order.
 #include "unet.hpp"
 
 namespace cfdhip {
@@ -22,9 +26,12 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf8v;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4v;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTH = 8, kTW = 32;
 constexpr int kInRows = kTH + 2, kInCols = kTW + 2;
+constexpr int kInPix = kInRows * kInCols;
 
 __device__ inline float bf2f(bf16 h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
@@ -37,57 +44,61 @@ __device__ inline bf16 f2bf(float f) {
   return static_cast<bf16>(u >> 16);
 }
 
-__device__ inline uint32_t pack2(float lo, float hi) {
-  return static_cast<uint32_t>(f2bf(lo)) | (static_cast<uint32_t>(f2bf(hi)) << 16);
-}
-
-// 8 channels starting at c of the gathered input at pixel (y, x), inside the image.
+// 4 channels starting at c of the gathered input at pixel (y, x) inside the image.
 template <int C, int CA, int G>
-__device__ inline u32x4 gather8(const bf16* __restrict__ A, const bf16* __restrict__ B, int n,
-                                int h, int w, int y, int x, int c) {
+__device__ inline fx4 gather4(const float* __restrict__ A, const float* __restrict__ B, int n,
+                              int h, int w, int y, int x, int c) {
   if constexpr (G == kGatherPlain) {
-    return *reinterpret_cast<const u32x4*>(A + ((static_cast<int64_t>(n) * h + y) * w + x) * C + c);
+    return *reinterpret_cast<const fx4*>(A + ((static_cast<int64_t>(n) * h + y) * w + x) * C + c);
   } else if constexpr (G == kGatherConcatUp) {
     if (c < CA) {
       const int hh = h >> 1, hw = w >> 1;
-      return *reinterpret_cast<const u32x4*>(
+      return *reinterpret_cast<const fx4*>(
           A + ((static_cast<int64_t>(n) * hh + (y >> 1)) * hw + (x >> 1)) * CA + c);
     }
-    return *reinterpret_cast<const u32x4*>(
+    return *reinterpret_cast<const fx4*>(
         B + ((static_cast<int64_t>(n) * h + y) * w + x) * (C - CA) + (c - CA));
   } else {
     const int hh = h >> 1, hw = w >> 1;
-    const u32x4 va =
-        *reinterpret_cast<const u32x4*>(A + ((static_cast<int64_t>(n) * h + y) * w + x) * C + c);
-    const u32x4 vb = *reinterpret_cast<const u32x4*>(
+    const fx4 va =
+        *reinterpret_cast<const fx4*>(A + ((static_cast<int64_t>(n) * h + y) * w + x) * C + c);
+    const fx4 vb = *reinterpret_cast<const fx4*>(
         B + ((static_cast<int64_t>(n) * hh + (y >> 1)) * hw + (x >> 1)) * C + c);
-    u32x4 r;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float lo = __uint_as_float(va[k] << 16) + 0.25f * __uint_as_float(vb[k] << 16);
-      const float hi =
-          __uint_as_float(va[k] & 0xffff0000u) + 0.25f * __uint_as_float(vb[k] & 0xffff0000u);
-      r[k] = pack2(lo, hi);
-    }
-    return r;
+    return va + 0.25f * vb;
   }
 }
 
-// Stages rows x cols pixels with origin (y0, x0) (zero outside the image) into
-// lds[(ty * cols + tx) * cs + c].
-template <int C, int CA, int G>
-__device__ inline void stageTile(bf16* lds, int cs, int rows, int cols, const bf16* A,
-                                 const bf16* B, int n, int h, int w, int y0, int x0) {
-  constexpr int C8 = C / 8;
-  const int items = rows * cols * C8;
+// Stages rows x cols pixels at origin (y0, x0) (zero outside the image),
+// channels [c0, c0 + CC) of the gathered input: hi[(ty cols + tx) cs + c] =
+// bf16(v), lo[...] = bf16(v - hi) (lo == nullptr: hi only).
+template <int C, int CA, int G, int CC>
+__device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, int rows, int cols, const float* A,
+                                  const float* B, int n, int h, int w, int y0, int x0, int c0) {
+  constexpr int C4 = CC / 4;
+  const int items = rows * cols * C4;
   for (int it = threadIdx.x; it < items; it += blockDim.x) {
-    const int c8 = it % C8;
-    const int px = it / C8;
+    const int c4 = it % C4;
+    const int px = it / C4;
     const int ty = px / cols, tx = px - ty * cols;
     const int y = y0 + ty, x = x0 + tx;
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (y >= 0 && y < h && x >= 0 && x < w) v = gather8<C, CA, G>(A, B, n, h, w, y, x, c8 * 8);
-    *reinterpret_cast<u32x4*>(lds + (ty * cols + tx) * cs + c8 * 8) = v;
+    fx4 v = {0.f, 0.f, 0.f, 0.f};
+    if (y >= 0 && y < h && x >= 0 && x < w) v = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+    u32x2 vh, vl;
+    bf16 hb[4], lb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      hb[k] = f2bf(v[k]);
+      lb[k] = f2bf(v[k] - bf2f(hb[k]));
+    }
+    vh[0] = static_cast<uint32_t>(hb[0]) | (static_cast<uint32_t>(hb[1]) << 16);
+    vh[1] = static_cast<uint32_t>(hb[2]) | (static_cast<uint32_t>(hb[3]) << 16);
+    const int o = (ty * cols + tx) * cs + c4 * 4;
+    *reinterpret_cast<u32x2*>(hi + o) = vh;
+    if (lo != nullptr) {
+      vl[0] = static_cast<uint32_t>(lb[0]) | (static_cast<uint32_t>(lb[1]) << 16);
+      vl[1] = static_cast<uint32_t>(lb[2]) | (static_cast<uint32_t>(lb[3]) << 16);
+      *reinterpret_cast<u32x2*>(lo + o) = vl;
+    }
   }
 }
 
@@ -96,15 +107,26 @@ __device__ inline f4v mfma(u32x4 a, u32x4 b, f4v c) {
                                                  __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
 }
 
+template <int A, int B>
+constexpr int cmax() {
+  return A > B ? A : B;
+}
+
 template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
 __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
-  constexpr int CS = CIN + 8;
-  constexpr int IN_ELEMS = kInRows * kInCols * CS;
-  constexpr int SS = NBG * 16 + 8;
-  constexpr int OUT_ELEMS = kTH * kTW * SS;
-  constexpr int ELEMS = IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS;
-  constexpr int MS = (9 * CIN + 31) / 32;
-  __shared__ __attribute__((aligned(16))) bf16 lds[ELEMS];
+  constexpr int CC = kChunk(CIN);
+  constexpr int NCH = CIN / CC;
+  constexpr int MSC = (9 * CC + 31) / 32;
+  constexpr int CS = CC + 8;  // LDS channel stride (BF16 elements), 16-byte rows
+  constexpr int TILE_E = kInPix * CS;
+  constexpr int W_FRAGS = NCH * MSC * NBG * 64;
+  constexpr int SS = NBG * 16 + 4;  // FP32 output staging stride
+  constexpr int REGION = cmax<2 * TILE_E * 2, kTH * kTW * SS * 4>();
+  __shared__ u32x4 wl[W_FRAGS];
+  __shared__ __attribute__((aligned(16))) unsigned char region[REGION];
+  bf16* const hiT = reinterpret_cast<bf16*>(region);
+  bf16* const loT = hiT + TILE_E;
+  float* const stage = reinterpret_cast<float*>(region);
 
   int tile = blockIdx.x;
   const int tx = tile % a.tilesX;
@@ -114,8 +136,14 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   const int y0 = ty * kTH, x0 = tx * kTW;
   const int nb0 = blockIdx.y * NBG;
 
-  stageTile<CIN, CA, G>(lds, CS, kInRows, kInCols, a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1);
-  __syncthreads();
+  // this group's weight fragments: packed [(chunk step) * nbTotal + nb] * 64 + lane
+  const u32x4* __restrict__ wp = reinterpret_cast<const u32x4*>(a.w);
+  for (int i = threadIdx.x; i < W_FRAGS; i += blockDim.x) {
+    const int lane = i & 63;
+    const int t = i >> 6;
+    const int nb = t % NBG, qm = t / NBG;
+    wl[i] = wp[(static_cast<int64_t>(qm) * a.nbTotal + nb0 + nb) * 64 + lane];
+  }
 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int li = lane & 15, g = lane >> 4;
@@ -128,31 +156,43 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   for (int q = 0; q < 4; ++q)
 #pragma unroll
     for (int nb = 0; nb < NBG; ++nb) acc[q][nb] = f4v{0.f, 0.f, 0.f, 0.f};
-  const u32x4* __restrict__ wp = reinterpret_cast<const u32x4*>(a.w);
-  for (int m = 0; m < MS; ++m) {
-    const int kk = 32 * m + 8 * g;
-    u32x4 af[4];
-    if (kk < 9 * CIN) {
-      const int tap = kk / CIN;
-      const int c0 = kk - tap * CIN;
-      const int ky = tap / 3, kx = tap - 3 * ky;
-      const bf16* base = lds + ((ayl + ky) * kInCols + axl + kx) * CS + c0;
+
+  for (int ch = 0; ch < NCH; ++ch) {
+    if (ch > 0) __syncthreads();  // the previous chunk is consumed
+    stageSplit<CIN, CA, G, CC>(hiT, loT, CS, kInRows, kInCols, a.a, a.b, n, a.h, a.w_, y0 - 1,
+                               x0 - 1, ch * CC);
+    __syncthreads();
+#pragma unroll 2
+    for (int m = 0; m < MSC; ++m) {
+      const int kk = 32 * m + 8 * g;
+      u32x4 ah[4], al[4];
+      if (kk < 9 * CC) {
+        const int tap = kk / CC;
+        const int c0 = kk - tap * CC;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        const int base = ((ayl + ky) * kInCols + axl + kx) * CS + c0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) af[q] = *reinterpret_cast<const u32x4*>(base + 8 * q * CS);
-    } else {
+        for (int q = 0; q < 4; ++q) {
+          ah[q] = *reinterpret_cast<const u32x4*>(hiT + base + 8 * q * CS);
+          al[q] = *reinterpret_cast<const u32x4*>(loT + base + 8 * q * CS);
+        }
+      } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) af[q] = u32x4{0u, 0u, 0u, 0u};
+        for (int q = 0; q < 4; ++q) ah[q] = al[q] = u32x4{0u, 0u, 0u, 0u};
+      }
+      const int wbase = ((ch * MSC + m) * NBG) * 64 + lane;
+#pragma unroll
+      for (int nb = 0; nb < NBG; ++nb) {
+        const u32x4 bw = wl[wbase + nb * 64];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc[q][nb] = mfma(ah[q], bw, acc[q][nb]);
+          acc[q][nb] = mfma(al[q], bw, acc[q][nb]);
+        }
+      }
     }
-    u32x4 bw[NBG];
-#pragma unroll
-    for (int nb = 0; nb < NBG; ++nb)
-      bw[nb] = wp[(static_cast<int64_t>(m) * a.nbTotal + nb0 + nb) * 64 + lane];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int nb = 0; nb < NBG; ++nb) acc[q][nb] = mfma(af[q], bw[nb], acc[q][nb]);
   }
-  __syncthreads();  // the input tile is dead; its LDS becomes the output staging
+  __syncthreads();  // the tile is dead; its LDS becomes the output staging
 
   // Lane holds D[4 g + r][li]: pixel (2 wv + (r >> 1), 8 q + 2 g + (r & 1)),
   // channel 16 (nb0 + nb) + li.
@@ -166,11 +206,10 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
       const int px = (x0 >> 1) + 4 * q + g;
 #pragma unroll
       for (int nb = 0; nb < NBG; ++nb) {
-        float s = acc[q][nb][0] + acc[q][nb][1] + acc[q][nb][2] + acc[q][nb][3];
+        float s = (acc[q][nb][0] + acc[q][nb][1]) + (acc[q][nb][2] + acc[q][nb][3]);
         if (E == kEpiStorePool) s *= 0.25f;
         if (py < hh && px < hw)
-          a.aux[((static_cast<int64_t>(n) * hh + py) * hw + px) * AUXC + 16 * (nb0 + nb) + li] =
-              f2bf(s);
+          a.aux[((static_cast<int64_t>(n) * hh + py) * hw + px) * AUXC + 16 * (nb0 + nb) + li] = s;
       }
     }
   }
@@ -182,21 +221,21 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int yl = 2 * wv + (r >> 1), xl = 8 * q + 2 * g + (r & 1);
-          lds[(yl * kTW + xl) * SS + 16 * nb + li] = f2bf(acc[q][nb][r]);
+          stage[(yl * kTW + xl) * SS + 16 * nb + li] = acc[q][nb][r];
         }
     __syncthreads();
     constexpr int OUTC = E == kEpiSplitUp ? COUT - SPLIT : COUT;
     const int cOff = E == kEpiSplitUp ? nb0 * 16 - SPLIT : nb0 * 16;
-    constexpr int G8 = NBG * 2;
-    for (int it = threadIdx.x; it < kTH * kTW * G8; it += blockDim.x) {
-      const int c8 = it % G8;
-      const int px = it / G8;
+    constexpr int G4 = NBG * 4;
+    for (int it = threadIdx.x; it < kTH * kTW * G4; it += blockDim.x) {
+      const int c4 = it % G4;
+      const int px = it / G4;
       const int yl = px / kTW, xl = px - yl * kTW;
       const int y = y0 + yl, x = x0 + xl;
       if (y < a.h && x < a.w_)
-        *reinterpret_cast<u32x4*>(a.out + ((static_cast<int64_t>(n) * a.h + y) * a.w_ + x) * OUTC +
-                                  cOff + c8 * 8) =
-            *reinterpret_cast<const u32x4*>(lds + px * SS + c8 * 8);
+        *reinterpret_cast<fx4*>(a.out + ((static_cast<int64_t>(n) * a.h + y) * a.w_ + x) * OUTC +
+                                cOff + c4 * 4) =
+            *reinterpret_cast<const fx4*>(stage + px * SS + c4 * 4);
     }
   }
 }
@@ -215,7 +254,7 @@ template <int CIN, int CA, int COUT, int GX, int GG, int PPW>
 __global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
   constexpr int CS = CIN + 8, GS = COUT + 8;
   constexpr int NCB = CIN / 16, NOB = COUT / 16, NPAIR = 9 * NCB;
-  __shared__ __attribute__((aligned(16))) bf16 xs[kInRows * kInCols * CS];
+  __shared__ __attribute__((aligned(16))) bf16 xs[kInPix * CS];
   __shared__ __attribute__((aligned(16))) bf16 gs[kTH * kTW * GS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int li = lane & 15, g = lane >> 4;
@@ -242,8 +281,10 @@ __global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
     const int n = static_cast<int>(tile / a.tilesY);
     const int y0 = ty * kTH, x0 = tx * kTW;
     __syncthreads();
-    stageTile<CIN, CA, GX>(xs, CS, kInRows, kInCols, a.xa, a.xb, n, a.h, a.w_, y0 - 1, x0 - 1);
-    stageTile<COUT, COUT, GG>(gs, GS, kTH, kTW, a.ga, a.gb, n, a.h, a.w_, y0, x0);
+    stageSplit<CIN, CA, GX, CIN>(xs, nullptr, CS, kInRows, kInCols, a.xa, a.xb, n, a.h, a.w_,
+                                 y0 - 1, x0 - 1, 0);
+    stageSplit<COUT, COUT, GG, COUT>(gs, nullptr, GS, kTH, kTW, a.ga, a.gb, n, a.h, a.w_, y0, x0,
+                                     0);
     __syncthreads();
     for (int s = 0; s < kTH; ++s) {
       u32x4 bfr[NOB];
@@ -292,9 +333,10 @@ int dw(hipStream_t s, const DwArgs& a, int splits) {
 
 }  // namespace
 
-// Every (cin, ca, cout, gather, epilogue, split) the network uses; the N-group
-// width keeps the accumulators at <= 64 registers and never straddles the
-// split of a transposed concat layer.
+// Every (cin, ca, cout, gather, epilogue, split) the network uses. The
+// N-group width (NBG 16-channel blocks per workgroup) keeps the staged weights
+// and the accumulators small and never straddles the split of a transposed
+// concat layer.
 void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, int split,
                 const ConvArgs& a) {
 #define CFDHIP_CONV(CI, CAV, CO, NBG, G, E, SP)                                            \
@@ -307,10 +349,10 @@ void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, i
   CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStorePool, 0)
   CFDHIP_CONV(16, 16, 32, 2, kGatherPlain, kEpiStore, 0)
   CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStorePool, 0)
-  CFDHIP_CONV(32, 32, 64, 4, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(64, 64, 64, 4, kGatherPlain, kEpiStorePool, 0)
-  CFDHIP_CONV(64, 64, 64, 4, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(128, 64, 64, 4, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 64, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(64, 64, 64, 2, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(64, 64, 64, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(128, 64, 64, 2, kGatherConcatUp, kEpiStore, 0)
   CFDHIP_CONV(96, 64, 32, 2, kGatherConcatUp, kEpiStore, 0)
   CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStore, 0)
   CFDHIP_CONV(48, 32, 16, 1, kGatherConcatUp, kEpiStore, 0)
@@ -318,8 +360,8 @@ void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, i
   // transposed (gradient of the input)
   CFDHIP_CONV(16, 16, 48, 1, kGatherPlain, kEpiSplitUp, 32)
   CFDHIP_CONV(32, 32, 96, 2, kGatherPlain, kEpiSplitUp, 64)
-  CFDHIP_CONV(64, 64, 128, 4, kGatherPlain, kEpiSplitUp, 64)
-  CFDHIP_CONV(64, 64, 64, 4, kGatherSumUp, kEpiStore, 0)
+  CFDHIP_CONV(64, 64, 128, 2, kGatherPlain, kEpiSplitUp, 64)
+  CFDHIP_CONV(64, 64, 64, 2, kGatherSumUp, kEpiStore, 0)
   CFDHIP_CONV(64, 64, 32, 2, kGatherPlain, kEpiStore, 0)
   CFDHIP_CONV(32, 32, 32, 2, kGatherSumUp, kEpiStore, 0)
   CFDHIP_CONV(32, 32, 16, 1, kGatherPlain, kEpiStore, 0)

@@ -5,7 +5,7 @@
 
 namespace cfdhip {
 
-// Activation and gradient tensors (BF16, NHWC), in allocation order: the
+// Activation and gradient tensors (FP32, NHWC), in allocation order: the
 // forward tensors first, so an inference-only shape allocates only those.
 enum UnetTensor : int {
   kTe0a = 0, kTe0, kTp0, kTe1a, kTe1, kTp1, kTe2a, kTe2, kTp2, kTb3a, kTb3,
@@ -51,7 +51,7 @@ class Unet {
   void adam(float lr, float b1, float b2, float eps);
 
   float* output() { return y.get(); }
-  // BF16 tensor t of the current shape, widened to FP32 on the host.
+  // Tensor t of the current shape, copied to the host.
   int64_t tensorElems(int t) const;
   void readTensor(int t, float* host, int64_t count);
   float* outputGrad() { return gy.get(); }
@@ -59,16 +59,17 @@ class Unet {
 
  private:
   void repack();
-  bf16* T(int t) { return act.get() + tensorOff[t]; }
-  void conv(int layer, bool transposed, int gather, int epi, int split, const bf16* a,
-            const bf16* b, bf16* out, bf16* aux, int n, int h, int w);
-  void convDw(int layer, int gx, int gg, const bf16* xa, const bf16* xb, const bf16* ga,
-              const bf16* gb, int n, int h, int w);
-  void singleDw(int mode, const float* s1, const bf16* m16, int n, int h, int w, float* out);
+  float* T(int t) { return act.get() + tensorOff[t]; }
+  void conv(int layer, bool transposed, int gather, int epi, int split, const float* a,
+            const float* b, float* out, float* aux, int n, int h, int w);
+  void convDw(int layer, int gx, int gg, const float* xa, const float* xb, const float* ga,
+              const float* gb, int n, int h, int w);
+  void singleDw(int mode, const float* s1, const float* m16, int n, int h, int w, float* out);
 
   DeviceArray<float> master, grad, adamM, adamV, y, gy, dwPartial;
   DeviceArray<double> lossPartial;
-  DeviceArray<bf16> fwdPack, bwdPack, act;
+  DeviceArray<bf16> fwdPack, bwdPack;
+  DeviceArray<float> act;
   int64_t fwdOff[kNumLayers] = {};
   int64_t bwdOff[kNumLayers] = {};
   int64_t tensorOff[kNumTensors] = {};

@@ -174,10 +174,11 @@ def backward(w, t, gy):
 
 
 # ---------------------------------------------------------------------------
-# BF16 storage emulation: the device keeps weights and activations in BF16
-# (round to nearest even) and accumulates in FP32. These variants round at
-# exactly the points where the device stores, so the remaining difference is
-# the FP32 accumulation order.
+# Device arithmetic emulation. The device keeps weights in BF16 (round to
+# nearest even) and activations in FP32; a convolution's input enters the MFMA
+# as hi + lo BF16 halves (split2), the weight gradients use BF16 operands, and
+# everything accumulates in FP32. These variants round at exactly those points,
+# so what remains against the device is the FP32 accumulation order.
 
 def bf(a):
     """Round to BF16 (nearest even) and back to float64."""
@@ -187,66 +188,83 @@ def bf(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def forward_bf16(w, x):
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def split2(a):
+    """The hi + lo BF16 pair an FP32 activation becomes in the MFMA."""
+    a = f32(a)
+    hi = bf(a)
+    return hi + bf(a - hi)
+
+
+def forward_device(w, x):
     """Forward pass with device rounding: returns (y, tape)."""
     wb = {k: bf(v) for k, v in w.items()}
     t = {"x": np.asarray(x, dtype=np.float64)}
 
-    def conv_pool(name, inp):
-        acc = conv3x3(inp, wb[name])
-        return bf(acc), bf(pool(acc))
+    def conv(name, inp):
+        return conv3x3(split2(inp), wb[name])
 
-    t["e0a"] = bf(conv3x3(t["x"], wb["e0a"]))
+    def conv_pool(name, inp):
+        acc = conv(name, inp)
+        return f32(acc), f32(pool(acc))
+
+    t["e0a"] = f32(conv3x3(f32(t["x"]), wb["e0a"]))
     t["e0"], t["p0"] = conv_pool("e0b", t["e0a"])
-    t["e1a"] = bf(conv3x3(t["p0"], wb["e1a"]))
+    t["e1a"] = f32(conv("e1a", t["p0"]))
     t["e1"], t["p1"] = conv_pool("e1b", t["e1a"])
-    t["e2a"] = bf(conv3x3(t["p1"], wb["e2a"]))
+    t["e2a"] = f32(conv("e2a", t["p1"]))
     t["e2"], t["p2"] = conv_pool("e2b", t["e2a"])
-    t["b3a"] = bf(conv3x3(t["p2"], wb["b3a"]))
-    t["b3"] = bf(conv3x3(t["b3a"], wb["b3b"]))
+    t["b3a"] = f32(conv("b3a", t["p2"]))
+    t["b3"] = f32(conv("b3b", t["b3a"]))
     t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
-    t["d2a"] = bf(conv3x3(t["c2"], wb["d2a"]))
-    t["d2"] = bf(conv3x3(t["d2a"], wb["d2b"]))
+    t["d2a"] = f32(conv("d2a", t["c2"]))
+    t["d2"] = f32(conv("d2b", t["d2a"]))
     t["c1"] = np.concatenate([up(t["d2"]), t["e1"]], axis=3)
-    t["d1a"] = bf(conv3x3(t["c1"], wb["d1a"]))
-    t["d1"] = bf(conv3x3(t["d1a"], wb["d1b"]))
+    t["d1a"] = f32(conv("d1a", t["c1"]))
+    t["d1"] = f32(conv("d1b", t["d1a"]))
     t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
-    t["d0a"] = bf(conv3x3(t["c0"], wb["d0a"]))
-    t["d0"] = bf(conv3x3(t["d0a"], wb["d0b"]))
-    t["y"] = conv3x3(t["d0"], wb["out"])
+    t["d0a"] = f32(conv("d0a", t["c0"]))
+    t["d0"] = f32(conv("d0b", t["d0a"]))
+    t["y"] = f32(conv3x3(t["d0"], wb["out"]))
     return t["y"], t
 
 
-def backward_bf16(w, t, gy):
-    """Weight gradients with device rounding of the stored gradient tensors."""
+def backward_device(w, t, gy):
+    """Weight gradients with device rounding (FP32 gradient tensors, split
+    inputs to the transposed convolutions, BF16 operands in the weight sums)."""
     wb = {k: bf(v) for k, v in w.items()}
     dw = {}
+    gy = f32(gy)
 
     def back(name, inp, g):
-        dw[name] = conv3x3_weight_grad(inp, g)
-        return conv3x3_input_grad(g, wb[name])
+        dw[name] = conv3x3_weight_grad(bf(inp), bf(g))
+        return conv3x3_input_grad(split2(g), wb[name])
 
     def split(gc, c):  # transposed concat layer: coarse part summed 2x2, skip part
-        return bf(up_grad(gc[..., :c])), bf(gc[..., c:])
+        return f32(up_grad(gc[..., :c])), f32(gc[..., c:])
 
-    gd0 = bf(back("out", t["d0"], gy))
-    gd0a = bf(back("d0b", t["d0a"], gd0))
+    dw["out"] = conv3x3_weight_grad(bf(t["d0"]), bf(gy))
+    gd0 = f32(conv3x3_input_grad(gy, wb["out"]))
+    gd0a = f32(back("d0b", t["d0a"], gd0))
     gd1, ge0s = split(back("d0a", t["c0"], gd0a), t["d1"].shape[3])
-    gd1a = bf(back("d1b", t["d1a"], gd1))
+    gd1a = f32(back("d1b", t["d1a"], gd1))
     gd2, ge1s = split(back("d1a", t["c1"], gd1a), t["d2"].shape[3])
-    gd2a = bf(back("d2b", t["d2a"], gd2))
+    gd2a = f32(back("d2b", t["d2a"], gd2))
     gb3, ge2s = split(back("d2a", t["c2"], gd2a), t["b3"].shape[3])
-    gb3a = bf(back("b3b", t["b3a"], gb3))
-    gp2 = bf(back("b3a", t["p2"], gb3a))
-    ge2 = bf(ge2s + pool_grad(gp2))
-    ge2a = bf(back("e2b", t["e2a"], ge2))
-    gp1 = bf(back("e2a", t["p1"], ge2a))
-    ge1 = bf(ge1s + pool_grad(gp1))
-    ge1a = bf(back("e1b", t["e1a"], ge1))
-    gp0 = bf(back("e1a", t["p0"], ge1a))
-    ge0 = bf(ge0s + pool_grad(gp0))
-    ge0a = bf(back("e0b", t["e0a"], ge0))
-    dw["e0a"] = conv3x3_weight_grad(t["x"], ge0a)
+    gb3a = f32(back("b3b", t["b3a"], gb3))
+    gp2 = f32(back("b3a", t["p2"], gb3a))
+    ge2 = f32(ge2s + pool_grad(gp2))
+    ge2a = f32(back("e2b", t["e2a"], ge2))
+    gp1 = f32(back("e2a", t["p1"], ge2a))
+    ge1 = f32(ge1s + pool_grad(gp1))
+    ge1a = f32(back("e1b", t["e1a"], ge1))
+    gp0 = f32(back("e1a", t["p0"], ge1a))
+    ge0 = f32(ge0s + pool_grad(gp0))
+    ge0a = f32(back("e0b", t["e0a"], ge0))
+    dw["e0a"] = conv3x3_weight_grad(bf(t["x"]), bf(ge0a))
     return dw
 
 
