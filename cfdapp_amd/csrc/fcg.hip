@@ -1,17 +1,23 @@
 // Flexible CG with the learned preconditioner on the 2D p' operator.
 //
-// Inner solve (FP32 vectors, FP64 reductions), truncated flexible CG with one
-// stored direction (Notay's FCG(1)):
+// Inner solve (vectors in FP64, or FP32 with precision CFDHIP_MIXED; FP64
+// reductions), truncated flexible CG with one stored direction (Notay's
+// FCG(1)):
 //   z = M r;  beta = -(z, q_old) / (p_old, q_old);  p = z + beta p_old
 //   q = A p;  alpha = (p, r) / (p, q);  d += alpha p;  r -= alpha q
-// with M r = D^-1/2 N(D^-1/2 r), N the network, and z = r / a_P at the
-// pinned reference cell. Outer loop (FP64): r = b - A x, inner solve for the
-// correction, x += d, until the FP64 true residual meets the tolerance
-// relative to |b - A x0|. An inner solve whose residual has not reached a new
+// with M r = D^-1/2 N(D^-1/2 r), N the network (FP32 activations, BF16
+// weights), and z = r / a_P at the pinned reference cell. Outer loop (FP64):
+// r = b - A x, inner solve for the correction, x += d, until the FP64 true
+// residual meets the tolerance relative to |b - A x0|. FP32 inner vectors lose
+// A-orthogonality fast enough to cost more iterations than their bandwidth
+// saves (measured: 1024^2 p' system, 1035 FP32 iterations with restarts every
+// 1e-4 versus 3497 without), so they restart every 1e-4 of reduction; FP64
+// vectors run the inner solve to the final tolerance in one cycle. An inner solve whose residual has not reached a new
 // minimum for `stagnation_window` iterations hands the solve to FP64
 // Jacobi-PCG from the current x (the fallback, reported in fallback_used).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "reduce.hpp"
@@ -23,10 +29,11 @@ namespace {
 
 constexpr int kMaxParts = 2048;
 
-struct Op32 {
+template <class T>
+struct OpT {
   int64_t nx, ny, pin, offY;
-  const float* diag;
-  const float* coeff;
+  const T* diag;
+  const T* coeff;
 };
 
 inline dim3 stride(int64_t n) {
@@ -35,33 +42,34 @@ inline dim3 stride(int64_t n) {
   return dim3(static_cast<unsigned>(b < 1 ? 1 : b));
 }
 
-__device__ inline float pAt(const float* __restrict__ z, const float* __restrict__ pOld, float beta,
-                            bool first, int64_t c) {
+template <class T>
+__device__ inline T pAt(const T* __restrict__ z, const T* __restrict__ pOld, T beta, bool first,
+                        int64_t c) {
   return first ? z[c] : z[c] + beta * pOld[c];
 }
 
 // p = z + beta pOld; q = A p; partials (p, q), (p, r).
-__global__ __launch_bounds__(256) void fcgDirSpmvKernel(Op32 op, const float* __restrict__ z,
-                                                        const float* __restrict__ pOld,
-                                                        float* __restrict__ pNew,
-                                                        float* __restrict__ q,
-                                                        const float* __restrict__ r,
+template <class T>
+__global__ __launch_bounds__(256) void fcgDirSpmvKernel(OpT<T> op, const T* __restrict__ z,
+                                                        const T* __restrict__ pOld,
+                                                        T* __restrict__ pNew, T* __restrict__ q,
+                                                        const T* __restrict__ r,
                                                         const KState* __restrict__ st, int first,
                                                         double* partial) {
   double acc[2] = {0.0, 0.0};
   if (!st->done) {
-    const float beta = static_cast<float>(st->beta);
+    const T beta = static_cast<T>(st->beta);
     const bool f = first != 0;
     const int64_t n = op.nx * op.ny;
     for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
       const int64_t i = c % op.nx, j = c / op.nx;
-      const float pc = pAt(z, pOld, beta, f, c);
-      float y;
+      const T pc = pAt(z, pOld, beta, f, c);
+      T y;
       if (c == op.pin) {
-        y = 0.f + op.diag[c] * pc;
+        y = T(0) + op.diag[c] * pc;
       } else {
-        y = 0.f;
+        y = T(0);
         if (j > 0) y += (-op.coeff[op.offY + j * op.nx + i]) * pAt(z, pOld, beta, f, c - op.nx);
         if (i > 0) y += (-op.coeff[j * (op.nx + 1) + i]) * pAt(z, pOld, beta, f, c - 1);
         y += op.diag[c] * pc;
@@ -79,18 +87,19 @@ __global__ __launch_bounds__(256) void fcgDirSpmvKernel(Op32 op, const float* __
 }
 
 // d += alpha p; r -= alpha q; partial |r|^2.
-__global__ __launch_bounds__(256) void fcgUpdateKernel(int64_t n, const float* __restrict__ p,
-                                                       const float* __restrict__ q,
-                                                       float* __restrict__ d, float* __restrict__ r,
+template <class T>
+__global__ __launch_bounds__(256) void fcgUpdateKernel(int64_t n, const T* __restrict__ p,
+                                                       const T* __restrict__ q, T* __restrict__ d,
+                                                       T* __restrict__ r,
                                                        const KState* __restrict__ st,
                                                        double* partial) {
   double acc[1] = {0.0};
   if (!st->done) {
-    const float alpha = static_cast<float>(st->alpha);
+    const T alpha = static_cast<T>(st->alpha);
     for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
       d[c] += alpha * p[c];
-      const float rc = r[c] - alpha * q[c];
+      const T rc = r[c] - alpha * q[c];
       r[c] = rc;
       acc[0] += static_cast<double>(rc) * rc;
     }
@@ -98,7 +107,8 @@ __global__ __launch_bounds__(256) void fcgUpdateKernel(int64_t n, const float* _
   blockPartials<1>(acc, partial);
 }
 
-__global__ void normKernel32(int64_t n, const float* __restrict__ r, double* partial) {
+template <class T>
+__global__ void normKernelT(int64_t n, const T* __restrict__ r, double* partial) {
   double acc[1] = {0.0};
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -119,26 +129,29 @@ __global__ void zeroKernelD64(int64_t n, double* a) {
     a[c] = 0.0;
 }
 
-__global__ void zeroF(int64_t n, float* a) {
+template <class T>
+__global__ void zeroT(int64_t n, T* a) {
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    a[c] = 0.f;
+    a[c] = T(0);
 }
 
-__global__ void addCorrection(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
+template <class T>
+__global__ void addCorrection(int64_t n, const T* __restrict__ d, double* __restrict__ x) {
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x)
     x[c] += static_cast<double>(d[c]);
 }
 
-// r64 = b - A x (FP64) -> r32, |r64|^2 partials.
+// r64 = b - A x (FP64) -> rOut (FP32 or FP64), |r64|^2 partials.
+template <class T>
 __global__ __launch_bounds__(256) void outerResidualKernel(int64_t nx, int64_t ny, int64_t pin,
                                                            int64_t offY,
                                                            const double* __restrict__ diag,
                                                            const double* __restrict__ coeff,
                                                            const double* __restrict__ b,
                                                            const double* __restrict__ x,
-                                                           float* __restrict__ r32,
+                                                           T* __restrict__ r32,
                                                            double* partial) {
   const int64_t n = nx * ny;
   double acc[1] = {0.0};
@@ -157,7 +170,7 @@ __global__ __launch_bounds__(256) void outerResidualKernel(int64_t nx, int64_t n
       if (j < ny - 1) y += (-coeff[offY + (j + 1) * nx + i]) * x[c + nx];
     }
     const double rc = b[c] - y;
-    r32[c] = static_cast<float>(rc);
+    r32[c] = static_cast<T>(rc);
     acc[0] += rc * rc;
   }
   blockPartials<1>(acc, partial);
@@ -170,26 +183,27 @@ struct InnerResult {
 };
 
 // The preconditioner: z = M r on the solver grid (ny x nx), network grid h x w.
-void applyM(Context& ctx, const float* r, const float* q, float* z, double* partial, int parts,
-            int h, int w) {
+template <class T>
+void applyM(Context& ctx, const T* r, const T* q, T* z, double* partial, int parts, int h, int w) {
   const View& v = ctx.view;
   Unet& net = *ctx.net;
-  net.forward(r, ctx.kw.dis32.get(), static_cast<int>(v.ny), static_cast<int>(v.nx), 0, 1, h, w);
-  net.outputApply(ctx.kw.dis32.get(), r, q, z, static_cast<int>(v.ny), static_cast<int>(v.nx),
-                  ctx.pin, h, w, partial, parts);
+  net.forwardFrom(r, std::is_same_v<T, double>, ctx.kw.dis32.get(), static_cast<int>(v.ny),
+                  static_cast<int>(v.nx), 0, 1, h, w);
+  net.outputApply<T>(ctx.kw.dis32.get(), r, q, z, static_cast<int>(v.ny), static_cast<int>(v.nx),
+                     ctx.pin, h, w, partial, parts);
 }
 
-InnerResult innerFcg(Context& ctx, double relTol, int64_t maxIter, int64_t window, float* r,
-                     float* d, std::vector<double>& history) {
+template <class T>
+InnerResult innerFcg(Context& ctx, const OpT<T>& op, double relTol, int64_t maxIter,
+                     int64_t window, T* r, T* d, T* const work[4], std::vector<double>& history) {
   const View& v = ctx.view;
   const int64_t n = v.n;
   KrylovWork& w = ctx.kw;
   const int h = static_cast<int>((v.ny + 7) / 8 * 8), wd = static_cast<int>((v.nx + 7) / 8 * 8);
   ctx.net->ensureShape(1, h, wd, false);
-  float* z = w.f8[0].get();
-  float* q = w.f8[1].get();
-  float* p[2] = {w.f8[2].get(), w.f8[3].get()};
-  const Op32 op{v.nx, v.ny, ctx.pin, v.offY, w.dinv32.get(), w.w32.get()};
+  T* z = work[0];
+  T* q = work[1];
+  T* p[2] = {work[2], work[3]};
   const dim3 g = stride(n);
   const int parts = static_cast<int>(g.x);
   KState k{};
@@ -205,8 +219,8 @@ InnerResult innerFcg(Context& ctx, double relTol, int64_t maxIter, int64_t windo
     launchFinalize(ctx, FinalArgs{w.partial.get(), nParts, w.state.get(), w.hist.get(), nullptr,
                                   step});
   };
-  hipLaunchKernelGGL(zeroF, g, dim3(256), 0, ctx.stream, n, d);
-  hipLaunchKernelGGL(normKernel32, g, dim3(256), 0, ctx.stream, n, r, w.partial.get());
+  hipLaunchKernelGGL((zeroT<T>), g, dim3(256), 0, ctx.stream, n, d);
+  hipLaunchKernelGGL((normKernelT<T>), g, dim3(256), 0, ctx.stream, n, r, w.partial.get());
   fin(kStepStartNorm, parts);
   CFDHIP_CHECK(hipGetLastError());
   const int chunk = 4;
@@ -217,12 +231,12 @@ InnerResult innerFcg(Context& ctx, double relTol, int64_t maxIter, int64_t windo
     for (int c = 0; c < chunk && launched < maxIter; ++c, ++launched) {
       const bool first = launched == 0;
       // z = M r and (z, q_old) -> beta
-      applyM(ctx, r, first ? nullptr : q, z, first ? nullptr : w.partial.get(), parts, h, wd);
+      applyM<T>(ctx, r, first ? nullptr : q, z, first ? nullptr : w.partial.get(), parts, h, wd);
       if (!first) fin(kStepFcgBeta, parts);
-      hipLaunchKernelGGL(fcgDirSpmvKernel, g, dim3(256), 0, ctx.stream, op, z, p[1 - cur], p[cur], q,
-                         r, w.state.get(), first ? 1 : 0, w.partial.get());
+      hipLaunchKernelGGL((fcgDirSpmvKernel<T>), g, dim3(256), 0, ctx.stream, op, z, p[1 - cur],
+                         p[cur], q, r, w.state.get(), first ? 1 : 0, w.partial.get());
       fin(kStepFcgAlpha, parts);
-      hipLaunchKernelGGL(fcgUpdateKernel, g, dim3(256), 0, ctx.stream, n, p[cur], q, d, r,
+      hipLaunchKernelGGL((fcgUpdateKernel<T>), g, dim3(256), 0, ctx.stream, n, p[cur], q, d, r,
                          w.state.get(), w.partial.get());
       fin(kStepFcgUpdate, parts);
       cur = 1 - cur;
@@ -259,15 +273,35 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
           "the learned preconditioner has no weights: call cfdhip_set_learned_weights first");
   const View& v = ctx.view;
   const int64_t n = v.n, nf = v.nf;
+  const bool mixed = set.precision == CFDHIP_MIXED;
   KrylovWork& w = ctx.kw;
-  w.r32.ensure(n);
-  w.x32.ensure(n);
-  for (int q = 0; q < 4; ++q) w.f8[q].ensure(n);
-  w.dinv32.ensure(n);
-  w.w32.ensure(nf);
   w.dis32.ensure(n);
-  castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
-  castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
+  float* r32 = nullptr;
+  float* d32 = nullptr;
+  float* work32[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* r64 = nullptr;
+  double* d64 = nullptr;
+  double* work64[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (mixed) {
+    w.r32.ensure(n);
+    w.x32.ensure(n);
+    for (int q = 0; q < 4; ++q) w.f8[q].ensure(n);
+    w.dinv32.ensure(n);
+    w.w32.ensure(nf);
+    castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
+    castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
+    r32 = w.r32.get();
+    d32 = w.x32.get();
+    for (int q = 0; q < 4; ++q) work32[q] = w.f8[q].get();
+  } else {
+    for (int q = 0; q < 6; ++q) w.d8[q].ensure(n);
+    r64 = w.d8[4].get();
+    d64 = w.d8[5].get();
+    for (int q = 0; q < 4; ++q) work64[q] = w.d8[q].get();
+  }
+  const OpT<float> op32{v.nx, v.ny, ctx.pin, v.offY, w.dinv32.get(), w.w32.get()};
+  const OpT<double> op64{v.nx, v.ny, ctx.pin, v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                         ctx.f(CFDHIP_FIELD_PC_FACE_COEFF)};
   const dim3 g = stride(n);
   hipLaunchKernelGGL(invSqrtKernel, g, dim3(256), 0, ctx.stream, n, ctx.f(CFDHIP_FIELD_PC_DIAG),
                      w.dis32.get());
@@ -287,20 +321,30 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
     const double t = e ? std::atof(e) : 1e-4;
     return t > 0.0 && t < 1.0 ? t : 1e-4;
   }();
+  auto outerResidual = [&]() {
+    if (mixed)
+      hipLaunchKernelGGL((outerResidualKernel<float>), g, dim3(256), 0, ctx.stream, v.nx, v.ny,
+                         ctx.pin, v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                         ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, r32, w.partial.get());
+    else
+      hipLaunchKernelGGL((outerResidualKernel<double>), g, dim3(256), 0, ctx.stream, v.nx, v.ny,
+                         ctx.pin, v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                         ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, r64, w.partial.get());
+    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                  nullptr, normOut, kStepNorm});
+    CFDHIP_CHECK(hipGetLastError());
+    double v0 = 0.0;
+    CFDHIP_CHECK(hipMemcpyAsync(&v0, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    return v0;
+  };
   std::vector<double> history;
   double b0 = 0.0, rTrue = 0.0;
   int64_t total = 0, cycles = 0;
   int status = CFDHIP_MAX_ITERATIONS;
   bool fallback = false;
   while (true) {
-    hipLaunchKernelGGL(outerResidualKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin,
-                       v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b,
-                       x, w.r32.get(), w.partial.get());
-    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
-                                  nullptr, normOut, kStepNorm});
-    CFDHIP_CHECK(hipGetLastError());
-    CFDHIP_CHECK(hipMemcpyAsync(&rTrue, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
-    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    rTrue = outerResidual();
     if (cycles == 0) b0 = rTrue;
     if (!std::isfinite(rTrue)) {
       status = CFDHIP_NONFINITE_RESIDUAL;
@@ -314,12 +358,23 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
     if (total >= set.max_iterations) break;
     const double need = std::max(set.relative_tolerance * b0, set.absolute_tolerance) / rTrue;
     const int64_t window = set.stagnation_window == 0 ? 200 : std::max(0, set.stagnation_window);
-    const InnerResult ir = innerFcg(ctx, std::max(innerTol, need), set.max_iterations - total,
-                                    window, w.r32.get(), w.x32.get(), history);
+    // FP64 vectors aim at the final target (with a small margin for the
+    // recurrence drift); FP32 vectors restart every innerTol of reduction
+    const double tol = mixed ? std::max(innerTol, need) : std::max(0.5 * need, 1e-15);
+    InnerResult ir;
+    if (mixed)
+      ir = innerFcg<float>(ctx, op32, tol, set.max_iterations - total, window, r32, d32, work32,
+                           history);
+    else
+      ir = innerFcg<double>(ctx, op64, tol, set.max_iterations - total, window, r64, d64, work64,
+                            history);
     total += ir.iterations;
     ++cycles;
     if (ir.iterations > 0) {
-      hipLaunchKernelGGL(addCorrection, g, dim3(256), 0, ctx.stream, n, w.x32.get(), x);
+      if (mixed)
+        hipLaunchKernelGGL((addCorrection<float>), g, dim3(256), 0, ctx.stream, n, d32, x);
+      else
+        hipLaunchKernelGGL((addCorrection<double>), g, dim3(256), 0, ctx.stream, n, d64, x);
       CFDHIP_CHECK(hipGetLastError());
     }
     if (ir.stagnated || (ir.iterations == 0 && ir.status != CFDHIP_CONVERGED)) {
@@ -331,14 +386,7 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
       fb.precision = CFDHIP_FP64;
       fb.max_iterations = std::max<int64_t>(1, set.max_iterations - total);
       // its relative test is against |b - A x| at its start; rescale to |b - A x0|
-      hipLaunchKernelGGL(outerResidualKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin,
-                         v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b,
-                         x, w.r32.get(), w.partial.get());
-      launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
-                                    nullptr, normOut, kStepNorm});
-      double rNow = 0.0;
-      CFDHIP_CHECK(hipMemcpyAsync(&rNow, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
-      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+      const double rNow = outerResidual();
       fb.relative_tolerance = rNow > 0.0 ? std::min(1.0, set.relative_tolerance * b0 / rNow) : 0.0;
       double* work[8];
       for (int q = 0; q < 8; ++q) {

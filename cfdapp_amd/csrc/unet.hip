@@ -69,7 +69,8 @@ __global__ void packKernel(const float* __restrict__ master, bf16* __restrict__ 
 
 // e0a = conv(x, 1 -> 16). x(y, x) = src[y srcW + x] * scale (inside srcH x srcW,
 // zero elsewhere); one pixel per thread.
-__global__ __launch_bounds__(256) void firstConvKernel(const float* __restrict__ src,
+template <class TS>
+__global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ src,
                                                        const float* __restrict__ scale, int srcH,
                                                        int srcW, int64_t srcBatch,
                                                        const float* __restrict__ w,
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(256) void firstConvKernel(const float* __restrict__
       v[tap] = 0.f;
       if (yy >= 0 && yy < srcH && xx >= 0 && xx < srcW) {
         const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
-        v[tap] = scale != nullptr ? src[si] * scale[si] : src[si];
+        const float sv = static_cast<float>(src[si]);
+        v[tap] = scale != nullptr ? sv * scale[si] : sv;
       }
     }
     fx4* o = reinterpret_cast<fx4*>(out + p * 16);
@@ -147,12 +149,13 @@ __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ 
 // Preconditioner output on the solver grid (ny x nx inside the network's
 // h x w): z = scale * y, z[pin] = r[pin] scale[pin]^2, and block partials of
 // (z, q).
+template <class TZ>
 __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restrict__ d0,
                                                            const float* __restrict__ w,
                                                            const float* __restrict__ scale,
-                                                           const float* __restrict__ r,
-                                                           const float* __restrict__ q,
-                                                           float* __restrict__ z, int ny, int nx,
+                                                           const TZ* __restrict__ r,
+                                                           const TZ* __restrict__ q,
+                                                           TZ* __restrict__ z, int ny, int nx,
                                                            int64_t pin, int h, int wd,
                                                            double* partial) {
   __shared__ float ws[144];
@@ -163,8 +166,8 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < total;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
-    float v = lastConvAt(d0, ws, 0, y, x, h, wd) * scale[c];
-    if (c == pin) v = r[c] * scale[c] * scale[c];
+    TZ v = static_cast<TZ>(lastConvAt(d0, ws, 0, y, x, h, wd) * scale[c]);
+    if (c == pin) v = r[c] * static_cast<TZ>(scale[c]) * static_cast<TZ>(scale[c]);
     z[c] = v;
     if (q != nullptr) acc[0] += static_cast<double>(v) * static_cast<double>(q[c]);
   }
@@ -496,11 +499,22 @@ void Unet::convDw(int layer, int gx, int gg, const float* xa, const float* xb, c
 
 void Unet::forward(const float* src, const float* scale, int srcH, int srcW, int64_t srcBatch,
                    int n, int h, int w) {
+  forwardFrom(src, false, scale, srcH, srcW, srcBatch, n, h, w);
+}
+
+void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
+                       int64_t srcBatch, int n, int h, int w) {
   if (!loaded) throw Error(CFDHIP_ERR_INVALID, "network weights are not loaded");
   const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
-  hipLaunchKernelGGL(firstConvKernel, grid1(p0), dim3(256), 0, stream, src, scale, srcH, srcW,
-                     srcBatch, master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+  if (srcDouble)
+    hipLaunchKernelGGL((firstConvKernel<double>), grid1(p0), dim3(256), 0, stream,
+                       static_cast<const double*>(src), scale, srcH, srcW, srcBatch,
+                       master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+  else
+    hipLaunchKernelGGL((firstConvKernel<float>), grid1(p0), dim3(256), 0, stream,
+                       static_cast<const float*>(src), scale, srcH, srcW, srcBatch,
+                       master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
   CFDHIP_CHECK(hipGetLastError());
   conv(kE0b, false, kGatherPlain, kEpiStorePool, 0, T(kTe0a), nullptr, T(kTe0), T(kTp0), n, h, w);
   conv(kE1a, false, kGatherPlain, kEpiStore, 0, T(kTp0), nullptr, T(kTe1a), nullptr, n, h1, w1);
@@ -524,13 +538,18 @@ void Unet::outputTrain(int n, int h, int w) {
   CFDHIP_CHECK(hipGetLastError());
 }
 
-void Unet::outputApply(const float* scale, const float* r, const float* q, float* z, int ny,
-                       int nx, int64_t pin, int h, int w, double* partial, int parts) {
-  hipLaunchKernelGGL(lastConvApplyKernel, dim3(static_cast<unsigned>(parts)), dim3(256), 0, stream,
-                     T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin, h, w,
-                     partial);
+template <class TZ>
+void Unet::outputApply(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
+                       int64_t pin, int h, int w, double* partial, int parts) {
+  hipLaunchKernelGGL((lastConvApplyKernel<TZ>), dim3(static_cast<unsigned>(parts)), dim3(256), 0,
+                     stream, T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin,
+                     h, w, partial);
   CFDHIP_CHECK(hipGetLastError());
 }
+template void Unet::outputApply<float>(const float*, const float*, const float*, float*, int, int,
+                                       int64_t, int, int, double*, int);
+template void Unet::outputApply<double>(const float*, const double*, const double*, double*, int,
+                                        int, int64_t, int, int, double*, int);
 
 void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
   const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;

@@ -36,12 +36,16 @@ class Unet {
   // zero-extended to h x w).
   void forward(const float* src, const float* scale, int srcH, int srcW, int64_t srcBatch, int n,
                int h, int w);
+  // The same from an FP32 or FP64 source.
+  void forwardFrom(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
+                   int64_t srcBatch, int n, int h, int w);
   // y = conv(d0) into the FP32 training output.
   void outputTrain(int n, int h, int w);
   // z = scale * conv(d0) on the ny x nx solver grid, z[pin] = r[pin] scale^2
   // (the inverse of the decoupled pinned row), and block partials of (z, q)
   // (parts blocks; none when partial is null).
-  void outputApply(const float* scale, const float* r, const float* q, float* z, int ny, int nx,
+  template <class TZ>
+  void outputApply(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
                    int64_t pin, int h, int w, double* partial, int parts);
   // Weight gradients for dL/dy = gyIn (x: the FP32 network input).
   void backward(const float* x, const float* gyIn, int n, int h, int w);
