@@ -1,13 +1,20 @@
-"""Benchmark: the 2048x2048 stand-alone p' solve and SIMPLE iterations on one MI355X.
+"""Benchmark: the 2048x2048 stand-alone p' solve, learned FCG versus Jacobi-PCG,
+and SIMPLE iterations on one MI355X.
 
-Workload (the plain method at scale, FP64 on one GPU): 2D lid-driven cavity,
-2048x2048 uniform cells. A step is one stand-alone pressure-correction solve on
-the 5-point p' operator (d = 1, rho = 1, cell 0 pinned) with an RHS drawn
-seeded-uniform in [-1, 1] (splitmix64, seed 20480001), its serial mean removed
-and b_0 = 0, solved from x0 = 0 by Jacobi-PCG to |r| <= 1e-8 |b| (FP64). The
-operator and RHS are resident in HBM before the timed region. Also reported:
-SIMPLE outer iterations per second at 2048x2048, Re = 1000 (Rhie-Chow,
-upwind, alpha_u 0.7, alpha_p 0.3, BiCGSTAB momentum, CG+Jacobi p').
+Workload: 2D lid-driven cavity, 2048x2048 uniform cells. A step is one
+stand-alone pressure-correction solve on the 5-point p' operator (d = 1,
+rho = 1, cell 0 pinned) with an RHS drawn seeded-uniform in [-1, 1]
+(splitmix64, seed 20480001), its serial mean removed and b_0 = 0, solved from
+x0 = 0 to an FP64 true residual |b - A x| <= 1e-8 |b|. Two methods, timed
+back to back on the same resident operator and RHS:
+  * learned: flexible CG (FP64 vectors) preconditioned by the U-Net (BF16
+    weights, FP32 activations), weights from cfdapp_amd/weights when present,
+    else the multilevel initialisation the training starts from;
+  * plain: Jacobi-PCG, FP64 single-kernel iteration.
+`value` is the learned solve's time. Also reported: the FP32-inner mixed
+Jacobi-PCG, SIMPLE outer iterations per second at 2048x2048, Re = 1000, the
+per-layer kernel times of the preconditioner and the plain iteration kernel's
+HBM roofline.
 
 Prints one JSON line (see the repository's bench contract).
 """
@@ -26,6 +33,33 @@ sys.path.insert(0, REPO)
 N = 2048
 SEED = 20480001
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one cgIterKernel<double> launch at 2048^2 from rocprofv3 counters
+# (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 512 MiB copy kernel; see profiles/)
+TRAFFIC_BYTES = 530.7e6
+
+
+def layer_bytes(p0):
+    """Algorithmic HBM bytes of each preconditioner layer on the solve path
+    (FP32 activations, each tensor read and written once; p0 = fine pixels)."""
+    p1, p2, p3 = p0 / 4, p0 / 16, p0 / 64
+    f = 4.0
+    return {
+        "e0b": p0 * (8 + 4) + f * (16 * p0 + 16 * p1),   # r (FP64) + D^-1/2 -> e0, p0
+        "e1a": f * (16 * p1 + 32 * p1),
+        "e1b": f * (32 * p1 + 32 * p1 + 32 * p2),
+        "e2a": f * (32 * p2 + 64 * p2),
+        "e2b": f * (64 * p2 + 64 * p2 + 64 * p3),
+        "b3a": f * (64 * p3 + 64 * p3),
+        "b3b": f * (64 * p3 + 64 * p3),
+        "d2a": f * (64 * p3 + 64 * p2 + 64 * p2),
+        "d2b": f * (64 * p2 + 64 * p2),
+        "d1a": f * (64 * p2 + 32 * p1 + 32 * p1),
+        "d1b": f * (32 * p1 + 32 * p1),
+        "d0a": f * (32 * p1 + 16 * p0 + 16 * p0),
+        "d0b": f * (16 * p0 + 16 * p0),
+        "out": f * 16 * p0 + p0 * (4 + 8 + 8),         # d0, D^-1/2, q -> z (FP64)
+        "e0a": 0.0,
+    }
 
 
 def splitmix_uniform(seed, count):
@@ -94,11 +128,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--simple-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device copy 20 times (counter calibration)")
+    ap.add_argument("--jacobi-only", action="store_true",
+                    help="only the Jacobi-PCG solves (for kernel counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "the multi-GPU decomposition is a design, not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 * 2 ** 20, dtype=torch.float64, device="cuda")  # 512 MiB
+        c = torch.empty_like(a)
+        for _ in range(20):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8, "launches": 20}))
+        return
 
     import cfdapp_amd as cfd
     if not cfd.device_available():
@@ -114,32 +162,49 @@ def main():
     d.upload("FLUX_STAR", np.zeros(nf))
     d.assemble_pressure_correction_resident(0)
     b = standalone_rhs(n)
+    bnorm = float(np.linalg.norm(b))
     d.upload("PC_RHS", b)
     pcg = cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi", check_interval=64)
-    log("setup done")
+    weights_kind = "trained" if os.path.exists(cfd.WEIGHTS_PATH) else "multilevel initialisation"
+    if os.path.exists(cfd.WEIGHTS_PATH):
+        d.set_learned_weights()
+    else:
+        from cfdapp_amd import learned
+        d.set_learned_weights(learned.structured_init(0, noise=0.0))
+    fcg = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
+    log(f"setup done (learned weights: {weights_kind})")
 
-    for w in range(args.warmup):
-        r = d.solve_pressure_rhs(pcg, None, download=False)
-        log(f"warmup {w}: {r.status_name} {r.iterations} it {r.device_ms:.1f} ms")
+    def timed(settings, label):
+        for w in range(args.warmup):
+            r = d.solve_pressure_rhs(settings, None, download=False)
+            log(f"{label} warmup {w}: {r.status_name} {r.iterations} it {r.device_ms:.1f} ms")
+        d._ctx.call("cfdhip_synchronize")
+        rs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rs.append(d.solve_pressure_rhs(settings, None, download=False))
+        d._ctx.call("cfdhip_synchronize")
+        sec = (time.perf_counter() - t0) / args.steps
+        r = rs[-1]
+        log(f"{label}: {args.steps} solves, {sec * 1e3:.1f} ms/solve, {r.status_name}, "
+            f"{r.iterations} it, true rel res {r.true_residual / bnorm:.3e}, "
+            f"fallback {r.fallback_used}")
+        return sec, rs
 
-    d._ctx.call("cfdhip_synchronize")
-    times, iters, results = [], [], []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        r = d.solve_pressure_rhs(pcg, None, download=False)
-        results.append(r)
-        iters.append(r.iterations)
-    d._ctx.call("cfdhip_synchronize")
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    r = results[-1]
-    log(f"timed: {args.steps} solves, {ms_per_step:.1f} ms/solve, {r.iterations} it, "
-        f"true res {r.true_residual:.3e} (|b| {np.linalg.norm(b):.3e})")
+    if args.jacobi_only:
+        timed(pcg, "Jacobi-PCG")
+        return
+    fcg_s, fcg_rs = timed(fcg, "learned FCG")
+    jac_s, jac_rs = timed(pcg, "Jacobi-PCG")
+    rf, rj = fcg_rs[-1], jac_rs[-1]
+    ms_per_step = fcg_s * 1e3
 
-    # --- dominant kernel timing (p' SpMV fused with p.Ap), HIP events ------------
+    # --- kernel timing pass: plain iteration kernel and the preconditioner layers
     d.set_timing(True)
-    rt = d.solve_pressure_rhs(pcg, None, download=False)
+    d.solve_pressure_rhs(pcg, None, download=False)
     kt = d.kernel_timing()
+    d.solve_pressure_rhs(fcg, None, download=False)
+    layer_ms, layer_calls = d.preconditioner_timing()
     d.set_timing(False)
     spmv_us = kt["mean_us"]
     # algorithmic bytes of one single-kernel PCG iteration (FP64): reads p, s,
@@ -148,9 +213,22 @@ def main():
     achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9 if spmv_us > 0 else 0.0
     log(f"PCG iteration kernel: {kt['launches']:.0f} launches, mean {spmv_us:.2f} us, "
         f"{achieved:.0f} GB/s algorithmic")
+    names = ["e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "b3a", "b3b", "d2a", "d2b", "d1a", "d1b",
+             "d0a", "d0b", "out"]
+    p0 = float(n)
+    lb = layer_bytes(p0)
+    layers = {}
+    for i, nm in enumerate(names):
+        if layer_calls[i] > 0:
+            us = 1e3 * layer_ms[i] / layer_calls[i]
+            layers[nm] = {"us": us, "gbs": lb[nm] / (us * 1e-6) / 1e9}
+    per_apply_us = sum(v["us"] for v in layers.values())
+    log("preconditioner layers (us, GB/s): " +
+        ", ".join(f"{k} {v['us']:.0f} {v['gbs']:.0f}" for k, v in layers.items()) +
+        f"; total {per_apply_us:.0f} us per application")
+    dom = max(layers, key=lambda k: layers[k]["us"])
 
     # --- mixed precision (FP32 inner PCG, FP64 refinement) on the same system ------
-    mixed = {}
     mx = cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi", precision="mixed",
                                   check_interval=64)
     d.solve_pressure_rhs(mx, None, download=False)
@@ -158,12 +236,11 @@ def main():
     t2 = time.perf_counter()
     rm = d.solve_pressure_rhs(mx, None, download=False)
     dtm = time.perf_counter() - t2
-    mixed = {"mixed_solve_s": dtm, "mixed_iterations": rm.iterations,
+    mixed = {"mixed_jacobi_pcg_s": dtm, "mixed_iterations": rm.iterations,
              "mixed_refinements": rm.restarts,
-             "mixed_true_relative_residual": rm.true_residual / np.linalg.norm(b),
-             "mixed_speedup_vs_fp64": (ms_per_step / 1e3) / dtm}
-    log(f"mixed: {rm.status_name} {rm.iterations} it ({rm.restarts} cycles) {dtm * 1e3:.1f} ms, "
-        f"true rel res {rm.true_residual / np.linalg.norm(b):.3e}")
+             "mixed_true_relative_residual": rm.true_residual / bnorm}
+    log(f"mixed Jacobi-PCG: {rm.status_name} {rm.iterations} it ({rm.restarts} cycles) "
+        f"{dtm * 1e3:.1f} ms, true rel res {rm.true_residual / bnorm:.3e}")
 
     # --- SIMPLE outer iterations at Re = 1000 --------------------------------------
     simple = {}
@@ -184,12 +261,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(int(np.median(iters)))
+        cpu = cpu_baseline(rj.iterations)
         log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['kind']}, {cpu['cores']} cores)")
 
     line = {
         "metric": "p' solve time to 1e-8 rel.res + SIMPLE iter/s: learned FCG vs Jacobi-PCG",
-        "value": ms_per_step / 1e3,
+        "value": fcg_s,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -200,16 +277,29 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "2D cavity 2048x2048: stand-alone p' solve, Jacobi-PCG FP64 to "
-                               "1e-8 relative residual (seeded mean-free RHS)",
-                   "cells": n, "solver": "CG+Jacobi", "precision": "fp64",
-                   "iterations": int(np.median(iters)),
-                   "true_relative_residual": r.true_residual / np.linalg.norm(b),
+        "config": {"workload": "2D cavity 2048x2048: stand-alone p' solve to 1e-8 FP64 true "
+                               "relative residual (seeded mean-free RHS), learned FCG vs "
+                               "Jacobi-PCG",
+                   "cells": n,
+                   "learned_fcg_s": fcg_s, "learned_fcg_iterations": rf.iterations,
+                   "learned_fcg_fallback": bool(rf.fallback_used),
+                   "learned_fcg_true_relative_residual": rf.true_residual / bnorm,
+                   "learned_weights": weights_kind,
+                   "learned_precision": "FCG FP64 vectors; U-Net BF16 weights, FP32 activations",
+                   "preconditioner_us_per_application": per_apply_us,
+                   "preconditioner_layers": layers,
+                   "jacobi_pcg_s": jac_s, "jacobi_pcg_iterations": rj.iterations,
+                   "jacobi_pcg_true_relative_residual": rj.true_residual / bnorm,
+                   "speedup_learned_vs_jacobi": jac_s / fcg_s,
                    **mixed, **simple},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "cgIterKernel<double> (one Jacobi-PCG iteration: updates + 5-point SpMV + dots)",
-                     "kernel_mean_us": spmv_us, "bytes_per_launch": spmv_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES,
+                     "kernel": "cgIterKernel<double> (one Jacobi-PCG iteration: updates + "
+                               "5-point SpMV + dots)",
+                     "kernel_mean_us": spmv_us, "bytes_per_launch": spmv_bytes,
+                     "learned_dominant_layer": {"layer": dom, **layers[dom],
+                                                "bytes": lb[dom],
+                                                "frac": layers[dom]["gbs"] / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

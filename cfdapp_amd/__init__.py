@@ -380,6 +380,13 @@ class GpuSimpleDiscretization:
     def set_timing(self, enabled):
         self._ctx.call("cfdhip_set_timing", 1 if enabled else 0)
 
+    def preconditioner_timing(self):
+        """Per-layer (ms, launches) of the learned preconditioner while timing is on."""
+        ms = np.zeros(15)
+        calls = np.zeros(15, np.int64)
+        self._ctx.call("cfdhip_preconditioner_timing", _pf(ms), _pi(calls))
+        return ms, calls
+
     def kernel_timing(self):
         out = np.zeros(3)
         self._ctx.call("cfdhip_kernel_timing", _pf(out))
@@ -514,6 +521,10 @@ class UnetNet:
 
     def forward(self, x, y, batch, h, w):
         self._call("cfdhip_unet_forward", _dev(x), _dev(y), int(batch), int(h), int(w))
+
+    def apply(self, x, y, batch, h, w):
+        """The inference path (first layer fused into the second)."""
+        self._call("cfdhip_unet_apply", _dev(x), _dev(y), int(batch), int(h), int(w))
 
     def backward(self, x, gy, batch, h, w) -> np.ndarray:
         g = np.zeros(unet_param_count(), dtype=np.float32)

@@ -111,6 +111,7 @@ SIGNATURES = {
     "cfdhip_solve_history": (ctypes.c_int, [ctypes.c_void_p, P_F64, ctypes.c_int64, P_I64]),
     "cfdhip_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "cfdhip_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
+    "cfdhip_preconditioner_timing": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_I64]),
     # learned preconditioner (device buffers are passed as void*)
     "cfdhip_unet_param_count": (ctypes.c_int64, []),
     "cfdhip_set_learned_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
@@ -125,6 +126,8 @@ SIGNATURES = {
     "cfdhip_unet_get_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
     "cfdhip_unet_forward": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, ctypes.c_int64,
                                            ctypes.c_int64]),
+    "cfdhip_unet_apply": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64]),
     "cfdhip_unet_backward": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, P_F32]),
     "cfdhip_unet_tensor_size": (ctypes.c_int64, [VP, ctypes.c_int]),

@@ -420,7 +420,20 @@ int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t*
 }
 
 int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled) {
-  return guarded(ctx, [&](Context& c) { c.timing = enabled != 0; });
+  return guarded(ctx, [&](Context& c) {
+    c.timing = enabled != 0;
+    if (c.net) c.net->setTiming(c.timing);
+  });
+}
+
+int cfdhip_preconditioner_timing(cfdhip_ctx* ctx, double* ms, int64_t* calls) {
+  return guarded(ctx, [&](Context& c) {
+    require(ms != nullptr && calls != nullptr, "output arrays are required");
+    for (int l = 0; l < kNumLayers; ++l) {
+      ms[l] = c.net ? c.net->layerMs[l] : 0.0;
+      calls[l] = c.net ? c.net->layerCalls[l] : 0;
+    }
+  });
 }
 
 int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out) {

@@ -167,6 +167,20 @@ int cfdhip_unet_forward(cfdhip_unet* h, const float* x, float* y, int64_t batch,
   });
 }
 
+int cfdhip_unet_apply(cfdhip_unet* h, const float* x, float* y, int64_t batch, int64_t hh,
+                      int64_t w) {
+  return guardedNet(h, [&](Unet& net) {
+    checkShape(batch, hh, w);
+    require(x != nullptr && y != nullptr, "device buffers are required");
+    const int n = static_cast<int>(batch), H = static_cast<int>(hh), W = static_cast<int>(w);
+    net.ensureShape(n, H, W, false);
+    net.forwardFrom(x, false, nullptr, H, W, hh * w, n, H, W, true);
+    net.outputTrain(n, H, W);
+    CFDHIP_CHECK(hipMemcpyAsync(y, net.output(), sizeof(float) * batch * hh * w,
+                                hipMemcpyDeviceToDevice, net.stream));
+  });
+}
+
 int cfdhip_unet_backward(cfdhip_unet* h, const float* x, const float* gy, int64_t batch, int64_t hh,
                          int64_t w, float* grad) {
   return guardedNet(h, [&](Unet& net) {

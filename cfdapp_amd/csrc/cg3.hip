@@ -1,0 +1,336 @@
+// Jacobi-PCG for the 3D p' operator (7-point), in FP64 or FP32, and the
+// mixed-precision refinement around the FP32 variant.
+//
+// Same Chronopoulos-Gear recurrence and device-side scalar steps as the 2D
+// single-kernel iteration (cg1.hip; src/algebra/CG.cpp:77-142 in exact
+// arithmetic), split in two kernels per iteration because the 3D halo of
+// u = M r is not worth an LDS tile here:
+//   update: s = w + beta s, r -= alpha s, p = M r_old + beta p, x += alpha p,
+//           u = M r                                  (pointwise, in place)
+//   apply:  w = A u; (r, u), (w, u), |r|^2           (7-point stencil)
+// Per iteration: update reads p, s, r, w, x, diag and writes p, s, r, x, u;
+// apply reads u (+ neighbours, from cache), r, diag, the three face arrays and
+// writes w: 8 * 19 n bytes in FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "reduce.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+constexpr int kMaxBlocks3 = 2048;
+
+template <class T>
+struct Op3 {
+  int64_t nx, ny, nz, pin, offY, offZ;
+  const T* diag;
+  const T* coeff;
+};
+
+// Row c of A applied to u, ascending column order from 0 (PressureOp::apply).
+template <class T>
+__device__ inline T applyRow3(const Op3<T>& g, int64_t i, int64_t j, int64_t k, int64_t c,
+                              const T* __restrict__ u) {
+  if (c == g.pin) return T(0) + g.diag[c] * u[c];
+  const int64_t sxy = g.nx * g.ny;
+  T y = T(0);
+  if (k > 0) y += (-g.coeff[g.offZ + (k * g.ny + j) * g.nx + i]) * u[c - sxy];
+  if (j > 0) y += (-g.coeff[g.offY + (k * (g.ny + 1) + j) * g.nx + i]) * u[c - g.nx];
+  if (i > 0) y += (-g.coeff[(k * g.ny + j) * (g.nx + 1) + i]) * u[c - 1];
+  y += g.diag[c] * u[c];
+  if (i < g.nx - 1) y += (-g.coeff[(k * g.ny + j) * (g.nx + 1) + i + 1]) * u[c + 1];
+  if (j < g.ny - 1) y += (-g.coeff[g.offY + (k * (g.ny + 1) + j + 1) * g.nx + i]) * u[c + g.nx];
+  if (k < g.nz - 1) y += (-g.coeff[g.offZ + ((k + 1) * g.ny + j) * g.nx + i]) * u[c + sxy];
+  return y;
+}
+
+inline dim3 grid3(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > kMaxBlocks3) b = kMaxBlocks3;
+  return dim3(static_cast<unsigned>(b < 1 ? 1 : b));
+}
+
+// r0 = b - A x0, |r0|^2.
+template <class T>
+__global__ __launch_bounds__(256) void start3Kernel(Op3<T> g, const T* __restrict__ b,
+                                                    const T* __restrict__ x, T* __restrict__ r,
+                                                    double* partial) {
+  const int64_t n = g.nx * g.ny * g.nz;
+  double acc[1] = {0.0};
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % g.nx, t = c / g.nx, j = t % g.ny, k = t / g.ny;
+    const T ri = b[c] - applyRow3<T>(g, i, j, k, c, x);
+    r[c] = ri;
+    acc[0] += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  blockPartials<1>(acc, partial);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void update3Kernel(int64_t n, const T* __restrict__ diag,
+                                                     T* __restrict__ p, T* __restrict__ s,
+                                                     T* __restrict__ r, const T* __restrict__ w,
+                                                     T* __restrict__ x, T* __restrict__ u,
+                                                     const KState* __restrict__ st, int start) {
+  if (st->done) return;
+  const T alpha = static_cast<T>(st->alpha);
+  const T beta = static_cast<T>(st->beta);
+  const bool first = st->iter == 0;
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const T d = diag[c];
+    T rn = r[c];
+    if (!start) {
+      const T w0 = w[c];
+      const T sn = first ? w0 : w0 + (s[c] * beta);
+      const T uOld = (T(1) / d) * rn;
+      const T pn = first ? uOld : uOld + (p[c] * beta);
+      rn = rn - (sn * alpha);
+      s[c] = sn;
+      p[c] = pn;
+      r[c] = rn;
+      x[c] = x[c] + (pn * alpha);
+    }
+    u[c] = (T(1) / d) * rn;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void apply3Kernel(Op3<T> g, const T* __restrict__ u,
+                                                    const T* __restrict__ r, T* __restrict__ w,
+                                                    const KState* __restrict__ st,
+                                                    double* partial) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (!st->done) {
+    const int64_t n = g.nx * g.ny * g.nz;
+    for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const int64_t i = c % g.nx, t = c / g.nx, j = t % g.ny, k = t / g.ny;
+      const T wc = applyRow3<T>(g, i, j, k, c, u);
+      w[c] = wc;
+      const T uc = u[c], rc = r[c];
+      acc[0] += static_cast<double>(rc) * static_cast<double>(uc);
+      acc[1] += static_cast<double>(wc) * static_cast<double>(uc);
+      acc[2] += static_cast<double>(rc) * static_cast<double>(rc);
+    }
+  }
+  blockPartials<3>(acc, partial);
+}
+
+// r64 = b - A x (FP64) -> r32, |r64|^2 partials (refinement step).
+__global__ __launch_bounds__(256) void refine3Kernel(Op3<double> g, const double* __restrict__ b,
+                                                     const double* __restrict__ x,
+                                                     float* __restrict__ r32, double* partial) {
+  const int64_t n = g.nx * g.ny * g.nz;
+  double acc[1] = {0.0};
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % g.nx, t = c / g.nx, j = t % g.ny, k = t / g.ny;
+    const double rc = b[c] - applyRow3<double>(g, i, j, k, c, x);
+    r32[c] = static_cast<float>(rc);
+    acc[0] += rc * rc;
+  }
+  blockPartials<1>(acc, partial);
+}
+
+__global__ void zero3(int64_t n, float* a) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[c] = 0.f;
+}
+
+__global__ void zero3d(int64_t n, double* a) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[c] = 0.0;
+}
+
+__global__ void add3(int64_t n, const float* __restrict__ e, double* __restrict__ x) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[c] += static_cast<double>(e[c]);
+}
+
+}  // namespace
+
+// work: r s w p u (n each)
+template <class T>
+cfdhip_solve_result pcg3d(Context& ctx, const cfdhip_solver_settings& set, const T* diag,
+                          const T* coeff, const T* b, T* x, T* const work[5]) {
+  const View& v = ctx.view;
+  require(v.dim == 3, "the 3D PCG needs a 3D mesh");
+  KrylovWork& w = ctx.kw;
+  const int64_t n = v.n;
+  const dim3 g = grid3(n);
+  w.partial.ensure(std::max<int64_t>(w.partial.count, kMaxBlocks3 * 3 + 8));
+  w.state.ensure(1);
+  if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+  w.hist.ensure(set.max_iterations + 2);
+  KState k{};
+  k.absTol = set.absolute_tolerance;
+  k.relTol = set.relative_tolerance;
+  k.maxIter = set.max_iterations;
+  k.status = -1;
+  *w.hostState = k;
+  CFDHIP_CHECK(hipMemcpyAsync(w.state.get(), w.hostState, sizeof(KState), hipMemcpyHostToDevice,
+                              ctx.stream));
+  const Op3<T> op{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ, diag, coeff};
+  T* r = work[0];
+  T* s = work[1];
+  T* wv = work[2];
+  T* p = work[3];
+  T* u = work[4];
+  auto fin = [&](int step) {
+    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                  w.hist.get(), nullptr, step});
+  };
+  hipEvent_t t0, t1;
+  CFDHIP_CHECK(hipEventCreate(&t0));
+  CFDHIP_CHECK(hipEventCreate(&t1));
+  CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
+  hipLaunchKernelGGL((start3Kernel<T>), g, dim3(256), 0, ctx.stream, op, b, x, r, w.partial.get());
+  fin(kStepStartNorm);
+  hipLaunchKernelGGL((update3Kernel<T>), g, dim3(256), 0, ctx.stream, n, diag, p, s, r, wv, x, u,
+                     w.state.get(), 1);
+  hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv, w.state.get(),
+                     w.partial.get());
+  fin(kStepCg1Start);
+  CFDHIP_CHECK(hipGetLastError());
+  const int chunk = set.check_interval > 0 ? set.check_interval : 32;
+  int64_t launched = 0;
+  while (true) {
+    for (int q = 0; q < chunk && launched < set.max_iterations; ++q, ++launched) {
+      hipLaunchKernelGGL((update3Kernel<T>), g, dim3(256), 0, ctx.stream, n, diag, p, s, r, wv, x,
+                         u, w.state.get(), 0);
+      hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv,
+                         w.state.get(), w.partial.get());
+      fin(kStepCg1Iter);
+    }
+    CFDHIP_CHECK(hipGetLastError());
+    CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState), hipMemcpyDeviceToHost,
+                                ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    if (w.hostState->done || launched >= set.max_iterations) break;
+  }
+  CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
+  CFDHIP_CHECK(hipEventSynchronize(t1));
+  float ms = 0.f;
+  CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  const KState& S = *w.hostState;
+  cfdhip_solve_result res{};
+  res.status = S.done ? S.status : CFDHIP_MAX_ITERATIONS;
+  res.iterations = S.iter;
+  res.initial_residual = S.b0;
+  res.final_residual = S.rnorm;
+  res.true_residual = S.rnorm;
+  res.device_ms = ms;
+  ctx.lastHistory.resize(S.histLen);
+  if (S.histLen > 0)
+    CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), w.hist.get(), S.histLen * sizeof(double),
+                           hipMemcpyDeviceToHost));
+  res.history_length = S.histLen;
+  return res;
+}
+
+template cfdhip_solve_result pcg3d<double>(Context&, const cfdhip_solver_settings&, const double*,
+                                           const double*, const double*, double*,
+                                           double* const[5]);
+template cfdhip_solve_result pcg3d<float>(Context&, const cfdhip_solver_settings&, const float*,
+                                          const float*, const float*, float*, float* const[5]);
+
+// Mixed precision in 3D: FP32 operator copy and FP32 inner PCG, FP64 true
+// residual and correction until the FP64 relative residual (against
+// |b - A x0|, x0 = 0) meets the tolerance.
+cfdhip_solve_result mixedPcg3d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
+                               double* x) {
+  const View& v = ctx.view;
+  const int64_t n = v.n, nf = v.nf;
+  KrylovWork& w = ctx.kw;
+  w.r32.ensure(n);
+  w.x32.ensure(n);
+  for (int q = 0; q < 5; ++q) w.f8[q].ensure(n);
+  w.dinv32.ensure(n);
+  w.w32.ensure(nf);
+  castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
+  castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
+  w.partial.ensure(std::max<int64_t>(w.partial.count, kMaxBlocks3 * 3 + 8));
+  w.state.ensure(1);
+  w.scal.ensure(4);
+  const dim3 g = grid3(n);
+  const Op3<double> op64{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                         ctx.f(CFDHIP_FIELD_PC_FACE_COEFF)};
+  hipEvent_t t0, t1;
+  CFDHIP_CHECK(hipEventCreate(&t0));
+  CFDHIP_CHECK(hipEventCreate(&t1));
+  CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
+  hipLaunchKernelGGL(zero3d, g, dim3(256), 0, ctx.stream, n, x);
+  std::vector<double> history;
+  double b0 = 0.0, rTrue = 0.0;
+  int64_t total = 0, cycles = 0;
+  int status = CFDHIP_MAX_ITERATIONS;
+  const double innerTol = 1e-5;
+  while (true) {
+    hipLaunchKernelGGL(refine3Kernel, g, dim3(256), 0, ctx.stream, op64, b, x, w.r32.get(),
+                       w.partial.get());
+    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                  nullptr, w.scal.get(), kStepNorm});
+    CFDHIP_CHECK(hipGetLastError());
+    CFDHIP_CHECK(hipMemcpyAsync(&rTrue, w.scal.get(), sizeof(double), hipMemcpyDeviceToHost,
+                                ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    if (cycles == 0) b0 = rTrue;
+    history.push_back(rTrue);
+    if (!std::isfinite(rTrue)) {
+      status = CFDHIP_NONFINITE_RESIDUAL;
+      break;
+    }
+    if (rTrue == 0.0 || rTrue <= set.absolute_tolerance ||
+        (b0 > 0.0 && rTrue / b0 <= set.relative_tolerance)) {
+      status = CFDHIP_CONVERGED;
+      break;
+    }
+    if (total >= set.max_iterations) break;
+    cfdhip_solver_settings inner = set;
+    inner.absolute_tolerance = 0.0;
+    inner.relative_tolerance = std::max(innerTol, set.relative_tolerance * b0 / rTrue);
+    inner.max_iterations = set.max_iterations - total;
+    hipLaunchKernelGGL(zero3, g, dim3(256), 0, ctx.stream, n, w.x32.get());
+    float* work[5];
+    for (int q = 0; q < 5; ++q) work[q] = w.f8[q].get();
+    const cfdhip_solve_result ir =
+        pcg3d<float>(ctx, inner, w.dinv32.get(), w.w32.get(), w.r32.get(), w.x32.get(), work);
+    total += ir.iterations;
+    ++cycles;
+    if (ir.iterations == 0 && ir.status != CFDHIP_CONVERGED) {
+      status = ir.status;
+      break;
+    }
+    hipLaunchKernelGGL(add3, g, dim3(256), 0, ctx.stream, n, w.x32.get(), x);
+    CFDHIP_CHECK(hipGetLastError());
+  }
+  CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
+  CFDHIP_CHECK(hipEventSynchronize(t1));
+  float ms = 0.f;
+  CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  cfdhip_solve_result res{};
+  res.status = status;
+  res.iterations = total;
+  res.restarts = cycles;
+  res.initial_residual = b0;
+  res.final_residual = rTrue;
+  res.true_residual = rTrue;
+  res.device_ms = ms;
+  ctx.lastHistory = history;
+  res.history_length = static_cast<int64_t>(history.size());
+  return res;
+}
+
+}  // namespace cfdhip

@@ -131,6 +131,12 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
 cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                                double* x);
 void castToFloat(Context& ctx, int64_t n, const double* src, float* dst);
+// cg3.hip: 3D Jacobi-PCG (work = r s w p u) and its mixed-precision refinement
+template <class T>
+cfdhip_solve_result pcg3d(Context& ctx, const cfdhip_solver_settings& set, const T* diag,
+                          const T* coeff, const T* b, T* x, T* const work[5]);
+cfdhip_solve_result mixedPcg3d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
+                               double* x);
 // fcg.hip: flexible CG with the learned preconditioner (FP32 inner, FP64
 // refinement, Jacobi-PCG fallback on stagnation)
 cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,

@@ -188,7 +188,7 @@ void applyM(Context& ctx, const T* r, const T* q, T* z, double* partial, int par
   const View& v = ctx.view;
   Unet& net = *ctx.net;
   net.forwardFrom(r, std::is_same_v<T, double>, ctx.kw.dis32.get(), static_cast<int>(v.ny),
-                  static_cast<int>(v.nx), 0, 1, h, w);
+                  static_cast<int>(v.nx), 0, 1, h, w, true);
   net.outputApply<T>(ctx.kw.dis32.get(), r, q, z, static_cast<int>(v.ny), static_cast<int>(v.nx),
                      ctx.pin, h, w, partial, parts);
 }

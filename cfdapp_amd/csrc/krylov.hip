@@ -714,11 +714,10 @@ cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s,
     }
     return learnedFcg2d(ctx, s, b, x);
   }
-  const bool fused = ctx.view.dim == 2 && s.type == CFDHIP_SOLVER_CG &&
-                     s.preconditioner == CFDHIP_PREC_JACOBI;
+  const bool fused = s.type == CFDHIP_SOLVER_CG && s.preconditioner == CFDHIP_PREC_JACOBI;
   if (s.precision == CFDHIP_MIXED)
     require(fused && zeroGuess,
-            "mixed precision is implemented for the 2D p' operator with CG + Jacobi from x0 = 0",
+            "mixed precision is implemented for the p' operator with CG + Jacobi from x0 = 0",
             CFDHIP_ERR_UNSUPPORTED);
   if (!fused) return dispatch(ctx, pressureOp(ctx), n, s, b, x, opArrays);
   // fused 2D Jacobi-PCG: same input checks and Jacobi rejection as runCG
@@ -739,14 +738,19 @@ cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s,
     ctx.lastHistory.clear();
     return res;
   }
-  if (s.precision == CFDHIP_MIXED) return mixedPcg2d(ctx, s, b, x);
+  if (s.precision == CFDHIP_MIXED)
+    return ctx.view.dim == 2 ? mixedPcg2d(ctx, s, b, x) : mixedPcg3d(ctx, s, b, x);
   double* work[8];
   for (int q = 0; q < 8; ++q) {
     w.d8[q].ensure(n);
     work[q] = w.d8[q].get();
   }
-  res = singleKernelPcg2d<double>(ctx, s, ctx.f(CFDHIP_FIELD_PC_DIAG),
-                                  ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, work);
+  if (ctx.view.dim == 2)
+    res = singleKernelPcg2d<double>(ctx, s, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                                    ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, work);
+  else
+    res = pcg3d<double>(ctx, s, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b,
+                        x, work);
   res.true_residual = trueResidual(ctx, A, n, b, x);
   return res;
 }

@@ -110,39 +110,91 @@ __global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ sr
   }
 }
 
-// y = conv(d0, 16 -> 1) at pixel (b, y, x).
-__device__ inline float lastConvAt(const float* __restrict__ d0, const float* ws, int b, int y, int x,
-                                   int h, int wd) {
-  float acc = 0.f;
+// y = conv(d0, 16 -> 1) on an 8 x 64 output tile: every pixel of the haloed
+// tile is read once and reduced against the nine taps (s[tap][pixel] in LDS),
+// then each output sums its nine neighbours' tap values.
+constexpr int kLTH = 8, kLTW = 64;
+constexpr int kLR = kLTH + 2, kLC = kLTW + 2, kLNP = kLR * kLC;
+
+struct LastTileLds {
+  float ws[144];
+  float s[9 * kLNP];
+};
+
+__device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, int b, int y0, int x0,
+                                int h, int wd, float out[2]) {
+  constexpr int PER = (kLNP + 255) / 256;
+  fx4 v[PER][4];
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap) {
-    const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-    if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
-    const fx4* src =
-        reinterpret_cast<const fx4*>(d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
+  for (int k = 0; k < PER; ++k) {
+    const int px = threadIdx.x + k * 256;
 #pragma unroll
-    for (int c4 = 0; c4 < 4; ++c4) {
-      const fx4 v = src[c4];
+    for (int c4 = 0; c4 < 4; ++c4) v[k][c4] = fx4{0.f, 0.f, 0.f, 0.f};
+    if (px < kLNP) {
+      const int ty = px / kLC, tx = px - ty * kLC;
+      const int yy = y0 - 1 + ty, xx = x0 - 1 + tx;
+      if (yy >= 0 && yy < h && xx >= 0 && xx < wd) {
+        const fx4* src = reinterpret_cast<const fx4*>(
+            d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc += v[k] * ws[tap * 16 + c4 * 4 + k];
+        for (int c4 = 0; c4 < 4; ++c4) v[k][c4] = src[c4];
+      }
     }
   }
-  return acc;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int px = threadIdx.x + k * 256;
+    if (px >= kLNP) continue;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      float dot = 0.f;
+#pragma unroll
+      for (int c4 = 0; c4 < 4; ++c4)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dot += v[k][c4][q] * L.ws[tap * 16 + c4 * 4 + q];
+      L.s[tap * kLNP + px] = dot;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int o = threadIdx.x + j * 256;
+    const int ty = o / kLTW, tx = o - ty * kLTW;
+    float acc = 0.f;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+      acc += L.s[tap * kLNP + (ty + tap / 3) * kLC + tx + tap % 3];
+    out[j] = acc;
+  }
+}
+
+__device__ inline void lastTileCoords(int64_t t, int tilesX, int tilesY, int& b, int& y0, int& x0) {
+  x0 = static_cast<int>(t % tilesX) * kLTW;
+  t /= tilesX;
+  y0 = static_cast<int>(t % tilesY) * kLTH;
+  b = static_cast<int>(t / tilesY);
 }
 
 // Training output: y (FP32, n x h x w).
 __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ d0,
                                                       const float* __restrict__ w,
                                                       float* __restrict__ y, int n, int h, int wd) {
-  __shared__ float ws[144];
-  if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
-  __syncthreads();
-  const int64_t total = static_cast<int64_t>(n) * h * wd;
-  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
-       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int x = static_cast<int>(p % wd);
-    const int64_t t = p / wd;
-    y[p] = lastConvAt(d0, ws, static_cast<int>(t / h), static_cast<int>(t % h), x, h, wd);
+  __shared__ LastTileLds L;
+  if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
+  const int tilesX = (wd + kLTW - 1) / kLTW, tilesY = (h + kLTH - 1) / kLTH;
+  const int64_t tiles = static_cast<int64_t>(tilesX) * tilesY * n;
+  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    int b, y0, x0;
+    lastTileCoords(t, tilesX, tilesY, b, y0, x0);
+    __syncthreads();
+    float out[2];
+    lastTile(d0, L, b, y0, x0, h, wd, out);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int o = threadIdx.x + j * 256;
+      const int yy = y0 + o / kLTW, xx = x0 + o % kLTW;
+      if (yy < h && xx < wd) y[(static_cast<int64_t>(b) * h + yy) * wd + xx] = out[j];
+    }
   }
 }
 
@@ -158,18 +210,28 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
                                                            TZ* __restrict__ z, int ny, int nx,
                                                            int64_t pin, int h, int wd,
                                                            double* partial) {
-  __shared__ float ws[144];
-  if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
-  __syncthreads();
-  const int64_t total = static_cast<int64_t>(ny) * nx;
+  __shared__ LastTileLds L;
+  if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
+  const int tilesX = (nx + kLTW - 1) / kLTW, tilesY = (ny + kLTH - 1) / kLTH;
+  const int64_t tiles = static_cast<int64_t>(tilesX) * tilesY;
   double acc[1] = {0.0};
-  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < total;
-       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
-    TZ v = static_cast<TZ>(lastConvAt(d0, ws, 0, y, x, h, wd) * scale[c]);
-    if (c == pin) v = r[c] * static_cast<TZ>(scale[c]) * static_cast<TZ>(scale[c]);
-    z[c] = v;
-    if (q != nullptr) acc[0] += static_cast<double>(v) * static_cast<double>(q[c]);
+  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    int b, y0, x0;
+    lastTileCoords(t, tilesX, tilesY, b, y0, x0);
+    __syncthreads();
+    float out[2];
+    lastTile(d0, L, 0, y0, x0, h, wd, out);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int o = threadIdx.x + j * 256;
+      const int yy = y0 + o / kLTW, xx = x0 + o % kLTW;
+      if (yy >= ny || xx >= nx) continue;
+      const int64_t c = static_cast<int64_t>(yy) * nx + xx;
+      TZ v = static_cast<TZ>(out[j] * scale[c]);
+      if (c == pin) v = r[c] * static_cast<TZ>(scale[c]) * static_cast<TZ>(scale[c]);
+      z[c] = v;
+      if (q != nullptr) acc[0] += static_cast<double>(v) * static_cast<double>(q[c]);
+    }
   }
   if (partial != nullptr) blockPartials<1>(acc, partial);
 }
@@ -446,6 +508,37 @@ void Unet::readTensor(int t, float* host, int64_t count) {
   CFDHIP_CHECK(hipStreamSynchronize(stream));
 }
 
+void Unet::setTiming(bool on) {
+  timing = on;
+  for (int l = 0; l < kNumLayers; ++l) {
+    layerMs[l] = 0.0;
+    layerCalls[l] = 0;
+  }
+}
+
+void Unet::mark(int layer, bool begin) {
+  if (!timing) return;
+  hipEvent_t e;
+  CFDHIP_CHECK(hipEventCreate(&e));
+  CFDHIP_CHECK(hipEventRecord(e, stream));
+  events.push_back(e);
+  eventLayer.push_back(begin ? layer : -1);
+}
+
+void Unet::collectTiming() {
+  if (!timing || events.empty()) return;
+  CFDHIP_CHECK(hipEventSynchronize(events.back()));
+  for (size_t i = 0; i + 1 < events.size(); i += 2) {
+    float ms = 0.f;
+    CFDHIP_CHECK(hipEventElapsedTime(&ms, events[i], events[i + 1]));
+    layerMs[eventLayer[i]] += ms;
+    layerCalls[eventLayer[i]] += 1;
+  }
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  events.clear();
+  eventLayer.clear();
+}
+
 void Unet::conv(int layer, bool transposed, int gather, int epi, int split, const float* a,
                 const float* b, float* out, float* aux, int n, int h, int w) {
   ConvArgs args{};
@@ -465,7 +558,17 @@ void Unet::conv(int layer, bool transposed, int gather, int epi, int split, cons
   args.nbTotal = co / 16;
   int ca = ci;
   if (gather == kGatherConcatUp) ca = ci - (layer == kD2a ? 64 : layer == kD1a ? 32 : 16);
+  if (gather == kGatherFirst) {
+    args.src = first.src;
+    args.scale = first.scale;
+    args.w0 = master.get() + layerOffset(kE0a);
+    args.srcH = first.srcH;
+    args.srcW = first.srcW;
+    args.srcDouble = first.srcDouble;
+  }
+  if (!transposed) mark(layer, true);
   launchConv(stream, ci, ca, co, gather, epi, split, args);
+  if (!transposed) mark(layer, false);
   CFDHIP_CHECK(hipGetLastError());
 }
 
@@ -503,20 +606,29 @@ void Unet::forward(const float* src, const float* scale, int srcH, int srcW, int
 }
 
 void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
-                       int64_t srcBatch, int n, int h, int w) {
+                       int64_t srcBatch, int n, int h, int w, bool fuseFirst) {
   if (!loaded) throw Error(CFDHIP_ERR_INVALID, "network weights are not loaded");
   const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
-  if (srcDouble)
-    hipLaunchKernelGGL((firstConvKernel<double>), grid1(p0), dim3(256), 0, stream,
-                       static_cast<const double*>(src), scale, srcH, srcW, srcBatch,
-                       master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
-  else
-    hipLaunchKernelGGL((firstConvKernel<float>), grid1(p0), dim3(256), 0, stream,
-                       static_cast<const float*>(src), scale, srcH, srcW, srcBatch,
-                       master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
-  CFDHIP_CHECK(hipGetLastError());
-  conv(kE0b, false, kGatherPlain, kEpiStorePool, 0, T(kTe0a), nullptr, T(kTe0), T(kTp0), n, h, w);
+  if (fuseFirst) {
+    // inference: e0a is computed inside e0b's tile staging and never stored
+    require(n == 1 || srcBatch == static_cast<int64_t>(srcH) * srcW,
+            "the fused first layer reads a dense batch");
+    first = FirstSource{src, scale, srcH, srcW, srcDouble ? 1 : 0};
+    conv(kE0b, false, kGatherFirst, kEpiStorePool, 0, nullptr, nullptr, T(kTe0), T(kTp0), n, h, w);
+  } else {
+    if (srcDouble)
+      hipLaunchKernelGGL((firstConvKernel<double>), grid1(p0), dim3(256), 0, stream,
+                         static_cast<const double*>(src), scale, srcH, srcW, srcBatch,
+                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+    else
+      hipLaunchKernelGGL((firstConvKernel<float>), grid1(p0), dim3(256), 0, stream,
+                         static_cast<const float*>(src), scale, srcH, srcW, srcBatch,
+                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+    CFDHIP_CHECK(hipGetLastError());
+    conv(kE0b, false, kGatherPlain, kEpiStorePool, 0, T(kTe0a), nullptr, T(kTe0), T(kTp0), n, h,
+         w);
+  }
   conv(kE1a, false, kGatherPlain, kEpiStore, 0, T(kTp0), nullptr, T(kTe1a), nullptr, n, h1, w1);
   conv(kE1b, false, kGatherPlain, kEpiStorePool, 0, T(kTe1a), nullptr, T(kTe1), T(kTp1), n, h1, w1);
   conv(kE2a, false, kGatherPlain, kEpiStore, 0, T(kTp1), nullptr, T(kTe2a), nullptr, n, h2, w2);
@@ -533,7 +645,7 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
 
 void Unet::outputTrain(int n, int h, int w) {
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
-  hipLaunchKernelGGL(lastConvKernel, grid1(p0), dim3(256), 0, stream, T(kTd0),
+  hipLaunchKernelGGL(lastConvKernel, grid1(p0 / 2, 4096), dim3(256), 0, stream, T(kTd0),
                      master.get() + layerOffset(kOut), y.get(), n, h, w);
   CFDHIP_CHECK(hipGetLastError());
 }
@@ -541,10 +653,13 @@ void Unet::outputTrain(int n, int h, int w) {
 template <class TZ>
 void Unet::outputApply(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
                        int64_t pin, int h, int w, double* partial, int parts) {
+  mark(kOut, true);
   hipLaunchKernelGGL((lastConvApplyKernel<TZ>), dim3(static_cast<unsigned>(parts)), dim3(256), 0,
                      stream, T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin,
                      h, w, partial);
+  mark(kOut, false);
   CFDHIP_CHECK(hipGetLastError());
+  collectTiming();
 }
 template void Unet::outputApply<float>(const float*, const float*, const float*, float*, int, int,
                                        int64_t, int, int, double*, int);

@@ -67,7 +67,8 @@ constexpr int64_t packedElems(int cin, int cout) { return packedSteps(cin) * (co
 enum Gather : int {
   kGatherPlain = 0,   // channels of a at (y, x)
   kGatherConcatUp,    // [a at (y/2, x/2) (ca channels), b at (y, x)]
-  kGatherSumUp        // a at (y, x) + 0.25 b at (y/2, x/2)  (gradient of pool + skip)
+  kGatherSumUp,       // a at (y, x) + 0.25 b at (y/2, x/2)  (gradient of pool + skip)
+  kGatherFirst        // conv(x, 1 -> 16) of the scalar input, computed while staging
 };
 // What the conv does with its FP32 accumulators.
 enum Epilogue : int {
@@ -85,6 +86,12 @@ struct ConvArgs {
   int n, h, w_;   // batch, image size of the output (= input) grid
   int tilesX, tilesY;
   int nbTotal;    // N-blocks of the layer (cout / 16)
+  // kGatherFirst: x = src * scale on srcH x srcW (row stride srcW, zero
+  // outside), first-layer master weights w0[tap][16]
+  const void* src;
+  const float* scale;
+  const float* w0;
+  int srcH, srcW, srcDouble;
 };
 
 struct DwArgs {

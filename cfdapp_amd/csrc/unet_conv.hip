@@ -68,37 +68,161 @@ __device__ inline fx4 gather4(const float* __restrict__ A, const float* __restri
   }
 }
 
-// Stages rows x cols pixels at origin (y0, x0) (zero outside the image),
-// channels [c0, c0 + CC) of the gathered input: hi[(ty cols + tx) cs + c] =
-// bf16(v), lo[...] = bf16(v - hi) (lo == nullptr: hi only).
-template <int C, int CA, int G, int CC>
-__device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, int rows, int cols, const float* A,
-                                  const float* B, int n, int h, int w, int y0, int x0, int c0) {
+// Stages ROWS x COLS pixels at origin (y0, x0) (zero outside the image),
+// channels [c0, c0 + CC) of the gathered input: hi[(ty COLS + tx) cs + c] =
+// bf16(v), lo[...] = bf16(v - hi) (lo == nullptr: hi only). Every load of the
+// thread's items is issued before the first conversion, so a thread waits on
+// one memory round trip, not one per item.
+template <int C, int CA, int G, int CC, int ROWS, int COLS>
+__device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, const float* B,
+                                  int n, int h, int w, int y0, int x0, int c0) {
   constexpr int C4 = CC / 4;
-  const int items = rows * cols * C4;
-  for (int it = threadIdx.x; it < items; it += blockDim.x) {
+  constexpr int ITEMS = ROWS * COLS * C4;
+  constexpr int PER = (ITEMS + 255) / 256;
+  fx4 v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int it = threadIdx.x + k * 256;
+    v[k] = fx4{0.f, 0.f, 0.f, 0.f};
+    if (it < ITEMS) {
+      const int c4 = it % C4;
+      const int px = it / C4;
+      const int ty = px / COLS, tx = px - ty * COLS;
+      const int y = y0 + ty, x = x0 + tx;
+      if (y >= 0 && y < h && x >= 0 && x < w) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int it = threadIdx.x + k * 256;
+    if (it >= ITEMS) continue;
     const int c4 = it % C4;
     const int px = it / C4;
-    const int ty = px / cols, tx = px - ty * cols;
-    const int y = y0 + ty, x = x0 + tx;
-    fx4 v = {0.f, 0.f, 0.f, 0.f};
-    if (y >= 0 && y < h && x >= 0 && x < w) v = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
-    u32x2 vh, vl;
     bf16 hb[4], lb[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      hb[k] = f2bf(v[k]);
-      lb[k] = f2bf(v[k] - bf2f(hb[k]));
+    for (int q = 0; q < 4; ++q) {
+      hb[q] = f2bf(v[k][q]);
+      lb[q] = f2bf(v[k][q] - bf2f(hb[q]));
     }
+    u32x2 vh;
     vh[0] = static_cast<uint32_t>(hb[0]) | (static_cast<uint32_t>(hb[1]) << 16);
     vh[1] = static_cast<uint32_t>(hb[2]) | (static_cast<uint32_t>(hb[3]) << 16);
-    const int o = (ty * cols + tx) * cs + c4 * 4;
+    const int o = px * cs + c4 * 4;
     *reinterpret_cast<u32x2*>(hi + o) = vh;
     if (lo != nullptr) {
+      u32x2 vl;
       vl[0] = static_cast<uint32_t>(lb[0]) | (static_cast<uint32_t>(lb[1]) << 16);
       vl[1] = static_cast<uint32_t>(lb[2]) | (static_cast<uint32_t>(lb[3]) << 16);
       *reinterpret_cast<u32x2*>(lo + o) = vl;
     }
+  }
+}
+
+// The two halves of stageSplit for software pipelining: loadChunk issues a
+// chunk's loads into registers, storeChunk converts and writes them to LDS.
+template <int C, int CA, int G, int CC, int ROWS, int COLS>
+struct ChunkStage {
+  static constexpr int C4 = CC / 4;
+  static constexpr int ITEMS = ROWS * COLS * C4;
+  static constexpr int PER = (ITEMS + 255) / 256;
+  fx4 v[PER];
+  __device__ inline void load(const float* A, const float* B, int n, int h, int w, int y0, int x0,
+                              int c0) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int it = threadIdx.x + k * 256;
+      v[k] = fx4{0.f, 0.f, 0.f, 0.f};
+      if (it < ITEMS) {
+        const int c4 = it % C4;
+        const int px = it / C4;
+        const int ty = px / COLS, tx = px - ty * COLS;
+        const int y = y0 + ty, x = x0 + tx;
+        if (y >= 0 && y < h && x >= 0 && x < w)
+          v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+      }
+    }
+  }
+  __device__ inline void store(bf16* hi, bf16* lo, int cs) const {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int it = threadIdx.x + k * 256;
+      if (it >= ITEMS) continue;
+      const int c4 = it % C4;
+      const int px = it / C4;
+      bf16 hb[4], lb[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        hb[q] = f2bf(v[k][q]);
+        lb[q] = f2bf(v[k][q] - bf2f(hb[q]));
+      }
+      u32x2 vh, vl;
+      vh[0] = static_cast<uint32_t>(hb[0]) | (static_cast<uint32_t>(hb[1]) << 16);
+      vh[1] = static_cast<uint32_t>(hb[2]) | (static_cast<uint32_t>(hb[3]) << 16);
+      vl[0] = static_cast<uint32_t>(lb[0]) | (static_cast<uint32_t>(lb[1]) << 16);
+      vl[1] = static_cast<uint32_t>(lb[2]) | (static_cast<uint32_t>(lb[3]) << 16);
+      const int o = px * cs + c4 * 4;
+      *reinterpret_cast<u32x2*>(hi + o) = vh;
+      *reinterpret_cast<u32x2*>(lo + o) = vl;
+    }
+  }
+};
+
+__device__ inline void splitStore(fx4 v, bf16* hi, bf16* lo, int o) {
+  bf16 hb[4], lb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    hb[k] = f2bf(v[k]);
+    lb[k] = f2bf(v[k] - bf2f(hb[k]));
+  }
+  u32x2 vh, vl;
+  vh[0] = static_cast<uint32_t>(hb[0]) | (static_cast<uint32_t>(hb[1]) << 16);
+  vh[1] = static_cast<uint32_t>(hb[2]) | (static_cast<uint32_t>(hb[3]) << 16);
+  vl[0] = static_cast<uint32_t>(lb[0]) | (static_cast<uint32_t>(lb[1]) << 16);
+  vl[1] = static_cast<uint32_t>(lb[2]) | (static_cast<uint32_t>(lb[3]) << 16);
+  *reinterpret_cast<u32x2*>(hi + o) = vh;
+  *reinterpret_cast<u32x2*>(lo + o) = vl;
+}
+
+// The first layer fused into the staging of the second: the 16 channels of
+// e0a = conv(x, 1 -> 16) on the (kInRows x kInCols) tile with origin (y0, x0),
+// from x staged once with its own halo. Same tap order and FP32 arithmetic as
+// the stand-alone first-layer kernel, so the values are identical.
+template <class TS>
+__device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a, int n, int y0,
+                                  int x0) {
+  constexpr int XR = kInRows + 2, XC = kInCols + 2;
+  __shared__ float xs[XR * XC];
+  __shared__ float w0s[144];
+  if (threadIdx.x < 144) w0s[threadIdx.x] = bf2f(f2bf(a.w0[threadIdx.x]));
+  const TS* src = static_cast<const TS*>(a.src);
+  for (int i = threadIdx.x; i < XR * XC; i += blockDim.x) {
+    const int ty = i / XC, tx = i - ty * XC;
+    const int y = y0 - 1 + ty, x = x0 - 1 + tx;
+    float v = 0.f;
+    if (y >= 0 && y < a.srcH && x >= 0 && x < a.srcW) {
+      const int64_t si = static_cast<int64_t>(n) * a.srcH * a.srcW +
+                         static_cast<int64_t>(y) * a.srcW + x;
+      const float sv = static_cast<float>(src[si]);
+      v = a.scale != nullptr ? sv * a.scale[si] : sv;
+    }
+    xs[i] = v;
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < kInPix * 4; it += blockDim.x) {
+    const int c4 = it & 3;
+    const int px = it >> 2;
+    const int ty = px / kInCols, tx = px - ty * kInCols;
+    const int y = y0 + ty, x = x0 + tx;
+    fx4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (y >= 0 && y < a.h && x >= 0 && x < a.w_) {
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const float v = xs[(ty + tap / 3) * XC + tx + tap % 3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] += v * w0s[tap * 16 + c4 * 4 + k];
+      }
+    }
+    splitStore(acc, hi, lo, px * cs + c4 * 4);
   }
 }
 
@@ -136,13 +260,34 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   const int y0 = ty * kTH, x0 = tx * kTW;
   const int nb0 = blockIdx.y * NBG;
 
+  // the first chunk's loads go out before the weights are staged; chunk
+  // ch + 1's loads are in flight while chunk ch is multiplied
+  ChunkStage<CIN, CA, (G == kGatherFirst ? kGatherPlain : G), CC, kInRows, kInCols> stageBuf;
+  if constexpr (G != kGatherFirst) stageBuf.load(a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1, 0);
+
   // this group's weight fragments: packed [(chunk step) * nbTotal + nb] * 64 + lane
   const u32x4* __restrict__ wp = reinterpret_cast<const u32x4*>(a.w);
-  for (int i = threadIdx.x; i < W_FRAGS; i += blockDim.x) {
-    const int lane = i & 63;
-    const int t = i >> 6;
-    const int nb = t % NBG, qm = t / NBG;
-    wl[i] = wp[(static_cast<int64_t>(qm) * a.nbTotal + nb0 + nb) * 64 + lane];
+  {
+    constexpr int PER = (W_FRAGS + 255) / 256;
+    constexpr int BATCH = PER < 8 ? PER : 8;
+#pragma unroll
+    for (int k0 = 0; k0 < PER; k0 += BATCH) {
+      u32x4 t[BATCH];
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k) {
+        const int i = threadIdx.x + (k0 + k) * 256;
+        if (k0 + k < PER && i < W_FRAGS) {
+          const int ln = i & 63, f = i >> 6;
+          const int nb = f % NBG, qm = f / NBG;
+          t[k] = wp[(static_cast<int64_t>(qm) * a.nbTotal + nb0 + nb) * 64 + ln];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k) {
+        const int i = threadIdx.x + (k0 + k) * 256;
+        if (k0 + k < PER && i < W_FRAGS) wl[i] = t[k];
+      }
+    }
   }
 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -159,9 +304,19 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
 
   for (int ch = 0; ch < NCH; ++ch) {
     if (ch > 0) __syncthreads();  // the previous chunk is consumed
-    stageSplit<CIN, CA, G, CC>(hiT, loT, CS, kInRows, kInCols, a.a, a.b, n, a.h, a.w_, y0 - 1,
-                               x0 - 1, ch * CC);
+    if constexpr (G == kGatherFirst) {
+      if (a.srcDouble)
+        stageFirst<double>(hiT, loT, CS, a, n, y0 - 1, x0 - 1);
+      else
+        stageFirst<float>(hiT, loT, CS, a, n, y0 - 1, x0 - 1);
+    } else {
+      stageBuf.store(hiT, loT, CS);
+    }
     __syncthreads();
+    if constexpr (G != kGatherFirst && NCH > 1) {
+      if (ch + 1 < NCH)
+        stageBuf.load(a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1, (ch + 1) * CC);
+    }
 #pragma unroll 2
     for (int m = 0; m < MSC; ++m) {
       const int kk = 32 * m + 8 * g;
@@ -281,10 +436,17 @@ __global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
     const int n = static_cast<int>(tile / a.tilesY);
     const int y0 = ty * kTH, x0 = tx * kTW;
     __syncthreads();
-    stageSplit<CIN, CA, GX, CIN>(xs, nullptr, CS, kInRows, kInCols, a.xa, a.xb, n, a.h, a.w_,
-                                 y0 - 1, x0 - 1, 0);
-    stageSplit<COUT, COUT, GG, COUT>(gs, nullptr, GS, kTH, kTW, a.ga, a.gb, n, a.h, a.w_, y0, x0,
-                                     0);
+    if constexpr (CIN <= 64) {
+      stageSplit<CIN, CA, GX, CIN, kInRows, kInCols>(xs, nullptr, CS, a.xa, a.xb, n, a.h, a.w_,
+                                                     y0 - 1, x0 - 1, 0);
+    } else {  // two halves keep the register batch small
+      stageSplit<CIN, CA, GX, CIN / 2, kInRows, kInCols>(xs, nullptr, CS, a.xa, a.xb, n, a.h,
+                                                         a.w_, y0 - 1, x0 - 1, 0);
+      stageSplit<CIN, CA, GX, CIN / 2, kInRows, kInCols>(xs + CIN / 2, nullptr, CS, a.xa, a.xb, n,
+                                                         a.h, a.w_, y0 - 1, x0 - 1, CIN / 2);
+    }
+    stageSplit<COUT, COUT, GG, COUT, kTH, kTW>(gs, nullptr, GS, a.ga, a.gb, n, a.h, a.w_, y0, x0,
+                                               0);
     __syncthreads();
     for (int s = 0; s < kTH; ++s) {
       u32x4 bfr[NOB];
@@ -346,15 +508,16 @@ void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, i
     return;                                                                                \
   }
   // forward
+  CFDHIP_CONV(16, 16, 16, 1, kGatherFirst, kEpiStorePool, 0)
   CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStorePool, 0)
   CFDHIP_CONV(16, 16, 32, 2, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStorePool, 0)
-  CFDHIP_CONV(32, 32, 64, 2, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(64, 64, 64, 2, kGatherPlain, kEpiStorePool, 0)
-  CFDHIP_CONV(64, 64, 64, 2, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(128, 64, 64, 2, kGatherConcatUp, kEpiStore, 0)
-  CFDHIP_CONV(96, 64, 32, 2, kGatherConcatUp, kEpiStore, 0)
-  CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 1, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(32, 32, 64, 1, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(64, 64, 64, 1, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(64, 64, 64, 1, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(128, 64, 64, 1, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(96, 64, 32, 1, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 1, kGatherPlain, kEpiStore, 0)
   CFDHIP_CONV(48, 32, 16, 1, kGatherConcatUp, kEpiStore, 0)
   CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStore, 0)
   // transposed (gradient of the input)

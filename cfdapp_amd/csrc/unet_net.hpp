@@ -1,6 +1,8 @@
 // Device state of the learned preconditioner's network.
 #pragma once
 
+#include <vector>
+
 #include "unet.hpp"
 
 namespace cfdhip {
@@ -37,8 +39,10 @@ class Unet {
   void forward(const float* src, const float* scale, int srcH, int srcW, int64_t srcBatch, int n,
                int h, int w);
   // The same from an FP32 or FP64 source.
+  // fuseFirst: compute the first layer inside the second's tile staging
+  // (inference only: the first layer's output is not stored).
   void forwardFrom(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
-                   int64_t srcBatch, int n, int h, int w);
+                   int64_t srcBatch, int n, int h, int w, bool fuseFirst = false);
   // y = conv(d0) into the FP32 training output.
   void outputTrain(int n, int h, int w);
   // z = scale * conv(d0) on the ny x nx solver grid, z[pin] = r[pin] scale^2
@@ -61,6 +65,12 @@ class Unet {
   float* outputGrad() { return gy.get(); }
   hipStream_t stream;
 
+  // Per-layer kernel timing (HIP events on the stream around each layer's
+  // launch; the host waits at the end of every forward pass while enabled).
+  void setTiming(bool on);
+  double layerMs[kNumLayers] = {};
+  int64_t layerCalls[kNumLayers] = {};
+
  private:
   void repack();
   float* T(int t) { return act.get() + tensorOff[t]; }
@@ -77,9 +87,19 @@ class Unet {
   int64_t fwdOff[kNumLayers] = {};
   int64_t bwdOff[kNumLayers] = {};
   int64_t tensorOff[kNumTensors] = {};
+  struct FirstSource {
+    const void* src = nullptr;
+    const float* scale = nullptr;
+    int srcH = 0, srcW = 0, srcDouble = 0;
+  } first;
   int shapeN = 0, shapeH = 0, shapeW = 0;
   bool shapeTraining = false;
   bool loaded = false;
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> eventLayer;
+  void mark(int layer, bool begin);
+  void collectTiming();
   int64_t adamStep = 0;
 };
 

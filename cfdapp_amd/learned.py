@@ -114,7 +114,7 @@ def assemble_pprime(d, warm_iterations, alpha_u=0.7):
     pres = LinearSolverSettings(0.0, 1e-8, 200000, "CG", "Jacobi", check_interval=64)
     if warm_iterations > 0:
         d.simple_run(warm_iterations, pressure=pres, momentum=mom)
-    d.mass_flux()
+    d._ctx.call("cfdhip_mass_flux")
     d.assemble_momentum_resident(alpha_u)
     for k in range(2):
         d.solve_momentum_resident(k, mom)

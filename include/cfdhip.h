@@ -242,6 +242,10 @@ int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t*
  * stream), out[2] = iterations timed. Filled when timing is enabled. */
 int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled);
 int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out);
+/* Learned preconditioner kernels timed while timing is enabled: for each of the
+ * 15 layers (e0a .. out; e0a runs inside e0b's kernel on the solve path),
+ * total milliseconds and launches since timing was enabled. */
+int cfdhip_preconditioner_timing(cfdhip_ctx* ctx, double* ms, int64_t* calls);
 
 /* ---- learned preconditioner ------------------------------------------------------ */
 /* No counterpart in the reference: the p' solve with preconditioner
@@ -283,6 +287,10 @@ int cfdhip_unet_get_weights(cfdhip_unet* net, float* weights, int64_t count);
 /* y = N(x). */
 int cfdhip_unet_forward(cfdhip_unet* net, const float* x, float* y, int64_t batch, int64_t h,
                         int64_t w);
+/* y = N(x) along the preconditioner's inference path (the first layer fused
+ * into the second; its output is not kept). Same values as cfdhip_unet_forward. */
+int cfdhip_unet_apply(cfdhip_unet* net, const float* x, float* y, int64_t batch, int64_t h,
+                      int64_t w);
 /* grad (host, param_count floats) = d/dW of sum(gy * N(x)). */
 int cfdhip_unet_backward(cfdhip_unet* net, const float* x, const float* gy, int64_t batch, int64_t h,
                          int64_t w, float* grad);

@@ -95,15 +95,18 @@ def test_pressure_cg_solution(name):
     assert r.final_residual <= 1e-8 * r.initial_residual
 
 
-@pytest.mark.parametrize("shape", [(129, 129), (257, 96)])
+@pytest.mark.parametrize("shape", [(129, 129), (257, 96), (40, 36, 24)])
 def test_mixed_precision_pressure_solve(shape):
     """FP32 inner PCG + FP64 refinement reaches the FP64 true-residual target
-    and the oracle's FP64 CG solution (within what that target guarantees)."""
+    and the oracle's FP64 CG solution (within what that target guarantees);
+    2D (single-kernel iteration) and 3D (7-point, two kernels per iteration)."""
     print_case("mixed", shape)
-    nx, ny = shape
-    d = cfd.GpuSimpleDiscretization.cavity(2, nx, ny)
-    c = oracle.Case.cavity(2, nx, ny)
-    rng = np.random.default_rng(nx + ny)
+    dim = len(shape)
+    nx, ny = shape[:2]
+    nz = shape[2] if dim == 3 else 1
+    d = cfd.GpuSimpleDiscretization.cavity(dim, nx, ny, nz)
+    c = oracle.Case.cavity(dim, nx, ny, nz)
+    rng = np.random.default_rng(nx + ny + nz)
     D = rng.uniform(0.5, 2.0, d.n_cells)  # variable response coefficients
     Fs = rng.uniform(-1, 1, d.n_faces)
     d.upload("RESPONSE", D)

@@ -55,6 +55,19 @@ def test_forward_matches_oracle(n, h, w):
     assert _rel(y, y64[..., 0]) < 1e-3
 
 
+@pytest.mark.parametrize("n,h,w", [(1, 64, 96), (2, 24, 40)])
+def test_fused_first_layer_is_identical(n, h, w):
+    wts = unet.init_weights(3)
+    net = ca.UnetNet(unet.flatten(wts))
+    x = np.random.default_rng(7).standard_normal((n, h, w)).astype(np.float32)
+    xd = _dev(x)
+    y0 = torch.empty_like(xd)
+    y1 = torch.empty_like(xd)
+    net.forward(xd, y0, n, h, w)
+    net.apply(xd, y1, n, h, w)
+    np.testing.assert_array_equal(y0.cpu().numpy(), y1.cpu().numpy())
+
+
 @pytest.mark.parametrize("n,h,w", [(2, 32, 64), (1, 24, 40)])
 def test_backward_matches_oracle(n, h, w):
     print(f"backward n={n} h={h} w={w}", flush=True)
