@@ -546,6 +546,15 @@ class UnetNet:
         self._call("cfdhip_unet_tensor", t, out.ctypes.data_as(_lib.P_F32), m)
         return out
 
+    def loss_grad(self, x, e, ce, cn, sys_offset, den, batch, h, w):
+        """A-norm loss and its gradient in the master weights (no update)."""
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(unet_param_count(), dtype=np.float32)
+        self._call("cfdhip_unet_loss_grad", _dev(x), _dev(e), _dev(ce), _dev(cn),
+                   _dev(sys_offset), _dev(den), int(batch), int(h), int(w), ctypes.byref(loss),
+                   g.ctypes.data_as(_lib.P_F32))
+        return loss.value, g
+
     def train_step(self, x, e, ce, cn, sys_offset, den, batch, h, w, lr, update=True) -> float:
         loss = ctypes.c_double(0.0)
         self._call("cfdhip_unet_train_step", _dev(x), _dev(e), _dev(ce), _dev(cn),

@@ -3,9 +3,9 @@
 // over three operators: the resident p' operator (diagonal + face couplings),
 // a 5-/7-point stencil (the momentum matrix) and a general CSR matrix.
 //
-// Every inner product is a two-stage deterministic reduction: wave shuffles,
-// block partials, and the block that arrives last sums all partials in a
-// fixed order and evaluates the solver's scalar recurrence and its
+// Every inner product is a two-stage deterministic reduction: wave shuffles
+// give block partials, then a one-block finalize kernel sums the partials in
+// a fixed order and evaluates the solver's scalar recurrence and its
 // convergence / breakdown tests on the device. The host reads the verdict
 // only every check_interval iterations, so an iteration costs no host round
 // trip (the reference pays 5 host syncs per BiCGSTAB iteration,

@@ -32,6 +32,10 @@ typedef float fx4 __attribute__((ext_vector_type(4)));
 constexpr int kTH = 8, kTW = 32;
 constexpr int kInRows = kTH + 2, kInCols = kTW + 2;
 constexpr int kInPix = kInRows * kInCols;
+// The upsampled half of a concatenated input is staged at coarse resolution:
+// fine halo rows y0 - 1 .. y0 + kTH map to coarse rows y0/2 - 1 .. y0/2 +
+// kTH/2 (tile row t -> coarse row (t + 1) >> 1), likewise for columns.
+constexpr int kCoRows = kTH / 2 + 2, kCoCols = kTW / 2 + 2;
 
 __device__ inline float bf2f(bf16 h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
@@ -208,8 +212,15 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
     xs[i] = v;
   }
   __syncthreads();
+  // a thread always handles the same 4 channels (blockDim is a multiple of 4):
+  // their 36 weights stay in registers
+  const int c4 = threadIdx.x & 3;
+  float wr[36];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wr[tap * 4 + k] = w0s[tap * 16 + c4 * 4 + k];
   for (int it = threadIdx.x; it < kInPix * 4; it += blockDim.x) {
-    const int c4 = it & 3;
     const int px = it >> 2;
     const int ty = px / kInCols, tx = px - ty * kInCols;
     const int y = y0 + ty, x = x0 + tx;
@@ -219,7 +230,7 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
       for (int tap = 0; tap < 9; ++tap) {
         const float v = xs[(ty + tap / 3) * XC + tx + tap % 3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc[k] += v * w0s[tap * 16 + c4 * 4 + k];
+        for (int k = 0; k < 4; ++k) acc[k] += v * wr[tap * 4 + k];
       }
     }
     splitStore(acc, hi, lo, px * cs + c4 * 4);
@@ -261,9 +272,28 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   const int nb0 = blockIdx.y * NBG;
 
   // the first chunk's loads go out before the weights are staged; chunk
-  // ch + 1's loads are in flight while chunk ch is multiplied
-  ChunkStage<CIN, CA, (G == kGatherFirst ? kGatherPlain : G), CC, kInRows, kInCols> stageBuf;
-  if constexpr (G != kGatherFirst) stageBuf.load(a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1, 0);
+  // ch + 1's loads are in flight while chunk ch is multiplied. For a
+  // concatenation, chunks below CA come from the coarse tensor at coarse
+  // resolution (a third of the pixels), the rest from the fine tensor.
+  constexpr bool kConcat = G == kGatherConcatUp;
+  constexpr int NCO = kConcat ? CA / CC : 0;  // coarse chunks (CC divides CA)
+  constexpr int CF = kConcat ? CIN - CA : CIN;  // channels of the tensor stageBuf reads
+  ChunkStage<CF, kConcat ? CF : CA, (G == kGatherFirst || kConcat) ? kGatherPlain : G, CC, kInRows,
+             kInCols>
+      stageBuf;
+  ChunkStage<CA, CA, kGatherPlain, CC, kCoRows, kCoCols> coarseBuf;
+  const int cy0 = (y0 >> 1) - 1, cx0 = (x0 >> 1) - 1;
+  auto loadChunk = [&](int ch) {
+    if constexpr (kConcat) {
+      if (ch < NCO)
+        coarseBuf.load(a.a, nullptr, n, a.h >> 1, a.w_ >> 1, cy0, cx0, ch * CC);
+      else
+        stageBuf.load(a.b, nullptr, n, a.h, a.w_, y0 - 1, x0 - 1, ch * CC - CA);
+    } else if constexpr (G != kGatherFirst) {
+      stageBuf.load(a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1, ch * CC);
+    }
+  };
+  loadChunk(0);
 
   // this group's weight fragments: packed [(chunk step) * nbTotal + nb] * 64 + lane
   const u32x4* __restrict__ wp = reinterpret_cast<const u32x4*>(a.w);
@@ -309,14 +339,16 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
         stageFirst<double>(hiT, loT, CS, a, n, y0 - 1, x0 - 1);
       else
         stageFirst<float>(hiT, loT, CS, a, n, y0 - 1, x0 - 1);
+    } else if (kConcat && ch < NCO) {
+      coarseBuf.store(hiT, loT, CS);
     } else {
       stageBuf.store(hiT, loT, CS);
     }
     __syncthreads();
     if constexpr (G != kGatherFirst && NCH > 1) {
-      if (ch + 1 < NCH)
-        stageBuf.load(a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1, (ch + 1) * CC);
+      if (ch + 1 < NCH) loadChunk(ch + 1);
     }
+    const bool coarse = kConcat && ch < NCO;
 #pragma unroll 2
     for (int m = 0; m < MSC; ++m) {
       const int kk = 32 * m + 8 * g;
@@ -325,11 +357,14 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
         const int tap = kk / CC;
         const int c0 = kk - tap * CC;
         const int ky = tap / 3, kx = tap - 3 * ky;
-        const int base = ((ayl + ky) * kInCols + axl + kx) * CS + c0;
+        // fine: tile pixel (ayl + ky, axl + kx + 8 q); coarse: its (t + 1) >> 1
+        const int base = coarse ? (((ayl + ky + 1) >> 1) * kCoCols + ((axl + kx + 1) >> 1)) * CS + c0
+                                : ((ayl + ky) * kInCols + axl + kx) * CS + c0;
+        const int qs = coarse ? 4 * CS : 8 * CS;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          ah[q] = *reinterpret_cast<const u32x4*>(hiT + base + 8 * q * CS);
-          al[q] = *reinterpret_cast<const u32x4*>(loT + base + 8 * q * CS);
+          ah[q] = *reinterpret_cast<const u32x4*>(hiT + base + q * qs);
+          al[q] = *reinterpret_cast<const u32x4*>(loT + base + q * qs);
         }
       } else {
 #pragma unroll
@@ -511,13 +546,13 @@ void launchConv(hipStream_t s, int cin, int ca, int cout, int gather, int epi, i
   CFDHIP_CONV(16, 16, 16, 1, kGatherFirst, kEpiStorePool, 0)
   CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStorePool, 0)
   CFDHIP_CONV(16, 16, 32, 2, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(32, 32, 32, 1, kGatherPlain, kEpiStorePool, 0)
-  CFDHIP_CONV(32, 32, 64, 1, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(64, 64, 64, 1, kGatherPlain, kEpiStorePool, 0)
-  CFDHIP_CONV(64, 64, 64, 1, kGatherPlain, kEpiStore, 0)
-  CFDHIP_CONV(128, 64, 64, 1, kGatherConcatUp, kEpiStore, 0)
-  CFDHIP_CONV(96, 64, 32, 1, kGatherConcatUp, kEpiStore, 0)
-  CFDHIP_CONV(32, 32, 32, 1, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(32, 32, 64, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(64, 64, 64, 2, kGatherPlain, kEpiStorePool, 0)
+  CFDHIP_CONV(64, 64, 64, 2, kGatherPlain, kEpiStore, 0)
+  CFDHIP_CONV(128, 64, 64, 2, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(96, 64, 32, 2, kGatherConcatUp, kEpiStore, 0)
+  CFDHIP_CONV(32, 32, 32, 2, kGatherPlain, kEpiStore, 0)
   CFDHIP_CONV(48, 32, 16, 1, kGatherConcatUp, kEpiStore, 0)
   CFDHIP_CONV(16, 16, 16, 1, kGatherPlain, kEpiStore, 0)
   // transposed (gradient of the input)

@@ -89,6 +89,157 @@ def structured_init(seed=0, ratio=8.0, noise=0.0) -> np.ndarray:
     return flat
 
 
+class SymmetricMultilevel:
+    """A symmetric parametrisation of the network: theta -> weights.
+
+    The level-l term of M r = sum_l w_l P_l S_l R_l r uses a restriction
+    R = average(K_j r) per level and a prolongation P = K_j(upsample) that is
+    its transpose (K_j a 3 x 3 kernel symmetric under 90-degree rotations:
+    centre, edge and corner weights), and a symmetric level kernel S_l. Every
+    theta gives a symmetric operator, so flexible CG keeps the behaviour of
+    preconditioned CG; training moves theta with the gradients the device's
+    backward pass returns for the full weight set (dL/dtheta = J^T dL/dW,
+    J by central differences of the construction, exact for its linear and
+    bilinear terms).
+    theta = [log w_0..3, K_0..2 (3 each), S_0..3 (3 each), log scale].
+    """
+    N_THETA = 4 + 9 + 12 + 1
+
+    @staticmethod
+    def initial(ratio=8.0):
+        t = np.zeros(SymmetricMultilevel.N_THETA)
+        t[0:4] = np.log([1.0, ratio, ratio ** 2, ratio ** 3])
+        for j in range(3):                       # binomial kernels
+            t[4 + 3 * j:7 + 3 * j] = [4 / 16, 2 / 16, 1 / 16]
+        for j in range(3):                       # identity level kernels
+            t[13 + 3 * j:16 + 3 * j] = [1.0, 0.0, 0.0]
+        t[22:25] = [4 / 16, 2 / 16, 1 / 16]      # the coarsest level smooths
+        t[25] = 0.0
+        return t
+
+    @staticmethod
+    def kernel(c, e, k):
+        return np.array([[k, e, k], [e, c, e], [k, e, k]])
+
+    @staticmethod
+    def weights(theta) -> np.ndarray:
+        th = np.asarray(theta, np.float64)
+        wl = np.exp(th[0:4])
+        K = [SymmetricMultilevel.kernel(*th[4 + 3 * j:7 + 3 * j]) for j in range(3)]
+        S = [SymmetricMultilevel.kernel(*th[13 + 3 * j:16 + 3 * j]) for j in range(4)]
+        scale = np.exp(th[25])
+        w = {name: np.zeros((3, 3, ci, co)) for name, ci, co in LAYERS}
+
+        def ident(name, o, i, s=1.0):
+            w[name][1, 1, i, o] += s
+
+        ident("e0a", 0, 0)
+        ident("e0a", 1, 0)
+        for lvl, b in enumerate(("e0b", "e1b", "e2b")):
+            ident(b, 0, 0)
+            w[b][:, :, 1, 1] += K[lvl]
+        for a in ("e1a", "e2a"):
+            ident(a, 0, 1)
+            ident(a, 1, 1)
+        ident("b3a", 0, 1)
+        w["b3b"][:, :, 0, 0] += wl[3] * S[3]
+        w["d2a"][:, :, 0, 0] += K[2]
+        w["d2a"][:, :, 64, 0] += wl[2] * S[2]
+        w["d1a"][:, :, 0, 0] += K[1]
+        w["d1a"][:, :, 64, 0] += wl[1] * S[1]
+        w["d0a"][:, :, 0, 0] += K[0]
+        w["d0a"][:, :, 32, 0] += wl[0] * S[0]
+        for n in ("d2b", "d1b", "d0b"):
+            ident(n, 0, 0)
+        ident("out", 0, 0, scale)
+        return np.concatenate([w[name].astype(np.float32).ravel() for name, _, _ in LAYERS])
+
+    @staticmethod
+    def _slot(name, ci, co):
+        """Flat indices of the 3 x 3 kernel W[:, :, ci, co] of a layer, row-major (ky, kx)."""
+        off = 0
+        for nm, cin, cout in LAYERS:
+            if nm == name:
+                return np.array([off + ((ky * 3 + kx) * cin + ci) * cout + co
+                                 for ky in range(3) for kx in range(3)])
+            off += 9 * cin * cout
+        raise KeyError(name)
+
+    # centre / edge / corner positions of a 3 x 3 kernel (row-major)
+    _MASKS = (np.array([0, 0, 0, 0, 1, 0, 0, 0, 0.]), np.array([0, 1, 0, 1, 0, 1, 0, 1, 0.]),
+              np.array([1, 0, 1, 0, 0, 0, 1, 0, 1.]))
+
+    @staticmethod
+    def theta_grad(theta, gw):
+        """J^T gw: dL/dtheta from dL/dW (the construction is linear in the
+        kernels and in the level weights, so J is exact)."""
+        S_ = SymmetricMultilevel
+        th = np.asarray(theta, np.float64)
+        gw = np.asarray(gw, np.float64)
+        wl = np.exp(th[0:4])
+        skip = [("d0a", 32), ("d1a", 64), ("d2a", 64), ("b3b", 0)]
+        g = np.zeros(len(th))
+        for lvl in range(4):
+            name, ci = skip[lvl]
+            gs = gw[S_._slot(name, ci, 0)]
+            Sk = S_.kernel(*th[13 + 3 * lvl:16 + 3 * lvl]).ravel()
+            g[lvl] = wl[lvl] * float(gs @ Sk)                  # d / d log w_l
+            for q in range(3):
+                g[13 + 3 * lvl + q] = wl[lvl] * float(gs @ S_._MASKS[q])
+        restr = [("e0b", 1, 1), ("e1b", 1, 1), ("e2b", 1, 1)]
+        prol = [("d0a", 0, 0), ("d1a", 0, 0), ("d2a", 0, 0)]
+        for j in range(3):
+            gk = gw[S_._slot(*restr[j])] + gw[S_._slot(*prol[j])]
+            for q in range(3):
+                g[4 + 3 * j + q] = float(gk @ S_._MASKS[q])
+        g[25] = np.exp(th[25]) * float(gw[S_._slot("out", 0, 0)][4])
+        return g
+
+
+def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_every=25,
+                    eval_fn=None, eval_every=0):
+    """Adam on theta with the device's loss gradients; returns (theta, losses)."""
+    net = UnetNet(SymmetricMultilevel.weights(theta))
+    rng = np.random.default_rng(seed)
+    sizes = [n for n in bank.count if bank.count[n] > 0]
+    probs = np.array([bank.count[n] * n * n for n in sizes], np.float64)
+    probs /= probs.sum()
+    m = np.zeros(len(theta))
+    v = np.zeros(len(theta))
+    b1, b2 = 0.9, 0.999
+    losses, evals = [], []
+    t0 = time.time()
+    theta = np.array(theta, np.float64)
+    for step in range(steps):
+        if deadline is not None and time.time() > deadline:
+            _log(f"train: time budget reached at step {step}")
+            break
+        n = sizes[int(rng.choice(len(sizes), p=probs))]
+        b = min(BATCH[n], bank.count[n])
+        idx = rng.choice(bank.count[n], size=b, replace=False)
+        it = torch.from_numpy(idx).cuda()
+        x = bank.r[n].index_select(0, it)
+        e = bank.e[n].index_select(0, it)
+        off = torch.from_numpy(bank.sys[n][idx] * n * n).cuda()
+        den = torch.from_numpy(bank.den[n][idx]).cuda()
+        net.set_weights(SymmetricMultilevel.weights(theta))
+        loss, gw = net.loss_grad(x, e, bank.ce[n], bank.cn[n], off, den, b, n, n)
+        g = SymmetricMultilevel.theta_grad(theta, gw)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        f = 0.5 * (1.0 + math.cos(math.pi * step / steps))
+        theta -= lr * f * (m / (1 - b1 ** (step + 1))) / (np.sqrt(v / (1 - b2 ** (step + 1))) + 1e-8)
+        losses.append(loss)
+        if step % log_every == 0 or step == steps - 1:
+            _log(f"step {step}: loss {np.mean(losses[-log_every:]):.4f} ({n}^2 x{b}, "
+                 f"{time.time() - t0:.0f} s) w {np.round(np.exp(theta[0:4]), 3).tolist()}")
+        if eval_fn is not None and eval_every and (step + 1) % eval_every == 0:
+            r = eval_fn(SymmetricMultilevel.weights(theta))
+            evals.append((step + 1, r))
+            _log(f"eval at {step + 1}: {r}")
+    return theta, losses, evals
+
+
 # ---------------------------------------------------------------------------
 # cases and harvesting
 
@@ -294,6 +445,8 @@ def main(argv=None):
     ap.add_argument("--harvest-seconds", type=float, default=300)
     ap.add_argument("--train-seconds", type=float, default=420)
     ap.add_argument("--init", default=None, help="start from these weights instead")
+    ap.add_argument("--free", action="store_true",
+                    help="train every weight (default: the symmetric multilevel parametrisation)")
     ap.add_argument("--noise", type=float, default=1e-3,
                     help="random weights in the unused channels of the initial network")
     ap.add_argument("--eval", type=int, nargs="*", default=[256, 1024])
@@ -306,17 +459,29 @@ def main(argv=None):
     if not args.out:
         ap.error("--out is required for training")
     import torch
-    w0 = (np.fromfile(args.init, dtype="<f4") if args.init
-          else structured_init(args.seed, noise=args.noise))
-    summary = dict(args=vars(args))
-    summary["eval_init"] = [evaluate(w0, n) for n in args.eval]
-    _log("init:", summary["eval_init"])
     cap = {256: (13000, 200), 512: (6500, 100), 1024: (1600, 30)}
     cases = default_cases(args.seed, args.cases)
+    summary = dict(args=vars(args))
+    if args.free:
+        w0 = (np.fromfile(args.init, dtype="<f4") if args.init
+              else structured_init(args.seed, noise=args.noise))
+    else:
+        theta0 = SymmetricMultilevel.initial()
+        w0 = SymmetricMultilevel.weights(theta0)
+    summary["eval_init"] = [evaluate(w0, n) for n in args.eval]
+    _log("init:", summary["eval_init"])
     bank = harvest_cases(torch, cases, cap, args.seed, time.time() + args.harvest_seconds)
     summary["pairs"] = {str(n): int(c) for n, c in bank.count.items()}
-    w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
-                      time.time() + args.train_seconds)
+    if args.free:
+        w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
+                          time.time() + args.train_seconds)
+    else:
+        theta, losses, evals = train_symmetric(
+            torch, bank, theta0, args.steps, args.lr, args.seed, time.time() + args.train_seconds,
+            eval_fn=lambda ww: evaluate(ww, 512), eval_every=max(1, args.steps // 4))
+        summary["theta"] = theta.tolist()
+        summary["evals"] = evals
+        w = SymmetricMultilevel.weights(theta)
     w.astype("<f4").tofile(args.out)
     summary["loss_first"] = float(np.mean(losses[:50])) if losses else None
     summary["loss_last"] = float(np.mean(losses[-50:])) if losses else None

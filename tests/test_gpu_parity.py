@@ -112,7 +112,7 @@ def test_mixed_precision_pressure_solve(shape):
     d.upload("RESPONSE", D)
     d.upload("FLUX_STAR", Fs)
     d.assemble_pressure_correction_resident(0)
-    A, rhs, _ = c.pressure_system(Fs, D, D)
+    A, rhs, _ = c.pressure_system(Fs, D, D, D if dim == 3 else None)
     ref = A.cg(rhs, rel_tol=1e-12, max_iter=50000)
     s = cfd.LinearSolverSettings(0.0, 1e-8, 50000, "CG", "Jacobi", precision="mixed")
     r = d.solve_pressure_rhs(s, rhs)
