@@ -30,7 +30,7 @@ from ._lib import (BREAKDOWN, CONVERGED, FIELDS, FIXED_GRADIENT, FIXED_VALUE, FP
 
 __all__ = ["device_available", "LinearSolverSettings", "SolverResult", "make_gpu_cg",
            "make_gpu_bicgstab", "GpuSimpleDiscretization", "CfdHipError", "UnetNet",
-           "unet_param_count", "load_learned_weights", "WEIGHTS_PATH"]
+           "unet_param_count", "load_learned_weights", "WEIGHTS_PATH", "BatchSolver"]
 
 
 class CfdHipError(RuntimeError):
@@ -446,6 +446,80 @@ FACE_FIELDS = _lib.FACE_FIELDS
 # Learned preconditioner
 
 WEIGHTS_PATH = __import__("os").path.join(_lib.HERE, "weights", "pprime_unet.f32")
+
+
+class BatchSolver:
+    """Many independent 2D p' systems of uneven size solved together
+    (cfdhip_batch_*): Jacobi-PCG, one launch per step for the whole batch,
+    FP32 inner iterations inside FP64 refinement (precision "mixed") or FP64.
+    No reference counterpart; each case meets the same FP64 true-residual
+    test as a single solve (src/algebra/CG.cpp:14-148)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.cfdhip_batch_create(device, ctypes.byref(h))
+        if rc != _lib.OK:
+            raise CfdHipError(rc, "cannot create a batch")
+        self.h = h
+        self.shapes = []
+        self.stats = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cfdhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != _lib.OK:
+            msg = self.lib.cfdhip_batch_last_error(self.h)
+            raise CfdHipError(rc, msg.decode() if msg else "")
+
+    def add_system(self, nx, ny, pin, diag, face_coeff, rhs) -> int:
+        """Host arrays: diagonal (nx*ny), face couplings (x-faces then y-faces),
+        rhs; pin = pinned cell or -1."""
+        diag, face_coeff, rhs = _f64(diag), _f64(face_coeff), _f64(rhs)
+        if diag.size != nx * ny or rhs.size != nx * ny or \
+                face_coeff.size != (nx + 1) * ny + nx * (ny + 1):
+            raise ValueError("batch system: array sizes do not match nx, ny")
+        idx = ctypes.c_int64(-1)
+        self._call("cfdhip_batch_add_system", nx, ny, pin, _pf(diag), _pf(face_coeff), _pf(rhs),
+                   ctypes.byref(idx))
+        self.shapes.append((nx, ny))
+        return idx.value
+
+    def add_pressure(self, d: "GpuSimpleDiscretization") -> int:
+        """The p' system the discretization has assembled (copied on the device)."""
+        idx = ctypes.c_int64(-1)
+        self._call("cfdhip_batch_add_pressure", d._ctx.h, ctypes.byref(idx))
+        self.shapes.append(tuple(d.shape[:2]))
+        return idx.value
+
+    def solve(self, settings: LinearSolverSettings, tile_rows=0, use_graph=True):
+        """Returns a list of SolverResult (solution None; device_ms = the case's
+        completion time from the start of the batch); self.stats holds the
+        batch figures."""
+        n = len(self.shapes)
+        res = (_lib.SolveResult * max(n, 1))()
+        lat = np.zeros(max(n, 1))
+        st = np.zeros(8)
+        s = settings.c()
+        self._call("cfdhip_batch_solve", ctypes.byref(s), int(tile_rows), int(bool(use_graph)),
+                   res, _pf(lat), _pf(st))
+        self.stats = {"device_ms": st[0], "steps": int(st[1]), "cell_iterations": st[2],
+                      "tile_rows": int(st[3]), "graph_captures": int(st[4]),
+                      "tune_us": {16: st[5], 32: st[6], 64: st[7]}}
+        return [_result(res[k], None, None) for k in range(n)]
+
+    def solution(self, index) -> np.ndarray:
+        nx, ny = self.shapes[index]
+        x = np.empty(nx * ny)
+        self._call("cfdhip_batch_solution", int(index), _pf(x), x.size)
+        return x
 
 
 def unet_param_count() -> int:

@@ -137,6 +137,17 @@ SIGNATURES = {
     "cfdhip_unet_train_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                               ctypes.c_int, P_F64]),
+    "cfdhip_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
+    "cfdhip_batch_destroy": (None, [VP]),
+    "cfdhip_batch_last_error": (ctypes.c_char_p, [VP]),
+    "cfdhip_batch_size": (ctypes.c_int64, [VP]),
+    "cfdhip_batch_add_system": (ctypes.c_int, [VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                               P_F64, P_F64, P_F64, P_I64]),
+    "cfdhip_batch_add_pressure": (ctypes.c_int, [VP, VP, P_I64]),
+    "cfdhip_batch_solve": (ctypes.c_int, [VP, ctypes.POINTER(SolverSettings), ctypes.c_int,
+                                          ctypes.c_int, ctypes.POINTER(SolveResult), P_F64,
+                                          P_F64]),
+    "cfdhip_batch_solution": (ctypes.c_int, [VP, ctypes.c_int64, P_F64, ctypes.c_int64]),
 }
 
 _lib = None

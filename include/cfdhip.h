@@ -251,11 +251,13 @@ int cfdhip_preconditioner_timing(cfdhip_ctx* ctx, double* ms, int64_t* calls);
 /* No counterpart in the reference: the p' solve with preconditioner
  * CFDHIP_PREC_LEARNED is flexible CG (type CFDHIP_SOLVER_FCG) preconditioned by
  * a linear 4-level convolutional U-Net (3x3 convolutions, 16-64 channels,
- * BF16 weights and activations, FP32 accumulation) applied to the
- * Jacobi-scaled residual, z = D^-1/2 N(D^-1/2 r). Inner iterations run in
- * FP32, FP64 refinement reaches the requested FP64 true residual, and an
- * inner solve that stops improving for stagnation_window iterations hands the
- * solve to FP64 Jacobi-PCG (fallback_used = 1). 2D p' operator only. */
+ * BF16 weights, FP32 activations split into two BF16 halves at the matrix
+ * cores, FP32 accumulation) applied to the Jacobi-scaled residual,
+ * z = D^-1/2 N(D^-1/2 r). The Krylov vectors are FP64 (precision CFDHIP_FP64)
+ * or FP32 inside FP64 refinement (CFDHIP_MIXED); the solve ends at the
+ * requested FP64 true residual, and an inner solve that stops improving for
+ * stagnation_window iterations hands the solve to FP64 Jacobi-PCG
+ * (fallback_used = 1). 2D p' operator only. */
 
 /* Number of FP32 master weights: layers e0a e0b e1a e1b e2a e2b b3a b3b d2a
  * d2b d1a d1b d0a d0b out, each W[ky][kx][cin][cout]. */
@@ -301,17 +303,53 @@ int64_t cfdhip_unet_tensor_size(cfdhip_unet* net, int tensor);
 int cfdhip_unet_tensor(cfdhip_unet* net, int tensor, float* out, int64_t count);
 /* A-norm loss L = mean_s (N(x_s) - e_s, A_s (N(x_s) - e_s)) / den_s with A_s the
  * unit-diagonal operator of couplings ce, cn + sys_offset[s] (device int64 per
- * sample); with update != 0 one Adam step (lr, betas 0.9 / 0.999). */
-/* The same loss and its gradient in the master weights (grad: host,
- * param_count floats), no update: for training a parametrisation of the
- * weights on the host. */
+ * sample) and its gradient in the master weights (grad: host, param_count
+ * floats), no update: for training a parametrisation of the weights on the host. */
 int cfdhip_unet_loss_grad(cfdhip_unet* net, const float* x, const float* e, const float* ce,
                           const float* cn, const int64_t* sys_offset, const float* den,
                           int64_t batch, int64_t h, int64_t w, double* loss, float* grad);
+/* The same loss; with update != 0 one Adam step (lr, betas 0.9 / 0.999). */
 int cfdhip_unet_train_step(cfdhip_unet* net, const float* x, const float* e, const float* ce,
                            const float* cn, const int64_t* sys_offset, const float* den,
                            int64_t batch, int64_t h, int64_t w, float lr, int update,
                            double* loss);
+
+/* ---- batched p' solves ------------------------------------------------------------ */
+/* No counterpart in the reference, which solves one case at a time
+ * (src/algebra/CG.cpp:14-148 per SIMPLE iteration): many independent 2D p'
+ * systems of uneven size, solved together by Jacobi-PCG with one launch per
+ * step for the whole batch. Each case reaches |b - A x| <= max(abs, rel |b|)
+ * in FP64 from x0 = 0; precision CFDHIP_MIXED runs the inner iterations in
+ * FP32 inside FP64 refinement, CFDHIP_FP64 runs them in FP64. Cases that are
+ * done leave the launch grid at the next host check (every check_interval
+ * steps). The batch owns its stream and its device memory. */
+typedef struct cfdhip_batch cfdhip_batch;
+int cfdhip_batch_create(int device, cfdhip_batch** out);
+void cfdhip_batch_destroy(cfdhip_batch* batch);
+const char* cfdhip_batch_last_error(const cfdhip_batch* batch);
+int64_t cfdhip_batch_size(const cfdhip_batch* batch);
+/* Adds the 2D system of an nx x ny case from host arrays: diagonal (n), face
+ * couplings D_f in face order (x-faces j-outer, then y-faces; (nx+1) ny +
+ * nx (ny+1) values; boundary faces carry no coupling) and right-hand side
+ * (n); pin = the cell whose row is the identity-scaled diagonal, or -1. */
+int cfdhip_batch_add_system(cfdhip_batch* batch, int64_t nx, int64_t ny, int64_t pin,
+                            const double* diag, const double* face_coeff, const double* rhs,
+                            int64_t* index);
+/* Adds the p' system a context has assembled (copied on the device). */
+int cfdhip_batch_add_pressure(cfdhip_batch* batch, cfdhip_ctx* ctx, int64_t* index);
+/* Solves every case (type CFDHIP_SOLVER_CG, preconditioner
+ * CFDHIP_PREC_JACOBI). tile_rows: 16, 32 or 64 rows per 64-column tile, or 0
+ * to time the three on the batch and take the fastest; use_graph != 0 replays
+ * each chunk of steps from a captured HIP graph. results: one per case
+ * (device_ms = the case's completion time from the start of the batch);
+ * latency_ms: the same times, or NULL; stats (NULL or 8 doubles): device ms of
+ * the batch, steps, sum of cells x iterations, tile rows used, graph captures,
+ * tuning times of the 16/32/64-row steps (us). */
+int cfdhip_batch_solve(cfdhip_batch* batch, const cfdhip_solver_settings* s, int tile_rows,
+                       int use_graph, cfdhip_solve_result* results, double* latency_ms,
+                       double* stats);
+/* Copies the solution of case `index` of the last solve to the host. */
+int cfdhip_batch_solution(cfdhip_batch* batch, int64_t index, double* host, int64_t count);
 
 #ifdef __cplusplus
 }
