@@ -13,8 +13,12 @@ back to back on the same resident operator and RHS:
   * plain: Jacobi-PCG, FP64 single-kernel iteration.
 `value` is the learned solve's time. Also reported: the FP32-inner mixed
 Jacobi-PCG, SIMPLE outer iterations per second at 2048x2048, Re = 1000, the
-per-layer kernel times of the preconditioner and the plain iteration kernel's
-HBM roofline.
+per-layer kernel times of the preconditioner, the plain iteration kernel's
+HBM roofline, and the batched leg: the p' systems of the first SIMPLE
+iteration of 256 cavities and channels of uneven size (seed 5005) solved
+together by the batched mixed-precision Jacobi-PCG (aggregate cells x
+iterations per second, per-case p50 / p99 completion time) and one after
+another by the single-case solve.
 
 Prints one JSON line (see the repository's bench contract).
 """
@@ -121,6 +125,88 @@ def cpu_baseline(iters_gpu):
                       f"{iters_gpu} iterations"}
 
 
+def batch_cases(seed=5005, count=256):
+    """256 independent 2D cases: nx, ny uniform in 64..512 with aspect ratio at
+    most 4 (ny redrawn otherwise), cavity or channel, Re log-uniform in
+    100-3200, unit lid / inlet speed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx = int(rng.integers(64, 513))
+        while True:
+            ny = int(rng.integers(64, 513))
+            if max(nx, ny) <= 4 * min(nx, ny):
+                break
+        kind = "cavity" if rng.random() < 0.5 else "channel"
+        re = float(np.exp(rng.uniform(np.log(100.0), np.log(3200.0))))
+        out.append((kind, nx, ny, re))
+    return out
+
+
+def batch_leg(cfd, log, cases):
+    """p' systems of the first SIMPLE iteration of each case, solved (a) by the
+    batched mixed-precision Jacobi-PCG in one launch per step and (b) one case
+    after another by the single-case mixed-precision solve, both to 1e-8 FP64
+    true relative residual."""
+    from cfdapp_amd import learned
+    t0 = time.perf_counter()
+    batch = cfd.BatchSolver()
+    seq_s, seq_lat, seq_iters = 0.0, [], 0
+    mx = cfd.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi", precision="mixed",
+                                  check_interval=64)
+    cells = 0
+    for kind, nx, ny, re in cases:
+        d = learned.make_case(kind, nx, ny=ny, re=re, speed=1.0)
+        try:
+            learned.assemble_pprime(d, 0)
+            batch.add_pressure(d)
+            cells += nx * ny
+            d._ctx.call("cfdhip_synchronize")
+            t1 = time.perf_counter()
+            r = d.solve_pressure_rhs(mx, None, download=False)
+            dt = time.perf_counter() - t1
+            if r.status != 0:
+                raise RuntimeError(f"sequential {kind} {nx}x{ny}: {r.status_name}")
+            seq_s += dt
+            seq_lat.append(dt * 1e3)
+            seq_iters += r.iterations
+        finally:
+            d.close()
+    log(f"batch: {len(cases)} cases, {cells} cells, set up in {time.perf_counter() - t0:.1f} s")
+    bs = cfd.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi", precision="mixed",
+                                  check_interval=32)
+    batch.solve(bs)  # warm-up (also tunes the tile height)
+    t1 = time.perf_counter()
+    res = batch.solve(bs)
+    wall = time.perf_counter() - t1
+    st = batch.stats
+    bad = [k for k, r in enumerate(res) if r.status != 0]
+    if bad:
+        raise RuntimeError(f"batched solve: {len(bad)} cases did not converge")
+    lat = np.array([r.device_ms for r in res])
+    out = {"cases": len(cases), "cells": cells, "batched_s": wall,
+           "batched_device_ms": st["device_ms"],
+           "batched_cell_iter_per_s": st["cell_iterations"] / wall,
+           "batched_p50_ms": float(np.percentile(lat, 50)),
+           "batched_p99_ms": float(np.percentile(lat, 99)),
+           "batched_tile_rows": st["tile_rows"], "batched_steps": st["steps"],
+           "batched_graph_captures": st["graph_captures"],
+           "tile_tuning_us": st["tune_us"],
+           "sequential_s": seq_s,
+           "sequential_cell_iter_per_s": sum(r.iterations * nx * ny for r, (_, nx, ny, _) in
+                                             zip(res, cases)) / seq_s,
+           "sequential_p50_ms": float(np.percentile(seq_lat, 50)),
+           "sequential_p99_ms": float(np.percentile(seq_lat, 99)),
+           "speedup_batched_vs_sequential": seq_s / wall}
+    log(f"batch: batched {wall * 1e3:.1f} ms ({out['batched_cell_iter_per_s']:.3e} cell-it/s, "
+        f"p50 {out['batched_p50_ms']:.1f} ms, p99 {out['batched_p99_ms']:.1f} ms, "
+        f"{st['tile_rows']}-row tiles, {st['graph_captures']} graph captures); sequential "
+        f"{seq_s * 1e3:.1f} ms (p50 {out['sequential_p50_ms']:.1f} ms, "
+        f"p99 {out['sequential_p99_ms']:.1f} ms)")
+    batch.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +214,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--simple-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-cases", type=int, default=256,
+                    help="cases of the batched-solve leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy 20 times (counter calibration)")
     ap.add_argument("--jacobi-only", action="store_true",
@@ -259,6 +347,10 @@ def main():
             f"p' iterations {res['pressure_iterations']}")
         ds.close()
 
+    batched = {}
+    if args.batch_cases > 0:
+        batched = batch_leg(cfd, log, batch_cases(count=args.batch_cases))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rj.iterations)
@@ -291,7 +383,8 @@ def main():
                    "jacobi_pcg_s": jac_s, "jacobi_pcg_iterations": rj.iterations,
                    "jacobi_pcg_true_relative_residual": rj.true_residual / bnorm,
                    "speedup_learned_vs_jacobi": jac_s / fcg_s,
-                   **mixed, **simple},
+                   **mixed, **simple,
+                   "batched_pprime": batched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES,
                      "kernel": "cgIterKernel<double> (one Jacobi-PCG iteration: updates + "

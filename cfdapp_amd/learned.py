@@ -22,8 +22,8 @@ import time
 
 import numpy as np
 
-from . import (FIXED_GRADIENT, FIXED_VALUE, INLET, OUTLET, WALL, GpuSimpleDiscretization,
-               LinearSolverSettings, UnetNet, unet_param_count)
+from . import (FIXED_GRADIENT, FIXED_VALUE, INLET, MOVING_WALL, OUTLET, WALL,
+               GpuSimpleDiscretization, LinearSolverSettings, UnetNet, unet_param_count)
 
 LAYERS = [("e0a", 1, 16), ("e0b", 16, 16), ("e1a", 16, 32), ("e1b", 32, 32),
           ("e2a", 32, 64), ("e2b", 64, 64), ("b3a", 64, 64), ("b3b", 64, 64),
@@ -243,17 +243,28 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
 # ---------------------------------------------------------------------------
 # cases and harvesting
 
-def make_case(kind, n, re, speed):
-    """A square 2D cavity (lid speed) or channel (inlet speed) with Re = speed L / nu."""
-    nu = speed * 1.0 / re
-    if kind == "cavity":
-        return GpuSimpleDiscretization.cavity(2, n, n, lid=speed, viscosity=nu)
+def make_case(kind, n, re, speed, ny=None):
+    """A 2D cavity (lid speed) or channel (inlet speed, left to right) of
+    n x ny uniform cells with Re = speed L / nu, L the longer side. A square
+    case is the unit square; a rectangle uses the cell size 2^-k that puts
+    its longer side in (1/2, 1], so that every coordinate is exact (the
+    structured path accepts only boundary faces exactly aligned with their
+    owner cells)."""
+    square = ny is None or ny == n
+    ny = n if ny is None else ny
+    h = 1.0 / n if square else 2.0 ** -math.ceil(math.log2(max(n, ny)))
+    nu = speed * (max(n, ny) * h) / re
     d = GpuSimpleDiscretization()
-    vel = [(INLET, (speed, 0.0, 0.0)), (OUTLET, (0.0, 0.0, 0.0)),
-           (WALL, (0.0, 0.0, 0.0)), (WALL, (0.0, 0.0, 0.0))]
-    pres = [(FIXED_GRADIENT, 0.0), (FIXED_VALUE, 0.0), (FIXED_GRADIENT, 0.0),
-            (FIXED_GRADIENT, 0.0)]
-    ok, why = d.prepare(2, n, n, velocity=vel, pressure=pres, viscosity=nu)
+    if kind == "cavity":
+        vel = [(WALL, (0.0, 0.0, 0.0))] * 3 + [(MOVING_WALL, (speed, 0.0, 0.0))]
+        pres = [(FIXED_GRADIENT, 0.0)] * 4
+    else:
+        vel = [(INLET, (speed, 0.0, 0.0)), (OUTLET, (0.0, 0.0, 0.0)),
+               (WALL, (0.0, 0.0, 0.0)), (WALL, (0.0, 0.0, 0.0))]
+        pres = [(FIXED_GRADIENT, 0.0), (FIXED_VALUE, 0.0), (FIXED_GRADIENT, 0.0),
+                (FIXED_GRADIENT, 0.0)]
+    ok, why = d.prepare(2, n, ny, lx=n * h, ly=ny * h, velocity=vel, pressure=pres,
+                        viscosity=nu)
     if not ok:
         raise RuntimeError(why)
     return d
