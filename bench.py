@@ -18,7 +18,9 @@ HBM roofline, and the batched leg: the p' systems of the first SIMPLE
 iteration of 256 cavities and channels of uneven size (seed 5005) solved
 together by the batched mixed-precision Jacobi-PCG (aggregate cells x
 iterations per second, per-case p50 / p99 completion time) and one after
-another by the single-case solve.
+another by the single-case solve; and the 3D leg: SIMPLE iterations on the
+256^3 cavity at Re = 400 with FP64 and with mixed-precision p' solves, and the
+mixed run's pressure / velocity error against the FP64 run.
 
 Prints one JSON line (see the repository's bench contract).
 """
@@ -207,6 +209,40 @@ def batch_leg(cfd, log, cases):
     return out
 
 
+def mixed3d_leg(cfd, log, n, iters):
+    """3D lid-driven cavity n^3 at Re = 400 (mu = 0.0025, lid ymax (1, 0, 0),
+    Rhie-Chow): `iters` SIMPLE iterations with FP64 Jacobi-PCG p' solves, then
+    the same with FP32 inner PCG inside FP64 refinement; the mixed run's
+    pressure and velocity against the FP64 run's (max-norm relative errors)."""
+    out, fields = {"cells": n ** 3, "simple_iterations": iters}, {}
+    for prec in ("fp64", "mixed"):
+        d = cfd.GpuSimpleDiscretization.cavity(3, n, n, n, viscosity=0.0025)
+        try:
+            ps = cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi", precision=prec,
+                                          check_interval=64)
+            d._ctx.call("cfdhip_synchronize")
+            t0 = time.perf_counter()
+            res = d.simple_run(iters, pressure=ps)
+            dt = time.perf_counter() - t0
+            if res["iterations"] != iters:
+                raise RuntimeError(f"3D SIMPLE ({prec}) stopped: status {res['status']}")
+            fields[prec] = (d.download("P"), d.download_velocity())
+            out[f"{prec}_simple_iter_per_s"] = iters / dt
+            out[f"{prec}_pressure_iterations"] = res["pressure_iterations"]
+        finally:
+            d.close()
+    p64, u64 = fields["fp64"]
+    pm, um = fields["mixed"]
+    out["p_rel_err_inf"] = float(np.max(np.abs(pm - p64)) / np.max(np.abs(p64)))
+    umax = max(float(np.max(np.abs(c))) for c in u64)
+    out["u_rel_err_inf"] = max(float(np.max(np.abs(a - b))) for a, b in zip(um, u64)) / umax
+    log(f"3D {n}^3 Re 400, {iters} SIMPLE iterations: FP64 {out['fp64_simple_iter_per_s']:.3f} it/s "
+        f"({out['fp64_pressure_iterations']} p' it), mixed {out['mixed_simple_iter_per_s']:.3f} "
+        f"it/s ({out['mixed_pressure_iterations']} p' it); mixed vs FP64: |dp|/|p| "
+        f"{out['p_rel_err_inf']:.2e}, |du|/|u| {out['u_rel_err_inf']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,12 +250,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--simple-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cube", type=int, default=256, help="3D mixed-precision leg: cells per side")
+    ap.add_argument("--cube-iters", type=int, default=2,
+                    help="SIMPLE iterations of the 3D leg (0 skips it)")
     ap.add_argument("--batch-cases", type=int, default=256,
                     help="cases of the batched-solve leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy 20 times (counter calibration)")
     ap.add_argument("--jacobi-only", action="store_true",
                     help="only the Jacobi-PCG solves (for kernel counters)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched-solve leg (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -242,6 +283,10 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.batch_only:
+        print(json.dumps(batch_leg(cfd, log, batch_cases(count=args.batch_cases))), flush=True)
+        return
 
     # --- resident operator + RHS ------------------------------------------------
     d = cfd.GpuSimpleDiscretization.cavity(2, N, N)
@@ -347,6 +392,10 @@ def main():
             f"p' iterations {res['pressure_iterations']}")
         ds.close()
 
+    cube = {}
+    if args.cube_iters > 0:
+        cube = mixed3d_leg(cfd, log, args.cube, args.cube_iters)
+
     batched = {}
     if args.batch_cases > 0:
         batched = batch_leg(cfd, log, batch_cases(count=args.batch_cases))
@@ -384,6 +433,7 @@ def main():
                    "jacobi_pcg_true_relative_residual": rj.true_residual / bnorm,
                    "speedup_learned_vs_jacobi": jac_s / fcg_s,
                    **mixed, **simple,
+                   "mixed_precision_3d": cube,
                    "batched_pprime": batched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES,

@@ -415,6 +415,17 @@ class GpuSimpleDiscretization:
                        den.ctypes.data_as(_lib.P_F32), ctypes.byref(pairs), ctypes.byref(r))
         return pairs.value, den[:pairs.value].copy(), _result(r, None, self._ctx.history())
 
+    def harvest_noise(self, seed, count, h, w, r_out, e_out):
+        """White-noise pairs on the assembled p' operator (see
+        cfdhip_pressure_harvest_noise). Returns (pairs, den)."""
+        den = np.zeros(max(1, int(count)), dtype=np.float32)
+        pairs = ctypes.c_int64(0)
+        _sync_producer()
+        self._ctx.call("cfdhip_pressure_harvest_noise", int(seed), int(count), int(h), int(w),
+                       _dev(r_out), _dev(e_out), den.ctypes.data_as(_lib.P_F32),
+                       ctypes.byref(pairs))
+        return pairs.value, den[:pairs.value].copy()
+
     def simple_run(self, iterations, alpha_u=0.7, alpha_p=0.3, velocity_tol=1e-6,
                    pressure_tol=1e-6, continuity_tol=1e-6, rhie_chow=True, reference_cell=0,
                    momentum: LinearSolverSettings | None = None,

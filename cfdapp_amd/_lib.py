@@ -119,6 +119,9 @@ SIGNATURES = {
                                                ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                ctypes.c_int64, VP, VP, VP, VP, P_F32, P_I64,
                                                ctypes.POINTER(SolveResult)]),
+    "cfdhip_pressure_harvest_noise": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_int64,
+                                                     ctypes.c_int64, ctypes.c_int64, VP, VP,
+                                                     P_F32, P_I64]),
     "cfdhip_unet_create": (ctypes.c_int, [ctypes.POINTER(VP)]),
     "cfdhip_unet_destroy": (None, [VP]),
     "cfdhip_unet_last_error": (ctypes.c_char_p, [VP]),

@@ -30,6 +30,8 @@ int64_t harvestPairs(Context& ctx, const cfdhip_solver_settings& set, const doub
                      int64_t stride, int64_t maxPairs, int64_t h, int64_t w, float* rOut,
                      float* eOut, float* ceOut, float* cnOut, float* denOut,
                      cfdhip_solve_result* result);
+int64_t harvestNoisePairs(Context& ctx, uint64_t seed, int64_t count, int64_t h, int64_t w,
+                          float* rOut, float* eOut, float* denOut);
 }
 
 namespace {
@@ -103,6 +105,18 @@ int cfdhip_pressure_harvest(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, co
                                   hipMemcpyHostToDevice, c.stream));
     const int64_t k = harvestPairs(c, *s, c.f(CFDHIP_FIELD_PC_RHS), stride, max_pairs, h, w, r_out,
                                    e_out, ce_out, cn_out, den_out, result);
+    if (pairs) *pairs = k;
+  });
+}
+
+int cfdhip_pressure_harvest_noise(cfdhip_ctx* ctx, uint64_t seed, int64_t count, int64_t h,
+                                  int64_t w, float* r_out, float* e_out, float* den_out,
+                                  int64_t* pairs) {
+  return guardedCtx(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(c.pressureAssembled, "no p' system assembled");
+    require(count == 0 || (r_out && e_out && den_out), "output buffers are required");
+    const int64_t k = harvestNoisePairs(c, seed, count, h, w, r_out, e_out, den_out);
     if (pairs) *pairs = k;
   });
 }

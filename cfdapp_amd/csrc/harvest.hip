@@ -5,7 +5,9 @@
 // written in the symmetrically scaled system (unit diagonal):
 //   e^ = D^1/2 e,  r^ = D^-1/2 r = D^-1/2 A D^-1/2 e^,
 // normalised to unit RMS of r^, zero-padded to the network grid h x w, with
-// den = (e^, r^) (the squared A-norm of the error, for the loss).
+// den = (e^, r^) (the squared A-norm of the error, for the loss). White-noise
+// errors (harvestNoisePairs) add the rough modes a converging solve no longer
+// carries.
 #include <cmath>
 #include <vector>
 
@@ -100,7 +102,78 @@ __global__ void couplingKernel(int64_t nx, int64_t ny, int64_t pin, int64_t offY
   }
 }
 
+// White-noise error vectors: e_i uniform in [-1, 1] from splitmix64 of
+// (seed, pair, cell), zero outside the mesh (x* - x_k with x_k = 0 and x* = e
+// in pairKernel).
+__global__ void noiseKernel(int64_t n, uint64_t seed, double* __restrict__ e) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(c + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    e[c] = 2.0 * (static_cast<double>(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+__global__ void zeroKernelH(int64_t n, double* __restrict__ a) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[c] = 0.0;
+}
+
+// Writes pair k from the error x* - x_k (device vectors of n): scaled pair
+// into rOut / eOut, den into denOut[k]; false when the error is zero.
+bool emitPair(Context& ctx, const double* xs, const double* xk, int64_t h, int64_t w, float* r,
+              float* e, float* den, DeviceArray<double>& part, DeviceArray<double>& sums) {
+  const View& v = ctx.view;
+  const dim3 g(kParts);
+  hipLaunchKernelGGL(pairKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin, v.offY,
+                     ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), xs, xk, h, w,
+                     r, e, part.get());
+  hipLaunchKernelGGL(sumPairKernel, dim3(1), dim3(kFinThreads), 0, ctx.stream, part.get(), kParts,
+                     sums.get());
+  double hs[2];
+  CFDHIP_CHECK(hipMemcpyAsync(hs, sums.get(), sizeof(hs), hipMemcpyDeviceToHost, ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  const double ms = hs[0] / static_cast<double>(v.n);
+  if (!(ms > 0.0) || !std::isfinite(ms) || !(hs[1] > 0.0)) return false;  // x_k == x*
+  const double sc = 1.0 / std::sqrt(ms);
+  hipLaunchKernelGGL(scalePairKernel, g, dim3(256), 0, ctx.stream, h * w, static_cast<float>(sc),
+                     r, e);
+  *den = static_cast<float>(hs[1] * sc * sc);
+  return true;
+}
+
 }  // namespace
+
+int64_t harvestNoisePairs(Context& ctx, uint64_t seed, int64_t count, int64_t h, int64_t w,
+                          float* rOut, float* eOut, float* denOut) {
+  const View& v = ctx.view;
+  require(v.dim == 2, "harvesting needs a 2D mesh");
+  require(h >= v.ny && w >= v.nx, "the network grid must cover the mesh");
+  require(count >= 0, "count must be >= 0");
+  const int64_t n = v.n;
+  DeviceArray<double> e, zero, part, sums;
+  e.ensure(n);
+  zero.ensure(n);
+  part.ensure(kParts * 2 + 8);
+  sums.ensure(2);
+  const dim3 g(kParts);
+  hipLaunchKernelGGL(zeroKernelH, g, dim3(256), 0, ctx.stream, n, zero.get());
+  int64_t kept = 0;
+  for (int64_t k = 0; k < count; ++k) {
+    hipLaunchKernelGGL(noiseKernel, g, dim3(256), 0, ctx.stream, n,
+                       seed * 0x2545F4914F6CDD1Dull + static_cast<uint64_t>(k) * 0xD1B54A32D192ED03ull,
+                       e.get());
+    if (emitPair(ctx, e.get(), zero.get(), h, w, rOut + kept * h * w, eOut + kept * h * w,
+                 denOut + kept, part, sums))
+      ++kept;
+  }
+  CFDHIP_CHECK(hipGetLastError());
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  return kept;
+}
 
 int64_t harvestPairs(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                      int64_t stride, int64_t maxPairs, int64_t h, int64_t w, float* rOut,
@@ -140,23 +213,9 @@ int64_t harvestPairs(Context& ctx, const cfdhip_solver_settings& set, const doub
   int64_t kept = 0;
   const dim3 g(kParts);
   for (int64_t k = 0; k < usable; ++k) {
-    float* r = rOut + kept * h * w;
-    float* e = eOut + kept * h * w;
-    hipLaunchKernelGGL(pairKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin, v.offY,
-                       ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), x,
-                       snaps.get() + k * n, h, w, r, e, part.get());
-    hipLaunchKernelGGL(sumPairKernel, dim3(1), dim3(kFinThreads), 0, ctx.stream, part.get(), kParts,
-                       sums.get());
-    double hs[2];
-    CFDHIP_CHECK(hipMemcpyAsync(hs, sums.get(), sizeof(hs), hipMemcpyDeviceToHost, ctx.stream));
-    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
-    const double ms = hs[0] / static_cast<double>(n);
-    if (!(ms > 0.0) || !std::isfinite(ms) || !(hs[1] > 0.0)) continue;  // x_k == x*
-    const double sc = 1.0 / std::sqrt(ms);
-    hipLaunchKernelGGL(scalePairKernel, g, dim3(256), 0, ctx.stream, h * w, static_cast<float>(sc),
-                       r, e);
-    denOut[kept] = static_cast<float>(hs[1] * sc * sc);
-    ++kept;
+    if (emitPair(ctx, x, snaps.get() + k * n, h, w, rOut + kept * h * w, eOut + kept * h * w,
+                 denOut + kept, part, sums))
+      ++kept;
   }
   if (ceOut != nullptr && cnOut != nullptr)
     hipLaunchKernelGGL(couplingKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin, v.offY,

@@ -117,6 +117,8 @@ cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, 
     return t > 0.0 && t < 1.0 ? t : 1e-5;
   }();
   double spmvMs = 0.0, spmvLaunches = 0.0;
+  bool fallback = false;
+  double prev = 0.0;
   while (true) {
     hipLaunchKernelGGL(refineResidualKernel, g, dim3(256), 0, ctx.stream, op,
                        ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
@@ -138,6 +140,34 @@ cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, 
       break;
     }
     if (total >= set.max_iterations) break;
+    if (cycles > 0 && rTrue > 0.5 * prev) {
+      // a cycle that did not halve the true residual: finish in FP64 from x
+      cfdhip_solver_settings rest = set;
+      rest.relative_tolerance = set.relative_tolerance * b0 / rTrue;
+      rest.max_iterations = set.max_iterations - total;
+      double* work64[8];
+      for (int q = 0; q < 8; ++q) {
+        w.d8[q].ensure(n);
+        work64[q] = w.d8[q].get();
+      }
+      const cfdhip_solve_result fr = singleKernelPcg2d<double>(
+          ctx, rest, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
+          work64);
+      total += fr.iterations;
+      fallback = true;
+      hipLaunchKernelGGL(refineResidualKernel, g, dim3(256), 0, ctx.stream, op,
+                         ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
+                         w.r32.get(), w.partial.get());
+      launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                    nullptr, normOut, kStepNorm});
+      CFDHIP_CHECK(hipMemcpyAsync(&rTrue, normOut, sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+      history.push_back(rTrue);
+      status = fr.status;
+      break;
+    }
+    prev = rTrue;
     cfdhip_solver_settings inner = set;
     inner.absolute_tolerance = 0.0;
     // the inner solve aims at the outer target when that is within FP32 reach
@@ -176,6 +206,7 @@ cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, 
   }
   cfdhip_solve_result res{};
   res.status = status;
+  res.fallback_used = fallback ? 1 : 0;
   res.iterations = total;
   res.restarts = cycles;
   res.initial_residual = b0;

@@ -170,6 +170,23 @@ class SymmetricMultilevel:
               np.array([1, 0, 1, 0, 0, 0, 1, 0, 1.]))
 
     @staticmethod
+    def repair(theta, margin=0.01):
+        """Keeps every level kernel S_l positive semi-definite: the symbol
+        c + 2e (cos a + cos b) + 4k cos a cos b of [[k, e, k], [e, c, e], [k, e, k]]
+        is bilinear in (cos a, cos b), so it is >= 0 on [-1, 1]^2 iff it is at
+        the four corners: c >= 4|k| and c + 4k >= 4|e| (kept with a relative
+        margin). With every S_l >= 0 and S_0 > 0 the network is SPD."""
+        th = np.array(theta, np.float64)
+        for lvl in range(4):
+            c, e, k = th[13 + 3 * lvl:16 + 3 * lvl]
+            c = max(c, 1e-6)
+            k = float(np.clip(k, -(1 - margin) * c / 4, (1 - margin) * c / 4))
+            lim = (1 - margin) * (c + 4 * k) / 4
+            e = float(np.clip(e, -lim, lim))
+            th[13 + 3 * lvl:16 + 3 * lvl] = (c, e, k)
+        return th
+
+    @staticmethod
     def theta_grad(theta, gw):
         """J^T gw: dL/dtheta from dL/dW (the construction is linear in the
         kernels and in the level weights, so J is exact)."""
@@ -198,7 +215,10 @@ class SymmetricMultilevel:
 
 def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_every=25,
                     eval_fn=None, eval_every=0):
-    """Adam on theta with the device's loss gradients; returns (theta, losses)."""
+    """Adam on theta with the device's loss gradients. With eval_fn (theta's
+    weights -> score, lower is better) the model is scored before training and
+    every eval_every steps and the best-scoring theta is returned (model
+    selection on held-out systems); returns (theta, losses, evals)."""
     net = UnetNet(SymmetricMultilevel.weights(theta))
     rng = np.random.default_rng(seed)
     sizes = [n for n in bank.count if bank.count[n] > 0]
@@ -210,6 +230,12 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
     losses, evals = [], []
     t0 = time.time()
     theta = np.array(theta, np.float64)
+    best = (None, theta.copy(), 0)
+    if eval_fn is not None:
+        sc = eval_fn(SymmetricMultilevel.weights(theta))
+        evals.append((0, sc))
+        best = (sc[0], theta.copy(), 0)
+        _log(f"eval at 0: {sc}")
     for step in range(steps):
         if deadline is not None and time.time() > deadline:
             _log(f"train: time budget reached at step {step}")
@@ -229,15 +255,57 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
         v = b2 * v + (1 - b2) * g * g
         f = 0.5 * (1.0 + math.cos(math.pi * step / steps))
         theta -= lr * f * (m / (1 - b1 ** (step + 1))) / (np.sqrt(v / (1 - b2 ** (step + 1))) + 1e-8)
+        theta = SymmetricMultilevel.repair(theta)
         losses.append(loss)
         if step % log_every == 0 or step == steps - 1:
             _log(f"step {step}: loss {np.mean(losses[-log_every:]):.4f} ({n}^2 x{b}, "
                  f"{time.time() - t0:.0f} s) w {np.round(np.exp(theta[0:4]), 3).tolist()}")
-        if eval_fn is not None and eval_every and (step + 1) % eval_every == 0:
-            r = eval_fn(SymmetricMultilevel.weights(theta))
-            evals.append((step + 1, r))
-            _log(f"eval at {step + 1}: {r}")
+        if eval_fn is not None and eval_every and ((step + 1) % eval_every == 0 or
+                                                   step == steps - 1):
+            sc = eval_fn(SymmetricMultilevel.weights(theta))
+            evals.append((step + 1, sc))
+            _log(f"eval at {step + 1}: {sc}")
+            if sc[0] < best[0]:
+                best = (sc[0], theta.copy(), step + 1)
+    if eval_fn is not None:
+        _log(f"selected the model of step {best[2]} (score {best[0]:.4f})")
+        return best[1], losses, evals
     return theta, losses, evals
+
+
+class Validation:
+    """Held-out p' systems for model selection (none is among the training
+    cases, which use seed-drawn Re and speeds at 256^2-1024^2): a 256^2 cavity at
+    Re 3200 after two SIMPLE iterations, a 512^2 channel at Re 1000 after one,
+    and the d = 1 1024^2 cavity operator with a seeded mean-free RHS. The
+    score is the mean log of the learned FCG iteration counts to 1e-8 (a
+    solve that falls back or fails counts as 20000 iterations)."""
+
+    def __init__(self):
+        self.items = []
+        d = make_case("cavity", 256, 3200.0, 1.0)
+        assemble_pprime(d, 2)
+        self.items.append(("cavity 256^2 Re 3200", d))
+        d = make_case("channel", 512, 1000.0, 1.0)
+        assemble_pprime(d, 1)
+        self.items.append(("channel 512^2 Re 1000", d))
+        d, b = standalone_system(1024, seed=1024)
+        d.upload("PC_RHS", b)
+        self.items.append(("d=1 1024^2", d))
+        self.fcg = LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
+
+    def score(self, weights):
+        its = []
+        for _, d in self.items:
+            d.set_learned_weights(weights)
+            r = d.solve_pressure_rhs(self.fcg, None, download=False)
+            its.append(r.iterations if r.status == 0 and not r.fallback_used else 20000)
+        return float(np.mean(np.log(its))), its
+
+    def close(self):
+        for _, d in self.items:
+            d.close()
+        self.items = []
 
 
 # ---------------------------------------------------------------------------
@@ -304,14 +372,22 @@ class PairBank:
             self.count[n] = 0
             self.systems[n] = 0
 
-    def harvest(self, d, n, stride, max_pairs):
+    def harvest(self, d, n, stride, max_pairs, noise=0, seed=0):
+        """Pairs from an FP64 Jacobi-PCG solve of d's assembled p' system, plus
+        `noise` white-noise pairs on the same operator."""
         k0, s = self.count[n], self.systems[n]
-        room = min(max_pairs, self.capacity[n][0] - k0)
+        room = min(max_pairs + noise, self.capacity[n][0] - k0)
         if room <= 0 or s >= self.capacity[n][1]:
             return 0, None
         settings = LinearSolverSettings(0.0, 1e-10, 400000, "CG", "Jacobi", check_interval=64)
-        got, den, res = d.harvest(settings, stride, room, n, n, self.r[n][k0], self.e[n][k0],
-                                  self.ce[n][s], self.cn[n][s])
+        got, den, res = d.harvest(settings, stride, max(0, room - noise), n, n, self.r[n][k0],
+                                  self.e[n][k0], self.ce[n][s], self.cn[n][s])
+        k1 = k0 + got
+        wn = min(noise, self.capacity[n][0] - k1)
+        if wn > 0:
+            gn, dn = d.harvest_noise(seed, wn, n, n, self.r[n][k1], self.e[n][k1])
+            den = np.concatenate([den, dn])
+            got += gn
         self.den[n][k0:k0 + got] = den
         self.sys[n][k0:k0 + got] = s
         self.count[n] += got
@@ -322,8 +398,9 @@ class PairBank:
         return sum(self.count.values())
 
 
-def harvest_cases(torch, cases, capacity, seed=0, deadline=None):
-    """Runs SIMPLE cases and harvests p' pairs; cases: list of (kind, n, re, speed, warm)."""
+def harvest_cases(torch, cases, capacity, seed=0, deadline=None, noise=0):
+    """Runs SIMPLE cases and harvests p' pairs; cases: list of (kind, n, re, speed, warm);
+    `noise` white-noise pairs per case besides the solve's own."""
     bank = PairBank(torch, capacity)
     t0 = time.time()
     for i, (kind, n, re, speed, warm) in enumerate(cases):
@@ -334,7 +411,7 @@ def harvest_cases(torch, cases, capacity, seed=0, deadline=None):
         try:
             assemble_pprime(d, warm)
             stride = max(4, n // 16)
-            got, res = bank.harvest(d, n, stride, 200)
+            got, res = bank.harvest(d, n, stride, 200, noise=noise, seed=seed * 1000 + i)
         finally:
             d.close()
         it = res.iterations if res is not None else 0
@@ -421,6 +498,8 @@ def evaluate(weights, n, tol=1e-8):
         rj = d.solve_pressure_rhs(jac, None, download=False)
         rf = d.solve_pressure_rhs(fcg, None, download=False)
         return dict(n=n, jacobi_iterations=rj.iterations, jacobi_ms=rj.device_ms,
+                    jacobi_status=rj.status_name,
+                    jacobi_true_rel=rj.true_residual / rj.initial_residual,
                     fcg_iterations=rf.iterations, fcg_ms=rf.device_ms,
                     fcg_fallback=rf.fallback_used, fcg_status=rf.status_name,
                     fcg_true_rel=rf.true_residual / rf.initial_residual)
@@ -461,6 +540,10 @@ def main(argv=None):
     ap.add_argument("--noise", type=float, default=1e-3,
                     help="random weights in the unused channels of the initial network")
     ap.add_argument("--eval", type=int, nargs="*", default=[256, 1024])
+    ap.add_argument("--noise-pairs", type=int, default=20,
+                    help="white-noise pairs per harvested system")
+    ap.add_argument("--checkpoints", type=int, default=10,
+                    help="validation scores during training (model selection)")
     ap.add_argument("--summary", default=None)
     args = ap.parse_args(argv)
     if args.time_forward:
@@ -481,15 +564,21 @@ def main(argv=None):
         w0 = SymmetricMultilevel.weights(theta0)
     summary["eval_init"] = [evaluate(w0, n) for n in args.eval]
     _log("init:", summary["eval_init"])
-    bank = harvest_cases(torch, cases, cap, args.seed, time.time() + args.harvest_seconds)
+    bank = harvest_cases(torch, cases, cap, args.seed, time.time() + args.harvest_seconds,
+                         noise=args.noise_pairs)
     summary["pairs"] = {str(n): int(c) for n, c in bank.count.items()}
     if args.free:
         w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
                           time.time() + args.train_seconds)
     else:
-        theta, losses, evals = train_symmetric(
-            torch, bank, theta0, args.steps, args.lr, args.seed, time.time() + args.train_seconds,
-            eval_fn=lambda ww: evaluate(ww, 512), eval_every=max(1, args.steps // 4))
+        val = Validation()
+        try:
+            theta, losses, evals = train_symmetric(
+                torch, bank, theta0, args.steps, args.lr, args.seed,
+                time.time() + args.train_seconds, eval_fn=val.score,
+                eval_every=max(1, args.steps // args.checkpoints))
+        finally:
+            val.close()
         summary["theta"] = theta.tolist()
         summary["evals"] = evals
         w = SymmetricMultilevel.weights(theta)

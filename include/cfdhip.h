@@ -81,7 +81,8 @@ typedef struct {
  * vector (copied separately into caller memory). */
 typedef struct {
   int32_t status;          /* CFDHIP_CONVERGED ... */
-  int32_t fallback_used;   /* FCG: 1 if the Jacobi-PCG fallback finished the solve */
+  int32_t fallback_used;   /* 1 if FP64 Jacobi-PCG finished the solve (FCG stagnation, or a
+                              mixed-precision refinement cycle that did not halve |r|) */
   int64_t iterations;
   int64_t restarts;        /* BiCGSTAB restarts / refinement cycles (mixed) */
   double initial_residual; /* |b - A x0| */
@@ -277,6 +278,15 @@ int cfdhip_pressure_harvest(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, co
                             int64_t stride, int64_t max_pairs, int64_t h, int64_t w, float* r_out,
                             float* e_out, float* ce_out, float* cn_out, float* den_out,
                             int64_t* pairs, cfdhip_solve_result* result);
+
+/* White-noise training pairs on the assembled p' operator: e uniform in [-1, 1]
+ * per cell (splitmix64 of seed, pair and cell), r = A e, written like the pairs
+ * of cfdhip_pressure_harvest (scaled system, unit RMS of r^, padded to h x w;
+ * r_out, e_out device, den_out host). They carry the rough error modes a
+ * converging solve has already removed. */
+int cfdhip_pressure_harvest_noise(cfdhip_ctx* ctx, uint64_t seed, int64_t count, int64_t h,
+                                  int64_t w, float* r_out, float* e_out, float* den_out,
+                                  int64_t* pairs);
 
 /* A network handle for training and evaluation on caller-owned device buffers
  * (FP32, batch x h x w, h and w multiples of 8). Calls are synchronous. */

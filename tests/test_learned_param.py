@@ -40,3 +40,22 @@ def test_every_theta_gives_a_symmetric_operator():
     ny = U.forward(params, y)
     # (y, N x) == (N y, x) for a symmetric N
     assert abs(np.sum(y * nx) - np.sum(ny * x)) <= 1e-9 * np.sqrt(np.sum(nx * nx) * np.sum(y * y))
+
+
+def test_repair_keeps_level_kernels_positive_semidefinite():
+    """After repair every level kernel has a nonnegative symbol on the whole
+    frequency square, and the network is positive definite on random inputs."""
+    rng = np.random.default_rng(4)
+    a = np.linspace(0.0, np.pi, 33)
+    X, Y = np.meshgrid(np.cos(a), np.cos(a))
+    for _ in range(50):
+        th = S.repair(S.initial() + rng.normal(0.0, 0.5, S.N_THETA))
+        for lvl in range(4):
+            c, e, k = th[13 + 3 * lvl:16 + 3 * lvl]
+            assert np.min(c + 2 * e * (X + Y) + 4 * k * X * Y) >= -1e-12
+    params = U.unflatten(S.weights(th).astype(np.float64))
+    for _ in range(5):
+        x = rng.normal(size=(1, 16, 16, 1))
+        assert np.sum(x * U.forward(params, x)) > 0.0
+    # the multilevel start is already feasible (up to the margin)
+    assert np.max(np.abs(S.repair(S.initial()) - S.initial())) < 5e-3
