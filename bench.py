@@ -18,7 +18,9 @@ HBM roofline, and the batched leg: the p' systems of the first SIMPLE
 iteration of 256 cavities and channels of uneven size (seed 5005) solved
 together by the batched mixed-precision Jacobi-PCG (aggregate cells x
 iterations per second, per-case p50 / p99 completion time) and one after
-another by the single-case solve; and the 3D leg: SIMPLE iterations on the
+another by the single-case solve; the reference's own 129^2 cavity (Re 100)
+SIMPLE iterations per second with stream-launched and graph-replayed p'
+iteration chunks; and the 3D leg: SIMPLE iterations on the
 256^3 cavity at Re = 400 with FP64 and with mixed-precision p' solves, and the
 mixed run's pressure / velocity error against the FP64 run.
 
@@ -145,7 +147,7 @@ def batch_cases(seed=5005, count=256):
     return out
 
 
-def batch_leg(cfd, log, cases):
+def batch_leg(cfd, log, cases, use_graph=True, sequential=True):
     """p' systems of the first SIMPLE iteration of each case, solved (a) by the
     batched mixed-precision Jacobi-PCG in one launch per step and (b) one case
     after another by the single-case mixed-precision solve, both to 1e-8 FP64
@@ -163,6 +165,8 @@ def batch_leg(cfd, log, cases):
             learned.assemble_pprime(d, 0)
             batch.add_pressure(d)
             cells += nx * ny
+            if not sequential:
+                continue
             d._ctx.call("cfdhip_synchronize")
             t1 = time.perf_counter()
             r = d.solve_pressure_rhs(mx, None, download=False)
@@ -177,9 +181,9 @@ def batch_leg(cfd, log, cases):
     log(f"batch: {len(cases)} cases, {cells} cells, set up in {time.perf_counter() - t0:.1f} s")
     bs = cfd.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi", precision="mixed",
                                   check_interval=32)
-    batch.solve(bs)  # warm-up (also tunes the tile height)
+    batch.solve(bs, use_graph=use_graph)  # warm-up (also tunes the tile height)
     t1 = time.perf_counter()
-    res = batch.solve(bs)
+    res = batch.solve(bs, use_graph=use_graph)
     wall = time.perf_counter() - t1
     st = batch.stats
     bad = [k for k, r in enumerate(res) if r.status != 0]
@@ -194,18 +198,52 @@ def batch_leg(cfd, log, cases):
            "batched_tile_rows": st["tile_rows"], "batched_steps": st["steps"],
            "batched_graph_captures": st["graph_captures"],
            "tile_tuning_us": st["tune_us"],
-           "sequential_s": seq_s,
-           "sequential_cell_iter_per_s": sum(r.iterations * nx * ny for r, (_, nx, ny, _) in
-                                             zip(res, cases)) / seq_s,
-           "sequential_p50_ms": float(np.percentile(seq_lat, 50)),
-           "sequential_p99_ms": float(np.percentile(seq_lat, 99)),
-           "speedup_batched_vs_sequential": seq_s / wall}
+           }
+    if sequential:
+        out.update({"sequential_s": seq_s,
+                    "sequential_cell_iter_per_s": sum(r.iterations * nx * ny for r, (_, nx, ny, _)
+                                                      in zip(res, cases)) / seq_s,
+                    "sequential_p50_ms": float(np.percentile(seq_lat, 50)),
+                    "sequential_p99_ms": float(np.percentile(seq_lat, 99)),
+                    "speedup_batched_vs_sequential": seq_s / wall})
     log(f"batch: batched {wall * 1e3:.1f} ms ({out['batched_cell_iter_per_s']:.3e} cell-it/s, "
         f"p50 {out['batched_p50_ms']:.1f} ms, p99 {out['batched_p99_ms']:.1f} ms, "
-        f"{st['tile_rows']}-row tiles, {st['graph_captures']} graph captures); sequential "
-        f"{seq_s * 1e3:.1f} ms (p50 {out['sequential_p50_ms']:.1f} ms, "
-        f"p99 {out['sequential_p99_ms']:.1f} ms)")
+        f"{st['tile_rows']}-row tiles, {st['graph_captures']} graph captures)" +
+        (f"; sequential {seq_s * 1e3:.1f} ms (p50 {out['sequential_p50_ms']:.1f} ms, "
+         f"p99 {out['sequential_p99_ms']:.1f} ms)" if sequential else ""))
     batch.close()
+    return out
+
+
+def cavity129_leg(cfd, log, iters):
+    """The reference's CPU-runnable case: 129^2 cavity, Re 100 (mu = 0.01),
+    Rhie-Chow, alpha_u 0.7, alpha_p 0.3, CG + Jacobi p' to 1e-8, BiCGSTAB
+    momentum; `iters` SIMPLE iterations from rest with stream-launched p'
+    iteration chunks and again with graph-replayed chunks."""
+    out = {}
+    for graph in (False, True):
+        d = cfd.GpuSimpleDiscretization.cavity(2, 129, 129, viscosity=0.01)
+        try:
+            ps = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "CG", "Jacobi", check_interval=32)
+            ms = cfd.LinearSolverSettings(1e-10, 1e-8, 500, "BiCGSTAB")
+            d.simple_run(2, pressure=ps, momentum=ms, use_graph=graph)  # warm-up
+            d.upload("U", np.zeros(d.n_cells))
+            d.upload("V", np.zeros(d.n_cells))
+            d.upload("P", np.zeros(d.n_cells))
+            d.upload("FLUX", np.zeros(d.n_faces))
+            d._ctx.call("cfdhip_synchronize")
+            t0 = time.perf_counter()
+            res = d.simple_run(iters, pressure=ps, momentum=ms, use_graph=graph)
+            dt = time.perf_counter() - t0
+            key = "graph" if graph else "stream"
+            out[f"{key}_simple_iter_per_s"] = res["iterations"] / dt
+            out[f"{key}_pressure_iterations"] = res["pressure_iterations"]
+        finally:
+            d.close()
+    out["simple_iterations"] = iters
+    log(f"129^2 cavity Re 100: {iters} SIMPLE iterations, stream launches "
+        f"{out['stream_simple_iter_per_s']:.1f} it/s, graph replay "
+        f"{out['graph_simple_iter_per_s']:.1f} it/s ({out['graph_pressure_iterations']} p' it)")
     return out
 
 
@@ -250,6 +288,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--simple-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c129-iters", type=int, default=200,
+                    help="SIMPLE iterations of the 129^2 cavity leg (0 skips it)")
     ap.add_argument("--cube", type=int, default=256, help="3D mixed-precision leg: cells per side")
     ap.add_argument("--cube-iters", type=int, default=2,
                     help="SIMPLE iterations of the 3D leg (0 skips it)")
@@ -261,6 +301,8 @@ def main():
                     help="only the Jacobi-PCG solves (for kernel counters)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched-solve leg (for kernel profiles)")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="batched leg with stream launches instead of graph replay")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -285,7 +327,9 @@ def main():
         print(*a, file=sys.stderr, flush=True)
 
     if args.batch_only:
-        print(json.dumps(batch_leg(cfd, log, batch_cases(count=args.batch_cases))), flush=True)
+        print(json.dumps(batch_leg(cfd, log, batch_cases(count=args.batch_cases),
+                                   use_graph=not args.batch_no_graph, sequential=False)),
+              flush=True)
         return
 
     # --- resident operator + RHS ------------------------------------------------
@@ -306,6 +350,29 @@ def main():
         d.set_learned_weights(learned.structured_init(0, noise=0.0))
     fcg = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
     log(f"setup done (learned weights: {weights_kind})")
+
+    # --- the plain path's SpMV on its own (y = A x, device buffers) ----------------
+    import torch
+    xs = torch.rand(n, dtype=torch.float64, device="cuda")
+    ys = torch.empty_like(xs)
+    torch.cuda.synchronize()
+    d._ctx.call("cfdhip_pressure_spmv_device", xs.data_ptr(), ys.data_ptr(), 5)
+    d.set_timing(True)
+    d._ctx.call("cfdhip_pressure_spmv_device", xs.data_ptr(), ys.data_ptr(), 50)
+    d._ctx.call("cfdhip_synchronize")
+    spmv_us = d.kernel_timing()["mean_us"]
+    d.set_timing(False)
+    # operator format: diag (n) + one coupling per face (nf), x (n), y (n)
+    spmv_fmt_bytes = 8.0 * (3 * n + nf)
+    spmv_dia_bytes = 8.0 * (5 + 2) * n  # (k + 2) n s with k = 5
+    spmv_bytes = min(spmv_fmt_bytes, spmv_dia_bytes)
+    spmv = {"kernel": "spmv2dKernel", "mean_us": spmv_us, "bytes": spmv_bytes,
+            "gbs": spmv_bytes / (spmv_us * 1e-6) / 1e9,
+            "frac": spmv_bytes / (spmv_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "dia_count_bytes": spmv_dia_bytes}
+    log(f"SpMV y = A x: {spmv_us:.2f} us, {spmv['gbs']:.0f} GB/s on {spmv_bytes / 1e6:.1f} MB "
+        f"({100 * spmv['frac']:.1f} % of HBM peak)")
+    del xs, ys
 
     def timed(settings, label):
         for w in range(args.warmup):
@@ -392,6 +459,10 @@ def main():
             f"p' iterations {res['pressure_iterations']}")
         ds.close()
 
+    c129 = {}
+    if args.c129_iters > 0:
+        c129 = cavity129_leg(cfd, log, args.c129_iters)
+
     cube = {}
     if args.cube_iters > 0:
         cube = mixed3d_leg(cfd, log, args.cube, args.cube_iters)
@@ -432,7 +503,9 @@ def main():
                    "jacobi_pcg_s": jac_s, "jacobi_pcg_iterations": rj.iterations,
                    "jacobi_pcg_true_relative_residual": rj.true_residual / bnorm,
                    "speedup_learned_vs_jacobi": jac_s / fcg_s,
+                   "plain_spmv": spmv,
                    **mixed, **simple,
+                   "cavity129_simple": c129,
                    "mixed_precision_3d": cube,
                    "batched_pprime": batched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

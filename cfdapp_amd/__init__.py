@@ -76,6 +76,7 @@ class LinearSolverSettings:
     precision: str = "fp64"         # "fp64" or "mixed"
     check_interval: int = 0
     stagnation_window: int = 0
+    graph: bool = False             # replay iteration chunks from a captured HIP graph
 
     def __post_init__(self):
         # LinearSolver's constructor validation (src/algebra/LinearSolver.cpp:9-32)
@@ -104,6 +105,7 @@ class LinearSolverSettings:
         s.precision = MIXED if self.precision == "mixed" else FP64
         s.check_interval = int(self.check_interval)
         s.stagnation_window = int(self.stagnation_window)
+        s.graph = 1 if self.graph else 0
         return s
 
 

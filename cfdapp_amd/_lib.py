@@ -45,7 +45,7 @@ class SolverSettings(ctypes.Structure):
                 ("precision", ctypes.c_int32),
                 ("check_interval", ctypes.c_int32),
                 ("stagnation_window", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("graph", ctypes.c_int32)]
 
 
 class SolveResult(ctypes.Structure):
@@ -140,6 +140,7 @@ SIGNATURES = {
     "cfdhip_unet_train_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                               ctypes.c_int, P_F64]),
+    "cfdhip_pressure_spmv_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),
     "cfdhip_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "cfdhip_batch_destroy": (None, [VP]),
     "cfdhip_batch_last_error": (ctypes.c_char_p, [VP]),

@@ -389,6 +389,32 @@ int cfdhip_pressure_spmv(cfdhip_ctx* ctx, const double* x, double* y) {
   });
 }
 
+int cfdhip_pressure_spmv_device(cfdhip_ctx* ctx, const double* x, double* y, int repeat) {
+  return guarded(ctx, [&](Context& c) {
+    require(x && y, "x and y are required");
+    require(repeat >= 1, "repeat must be >= 1");
+    c.requireMesh();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c.timing) {
+      CFDHIP_CHECK(hipEventCreate(&e0));
+      CFDHIP_CHECK(hipEventCreate(&e1));
+      CFDHIP_CHECK(hipEventRecord(e0, c.stream));
+    }
+    for (int k = 0; k < repeat; ++k) pressureSpmv(c, x, y);
+    if (c.timing) {
+      CFDHIP_CHECK(hipEventRecord(e1, c.stream));
+      CFDHIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      CFDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      c.spmvCount = repeat;
+      c.spmvMeanUs = 1000.0 * ms / repeat;
+      c.timedIters = 0;
+    }
+  });
+}
+
 int cfdhip_pressure_solve_rhs(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
                               double* x, cfdhip_solve_result* r) {
   return guarded(ctx, [&](Context& c) {

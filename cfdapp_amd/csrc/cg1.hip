@@ -237,9 +237,31 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   double kMs = 0.0, kLaunches = 0.0;
   int64_t launched = 0;
   int cur = 1;
+  // graph mode: one chunk (an even number of iterations, so the buffer
+  // parity returns to where it started) is captured once and replayed; the
+  // kernels of iterations past the device verdict do nothing
+  const bool graphed = set.graph != 0 && snap == nullptr && !ctx.timing;
+  const int gChunk = chunk + (chunk & 1);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
   while (true) {
     int q = 0;
-    for (; q < chunk && launched < set.max_iterations; ++q, ++launched) {
+    if (graphed) {
+      if (exec == nullptr) {
+        CFDHIP_CHECK(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < gChunk; ++k) {
+          launch(cur, 0);
+          finalize(kStepCg1Iter, grid.x);
+          cur = 1 - cur;
+        }
+        CFDHIP_CHECK(hipStreamEndCapture(ctx.stream, &graph));
+        CFDHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      }
+      CFDHIP_CHECK(hipGraphLaunch(exec, ctx.stream));
+      launched += gChunk;
+      q = gChunk;
+    }
+    for (; !graphed && q < chunk && launched < set.max_iterations; ++q, ++launched) {
       if (ctx.timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q], ctx.stream));
       launch(cur, 0);
       if (ctx.timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q + 1], ctx.stream));
@@ -269,6 +291,8 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
     }
     if (w.hostState->done || launched >= set.max_iterations) break;
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
   CFDHIP_CHECK(hipEventSynchronize(t1));
   float ms = 0.f;

@@ -770,11 +770,58 @@ cfdhip_solve_result solveCsr(Context& ctx, const cfdhip_solver_settings& s, cons
   return dispatch(ctx, A, M.n, s, b, x, {{M.val.get(), M.val.count}});
 }
 
+namespace {
+
+// y = A x for the 2D p' operator (SparseMatrix::multiply, SparseMatrix.cpp:72-100,
+// on the operator of PressureCorrectionEquation.cpp:181-339): one lane per
+// column, 64 columns x 16 rows per block (4 rows per thread), ascending column
+// order from 0 (south, west, centre, east, north) as the CSR row holds them.
+// Streams diag, the two face-coefficient arrays, x and y once each.
+constexpr int kSpmvRows = 16;
+
+__global__ __launch_bounds__(256) void spmv2dKernel(int64_t nx, int64_t ny, int64_t pin,
+                                                    int64_t offY, const double* __restrict__ diag,
+                                                    const double* __restrict__ coeff,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ y) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63);
+  const int64_t jb = static_cast<int64_t>(blockIdx.y) * kSpmvRows + (threadIdx.x >> 6);
+  if (i >= nx) return;
+#pragma unroll
+  for (int k = 0; k < kSpmvRows / 4; ++k) {
+    const int64_t j = jb + 4 * k;
+    if (j >= ny) break;
+    const int64_t c = j * nx + i;
+    double v;
+    if (c == pin) {
+      v = 0.0 + diag[c] * x[c];
+    } else {
+      v = 0.0;
+      if (j > 0) v += (-coeff[offY + j * nx + i]) * x[c - nx];
+      if (i > 0) v += (-coeff[j * (nx + 1) + i]) * x[c - 1];
+      v += diag[c] * x[c];
+      if (i < nx - 1) v += (-coeff[j * (nx + 1) + i + 1]) * x[c + 1];
+      if (j < ny - 1) v += (-coeff[offY + (j + 1) * nx + i]) * x[c + nx];
+    }
+    y[c] = v;
+  }
+}
+
+}  // namespace
+
 void pressureSpmv(Context& ctx, const double* x, double* y) {
   ctx.requireMesh();
   require(ctx.pressureAssembled, "no p' system assembled");
-  hipLaunchKernelGGL((spmvKernel<PressureOp>), gridStride(ctx.view.n), dim3(kBlock), 0, ctx.stream,
-                     pressureOp(ctx), ctx.view.n, x, y);
+  const View& v = ctx.view;
+  if (v.dim == 2) {
+    const dim3 grid(static_cast<unsigned>((v.nx + 63) / 64),
+                    static_cast<unsigned>((v.ny + kSpmvRows - 1) / kSpmvRows));
+    hipLaunchKernelGGL(spmv2dKernel, grid, dim3(256), 0, ctx.stream, v.nx, v.ny, ctx.pin, v.offY,
+                       ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), x, y);
+  } else {
+    hipLaunchKernelGGL((spmvKernel<PressureOp>), gridStride(v.n), dim3(kBlock), 0, ctx.stream,
+                       pressureOp(ctx), v.n, x, y);
+  }
   CFDHIP_CHECK(hipGetLastError());
 }
 

@@ -43,6 +43,9 @@ void simpleRun(Context& ctx, const double* s, const cfdhip_solver_settings& mom,
           "validateSIMPLESettings: relaxation must be finite and in (0, 1]");
   require(vtol > 0.0 && ptol > 0.0 && ctol > 0.0, "validateSIMPLESettings: tolerances must be > 0");
   require(ref >= 0 && ref < n, "SIMPLE::solve: referenceCell out of range");
+  // settings[8]: replay the p' solve's iteration chunks from a captured HIP graph
+  cfdhip_solver_settings presRun = pres;
+  if (s[8] != 0.0) presRun.graph = 1;
 
   for (int k = 0; k < 6; ++k) meta[k] = 0.0;
   int status = kMaxIterations;
@@ -104,7 +107,7 @@ void simpleRun(Context& ctx, const double* s, const cfdhip_solver_settings& mom,
       break;
     }
     const cfdhip_solve_result pr =
-        solvePressure(ctx, pres, ctx.f(CFDHIP_FIELD_PC_RHS), ctx.f(CFDHIP_FIELD_PPRIME), true);
+        solvePressure(ctx, presRun, ctx.f(CFDHIP_FIELD_PC_RHS), ctx.f(CFDHIP_FIELD_PPRIME), true);
     if (pr.status != CFDHIP_CONVERGED) {
       status = kPressureFailure;
       break;

@@ -17,6 +17,8 @@
 // The weight gradient is a GEMM over pixels, D[ci][co] += X[p + tap][ci] G[p][co]
 // per (tap, 16-channel block), with BF16 operands (gradients need no more),
 // X and G tiles in LDS, partial sums per pixel split reduced in a fixed order.
+#include <algorithm>
+
 #include "unet.hpp"
 
 namespace cfdhip {
@@ -263,12 +265,20 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   bf16* const loT = hiT + TILE_E;
   float* const stage = reinterpret_cast<float*>(region);
 
+  // Persistent over tiles: block x takes tiles x, x + gridDim.x, ... (the
+  // same count for every thread of the block); the weights are staged once and
+  // the next tile's first chunk is loaded while this tile is multiplied.
+  const int nTiles = a.tilesX * a.tilesY * a.n;
   int tile = blockIdx.x;
-  const int tx = tile % a.tilesX;
-  tile /= a.tilesX;
-  const int ty = tile % a.tilesY;
-  const int n = tile / a.tilesY;
-  const int y0 = ty * kTH, x0 = tx * kTW;
+  if (tile >= nTiles) return;
+  auto origin = [&](int t, int& tn, int& ty0, int& tx0) {
+    tx0 = (t % a.tilesX) * kTW;
+    t /= a.tilesX;
+    ty0 = (t % a.tilesY) * kTH;
+    tn = t / a.tilesY;
+  };
+  int n, y0, x0;
+  origin(tile, n, y0, x0);
   const int nb0 = blockIdx.y * NBG;
 
   // the first chunk's loads go out before the weights are staged; chunk
@@ -282,18 +292,18 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
              kInCols>
       stageBuf;
   ChunkStage<CA, CA, kGatherPlain, CC, kCoRows, kCoCols> coarseBuf;
-  const int cy0 = (y0 >> 1) - 1, cx0 = (x0 >> 1) - 1;
-  auto loadChunk = [&](int ch) {
+  auto loadChunk = [&](int ch, int tn, int ty0, int tx0) {
     if constexpr (kConcat) {
       if (ch < NCO)
-        coarseBuf.load(a.a, nullptr, n, a.h >> 1, a.w_ >> 1, cy0, cx0, ch * CC);
+        coarseBuf.load(a.a, nullptr, tn, a.h >> 1, a.w_ >> 1, (ty0 >> 1) - 1, (tx0 >> 1) - 1,
+                       ch * CC);
       else
-        stageBuf.load(a.b, nullptr, n, a.h, a.w_, y0 - 1, x0 - 1, ch * CC - CA);
+        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC - CA);
     } else if constexpr (G != kGatherFirst) {
-      stageBuf.load(a.a, a.b, n, a.h, a.w_, y0 - 1, x0 - 1, ch * CC);
+      stageBuf.load(a.a, a.b, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC);
     }
   };
-  loadChunk(0);
+  loadChunk(0, n, y0, x0);
 
   // this group's weight fragments: packed [(chunk step) * nbTotal + nb] * 64 + lane
   const u32x4* __restrict__ wp = reinterpret_cast<const u32x4*>(a.w);
@@ -326,11 +336,15 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   // the 2 x 2 block li >> 2; M-block q adds 8 q columns.
   const int ayl = 2 * wv + ((li & 3) >> 1);
   const int axl = 2 * (li >> 2) + (li & 1);
+  for (;;) {
   f4v acc[4][NBG];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
     for (int nb = 0; nb < NBG; ++nb) acc[q][nb] = f4v{0.f, 0.f, 0.f, 0.f};
+  const int nextTile = tile + static_cast<int>(gridDim.x);
+  int nn = 0, ny0 = 0, nx0 = 0;
+  if (nextTile < nTiles) origin(nextTile, nn, ny0, nx0);
 
   for (int ch = 0; ch < NCH; ++ch) {
     if (ch > 0) __syncthreads();  // the previous chunk is consumed
@@ -345,8 +359,11 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
       stageBuf.store(hiT, loT, CS);
     }
     __syncthreads();
-    if constexpr (G != kGatherFirst && NCH > 1) {
-      if (ch + 1 < NCH) loadChunk(ch + 1);
+    if constexpr (G != kGatherFirst) {
+      if (ch + 1 < NCH)
+        loadChunk(ch + 1, n, y0, x0);
+      else if (nextTile < nTiles)
+        loadChunk(0, nn, ny0, nx0);  // the next tile's first chunk
     }
     const bool coarse = kConcat && ch < NCO;
 #pragma unroll 2
@@ -427,6 +444,13 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
                                 cOff + c4 * 4) =
             *reinterpret_cast<const fx4*>(stage + px * SS + c4 * 4);
     }
+  }
+  tile = nextTile;
+  if (tile >= nTiles) break;
+  n = nn;
+  y0 = ny0;
+  x0 = nx0;
+  __syncthreads();  // the staged output is read; the region takes the next tile
   }
 }
 
@@ -510,10 +534,24 @@ __global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
   }
 }
 
+// Resident blocks of a kernel on the whole device (occupancy x compute units).
+template <class K>
+int residentBlocks(K kernel) {
+  int dev = 0, cus = 0, perCu = 0;
+  CFDHIP_CHECK(hipGetDevice(&dev));
+  CFDHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  CFDHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0));
+  return std::max(1, cus) * std::max(1, perCu);
+}
+
 template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
 void conv(hipStream_t s, const ConvArgs& a) {
-  const dim3 grid(static_cast<unsigned>(a.tilesX * a.tilesY * a.n),
-                  static_cast<unsigned>(COUT / (16 * NBG)));
+  static const int resident = residentBlocks(convKernel<CIN, CA, COUT, NBG, G, E, SPLIT>);
+  constexpr int groups = COUT / (16 * NBG);
+  const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
+  require(tiles < (int64_t{1} << 31), "the network batch has too many tiles");
+  const int64_t perGroup = std::max(1, resident / groups);
+  const dim3 grid(static_cast<unsigned>(std::min(tiles, perGroup)), static_cast<unsigned>(groups));
   hipLaunchKernelGGL((convKernel<CIN, CA, COUT, NBG, G, E, SPLIT>), grid, dim3(256), 0, s, a);
 }
 

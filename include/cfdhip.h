@@ -74,7 +74,8 @@ typedef struct {
   int32_t precision;      /* CFDHIP_FP64 or CFDHIP_MIXED */
   int32_t check_interval; /* iterations between host reads of the device verdict (0 = 32) */
   int32_t stagnation_window; /* FCG: fall back to Jacobi-PCG after this many non-improving iterations (0 = 200) */
-  int32_t reserved;
+  int32_t graph;          /* 1: replay each chunk of check_interval iterations from a captured
+                             HIP graph (2D Jacobi-PCG and the p' solves of cfdhip_simple_run) */
 } cfdhip_solver_settings;
 
 /* cfd::algebra::SolverResult (LinearSolver.hpp:103-134) without the solution
@@ -193,7 +194,9 @@ int cfdhip_continuity(cfdhip_ctx* ctx, double* out);
  * resident loop, laminar, default absolute convergence gate).
  * settings[0..10] = {max_iterations, alpha_u, alpha_p, velocity_tol,
  *   pressure_tol, continuity_tol, rhie_chow (0/1), reference_cell,
- *   use_graph (0/1), 0, 0}; momentum/pressure are the two linear solvers.
+ *   use_graph (0/1: the p' solves replay their iteration chunks from a
+ *   captured HIP graph, as settings.graph = 1), 0, 0}; momentum/pressure are
+ *   the two linear solvers.
  * meta[0..5] out = {status (SIMPLEStatus: 0 converged, 1 max iterations,
  *   2 momentum failure, 3 pressure failure, 4 non-finite), iterations,
  *   momentum linear iterations, pressure linear iterations, last continuity
@@ -232,6 +235,12 @@ int cfdhip_csr_solve(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, int64_t n
 int cfdhip_pressure_spmv(cfdhip_ctx* ctx, const double* x, double* y);
 /* Solve the resident p' operator for a caller RHS (the stand-alone p' solve):
  * x0 = 0, solution into x (may be NULL to keep it on the device in PPRIME). */
+/* y = A x on device buffers (n doubles each), `repeat` times back to back on
+ * the context stream (asynchronous; cfdhip_synchronize waits). With timing on
+ * (cfdhip_set_timing) the mean kernel time is reported by
+ * cfdhip_kernel_timing. SparseMatrix::multiply (SparseMatrix.cpp:72-100) on
+ * the resident p' operator, without the host copies of cfdhip_pressure_spmv. */
+int cfdhip_pressure_spmv_device(cfdhip_ctx* ctx, const double* x, double* y, int repeat);
 int cfdhip_pressure_solve_rhs(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
                               double* x, cfdhip_solve_result* r);
 /* Residual history of the last solve on this context (up to cap entries). */

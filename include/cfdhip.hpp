@@ -55,6 +55,7 @@ struct LinearSolverSettings {
   LinearSolverBackend backend{LinearSolverBackend::GPU};
   PreconditionerType preconditioner{PreconditionerType::None};
   Precision precision{Precision::FP64};
+  bool graph{false};  // replay iteration chunks from a captured HIP graph
 
   // LinearSolver's constructor validation (src/algebra/LinearSolver.cpp:9-32).
   void validate() const {
@@ -80,6 +81,7 @@ struct LinearSolverSettings {
                        : preconditioner == PreconditionerType::Learned ? CFDHIP_PREC_LEARNED
                                                                       : CFDHIP_PREC_NONE;
     s.precision = precision == Precision::Mixed ? CFDHIP_MIXED : CFDHIP_FP64;
+    s.graph = graph ? 1 : 0;
     return s;
   }
 };

@@ -6,9 +6,10 @@ Re = 2000 after three SIMPLE iterations (non-uniform response coefficients;
 a case neither trained on nor used for model selection). Both methods start from x0 = 0 and
 must reach the same FP64 true residual, |b - A x| <= 1e-8 |b|. Flexible CG
 with the shipped trained network must need at most a fifth of the
-iterations of Jacobi-PCG (the method without it). The iteration counts and
-device times of the trained network, of the untrained multilevel start and
-of Jacobi-PCG are printed side by side.
+iterations of Jacobi-PCG (the method without it), and fewer than the untrained
+multilevel start on the d = 1 systems (within 5 % on the SIMPLE operator).
+The iteration counts and device times of the trained network, of the
+untrained start and of Jacobi-PCG are printed side by side.
 """
 import numpy as np
 import pytest
@@ -43,6 +44,9 @@ def test_trained_network_gain_standalone(n):
         # Jacobi-PCG stops on its recurrence residual (reference semantics)
         assert r["jacobi_true_rel"] <= 1.05e-8
     assert 5 * tr["fcg_iterations"] <= tr["jacobi_iterations"]
+    # training gains over its own starting point (the shipped model was selected
+    # on other held-out systems: 256^2 Re 3200, 512^2 channel, 1024^2)
+    assert tr["fcg_iterations"] < init["fcg_iterations"]
 
 
 def test_trained_network_gain_simple_operator():
@@ -68,5 +72,6 @@ def test_trained_network_gain_simple_operator():
         # the two solutions agree to what the tolerance allows
         assert np.linalg.norm(rf.solution - rj.solution) <= 1e-4 * np.linalg.norm(rj.solution)
         assert 5 * rf.iterations <= rj.iterations
+        assert rf.iterations <= 1.05 * ri.iterations
     finally:
         d.close()
