@@ -30,7 +30,7 @@ from ._lib import (BREAKDOWN, CONVERGED, FIELDS, FIXED_GRADIENT, FIXED_VALUE, FP
 
 __all__ = ["device_available", "LinearSolverSettings", "SolverResult", "make_gpu_cg",
            "make_gpu_bicgstab", "GpuSimpleDiscretization", "CfdHipError", "UnetNet",
-           "unet_param_count", "load_learned_weights", "WEIGHTS_PATH", "BatchSolver"]
+           "unet_param_count", "load_learned_weights", "load_learned_options", "WEIGHTS_PATH", "BatchSolver"]
 
 
 class CfdHipError(RuntimeError):
@@ -286,6 +286,12 @@ class GpuSimpleDiscretization:
         self._ctx.call("cfdhip_field_download", FIELDS[name], _pf(a), len(a))
         return a
 
+    def field_device(self, name):
+        """Device address of a resident field (borrowed from the context)."""
+        ptr = ctypes.c_void_p()
+        self._ctx.call("cfdhip_field_device", FIELDS[name], ctypes.byref(ptr))
+        return ptr.value
+
     def upload_initial_state(self, velocity, pressure, mass_flux=None, viscosity=None):
         for k, name in enumerate("UVW"[:self.dim]):
             self.upload(name, velocity[k])
@@ -384,8 +390,8 @@ class GpuSimpleDiscretization:
 
     def preconditioner_timing(self):
         """Per-layer (ms, launches) of the learned preconditioner while timing is on."""
-        ms = np.zeros(15)
-        calls = np.zeros(15, np.int64)
+        ms = np.zeros(16)
+        calls = np.zeros(16, np.int64)
         self._ctx.call("cfdhip_preconditioner_timing", _pf(ms), _pi(calls))
         return ms, calls
 
@@ -394,12 +400,22 @@ class GpuSimpleDiscretization:
         self._ctx.call("cfdhip_kernel_timing", _pf(out))
         return dict(launches=out[0], mean_us=out[1], iterations=out[2])
 
-    def set_learned_weights(self, weights=None):
-        """Loads the learned p' preconditioner (FP32 master weights; the
-        trained set shipped with the package by default)."""
-        w = np.ascontiguousarray(load_learned_weights() if weights is None else weights,
-                                 dtype=np.float32)
+    def set_learned_weights(self, weights=None, options=None):
+        """Loads the learned p' preconditioner (FP32 master weights and how they
+        are applied; the trained set shipped with the package by default)."""
+        if weights is None:
+            weights = load_learned_weights()
+            if options is None:
+                options = load_learned_options()
+        self.set_learned_options(options)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
         self._ctx.call("cfdhip_set_learned_weights", w.ctypes.data_as(_lib.P_F32), w.size)
+
+    def set_learned_options(self, options=None):
+        """Padding and coarse tail of the learned preconditioner (a dict, see
+        _unet_options; None: zero padding, no tail)."""
+        o = _unet_options(options)
+        self._ctx.call("cfdhip_set_learned_options", ctypes.byref(o))
 
     def harvest(self, settings: LinearSolverSettings, stride, max_pairs, h, w, r_out, e_out,
                 ce_out=None, cn_out=None, b=None):
@@ -549,6 +565,35 @@ def load_learned_weights(path: str | None = None) -> np.ndarray:
     return w
 
 
+def load_learned_options(path: str | None = None) -> dict:
+    """How the shipped weights are applied (the "options" record of the JSON
+    beside the weight file; zero padding and no tail when it has none)."""
+    import json
+    import os
+    meta = os.path.splitext(path or WEIGHTS_PATH)[0] + ".json"
+    if os.path.exists(meta):
+        with open(meta) as f:
+            opts = json.load(f).get("options")
+        if opts:
+            return opts
+    return {"padding": "zero"}
+
+
+def _unet_options(opts) -> "_lib.UnetOptions":
+    """cfdhip_unet_options from a dict: padding "zero" / "neumann", and an
+    optional tail {levels, omega, a, k}."""
+    o = _lib.UnetOptions()
+    opts = opts or {}
+    o.padding = 1 if opts.get("padding", "zero") == "neumann" else 0
+    tail = opts.get("tail") or {}
+    o.tail_levels = int(tail.get("levels", 0))
+    o.tail_omega = float(tail.get("omega", 0.0))
+    for q in range(3):
+        o.tail_a[q] = float((tail.get("a") or (0.0, 0.0, 0.0))[q])
+        o.tail_k[q] = float((tail.get("k") or (0.0, 0.0, 0.0))[q])
+    return o
+
+
 def _sync_producer():
     """Device buffers handed to the library are usually filled by torch on its
     own stream; the library works on its own stream, so wait for torch first."""
@@ -572,15 +617,21 @@ class UnetNet:
     every call is synchronous with respect to the library's own stream, so
     the caller synchronises its producer stream before calling."""
 
-    def __init__(self, weights=None):
+    def __init__(self, weights=None, options=None):
         self.lib = _lib.load()
         h = ctypes.c_void_p()
         rc = self.lib.cfdhip_unet_create(ctypes.byref(h))
         if rc != _lib.OK:
             raise CfdHipError(rc, "cfdhip_unet_create failed")
         self.h = h
+        if options is not None:
+            self.set_options(options)
         if weights is not None:
             self.set_weights(weights)
+
+    def set_options(self, options):
+        o = _unet_options(options)
+        self._call("cfdhip_unet_set_options", ctypes.byref(o))
 
     def close(self):
         if getattr(self, "h", None):

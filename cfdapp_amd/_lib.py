@@ -60,6 +60,14 @@ class SolveResult(ctypes.Structure):
                 ("history_length", ctypes.c_int64)]
 
 
+class UnetOptions(ctypes.Structure):
+    _fields_ = [("padding", ctypes.c_int32),
+                ("tail_levels", ctypes.c_int32),
+                ("tail_omega", ctypes.c_float),
+                ("tail_a", ctypes.c_float * 3),
+                ("tail_k", ctypes.c_float * 3)]
+
+
 # Every exported symbol with its signature; the not-gpu tests check that the
 # built library exports exactly these names.
 SIGNATURES = {
@@ -115,6 +123,7 @@ SIGNATURES = {
     # learned preconditioner (device buffers are passed as void*)
     "cfdhip_unet_param_count": (ctypes.c_int64, []),
     "cfdhip_set_learned_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
+    "cfdhip_set_learned_options": (ctypes.c_int, [VP, ctypes.POINTER(UnetOptions)]),
     "cfdhip_pressure_harvest": (ctypes.c_int, [VP, ctypes.POINTER(SolverSettings), P_F64,
                                                ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                ctypes.c_int64, VP, VP, VP, VP, P_F32, P_I64,
@@ -126,6 +135,7 @@ SIGNATURES = {
     "cfdhip_unet_destroy": (None, [VP]),
     "cfdhip_unet_last_error": (ctypes.c_char_p, [VP]),
     "cfdhip_unet_set_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
+    "cfdhip_unet_set_options": (ctypes.c_int, [VP, ctypes.POINTER(UnetOptions)]),
     "cfdhip_unet_get_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
     "cfdhip_unet_forward": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, ctypes.c_int64,
                                            ctypes.c_int64]),
@@ -141,6 +151,7 @@ SIGNATURES = {
                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                               ctypes.c_int, P_F64]),
     "cfdhip_pressure_spmv_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),
+    "cfdhip_field_device": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(VP)]),
     "cfdhip_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "cfdhip_batch_destroy": (None, [VP]),
     "cfdhip_batch_last_error": (ctypes.c_char_p, [VP]),

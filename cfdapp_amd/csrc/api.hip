@@ -206,6 +206,15 @@ int cfdhip_field_upload(cfdhip_ctx* ctx, int field, const double* host, int64_t 
   });
 }
 
+int cfdhip_field_device(cfdhip_ctx* ctx, int field, void** out) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(field >= 0 && field < CFDHIP_FIELD_COUNT, "unknown field");
+    require(out != nullptr, "out is required");
+    *out = c.f(field);
+  });
+}
+
 int cfdhip_field_download(cfdhip_ctx* ctx, int field, double* host, int64_t count) {
   return guarded(ctx, [&](Context& c) {
     c.requireMesh();
@@ -455,7 +464,7 @@ int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled) {
 int cfdhip_preconditioner_timing(cfdhip_ctx* ctx, double* ms, int64_t* calls) {
   return guarded(ctx, [&](Context& c) {
     require(ms != nullptr && calls != nullptr, "output arrays are required");
-    for (int l = 0; l < kNumLayers; ++l) {
+    for (int l = 0; l <= kNumLayers; ++l) {
       ms[l] = c.net ? c.net->layerMs[l] : 0.0;
       calls[l] = c.net ? c.net->layerCalls[l] : 0;
     }

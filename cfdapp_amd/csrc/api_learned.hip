@@ -71,6 +71,19 @@ int guardedNet(cfdhip_unet* h, F&& fn) {
   }
 }
 
+TailParams tailFrom(const cfdhip_unet_options& o) {
+  TailParams t{};
+  t.levels = o.tail_levels;
+  t.omega = o.tail_omega;
+  for (int q = 0; q < 3; ++q) {
+    t.a[q] = o.tail_a[q];
+    t.k[q] = o.tail_k[q];
+  }
+  t.src = 1;
+  t.dst = 2;
+  return t;
+}
+
 void checkShape(int64_t batch, int64_t h, int64_t w) {
   require(batch > 0 && h >= 8 && w >= 8 && h % 8 == 0 && w % 8 == 0,
           "the network grid must be a positive multiple of 8 in each direction");
@@ -88,6 +101,14 @@ int cfdhip_set_learned_weights(cfdhip_ctx* ctx, const float* weights, int64_t co
     require(weights != nullptr && count == kUnetParams,
             "learned weights: expected cfdhip_unet_param_count() floats");
     loadLearnedWeights(c, weights);
+  });
+}
+
+int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* o) {
+  return guardedCtx(ctx, [&](Context& c) {
+    require(o != nullptr, "options are required");
+    if (!c.net) c.net = std::make_unique<Unet>(c.stream);
+    c.net->setOptions(o->padding, tailFrom(*o));
   });
 }
 
@@ -156,6 +177,13 @@ int cfdhip_unet_set_weights(cfdhip_unet* h, const float* weights, int64_t count)
     require(weights != nullptr && count == kUnetParams,
             "expected cfdhip_unet_param_count() floats");
     net.setWeights(weights);
+  });
+}
+
+int cfdhip_unet_set_options(cfdhip_unet* h, const cfdhip_unet_options* o) {
+  return guardedNet(h, [&](Unet& net) {
+    require(o != nullptr, "options are required");
+    net.setOptions(o->padding, tailFrom(*o));
   });
 }
 

@@ -123,6 +123,30 @@ __global__ void invSqrtKernel(int64_t n, const double* __restrict__ diag, float*
     dis[c] = static_cast<float>(1.0 / std::sqrt(diag[c]));
 }
 
+// For a Neumann-padded network: the inverse square root of the
+// interior-equivalent diagonal, the sum of a cell's internal-face couplings
+// times 4 / their count (what the cell's diagonal would be with all four
+// neighbours, so that a constant field scales like the interior); the pinned
+// cell, or a cell without internal faces, keeps D^-1/2. 2D face order:
+// x-faces j (nx + 1) + i, y-faces offY + j nx + i.
+__global__ void interiorScaleKernel(int64_t nx, int64_t ny, int64_t offY, int64_t pin,
+                                    const double* __restrict__ diag,
+                                    const double* __restrict__ coeff, float* __restrict__ dis) {
+  const int64_t n = nx * ny;
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % nx, j = c / nx;
+    double s = 0.0;
+    int m = 0;
+    if (i > 0) { s += coeff[j * (nx + 1) + i]; ++m; }
+    if (i < nx - 1) { s += coeff[j * (nx + 1) + i + 1]; ++m; }
+    if (j > 0) { s += coeff[offY + j * nx + i]; ++m; }
+    if (j < ny - 1) { s += coeff[offY + (j + 1) * nx + i]; ++m; }
+    const double dt = (c == pin || m == 0 || !(s > 0.0)) ? diag[c] : s * 4.0 / m;
+    dis[c] = static_cast<float>(1.0 / std::sqrt(dt));
+  }
+}
+
 __global__ void zeroKernelD64(int64_t n, double* a) {
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -303,8 +327,13 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
   const OpT<double> op64{v.nx, v.ny, ctx.pin, v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG),
                          ctx.f(CFDHIP_FIELD_PC_FACE_COEFF)};
   const dim3 g = stride(n);
-  hipLaunchKernelGGL(invSqrtKernel, g, dim3(256), 0, ctx.stream, n, ctx.f(CFDHIP_FIELD_PC_DIAG),
-                     w.dis32.get());
+  if (ctx.net->padding() == 1)
+    hipLaunchKernelGGL(interiorScaleKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, v.offY,
+                       ctx.pin, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
+                       w.dis32.get());
+  else
+    hipLaunchKernelGGL(invSqrtKernel, g, dim3(256), 0, ctx.stream, n, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                       w.dis32.get());
   w.partial.ensure(kMaxParts * 4 + 8);
   w.state.ensure(1);
   w.scal.ensure(4);

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ sr
                                                        int srcW, int64_t srcBatch,
                                                        const float* __restrict__ w,
                                                        float* __restrict__ out, int n, int h,
-                                                       int wd) {
+                                                       int wd, int neumann) {
   __shared__ float ws[144];
   if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   __syncthreads();
@@ -89,8 +89,9 @@ __global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ sr
     float v[9];
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
-      const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+      int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
       v[tap] = 0.f;
+      if (neumann) padPixel(yy, xx, h, wd, 1);
       if (yy >= 0 && yy < srcH && xx >= 0 && xx < srcW) {
         const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
         const float sv = static_cast<float>(src[si]);
@@ -122,7 +123,7 @@ struct LastTileLds {
 };
 
 __device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, int b, int y0, int x0,
-                                int h, int wd, float out[2]) {
+                                int h, int wd, int neumann, float out[2]) {
   constexpr int PER = (kLNP + 255) / 256;
   fx4 v[PER][4];
 #pragma unroll
@@ -132,8 +133,8 @@ __device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, in
     for (int c4 = 0; c4 < 4; ++c4) v[k][c4] = fx4{0.f, 0.f, 0.f, 0.f};
     if (px < kLNP) {
       const int ty = px / kLC, tx = px - ty * kLC;
-      const int yy = y0 - 1 + ty, xx = x0 - 1 + tx;
-      if (yy >= 0 && yy < h && xx >= 0 && xx < wd) {
+      int yy = y0 - 1 + ty, xx = x0 - 1 + tx;
+      if (padPixel(yy, xx, h, wd, neumann)) {
         const fx4* src = reinterpret_cast<const fx4*>(
             d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
 #pragma unroll
@@ -178,7 +179,8 @@ __device__ inline void lastTileCoords(int64_t t, int tilesX, int tilesY, int& b,
 // Training output: y (FP32, n x h x w).
 __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ d0,
                                                       const float* __restrict__ w,
-                                                      float* __restrict__ y, int n, int h, int wd) {
+                                                      float* __restrict__ y, int n, int h, int wd,
+                                                      int neumann) {
   __shared__ LastTileLds L;
   if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   const int tilesX = (wd + kLTW - 1) / kLTW, tilesY = (h + kLTH - 1) / kLTH;
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ 
     lastTileCoords(t, tilesX, tilesY, b, y0, x0);
     __syncthreads();
     float out[2];
-    lastTile(d0, L, b, y0, x0, h, wd, out);
+    lastTile(d0, L, b, y0, x0, h, wd, neumann, out);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int o = threadIdx.x + j * 256;
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
                                                            const TZ* __restrict__ q,
                                                            TZ* __restrict__ z, int ny, int nx,
                                                            int64_t pin, int h, int wd,
-                                                           double* partial) {
+                                                           int neumann, double* partial) {
   __shared__ LastTileLds L;
   if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   const int tilesX = (nx + kLTW - 1) / kLTW, tilesY = (ny + kLTH - 1) / kLTH;
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
     lastTileCoords(t, tilesX, tilesY, b, y0, x0);
     __syncthreads();
     float out[2];
-    lastTile(d0, L, 0, y0, x0, h, wd, out);
+    lastTile(d0, L, 0, y0, x0, h, wd, neumann, out);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int o = threadIdx.x + j * 256;
@@ -236,11 +238,12 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
   if (partial != nullptr) blockPartials<1>(acc, partial);
 }
 
-// gd0 = conv^T(gy, 1 -> 16): gd0[p][c] = sum_tap gy[p - off(tap)] W[tap][c].
+// gd0 = conv^T(gy, 1 -> 16): gd0[p][c] = sum_tap gy[p - off(tap)] W[tap][c]
+// (Neumann padding: gy at the nearest pixel inside, exact for W mirror-symmetric in both axes).
 __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict__ gy,
                                                          const float* __restrict__ w,
                                                          float* __restrict__ gd0, int n, int h,
-                                                         int wd) {
+                                                         int wd, int neumann) {
   __shared__ float ws[144];
   if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   __syncthreads();
@@ -254,10 +257,9 @@ __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict
     float g[9];
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
-      const int yy = y - tap / 3 + 1, xx = x - tap % 3 + 1;
-      g[tap] = (yy < 0 || yy >= h || xx < 0 || xx >= wd)
-                   ? 0.f
-                   : gy[(static_cast<int64_t>(b) * h + yy) * wd + xx];
+      int yy = y - tap / 3 + 1, xx = x - tap % 3 + 1;
+      g[tap] = padPixel(yy, xx, h, wd, neumann) ? gy[(static_cast<int64_t>(b) * h + yy) * wd + xx]
+                                                : 0.f;
     }
     fx4* o = reinterpret_cast<fx4*>(gd0 + p * 16);
 #pragma unroll
@@ -279,7 +281,8 @@ __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict
 // x / gy are FP32 scalar images, g16 / a16 FP32 16-channel images.
 __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ s1,
                                                       const float* __restrict__ m16, int mode,
-                                                      int n, int h, int wd, float* partial) {
+                                                      int n, int h, int wd, int neumann,
+                                                      float* partial) {
   float acc[144];
 #pragma unroll
   for (int k = 0; k < 144; ++k) acc[k] = 0.f;
@@ -296,8 +299,8 @@ __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ 
       for (int c = 0; c < 16; ++c) gv[c] = roundBf(m16[p * 16 + c]);
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
-        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-        if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+        int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (!padPixel(yy, xx, h, wd, neumann)) continue;
         const float xv = roundBf(s1[(static_cast<int64_t>(b) * h + yy) * wd + xx]);
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += xv * gv[c];
@@ -306,8 +309,8 @@ __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ 
       const float g = roundBf(s1[p]);
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
-        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-        if (yy < 0 || yy >= h || xx < 0 || xx >= wd) continue;
+        int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (!padPixel(yy, xx, h, wd, neumann)) continue;
         const float* ap = m16 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16;
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += roundBf(ap[c]) * g;
@@ -427,6 +430,20 @@ Unet::Unet(hipStream_t s) : stream(s) {
 }
 
 void Unet::setWeights(const float* host) {
+  mirrorSymmetric = true;
+  for (int l = 0; l < kNumLayers && mirrorSymmetric; ++l) {
+    const float* w = host + layerOffset(l);
+    const int64_t pair = static_cast<int64_t>(kLayers[l].cin) * kLayers[l].cout;
+    for (int tap = 0; tap < 9 && mirrorSymmetric; ++tap) {
+      const int ky = tap / 3, kx = tap % 3;
+      const int flipY = (2 - ky) * 3 + kx, flipX = ky * 3 + (2 - kx);
+      for (int64_t i = 0; i < pair; ++i)
+        if (w[tap * pair + i] != w[flipY * pair + i] || w[tap * pair + i] != w[flipX * pair + i]) {
+          mirrorSymmetric = false;
+          break;
+        }
+    }
+  }
   CFDHIP_CHECK(hipMemcpyAsync(master.get(), host, kUnetParams * sizeof(float),
                               hipMemcpyHostToDevice, stream));
   repack();
@@ -510,7 +527,7 @@ void Unet::readTensor(int t, float* host, int64_t count) {
 
 void Unet::setTiming(bool on) {
   timing = on;
-  for (int l = 0; l < kNumLayers; ++l) {
+  for (int l = 0; l <= kNumLayers; ++l) {
     layerMs[l] = 0.0;
     layerCalls[l] = 0;
   }
@@ -551,6 +568,7 @@ void Unet::conv(int layer, bool transposed, int gather, int epi, int split, cons
   args.w_ = w;
   args.tilesX = (w + 31) / 32;
   args.tilesY = (h + 7) / 8;
+  args.neumann = neumann;
   const int ci = transposed ? kLayers[layer].cout : kLayers[layer].cin;
   const int co = transposed ? kLayers[layer].cin : kLayers[layer].cout;
   args.w = transposed ? static_cast<const void*>(bwdPack.get() + bwdOff[layer])
@@ -585,6 +603,7 @@ void Unet::convDw(int layer, int gx, int gg, const float* xa, const float* xb, c
   a.tilesX = (w + 31) / 32;
   a.tilesY = (h + 7) / 8;
   a.tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * n;
+  a.neumann = neumann;
   const int ci = kLayers[layer].cin, co = kLayers[layer].cout;
   const int npair = 9 * (ci / 16);
   const int groups = (npair + 7) / 8;
@@ -620,11 +639,11 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
     if (srcDouble)
       hipLaunchKernelGGL((firstConvKernel<double>), grid1(p0), dim3(256), 0, stream,
                          static_cast<const double*>(src), scale, srcH, srcW, srcBatch,
-                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w, neumann);
     else
       hipLaunchKernelGGL((firstConvKernel<float>), grid1(p0), dim3(256), 0, stream,
                          static_cast<const float*>(src), scale, srcH, srcW, srcBatch,
-                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w);
+                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w, neumann);
     CFDHIP_CHECK(hipGetLastError());
     conv(kE0b, false, kGatherPlain, kEpiStorePool, 0, T(kTe0a), nullptr, T(kTe0), T(kTp0), n, h,
          w);
@@ -634,6 +653,11 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
   conv(kE2a, false, kGatherPlain, kEpiStore, 0, T(kTp1), nullptr, T(kTe2a), nullptr, n, h2, w2);
   conv(kE2b, false, kGatherPlain, kEpiStorePool, 0, T(kTe2a), nullptr, T(kTe2), T(kTp2), n, h2, w2);
   conv(kB3a, false, kGatherPlain, kEpiStore, 0, T(kTp2), nullptr, T(kTb3a), nullptr, n, h3, w3);
+  if (tail.levels > 0) {
+    mark(kNumLayers, true);
+    tailApply(T(kTb3a), n, h3, w3, false);
+    mark(kNumLayers, false);
+  }
   conv(kB3b, false, kGatherPlain, kEpiStore, 0, T(kTb3a), nullptr, T(kTb3), nullptr, n, h3, w3);
   conv(kD2a, false, kGatherConcatUp, kEpiStore, 0, T(kTb3), T(kTe2), T(kTd2a), nullptr, n, h2, w2);
   conv(kD2b, false, kGatherPlain, kEpiStore, 0, T(kTd2a), nullptr, T(kTd2), nullptr, n, h2, w2);
@@ -646,7 +670,7 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
 void Unet::outputTrain(int n, int h, int w) {
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   hipLaunchKernelGGL(lastConvKernel, grid1(p0 / 2, 4096), dim3(256), 0, stream, T(kTd0),
-                     master.get() + layerOffset(kOut), y.get(), n, h, w);
+                     master.get() + layerOffset(kOut), y.get(), n, h, w, neumann);
   CFDHIP_CHECK(hipGetLastError());
 }
 
@@ -656,7 +680,7 @@ void Unet::outputApply(const float* scale, const TZ* r, const TZ* q, TZ* z, int 
   mark(kOut, true);
   hipLaunchKernelGGL((lastConvApplyKernel<TZ>), dim3(static_cast<unsigned>(parts)), dim3(256), 0,
                      stream, T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin,
-                     h, w, partial);
+                     h, w, neumann, partial);
   mark(kOut, false);
   CFDHIP_CHECK(hipGetLastError());
   collectTiming();
@@ -667,11 +691,15 @@ template void Unet::outputApply<double>(const float*, const double*, const doubl
                                         int, int64_t, int, int, double*, int);
 
 void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
+  require(!neumann || mirrorSymmetric,
+          "Neumann padding: the backward pass needs kernels mirror-symmetric in both axes "
+          "(W[ky][kx] == W[2-ky][kx] == W[ky][2-kx]); load such weights with "
+          "cfdhip_unet_set_weights");
   const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   // output layer
   hipLaunchKernelGGL(lastConvBwdKernel, grid1(p0), dim3(256), 0, stream, gyIn,
-                     master.get() + layerOffset(kOut), T(kTgd0), n, h, w);
+                     master.get() + layerOffset(kOut), T(kTgd0), n, h, w, neumann);
   singleDw(1, gyIn, T(kTd0), n, h, w, grad.get() + layerOffset(kOut));
   // decoder, level 0
   conv(kD0b, true, kGatherPlain, kEpiStore, 0, T(kTgd0), nullptr, T(kTgd0a), nullptr, n, h, w);
@@ -691,6 +719,7 @@ void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
   // bottom
   conv(kB3b, true, kGatherPlain, kEpiStore, 0, T(kTgb3), nullptr, T(kTgb3a), nullptr, n, h3, w3);
   convDw(kB3b, kGatherPlain, kGatherPlain, T(kTb3a), nullptr, T(kTgb3), nullptr, n, h3, w3);
+  tailApply(T(kTgb3a), n, h3, w3, true);
   conv(kB3a, true, kGatherPlain, kEpiStore, 0, T(kTgb3a), nullptr, T(kTgp2), nullptr, n, h3, w3);
   convDw(kB3a, kGatherPlain, kGatherPlain, T(kTp2), nullptr, T(kTgb3a), nullptr, n, h3, w3);
   // encoder, level 2: d e2 = skip + pool gradient
@@ -713,7 +742,7 @@ void Unet::singleDw(int mode, const float* s1, const float* m16, int n, int h, i
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   const dim3 g = grid1(p0, 1024);
   dwPartial.ensure(static_cast<int64_t>(g.x) * 144);
-  hipLaunchKernelGGL(singleDwKernel, g, dim3(256), 0, stream, s1, m16, mode, n, h, w,
+  hipLaunchKernelGGL(singleDwKernel, g, dim3(256), 0, stream, s1, m16, mode, n, h, w, neumann,
                      dwPartial.get());
   hipLaunchKernelGGL(singleDwReduceKernel, dim3(1), dim3(256), 0, stream, dwPartial.get(),
                      static_cast<int>(g.x), out);
@@ -748,6 +777,7 @@ void Unet::adam(float lr, float b1, float b2, float eps) {
   ++adamStep;
   const float bc1 = 1.f - std::pow(b1, static_cast<float>(adamStep));
   const float bc2 = 1.f - std::pow(b2, static_cast<float>(adamStep));
+  mirrorSymmetric = false;  // the update is not constrained to mirror-symmetric kernels
   hipLaunchKernelGGL(adamKernel, grid1(kUnetParams), dim3(256), 0, stream,
                      static_cast<int64_t>(kUnetParams), master.get(), adamM.get(), adamV.get(),
                      grad.get(), lr, b1, b2, eps, bc1, bc2);

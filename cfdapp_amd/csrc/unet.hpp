@@ -92,6 +92,9 @@ struct ConvArgs {
   const float* scale;
   const float* w0;
   int srcH, srcW, srcDouble;
+  // 1: Neumann (replicate) padding -- a tap outside the image reads the
+  // nearest pixel inside it; 0: zero padding
+  int neumann;
 };
 
 struct DwArgs {
@@ -103,6 +106,42 @@ struct DwArgs {
   int n, h, w_;
   int tilesX, tilesY;
   int64_t tiles;
+  int neumann;     // padding of the staged input tile (as ConvArgs::neumann)
+};
+
+// Pixel (y, x) of a staged window. Zero padding: false outside the h x w
+// image (the value is 0). Neumann padding: moves (y, x) to the nearest pixel
+// inside and returns true. A conv with Neumann padding and a kernel that is
+// mirror-symmetric in both axes (W[ky][kx] == W[2-ky][kx] == W[ky][2-kx] per
+// channel pair) has as its adjoint the Neumann-padded conv with the
+// transposed kernel, so the backward pass keeps the same staging; the host
+// holds training to such kernels in that mode.
+__device__ inline bool padPixel(int& y, int& x, int h, int w, int neumann) {
+  if (y >= 0 && y < h && x >= 0 && x < w) return true;
+  if (!neumann) return false;
+  y = y < 0 ? 0 : (y >= h ? h - 1 : y);
+  x = x < 0 ? 0 : (x >= w ? w - 1 : x);
+  return true;
+}
+
+// The coarse tail below the network's deepest level: a symmetric V(1,1)
+// multigrid cycle on one channel, Neumann padding throughout, levels H/16 ..
+// down to the coarsest grid with even sides (at most `levels` of them):
+//   r_{t+1} = 4 pool(K r_t - omega K A r_t)   (restriction of the residual
+//             after one Jacobi sweep from zero),
+//   e_t     = (I - omega A)(omega r_t + K up(e_{t+1})) + omega r_t,
+//   coarsest: the same without the coarse term, e = (I - omega A) omega r + omega r.
+// Applied to channel `src` of the H/8 x W/8 bottom tensor (b3a), the tail
+// writes K up(e_0) (the prolonged coarse correction) into channel `dst`;
+// its adjoint (the backward pass) adds the same operator applied to
+// channel dst into channel src and clears dst. A and K are symmetric 3 x 3
+// stencils (centre, edge, corner).
+struct TailParams {
+  int levels;         // 0: no tail
+  float omega;        // damped-Jacobi weight
+  float a[3];         // level operator A: centre, edge, corner
+  float k[3];         // transfer kernel K: centre, edge, corner
+  int src, dst;       // channels of the bottom tensor
 };
 
 // One convolution (forward or transposed) on the device.

@@ -81,7 +81,7 @@ __device__ inline fx4 gather4(const float* __restrict__ A, const float* __restri
 // one memory round trip, not one per item.
 template <int C, int CA, int G, int CC, int ROWS, int COLS>
 __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, const float* B,
-                                  int n, int h, int w, int y0, int x0, int c0) {
+                                  int n, int h, int w, int y0, int x0, int c0, int neumann) {
   constexpr int C4 = CC / 4;
   constexpr int ITEMS = ROWS * COLS * C4;
   constexpr int PER = (ITEMS + 255) / 256;
@@ -94,8 +94,8 @@ __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, co
       const int c4 = it % C4;
       const int px = it / C4;
       const int ty = px / COLS, tx = px - ty * COLS;
-      const int y = y0 + ty, x = x0 + tx;
-      if (y >= 0 && y < h && x >= 0 && x < w) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+      int y = y0 + ty, x = x0 + tx;
+      if (padPixel(y, x, h, w, neumann)) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
     }
   }
 #pragma unroll
@@ -133,7 +133,7 @@ struct ChunkStage {
   static constexpr int PER = (ITEMS + 255) / 256;
   fx4 v[PER];
   __device__ inline void load(const float* A, const float* B, int n, int h, int w, int y0, int x0,
-                              int c0) {
+                              int c0, int neumann) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int it = threadIdx.x + k * 256;
@@ -142,9 +142,8 @@ struct ChunkStage {
         const int c4 = it % C4;
         const int px = it / C4;
         const int ty = px / COLS, tx = px - ty * COLS;
-        const int y = y0 + ty, x = x0 + tx;
-        if (y >= 0 && y < h && x >= 0 && x < w)
-          v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+        int y = y0 + ty, x = x0 + tx;
+        if (padPixel(y, x, h, w, neumann)) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
       }
     }
   }
@@ -203,8 +202,9 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
   const TS* src = static_cast<const TS*>(a.src);
   for (int i = threadIdx.x; i < XR * XC; i += blockDim.x) {
     const int ty = i / XC, tx = i - ty * XC;
-    const int y = y0 - 1 + ty, x = x0 - 1 + tx;
+    int y = y0 - 1 + ty, x = x0 - 1 + tx;
     float v = 0.f;
+    if (a.neumann) padPixel(y, x, a.h, a.w_, 1);
     if (y >= 0 && y < a.srcH && x >= 0 && x < a.srcW) {
       const int64_t si = static_cast<int64_t>(n) * a.srcH * a.srcW +
                          static_cast<int64_t>(y) * a.srcW + x;
@@ -224,10 +224,12 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
     for (int k = 0; k < 4; ++k) wr[tap * 4 + k] = w0s[tap * 16 + c4 * 4 + k];
   for (int it = threadIdx.x; it < kInPix * 4; it += blockDim.x) {
     const int px = it >> 2;
-    const int ty = px / kInCols, tx = px - ty * kInCols;
-    const int y = y0 + ty, x = x0 + tx;
+    // a halo pixel outside the image takes e0a of the nearest pixel inside
+    // (Neumann padding), whose taps lie inside the staged window
+    int y = y0 + px / kInCols, x = x0 + px % kInCols;
     fx4 acc = {0.f, 0.f, 0.f, 0.f};
-    if (y >= 0 && y < a.h && x >= 0 && x < a.w_) {
+    if (padPixel(y, x, a.h, a.w_, a.neumann)) {
+      const int ty = y - y0, tx = x - x0;
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         const float v = xs[(ty + tap / 3) * XC + tx + tap % 3];
@@ -296,11 +298,11 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
     if constexpr (kConcat) {
       if (ch < NCO)
         coarseBuf.load(a.a, nullptr, tn, a.h >> 1, a.w_ >> 1, (ty0 >> 1) - 1, (tx0 >> 1) - 1,
-                       ch * CC);
+                       ch * CC, a.neumann);
       else
-        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC - CA);
+        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC - CA, a.neumann);
     } else if constexpr (G != kGatherFirst) {
-      stageBuf.load(a.a, a.b, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC);
+      stageBuf.load(a.a, a.b, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC, a.neumann);
     }
   };
   loadChunk(0, n, y0, x0);
@@ -497,15 +499,17 @@ __global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
     __syncthreads();
     if constexpr (CIN <= 64) {
       stageSplit<CIN, CA, GX, CIN, kInRows, kInCols>(xs, nullptr, CS, a.xa, a.xb, n, a.h, a.w_,
-                                                     y0 - 1, x0 - 1, 0);
+                                                     y0 - 1, x0 - 1, 0, a.neumann);
     } else {  // two halves keep the register batch small
       stageSplit<CIN, CA, GX, CIN / 2, kInRows, kInCols>(xs, nullptr, CS, a.xa, a.xb, n, a.h,
-                                                         a.w_, y0 - 1, x0 - 1, 0);
+                                                         a.w_, y0 - 1, x0 - 1, 0, a.neumann);
       stageSplit<CIN, CA, GX, CIN / 2, kInRows, kInCols>(xs + CIN / 2, nullptr, CS, a.xa, a.xb, n,
-                                                         a.h, a.w_, y0 - 1, x0 - 1, CIN / 2);
+                                                         a.h, a.w_, y0 - 1, x0 - 1, CIN / 2,
+                                                         a.neumann);
     }
+    // the output-gradient tile has no halo: zero beyond the image
     stageSplit<COUT, COUT, GG, COUT, kTH, kTW>(gs, nullptr, GS, a.ga, a.gb, n, a.h, a.w_, y0, x0,
-                                               0);
+                                               0, 0);
     __syncthreads();
     for (int s = 0; s < kTH; ++s) {
       u32x4 bfr[NOB];

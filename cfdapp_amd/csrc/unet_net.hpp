@@ -23,6 +23,10 @@ class Unet {
   explicit Unet(hipStream_t s);
 
   void setWeights(const float* host);
+  // Padding of every convolution (0 zero, 1 Neumann) and the coarse tail
+  // below the bottom level (levels 0: none; needs Neumann padding).
+  void setOptions(int neumannPadding, const TailParams& t);
+  int padding() const { return neumann; }
   void getWeights(float* host);
   void getGrads(float* host) {
     CFDHIP_CHECK(hipMemcpyAsync(host, grad.get(), kUnetParams * sizeof(float),
@@ -68,8 +72,9 @@ class Unet {
   // Per-layer kernel timing (HIP events on the stream around each layer's
   // launch; the host waits at the end of every forward pass while enabled).
   void setTiming(bool on);
-  double layerMs[kNumLayers] = {};
-  int64_t layerCalls[kNumLayers] = {};
+  // index kNumLayers: the coarse tail
+  double layerMs[kNumLayers + 1] = {};
+  int64_t layerCalls[kNumLayers + 1] = {};
 
  private:
   void repack();
@@ -79,6 +84,15 @@ class Unet {
   void convDw(int layer, int gx, int gg, const float* xa, const float* xb, const float* ga,
               const float* gb, int n, int h, int w);
   void singleDw(int mode, const float* s1, const float* m16, int n, int h, int w, float* out);
+  // The coarse tail on the bottom tensor (h3 x w3 per image): forward writes
+  // channel tail.dst from tail.src; backward is its adjoint on the gradient.
+  void tailApply(float* tensor, int n, int h3, int w3, bool backward);
+  int neumann = 0;
+  TailParams tail{};
+  // every kernel of the loaded weights is mirror-symmetric in both axes (the Neumann-padded
+  // backward pass is the exact adjoint only for such kernels)
+  bool mirrorSymmetric = false;
+  DeviceArray<float> tailWork;
 
   DeviceArray<float> master, grad, adamM, adamV, y, gy, dwPartial;
   DeviceArray<double> lossPartial;

@@ -213,6 +213,118 @@ class SymmetricMultilevel:
         return g
 
 
+class VCycle:
+    """The network as a symmetric V(1,1) multigrid cycle (Neumann padding).
+
+    Level l (H/2^l, l = 0..3) has a damped-Jacobi weight omega_l and a
+    symmetric 3 x 3 operator A_l (centre, edge, corner); K_l (l = 0..2) is the
+    transfer kernel between levels l and l + 1: restriction R = 4 pool(K .),
+    prolongation P = K up(.) = R^T. With r_l the level's residual:
+        encoder:  s_l = (I - omega_l A_l) r_l,  r_{l+1} = R s_l
+        bottom:   x_3 = (I - omega_3 A_3)(omega_3 r_3 + c_3) + omega_3 r_3
+        decoder:  x_l = (I - omega_l A_l)(omega_l r_l + P x_{l+1}) + omega_l r_l
+    (one Jacobi sweep from zero, coarse correction, one sweep back), output
+    y = scale x_0. c_3 is the coarse tail's correction (options "tail":
+    the same cycle with fixed standard coefficients down to the coarsest
+    grid, written into channel 2 of b3a) or 0. Channel use: encoder channel
+    0 carries r_l along the skip path, channel 1 carries s_l (then K s_l,
+    pooled); decoder channel 0 carries the correction, channel 1 r_l. Every
+    kernel is mirror-symmetric in both axes and the cycle is symmetric positive
+    semi-definite when omega_l (max of A_l's symbol) <= 2 (repair()).
+    theta = [omega_0..3, A_0..3 (3 each), K_0..2 (3 each), log scale].
+    """
+    N_THETA = 4 + 12 + 9 + 1
+    LAP = (1.0, -0.25, 0.0)
+    BINOMIAL = (0.25, 0.125, 0.0625)
+    # the tail's fixed coefficients (exact in BF16 and FP32)
+    TAIL = {"levels": 16, "omega": 0.75, "a": LAP, "k": BINOMIAL}
+
+    @staticmethod
+    def options(tail=True):
+        return {"padding": "neumann", "tail": dict(VCycle.TAIL) if tail else None}
+
+    @staticmethod
+    def initial(omega=0.75):
+        t = np.zeros(VCycle.N_THETA)
+        t[0:4] = omega
+        for l in range(4):
+            t[4 + 3 * l:7 + 3 * l] = VCycle.LAP
+        for j in range(3):
+            t[16 + 3 * j:19 + 3 * j] = VCycle.BINOMIAL
+        t[25] = 0.0
+        return t
+
+    @staticmethod
+    def kernel(c, e, k):
+        return np.array([[k, e, k], [e, c, e], [k, e, k]])
+
+    @staticmethod
+    def weights(theta) -> np.ndarray:
+        return VCycle.weights64(theta).astype(np.float32)
+
+    @staticmethod
+    def weights64(theta) -> np.ndarray:
+        th = np.asarray(theta, np.float64)
+        om = th[0:4]
+        A = [VCycle.kernel(*th[4 + 3 * l:7 + 3 * l]) for l in range(4)]
+        K = [VCycle.kernel(*th[16 + 3 * j:19 + 3 * j]) for j in range(3)]
+        scale = np.exp(th[25])
+        d = VCycle.kernel(1.0, 0.0, 0.0)
+        w = {name: np.zeros((3, 3, ci, co)) for name, ci, co in LAYERS}
+
+        def put(name, co, ci, k):
+            w[name][:, :, ci, co] += k
+
+        sweep = [d - om[l] * A[l] for l in range(4)]   # I - omega A
+        put("e0a", 0, 0, d)
+        put("e0a", 1, 0, sweep[0])
+        for l, (a, b) in enumerate((("e1a", "e0b"), ("e2a", "e1b"), ("b3a", "e2b"))):
+            put(b, 0, 0, d)               # r_l along the skip path
+            put(b, 1, 1, K[l])            # K s_l, pooled into the next level
+            put(a, 0, 1, 4.0 * d)         # r_{l+1} = 4 pool(K s_l)
+            put(a, 1, 1, 4.0 * sweep[l + 1])
+        put("b3b", 0, 0, om[3] * sweep[3] + om[3] * d)
+        put("b3b", 0, 2, sweep[3])        # the tail's coarse correction
+        for l, (a, b, ca) in enumerate((("d0a", "d0b", 32), ("d1a", "d1b", 64),
+                                        ("d2a", "d2b", 64))):
+            put(a, 0, 0, K[l])            # P x_{l+1}
+            put(a, 0, ca, om[l] * d)      # + omega r_l
+            put(a, 1, ca, d)              # r_l
+            put(b, 0, 0, sweep[l])
+            put(b, 0, 1, om[l] * d)
+        put("out", 0, 0, scale * d)
+        return np.concatenate([w[name].ravel() for name, _, _ in LAYERS])
+
+    @staticmethod
+    def repair(theta, margin=0.02):
+        """0 < omega_l <= (2 - margin) / max(symbol of A_l): keeps the cycle PSD."""
+        th = np.array(theta, np.float64)
+        for l in range(4):
+            c, e, k = th[4 + 3 * l:7 + 3 * l]
+            top = max(c + 4 * e + 4 * k, c - 4 * e + 4 * k, c - 4 * k, 1e-6)
+            th[l] = float(np.clip(th[l], 0.05, (2.0 - margin) / top))
+        return th
+
+    @staticmethod
+    def theta_grad(theta, gw, h=1e-3):
+        """J^T gw by central differences of the construction (exact: the
+        weights are at most quadratic in each parameter; the scale enters
+        through exp and is differentiated exactly)."""
+        th = np.asarray(theta, np.float64)
+        gw = np.asarray(gw, np.float64)
+        g = np.zeros(len(th))
+        for i in range(len(th) - 1):
+            tp, tm = th.copy(), th.copy()
+            tp[i] += h
+            tm[i] -= h
+            g[i] = float(gw @ ((VCycle.weights64(tp) - VCycle.weights64(tm)) / (2 * h)))
+        w0 = VCycle.weights64(th)
+        # centre tap of the output layer, input channel 0: ((1 * 3 + 1) * 16 + 0) * 1
+        c = sum(9 * ci * co for name, ci, co in LAYERS[:-1]) + 4 * 16
+        g[-1] = float(gw[c] * w0[c])
+        return g
+
+
 def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_every=25,
                     eval_fn=None, eval_every=0):
     """Adam on theta with the device's loss gradients. With eval_fn (theta's

@@ -153,6 +153,11 @@ int cfdhip_set_fluid(cfdhip_ctx* ctx, double density, double viscosity);
 /* ---- fields (uploadInitialState / download*, GpuSimpleDiscretization.hpp) ---- */
 int cfdhip_field_upload(cfdhip_ctx* ctx, int field, const double* host, int64_t count);
 int cfdhip_field_download(cfdhip_ctx* ctx, int field, double* host, int64_t count);
+/* Device address of a resident field (borrowed: owned by the context, valid
+ * until it is destroyed or its mesh changes), for callers that hand device
+ * buffers to the *_device entry points, as DeviceCsrMatrix::adoptDevice
+ * borrows the assembly buffers (include/cfd/gpu/DeviceCsrMatrix.hpp:75-88). */
+int cfdhip_field_device(cfdhip_ctx* ctx, int field, void** out);
 
 /* ---- SIMPLE stages (GpuSimpleDiscretization resident methods) ---------------- */
 
@@ -233,14 +238,14 @@ int cfdhip_csr_solve(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, int64_t n
                      const double* b, const double* x0, double* x, cfdhip_solve_result* r);
 /* y = A x with the resident p' operator (PC_DIAG / PC_FACE_COEFF). */
 int cfdhip_pressure_spmv(cfdhip_ctx* ctx, const double* x, double* y);
-/* Solve the resident p' operator for a caller RHS (the stand-alone p' solve):
- * x0 = 0, solution into x (may be NULL to keep it on the device in PPRIME). */
 /* y = A x on device buffers (n doubles each), `repeat` times back to back on
  * the context stream (asynchronous; cfdhip_synchronize waits). With timing on
  * (cfdhip_set_timing) the mean kernel time is reported by
  * cfdhip_kernel_timing. SparseMatrix::multiply (SparseMatrix.cpp:72-100) on
  * the resident p' operator, without the host copies of cfdhip_pressure_spmv. */
 int cfdhip_pressure_spmv_device(cfdhip_ctx* ctx, const double* x, double* y, int repeat);
+/* Solve the resident p' operator for a caller RHS (the stand-alone p' solve):
+ * x0 = 0, solution into x (may be NULL to keep it on the device in PPRIME). */
 int cfdhip_pressure_solve_rhs(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
                               double* x, cfdhip_solve_result* r);
 /* Residual history of the last solve on this context (up to cap entries). */
@@ -252,9 +257,10 @@ int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t*
  * stream), out[2] = iterations timed. Filled when timing is enabled. */
 int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled);
 int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out);
-/* Learned preconditioner kernels timed while timing is enabled: for each of the
- * 15 layers (e0a .. out; e0a runs inside e0b's kernel on the solve path),
- * total milliseconds and launches since timing was enabled. */
+/* Learned preconditioner kernels timed while timing is enabled: 16 entries,
+ * the 15 layers (e0a .. out; e0a runs inside e0b's kernel on the solve path)
+ * and the coarse tail, total milliseconds and launches since timing was
+ * enabled. */
 int cfdhip_preconditioner_timing(cfdhip_ctx* ctx, double* ms, int64_t* calls);
 
 /* ---- learned preconditioner ------------------------------------------------------ */
@@ -274,6 +280,28 @@ int cfdhip_preconditioner_timing(cfdhip_ctx* ctx, double* ms, int64_t* calls);
 int64_t cfdhip_unet_param_count(void);
 /* Loads the network weights used by CFDHIP_PREC_LEARNED on this context. */
 int cfdhip_set_learned_weights(cfdhip_ctx* ctx, const float* weights, int64_t count);
+
+/* How the network is applied (no counterpart in the reference).
+ * padding 0: every convolution zero-padded, z = D^-1/2 N(D^-1/2 r).
+ * padding 1: Neumann (replicate) padding, the padding that matches the p'
+ *   operator's zero-gradient walls, and z = S N(S r) with S the inverse square
+ *   root of the interior-equivalent diagonal (the sum of a cell's internal-face
+ *   couplings times 2 dim / their count; the pinned cell keeps its diagonal).
+ *   The backward pass then needs kernels mirror-symmetric in both axes.
+ * tail_levels > 0 (padding 1 only): a one-channel symmetric V(1,1) multigrid
+ *   cycle below the bottom level (H/16 .. the coarsest grid with even sides,
+ *   at most tail_levels levels), Neumann padding, damped-Jacobi weight
+ *   tail_omega, level operator tail_a and transfer kernel tail_k (centre,
+ *   edge, corner); it restricts channel 1 of the bottom tensor b3a and writes
+ *   the prolonged correction into its channel 2, before b3b. */
+typedef struct {
+  int32_t padding;
+  int32_t tail_levels;
+  float tail_omega;
+  float tail_a[3];
+  float tail_k[3];
+} cfdhip_unet_options;
+int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* options);
 
 /* Training pairs from the assembled p' system (b: host RHS of n doubles, or
  * NULL for the assembled one): an FP64 Jacobi-PCG solve (settings s) keeps its
@@ -304,6 +332,8 @@ int cfdhip_unet_create(cfdhip_unet** out);
 void cfdhip_unet_destroy(cfdhip_unet* net);
 const char* cfdhip_unet_last_error(const cfdhip_unet* net);
 int cfdhip_unet_set_weights(cfdhip_unet* net, const float* weights, int64_t count);
+/* Padding and coarse tail of this network (cfdhip_unet_options above). */
+int cfdhip_unet_set_options(cfdhip_unet* net, const cfdhip_unet_options* options);
 int cfdhip_unet_get_weights(cfdhip_unet* net, float* weights, int64_t count);
 /* y = N(x). */
 int cfdhip_unet_forward(cfdhip_unet* net, const float* x, float* y, int64_t batch, int64_t h,

@@ -20,6 +20,14 @@ preconditioner is a linear operator, z = M r, so flexible CG sees a fixed M):
   output:               y = conv(d0, 16->1)
 pool = 2x2 average, up = nearest-neighbour x2, [a, b] = channel concatenation
 (upsampled tensor first). Activations are NHWC; weights W[ky, kx, cin, cout].
+
+Options (``opts`` dict, the device's cfdhip_unet_options): ``padding``
+"zero" (default) or "neumann" (a tap outside the image reads the nearest pixel
+inside: the adjoint is then the transposed convolution on the grid extended by
+one ghost ring, folded back onto the boundary pixels), and ``tail`` (a
+TailParams-like dict: levels, omega, a, k) -- a one-channel symmetric V(1,1)
+multigrid cycle below the bottom level that restricts channel 1 of b3a and
+writes the prolonged correction into its channel 2 (``tail_apply``).
 """
 from __future__ import annotations
 
@@ -50,11 +58,33 @@ def init_weights(seed=0, scale=1.0):
     return w
 
 
-def conv3x3(x, w):
-    """x: (N, H, W, Cin), w: (3, 3, Cin, Cout) -> (N, H, W, Cout); zero padding."""
+def _pad1(x, neumann):
+    """One-pixel ring around the H x W axes: zeros, or the nearest pixel (Neumann)."""
+    x = np.asarray(x, dtype=np.float64)
+    if neumann:
+        return np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)), mode="edge")
+    return np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+
+
+def _fold1(xp):
+    """Adjoint of the Neumann ring: ghost pixels add onto the pixel they copy."""
+    x = xp[:, 1:-1, 1:-1, :].copy()
+    x[:, 0, :, :] += xp[:, 0, 1:-1, :]
+    x[:, -1, :, :] += xp[:, -1, 1:-1, :]
+    x[:, :, 0, :] += xp[:, 1:-1, 0, :]
+    x[:, :, -1, :] += xp[:, 1:-1, -1, :]
+    x[:, 0, 0, :] += xp[:, 0, 0, :]
+    x[:, 0, -1, :] += xp[:, 0, -1, :]
+    x[:, -1, 0, :] += xp[:, -1, 0, :]
+    x[:, -1, -1, :] += xp[:, -1, -1, :]
+    return x
+
+
+def conv3x3(x, w, neumann=False):
+    """x: (N, H, W, Cin), w: (3, 3, Cin, Cout) -> (N, H, W, Cout); zero or
+    Neumann padding."""
     n, h, wd, cin = x.shape
-    xp = np.zeros((n, h + 2, wd + 2, cin), dtype=np.float64)
-    xp[:, 1:-1, 1:-1, :] = x
+    xp = _pad1(x, neumann)
     out = np.zeros((n, h, wd, w.shape[3]), dtype=np.float64)
     for ky in range(3):
         for kx in range(3):
@@ -62,9 +92,16 @@ def conv3x3(x, w):
     return out
 
 
-def conv3x3_input_grad(g, w):
+def conv3x3_input_grad(g, w, neumann=False):
     """Adjoint of conv3x3 in x: dL/dx given dL/dout = g."""
     n, h, wd, cout = g.shape
+    if neumann:  # the transposed conv on the ghost-extended grid, folded back
+        dxp = np.zeros((n, h + 2, wd + 2, w.shape[2]), dtype=np.float64)
+        g64 = np.asarray(g, dtype=np.float64)
+        for ky in range(3):
+            for kx in range(3):
+                dxp[:, ky:ky + h, kx:kx + wd, :] += g64 @ w[ky, kx].T.astype(np.float64)
+        return _fold1(dxp)
     gp = np.zeros((n, h + 2, wd + 2, cout), dtype=np.float64)
     gp[:, 1:-1, 1:-1, :] = g
     dx = np.zeros((n, h, wd, w.shape[2]), dtype=np.float64)
@@ -76,11 +113,10 @@ def conv3x3_input_grad(g, w):
     return dx
 
 
-def conv3x3_weight_grad(x, g):
+def conv3x3_weight_grad(x, g, neumann=False):
     """dL/dw given the conv input x and dL/dout = g."""
     n, h, wd, cin = x.shape
-    xp = np.zeros((n, h + 2, wd + 2, cin), dtype=np.float64)
-    xp[:, 1:-1, 1:-1, :] = x
+    xp = _pad1(x, neumann)
     dw = np.zeros((3, 3, cin, g.shape[3]), dtype=np.float64)
     for ky in range(3):
         for kx in range(3):
@@ -107,41 +143,110 @@ def up_grad(g):
     return g.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
 
 
-def forward(w, x, keep=False):
+def _neumann(opts):
+    return bool(opts) and opts.get("padding", "zero") == "neumann"
+
+
+def _stencil(v, c, e, k):
+    """Symmetric 3 x 3 stencil (centre, edge, corner) with Neumann padding, v: (N, H, W)."""
+    p = np.pad(v, ((0, 0), (1, 1), (1, 1)), mode="edge")
+    h, w = v.shape[1:]
+    edges = p[:, 0:h, 1:w + 1] + p[:, 2:h + 2, 1:w + 1] + p[:, 1:h + 1, 0:w] + p[:, 1:h + 1, 2:w + 2]
+    corners = p[:, 0:h, 0:w] + p[:, 0:h, 2:w + 2] + p[:, 2:h + 2, 0:w] + p[:, 2:h + 2, 2:w + 2]
+    return c * v + e * edges + k * corners
+
+
+def tail_apply(s, tail):
+    """The coarse tail on (N, H, W) images: K up(V(4 pool(K s))) with V the
+    symmetric V(1,1) cycle of the device's TailParams (Neumann padding;
+    levels H/2 .. the coarsest grid with even sides, at most tail["levels"])."""
+    s = np.asarray(s, dtype=np.float64)
+    om, a, k = tail["omega"], tail["a"], tail["k"]
+    h, w = s.shape[1:]
+    L = 0
+    while L < tail["levels"] and h % 2 == 0 and w % 2 == 0 and h >= 2 and w >= 2:
+        h, w, L = h // 2, w // 2, L + 1
+    if L == 0:
+        return np.zeros_like(s)
+
+    def restrict(v):
+        kv = _stencil(v, *k)
+        n, hh, ww = kv.shape
+        return kv.reshape(n, hh // 2, 2, ww // 2, 2).sum(axis=(2, 4))
+
+    def prolong(e):
+        return _stencil(np.repeat(np.repeat(e, 2, axis=1), 2, axis=2), *k)
+
+    r = [restrict(s)]
+    for _ in range(L - 1):
+        r.append(restrict(r[-1] - om * _stencil(r[-1], *a)))
+    e = 2.0 * om * r[L - 1] - om * om * _stencil(r[L - 1], *a)
+    for t in range(L - 2, -1, -1):
+        x2 = om * r[t] + prolong(e)
+        e = x2 - om * _stencil(x2, *a) + om * r[t]
+    return prolong(e)
+
+
+def _tail_forward(b3a, opts):
+    tail = (opts or {}).get("tail")
+    if not tail or tail.get("levels", 0) == 0:
+        return b3a
+    b3a = np.array(b3a, dtype=np.float64)
+    b3a[..., 2] = tail_apply(b3a[..., 1], tail)
+    return b3a
+
+
+def _tail_backward(g, opts):
+    tail = (opts or {}).get("tail")
+    if not tail or tail.get("levels", 0) == 0:
+        return g
+    g = np.array(g, dtype=np.float64)
+    g[..., 1] += tail_apply(g[..., 2], tail)
+    g[..., 2] = 0.0
+    return g
+
+
+def forward(w, x, keep=False, opts=None):
     """x: (N, H, W, 1) with H, W divisible by 8. Returns y (N, H, W, 1) [and the tape]."""
+    nm = _neumann(opts)
+
+    def cv(a, name):
+        return conv3x3(a, w[name], nm)
+
     t = {"x": x.astype(np.float64)}
-    t["e0a"] = conv3x3(t["x"], w["e0a"])
-    t["e0"] = conv3x3(t["e0a"], w["e0b"])
+    t["e0a"] = cv(t["x"], "e0a")
+    t["e0"] = cv(t["e0a"], "e0b")
     t["p0"] = pool(t["e0"])
-    t["e1a"] = conv3x3(t["p0"], w["e1a"])
-    t["e1"] = conv3x3(t["e1a"], w["e1b"])
+    t["e1a"] = cv(t["p0"], "e1a")
+    t["e1"] = cv(t["e1a"], "e1b")
     t["p1"] = pool(t["e1"])
-    t["e2a"] = conv3x3(t["p1"], w["e2a"])
-    t["e2"] = conv3x3(t["e2a"], w["e2b"])
+    t["e2a"] = cv(t["p1"], "e2a")
+    t["e2"] = cv(t["e2a"], "e2b")
     t["p2"] = pool(t["e2"])
-    t["b3a"] = conv3x3(t["p2"], w["b3a"])
-    t["b3"] = conv3x3(t["b3a"], w["b3b"])
+    t["b3a"] = _tail_forward(cv(t["p2"], "b3a"), opts)
+    t["b3"] = cv(t["b3a"], "b3b")
     t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
-    t["d2a"] = conv3x3(t["c2"], w["d2a"])
-    t["d2"] = conv3x3(t["d2a"], w["d2b"])
+    t["d2a"] = cv(t["c2"], "d2a")
+    t["d2"] = cv(t["d2a"], "d2b")
     t["c1"] = np.concatenate([up(t["d2"]), t["e1"]], axis=3)
-    t["d1a"] = conv3x3(t["c1"], w["d1a"])
-    t["d1"] = conv3x3(t["d1a"], w["d1b"])
+    t["d1a"] = cv(t["c1"], "d1a")
+    t["d1"] = cv(t["d1a"], "d1b")
     t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
-    t["d0a"] = conv3x3(t["c0"], w["d0a"])
-    t["d0"] = conv3x3(t["d0a"], w["d0b"])
-    t["y"] = conv3x3(t["d0"], w["out"])
+    t["d0a"] = cv(t["c0"], "d0a")
+    t["d0"] = cv(t["d0a"], "d0b")
+    t["y"] = cv(t["d0"], "out")
     return (t["y"], t) if keep else t["y"]
 
 
-def backward(w, t, gy):
+def backward(w, t, gy, opts=None):
     """Gradients of a scalar loss with dL/dy = gy: returns (dW dict, dL/dx)."""
     g = {}
     dw = {}
+    nm = _neumann(opts)
 
     def conv_back(name, inp, gout):
-        dw[name] = conv3x3_weight_grad(inp, gout)
-        return conv3x3_input_grad(gout, w[name])
+        dw[name] = conv3x3_weight_grad(inp, gout, nm)
+        return conv3x3_input_grad(gout, w[name], nm)
 
     gd0 = conv_back("out", t["d0"], gy)
     gd0a = conv_back("d0b", t["d0a"], gd0)
@@ -159,7 +264,7 @@ def backward(w, t, gy):
     c = t["b3"].shape[3]
     g["b3"] = up_grad(gc2[..., :c])
     g["e2"] = gc2[..., c:]
-    gb3a = conv_back("b3b", t["b3a"], g["b3"])
+    gb3a = _tail_backward(conv_back("b3b", t["b3a"], g["b3"]), opts)
     gp2 = conv_back("b3a", t["p2"], gb3a)
     g["e2"] = g["e2"] + pool_grad(gp2)
     ge2a = conv_back("e2b", t["e2a"], g["e2"])
@@ -199,25 +304,26 @@ def split2(a):
     return hi + bf(a - hi)
 
 
-def forward_device(w, x):
+def forward_device(w, x, opts=None):
     """Forward pass with device rounding: returns (y, tape)."""
     wb = {k: bf(v) for k, v in w.items()}
     t = {"x": np.asarray(x, dtype=np.float64)}
+    nm = _neumann(opts)
 
     def conv(name, inp):
-        return conv3x3(split2(inp), wb[name])
+        return conv3x3(split2(inp), wb[name], nm)
 
     def conv_pool(name, inp):
         acc = conv(name, inp)
         return f32(acc), f32(pool(acc))
 
-    t["e0a"] = f32(conv3x3(f32(t["x"]), wb["e0a"]))
+    t["e0a"] = f32(conv3x3(f32(t["x"]), wb["e0a"], nm))
     t["e0"], t["p0"] = conv_pool("e0b", t["e0a"])
     t["e1a"] = f32(conv("e1a", t["p0"]))
     t["e1"], t["p1"] = conv_pool("e1b", t["e1a"])
     t["e2a"] = f32(conv("e2a", t["p1"]))
     t["e2"], t["p2"] = conv_pool("e2b", t["e2a"])
-    t["b3a"] = f32(conv("b3a", t["p2"]))
+    t["b3a"] = f32(_tail_forward(f32(conv("b3a", t["p2"])), opts))
     t["b3"] = f32(conv("b3b", t["b3a"]))
     t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
     t["d2a"] = f32(conv("d2a", t["c2"]))
@@ -228,33 +334,34 @@ def forward_device(w, x):
     t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
     t["d0a"] = f32(conv("d0a", t["c0"]))
     t["d0"] = f32(conv("d0b", t["d0a"]))
-    t["y"] = f32(conv3x3(t["d0"], wb["out"]))
+    t["y"] = f32(conv3x3(t["d0"], wb["out"], nm))
     return t["y"], t
 
 
-def backward_device(w, t, gy):
+def backward_device(w, t, gy, opts=None):
     """Weight gradients with device rounding (FP32 gradient tensors, split
     inputs to the transposed convolutions, BF16 operands in the weight sums)."""
     wb = {k: bf(v) for k, v in w.items()}
     dw = {}
     gy = f32(gy)
+    nm = _neumann(opts)
 
     def back(name, inp, g):
-        dw[name] = conv3x3_weight_grad(bf(inp), bf(g))
-        return conv3x3_input_grad(split2(g), wb[name])
+        dw[name] = conv3x3_weight_grad(bf(inp), bf(g), nm)
+        return conv3x3_input_grad(split2(g), wb[name], nm)
 
     def split(gc, c):  # transposed concat layer: coarse part summed 2x2, skip part
         return f32(up_grad(gc[..., :c])), f32(gc[..., c:])
 
-    dw["out"] = conv3x3_weight_grad(bf(t["d0"]), bf(gy))
-    gd0 = f32(conv3x3_input_grad(gy, wb["out"]))
+    dw["out"] = conv3x3_weight_grad(bf(t["d0"]), bf(gy), nm)
+    gd0 = f32(conv3x3_input_grad(gy, wb["out"], nm))
     gd0a = f32(back("d0b", t["d0a"], gd0))
     gd1, ge0s = split(back("d0a", t["c0"], gd0a), t["d1"].shape[3])
     gd1a = f32(back("d1b", t["d1a"], gd1))
     gd2, ge1s = split(back("d1a", t["c1"], gd1a), t["d2"].shape[3])
     gd2a = f32(back("d2b", t["d2a"], gd2))
     gb3, ge2s = split(back("d2a", t["c2"], gd2a), t["b3"].shape[3])
-    gb3a = f32(back("b3b", t["b3a"], gb3))
+    gb3a = f32(_tail_backward(f32(back("b3b", t["b3a"], gb3)), opts))
     gp2 = f32(back("b3a", t["p2"], gb3a))
     ge2 = f32(ge2s + pool_grad(gp2))
     ge2a = f32(back("e2b", t["e2a"], ge2))
@@ -264,7 +371,7 @@ def backward_device(w, t, gy):
     gp0 = f32(back("e1a", t["p0"], ge1a))
     ge0 = f32(ge0s + pool_grad(gp0))
     ge0a = f32(back("e0b", t["e0a"], ge0))
-    dw["e0a"] = conv3x3_weight_grad(bf(t["x"]), bf(ge0a))
+    dw["e0a"] = conv3x3_weight_grad(bf(t["x"]), bf(ge0a), nm)
     return dw
 
 

@@ -1,0 +1,262 @@
+"""CPU checks of the Neumann-padded network and its V-cycle parametrisation.
+
+The learned preconditioner's default configuration runs every convolution
+with Neumann (replicate) padding, adds a one-channel multigrid tail below the
+bottom level, and builds its weights from the V(1,1) cycle of
+cfdapp_amd.learned.VCycle. None of this has a reference counterpart; it is
+pinned here by mathematics:
+  * the Neumann conv's adjoint (the oracle's ghost-ring fold) against the
+    inner-product identity, for arbitrary kernels;
+  * the fact the device backward pass relies on: for kernels mirror-symmetric
+    in both axes that adjoint is the Neumann-padded conv with the transposed
+    kernel (point symmetry alone is not enough at the corners);
+  * the tail and the whole network are symmetric positive semi-definite;
+  * the network with V-cycle weights equals a V(1,1) cycle built here from
+    dense matrices (an independent construction: explicit Neumann stencil
+    matrices, restriction and prolongation);
+  * backward against central finite differences with padding and tail;
+  * dL/dtheta against central differences of the loss through the oracle;
+  * as a PCG preconditioner on the pinned d = 1 p' operator it needs a grid-
+    independent handful of iterations.
+"""
+import numpy as np
+import pytest
+
+from cfdapp_amd.learned import VCycle
+from oracle import unet as U
+
+NEU = {"padding": "neumann"}
+
+
+def _opts(tail=True):
+    return VCycle.options(tail=tail)
+
+
+def _w64(theta):
+    """The V-cycle weights as FP64 layer arrays (no FP32 rounding)."""
+    v, w, o = VCycle.weights64(theta), {}, 0
+    for name, cin, cout in U.LAYERS:
+        w[name] = v[o:o + 9 * cin * cout].reshape(3, 3, cin, cout)
+        o += 9 * cin * cout
+    return w
+
+
+@pytest.mark.parametrize("cin,cout", [(1, 3), (4, 2), (3, 5)])
+def test_neumann_conv_adjoint_identity(cin, cout):
+    rng = np.random.default_rng(cin * 7 + cout)
+    x = rng.standard_normal((2, 6, 5, cin))
+    g = rng.standard_normal((2, 6, 5, cout))
+    w = rng.standard_normal((3, 3, cin, cout))
+    lhs = np.sum(U.conv3x3(x, w, True) * g)
+    assert np.sum(x * U.conv3x3_input_grad(g, w, True)) == pytest.approx(lhs, rel=1e-12)
+    assert np.sum(U.conv3x3_weight_grad(x, g, True) * w) == pytest.approx(lhs, rel=1e-12)
+
+
+def test_neumann_conv_matches_clamped_sum():
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((1, 4, 5, 2))
+    w = rng.standard_normal((3, 3, 2, 3))
+    y = U.conv3x3(x, w, True)
+    ref = np.zeros_like(y)
+    for i in range(4):
+        for j in range(5):
+            for ky in range(3):
+                for kx in range(3):
+                    ii = min(max(i + ky - 1, 0), 3)
+                    jj = min(max(j + kx - 1, 0), 4)
+                    ref[0, i, j] += x[0, ii, jj] @ w[ky, kx]
+    np.testing.assert_allclose(y, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_mirror_symmetric_adjoint_is_neumann_conv_of_transposed_kernel():
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal((3, 3, 3, 4))
+    w = 0.25 * (w + w[::-1] + w[:, ::-1] + w[::-1, ::-1])  # mirror-symmetric in y and x
+    g = rng.standard_normal((2, 8, 6, 4))
+    exact = U.conv3x3_input_grad(g, w, True)
+    wt = np.transpose(w[::-1, ::-1], (0, 1, 3, 2))
+    np.testing.assert_allclose(U.conv3x3(g, wt, True), exact, rtol=1e-12, atol=1e-12)
+    # point symmetry alone is not enough (the corners differ), so the device
+    # refuses a Neumann backward pass unless every kernel is mirror-symmetric
+    w2 = rng.standard_normal((3, 3, 3, 4))
+    w2 = 0.5 * (w2 + w2[::-1, ::-1])
+    wt2 = np.transpose(w2[::-1, ::-1], (0, 1, 3, 2))
+    assert np.abs(U.conv3x3(g, wt2, True) - U.conv3x3_input_grad(g, w2, True)).max() > 1e-3
+
+
+def test_tail_is_symmetric_positive_semidefinite():
+    rng = np.random.default_rng(8)
+    tail = VCycle.TAIL
+    a = rng.standard_normal((2, 32, 16))
+    b = rng.standard_normal((2, 32, 16))
+    ta, tb = U.tail_apply(a, tail), U.tail_apply(b, tail)
+    assert np.sum(ta * b) == pytest.approx(np.sum(a * tb), rel=1e-12)
+    assert np.sum(ta * a) > 0.0
+    # an odd side has no tail level: the correction is zero
+    assert not np.any(U.tail_apply(rng.standard_normal((1, 5, 8)), tail))
+
+
+def _dense_stencil(h, w, c, e, k):
+    """Matrix of the symmetric 3 x 3 stencil with Neumann (replicate) padding."""
+    m = np.zeros((h * w, h * w))
+    for y in range(h):
+        for x in range(w):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    wt = c if dy == dx == 0 else (k if dy and dx else e)
+                    yy, xx = min(max(y + dy, 0), h - 1), min(max(x + dx, 0), w - 1)
+                    m[y * w + x, yy * w + xx] += wt
+    return m
+
+
+def _dense_up(h, w):
+    """Nearest upsampling from (h/2, w/2) to (h, w) as a matrix."""
+    m = np.zeros((h * w, (h // 2) * (w // 2)))
+    for y in range(h):
+        for x in range(w):
+            m[y * w + x, (y // 2) * (w // 2) + x // 2] = 1.0
+    return m
+
+
+def _dense_vcycle(theta, h, w, tail):
+    """B_0 of the V(1,1) cycle from explicit matrices: levels 0..3 with
+    theta's coefficients, then the tail's fixed ones to the coarsest grid."""
+    om = list(theta[0:4])
+    A = [tuple(theta[4 + 3 * l:7 + 3 * l]) for l in range(4)]
+    K = [tuple(theta[16 + 3 * j:19 + 3 * j]) for j in range(3)]
+    levels = 4
+    while tail and (h >> (levels - 1)) % 2 == 0 and (w >> (levels - 1)) % 2 == 0 \
+            and (h >> levels) >= 1 and (w >> levels) >= 1:
+        om.append(VCycle.TAIL["omega"])
+        A.append(VCycle.TAIL["a"])
+        K.append(VCycle.TAIL["k"])
+        levels += 1
+    # K between level 3 and the tail is the tail's kernel
+    sizes = [(h >> l, w >> l) for l in range(levels)]
+    B = None
+    for l in range(levels - 1, -1, -1):
+        hh, ww = sizes[l]
+        n = hh * ww
+        Al = _dense_stencil(hh, ww, *A[l])
+        I = np.eye(n)
+        S = I - om[l] * Al
+        base = om[l] * (I + S)  # omega r + (I - omega A) omega r
+        if B is None:
+            B = base
+            continue
+        P = _dense_stencil(hh, ww, *K[l]) @ _dense_up(hh, ww)
+        B = base + S @ P @ B @ P.T @ S
+    return B
+
+
+def test_vcycle_network_equals_dense_cycle():
+    rng = np.random.default_rng(12)
+    th = VCycle.repair(VCycle.initial() + rng.normal(0.0, 0.03, VCycle.N_THETA))
+    th[25] = 0.0
+    w = _w64(th)
+    h, wd = 32, 16
+    B = _dense_vcycle(th, h, wd, tail=True)
+    x = rng.standard_normal((1, h, wd, 1))
+    y = U.forward(w, x, opts=_opts())[0, :, :, 0].ravel()
+    np.testing.assert_allclose(y, B @ x[0, :, :, 0].ravel(), rtol=1e-10, atol=1e-10)
+    assert np.abs(B - B.T).max() < 1e-10 * np.abs(B).max()
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > -1e-10
+
+
+def test_backward_with_neumann_and_tail_matches_finite_differences():
+    rng = np.random.default_rng(21)
+    th = VCycle.initial() + rng.normal(0.0, 0.05, VCycle.N_THETA)
+    w = {k: v.copy() for k, v in _w64(th).items()}
+    x = rng.standard_normal((2, 32, 32, 1))
+    gy = rng.standard_normal((2, 32, 32, 1))
+    opts = _opts()
+
+    def loss(wd):
+        return float(np.sum(U.forward(wd, x, opts=opts) * gy))
+
+    _, tape = U.forward(w, x, keep=True, opts=opts)
+    dw, gx = U.backward(w, tape, gy, opts=opts)
+    eps = 1e-5
+    for name, cin, cout in U.LAYERS:
+        picks = [(ky, kx, ci, co) for ky, kx in ((0, 0), (1, 2), (2, 1))
+                 for ci, co in ((0, 0), (min(1, cin - 1), min(1, cout - 1)))]
+        for idx in picks:
+            wp = {k: v.copy() for k, v in w.items()}
+            wm = {k: v.copy() for k, v in w.items()}
+            wp[name][idx] += eps
+            wm[name][idx] -= eps
+            fd = (loss(wp) - loss(wm)) / (2 * eps)
+            assert dw[name][idx] == pytest.approx(fd, rel=1e-6, abs=1e-7), (name, idx)
+    assert np.sum(gx * x) == pytest.approx(loss(w), rel=1e-10)
+
+
+def test_theta_grad_matches_loss_differences():
+    rng = np.random.default_rng(4)
+    th = VCycle.initial() + rng.normal(0.0, 0.05, VCycle.N_THETA)
+    x = rng.standard_normal((1, 16, 16, 1))
+    gy = rng.standard_normal((1, 16, 16, 1))
+    opts = _opts()
+
+    def loss(t):
+        return float(np.sum(U.forward(_w64(t), x, opts=opts) * gy))
+
+    w = _w64(th)
+    _, tape = U.forward(w, x, keep=True, opts=opts)
+    dw, _ = U.backward(w, tape, gy, opts=opts)
+    g = VCycle.theta_grad(th, np.concatenate([dw[n].ravel() for n, _, _ in U.LAYERS]))
+    for i in range(VCycle.N_THETA):
+        a, b = th.copy(), th.copy()
+        a[i] += 1e-5
+        b[i] -= 1e-5
+        fd = (loss(a) - loss(b)) / 2e-5
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-6 * max(1.0, abs(fd))), i
+
+
+def test_repair_bounds_omega_by_the_operator_symbol():
+    th = VCycle.initial()
+    th[0] = 5.0
+    th[1] = -1.0
+    r = VCycle.repair(th)
+    assert 0.05 <= r[1] and r[0] * 2.0 <= 2.0  # LAP's symbol peaks at 2
+
+
+def _pcg_iterations(n, prec, tol=1e-8):
+    rng = np.random.default_rng(n)
+    b = rng.uniform(-1.0, 1.0, (n, n))
+    b -= b.mean()
+    b[0, 0] = 0.0
+    A = lambda z: U.laplacian_apply(z, pin=True)  # noqa: E731
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = prec(r)
+    p = z.copy()
+    rz = np.sum(r * z)
+    for it in range(1, 5000):
+        q = A(p)
+        a = rz / np.sum(p * q)
+        x += a * p
+        r -= a * q
+        if np.linalg.norm(b - A(x)) <= tol * np.linalg.norm(b):
+            return it
+        z = prec(r)
+        rz2 = np.sum(r * z)
+        p = z + rz2 / rz * p
+        rz = rz2
+    return -1
+
+
+def test_vcycle_preconditioner_iterations_are_grid_independent():
+    """PCG on the pinned d = 1 operator with z = S N(S r), S = 1/2 (the
+    interior-equivalent diagonal is 4), z_pin = r_pin."""
+    w = _w64(VCycle.initial())
+
+    def prec(r):
+        z = 0.25 * U.forward(w, r[None, :, :, None], opts=_opts())[0, :, :, 0]
+        z[0, 0] = r[0, 0]
+        return z
+
+    its = {n: _pcg_iterations(n, prec) for n in (32, 64, 128)}
+    jac = _pcg_iterations(64, lambda r: r / 4.0)
+    assert all(0 < k <= 20 for k in its.values()), its
+    assert its[128] <= its[32] + 4, its
+    assert jac > 10 * its[64], (jac, its)
