@@ -8,8 +8,8 @@ rho = 1, cell 0 pinned) with an RHS drawn seeded-uniform in [-1, 1]
 x0 = 0 to an FP64 true residual |b - A x| <= 1e-8 |b|. Two methods, timed
 back to back on the same resident operator and RHS:
   * learned: flexible CG (FP64 vectors) preconditioned by the U-Net (BF16
-    weights, FP32 activations), weights from cfdapp_amd/weights when present,
-    else the multilevel initialisation the training starts from;
+    weights, FP32 activations), the shipped weights of cfdapp_amd/weights
+    (the V-cycle parametrisation with Neumann padding and the coarse tail);
   * plain: Jacobi-PCG, FP64 single-kernel iteration.
 `value` is the learned solve's time. Also reported: the FP32-inner mixed
 Jacobi-PCG, SIMPLE outer iterations per second at 2048x2048, Re = 1000, the
@@ -67,6 +67,9 @@ def layer_bytes(p0):
         "d0b": f * (16 * p0 + 16 * p0),
         "out": f * 16 * p0 + p0 * (4 + 8 + 8),         # d0, D^-1/2, q -> z (FP64)
         "e0a": 0.0,
+        # coarse tail: one channel of b3a in, one out, and its levels (H/16 ..)
+        # each read and written a few times
+        "tail": f * (2 * p3 + 6 * p3 / 3),
     }
 
 
@@ -342,12 +345,13 @@ def main():
     bnorm = float(np.linalg.norm(b))
     d.upload("PC_RHS", b)
     pcg = cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi", check_interval=64)
-    weights_kind = "trained" if os.path.exists(cfd.WEIGHTS_PATH) else "multilevel initialisation"
-    if os.path.exists(cfd.WEIGHTS_PATH):
-        d.set_learned_weights()
-    else:
-        from cfdapp_amd import learned
-        d.set_learned_weights(learned.structured_init(0, noise=0.0))
+    if not os.path.exists(cfd.WEIGHTS_PATH):
+        sys.exit(f"bench.py: no learned weights at {cfd.WEIGHTS_PATH}")
+    opts = cfd.load_learned_options()
+    weights_kind = ("V-cycle parametrisation, Neumann padding, coarse tail "
+                    f"({opts['tail']['levels']} levels max), model selected by the training run"
+                    if opts.get("padding") == "neumann" else "zero-padded network")
+    d.set_learned_weights()
     fcg = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
     log(f"setup done (learned weights: {weights_kind})")
 
@@ -411,7 +415,7 @@ def main():
     log(f"PCG iteration kernel: {kt['launches']:.0f} launches, mean {spmv_us:.2f} us, "
         f"{achieved:.0f} GB/s algorithmic")
     names = ["e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "b3a", "b3b", "d2a", "d2b", "d1a", "d1b",
-             "d0a", "d0b", "out"]
+             "d0a", "d0b", "out", "tail"]
     p0 = float(n)
     lb = layer_bytes(p0)
     layers = {}
@@ -494,7 +498,8 @@ def main():
                    "learned_fcg_fallback": bool(rf.fallback_used),
                    "learned_fcg_true_relative_residual": rf.true_residual / bnorm,
                    "learned_weights": weights_kind,
-                   "learned_precision": "FCG FP64 vectors; U-Net BF16 weights, FP32 activations",
+                   "learned_precision": "FCG FP64 vectors; U-Net BF16 weights, FP32 activations "
+                                        "(hi/lo BF16 pairs at the MFMA), FP32 coarse tail",
                    "preconditioner_us_per_application": per_apply_us,
                    "preconditioner_layers": layers,
                    "jacobi_pcg_s": jac_s, "jacobi_pcg_iterations": rj.iterations,

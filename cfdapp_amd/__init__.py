@@ -225,6 +225,8 @@ class GpuSimpleDiscretization:
         self._ctx = _Context(device)
         self.dim = 0
         self.n_cells = self.n_faces = self.n_patches = 0
+        self._fixed_value = set()  # patches with a FixedValue pressure condition
+        self.pin = -1              # p' reference cell of the last assembly (-1: none)
 
     def close(self):
         self._ctx.close()
@@ -266,6 +268,10 @@ class GpuSimpleDiscretization:
     def set_boundary(self, patch, vkind, vvalue, pkind, pvalue):
         v = _f64(list(vvalue) + [0.0] * (3 - len(vvalue)))
         self._ctx.call("cfdhip_set_boundary", patch, vkind, _pf(v), pkind, float(pvalue))
+        if pkind == FIXED_VALUE:
+            self._fixed_value.add(patch)
+        else:
+            self._fixed_value.discard(patch)
 
     def mesh_faces(self):
         o = np.zeros(self.n_faces, np.int64)
@@ -336,6 +342,9 @@ class GpuSimpleDiscretization:
 
     def assemble_pressure_correction_resident(self, reference_cell=0):
         self._ctx.call("cfdhip_assemble_pressure", int(reference_cell))
+        # the reference row is pinned only without a FixedValue pressure patch
+        # (PressureCorrectionEquation.cpp:221-228)
+        self.pin = -1 if self._fixed_value else int(reference_cell)
 
     def solve_pressure_correction_resident(self, settings: LinearSolverSettings) -> SolverResult:
         r = _lib.SolveResult()
@@ -684,18 +693,20 @@ class UnetNet:
         self._call("cfdhip_unet_tensor", t, out.ctypes.data_as(_lib.P_F32), m)
         return out
 
-    def loss_grad(self, x, e, ce, cn, sys_offset, den, batch, h, w):
-        """A-norm loss and its gradient in the master weights (no update)."""
+    def loss_grad(self, x, e, ce, cn, sys_offset, den, batch, h, w, cd=None):
+        """A-norm loss and its gradient in the master weights (no update);
+        cd: the operator's diagonal (None: unit diagonal)."""
         loss = ctypes.c_double(0.0)
         g = np.zeros(unet_param_count(), dtype=np.float32)
-        self._call("cfdhip_unet_loss_grad", _dev(x), _dev(e), _dev(ce), _dev(cn),
+        self._call("cfdhip_unet_loss_grad", _dev(x), _dev(e), _dev(ce), _dev(cn), _dev(cd),
                    _dev(sys_offset), _dev(den), int(batch), int(h), int(w), ctypes.byref(loss),
                    g.ctypes.data_as(_lib.P_F32))
         return loss.value, g
 
-    def train_step(self, x, e, ce, cn, sys_offset, den, batch, h, w, lr, update=True) -> float:
+    def train_step(self, x, e, ce, cn, sys_offset, den, batch, h, w, lr, update=True,
+                   cd=None) -> float:
         loss = ctypes.c_double(0.0)
-        self._call("cfdhip_unet_train_step", _dev(x), _dev(e), _dev(ce), _dev(cn),
+        self._call("cfdhip_unet_train_step", _dev(x), _dev(e), _dev(ce), _dev(cn), _dev(cd),
                    _dev(sys_offset), _dev(den), int(batch), int(h), int(w), float(lr),
                    1 if update else 0, ctypes.byref(loss))
         return loss.value

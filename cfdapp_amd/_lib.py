@@ -145,9 +145,9 @@ SIGNATURES = {
                                             ctypes.c_int64, P_F32]),
     "cfdhip_unet_tensor_size": (ctypes.c_int64, [VP, ctypes.c_int]),
     "cfdhip_unet_tensor": (ctypes.c_int, [VP, ctypes.c_int, P_F32, ctypes.c_int64]),
-    "cfdhip_unet_loss_grad": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
+    "cfdhip_unet_loss_grad": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
                                              ctypes.c_int64, ctypes.c_int64, P_F64, P_F32]),
-    "cfdhip_unet_train_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
+    "cfdhip_unet_train_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                               ctypes.c_int, P_F64]),
     "cfdhip_pressure_spmv_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),

@@ -253,8 +253,9 @@ int64_t cfdhip_unet_tensor_size(cfdhip_unet* h, int tensor) {
 }
 
 int cfdhip_unet_loss_grad(cfdhip_unet* h, const float* x, const float* e, const float* ce,
-                          const float* cn, const int64_t* sys_offset, const float* den,
-                          int64_t batch, int64_t hh, int64_t w, double* loss, float* grad) {
+                          const float* cn, const float* cd, const int64_t* sys_offset,
+                          const float* den, int64_t batch, int64_t hh, int64_t w, double* loss,
+                          float* grad) {
   return guardedNet(h, [&](Unet& net) {
     checkShape(batch, hh, w);
     require(x && e && ce && cn && sys_offset && den && grad, "buffers are required");
@@ -262,7 +263,7 @@ int cfdhip_unet_loss_grad(cfdhip_unet* h, const float* x, const float* e, const 
     net.ensureShape(n, H, W, true);
     net.forward(x, nullptr, H, W, hh * w, n, H, W);
     net.outputTrain(n, H, W);
-    const double l = net.loss(e, ce, cn, sys_offset, den, n, H, W);
+    const double l = net.loss(e, ce, cn, cd, sys_offset, den, n, H, W);
     if (loss) *loss = l;
     net.backward(x, net.outputGrad(), n, H, W);
     net.getGrads(grad);
@@ -270,9 +271,9 @@ int cfdhip_unet_loss_grad(cfdhip_unet* h, const float* x, const float* e, const 
 }
 
 int cfdhip_unet_train_step(cfdhip_unet* h, const float* x, const float* e, const float* ce,
-                           const float* cn, const int64_t* sys_offset, const float* den,
-                           int64_t batch, int64_t hh, int64_t w, float lr, int update,
-                           double* loss) {
+                           const float* cn, const float* cd, const int64_t* sys_offset,
+                           const float* den, int64_t batch, int64_t hh, int64_t w, float lr,
+                           int update, double* loss) {
   return guardedNet(h, [&](Unet& net) {
     checkShape(batch, hh, w);
     require(x && e && ce && cn && sys_offset && den, "device buffers are required");
@@ -281,7 +282,7 @@ int cfdhip_unet_train_step(cfdhip_unet* h, const float* x, const float* e, const
     net.ensureShape(n, H, W, update != 0);
     net.forward(x, nullptr, H, W, hh * w, n, H, W);
     net.outputTrain(n, H, W);
-    const double l = net.loss(e, ce, cn, sys_offset, den, n, H, W);
+    const double l = net.loss(e, ce, cn, cd, sys_offset, den, n, H, W);
     if (loss) *loss = l;
     if (update) {
       net.backward(x, net.outputGrad(), n, H, W);

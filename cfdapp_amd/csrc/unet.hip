@@ -360,12 +360,12 @@ __global__ void dwReduceKernel(const float* __restrict__ partial, int splits, in
   }
 }
 
-// A-norm loss of the scaled system: d = y - e, A d with unit diagonal and
-// couplings cE (to x + 1) and cN (to y + 1) of the sample's system;
+// A-norm loss of the scaled system: d = y - e, A d with diagonal cD (null:
+// 1) and couplings cE (to x + 1) and cN (to y + 1) of the sample's system;
 // L = (1 / n) sum_s (d, A d)_s / den_s, gy = 2 A d / (den_s n).
 __global__ __launch_bounds__(256) void anormLossKernel(
     const float* __restrict__ y, const float* __restrict__ e, const float* __restrict__ cE,
-    const float* __restrict__ cN, const int64_t* __restrict__ sysOff,
+    const float* __restrict__ cN, const float* __restrict__ cD, const int64_t* __restrict__ sysOff,
     const float* __restrict__ den, int n, int h, int wd, float* __restrict__ gy, double* partial) {
   const int64_t hw = static_cast<int64_t>(h) * wd, total = n * hw;
   double acc[1] = {0.0};
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void anormLossKernel(
     const float* ce = cE + sysOff[s];
     const float* cn = cN + sysOff[s];
     const float d = y[p] - e[p];
-    float ad = d;
+    float ad = cD != nullptr ? cD[sysOff[s] + c] * d : d;
     if (x + 1 < wd) ad -= ce[c] * (y[p + 1] - e[p + 1]);
     if (x > 0) ad -= ce[c - 1] * (y[p - 1] - e[p - 1]);
     if (yy + 1 < h) ad -= cn[c] * (y[p + wd] - e[p + wd]);
@@ -749,12 +749,13 @@ void Unet::singleDw(int mode, const float* s1, const float* m16, int n, int h, i
   CFDHIP_CHECK(hipGetLastError());
 }
 
-double Unet::loss(const float* e, const float* cE, const float* cN, const int64_t* sysOffDev,
+double Unet::loss(const float* e, const float* cE, const float* cN, const float* cD,
+                  const int64_t* sysOffDev,
                   const float* denDev, int n, int h, int w) {
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   const dim3 g = grid1(p0, 2048);
   lossPartial.ensure(static_cast<int64_t>(g.x) + 8);
-  hipLaunchKernelGGL(anormLossKernel, g, dim3(256), 0, stream, y.get(), e, cE, cN, sysOffDev,
+  hipLaunchKernelGGL(anormLossKernel, g, dim3(256), 0, stream, y.get(), e, cE, cN, cD, sysOffDev,
                      denDev, n, h, w, gy.get(), lossPartial.get());
   double* out = lossPartial.get() + g.x;
   hipLaunchKernelGGL(sumDoubleKernel, dim3(1), dim3(kFinThreads), 0, stream, lossPartial.get(),

@@ -58,7 +58,8 @@ class Unet {
   // Weight gradients for dL/dy = gyIn (x: the FP32 network input).
   void backward(const float* x, const float* gyIn, int n, int h, int w);
   // A-norm loss of the training output against e; fills gy, returns the loss.
-  double loss(const float* e, const float* cE, const float* cN, const int64_t* sysOffDev,
+  double loss(const float* e, const float* cE, const float* cN, const float* cD,
+              const int64_t* sysOffDev,
               const float* denDev, int n, int h, int w);
   void adam(float lr, float b1, float b2, float eps);
 

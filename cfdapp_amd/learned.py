@@ -264,16 +264,24 @@ class VCycle:
 
     @staticmethod
     def weights64(theta) -> np.ndarray:
+        w = {name: np.zeros((3, 3, ci, co)) for name, ci, co in LAYERS}
+        for (name, co, ci), k in VCycle.slots(theta).items():
+            w[name][:, :, ci, co] += k
+        return np.concatenate([w[name].ravel() for name, _, _ in LAYERS])
+
+    @staticmethod
+    def slots(theta) -> dict:
+        """The non-zero 3 x 3 kernels: {(layer, cout, cin): kernel}."""
         th = np.asarray(theta, np.float64)
         om = th[0:4]
         A = [VCycle.kernel(*th[4 + 3 * l:7 + 3 * l]) for l in range(4)]
         K = [VCycle.kernel(*th[16 + 3 * j:19 + 3 * j]) for j in range(3)]
         scale = np.exp(th[25])
         d = VCycle.kernel(1.0, 0.0, 0.0)
-        w = {name: np.zeros((3, 3, ci, co)) for name, ci, co in LAYERS}
+        w = {}
 
         def put(name, co, ci, k):
-            w[name][:, :, ci, co] += k
+            w[(name, co, ci)] = w.get((name, co, ci), 0.0) + k
 
         sweep = [d - om[l] * A[l] for l in range(4)]   # I - omega A
         put("e0a", 0, 0, d)
@@ -293,7 +301,7 @@ class VCycle:
             put(b, 0, 0, sweep[l])
             put(b, 0, 1, om[l] * d)
         put("out", 0, 0, scale * d)
-        return np.concatenate([w[name].ravel() for name, _, _ in LAYERS])
+        return w
 
     @staticmethod
     def repair(theta, margin=0.02):
@@ -307,31 +315,41 @@ class VCycle:
 
     @staticmethod
     def theta_grad(theta, gw, h=1e-3):
-        """J^T gw by central differences of the construction (exact: the
-        weights are at most quadratic in each parameter; the scale enters
-        through exp and is differentiated exactly)."""
+        """J^T gw by central differences of the construction on its non-zero
+        kernels (exact: they are at most quadratic in each parameter; the
+        scale enters through exp and is differentiated exactly)."""
         th = np.asarray(theta, np.float64)
         gw = np.asarray(gw, np.float64)
+        offs, o = {}, 0
+        for name, ci, co in LAYERS:
+            offs[name] = (o, ci, co)
+            o += 9 * ci * co
+        gs = {}
+        for (name, co, ci) in VCycle.slots(th):
+            base, cin, cout = offs[name]
+            idx = base + (np.arange(9) * cin + ci) * cout + co
+            gs[(name, co, ci)] = gw[idx].reshape(3, 3)
         g = np.zeros(len(th))
         for i in range(len(th) - 1):
             tp, tm = th.copy(), th.copy()
             tp[i] += h
             tm[i] -= h
-            g[i] = float(gw @ ((VCycle.weights64(tp) - VCycle.weights64(tm)) / (2 * h)))
-        w0 = VCycle.weights64(th)
-        # centre tap of the output layer, input channel 0: ((1 * 3 + 1) * 16 + 0) * 1
-        c = sum(9 * ci * co for name, ci, co in LAYERS[:-1]) + 4 * 16
-        g[-1] = float(gw[c] * w0[c])
+            sp, sm = VCycle.slots(tp), VCycle.slots(tm)
+            g[i] = sum(float(np.sum(gs[k] * (sp[k] - sm[k]))) for k in gs) / (2 * h)
+        g[-1] = float(gs[("out", 0, 0)][1, 1]) * np.exp(th[25])
         return g
 
 
 def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_every=25,
-                    eval_fn=None, eval_every=0):
-    """Adam on theta with the device's loss gradients. With eval_fn (theta's
-    weights -> score, lower is better) the model is scored before training and
-    every eval_every steps and the best-scoring theta is returned (model
-    selection on held-out systems); returns (theta, losses, evals)."""
-    net = UnetNet(SymmetricMultilevel.weights(theta))
+                    eval_fn=None, eval_every=0, param=None, options=None):
+    """Adam on theta with the device's loss gradients (param: the
+    parametrisation, SymmetricMultilevel by default; options: how the network
+    is applied). With eval_fn (theta's weights -> score, lower is better) the
+    model is scored before training and every eval_every steps and the
+    best-scoring theta is returned (model selection on held-out systems);
+    returns (theta, losses, evals)."""
+    P = param or SymmetricMultilevel
+    net = UnetNet(P.weights(theta), options=options)
     rng = np.random.default_rng(seed)
     sizes = [n for n in bank.count if bank.count[n] > 0]
     probs = np.array([bank.count[n] * n * n for n in sizes], np.float64)
@@ -344,7 +362,7 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
     theta = np.array(theta, np.float64)
     best = (None, theta.copy(), 0)
     if eval_fn is not None:
-        sc = eval_fn(SymmetricMultilevel.weights(theta))
+        sc = eval_fn(P.weights(theta))
         evals.append((0, sc))
         best = (sc[0], theta.copy(), 0)
         _log(f"eval at 0: {sc}")
@@ -360,21 +378,21 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
         e = bank.e[n].index_select(0, it)
         off = torch.from_numpy(bank.sys[n][idx] * n * n).cuda()
         den = torch.from_numpy(bank.den[n][idx]).cuda()
-        net.set_weights(SymmetricMultilevel.weights(theta))
-        loss, gw = net.loss_grad(x, e, bank.ce[n], bank.cn[n], off, den, b, n, n)
-        g = SymmetricMultilevel.theta_grad(theta, gw)
+        net.set_weights(P.weights(theta))
+        loss, gw = net.loss_grad(x, e, bank.ce[n], bank.cn[n], off, den, b, n, n, cd=bank.cd[n])
+        g = P.theta_grad(theta, gw)
         m = b1 * m + (1 - b1) * g
         v = b2 * v + (1 - b2) * g * g
         f = 0.5 * (1.0 + math.cos(math.pi * step / steps))
         theta -= lr * f * (m / (1 - b1 ** (step + 1))) / (np.sqrt(v / (1 - b2 ** (step + 1))) + 1e-8)
-        theta = SymmetricMultilevel.repair(theta)
+        theta = P.repair(theta)
         losses.append(loss)
         if step % log_every == 0 or step == steps - 1:
             _log(f"step {step}: loss {np.mean(losses[-log_every:]):.4f} ({n}^2 x{b}, "
-                 f"{time.time() - t0:.0f} s) w {np.round(np.exp(theta[0:4]), 3).tolist()}")
+                 f"{time.time() - t0:.0f} s) theta[0:4] {np.round(theta[0:4], 3).tolist()}")
         if eval_fn is not None and eval_every and ((step + 1) % eval_every == 0 or
                                                    step == steps - 1):
-            sc = eval_fn(SymmetricMultilevel.weights(theta))
+            sc = eval_fn(P.weights(theta))
             evals.append((step + 1, sc))
             _log(f"eval at {step + 1}: {sc}")
             if sc[0] < best[0]:
@@ -406,10 +424,10 @@ class Validation:
         self.items.append(("d=1 1024^2", d))
         self.fcg = LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
 
-    def score(self, weights):
+    def score(self, weights, options=None):
         its = []
         for _, d in self.items:
-            d.set_learned_weights(weights)
+            d.set_learned_weights(weights, options=options or {"padding": "zero"})
             r = d.solve_pressure_rhs(self.fcg, None, download=False)
             its.append(r.iterations if r.status == 0 and not r.fallback_used else 20000)
         return float(np.mean(np.log(its))), its
@@ -468,10 +486,14 @@ def assemble_pprime(d, warm_iterations, alpha_u=0.7):
 class PairBank:
     """Harvested pairs on the device, one group per grid size."""
 
-    def __init__(self, torch, capacity):
+    def __init__(self, torch, capacity, interior=False):
         self.torch = torch
         self.capacity = dict(capacity)  # n -> (max pairs, max systems)
+        # interior: pairs for a network applied with the interior-equivalent
+        # scaling (Neumann padding), see to_interior_scaling
+        self.interior = interior
         self.r, self.e, self.ce, self.cn, self.den, self.sys = {}, {}, {}, {}, {}, {}
+        self.cd = {}
         self.count, self.systems = {}, {}
         for n, (pairs, systems) in self.capacity.items():
             dev = dict(device="cuda", dtype=torch.float32)
@@ -479,6 +501,7 @@ class PairBank:
             self.e[n] = torch.zeros((pairs, n, n), **dev)
             self.ce[n] = torch.zeros((systems, n, n), **dev)
             self.cn[n] = torch.zeros((systems, n, n), **dev)
+            self.cd[n] = torch.ones((systems, n, n), **dev) if interior else None
             self.den[n] = np.zeros(pairs, np.float32)
             self.sys[n] = np.zeros(pairs, np.int64)
             self.count[n] = 0
@@ -502,18 +525,60 @@ class PairBank:
             got += gn
         self.den[n][k0:k0 + got] = den
         self.sys[n][k0:k0 + got] = s
+        if self.interior:
+            self.to_interior_scaling(d, n, k0, got, s)
         self.count[n] += got
         self.systems[n] += 1
         return got, res
+
+    def to_interior_scaling(self, d, n, k0, got, s):
+        """Rewrites system s and its pairs for a network applied as z = S N(S r)
+        with S the interior-equivalent diagonal^-1/2 (the Neumann-padded
+        network) instead of D^-1/2: with F = (D / D~)^1/2 the network input is
+        F r^, its target e^ / F, and the loss operator F A^ F (diagonal F^2,
+        couplings F_i c_ij F_j); (e^, r^) is unchanged."""
+        torch = self.torch
+        f = torch.from_numpy(interior_factor(d).astype(np.float32)).cuda()
+        self.r[n][k0:k0 + got] *= f
+        self.e[n][k0:k0 + got] /= f
+        self.ce[n][s][:, :-1] *= f[:, :-1] * f[:, 1:]
+        self.cn[n][s][:-1, :] *= f[:-1, :] * f[1:, :]
+        self.cd[n][s] = f * f
 
     def total(self):
         return sum(self.count.values())
 
 
-def harvest_cases(torch, cases, capacity, seed=0, deadline=None, noise=0):
+def interior_factor(d):
+    """F = (D / D~)^1/2 on d's assembled 2D p' system (ny x nx): D the
+    diagonal, D~ the interior-equivalent diagonal of the library's Neumann-
+    padded preconditioner (sum of internal-face couplings times 4 / their
+    count; the pinned cell and cells without internal faces keep D)."""
+    nx, ny = d.shape[0], d.shape[1]
+    diag = d.download("PC_DIAG").reshape(ny, nx)
+    coeff = d.download("PC_FACE_COEFF")
+    fx = coeff[:(nx + 1) * ny].reshape(ny, nx + 1)
+    fy = coeff[(nx + 1) * ny:].reshape(ny + 1, nx)
+    s = np.zeros((ny, nx))
+    m = np.zeros((ny, nx))
+    s[:, 1:] += fx[:, 1:nx]
+    m[:, 1:] += 1
+    s[:, :-1] += fx[:, 1:nx]
+    m[:, :-1] += 1
+    s[1:, :] += fy[1:ny, :]
+    m[1:, :] += 1
+    s[:-1, :] += fy[1:ny, :]
+    m[:-1, :] += 1
+    dt = np.where((m > 0) & (s > 0), 4.0 * s / np.maximum(m, 1), diag)
+    if d.pin >= 0:
+        dt.ravel()[d.pin] = diag.ravel()[d.pin]
+    return np.sqrt(diag / dt)
+
+
+def harvest_cases(torch, cases, capacity, seed=0, deadline=None, noise=0, interior=False):
     """Runs SIMPLE cases and harvests p' pairs; cases: list of (kind, n, re, speed, warm);
     `noise` white-noise pairs per case besides the solve's own."""
-    bank = PairBank(torch, capacity)
+    bank = PairBank(torch, capacity, interior=interior)
     t0 = time.time()
     for i, (kind, n, re, speed, warm) in enumerate(cases):
         if deadline is not None and time.time() > deadline:
@@ -600,13 +665,13 @@ def standalone_system(n, seed=20480001):
     return d, b
 
 
-def evaluate(weights, n, tol=1e-8):
+def evaluate(weights, n, tol=1e-8, options=None):
     d, b = standalone_system(n)
     try:
         d.upload("PC_RHS", b)
         jac = LinearSolverSettings(0.0, tol, 400000, "CG", "Jacobi", check_interval=64)
         fcg = LinearSolverSettings(0.0, tol, 20000, "FCG", "Learned")
-        d.set_learned_weights(weights)
+        d.set_learned_weights(weights, options=options or {"padding": "zero"})
         rj = d.solve_pressure_rhs(jac, None, download=False)
         rf = d.solve_pressure_rhs(fcg, None, download=False)
         return dict(n=n, jacobi_iterations=rj.iterations, jacobi_ms=rj.device_ms,
@@ -648,7 +713,10 @@ def main(argv=None):
     ap.add_argument("--train-seconds", type=float, default=420)
     ap.add_argument("--init", default=None, help="start from these weights instead")
     ap.add_argument("--free", action="store_true",
-                    help="train every weight (default: the symmetric multilevel parametrisation)")
+                    help="train every weight (zero padding; default: a symmetric parametrisation)")
+    ap.add_argument("--param", choices=["vcycle", "multilevel"], default="vcycle",
+                    help="vcycle: the V(1,1) cycle with Neumann padding and the coarse tail; "
+                         "multilevel: the additive (BPX-like) family with zero padding")
     ap.add_argument("--noise", type=float, default=1e-3,
                     help="random weights in the unused channels of the initial network")
     ap.add_argument("--eval", type=int, nargs="*", default=[256, 1024])
@@ -668,16 +736,19 @@ def main(argv=None):
     cap = {256: (13000, 200), 512: (6500, 100), 1024: (1600, 30)}
     cases = default_cases(args.seed, args.cases)
     summary = dict(args=vars(args))
+    vcycle = args.param == "vcycle" and not args.free
+    param = VCycle if vcycle else SymmetricMultilevel
+    opts = VCycle.options() if vcycle else {"padding": "zero"}
     if args.free:
         w0 = (np.fromfile(args.init, dtype="<f4") if args.init
               else structured_init(args.seed, noise=args.noise))
     else:
-        theta0 = SymmetricMultilevel.initial()
-        w0 = SymmetricMultilevel.weights(theta0)
-    summary["eval_init"] = [evaluate(w0, n) for n in args.eval]
+        theta0 = param.initial()
+        w0 = param.weights(theta0)
+    summary["eval_init"] = [evaluate(w0, n, options=opts) for n in args.eval]
     _log("init:", summary["eval_init"])
     bank = harvest_cases(torch, cases, cap, args.seed, time.time() + args.harvest_seconds,
-                         noise=args.noise_pairs)
+                         noise=args.noise_pairs, interior=vcycle)
     summary["pairs"] = {str(n): int(c) for n, c in bank.count.items()}
     if args.free:
         w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
@@ -687,22 +758,27 @@ def main(argv=None):
         try:
             theta, losses, evals = train_symmetric(
                 torch, bank, theta0, args.steps, args.lr, args.seed,
-                time.time() + args.train_seconds, eval_fn=val.score,
-                eval_every=max(1, args.steps // args.checkpoints))
+                time.time() + args.train_seconds,
+                eval_fn=lambda wts: val.score(wts, opts),
+                eval_every=max(1, args.steps // args.checkpoints), param=param, options=opts)
         finally:
             val.close()
         summary["theta"] = theta.tolist()
         summary["evals"] = evals
-        w = SymmetricMultilevel.weights(theta)
+        summary["parametrisation"] = param.__name__
+        w = param.weights(theta)
+    summary["options"] = opts
     w.astype("<f4").tofile(args.out)
     summary["loss_first"] = float(np.mean(losses[:50])) if losses else None
     summary["loss_last"] = float(np.mean(losses[-50:])) if losses else None
     summary["steps"] = len(losses)
-    summary["eval_trained"] = [evaluate(w, n) for n in args.eval]
+    summary["eval_trained"] = [evaluate(w, n, options=opts) for n in args.eval]
     _log("trained:", summary["eval_trained"])
-    if args.summary:
-        with open(args.summary, "w") as f:
-            json.dump(summary, f, indent=1)
+    # the record beside the weights: how they are applied (load_learned_options)
+    # and how they were made
+    import os
+    with open(args.summary or os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
     print(json.dumps(summary))
 
 

@@ -351,17 +351,19 @@ int cfdhip_unet_backward(cfdhip_unet* net, const float* x, const float* gy, int6
 int64_t cfdhip_unet_tensor_size(cfdhip_unet* net, int tensor);
 int cfdhip_unet_tensor(cfdhip_unet* net, int tensor, float* out, int64_t count);
 /* A-norm loss L = mean_s (N(x_s) - e_s, A_s (N(x_s) - e_s)) / den_s with A_s the
- * unit-diagonal operator of couplings ce, cn + sys_offset[s] (device int64 per
- * sample) and its gradient in the master weights (grad: host, param_count
- * floats), no update: for training a parametrisation of the weights on the host. */
+ * operator of couplings ce, cn (to x + 1, y + 1) and diagonal cd (NULL: unit
+ * diagonal), all + sys_offset[s] (device int64 per sample), and its gradient
+ * in the master weights (grad: host, param_count floats), no update: for
+ * training a parametrisation of the weights on the host. */
 int cfdhip_unet_loss_grad(cfdhip_unet* net, const float* x, const float* e, const float* ce,
-                          const float* cn, const int64_t* sys_offset, const float* den,
-                          int64_t batch, int64_t h, int64_t w, double* loss, float* grad);
+                          const float* cn, const float* cd, const int64_t* sys_offset,
+                          const float* den, int64_t batch, int64_t h, int64_t w, double* loss,
+                          float* grad);
 /* The same loss; with update != 0 one Adam step (lr, betas 0.9 / 0.999). */
 int cfdhip_unet_train_step(cfdhip_unet* net, const float* x, const float* e, const float* ce,
-                           const float* cn, const int64_t* sys_offset, const float* den,
-                           int64_t batch, int64_t h, int64_t w, float lr, int update,
-                           double* loss);
+                           const float* cn, const float* cd, const int64_t* sys_offset,
+                           const float* den, int64_t batch, int64_t h, int64_t w, float lr,
+                           int update, double* loss);
 
 /* ---- batched p' solves ------------------------------------------------------------ */
 /* No counterpart in the reference, which solves one case at a time

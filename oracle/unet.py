@@ -390,12 +390,12 @@ def unflatten(v):
     return w
 
 
-def anorm_loss(y, e, ce, cn, den):
-    """Mean over samples of (d, A d) / den with d = y - e, A the unit-diagonal
-    operator of couplings ce (to x + 1) and cn (to y + 1); returns (L, dL/dy).
-    y, e: (N, H, W); ce, cn: (N, H, W); den: (N,)."""
+def anorm_loss(y, e, ce, cn, den, cd=None):
+    """Mean over samples of (d, A d) / den with d = y - e, A the operator of
+    diagonal cd (None: unit diagonal) and couplings ce (to x + 1) and cn (to
+    y + 1); returns (L, dL/dy). y, e, ce, cn, cd: (N, H, W); den: (N,)."""
     d = np.asarray(y, np.float64) - np.asarray(e, np.float64)
-    ad = d.copy()
+    ad = d.copy() if cd is None else np.asarray(cd, np.float64) * d
     ad[:, :, :-1] -= ce[:, :, :-1] * d[:, :, 1:]
     ad[:, :, 1:] -= ce[:, :, :-1] * d[:, :, :-1]
     ad[:, :-1, :] -= cn[:, :-1, :] * d[:, 1:, :]

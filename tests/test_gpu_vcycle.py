@@ -165,3 +165,39 @@ def test_vcycle_fcg_on_simple_systems():
             assert r.iterations * 5 < jac.iterations
         finally:
             d.close()
+
+
+def test_interior_scaled_pairs_are_consistent():
+    """Pairs rewritten for the interior-equivalent scaling keep their A-norm:
+    with y = 0 the loss of every pair is (e', A' e') / den = 1, with A' = F A^ F
+    and e' = e^ / F; the device loss with the diagonal matches the oracle."""
+    from cfdapp_amd import learned
+    bank = learned.PairBank(torch, {64: (8, 1)}, interior=True)
+    d = learned.make_case("channel", 64, 800.0, 1.0)
+    try:
+        learned.assemble_pprime(d, 1)
+        got, _ = bank.harvest(d, 64, 8, 6, noise=2, seed=3)
+    finally:
+        d.close()
+    assert got == 8
+    e = bank.e[64][:got].cpu().numpy().astype(np.float64)
+    ce = np.broadcast_to(bank.ce[64][0].cpu().numpy(), e.shape)
+    cn = np.broadcast_to(bank.cn[64][0].cpu().numpy(), e.shape)
+    cd = np.broadcast_to(bank.cd[64][0].cpu().numpy(), e.shape)
+    den = bank.den[64][:got].astype(np.float64)
+    for k in range(got):
+        lk, _ = unet.anorm_loss(np.zeros((1,) + e.shape[1:]), e[k:k + 1], ce[k:k + 1],
+                                cn[k:k + 1], den[k:k + 1], cd=cd[k:k + 1])
+        assert lk == pytest.approx(1.0, rel=2e-3), k
+    # device loss with the diagonal against the oracle, V-cycle network output
+    opts = VCycle.options()
+    w = VCycle.weights(VCycle.initial())
+    net = ca.UnetNet(w, options=opts)
+    off = torch.zeros(got, dtype=torch.int64, device="cuda")
+    x = bank.r[64][:got].contiguous()
+    loss, _ = net.loss_grad(x, bank.e[64][:got].contiguous(), bank.ce[64][0], bank.cn[64][0],
+                            off, _dev(den), got, 64, 64, cd=bank.cd[64][0])
+    yd = torch.empty_like(x)
+    net.forward(x, yd, got, 64, 64)
+    lref, _ = unet.anorm_loss(yd.cpu().numpy(), e, ce, cn, den, cd=cd)
+    assert loss == pytest.approx(lref, rel=1e-4)

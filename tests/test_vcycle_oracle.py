@@ -260,3 +260,37 @@ def test_vcycle_preconditioner_iterations_are_grid_independent():
     assert all(0 < k <= 20 for k in its.values()), its
     assert its[128] <= its[32] + 4, its
     assert jac > 10 * its[64], (jac, its)
+
+
+def test_anorm_loss_with_diagonal_matches_finite_differences():
+    rng = np.random.default_rng(6)
+    n, h, wd = 2, 6, 5
+    y = rng.standard_normal((n, h, wd))
+    e = rng.standard_normal((n, h, wd))
+    ce = rng.uniform(0.1, 0.3, (n, h, wd))
+    cn = rng.uniform(0.1, 0.3, (n, h, wd))
+    cd = rng.uniform(0.8, 1.5, (n, h, wd))
+    den = rng.uniform(1.0, 2.0, n)
+    L, g = U.anorm_loss(y, e, ce, cn, den, cd=cd)
+    # the value against the dense operator
+    tot = 0.0
+    for s in range(n):
+        A = np.diag(cd[s].ravel())
+        for yy in range(h):
+            for xx in range(wd):
+                c = yy * wd + xx
+                if xx + 1 < wd:
+                    A[c, c + 1] = A[c + 1, c] = -ce[s, yy, xx]
+                if yy + 1 < h:
+                    A[c, c + wd] = A[c + wd, c] = -cn[s, yy, xx]
+        d = (y[s] - e[s]).ravel()
+        tot += d @ A @ d / den[s]
+    assert L == pytest.approx(tot / n, rel=1e-12)
+    eps = 1e-6
+    for idx in [(0, 0, 0), (1, 3, 2), (0, 5, 4)]:
+        yp, ym = y.copy(), y.copy()
+        yp[idx] += eps
+        ym[idx] -= eps
+        fd = (U.anorm_loss(yp, e, ce, cn, den, cd=cd)[0] -
+              U.anorm_loss(ym, e, ce, cn, den, cd=cd)[0]) / (2 * eps)
+        assert g[idx] == pytest.approx(fd, rel=1e-6)
