@@ -443,22 +443,58 @@ def main():
     log(f"mixed Jacobi-PCG: {rm.status_name} {rm.iterations} it ({rm.restarts} cycles) "
         f"{dtm * 1e3:.1f} ms, true rel res {rm.true_residual / bnorm:.3e}")
 
-    # --- SIMPLE outer iterations at Re = 1000 --------------------------------------
+    # --- mixed-precision learned FCG (FP32 Krylov vectors inside FP64 refinement) -------
+    fcg_mx = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned", precision="mixed")
+    d.solve_pressure_rhs(fcg_mx, None, download=False)
+    d._ctx.call("cfdhip_synchronize")
+    t2 = time.perf_counter()
+    rfm = d.solve_pressure_rhs(fcg_mx, None, download=False)
+    dtfm = time.perf_counter() - t2
+    mixed_fcg = {"learned_fcg_mixed_s": dtfm, "learned_fcg_mixed_iterations": rfm.iterations,
+                 "learned_fcg_mixed_refinements": rfm.restarts,
+                 "learned_fcg_mixed_fallback": bool(rfm.fallback_used),
+                 "learned_fcg_mixed_true_relative_residual": rfm.true_residual / bnorm}
+    log(f"mixed learned FCG: {rfm.status_name} {rfm.iterations} it ({rfm.restarts} cycles) "
+        f"{dtfm * 1e3:.1f} ms, true rel res {rfm.true_residual / bnorm:.3e}")
+
+    # --- SIMPLE outer iterations at Re = 1000: Jacobi-PCG vs learned FCG p' solves ------
     simple = {}
     if args.simple_iters > 0:
-        ds = cfd.GpuSimpleDiscretization.cavity(2, N, N, viscosity=1e-3)
-        t1 = time.perf_counter()
-        res = ds.simple_run(args.simple_iters,
-                            pressure=cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi",
-                                                              check_interval=64))
-        dt = time.perf_counter() - t1
-        simple = {"simple_iter_per_s": res["iterations"] / dt,
-                  "simple_iterations": res["iterations"],
-                  "simple_status": res["status"],
-                  "simple_pressure_iterations": res["pressure_iterations"]}
-        log(f"SIMPLE: {res['iterations']} outer it in {dt:.2f} s, status {res['status']}, "
-            f"p' iterations {res['pressure_iterations']}")
-        ds.close()
+        runs = {}
+        for label, ps in (("jacobi", cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi",
+                                                              check_interval=64)),
+                          ("learned", cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG",
+                                                               "Learned"))):
+            ds = cfd.GpuSimpleDiscretization.cavity(2, N, N, viscosity=1e-3)
+            try:
+                if label == "learned":
+                    ds.set_learned_weights()
+                ds._ctx.call("cfdhip_synchronize")
+                t1 = time.perf_counter()
+                res = ds.simple_run(args.simple_iters, pressure=ps)
+                dt = time.perf_counter() - t1
+                runs[label] = (res, dt, ds.download("P"), ds.download_velocity())
+            finally:
+                ds.close()
+            log(f"SIMPLE ({label} p'): {res['iterations']} outer it in {dt:.2f} s, status "
+                f"{res['status']}, p' iterations {res['pressure_iterations']}")
+        (rj_, tj, pj, uj), (rl_, tl, pl, ul) = runs["jacobi"], runs["learned"]
+        umax = max(float(np.max(np.abs(c))) for c in uj)
+        simple = {"simple_iter_per_s": rj_["iterations"] / tj,
+                  "simple_iterations": rj_["iterations"],
+                  "simple_status": rj_["status"],
+                  "simple_pressure_iterations": rj_["pressure_iterations"],
+                  "simple_learned_iter_per_s": rl_["iterations"] / tl,
+                  "simple_learned_pressure_iterations": rl_["pressure_iterations"],
+                  "simple_learned_status": rl_["status"],
+                  "simple_speedup_learned_vs_jacobi": tj / tl,
+                  "simple_learned_vs_jacobi_p_rel_err_inf":
+                      float(np.max(np.abs(pl - pj)) / np.max(np.abs(pj))),
+                  "simple_learned_vs_jacobi_u_rel_err_inf":
+                      max(float(np.max(np.abs(a - b))) for a, b in zip(ul, uj)) / umax}
+        log(f"SIMPLE learned vs Jacobi p': {tj / tl:.1f}x iter/s; |dp|/|p| "
+            f"{simple['simple_learned_vs_jacobi_p_rel_err_inf']:.2e}, |du|/|u| "
+            f"{simple['simple_learned_vs_jacobi_u_rel_err_inf']:.2e}")
 
     c129 = {}
     if args.c129_iters > 0:
@@ -506,7 +542,7 @@ def main():
                    "jacobi_pcg_true_relative_residual": rj.true_residual / bnorm,
                    "speedup_learned_vs_jacobi": jac_s / fcg_s,
                    "plain_spmv": spmv,
-                   **mixed, **simple,
+                   **mixed, **mixed_fcg, **simple,
                    "cavity129_simple": c129,
                    "mixed_precision_3d": cube,
                    "batched_pprime": batched},
