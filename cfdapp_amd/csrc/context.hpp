@@ -35,6 +35,7 @@ struct KState {
 struct KrylovWork {
   DeviceArray<double> r, p, Ap, z, x, b, dinv, rhat, v, s, t, phat, shat;
   DeviceArray<float> r32, x32, dinv32, w32, dis32;
+  DeviceArray<float> scaleTmp[2];  // smoothed preconditioner scaling (learned FCG)
   DeviceArray<double> d8[8];  // single-kernel PCG buffers (FP64)
   DeviceArray<float> f8[8];   // single-kernel PCG buffers (FP32)
   DeviceArray<double> partial;

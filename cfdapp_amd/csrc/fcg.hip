@@ -123,15 +123,22 @@ __global__ void invSqrtKernel(int64_t n, const double* __restrict__ diag, float*
     dis[c] = static_cast<float>(1.0 / std::sqrt(diag[c]));
 }
 
-// For a Neumann-padded network: the inverse square root of the
-// interior-equivalent diagonal, the sum of a cell's internal-face couplings
-// times 4 / their count (what the cell's diagonal would be with all four
-// neighbours, so that a constant field scales like the interior); the pinned
-// cell, or a cell without internal faces, keeps D^-1/2. 2D face order:
-// x-faces j (nx + 1) + i, y-faces offY + j nx + i.
-__global__ void interiorScaleKernel(int64_t nx, int64_t ny, int64_t offY, int64_t pin,
-                                    const double* __restrict__ diag,
-                                    const double* __restrict__ coeff, float* __restrict__ dis) {
+// Scaling of a Neumann-padded network, z = S N(S r). S^-2 is a smoothed
+// interior-equivalent diagonal: per cell the sum of its internal-face
+// couplings times 4 / their count (the diagonal the cell would have with all
+// four neighbours, so that a constant field scales like the interior),
+// geometric-mean filtered over a (2 kScaleRadius + 1)^2 window (indices
+// clamped to the grid). A cell-by-cell scaling would map the operator's
+// smooth near-null modes to rough ones wherever the coefficients are rough
+// (a one-cell wall layer, noisy response coefficients), which the network's
+// fixed stencils cannot follow; the filtered scaling keeps them smooth. The
+// pinned cell keeps D^-1/2 (z = r / a_P there). 2D face order: x-faces
+// j (nx + 1) + i, y-faces offY + j nx + i.
+constexpr int kScaleRadius = 4;
+
+__global__ void logInteriorDiagKernel(int64_t nx, int64_t ny, int64_t offY,
+                                      const double* __restrict__ diag,
+                                      const double* __restrict__ coeff, float* __restrict__ out) {
   const int64_t n = nx * ny;
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -142,8 +149,30 @@ __global__ void interiorScaleKernel(int64_t nx, int64_t ny, int64_t offY, int64_
     if (i < nx - 1) { s += coeff[j * (nx + 1) + i + 1]; ++m; }
     if (j > 0) { s += coeff[offY + j * nx + i]; ++m; }
     if (j < ny - 1) { s += coeff[offY + (j + 1) * nx + i]; ++m; }
-    const double dt = (c == pin || m == 0 || !(s > 0.0)) ? diag[c] : s * 4.0 / m;
-    dis[c] = static_cast<float>(1.0 / std::sqrt(dt));
+    const double dt = (m == 0 || !(s > 0.0)) ? diag[c] : s * 4.0 / m;
+    out[c] = static_cast<float>(std::log(dt));
+  }
+}
+
+// Window mean along x (dir 0) or y (dir 1), indices clamped to the grid; the
+// y pass also forms S = exp(-mean / 2) (final != 0), D^-1/2 at the pin.
+__global__ void boxMeanKernel(int64_t nx, int64_t ny, int dir, const float* __restrict__ in,
+                              float* __restrict__ out, int final, int64_t pin,
+                              const double* __restrict__ diag) {
+  const int64_t n = nx * ny;
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % nx, j = c / nx;
+    float s = 0.f;
+    for (int k = -kScaleRadius; k <= kScaleRadius; ++k) {
+      int64_t ii = i, jj = j;
+      if (dir == 0) ii = i + k < 0 ? 0 : (i + k >= nx ? nx - 1 : i + k);
+      else jj = j + k < 0 ? 0 : (j + k >= ny ? ny - 1 : j + k);
+      s += in[jj * nx + ii];
+    }
+    const float mean = s / static_cast<float>(2 * kScaleRadius + 1);
+    if (!final) out[c] = mean;
+    else out[c] = c == pin ? static_cast<float>(1.0 / std::sqrt(diag[c])) : std::exp(-0.5f * mean);
   }
 }
 
@@ -327,11 +356,19 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
   const OpT<double> op64{v.nx, v.ny, ctx.pin, v.offY, ctx.f(CFDHIP_FIELD_PC_DIAG),
                          ctx.f(CFDHIP_FIELD_PC_FACE_COEFF)};
   const dim3 g = stride(n);
-  if (ctx.net->padding() == 1)
-    hipLaunchKernelGGL(interiorScaleKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, v.offY,
-                       ctx.pin, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
-                       w.dis32.get());
-  else
+  if (ctx.net->padding() == 1) {
+    w.scaleTmp[0].ensure(n);
+    w.scaleTmp[1].ensure(n);
+    hipLaunchKernelGGL(logInteriorDiagKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, v.offY,
+                       ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
+                       w.scaleTmp[0].get());
+    hipLaunchKernelGGL(boxMeanKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, 0,
+                       w.scaleTmp[0].get(), w.scaleTmp[1].get(), 0, ctx.pin,
+                       ctx.f(CFDHIP_FIELD_PC_DIAG));
+    hipLaunchKernelGGL(boxMeanKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, 1,
+                       w.scaleTmp[1].get(), w.dis32.get(), 1, ctx.pin,
+                       ctx.f(CFDHIP_FIELD_PC_DIAG));
+  } else
     hipLaunchKernelGGL(invSqrtKernel, g, dim3(256), 0, ctx.stream, n, ctx.f(CFDHIP_FIELD_PC_DIAG),
                        w.dis32.get());
   w.partial.ensure(kMaxParts * 4 + 8);

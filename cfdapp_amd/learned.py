@@ -549,11 +549,12 @@ class PairBank:
         return sum(self.count.values())
 
 
-def interior_factor(d):
+def interior_factor(d, radius=4):
     """F = (D / D~)^1/2 on d's assembled 2D p' system (ny x nx): D the
-    diagonal, D~ the interior-equivalent diagonal of the library's Neumann-
-    padded preconditioner (sum of internal-face couplings times 4 / their
-    count; the pinned cell and cells without internal faces keep D)."""
+    diagonal, D~ = S^-2 the library's scaling of a Neumann-padded network:
+    the interior-equivalent diagonal (sum of internal-face couplings times 4
+    / their count; D where a cell has none) geometric-mean filtered over a
+    (2 radius + 1)^2 window with clamped indices; the pinned cell keeps D."""
     nx, ny = d.shape[0], d.shape[1]
     diag = d.download("PC_DIAG").reshape(ny, nx)
     coeff = d.download("PC_FACE_COEFF")
@@ -570,6 +571,13 @@ def interior_factor(d):
     s[:-1, :] += fy[1:ny, :]
     m[:-1, :] += 1
     dt = np.where((m > 0) & (s > 0), 4.0 * s / np.maximum(m, 1), diag)
+    lg = np.log(dt)
+    for axis in (1, 0):  # x then y, as the device does
+        k = np.arange(-radius, radius + 1)
+        size = lg.shape[axis]
+        idx = np.clip(np.arange(size)[:, None] + k[None, :], 0, size - 1)
+        lg = (np.take(lg, idx, axis=axis)).mean(axis=axis + 1)
+    dt = np.exp(lg)
     if d.pin >= 0:
         dt.ravel()[d.pin] = diag.ravel()[d.pin]
     return np.sqrt(diag / dt)

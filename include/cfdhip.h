@@ -284,9 +284,10 @@ int cfdhip_set_learned_weights(cfdhip_ctx* ctx, const float* weights, int64_t co
 /* How the network is applied (no counterpart in the reference).
  * padding 0: every convolution zero-padded, z = D^-1/2 N(D^-1/2 r).
  * padding 1: Neumann (replicate) padding, the padding that matches the p'
- *   operator's zero-gradient walls, and z = S N(S r) with S the inverse square
- *   root of the interior-equivalent diagonal (the sum of a cell's internal-face
- *   couplings times 2 dim / their count; the pinned cell keeps its diagonal).
+ *   operator's zero-gradient walls, and z = S N(S r) with S^-2 a smoothed
+ *   interior-equivalent diagonal (the sum of a cell's internal-face couplings
+ *   times 2 dim / their count, geometric-mean filtered over a 9 x 9 window;
+ *   the pinned cell keeps its diagonal).
  *   The backward pass then needs kernels mirror-symmetric in both axes.
  * tail_levels > 0 (padding 1 only): a one-channel symmetric V(1,1) multigrid
  *   cycle below the bottom level (H/16 .. the coarsest grid with even sides,
