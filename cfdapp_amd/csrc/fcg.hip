@@ -357,6 +357,12 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
                          ctx.f(CFDHIP_FIELD_PC_FACE_COEFF)};
   const dim3 g = stride(n);
   if (ctx.net->padding() == 1) {
+    // odd reflection on the sides with a FixedValue pressure patch (2D patch
+    // order left, right, bottom, top)
+    int sides = 0;
+    for (int p = 0; p < 4 && p < v.npatch; ++p)
+      if (v.patch[p].pkind == CFDHIP_FIXED_VALUE) sides |= 1 << p;
+    ctx.net->setSides(sides);
     w.scaleTmp[0].ensure(n);
     w.scaleTmp[1].ensure(n);
     hipLaunchKernelGGL(logInteriorDiagKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, v.offY,

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ sr
                                                        int srcW, int64_t srcBatch,
                                                        const float* __restrict__ w,
                                                        float* __restrict__ out, int n, int h,
-                                                       int wd, int neumann) {
+                                                       int wd, int pad) {
   __shared__ float ws[144];
   if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   __syncthreads();
@@ -91,11 +91,11 @@ __global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ sr
     for (int tap = 0; tap < 9; ++tap) {
       int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
       v[tap] = 0.f;
-      if (neumann) padPixel(yy, xx, h, wd, 1);
-      if (yy >= 0 && yy < srcH && xx >= 0 && xx < srcW) {
+      const float sg = padSign(yy, xx, h, wd, pad);
+      if (sg != 0.f && yy < srcH && xx < srcW) {
         const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
         const float sv = static_cast<float>(src[si]);
-        v[tap] = scale != nullptr ? sv * scale[si] : sv;
+        v[tap] = sg * (scale != nullptr ? sv * scale[si] : sv);
       }
     }
     fx4* o = reinterpret_cast<fx4*>(out + p * 16);
@@ -123,7 +123,7 @@ struct LastTileLds {
 };
 
 __device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, int b, int y0, int x0,
-                                int h, int wd, int neumann, float out[2]) {
+                                int h, int wd, int pad, float out[2]) {
   constexpr int PER = (kLNP + 255) / 256;
   fx4 v[PER][4];
 #pragma unroll
@@ -134,11 +134,12 @@ __device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, in
     if (px < kLNP) {
       const int ty = px / kLC, tx = px - ty * kLC;
       int yy = y0 - 1 + ty, xx = x0 - 1 + tx;
-      if (padPixel(yy, xx, h, wd, neumann)) {
+      const float sg = padSign(yy, xx, h, wd, pad);
+      if (sg != 0.f) {
         const fx4* src = reinterpret_cast<const fx4*>(
             d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
 #pragma unroll
-        for (int c4 = 0; c4 < 4; ++c4) v[k][c4] = src[c4];
+        for (int c4 = 0; c4 < 4; ++c4) v[k][c4] = sg * src[c4];
       }
     }
   }
@@ -180,7 +181,7 @@ __device__ inline void lastTileCoords(int64_t t, int tilesX, int tilesY, int& b,
 __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ d0,
                                                       const float* __restrict__ w,
                                                       float* __restrict__ y, int n, int h, int wd,
-                                                      int neumann) {
+                                                      int pad) {
   __shared__ LastTileLds L;
   if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   const int tilesX = (wd + kLTW - 1) / kLTW, tilesY = (h + kLTH - 1) / kLTH;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ 
     lastTileCoords(t, tilesX, tilesY, b, y0, x0);
     __syncthreads();
     float out[2];
-    lastTile(d0, L, b, y0, x0, h, wd, neumann, out);
+    lastTile(d0, L, b, y0, x0, h, wd, pad, out);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int o = threadIdx.x + j * 256;
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
                                                            const TZ* __restrict__ q,
                                                            TZ* __restrict__ z, int ny, int nx,
                                                            int64_t pin, int h, int wd,
-                                                           int neumann, double* partial) {
+                                                           int pad, double* partial) {
   __shared__ LastTileLds L;
   if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   const int tilesX = (nx + kLTW - 1) / kLTW, tilesY = (ny + kLTH - 1) / kLTH;
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
     lastTileCoords(t, tilesX, tilesY, b, y0, x0);
     __syncthreads();
     float out[2];
-    lastTile(d0, L, 0, y0, x0, h, wd, neumann, out);
+    lastTile(d0, L, 0, y0, x0, h, wd, pad, out);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int o = threadIdx.x + j * 256;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
 __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict__ gy,
                                                          const float* __restrict__ w,
                                                          float* __restrict__ gd0, int n, int h,
-                                                         int wd, int neumann) {
+                                                         int wd, int pad) {
   __shared__ float ws[144];
   if (threadIdx.x < 144) ws[threadIdx.x] = roundBf(w[threadIdx.x]);
   __syncthreads();
@@ -258,8 +259,8 @@ __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
       int yy = y - tap / 3 + 1, xx = x - tap % 3 + 1;
-      g[tap] = padPixel(yy, xx, h, wd, neumann) ? gy[(static_cast<int64_t>(b) * h + yy) * wd + xx]
-                                                : 0.f;
+      const float sg = padSign(yy, xx, h, wd, pad);
+      g[tap] = sg != 0.f ? sg * gy[(static_cast<int64_t>(b) * h + yy) * wd + xx] : 0.f;
     }
     fx4* o = reinterpret_cast<fx4*>(gd0 + p * 16);
 #pragma unroll
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(256) void lastConvBwdKernel(const float* __restrict
 // x / gy are FP32 scalar images, g16 / a16 FP32 16-channel images.
 __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ s1,
                                                       const float* __restrict__ m16, int mode,
-                                                      int n, int h, int wd, int neumann,
+                                                      int n, int h, int wd, int pad,
                                                       float* partial) {
   float acc[144];
 #pragma unroll
@@ -300,8 +301,9 @@ __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ 
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-        if (!padPixel(yy, xx, h, wd, neumann)) continue;
-        const float xv = roundBf(s1[(static_cast<int64_t>(b) * h + yy) * wd + xx]);
+        const float sg = padSign(yy, xx, h, wd, pad);
+        if (sg == 0.f) continue;
+        const float xv = sg * roundBf(s1[(static_cast<int64_t>(b) * h + yy) * wd + xx]);
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += xv * gv[c];
       }
@@ -310,10 +312,11 @@ __global__ __launch_bounds__(256) void singleDwKernel(const float* __restrict__ 
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-        if (!padPixel(yy, xx, h, wd, neumann)) continue;
+        const float sg = padSign(yy, xx, h, wd, pad);
+        if (sg == 0.f) continue;
         const float* ap = m16 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += roundBf(ap[c]) * g;
+        for (int c = 0; c < 16; ++c) acc[tap * 16 + c] += sg * roundBf(ap[c]) * g;
       }
     }
   }
@@ -568,7 +571,7 @@ void Unet::conv(int layer, bool transposed, int gather, int epi, int split, cons
   args.w_ = w;
   args.tilesX = (w + 31) / 32;
   args.tilesY = (h + 7) / 8;
-  args.neumann = neumann;
+  args.pad = padBits();
   const int ci = transposed ? kLayers[layer].cout : kLayers[layer].cin;
   const int co = transposed ? kLayers[layer].cin : kLayers[layer].cout;
   args.w = transposed ? static_cast<const void*>(bwdPack.get() + bwdOff[layer])
@@ -603,7 +606,7 @@ void Unet::convDw(int layer, int gx, int gg, const float* xa, const float* xb, c
   a.tilesX = (w + 31) / 32;
   a.tilesY = (h + 7) / 8;
   a.tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * n;
-  a.neumann = neumann;
+  a.pad = padBits();
   const int ci = kLayers[layer].cin, co = kLayers[layer].cout;
   const int npair = 9 * (ci / 16);
   const int groups = (npair + 7) / 8;
@@ -639,11 +642,11 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
     if (srcDouble)
       hipLaunchKernelGGL((firstConvKernel<double>), grid1(p0), dim3(256), 0, stream,
                          static_cast<const double*>(src), scale, srcH, srcW, srcBatch,
-                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w, neumann);
+                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w, padBits());
     else
       hipLaunchKernelGGL((firstConvKernel<float>), grid1(p0), dim3(256), 0, stream,
                          static_cast<const float*>(src), scale, srcH, srcW, srcBatch,
-                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w, neumann);
+                         master.get() + layerOffset(kE0a), T(kTe0a), n, h, w, padBits());
     CFDHIP_CHECK(hipGetLastError());
     conv(kE0b, false, kGatherPlain, kEpiStorePool, 0, T(kTe0a), nullptr, T(kTe0), T(kTp0), n, h,
          w);
@@ -670,7 +673,7 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
 void Unet::outputTrain(int n, int h, int w) {
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   hipLaunchKernelGGL(lastConvKernel, grid1(p0 / 2, 4096), dim3(256), 0, stream, T(kTd0),
-                     master.get() + layerOffset(kOut), y.get(), n, h, w, neumann);
+                     master.get() + layerOffset(kOut), y.get(), n, h, w, padBits());
   CFDHIP_CHECK(hipGetLastError());
 }
 
@@ -680,7 +683,7 @@ void Unet::outputApply(const float* scale, const TZ* r, const TZ* q, TZ* z, int 
   mark(kOut, true);
   hipLaunchKernelGGL((lastConvApplyKernel<TZ>), dim3(static_cast<unsigned>(parts)), dim3(256), 0,
                      stream, T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin,
-                     h, w, neumann, partial);
+                     h, w, padBits(), partial);
   mark(kOut, false);
   CFDHIP_CHECK(hipGetLastError());
   collectTiming();
@@ -691,7 +694,7 @@ template void Unet::outputApply<double>(const float*, const double*, const doubl
                                         int, int64_t, int, int, double*, int);
 
 void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
-  require(!neumann || mirrorSymmetric,
+  require(padBits() == 0 || mirrorSymmetric,
           "Neumann padding: the backward pass needs kernels mirror-symmetric in both axes "
           "(W[ky][kx] == W[2-ky][kx] == W[ky][2-kx]); load such weights with "
           "cfdhip_unet_set_weights");
@@ -699,7 +702,7 @@ void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   // output layer
   hipLaunchKernelGGL(lastConvBwdKernel, grid1(p0), dim3(256), 0, stream, gyIn,
-                     master.get() + layerOffset(kOut), T(kTgd0), n, h, w, neumann);
+                     master.get() + layerOffset(kOut), T(kTgd0), n, h, w, padBits());
   singleDw(1, gyIn, T(kTd0), n, h, w, grad.get() + layerOffset(kOut));
   // decoder, level 0
   conv(kD0b, true, kGatherPlain, kEpiStore, 0, T(kTgd0), nullptr, T(kTgd0a), nullptr, n, h, w);
@@ -742,7 +745,7 @@ void Unet::singleDw(int mode, const float* s1, const float* m16, int n, int h, i
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   const dim3 g = grid1(p0, 1024);
   dwPartial.ensure(static_cast<int64_t>(g.x) * 144);
-  hipLaunchKernelGGL(singleDwKernel, g, dim3(256), 0, stream, s1, m16, mode, n, h, w, neumann,
+  hipLaunchKernelGGL(singleDwKernel, g, dim3(256), 0, stream, s1, m16, mode, n, h, w, padBits(),
                      dwPartial.get());
   hipLaunchKernelGGL(singleDwReduceKernel, dim3(1), dim3(256), 0, stream, dwPartial.get(),
                      static_cast<int>(g.x), out);

@@ -92,9 +92,11 @@ struct ConvArgs {
   const float* scale;
   const float* w0;
   int srcH, srcW, srcDouble;
-  // 1: Neumann (replicate) padding -- a tap outside the image reads the
-  // nearest pixel inside it; 0: zero padding
-  int neumann;
+  // padding (padSign): 0 zero; bit 0 Neumann (replicate) padding, a tap
+  // outside the image reads the nearest pixel inside it; bits 1-4 (with bit
+  // 0) Dirichlet sides (left, right, bottom, top) where it reads the negated
+  // nearest pixel (odd reflection: zero at the face)
+  int pad;
 };
 
 struct DwArgs {
@@ -106,22 +108,32 @@ struct DwArgs {
   int n, h, w_;
   int tilesX, tilesY;
   int64_t tiles;
-  int neumann;     // padding of the staged input tile (as ConvArgs::neumann)
+  int pad;         // padding of the staged input tile (as ConvArgs::pad)
 };
 
-// Pixel (y, x) of a staged window. Zero padding: false outside the h x w
-// image (the value is 0). Neumann padding: moves (y, x) to the nearest pixel
-// inside and returns true. A conv with Neumann padding and a kernel that is
-// mirror-symmetric in both axes (W[ky][kx] == W[2-ky][kx] == W[ky][2-kx] per
-// channel pair) has as its adjoint the Neumann-padded conv with the
-// transposed kernel, so the backward pass keeps the same staging; the host
-// holds training to such kernels in that mode.
-__device__ inline bool padPixel(int& y, int& x, int h, int w, int neumann) {
-  if (y >= 0 && y < h && x >= 0 && x < w) return true;
-  if (!neumann) return false;
-  y = y < 0 ? 0 : (y >= h ? h - 1 : y);
-  x = x < 0 ? 0 : (x >= w ? w - 1 : x);
-  return true;
+// Pad bits (ConvArgs::pad): Neumann padding, and the Dirichlet sides.
+enum PadBits : int {
+  kPadNeumann = 1, kPadLeft = 2, kPadRight = 4, kPadBottom = 8, kPadTop = 16
+};
+
+// Pixel (y, x) of a staged window and the factor its value takes. Inside
+// the h x w image: 1. Zero padding: 0 outside. Neumann padding: (y, x) moves
+// to the nearest pixel inside, the factor is -1 per crossed Dirichlet side
+// (y < 0 bottom, y >= h top, x < 0 left, x >= w right), else 1. A conv with
+// this padding and a kernel that is mirror-symmetric in both axes
+// (W[ky][kx] == W[2-ky][kx] == W[ky][2-kx] per channel pair) has as its
+// adjoint the conv with the transposed kernel and the same padding, so the
+// backward pass keeps the same staging; the host holds training to such
+// kernels when the padding is not zero.
+__device__ inline float padSign(int& y, int& x, int h, int w, int pad) {
+  if (y >= 0 && y < h && x >= 0 && x < w) return 1.f;
+  if (!(pad & kPadNeumann)) return 0.f;
+  float s = 1.f;
+  if (y < 0) { y = 0; if (pad & kPadBottom) s = -s; }
+  else if (y >= h) { y = h - 1; if (pad & kPadTop) s = -s; }
+  if (x < 0) { x = 0; if (pad & kPadLeft) s = -s; }
+  else if (x >= w) { x = w - 1; if (pad & kPadRight) s = -s; }
+  return s;
 }
 
 // The coarse tail below the network's deepest level: a symmetric V(1,1)

@@ -81,7 +81,7 @@ __device__ inline fx4 gather4(const float* __restrict__ A, const float* __restri
 // one memory round trip, not one per item.
 template <int C, int CA, int G, int CC, int ROWS, int COLS>
 __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, const float* B,
-                                  int n, int h, int w, int y0, int x0, int c0, int neumann) {
+                                  int n, int h, int w, int y0, int x0, int c0, int pad) {
   constexpr int C4 = CC / 4;
   constexpr int ITEMS = ROWS * COLS * C4;
   constexpr int PER = (ITEMS + 255) / 256;
@@ -95,7 +95,8 @@ __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, co
       const int px = it / C4;
       const int ty = px / COLS, tx = px - ty * COLS;
       int y = y0 + ty, x = x0 + tx;
-      if (padPixel(y, x, h, w, neumann)) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+      const float sg = padSign(y, x, h, w, pad);
+      if (sg != 0.f) v[k] = sg * gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
     }
   }
 #pragma unroll
@@ -133,7 +134,7 @@ struct ChunkStage {
   static constexpr int PER = (ITEMS + 255) / 256;
   fx4 v[PER];
   __device__ inline void load(const float* A, const float* B, int n, int h, int w, int y0, int x0,
-                              int c0, int neumann) {
+                              int c0, int pad) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int it = threadIdx.x + k * 256;
@@ -143,7 +144,8 @@ struct ChunkStage {
         const int px = it / C4;
         const int ty = px / COLS, tx = px - ty * COLS;
         int y = y0 + ty, x = x0 + tx;
-        if (padPixel(y, x, h, w, neumann)) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+        const float sg = padSign(y, x, h, w, pad);
+      if (sg != 0.f) v[k] = sg * gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
       }
     }
   }
@@ -204,12 +206,12 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
     const int ty = i / XC, tx = i - ty * XC;
     int y = y0 - 1 + ty, x = x0 - 1 + tx;
     float v = 0.f;
-    if (a.neumann) padPixel(y, x, a.h, a.w_, 1);
-    if (y >= 0 && y < a.srcH && x >= 0 && x < a.srcW) {
+    const float sg = padSign(y, x, a.h, a.w_, a.pad);
+    if (sg != 0.f && y < a.srcH && x < a.srcW) {
       const int64_t si = static_cast<int64_t>(n) * a.srcH * a.srcW +
                          static_cast<int64_t>(y) * a.srcW + x;
       const float sv = static_cast<float>(src[si]);
-      v = a.scale != nullptr ? sv * a.scale[si] : sv;
+      v = sg * (a.scale != nullptr ? sv * a.scale[si] : sv);
     }
     xs[i] = v;
   }
@@ -225,10 +227,11 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
   for (int it = threadIdx.x; it < kInPix * 4; it += blockDim.x) {
     const int px = it >> 2;
     // a halo pixel outside the image takes e0a of the nearest pixel inside
-    // (Neumann padding), whose taps lie inside the staged window
+    // (times the padding's sign), whose taps lie inside the staged window
     int y = y0 + px / kInCols, x = x0 + px % kInCols;
     fx4 acc = {0.f, 0.f, 0.f, 0.f};
-    if (padPixel(y, x, a.h, a.w_, a.neumann)) {
+    const float sg = padSign(y, x, a.h, a.w_, a.pad);
+    if (sg != 0.f) {
       const int ty = y - y0, tx = x - x0;
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
@@ -236,6 +239,7 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[k] += v * wr[tap * 4 + k];
       }
+      acc = acc * sg;
     }
     splitStore(acc, hi, lo, px * cs + c4 * 4);
   }
@@ -298,11 +302,11 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
     if constexpr (kConcat) {
       if (ch < NCO)
         coarseBuf.load(a.a, nullptr, tn, a.h >> 1, a.w_ >> 1, (ty0 >> 1) - 1, (tx0 >> 1) - 1,
-                       ch * CC, a.neumann);
+                       ch * CC, a.pad);
       else
-        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC - CA, a.neumann);
+        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC - CA, a.pad);
     } else if constexpr (G != kGatherFirst) {
-      stageBuf.load(a.a, a.b, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC, a.neumann);
+      stageBuf.load(a.a, a.b, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC, a.pad);
     }
   };
   loadChunk(0, n, y0, x0);
@@ -499,13 +503,13 @@ __global__ __launch_bounds__(256) void dwKernel(DwArgs a) {
     __syncthreads();
     if constexpr (CIN <= 64) {
       stageSplit<CIN, CA, GX, CIN, kInRows, kInCols>(xs, nullptr, CS, a.xa, a.xb, n, a.h, a.w_,
-                                                     y0 - 1, x0 - 1, 0, a.neumann);
+                                                     y0 - 1, x0 - 1, 0, a.pad);
     } else {  // two halves keep the register batch small
       stageSplit<CIN, CA, GX, CIN / 2, kInRows, kInCols>(xs, nullptr, CS, a.xa, a.xb, n, a.h,
-                                                         a.w_, y0 - 1, x0 - 1, 0, a.neumann);
+                                                         a.w_, y0 - 1, x0 - 1, 0, a.pad);
       stageSplit<CIN, CA, GX, CIN / 2, kInRows, kInCols>(xs + CIN / 2, nullptr, CS, a.xa, a.xb, n,
                                                          a.h, a.w_, y0 - 1, x0 - 1, CIN / 2,
-                                                         a.neumann);
+                                                         a.pad);
     }
     // the output-gradient tile has no halo: zero beyond the image
     stageSplit<COUT, COUT, GG, COUT, kTH, kTW>(gs, nullptr, GS, a.ga, a.gb, n, a.h, a.w_, y0, x0,

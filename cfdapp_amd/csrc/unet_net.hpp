@@ -23,10 +23,15 @@ class Unet {
   explicit Unet(hipStream_t s);
 
   void setWeights(const float* host);
-  // Padding of every convolution (0 zero, 1 Neumann) and the coarse tail
-  // below the bottom level (levels 0: none; needs Neumann padding).
-  void setOptions(int neumannPadding, const TailParams& t);
+  // Padding of every convolution (0 zero, 1 Neumann), the Dirichlet sides
+  // of Neumann padding (bits: 1 left, 2 right, 4 bottom, 8 top; odd
+  // reflection there) and the coarse tail below the bottom level (levels 0:
+  // none; needs Neumann padding).
+  void setOptions(int neumannPadding, int dirichletSides, const TailParams& t);
+  void setSides(int dirichletSides);
   int padding() const { return neumann; }
+  // ConvArgs::pad of the current options
+  int padBits() const { return neumann ? (kPadNeumann | (sides << 1)) : 0; }
   void getWeights(float* host);
   void getGrads(float* host) {
     CFDHIP_CHECK(hipMemcpyAsync(host, grad.get(), kUnetParams * sizeof(float),
@@ -89,6 +94,7 @@ class Unet {
   // channel tail.dst from tail.src; backward is its adjoint on the gradient.
   void tailApply(float* tensor, int n, int h3, int w3, bool backward);
   int neumann = 0;
+  int sides = 0;
   TailParams tail{};
   // every kernel of the loaded weights is mirror-symmetric in both axes (the Neumann-padded
   // backward pass is the exact adjoint only for such kernels)

@@ -58,42 +58,45 @@ TailShape tailShape(int h3, int w3, int maxLevels) {
   return s;
 }
 
-// Symmetric 3 x 3 stencil (centre, edge, corner) at (y, x), Neumann padding.
+// Pixel (y, x) of an h x w image with the network's padding (padSign).
+__device__ inline float at(const float* img, int h, int w, int y, int x, int pad) {
+  const float s = padSign(y, x, h, w, pad);
+  return s * img[static_cast<int64_t>(y) * w + x];
+}
+
+// Symmetric 3 x 3 stencil (centre, edge, corner) at (y, x).
 __device__ inline float stencil3(const float* img, int h, int w, int y, int x, float c, float e,
-                                 float k) {
-  const int ym = y > 0 ? y - 1 : 0, yp = y < h - 1 ? y + 1 : h - 1;
-  const int xm = x > 0 ? x - 1 : 0, xp = x < w - 1 ? x + 1 : w - 1;
-  const float* r0 = img + static_cast<int64_t>(ym) * w;
-  const float* r1 = img + static_cast<int64_t>(y) * w;
-  const float* r2 = img + static_cast<int64_t>(yp) * w;
-  const float edges = (r0[x] + r2[x]) + (r1[xm] + r1[xp]);
-  const float corners = (r0[xm] + r0[xp]) + (r2[xm] + r2[xp]);
-  return c * r1[x] + e * edges + k * corners;
+                                 float k, int pad) {
+  const float edges = (at(img, h, w, y - 1, x, pad) + at(img, h, w, y + 1, x, pad)) +
+                      (at(img, h, w, y, x - 1, pad) + at(img, h, w, y, x + 1, pad));
+  const float corners = (at(img, h, w, y - 1, x - 1, pad) + at(img, h, w, y - 1, x + 1, pad)) +
+                        (at(img, h, w, y + 1, x - 1, pad) + at(img, h, w, y + 1, x + 1, pad));
+  return c * img[static_cast<int64_t>(y) * w + x] + e * edges + k * corners;
 }
 
 // 4 pool(K v) at coarse pixel (y, x) of v (fine h x w).
-__device__ inline float restrictK(const float* v, int h, int w, int y, int x, const TailParams& p) {
+__device__ inline float restrictK(const float* v, int h, int w, int y, int x, const TailParams& p,
+                                  int pad) {
   float s = 0.f;
 #pragma unroll
   for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
     for (int dx = 0; dx < 2; ++dx)
-      s += stencil3(v, h, w, 2 * y + dy, 2 * x + dx, p.k[0], p.k[1], p.k[2]);
+      s += stencil3(v, h, w, 2 * y + dy, 2 * x + dx, p.k[0], p.k[1], p.k[2], pad);
   return s;  // 4 x the 2 x 2 average
 }
 
 // K up(e) at fine pixel (y, x) of the h x w grid (e on h/2 x w/2).
-__device__ inline float prolongK(const float* e, int h, int w, int y, int x, const TailParams& p) {
-  const int hc = h >> 1, wc = w >> 1;
-  auto at = [&](int yy, int xx) {
-    yy = yy < 0 ? 0 : (yy >= h ? h - 1 : yy);
-    xx = xx < 0 ? 0 : (xx >= w ? w - 1 : xx);
-    return e[static_cast<int64_t>(yy >> 1) * wc + (xx >> 1)];
+__device__ inline float prolongK(const float* e, int h, int w, int y, int x, const TailParams& p,
+                                 int pad) {
+  const int wc = w >> 1;
+  auto up = [&](int yy, int xx) {  // the upsampled image with the padding
+    const float s = padSign(yy, xx, h, w, pad);
+    return s * e[static_cast<int64_t>(yy >> 1) * wc + (xx >> 1)];
   };
-  (void)hc;
-  const float edges = (at(y - 1, x) + at(y + 1, x)) + (at(y, x - 1) + at(y, x + 1));
-  const float corners = (at(y - 1, x - 1) + at(y - 1, x + 1)) + (at(y + 1, x - 1) + at(y + 1, x + 1));
-  return p.k[0] * at(y, x) + p.k[1] * edges + p.k[2] * corners;
+  const float edges = (up(y - 1, x) + up(y + 1, x)) + (up(y, x - 1) + up(y, x + 1));
+  const float corners = (up(y - 1, x - 1) + up(y - 1, x + 1)) + (up(y + 1, x - 1) + up(y + 1, x + 1));
+  return p.k[0] * up(y, x) + p.k[1] * edges + p.k[2] * corners;
 }
 
 __global__ __launch_bounds__(256) void tailGatherKernel(const float* __restrict__ t, int C, int ch,
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(256) void tailGatherKernel(const float* __restrict_
   }
 }
 
-__global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, TailParams p,
+__global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, TailParams p, int pad,
                                                                float* __restrict__ work) {
   float* base = work + static_cast<int64_t>(blockIdx.x) * s.perImage;
   const int L = s.levels;
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
   {
     const int h = tailH(s, 0), w = tailW(s, 0);
     for (int i = threadIdx.x; i < h * w; i += blockDim.x)
-      R(0)[i] = restrictK(base, s.h3, s.w3, i / w, i % w, p);
+      R(0)[i] = restrictK(base, s.h3, s.w3, i / w, i % w, p, pad);
   }
   __syncthreads();
   // down: one Jacobi sweep from zero, restriction of the remaining residual
@@ -126,12 +129,12 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
     const int h = tailH(s, t), w = tailW(s, t);
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
       const int y = i / w, x = i % w;
-      S(t)[i] = R(t)[i] - om * stencil3(R(t), h, w, y, x, p.a[0], p.a[1], p.a[2]);
+      S(t)[i] = R(t)[i] - om * stencil3(R(t), h, w, y, x, p.a[0], p.a[1], p.a[2], pad);
     }
     __syncthreads();
     const int hc = tailH(s, t + 1), wc = tailW(s, t + 1);
     for (int i = threadIdx.x; i < hc * wc; i += blockDim.x)
-      R(t + 1)[i] = restrictK(S(t), h, w, i / wc, i % wc, p);
+      R(t + 1)[i] = restrictK(S(t), h, w, i / wc, i % wc, p, pad);
     __syncthreads();
   }
   // coarsest level: the two sweeps without a coarse correction,
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
   {
     const int h = tailH(s, L - 1), w = tailW(s, L - 1);
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-      const float ar = stencil3(R(L - 1), h, w, i / w, i % w, p.a[0], p.a[1], p.a[2]);
+      const float ar = stencil3(R(L - 1), h, w, i / w, i % w, p.a[0], p.a[1], p.a[2], pad);
       E(L - 1)[i] = 2.f * om * R(L - 1)[i] - om * om * ar;
     }
   }
@@ -148,12 +151,12 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
   for (int t = L - 2; t >= 0; --t) {
     const int h = tailH(s, t), w = tailW(s, t);
     for (int i = threadIdx.x; i < h * w; i += blockDim.x)
-      S(t)[i] = om * R(t)[i] + prolongK(E(t + 1), h, w, i / w, i % w, p);
+      S(t)[i] = om * R(t)[i] + prolongK(E(t + 1), h, w, i / w, i % w, p, pad);
     __syncthreads();
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
       const int y = i / w, x = i % w;
       const float v = S(t)[i];
-      E(t)[i] = v - om * stencil3(S(t), h, w, y, x, p.a[0], p.a[1], p.a[2]) + om * R(t)[i];
+      E(t)[i] = v - om * stencil3(S(t), h, w, y, x, p.a[0], p.a[1], p.a[2], pad) + om * R(t)[i];
     }
     __syncthreads();
   }
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
 // consumed channel cleared (backward).
 __global__ __launch_bounds__(256) void tailScatterKernel(float* __restrict__ t, int C, int outCh,
                                                          int clearCh, int accumulate, TailShape s,
-                                                         TailParams p,
+                                                         TailParams p, int pad,
                                                          const float* __restrict__ work,
                                                          int64_t pix) {
   const int64_t hw = static_cast<int64_t>(s.h3) * s.w3;
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(256) void tailScatterKernel(float* __restrict__ t, 
     float v = 0.f;
     if (s.levels > 0) {
       const float* e0 = work + b * s.perImage + s.off[2];
-      v = prolongK(e0, s.h3, s.w3, static_cast<int>(i / s.w3), static_cast<int>(i % s.w3), p);
+      v = prolongK(e0, s.h3, s.w3, static_cast<int>(i / s.w3), static_cast<int>(i % s.w3), p, pad);
     }
     float* px = t + q * C;
     if (accumulate) {
@@ -187,8 +190,14 @@ __global__ __launch_bounds__(256) void tailScatterKernel(float* __restrict__ t, 
 
 }  // namespace
 
-void Unet::setOptions(int neumannPadding, const TailParams& t) {
+void Unet::setSides(int dirichletSides) {
+  require(dirichletSides >= 0 && dirichletSides < 16, "Dirichlet sides must be a 4-bit mask");
+  sides = dirichletSides;
+}
+
+void Unet::setOptions(int neumannPadding, int dirichletSides, const TailParams& t) {
   require(neumannPadding == 0 || neumannPadding == 1, "padding must be 0 (zero) or 1 (Neumann)");
+  setSides(dirichletSides);
   require(t.levels >= 0 && t.levels <= 16, "tail levels must be in 0..16");
   require(t.levels == 0 || (t.src >= 0 && t.src < 64 && t.dst >= 0 && t.dst < 64 && t.src != t.dst),
           "tail channels must be two distinct channels of the bottom tensor");
@@ -208,10 +217,10 @@ void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward) {
     hipLaunchKernelGGL(tailGatherKernel, tailGrid(pix), dim3(256), 0, stream, tensor, 64, inCh, pix,
                        s.perImage, hw, tailWork.get());
     hipLaunchKernelGGL(tailCycleKernel, dim3(static_cast<unsigned>(n)), dim3(kTailThreads), 0,
-                       stream, s, tail, tailWork.get());
+                       stream, s, tail, padBits(), tailWork.get());
   }
   hipLaunchKernelGGL(tailScatterKernel, tailGrid(pix), dim3(256), 0, stream, tensor, 64, outCh, inCh,
-                     backward ? 1 : 0, s, tail, tailWork.get(), pix);
+                     backward ? 1 : 0, s, tail, padBits(), tailWork.get(), pix);
   CFDHIP_CHECK(hipGetLastError());
 }
 

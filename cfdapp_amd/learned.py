@@ -351,8 +351,8 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
     P = param or SymmetricMultilevel
     net = UnetNet(P.weights(theta), options=options)
     rng = np.random.default_rng(seed)
-    sizes = [n for n in bank.count if bank.count[n] > 0]
-    probs = np.array([bank.count[n] * n * n for n in sizes], np.float64)
+    sizes = [k for k in bank.count if bank.count[k] > 0]
+    probs = np.array([bank.count[k] * k[0] * k[0] for k in sizes], np.float64)
     probs /= probs.sum()
     m = np.zeros(len(theta))
     v = np.zeros(len(theta))
@@ -370,16 +370,20 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
         if deadline is not None and time.time() > deadline:
             _log(f"train: time budget reached at step {step}")
             break
-        n = sizes[int(rng.choice(len(sizes), p=probs))]
-        b = min(BATCH[n], bank.count[n])
-        idx = rng.choice(bank.count[n], size=b, replace=False)
+        key = sizes[int(rng.choice(len(sizes), p=probs))]
+        n = key[0]
+        b = min(BATCH[n], bank.count[key])
+        idx = rng.choice(bank.count[key], size=b, replace=False)
         it = torch.from_numpy(idx).cuda()
-        x = bank.r[n].index_select(0, it)
-        e = bank.e[n].index_select(0, it)
-        off = torch.from_numpy(bank.sys[n][idx] * n * n).cuda()
-        den = torch.from_numpy(bank.den[n][idx]).cuda()
+        x = bank.r[key].index_select(0, it)
+        e = bank.e[key].index_select(0, it)
+        off = torch.from_numpy(bank.sys[key][idx] * n * n).cuda()
+        den = torch.from_numpy(bank.den[key][idx]).cuda()
+        if options and options.get("padding") == "neumann":
+            net.set_options(dict(options, dirichlet_sides=list(key[1])))
         net.set_weights(P.weights(theta))
-        loss, gw = net.loss_grad(x, e, bank.ce[n], bank.cn[n], off, den, b, n, n, cd=bank.cd[n])
+        loss, gw = net.loss_grad(x, e, bank.ce[key], bank.cn[key], off, den, b, n, n,
+                                 cd=bank.cd[key])
         g = P.theta_grad(theta, gw)
         m = b1 * m + (1 - b1) * g
         v = b2 * v + (1 - b2) * g * g
@@ -484,66 +488,82 @@ def assemble_pprime(d, warm_iterations, alpha_u=0.7):
 
 
 class PairBank:
-    """Harvested pairs on the device, one group per grid size."""
+    """Harvested pairs on the device, one group per key (n, sides): the grid
+    size and, for the interior-scaled (Neumann-padded) network, the Dirichlet
+    sides of the system's padding (its FixedValue pressure patches), so that
+    every training batch shares one padding."""
 
     def __init__(self, torch, capacity, interior=False):
         self.torch = torch
-        self.capacity = dict(capacity)  # n -> (max pairs, max systems)
+        self.capacity = dict(capacity)  # n -> (max pairs, max systems) per group
         # interior: pairs for a network applied with the interior-equivalent
         # scaling (Neumann padding), see to_interior_scaling
         self.interior = interior
         self.r, self.e, self.ce, self.cn, self.den, self.sys = {}, {}, {}, {}, {}, {}
         self.cd = {}
         self.count, self.systems = {}, {}
-        for n, (pairs, systems) in self.capacity.items():
+
+    def _group(self, key):
+        if key not in self.count:
+            torch = self.torch
+            n = key[0]
+            pairs, systems = self.capacity[n]
             dev = dict(device="cuda", dtype=torch.float32)
-            self.r[n] = torch.zeros((pairs, n, n), **dev)
-            self.e[n] = torch.zeros((pairs, n, n), **dev)
-            self.ce[n] = torch.zeros((systems, n, n), **dev)
-            self.cn[n] = torch.zeros((systems, n, n), **dev)
-            self.cd[n] = torch.ones((systems, n, n), **dev) if interior else None
-            self.den[n] = np.zeros(pairs, np.float32)
-            self.sys[n] = np.zeros(pairs, np.int64)
-            self.count[n] = 0
-            self.systems[n] = 0
+            self.r[key] = torch.zeros((pairs, n, n), **dev)
+            self.e[key] = torch.zeros((pairs, n, n), **dev)
+            self.ce[key] = torch.zeros((systems, n, n), **dev)
+            self.cn[key] = torch.zeros((systems, n, n), **dev)
+            self.cd[key] = torch.ones((systems, n, n), **dev) if self.interior else None
+            self.den[key] = np.zeros(pairs, np.float32)
+            self.sys[key] = np.zeros(pairs, np.int64)
+            self.count[key] = 0
+            self.systems[key] = 0
+        return key
+
+    def key_of(self, d, n):
+        if not self.interior:
+            return (n, ())
+        names = ("left", "right", "bottom", "top")
+        return (n, tuple(names[p] for p in sorted(d._fixed_value) if p < 4))
 
     def harvest(self, d, n, stride, max_pairs, noise=0, seed=0):
         """Pairs from an FP64 Jacobi-PCG solve of d's assembled p' system, plus
         `noise` white-noise pairs on the same operator."""
-        k0, s = self.count[n], self.systems[n]
+        key = self._group(self.key_of(d, n))
+        k0, s = self.count[key], self.systems[key]
         room = min(max_pairs + noise, self.capacity[n][0] - k0)
         if room <= 0 or s >= self.capacity[n][1]:
             return 0, None
         settings = LinearSolverSettings(0.0, 1e-10, 400000, "CG", "Jacobi", check_interval=64)
-        got, den, res = d.harvest(settings, stride, max(0, room - noise), n, n, self.r[n][k0],
-                                  self.e[n][k0], self.ce[n][s], self.cn[n][s])
+        got, den, res = d.harvest(settings, stride, max(0, room - noise), n, n, self.r[key][k0],
+                                  self.e[key][k0], self.ce[key][s], self.cn[key][s])
         k1 = k0 + got
         wn = min(noise, self.capacity[n][0] - k1)
         if wn > 0:
-            gn, dn = d.harvest_noise(seed, wn, n, n, self.r[n][k1], self.e[n][k1])
+            gn, dn = d.harvest_noise(seed, wn, n, n, self.r[key][k1], self.e[key][k1])
             den = np.concatenate([den, dn])
             got += gn
-        self.den[n][k0:k0 + got] = den
-        self.sys[n][k0:k0 + got] = s
+        self.den[key][k0:k0 + got] = den
+        self.sys[key][k0:k0 + got] = s
         if self.interior:
-            self.to_interior_scaling(d, n, k0, got, s)
-        self.count[n] += got
-        self.systems[n] += 1
+            self.to_interior_scaling(d, key, k0, got, s)
+        self.count[key] += got
+        self.systems[key] += 1
         return got, res
 
-    def to_interior_scaling(self, d, n, k0, got, s):
+    def to_interior_scaling(self, d, key, k0, got, s):
         """Rewrites system s and its pairs for a network applied as z = S N(S r)
-        with S the interior-equivalent diagonal^-1/2 (the Neumann-padded
-        network) instead of D^-1/2: with F = (D / D~)^1/2 the network input is
-        F r^, its target e^ / F, and the loss operator F A^ F (diagonal F^2,
-        couplings F_i c_ij F_j); (e^, r^) is unchanged."""
+        with S the library's scaling of a Neumann-padded network instead of
+        D^-1/2: with F = (D S^2)^1/2 the network input is F r^, its target
+        e^ / F, and the loss operator F A^ F (diagonal F^2, couplings
+        F_i c_ij F_j); (e^, r^) is unchanged."""
         torch = self.torch
         f = torch.from_numpy(interior_factor(d).astype(np.float32)).cuda()
-        self.r[n][k0:k0 + got] *= f
-        self.e[n][k0:k0 + got] /= f
-        self.ce[n][s][:, :-1] *= f[:, :-1] * f[:, 1:]
-        self.cn[n][s][:-1, :] *= f[:-1, :] * f[1:, :]
-        self.cd[n][s] = f * f
+        self.r[key][k0:k0 + got] *= f
+        self.e[key][k0:k0 + got] /= f
+        self.ce[key][s][:, :-1] *= f[:, :-1] * f[:, 1:]
+        self.cn[key][s][:-1, :] *= f[:-1, :] * f[1:, :]
+        self.cd[key][s] = f * f
 
     def total(self):
         return sum(self.count.values())
@@ -629,8 +649,8 @@ BATCH = {256: 16, 512: 4, 1024: 1}
 def train(torch, bank, weights, steps, lr, seed=0, deadline=None, log_every=50):
     net = UnetNet(weights)
     rng = np.random.default_rng(seed)
-    sizes = [n for n in bank.count if bank.count[n] > 0]
-    probs = np.array([bank.count[n] * n * n for n in sizes], np.float64)
+    sizes = [k for k in bank.count if bank.count[k] > 0]
+    probs = np.array([bank.count[k] * k[0] * k[0] for k in sizes], np.float64)
     probs /= probs.sum()
     losses = []
     t0 = time.time()
@@ -639,17 +659,18 @@ def train(torch, bank, weights, steps, lr, seed=0, deadline=None, log_every=50):
         if deadline is not None and time.time() > deadline:
             _log(f"train: time budget reached at step {step}")
             break
-        n = sizes[int(rng.choice(len(sizes), p=probs))]
-        b = min(BATCH[n], bank.count[n])
-        idx = rng.choice(bank.count[n], size=b, replace=False)
+        key = sizes[int(rng.choice(len(sizes), p=probs))]
+        n = key[0]
+        b = min(BATCH[n], bank.count[key])
+        idx = rng.choice(bank.count[key], size=b, replace=False)
         it = torch.from_numpy(idx).cuda()
-        x = bank.r[n].index_select(0, it)
-        e = bank.e[n].index_select(0, it)
-        off = torch.from_numpy(bank.sys[n][idx] * n * n).cuda()
-        den = torch.from_numpy(bank.den[n][idx]).cuda()
+        x = bank.r[key].index_select(0, it)
+        e = bank.e[key].index_select(0, it)
+        off = torch.from_numpy(bank.sys[key][idx] * n * n).cuda()
+        den = torch.from_numpy(bank.den[key][idx]).cuda()
         # linear warm-up, cosine decay
         f = min(1.0, (step + 1) / warm) * 0.5 * (1.0 + math.cos(math.pi * step / steps))
-        loss = net.train_step(x, e, bank.ce[n], bank.cn[n], off, den, b, n, n, lr * f)
+        loss = net.train_step(x, e, bank.ce[key], bank.cn[key], off, den, b, n, n, lr * f)
         losses.append(loss)
         if step % log_every == 0 or step == steps - 1:
             recent = float(np.mean(losses[-log_every:]))
@@ -757,7 +778,7 @@ def main(argv=None):
     _log("init:", summary["eval_init"])
     bank = harvest_cases(torch, cases, cap, args.seed, time.time() + args.harvest_seconds,
                          noise=args.noise_pairs, interior=vcycle)
-    summary["pairs"] = {str(n): int(c) for n, c in bank.count.items()}
+    summary["pairs"] = {f"{k[0]} {'+'.join(k[1]) or 'walls'}": int(c) for k, c in bank.count.items()}
     if args.free:
         w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
                           time.time() + args.train_seconds)

@@ -294,13 +294,18 @@ int cfdhip_set_learned_weights(cfdhip_ctx* ctx, const float* weights, int64_t co
  *   at most tail_levels levels), Neumann padding, damped-Jacobi weight
  *   tail_omega, level operator tail_a and transfer kernel tail_k (centre,
  *   edge, corner); it restricts channel 1 of the bottom tensor b3a and writes
- *   the prolonged correction into its channel 2, before b3b. */
+ *   the prolonged correction into its channel 2, before b3b.
+ * dirichlet_sides (padding 1): sides where the padding is the odd reflection
+ *   (zero value at the face) instead of the even one, bits 1 left, 2 right,
+ *   4 bottom, 8 top. On a solver context the sides are the case's FixedValue
+ *   pressure patches and this field is ignored. */
 typedef struct {
   int32_t padding;
   int32_t tail_levels;
   float tail_omega;
   float tail_a[3];
   float tail_k[3];
+  int32_t dirichlet_sides;
 } cfdhip_unet_options;
 int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* options);
 

@@ -24,7 +24,9 @@ pool = 2x2 average, up = nearest-neighbour x2, [a, b] = channel concatenation
 Options (``opts`` dict, the device's cfdhip_unet_options): ``padding``
 "zero" (default) or "neumann" (a tap outside the image reads the nearest pixel
 inside: the adjoint is then the transposed convolution on the grid extended by
-one ghost ring, folded back onto the boundary pixels), and ``tail`` (a
+one ghost ring, folded back onto the boundary pixels), ``dirichlet_sides``
+(with "neumann": the sides -- left, right, bottom, top; bottom is row 0 --
+where the ghost is the negated nearest pixel, zero at the face), and ``tail`` (a
 TailParams-like dict: levels, omega, a, k) -- a one-channel symmetric V(1,1)
 multigrid cycle below the bottom level that restricts channel 1 of b3a and
 writes the prolonged correction into its channel 2 (``tail_apply``).
@@ -58,25 +60,40 @@ def init_weights(seed=0, scale=1.0):
     return w
 
 
+def _signs(neumann):
+    """Ghost-ring factors (bottom, top, left, right): -1 on a Dirichlet side."""
+    sides = () if neumann is True else tuple(neumann)
+    return tuple(-1.0 if s in sides else 1.0 for s in ("bottom", "top", "left", "right"))
+
+
 def _pad1(x, neumann):
-    """One-pixel ring around the H x W axes: zeros, or the nearest pixel (Neumann)."""
+    """One-pixel ring around the H x W axes: zeros; or (neumann: True or the
+    Dirichlet sides) the nearest pixel, negated on a Dirichlet side (a corner
+    ghost takes both factors)."""
     x = np.asarray(x, dtype=np.float64)
-    if neumann:
-        return np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)), mode="edge")
-    return np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    if not neumann:
+        return np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)), mode="edge")
+    sb, st, sl, sr = _signs(neumann)
+    xp[:, 0] *= sb
+    xp[:, -1] *= st
+    xp[:, :, 0] *= sl
+    xp[:, :, -1] *= sr
+    return xp
 
 
-def _fold1(xp):
-    """Adjoint of the Neumann ring: ghost pixels add onto the pixel they copy."""
+def _fold1(xp, neumann=True):
+    """Adjoint of the ring: ghost pixels add onto the pixel they copy, with its factor."""
+    sb, st, sl, sr = _signs(neumann)
     x = xp[:, 1:-1, 1:-1, :].copy()
-    x[:, 0, :, :] += xp[:, 0, 1:-1, :]
-    x[:, -1, :, :] += xp[:, -1, 1:-1, :]
-    x[:, :, 0, :] += xp[:, 1:-1, 0, :]
-    x[:, :, -1, :] += xp[:, 1:-1, -1, :]
-    x[:, 0, 0, :] += xp[:, 0, 0, :]
-    x[:, 0, -1, :] += xp[:, 0, -1, :]
-    x[:, -1, 0, :] += xp[:, -1, 0, :]
-    x[:, -1, -1, :] += xp[:, -1, -1, :]
+    x[:, 0, :, :] += sb * xp[:, 0, 1:-1, :]
+    x[:, -1, :, :] += st * xp[:, -1, 1:-1, :]
+    x[:, :, 0, :] += sl * xp[:, 1:-1, 0, :]
+    x[:, :, -1, :] += sr * xp[:, 1:-1, -1, :]
+    x[:, 0, 0, :] += sb * sl * xp[:, 0, 0, :]
+    x[:, 0, -1, :] += sb * sr * xp[:, 0, -1, :]
+    x[:, -1, 0, :] += st * sl * xp[:, -1, 0, :]
+    x[:, -1, -1, :] += st * sr * xp[:, -1, -1, :]
     return x
 
 
@@ -101,7 +118,7 @@ def conv3x3_input_grad(g, w, neumann=False):
         for ky in range(3):
             for kx in range(3):
                 dxp[:, ky:ky + h, kx:kx + wd, :] += g64 @ w[ky, kx].T.astype(np.float64)
-        return _fold1(dxp)
+        return _fold1(dxp, neumann)
     gp = np.zeros((n, h + 2, wd + 2, cout), dtype=np.float64)
     gp[:, 1:-1, 1:-1, :] = g
     dx = np.zeros((n, h, wd, w.shape[2]), dtype=np.float64)
@@ -144,24 +161,28 @@ def up_grad(g):
 
 
 def _neumann(opts):
-    return bool(opts) and opts.get("padding", "zero") == "neumann"
+    """The padding of the options: False (zero), True (Neumann) or the Dirichlet sides."""
+    if not opts or opts.get("padding", "zero") != "neumann":
+        return False
+    return tuple(opts.get("dirichlet_sides") or ()) or True
 
 
-def _stencil(v, c, e, k):
-    """Symmetric 3 x 3 stencil (centre, edge, corner) with Neumann padding, v: (N, H, W)."""
-    p = np.pad(v, ((0, 0), (1, 1), (1, 1)), mode="edge")
+def _stencil(v, c, e, k, neumann=True):
+    """Symmetric 3 x 3 stencil (centre, edge, corner) with the ring of _pad1, v: (N, H, W)."""
+    p = _pad1(np.asarray(v, np.float64)[..., None], neumann)[..., 0]
     h, w = v.shape[1:]
     edges = p[:, 0:h, 1:w + 1] + p[:, 2:h + 2, 1:w + 1] + p[:, 1:h + 1, 0:w] + p[:, 1:h + 1, 2:w + 2]
     corners = p[:, 0:h, 0:w] + p[:, 0:h, 2:w + 2] + p[:, 2:h + 2, 0:w] + p[:, 2:h + 2, 2:w + 2]
     return c * v + e * edges + k * corners
 
 
-def tail_apply(s, tail):
+def tail_apply(s, tail, neumann=True):
     """The coarse tail on (N, H, W) images: K up(V(4 pool(K s))) with V the
-    symmetric V(1,1) cycle of the device's TailParams (Neumann padding;
+    symmetric V(1,1) cycle of the device's TailParams (the network's padding;
     levels H/2 .. the coarsest grid with even sides, at most tail["levels"])."""
     s = np.asarray(s, dtype=np.float64)
     om, a, k = tail["omega"], tail["a"], tail["k"]
+    st = lambda v, kern: _stencil(v, *kern, neumann)  # noqa: E731
     h, w = s.shape[1:]
     L = 0
     while L < tail["levels"] and h % 2 == 0 and w % 2 == 0 and h >= 2 and w >= 2:
@@ -170,20 +191,20 @@ def tail_apply(s, tail):
         return np.zeros_like(s)
 
     def restrict(v):
-        kv = _stencil(v, *k)
+        kv = st(v, k)
         n, hh, ww = kv.shape
         return kv.reshape(n, hh // 2, 2, ww // 2, 2).sum(axis=(2, 4))
 
     def prolong(e):
-        return _stencil(np.repeat(np.repeat(e, 2, axis=1), 2, axis=2), *k)
+        return st(np.repeat(np.repeat(e, 2, axis=1), 2, axis=2), k)
 
     r = [restrict(s)]
     for _ in range(L - 1):
-        r.append(restrict(r[-1] - om * _stencil(r[-1], *a)))
-    e = 2.0 * om * r[L - 1] - om * om * _stencil(r[L - 1], *a)
+        r.append(restrict(r[-1] - om * st(r[-1], a)))
+    e = 2.0 * om * r[L - 1] - om * om * st(r[L - 1], a)
     for t in range(L - 2, -1, -1):
         x2 = om * r[t] + prolong(e)
-        e = x2 - om * _stencil(x2, *a) + om * r[t]
+        e = x2 - om * st(x2, a) + om * r[t]
     return prolong(e)
 
 
@@ -192,7 +213,7 @@ def _tail_forward(b3a, opts):
     if not tail or tail.get("levels", 0) == 0:
         return b3a
     b3a = np.array(b3a, dtype=np.float64)
-    b3a[..., 2] = tail_apply(b3a[..., 1], tail)
+    b3a[..., 2] = tail_apply(b3a[..., 1], tail, _neumann(opts))
     return b3a
 
 
@@ -201,7 +222,7 @@ def _tail_backward(g, opts):
     if not tail or tail.get("levels", 0) == 0:
         return g
     g = np.array(g, dtype=np.float64)
-    g[..., 1] += tail_apply(g[..., 2], tail)
+    g[..., 1] += tail_apply(g[..., 2], tail, _neumann(opts))
     g[..., 2] = 0.0
     return g
 

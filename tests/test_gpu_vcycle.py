@@ -180,24 +180,70 @@ def test_interior_scaled_pairs_are_consistent():
     finally:
         d.close()
     assert got == 8
-    e = bank.e[64][:got].cpu().numpy().astype(np.float64)
-    ce = np.broadcast_to(bank.ce[64][0].cpu().numpy(), e.shape)
-    cn = np.broadcast_to(bank.cn[64][0].cpu().numpy(), e.shape)
-    cd = np.broadcast_to(bank.cd[64][0].cpu().numpy(), e.shape)
-    den = bank.den[64][:got].astype(np.float64)
+    key = (64, ("right",))  # the channel's FixedValue outlet
+    assert list(bank.count) == [key]
+    e = bank.e[key][:got].cpu().numpy().astype(np.float64)
+    ce = np.broadcast_to(bank.ce[key][0].cpu().numpy(), e.shape)
+    cn = np.broadcast_to(bank.cn[key][0].cpu().numpy(), e.shape)
+    cd = np.broadcast_to(bank.cd[key][0].cpu().numpy(), e.shape)
+    den = bank.den[key][:got].astype(np.float64)
     for k in range(got):
         lk, _ = unet.anorm_loss(np.zeros((1,) + e.shape[1:]), e[k:k + 1], ce[k:k + 1],
                                 cn[k:k + 1], den[k:k + 1], cd=cd[k:k + 1])
         assert lk == pytest.approx(1.0, rel=2e-3), k
     # device loss with the diagonal against the oracle, V-cycle network output
-    opts = VCycle.options()
+    opts = dict(VCycle.options(), dirichlet_sides=["right"])
     w = VCycle.weights(VCycle.initial())
     net = ca.UnetNet(w, options=opts)
     off = torch.zeros(got, dtype=torch.int64, device="cuda")
-    x = bank.r[64][:got].contiguous()
-    loss, _ = net.loss_grad(x, bank.e[64][:got].contiguous(), bank.ce[64][0], bank.cn[64][0],
-                            off, _dev(den), got, 64, 64, cd=bank.cd[64][0])
+    x = bank.r[key][:got].contiguous()
+    loss, _ = net.loss_grad(x, bank.e[key][:got].contiguous(), bank.ce[key][0], bank.cn[key][0],
+                            off, _dev(den), got, 64, 64, cd=bank.cd[key][0])
     yd = torch.empty_like(x)
     net.forward(x, yd, got, 64, 64)
     lref, _ = unet.anorm_loss(yd.cpu().numpy(), e, ce, cn, den, cd=cd)
     assert loss == pytest.approx(lref, rel=1e-4)
+
+
+@pytest.mark.parametrize("sides", [["right"], ["left", "bottom"]])
+def test_dirichlet_sides_forward_and_backward_match_oracle(sides):
+    print(f"dirichlet sides {sides}", flush=True)
+    opts = dict(VCycle.options(), dirichlet_sides=sides)
+    wts = _vcycle_weights(31)
+    net = ca.UnetNet(unet.flatten(wts), options=opts)
+    n, h, w = 2, 64, 32
+    rng = np.random.default_rng(len(sides))
+    x = rng.standard_normal((n, h, w)).astype(np.float32)
+    gy = rng.standard_normal((n, h, w)).astype(np.float32)
+    xd = _dev(x)
+    yd = torch.empty_like(xd)
+    net.forward(xd, yd, n, h, w)
+    yb, tape = unet.forward_device(_w64(wts), x[..., None], opts=opts)
+    assert _rel(yd.cpu().numpy(), yb[..., 0]) < 1e-4
+    y1 = torch.empty_like(xd)
+    net.apply(xd, y1, n, h, w)
+    np.testing.assert_array_equal(y1.cpu().numpy(), yd.cpu().numpy())
+    g = unet.unflatten(net.backward(xd, _dev(gy), n, h, w))
+    ref = unet.backward_device(_w64(wts), tape, gy[..., None].astype(np.float64), opts=opts)
+    errs = {name: _rel(g[name], ref[name]) for name, _, _ in unet.LAYERS}
+    print("gradient errors", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    for name, err in errs.items():
+        assert err < 2e-3, name
+
+
+def test_channel_outlet_uses_a_dirichlet_side():
+    """A channel's FixedValue outlet gets the odd reflection on the solver
+    context: the V-cycle network then needs a few dozen FCG iterations."""
+    from cfdapp_amd import learned
+    d = learned.make_case("channel", 256, 1000.0, 1.0, ny=128)
+    try:
+        learned.assemble_pprime(d, 1)
+        r = _fcg(d, VCycle.weights(VCycle.initial()), VCycle.options())
+        jac = d.solve_pressure_rhs(ca.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi"),
+                                   None, download=False)
+        print(f"channel 256x128: FCG {r.iterations} it, Jacobi-PCG {jac.iterations} it", flush=True)
+        assert r.status == 0 and not r.fallback_used
+        assert r.true_residual <= 1e-8 * r.initial_residual * (1 + 1e-9)
+        assert r.iterations <= 40 and 10 * r.iterations < jac.iterations
+    finally:
+        d.close()

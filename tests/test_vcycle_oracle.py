@@ -96,16 +96,22 @@ def test_tail_is_symmetric_positive_semidefinite():
     assert not np.any(U.tail_apply(rng.standard_normal((1, 5, 8)), tail))
 
 
-def _dense_stencil(h, w, c, e, k):
-    """Matrix of the symmetric 3 x 3 stencil with Neumann (replicate) padding."""
+def _dense_stencil(h, w, c, e, k, sides=()):
+    """Matrix of the symmetric 3 x 3 stencil with Neumann (replicate) padding,
+    negated ghosts on the Dirichlet sides (bottom = row 0)."""
     m = np.zeros((h * w, h * w))
     for y in range(h):
         for x in range(w):
             for dy in (-1, 0, 1):
                 for dx in (-1, 0, 1):
                     wt = c if dy == dx == 0 else (k if dy and dx else e)
+                    sg = 1.0
+                    if y + dy < 0 and "bottom" in sides or y + dy >= h and "top" in sides:
+                        sg = -sg
+                    if x + dx < 0 and "left" in sides or x + dx >= w and "right" in sides:
+                        sg = -sg
                     yy, xx = min(max(y + dy, 0), h - 1), min(max(x + dx, 0), w - 1)
-                    m[y * w + x, yy * w + xx] += wt
+                    m[y * w + x, yy * w + xx] += sg * wt
     return m
 
 
@@ -118,7 +124,7 @@ def _dense_up(h, w):
     return m
 
 
-def _dense_vcycle(theta, h, w, tail):
+def _dense_vcycle(theta, h, w, tail, sides=()):
     """B_0 of the V(1,1) cycle from explicit matrices: levels 0..3 with
     theta's coefficients, then the tail's fixed ones to the coarsest grid."""
     om = list(theta[0:4])
@@ -137,14 +143,14 @@ def _dense_vcycle(theta, h, w, tail):
     for l in range(levels - 1, -1, -1):
         hh, ww = sizes[l]
         n = hh * ww
-        Al = _dense_stencil(hh, ww, *A[l])
+        Al = _dense_stencil(hh, ww, *A[l], sides=sides)
         I = np.eye(n)
         S = I - om[l] * Al
         base = om[l] * (I + S)  # omega r + (I - omega A) omega r
         if B is None:
             B = base
             continue
-        P = _dense_stencil(hh, ww, *K[l]) @ _dense_up(hh, ww)
+        P = _dense_stencil(hh, ww, *K[l], sides=sides) @ _dense_up(hh, ww)
         B = base + S @ P @ B @ P.T @ S
     return B
 
@@ -294,3 +300,103 @@ def test_anorm_loss_with_diagonal_matches_finite_differences():
         fd = (U.anorm_loss(yp, e, ce, cn, den, cd=cd)[0] -
               U.anorm_loss(ym, e, ce, cn, den, cd=cd)[0]) / (2 * eps)
         assert g[idx] == pytest.approx(fd, rel=1e-6)
+
+
+SIDES = ("right", "bottom")
+
+
+@pytest.mark.parametrize("cin,cout", [(1, 3), (3, 5)])
+def test_dirichlet_sides_adjoint_identity(cin, cout):
+    rng = np.random.default_rng(cin + 17 * cout)
+    x = rng.standard_normal((2, 6, 5, cin))
+    g = rng.standard_normal((2, 6, 5, cout))
+    w = rng.standard_normal((3, 3, cin, cout))
+    lhs = np.sum(U.conv3x3(x, w, SIDES) * g)
+    assert np.sum(x * U.conv3x3_input_grad(g, w, SIDES)) == pytest.approx(lhs, rel=1e-12)
+    assert np.sum(U.conv3x3_weight_grad(x, g, SIDES) * w) == pytest.approx(lhs, rel=1e-12)
+    # mirror-symmetric kernels: the adjoint is the conv of the transposed kernel, same padding
+    wm = 0.25 * (w + w[::-1] + w[:, ::-1] + w[::-1, ::-1])
+    wt = np.transpose(wm[::-1, ::-1], (0, 1, 3, 2))
+    np.testing.assert_allclose(U.conv3x3(g, wt, SIDES), U.conv3x3_input_grad(g, wm, SIDES),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_dirichlet_sides_network_equals_dense_cycle():
+    rng = np.random.default_rng(13)
+    th = VCycle.repair(VCycle.initial() + rng.normal(0.0, 0.03, VCycle.N_THETA))
+    th[25] = 0.0
+    h, wd = 32, 16
+    B = _dense_vcycle(th, h, wd, tail=True, sides=SIDES)
+    opts = dict(_opts(), dirichlet_sides=list(SIDES))
+    x = rng.standard_normal((1, h, wd, 1))
+    y = U.forward(_w64(th), x, opts=opts)[0, :, :, 0].ravel()
+    np.testing.assert_allclose(y, B @ x[0, :, :, 0].ravel(), rtol=1e-10, atol=1e-10)
+    assert np.abs(B - B.T).max() < 1e-10 * np.abs(B).max()
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > 0.0  # no null space with a Dirichlet side
+
+
+def test_backward_with_dirichlet_sides_matches_finite_differences():
+    rng = np.random.default_rng(22)
+    th = VCycle.initial() + rng.normal(0.0, 0.05, VCycle.N_THETA)
+    w = {k: v.copy() for k, v in _w64(th).items()}
+    x = rng.standard_normal((1, 32, 32, 1))
+    gy = rng.standard_normal((1, 32, 32, 1))
+    opts = dict(_opts(), dirichlet_sides=["left", "top"])
+
+    def loss(wd):
+        return float(np.sum(U.forward(wd, x, opts=opts) * gy))
+
+    _, tape = U.forward(w, x, keep=True, opts=opts)
+    dw, gx = U.backward(w, tape, gy, opts=opts)
+    eps = 1e-5
+    for name in ("e0a", "e1b", "b3a", "b3b", "d1a", "d0b", "out"):
+        idx = (0, 1, 0, 0)
+        wp = {k: v.copy() for k, v in w.items()}
+        wm = {k: v.copy() for k, v in w.items()}
+        wp[name][idx] += eps
+        wm[name][idx] -= eps
+        fd = (loss(wp) - loss(wm)) / (2 * eps)
+        assert dw[name][idx] == pytest.approx(fd, rel=1e-6, abs=1e-7), name
+    assert np.sum(gx * x) == pytest.approx(loss(w), rel=1e-10)
+
+
+def _outlet_operator(z):
+    """d = 1 p' operator of a channel with a FixedValue outlet on the right
+    (boundary coupling 2: half-cell distance), no pinned cell."""
+    y = U.laplacian_apply(z, pin=False)
+    y[:, -1] += 2.0 * z[:, -1]
+    return y
+
+
+def test_dirichlet_side_preconditioner_on_an_outlet():
+    n = 64
+    w = _w64(VCycle.initial())
+    rng = np.random.default_rng(2)
+    b = rng.uniform(-1.0, 1.0, (n, n))
+
+    def pcg(prec):
+        x = np.zeros_like(b)
+        r = b.copy()
+        z = prec(r)
+        p = z.copy()
+        rz = np.sum(r * z)
+        for it in range(1, 3000):
+            q = _outlet_operator(p)
+            a = rz / np.sum(p * q)
+            x += a * p
+            r -= a * q
+            if np.linalg.norm(b - _outlet_operator(x)) <= 1e-8 * np.linalg.norm(b):
+                return it
+            z = prec(r)
+            rz2 = np.sum(r * z)
+            p = z + rz2 / rz * p
+            rz = rz2
+        return -1
+
+    def prec(sides):
+        opts = dict(_opts(), dirichlet_sides=sides)
+        return lambda r: 0.25 * U.forward(w, r[None, :, :, None], opts=opts)[0, :, :, 0]
+
+    with_side, without = pcg(prec(["right"])), pcg(prec([]))
+    assert 0 < with_side <= 20, with_side
+    assert with_side < without, (with_side, without)
