@@ -48,6 +48,8 @@ def lib():
         _LIB.orc_case_destroy.argtypes = [ctypes.c_void_p]
         _LIB.orc_csr_build.restype = ctypes.c_int64
         _LIB.orc_momentum.restype = ctypes.c_int64
+        _LIB.orc_momentum_transient.restype = ctypes.c_int64
+        _LIB.orc_piso.restype = ctypes.c_int64
         _LIB.orc_pressure_assemble.restype = ctypes.c_int64
         _LIB.orc_case_set_bc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -224,7 +226,9 @@ class Case:
                                     _p(gv), _p(gw))
         return gu.reshape(3, n), gv.reshape(3, n), gw.reshape(3, n)
 
-    def momentum(self, component, u, p, F, mu, prev, alpha):
+    def momentum(self, component, u, p, F, mu, prev, alpha, rho=1.0, dt=0.0):
+        """Relaxed momentum component; with dt > 0 (alpha 1) the transient one
+        (TransientMomentum.cpp)."""
         ux, uy, uz = self._u(u)
         n = self.n_cells
         cap = 7 * n + 16
@@ -232,10 +236,10 @@ class Case:
         col = np.zeros(cap, np.int64)
         val = np.zeros(cap)
         rhs, diag = np.zeros(n), np.zeros(n)
-        nnz = lib().orc_momentum(ctypes.c_void_p(self.h), component, _p(ux), _p(uy), _p(uz),
-                                 _p(f64(p)), _p(f64(F)), ctypes.c_double(mu), _p(f64(prev)),
-                                 ctypes.c_double(alpha), _p(rp), _p(col), _p(val), _p(rhs),
-                                 _p(diag))
+        nnz = lib().orc_momentum_transient(
+            ctypes.c_void_p(self.h), component, _p(ux), _p(uy), _p(uz), _p(f64(p)), _p(f64(F)),
+            ctypes.c_double(mu), _p(f64(prev)), ctypes.c_double(alpha), ctypes.c_double(rho),
+            ctypes.c_double(dt), _p(rp), _p(col), _p(val), _p(rhs), _p(diag))
         if nnz < 0:
             raise FloatingPointError("momentum system is not finite")
         return Csr(rp, col[:nnz], val[:nnz]), rhs, diag
@@ -311,3 +315,21 @@ class Case:
                     final_continuity=meta[4], global_imbalance=meta[5],
                     hist=dict(u=h[0], v=h[1], w=h[2] if self.dim == 3 else h[2][:0], p=h[3],
                               c=h[4]))
+
+    def piso(self, steps, dt, u, p, F, re=100.0, lid=1.0, mom=(1e-10, 1e-8, 500),
+             pres=(1e-10, 1e-8, 2000), ref=0):
+        """`steps` PISO time steps (PISO.cpp:126-350) from (u, p, F); returns the
+        final state and per-step {status, max_cfl, continuity, mass_imbalance,
+        momentum_iterations, pressure_iterations}."""
+        n = self.n_cells
+        s = f64([dt, steps, 1.0, lid / re, ref, mom[0], mom[1], mom[2], 0.0,
+                 pres[0], pres[1], pres[2], 0.0])
+        ux, uy = f64(u[0]).copy(), f64(u[1]).copy()
+        pv, Fv = f64(p).copy(), f64(F).copy()
+        meta = np.zeros(6 * int(steps))
+        k = lib().orc_piso(ctypes.c_void_p(self.h), _p(s), _p(ux), _p(uy), _p(pv), _p(Fv),
+                           _p(meta))
+        m = meta.reshape(-1, 6)[:k]
+        return dict(u=(ux, uy), p=pv, F=Fv, steps=[dict(status=int(r[0]), max_cfl=r[1],
+                    continuity=r[2], mass_imbalance=r[3], momentum_iterations=int(r[4]),
+                    pressure_iterations=int(r[5])) for r in m])

@@ -900,11 +900,15 @@ ScalarBC pressureBC(const Boundaries& b) { return ScalarBC{b.pkind, b.pvalue}; }
 // assembleRelaxedMomentumComponent (src/pressure_velocity/RelaxedMomentum.cpp:30-104):
 // diffusion (MomentumEquation.cpp:187-262 with NonOrthogonalDiffusion.cpp:51-113),
 // upwind convection (:264-327), pressure source (:415-430), implicit
-// relaxation (UnderRelaxation.cpp:13-36).
+// relaxation (UnderRelaxation.cpp:13-36). With dt > 0 (alpha 1) it is
+// assembleTransientMomentumComponent (src/pressure_velocity/TransientMomentum.cpp:
+// the same terms, then the implicit-Euler time term of TimeDerivative.cpp:13-34,
+// a_P,t = (rho V) / dt on the diagonal and a_P,t prev on the right-hand side,
+// added after the pressure source as applyTransientTerm does).
 Momentum assembleMomentum(const Mesh& m, const std::vector<V3>& u, const std::vector<double>& p,
                           const std::vector<double>& F, const std::vector<double>& mu,
                           const Boundaries& b, int component, const std::vector<double>& prev,
-                          double alpha) {
+                          double alpha, double rho = 1.0, double dt = 0.0) {
   const Index n = m.nCells();
   Builder B(n);
   std::vector<double> rhs(n, 0.0);
@@ -978,6 +982,13 @@ Momentum assembleMomentum(const Mesh& m, const std::vector<V3>& u, const std::ve
   }
   const std::vector<V3> gp = gradient(m, p, pressureBC(b));
   for (Index c = 0; c < n; ++c) rhs[c] += -m.vol[c] * comp(gp[c], component);
+  if (dt > 0.0) {
+    for (Index i = 0; i < n; ++i) {
+      const double aPTime = (rho * m.vol[i]) / dt;
+      B.add(i, i, aPTime);
+      rhs[i] += aPTime * prev[i];
+    }
+  }
   Momentum out;
   if (alpha != 1.0) {
     const Csr un = B.build();
@@ -1315,6 +1326,131 @@ SimpleResult simple(const Mesh& m, const Boundaries& b, const SimpleSettings& s,
   return R;
 }
 
+// ---------------------------------------------------------------------------
+// One PISO time step (src/pressure_velocity/PISO.cpp:126-350, detail::
+// solvePisoStep without ALE), laminar, 2D: transient momentum predictor,
+// linear predictor flux, two pressure corrections (the second assembled from
+// the first's corrected flux; p += p' unrelaxed; the same response
+// coefficients for both), final continuity from the last flux; pre-step CFL
+// (src/solver/CFL.cpp:12-45). Both linear solvers are BiCGSTAB, as PISO
+// constructs them. On failure the state is the previous one.
+enum PisoStatus { tConverged = 0, tMomentumFailure = 1, tPressureFailure = 2, tNonFinite = 3,
+                  tInvalid = 4 };
+struct PisoResult {
+  int status = tInvalid;
+  double maxCFL = 0.0, continuity = 0.0, massImbalance = 0.0;
+  std::vector<V3> u;
+  std::vector<double> p, F;
+  Index momentumIters = 0, pressureIters = 0;
+};
+
+double maxCFL(const Mesh& m, const std::vector<double>& F, double rho, double dt) {
+  double mx = 0.0;
+  for (Index c = 0; c < m.nCells(); ++c) {
+    double s = 0.0;
+    for (Index k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) s += std::abs(F[m.cf[k]]);
+    const double cfl = (dt / (2.0 * m.vol[c])) * (s / rho);
+    if (cfl > mx) mx = cfl;
+  }
+  return mx;
+}
+
+PisoResult pisoStep(const Mesh& m, const Boundaries& b, double rho, double mu,
+                    const std::vector<V3>& u0, const std::vector<double>& p0,
+                    const std::vector<double>& F0, double dt, Index ref, const Settings& mom,
+                    const Settings& pres) {
+  PisoResult R;
+  R.u = u0;
+  R.p = p0;
+  R.F = F0;
+  const Index n = m.nCells();
+  if (m.dim != 2 || !(dt > 0.0) || !std::isfinite(dt) || ref >= n) return R;  // tInvalid
+  if (!finiteField(u0) || !allFinite(p0) || !allFinite(F0)) {
+    R.status = tNonFinite;
+    return R;
+  }
+  const double cfl = maxCFL(m, F0, rho, dt);
+  const std::vector<double> muv(n, mu + 0.0);
+  std::vector<std::vector<double>> prev(2, std::vector<double>(n));
+  for (Index c = 0; c < n; ++c) {
+    prev[0][c] = u0[c].x;
+    prev[1][c] = u0[c].y;
+  }
+  const Momentum a0 = assembleMomentum(m, u0, p0, F0, muv, b, 0, prev[0], 1.0, rho, dt);
+  const Momentum a1 = assembleMomentum(m, u0, p0, F0, muv, b, 1, prev[1], 1.0, rho, dt);
+  if (!a0.ok || !a1.ok) {
+    R.status = tNonFinite;
+    return R;
+  }
+  const Result r0 = bicgstab(a0.A, a0.rhs, prev[0], mom);
+  if (r0.status != kConverged) {
+    R.status = tMomentumFailure;
+    return R;
+  }
+  const Result r1 = bicgstab(a1.A, a1.rhs, prev[1], mom);
+  if (r1.status != kConverged) {
+    R.status = tMomentumFailure;
+    return R;
+  }
+  std::vector<V3> ustar(n);
+  for (Index c = 0; c < n; ++c) ustar[c] = V3{r0.x[c], r1.x[c], 0.0};
+  if (!finiteField(ustar)) {
+    R.status = tNonFinite;
+    return R;
+  }
+  const std::vector<double> Fstar = massFlux(m, ustar, b, rho);
+  if (!allFinite(Fstar)) {
+    R.status = tNonFinite;
+    return R;
+  }
+  const std::vector<double> dU = response(m, a0.diag), dV = response(m, a1.diag);
+  const std::vector<double> zero(n, 0.0);
+  std::vector<V3> u = ustar;
+  std::vector<double> p = p0, F = Fstar;
+  Index presIters = 0;
+  for (int corr = 0; corr < 2; ++corr) {
+    const PressureSystem ps = assemblePressure(m, F, dU, dV, nullptr, rho, ref, b);
+    if (!matrixFinite(ps.A) || !allFinite(ps.rhs)) {
+      R.status = tNonFinite;
+      return R;
+    }
+    const Result pr = bicgstab(ps.A, ps.rhs, zero, pres);
+    if (pr.status != kConverged) {
+      R.status = tPressureFailure;
+      return R;
+    }
+    presIters += pr.iterations;
+    if (!allFinite(pr.x)) {
+      R.status = tNonFinite;
+      return R;
+    }
+    std::vector<double> pn(n);
+    for (Index c = 0; c < n; ++c) pn[c] = p[c] + pr.x[c];
+    std::vector<V3> un = correctVelocity(m, u, dU, dV, nullptr, pr.x, b, nullptr);
+    std::vector<double> Fn = correctFlux(m, F, ps.faceCoeff, pr.x);
+    if (!allFinite(pn) || !finiteField(un) || !allFinite(Fn)) {
+      R.status = tNonFinite;
+      return R;
+    }
+    p = std::move(pn);
+    u = std::move(un);
+    F = std::move(Fn);
+  }
+  const Continuity cont = continuity(m, F);
+  double mx = 0.0;
+  for (double v : cont.imbalance) mx = std::max(mx, std::abs(v));
+  R.status = tConverged;
+  R.maxCFL = cfl;
+  R.continuity = mx;
+  R.massImbalance = std::abs(cont.global);
+  R.u = std::move(u);
+  R.p = std::move(p);
+  R.F = std::move(F);
+  R.momentumIters = r0.iterations + r1.iterations;
+  R.pressureIters = presIters;
+  return R;
+}
+
 }  // namespace orc
 
 // ===========================================================================
@@ -1494,9 +1630,21 @@ void orc_velocity_gradient(void* h, const double* ux, const double* uy, const do
   if (c->mesh.dim == 3) fromV3(w, gw, gw + c->mesh.nCells(), gw + 2 * c->mesh.nCells());
 }
 // Returns nnz (or -1 when the assembled system is non-finite).
+int64_t orc_momentum_transient(void* h, int component, const double* ux, const double* uy,
+                               const double* uz, const double* p, const double* F, double mu,
+                               const double* prev, double alpha, double rho, double dt,
+                               int64_t* rp, int64_t* col, double* val, double* rhs, double* diag);
 int64_t orc_momentum(void* h, int component, const double* ux, const double* uy, const double* uz,
                      const double* p, const double* F, double mu, const double* prev, double alpha,
                      int64_t* rp, int64_t* col, double* val, double* rhs, double* diag) {
+  return orc_momentum_transient(h, component, ux, uy, uz, p, F, mu, prev, alpha, 1.0, 0.0, rp, col,
+                                val, rhs, diag);
+}
+// dt > 0: the transient momentum component (alpha must be 1).
+int64_t orc_momentum_transient(void* h, int component, const double* ux, const double* uy,
+                               const double* uz, const double* p, const double* F, double mu,
+                               const double* prev, double alpha, double rho, double dt,
+                               int64_t* rp, int64_t* col, double* val, double* rhs, double* diag) {
   auto* c = static_cast<Case*>(h);
   const Mesh& m = c->mesh;
   std::vector<V3> u;
@@ -1504,7 +1652,7 @@ int64_t orc_momentum(void* h, int component, const double* ux, const double* uy,
   const Momentum M = assembleMomentum(m, u, std::vector<double>(p, p + m.nCells()),
                                       std::vector<double>(F, F + m.nFaces()),
                                       std::vector<double>(m.nCells(), mu + 0.0), c->bc, component,
-                                      std::vector<double>(prev, prev + m.nCells()), alpha);
+                                      std::vector<double>(prev, prev + m.nCells()), alpha, rho, dt);
   std::copy(M.rhs.begin(), M.rhs.end(), rhs);
   std::copy(M.diag.begin(), M.diag.end(), diag);
   const int64_t nnz = csrOut(M.A, rp, col, val);
@@ -1570,6 +1718,43 @@ void orc_continuity(void* h, const double* F, double* imbalance, double* scalars
   for (double v : ct.imbalance) ss += v * v;
   scalars[0] = std::sqrt(ss / static_cast<double>(ct.imbalance.size()));
   scalars[1] = ct.global;
+}
+// PISO steps: settings {dt, steps, rho, mu, ref, mom abs, rel, max, jacobi,
+// pres abs, rel, max, jacobi}; the state (ux, uy, p, F) is advanced in place;
+// meta out per step {status, maxCFL, continuity, massImbalance, momIters, presIters}.
+// Returns the number of steps taken (a failed step ends the run, state unchanged).
+int64_t orc_piso(void* h, const double* s, double* ux, double* uy, double* p, double* F,
+                 double* meta) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  const Index n = m.nCells();
+  std::vector<V3> u;
+  toV3(m, ux, uy, nullptr, u);
+  std::vector<double> pv(p, p + n), Fv(F, F + m.nFaces());
+  const int64_t steps = static_cast<int64_t>(s[1]);
+  int64_t k = 0;
+  for (; k < steps; ++k) {
+    const PisoResult R = pisoStep(m, c->bc, s[2], s[3], u, pv, Fv, s[0], static_cast<Index>(s[4]),
+                                  settings(s + 5), settings(s + 9));
+    double* mt = meta + 6 * k;
+    mt[0] = R.status;
+    mt[1] = R.maxCFL;
+    mt[2] = R.continuity;
+    mt[3] = R.massImbalance;
+    mt[4] = static_cast<double>(R.momentumIters);
+    mt[5] = static_cast<double>(R.pressureIters);
+    if (R.status != tConverged) {
+      ++k;
+      break;
+    }
+    u = R.u;
+    pv = R.p;
+    Fv = R.F;
+  }
+  fromV3(u, ux, uy, nullptr);
+  std::copy(pv.begin(), pv.end(), p);
+  std::copy(Fv.begin(), Fv.end(), F);
+  return k;
 }
 // settings: {maxIter, au, ap, vtol, ptol, ctol, rho, mu, ref, rhieChow, presCG,
 //            mom abs, rel, max, jacobi, pres abs, rel, max, jacobi}

@@ -12,6 +12,11 @@
 //   simple  dim nx ny [nz] re iters flux psolver -> SIMPLE::solve fields+histories (a15)
 //   stages  dim nx ny [nz] re warm flux psolver  -> one hand-driven SIMPLE iteration,
 //                                                 every stage dumped (a3-a14)
+//   piso    nx ny re dt warm steps              -> `warm` PISO::solveTimeStep calls from rest,
+//                                                 the next step's transient momentum systems
+//                                                 (TransientMomentum.cpp) from that state, then
+//                                                 `steps` more PISO steps: per-step status, CFL,
+//                                                 continuity, and the fields after each step
 //   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
 //                                                 iterations on the seeded p' system
 //                                                 (CPU baseline; prints one JSON line)
@@ -39,7 +44,10 @@
 #include "cfd/pressure_velocity/PressureCorrectionEquation.hpp"
 #include "cfd/pressure_velocity/RelaxedMomentum.hpp"
 #include "cfd/pressure_velocity/RhieChow.hpp"
+#include "cfd/pressure_velocity/PISO.hpp"
 #include "cfd/pressure_velocity/SIMPLE.hpp"
+#include "cfd/pressure_velocity/TransientMomentum.hpp"
+#include "cfd/solver/TransientSolver.hpp"
 
 using namespace cfd;
 using cfd::algebra::LinearSolverSettings;
@@ -307,6 +315,62 @@ void cmdSimple() {
   dumpF("scalars", {res.finalContinuityResidual, res.globalMassImbalance});
 }
 
+// PISO time steps on the 2D cavity (src/pressure_velocity/PISO.cpp:126-350), both
+// linear solvers BiCGSTAB as PISO constructs them (pressure: psolver=bicgstab).
+void cmdPiso() {
+  const Case c = makeCavity();
+  const Mesh& m = c.mesh;
+  const size_t n = m.numberOfCells();
+  const double re = farg("re", "100");
+  const physics::FluidProperties fluid(1.0, farg("lid", "1.0") / re);
+  const double dt = farg("dt");
+  const pressure_velocity::PISOSettings settings{momentumSettings(), pressureSettings()};
+  const pressure_velocity::PISO piso(m, fluid, c.vbc, c.pbc, settings, 0);
+  solver::TransientState state{VectorField(n, Vector3{}), ScalarField(n, 0.0), SurfaceField()};
+  state.massFlux = physics::calculateMassFlux(m, state.velocity, fluid, c.vbc);
+  const long warm = iarg("warm", "0"), steps = iarg("steps", "1");
+  for (long s = 0; s < warm; ++s) {
+    const auto r = piso.solveTimeStep(state, dt);
+    if (r.status != solver::TransientStepStatus::Converged) {
+      std::fprintf(stderr, "warm step %ld failed\n", s);
+      std::exit(1);
+    }
+    state = r.state;
+  }
+  dumpVecField("in_u", state.velocity, false);
+  dumpF("in_p", toVec(state.pressure));
+  dumpF("in_F", toVec(state.massFlux));
+  const ScalarField mu(n, fluid.dynamicViscosity());
+  const char* names[2] = {"mu", "mv"};
+  for (int k = 0; k < 2; ++k) {
+    ScalarField prev(n);
+    for (size_t i = 0; i < n; ++i) prev[i] = k == 0 ? state.velocity[i].x : state.velocity[i].y;
+    const auto a = pressure_velocity::assembleTransientMomentumComponent(
+        m, state.velocity, state.pressure, state.massFlux, fluid, mu, c.vbc, c.pbc,
+        k == 0 ? physics::VelocityComponent::U : physics::VelocityComponent::V, prev, dt);
+    dumpMatrix(names[k], a.system.matrix());
+    dumpF(std::string(names[k]) + "_rhs", toVec(a.system.rhs()));
+    dumpF(std::string(names[k]) + "_diag", toVec(a.diagonal));
+  }
+  std::vector<double> scal;
+  std::vector<int64_t> status;
+  for (long s = 0; s < steps; ++s) {
+    const auto r = piso.solveTimeStep(state, dt);
+    status.push_back(static_cast<int64_t>(r.status));
+    scal.push_back(r.maxCFL);
+    scal.push_back(r.continuityResidual);
+    scal.push_back(r.massImbalance);
+    if (r.status != solver::TransientStepStatus::Converged) break;
+    state = r.state;
+    const std::string tag = "s" + std::to_string(s);
+    dumpVecField(tag + "_u", state.velocity, false);
+    dumpF(tag + "_p", toVec(state.pressure));
+    dumpF(tag + "_F", toVec(state.massFlux));
+  }
+  dumpI("status", status);
+  dumpF("scalars", scal);
+}
+
 // One outer iteration driven by hand through the same calls SIMPLE::solve makes
 // on its CPU path, starting from the state after `warm` SIMPLE iterations.
 void cmdStages() {
@@ -487,6 +551,7 @@ int main(int argc, char** argv) {
   else if (cmd == "simple") cmdSimple();
   else if (cmd == "stages") cmdStages();
   else if (cmd == "cgtime") cmdCgTime();
+  else if (cmd == "piso") cmdPiso();
   else {
     std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
     return 2;

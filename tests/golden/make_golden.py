@@ -43,6 +43,10 @@ FIXTURES = {
                                             flux="rhie_chow", psolver="cg_jacobi")),
     "simple3d_6_rc": ("simple", dict(dim=3, nx=6, ny=6, nz=6, re=400, iters=40,
                                      flux="rhie_chow", psolver="cg_jacobi")),
+    "piso2d_17": ("piso", dict(dim=2, nx=17, ny=17, re=100, dt=0.01, warm=3, steps=3,
+                               psolver="bicgstab")),
+    "piso2d_24x16_re400": ("piso", dict(dim=2, nx=24, ny=16, re=400, dt=0.004, warm=4, steps=2,
+                                        psolver="bicgstab")),
 }
 
 
