@@ -292,6 +292,27 @@ class GpuSimpleDiscretization:
         self._ctx.call("cfdhip_field_download", FIELDS[name], _pf(a), len(a))
         return a
 
+    def piso_run(self, steps, dt, momentum: LinearSolverSettings | None = None,
+                 pressure: LinearSolverSettings | None = None, reference_cell=0):
+        """PISO::solveTimeStep `steps` times on the resident state (cfdhip_piso_run).
+        Defaults are the reference's: BiCGSTAB for both systems. Returns
+        {steps: [{status, status_name, max_cfl, continuity, mass_imbalance,
+        momentum_iterations, pressure_iterations}], taken}."""
+        momentum = momentum or LinearSolverSettings(1e-10, 1e-8, 500, "BiCGSTAB")
+        pressure = pressure or LinearSolverSettings(1e-10, 1e-8, 20000, "BiCGSTAB")
+        meta = np.zeros(6 * int(steps))
+        taken = ctypes.c_int64(0)
+        ms, ps = momentum.c(), pressure.c()
+        self._ctx.call("cfdhip_piso_run", float(dt), int(steps), int(reference_cell),
+                       ctypes.byref(ms), ctypes.byref(ps), _pf(meta), ctypes.byref(taken))
+        names = ["Converged", "MomentumFailure", "PressureCorrectionFailure", "NonFiniteState",
+                 "InvalidConfiguration"]
+        rows = meta.reshape(-1, 6)[:taken.value]
+        return dict(taken=taken.value, steps=[
+            dict(status=int(r[0]), status_name=names[int(r[0])], max_cfl=float(r[1]),
+                 continuity=float(r[2]), mass_imbalance=float(r[3]),
+                 momentum_iterations=int(r[4]), pressure_iterations=int(r[5])) for r in rows])
+
     def field_device(self, name):
         """Device address of a resident field (borrowed from the context)."""
         ptr = ctypes.c_void_p()
@@ -327,6 +348,10 @@ class GpuSimpleDiscretization:
 
     def assemble_momentum_resident(self, alpha):
         self._ctx.call("cfdhip_assemble_momentum", float(alpha))
+
+    def assemble_transient_momentum(self, dt):
+        """The PISO predictor's momentum system (TransientMomentum.cpp)."""
+        self._ctx.call("cfdhip_assemble_transient_momentum", float(dt))
 
     def solve_momentum_resident(self, component, settings: LinearSolverSettings) -> SolverResult:
         r = _lib.SolveResult()

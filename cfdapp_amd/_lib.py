@@ -153,6 +153,10 @@ SIGNATURES = {
                                               ctypes.c_int, P_F64]),
     "cfdhip_pressure_spmv_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),
     "cfdhip_field_device": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(VP)]),
+    "cfdhip_assemble_transient_momentum": (ctypes.c_int, [VP, ctypes.c_double]),
+    "cfdhip_piso_run": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.POINTER(SolverSettings),
+                                       ctypes.POINTER(SolverSettings), P_F64, P_I64]),
     "cfdhip_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "cfdhip_batch_destroy": (None, [VP]),
     "cfdhip_batch_last_error": (ctypes.c_char_p, [VP]),

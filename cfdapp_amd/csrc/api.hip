@@ -250,6 +250,16 @@ int cfdhip_assemble_momentum(cfdhip_ctx* ctx, double alpha) {
   return guarded(ctx, [&](Context& c) { launchMomentum(c, alpha); });
 }
 
+int cfdhip_assemble_transient_momentum(cfdhip_ctx* ctx, double dt) {
+  return guarded(ctx, [&](Context& c) {
+    require(std::isfinite(dt) && dt > 0.0,
+            "assembleTransientMomentumComponent: dt must be finite and > 0");
+    require(c.view.dim == 2, "assembleTransientMomentumComponent: the mesh must be 2D",
+            CFDHIP_ERR_UNSUPPORTED);
+    launchMomentum(c, 1.0, dt);
+  });
+}
+
 int cfdhip_solve_momentum(cfdhip_ctx* ctx, int component, const cfdhip_solver_settings* s,
                           cfdhip_solve_result* r) {
   return guarded(ctx, [&](Context& c) {
@@ -324,6 +334,27 @@ int cfdhip_simple_run(cfdhip_ctx* ctx, const double* settings,
   return guarded(ctx, [&](Context& c) {
     require(settings && momentum && pressure && meta, "settings, solvers and meta are required");
     simpleRun(c, settings, *momentum, *pressure, meta, history, cap);
+  });
+}
+
+int cfdhip_piso_run(cfdhip_ctx* ctx, double dt, int64_t steps, int64_t reference_cell,
+                    const cfdhip_solver_settings* momentum, const cfdhip_solver_settings* pressure,
+                    double* meta, int64_t* taken) {
+  return guarded(ctx, [&](Context& c) {
+    require(momentum && pressure && meta && taken, "solvers, meta and taken are required");
+    // LinearSolver::validateSettings (src/algebra/LinearSolver.cpp:9-28): PISO
+    // reports bad settings as InvalidConfiguration (PISO.cpp:142-163)
+    auto valid = [](const cfdhip_solver_settings& s) {
+      return s.absolute_tolerance >= 0.0 && s.relative_tolerance >= 0.0 &&
+             (s.absolute_tolerance > 0.0 || s.relative_tolerance > 0.0) && s.max_iterations >= 1;
+    };
+    if (!valid(*momentum) || !valid(*pressure)) {
+      for (int k = 0; k < 6; ++k) meta[k] = 0.0;
+      meta[0] = 4;  // TransientStepStatus::InvalidConfiguration
+      *taken = 1;
+      return;
+    }
+    *taken = pisoRun(c, dt, steps, reference_cell, *momentum, *pressure, meta);
   });
 }
 

@@ -205,6 +205,10 @@ struct MomentumArgs {
   double* diag;
   unsigned* nonFinite;
   double alpha;
+  // transient momentum (dt > 0, alpha 1): the implicit-Euler time term of the
+  // previous velocity uOld with density rho
+  double rho, dt;
+  const double* uOld[3];
 };
 
 __global__ void momentumKernel(View v, MomentumArgs m) {
@@ -287,6 +291,14 @@ __global__ void momentumKernel(View v, MomentumArgs m) {
 
   // Pressure source: rhs += -V * dp/dx_k (MomentumEquation.cpp:415-430).
   for (int k = 0; k < D; ++k) rhs[k] += -v.vol * m.gp[k][c];
+
+  // Implicit-Euler time term (TimeDerivative.cpp:13-34), added after the
+  // pressure source as applyTransientTerm does (TransientMomentum.cpp).
+  if (m.dt > 0.0) {
+    const double aPTime = (m.rho * v.vol) / m.dt;
+    acc[P].add(aPTime);
+    for (int k = 0; k < D; ++k) rhs[k] += aPTime * m.uOld[k][c];
+  }
 
   // Implicit under-relaxation (UnderRelaxation.cpp:13-36): the unrelaxed
   // diagonal must exist (a zero sum is dropped by the CSR build).
@@ -687,10 +699,12 @@ void launchMassFlux(Context& ctx, const double* u, const double* v, const double
   CFDHIP_CHECK(hipGetLastError());
 }
 
-void launchMomentum(Context& ctx, double alpha) {
+void launchMomentum(Context& ctx, double alpha, double dt) {
   ctx.requireMesh();
   require(std::isfinite(alpha) && alpha > 0.0 && alpha <= 1.0,
           "applyImplicitUnderRelaxation: alpha must be finite and in (0, 1]");
+  require(dt == 0.0 || (std::isfinite(dt) && dt > 0.0 && alpha == 1.0),
+          "assembleTransientMomentumComponent: dt must be finite and > 0 (no relaxation)");
   require(ctx.fluidSet, "fluid properties not set");
   const View& v = ctx.view;
   const int D = v.dim;
@@ -710,6 +724,9 @@ void launchMomentum(Context& ctx, double alpha) {
   m.diag = ctx.f(CFDHIP_FIELD_MOM_DIAG);
   m.nonFinite = ctx.flags.get();
   m.alpha = alpha;
+  m.rho = ctx.rho;
+  m.dt = dt;
+  for (int k = 0; k < 3; ++k) m.uOld[k] = m.u[k];
   hipLaunchKernelGGL(momentumKernel, blocksFor(v.n), dim3(kBlock), 0, ctx.stream, v, m);
   CFDHIP_CHECK(hipGetLastError());
   const unsigned bad = readAndClear(ctx);
@@ -794,6 +811,72 @@ void launchCorrect(Context& ctx) {
                      ctx.f(CFDHIP_FIELD_FLUX_STAR), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
                      ctx.f(CFDHIP_FIELD_PPRIME), ctx.f(CFDHIP_FIELD_FLUX));
   CFDHIP_CHECK(hipGetLastError());
+}
+
+// Block maxima of v over a grid-stride range (0 when empty).
+__device__ inline void blockMax(double v, double* partial) {
+  __shared__ double lds[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t = fmax(t, lds[w]);
+    partial[blockIdx.x] = t;
+  }
+}
+
+// Courant number per cell (CFL.cpp:28-42): (dt / (2 V)) (sum over the cell's
+// faces, in face-list order, of |F|, / rho); block maxima.
+__global__ void cflKernel(View v, const double* __restrict__ F, double rho, double dt,
+                          double* partial) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double cfl = 0.0;
+  if (c < v.n) {
+    int64_t idx[3];
+    cellIndex(v, c, idx);
+    double s = 0.0;
+    for (int a = 0; a < v.dim; ++a)
+      for (int side = 0; side < 2; ++side) s += fabs(F[slotOf(v, c, idx, a, side).face]);
+    cfl = (dt / (2.0 * v.vol)) * (s / rho);
+  }
+  blockMax(cfl, partial);
+}
+
+__global__ void absMaxKernel(int64_t n, const double* __restrict__ x, double* partial) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  blockMax(c < n ? fabs(x[c]) : 0.0, partial);
+}
+
+__global__ void maxFinalKernel(const double* __restrict__ partial, int64_t parts, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int64_t b = 0; b < parts; ++b) t = fmax(t, partial[b]);
+  out[0] = t;
+}
+
+// max over cells of the Courant number (FLUX) or of |IMBALANCE| (exact: a max
+// does not depend on the order).
+double launchMaxCell(Context& ctx, bool cfl, double dt) {
+  ctx.requireMesh();
+  const View& v = ctx.view;
+  const dim3 grid = blocksFor(v.n);
+  DeviceArray<double>& part = ctx.kw.partial;
+  part.ensure(static_cast<int64_t>(grid.x) + 2);
+  if (cfl)
+    hipLaunchKernelGGL(cflKernel, grid, dim3(kBlock), 0, ctx.stream, v, ctx.f(CFDHIP_FIELD_FLUX),
+                       ctx.rho, dt, part.get());
+  else
+    hipLaunchKernelGGL(absMaxKernel, grid, dim3(kBlock), 0, ctx.stream, v.n,
+                       ctx.f(CFDHIP_FIELD_IMBALANCE), part.get());
+  hipLaunchKernelGGL(maxFinalKernel, dim3(1), dim3(64), 0, ctx.stream, part.get(),
+                     static_cast<int64_t>(grid.x), part.get() + grid.x);
+  CFDHIP_CHECK(hipGetLastError());
+  double out = 0.0;
+  CFDHIP_CHECK(hipMemcpyAsync(&out, part.get() + grid.x, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  return out;
 }
 
 void launchContinuity(Context& ctx, double* out) {

@@ -90,13 +90,16 @@ void exportFaces(Context& ctx, int64_t* owner, int64_t* neighbor);
 // assembly.hip
 void launchGradient(Context& ctx, const double* phi, int bcKind, double* gx, double* gy, double* gz);
 void launchMassFlux(Context& ctx, const double* u, const double* v, const double* w, double* F);
-void launchMomentum(Context& ctx, double alpha);
+// dt > 0: the transient momentum system (TransientMomentum.cpp), alpha 1
+void launchMomentum(Context& ctx, double alpha, double dt = 0.0);
 void launchResponse(Context& ctx);
 void launchPredictedFlux(Context& ctx, bool rhieChow, double alpha);
 void launchPressureAssembly(Context& ctx, int64_t ref);
 void launchUpdatePressure(Context& ctx, double alpha);
 void launchCorrect(Context& ctx);
 void launchContinuity(Context& ctx, double* out);
+// max over cells of the Courant number of FLUX (cfl) or of |IMBALANCE|
+double launchMaxCell(Context& ctx, bool cfl, double dt);
 int64_t exportPressureCsr(Context& ctx, int64_t* rp, int64_t* col, double* val, int64_t cap);
 int64_t exportMomentumCsr(Context& ctx, int64_t* rp, int64_t* col, double* val, int64_t cap);
 unsigned countNonFinite(Context& ctx, const double* const* arrays, const int64_t* counts, int k);
@@ -145,6 +148,10 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
 void loadLearnedWeights(Context& ctx, const float* weights);
 
 // simple.hip
+// piso.hip: PISO time steps on the resident state; meta 6 per step
+int64_t pisoRun(Context& ctx, double dt, int64_t steps, int64_t ref,
+                const cfdhip_solver_settings& mom, const cfdhip_solver_settings& pres,
+                double* meta);
 void simpleRun(Context& ctx, const double* settings, const cfdhip_solver_settings& mom,
                const cfdhip_solver_settings& pres, double* meta, double* history, int64_t cap);
 

@@ -415,7 +415,15 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
   int64_t total = 0, cycles = 0;
   int status = CFDHIP_MAX_ITERATIONS;
   bool fallback = false;
+  // a cycle whose inner solve and fallback add no iteration cannot change x:
+  // without this the loop would repeat it forever
+  bool noProgress = false;
+  int lastStatus = CFDHIP_MAX_ITERATIONS;
   while (true) {
+    if (cycles > 0 && noProgress) {  // the last cycle moved nothing: stop with its verdict
+      status = lastStatus == CFDHIP_CONVERGED ? CFDHIP_BREAKDOWN : lastStatus;
+      break;
+    }
     rTrue = outerResidual();
     if (cycles == 0) b0 = rTrue;
     if (!std::isfinite(rTrue)) {
@@ -442,6 +450,8 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
                             history);
     total += ir.iterations;
     ++cycles;
+    noProgress = ir.iterations == 0;
+    lastStatus = ir.status;
     if (ir.iterations > 0) {
       if (mixed)
         hipLaunchKernelGGL((addCorrection<float>), g, dim3(256), 0, ctx.stream, n, d32, x);
@@ -469,6 +479,8 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
           ctx, fb, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x, work);
       total += fr.iterations;
       history.insert(history.end(), ctx.lastHistory.begin(), ctx.lastHistory.end());
+      if (fr.iterations > 0) noProgress = false;
+      lastStatus = fr.status;
       continue;  // the outer test decides, in FP64
     }
   }

@@ -272,7 +272,9 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
     case kStepFcgAlpha: {  // flexible CG: alpha = (p, r) / (p, q)
       const double pq = t[0], pr = t[1];
       s.pAp = pq;
-      if (!isfinite(pq) || !isfinite(pr) || fabs(pq) < 1e-30) {
+      // scale-free: with A and M positive definite (p, A p) > 0 for any p != 0,
+      // however small the residual
+      if (!isfinite(pq) || !isfinite(pr) || !(pq > 0.0)) {
         finish(s, CFDHIP_BREAKDOWN);
         return;
       }

@@ -171,6 +171,12 @@ int cfdhip_mass_flux(cfdhip_ctx* ctx);
  * (src/pressure_velocity/RelaxedMomentum.cpp:30; the u, v, w matrices are
  * identical, the right-hand sides differ). assembleMomentumResident. */
 int cfdhip_assemble_momentum(cfdhip_ctx* ctx, double alpha);
+/* assembleTransientMomentumComponent (src/pressure_velocity/TransientMomentum.cpp,
+ * effective-viscosity overload) for both components of a 2D case: the same
+ * terms from (U, P, FLUX) plus the implicit-Euler time term
+ * (TimeDerivative.cpp:13-34) of the current velocity, no relaxation. The
+ * PISO predictor's system; exported by cfdhip_momentum_csr. */
+int cfdhip_assemble_transient_momentum(cfdhip_ctx* ctx, double dt);
 /* solveMomentumResident: BiCGSTAB (or the type in settings) for one
  * component, warm-started from that component; solution into USTAR+comp. */
 int cfdhip_solve_momentum(cfdhip_ctx* ctx, int component, const cfdhip_solver_settings* s,
@@ -215,6 +221,23 @@ int cfdhip_simple_run(cfdhip_ctx* ctx, const double* settings,
 /* ---- assembled systems as CSR (the host-round-trip assemble* variants,
  *      GpuSimpleDiscretizationCuda.cpp:44-57: exact zeros dropped) ------------ */
 /* Returns nnz through *nnz; buffers sized n+1 / capacity. */
+/* PISO time steps on the resident state (U, V, P, FLUX): `steps` calls of
+ * cfd::pressure_velocity::PISO::solveTimeStep (include/cfd/pressure_velocity/
+ * PISO.hpp:89-90; src/pressure_velocity/PISO.cpp:126-350), laminar, 2D. Per
+ * step: transient momentum predictor (TransientMomentum.cpp), linear predictor
+ * flux, two pressure corrections (p += p', unrelaxed), final continuity. The
+ * reference solves both systems with BiCGSTAB whatever the settings' type
+ * (PISO.cpp:142-162); here the settings choose (BiCGSTAB reproduces it; CG +
+ * Jacobi or FCG + learned use the resident p' solvers). meta: 6 doubles per
+ * step {status (cfd::solver::TransientStepStatus: 0 Converged, 1
+ * MomentumFailure, 2 PressureCorrectionFailure, 3 NonFiniteState, 4
+ * InvalidConfiguration), maxCFL (pre-step), continuityResidual (max cell
+ * imbalance), massImbalance (|global net flux|), momentum iterations, pressure
+ * iterations}. *taken = steps run; a failed step ends the run and leaves the
+ * state of the step before it. */
+int cfdhip_piso_run(cfdhip_ctx* ctx, double dt, int64_t steps, int64_t reference_cell,
+                    const cfdhip_solver_settings* momentum, const cfdhip_solver_settings* pressure,
+                    double* meta, int64_t* taken);
 int cfdhip_pressure_csr(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
                         int64_t capacity, int64_t* nnz);
 int cfdhip_momentum_csr(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,

@@ -1,0 +1,136 @@
+"""GPU parity of the PISO time step (cfdhip_piso_run) against the reference.
+
+The fixtures (tests/golden/piso*.npz) come from the reference's own
+PISO::solveTimeStep and assembleTransientMomentumComponent
+(tests/golden/make_golden.py); the oracle reproduces them bit for bit
+(tests/test_piso_oracle.py). On the device:
+  * the transient momentum system assembled from the fixture's state is
+    bitwise equal to the reference's (CSR, right-hand sides, diagonal);
+  * each PISO step from the fixture's state, with the reference's BiCGSTAB
+    solves and tolerances (abs 1e-10, rel 1e-8), reproduces the reference's
+    velocity, pressure and flux to what those tolerances allow (1e-5 relative
+    in the max norm: the Krylov reductions are ordered differently), its CFL
+    exactly (the pre-step flux is the fixture's) and its continuity;
+  * with tight solves (rel 1e-13, the piso2d_32_tight fixture) the fields
+    agree to 1e-10 relative, the north star's bar for the FP64 fields;
+  * p' solved by the learned FCG instead (the factory route the reference's
+    PISO lacks) gives the same step to 1e-10 with rel 1e-12 solves, in about
+    an eighth of BiCGSTAB's iterations; by Jacobi-PCG to 1e-6 with an absolute
+    stop at |r| <= 1e-14 (the reference's CG breakdown test, |p.Ap| < 1e-30,
+    would end it first on these small right-hand sides);
+  * a bad dt reports InvalidConfiguration and leaves the state untouched.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("piso2d_17", 17, 17), ("piso2d_24x16_re400", 24, 16)]
+
+
+def _params(g):
+    return dict(p.split("=") for p in g["params"])
+
+
+def _case(g, nx, ny):
+    import cfdapp_amd as cfd
+    prm = _params(g)
+    re = float(prm["re"])
+    d = cfd.GpuSimpleDiscretization.cavity(2, nx, ny, viscosity=1.0 / re)
+    d.upload("U", g["in_u_x"])
+    d.upload("V", g["in_u_y"])
+    d.upload("P", g["in_p"])
+    d.upload("FLUX", g["in_F"])
+    return d, re, float(prm["dt"])
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name,nx,ny", CASES)
+def test_transient_momentum_bitwise(golden, name, nx, ny):
+    g = golden(name)
+    d, _, dt = _case(g, nx, ny)
+    try:
+        d.assemble_transient_momentum(dt)
+        rp, col, val = d.momentum_csr()
+        assert np.array_equal(rp, g["mu_rowptr"]) and np.array_equal(col, g["mu_col"])
+        assert np.array_equal(val.view(np.int64), g["mu_val"].view(np.int64))
+        # u and v share the matrix on these meshes
+        assert np.array_equal(g["mu_val"].view(np.int64), g["mv_val"].view(np.int64))
+        for f, key in (("MOM_RHS_U", "mu_rhs"), ("MOM_RHS_V", "mv_rhs"), ("MOM_DIAG", "mu_diag")):
+            assert np.array_equal(d.download(f).view(np.int64), g[key].view(np.int64)), f
+    finally:
+        d.close()
+
+
+@pytest.mark.parametrize("name,nx,ny", CASES)
+def test_piso_steps_match_reference(golden, name, nx, ny):
+    print(f"PISO {name}", flush=True)
+    g = golden(name)
+    d, _, dt = _case(g, nx, ny)
+    try:
+        steps = len(g["status"])
+        for s in range(steps):
+            r = d.piso_run(1, dt)
+            st = r["steps"][0]
+            assert st["status_name"] == "Converged"
+            sc = g["scalars"][3 * s:3 * s + 3]
+            if s == 0:  # the pre-step flux is the fixture's: exact
+                assert st["max_cfl"] == sc[0]
+            else:
+                assert st["max_cfl"] == pytest.approx(sc[0], rel=1e-7)
+            assert st["continuity"] <= 10 * max(sc[1], 1e-12)
+            u, v = d.download("U"), d.download("V")
+            p, F = d.download("P"), d.download("FLUX")
+            errs = [_rel(u, g[f"s{s}_u_x"]), _rel(v, g[f"s{s}_u_y"]), _rel(F, g[f"s{s}_F"])]
+            ep = float(np.max(np.abs(p - g[f"s{s}_p"])) / np.max(np.abs(g[f"s{s}_p"])))
+            print(f"step {s}: u {errs[0]:.1e} v {errs[1]:.1e} F {errs[2]:.1e} p {ep:.1e}",
+                  flush=True)
+            assert max(errs) < 1e-5 and ep < 1e-5
+    finally:
+        d.close()
+
+
+def _tight(kind):
+    import cfdapp_amd as cfd
+    mom = cfd.LinearSolverSettings(0.0, 1e-13, 500, "BiCGSTAB")
+    if kind == "bicgstab":
+        return mom, cfd.LinearSolverSettings(0.0, 1e-13, 20000, "BiCGSTAB")
+    if kind == "cg":  # the reference's CG breaks down on |p.Ap| < 1e-30: stop on |r| first
+        return mom, cfd.LinearSolverSettings(1e-14, 1e-12, 20000, "CG", "Jacobi")
+    return mom, cfd.LinearSolverSettings(0.0, 1e-12, 2000, "FCG", "Learned")
+
+
+@pytest.mark.parametrize("pressure,bound", [("bicgstab", 1e-10), ("cg", 1e-6), ("learned", 1e-10)])
+def test_piso_tight_solves_match_reference(golden, pressure, bound):
+    print(f"PISO tight, p' by {pressure}", flush=True)
+    g = golden("piso2d_32_tight")
+    d, _, dt = _case(g, 32, 32)
+    try:
+        if pressure == "learned":
+            d.set_learned_weights()
+        mom, ps = _tight(pressure)
+        r = d.piso_run(2, dt, momentum=mom, pressure=ps)
+        print(r["steps"], flush=True)
+        assert r["taken"] == 2 and all(s["status"] == 0 for s in r["steps"])
+        u, v, p, F = d.download("U"), d.download("V"), d.download("P"), d.download("FLUX")
+        errs = [_rel(u, g["s1_u_x"]), _rel(v, g["s1_u_y"]), _rel(F, g["s1_F"]),
+                float(np.max(np.abs(p - g["s1_p"])) / np.max(np.abs(g["s1_p"])))]
+        print("errors u v F p", [f"{e:.1e}" for e in errs], flush=True)
+        assert max(errs) < bound
+    finally:
+        d.close()
+
+
+def test_piso_invalid_dt_keeps_the_state(golden):
+    g = golden("piso2d_17")
+    d, _, _ = _case(g, 17, 17)
+    try:
+        r = d.piso_run(1, 0.0)
+        assert r["steps"][0]["status_name"] == "InvalidConfiguration"
+        assert np.array_equal(d.download("U"), g["in_u_x"])
+        assert np.array_equal(d.download("FLUX"), g["in_F"])
+    finally:
+        d.close()
