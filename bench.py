@@ -20,7 +20,8 @@ together by the batched mixed-precision Jacobi-PCG (aggregate cells x
 iterations per second, per-case p50 / p99 completion time) and one after
 another by the single-case solve; the reference's own 129^2 cavity (Re 100)
 SIMPLE iterations per second with stream-launched and graph-replayed p'
-iteration chunks; and the 3D leg: SIMPLE iterations on the
+iteration chunks; the PISO leg: time steps of a 1024^2 cavity at Re 1000 with
+the reference's BiCGSTAB p' solves and with the learned FCG; and the 3D leg: SIMPLE iterations on the
 256^3 cavity at Re = 400 with FP64 and with mixed-precision p' solves, and the
 mixed run's pressure / velocity error against the FP64 run.
 
@@ -250,6 +251,46 @@ def cavity129_leg(cfd, log, iters):
     return out
 
 
+def piso_leg(cfd, log, n, steps):
+    """PISO time steps (PISO::solveTimeStep) on an n^2 cavity at Re 1000 from
+    rest, dt = h / 2 (lid Courant number about 1): p' by the reference's
+    BiCGSTAB, then by the learned FCG; time steps per second and the
+    velocity / pressure difference between the two runs."""
+    out, fields = {"cells": n * n, "steps": steps, "dt": 0.5 / n}, {}
+    mom = cfd.LinearSolverSettings(1e-10, 1e-8, 500, "BiCGSTAB")
+    for label, ps in (("bicgstab", cfd.LinearSolverSettings(1e-10, 1e-8, 40000, "BiCGSTAB")),
+                      ("learned", cfd.LinearSolverSettings(1e-10, 1e-8, 20000, "FCG",
+                                                           "Learned"))):
+        d = cfd.GpuSimpleDiscretization.cavity(2, n, n, viscosity=1e-3)
+        try:
+            if label == "learned":
+                d.set_learned_weights()
+            d._ctx.call("cfdhip_mass_flux")
+            d._ctx.call("cfdhip_synchronize")
+            t0 = time.perf_counter()
+            r = d.piso_run(steps, 0.5 / n, momentum=mom, pressure=ps)
+            dt = time.perf_counter() - t0
+            if r["taken"] != steps or any(s["status"] != 0 for s in r["steps"]):
+                raise RuntimeError(f"PISO ({label}) failed: {r['steps'][-1]['status_name']}")
+            fields[label] = (d.download("P"), d.download_velocity())
+            out[f"{label}_steps_per_s"] = steps / dt
+            out[f"{label}_pressure_iterations"] = sum(s["pressure_iterations"] for s in r["steps"])
+            out[f"{label}_max_cfl"] = r["steps"][-1]["max_cfl"]
+        finally:
+            d.close()
+    pb, ub = fields["bicgstab"]
+    pl, ul = fields["learned"]
+    out["speedup_learned_vs_bicgstab"] = out["learned_steps_per_s"] / out["bicgstab_steps_per_s"]
+    out["p_rel_err_inf"] = float(np.max(np.abs(pl - pb)) / np.max(np.abs(pb)))
+    umax = max(float(np.max(np.abs(c))) for c in ub)
+    out["u_rel_err_inf"] = max(float(np.max(np.abs(a - b))) for a, b in zip(ul, ub)) / umax
+    log(f"PISO {n}^2 Re 1000, {steps} steps: BiCGSTAB p' {out['bicgstab_steps_per_s']:.2f} "
+        f"steps/s ({out['bicgstab_pressure_iterations']} p' it), learned FCG "
+        f"{out['learned_steps_per_s']:.2f} steps/s ({out['learned_pressure_iterations']} p' it); "
+        f"|dp|/|p| {out['p_rel_err_inf']:.1e}, |du|/|u| {out['u_rel_err_inf']:.1e}")
+    return out
+
+
 def mixed3d_leg(cfd, log, n, iters):
     """3D lid-driven cavity n^3 at Re = 400 (mu = 0.0025, lid ymax (1, 0, 0),
     Rhie-Chow): `iters` SIMPLE iterations with FP64 Jacobi-PCG p' solves, then
@@ -296,6 +337,9 @@ def main():
     ap.add_argument("--cube", type=int, default=256, help="3D mixed-precision leg: cells per side")
     ap.add_argument("--cube-iters", type=int, default=2,
                     help="SIMPLE iterations of the 3D leg (0 skips it)")
+    ap.add_argument("--piso-n", type=int, default=1024, help="PISO leg: cells per side")
+    ap.add_argument("--piso-steps", type=int, default=3,
+                    help="PISO time steps of the PISO leg (0 skips it)")
     ap.add_argument("--batch-cases", type=int, default=256,
                     help="cases of the batched-solve leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -500,6 +544,10 @@ def main():
     if args.c129_iters > 0:
         c129 = cavity129_leg(cfd, log, args.c129_iters)
 
+    piso = {}
+    if args.piso_steps > 0:
+        piso = piso_leg(cfd, log, args.piso_n, args.piso_steps)
+
     cube = {}
     if args.cube_iters > 0:
         cube = mixed3d_leg(cfd, log, args.cube, args.cube_iters)
@@ -544,6 +592,7 @@ def main():
                    "plain_spmv": spmv,
                    **mixed, **mixed_fcg, **simple,
                    "cavity129_simple": c129,
+                   "piso": piso,
                    "mixed_precision_3d": cube,
                    "batched_pprime": batched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

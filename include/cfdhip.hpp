@@ -212,6 +212,24 @@ struct CartesianMesh {
 
 // The resident SIMPLE stages (GpuSimpleDiscretization): pimpl over one context,
 // non-copyable, movable; prepare() is all-or-nothing with a reason.
+// cfd::solver::TransientStepStatus / TransientStepResult
+// (include/cfd/solver/TransientSolver.hpp:29-53) without the state, which
+// stays resident (download it with GpuSimpleDiscretization::download).
+enum class TransientStepStatus {
+  Converged,
+  MomentumFailure,
+  PressureCorrectionFailure,
+  NonFiniteState,
+  InvalidConfiguration,
+};
+struct TransientStepResult {
+  TransientStepStatus status{TransientStepStatus::InvalidConfiguration};
+  double maxCFL{};
+  double continuityResidual{};
+  double massImbalance{};
+  std::size_t momentumIterations{}, pressureIterations{};
+};
+
 class GpuSimpleDiscretization {
  public:
   GpuSimpleDiscretization() = default;
@@ -293,6 +311,27 @@ class GpuSimpleDiscretization {
     return detail::fromC(r, ctx());
   }
   void updatePressure(double alpha) { detail::check(ctx(), cfdhip_update_pressure(ctx(), alpha)); }
+  // PISO::solveTimeStep (include/cfd/pressure_velocity/PISO.hpp:89-90) on the
+  // resident state (U, V, P, FLUX), 2D; the solvers' types are honoured (the
+  // reference always uses BiCGSTAB). A failed step leaves the previous state.
+  [[nodiscard]] TransientStepResult solveTimeStepResident(double dt,
+                                                          const LinearSolverSettings& momentum,
+                                                          const LinearSolverSettings& pressure,
+                                                          std::size_t referenceCell = 0) {
+    const cfdhip_solver_settings m = momentum.c(), p = pressure.c();
+    double meta[6] = {};
+    int64_t taken = 0;
+    detail::check(ctx(), cfdhip_piso_run(ctx(), dt, 1, static_cast<int64_t>(referenceCell), &m,
+                                         &p, meta, &taken));
+    TransientStepResult r;
+    r.status = static_cast<TransientStepStatus>(static_cast<int>(meta[0]));
+    r.maxCFL = meta[1];
+    r.continuityResidual = meta[2];
+    r.massImbalance = meta[3];
+    r.momentumIterations = static_cast<std::size_t>(meta[4]);
+    r.pressureIterations = static_cast<std::size_t>(meta[5]);
+    return r;
+  }
   void correctVelocityAndFluxResident() { detail::check(ctx(), cfdhip_correct(ctx())); }
   // out[0] = RMS imbalance, out[1] = global net boundary flux
   std::pair<double, double> continuity() {

@@ -129,6 +129,19 @@ void device() {
   double umax = 0.0;
   for (double x : u) umax = std::fmax(umax, std::fabs(x));
   EXPECT(umax > 0.01 && umax < 1.0);
+
+  // PISO time steps from this state (the reference's BiCGSTAB for both systems)
+  cfdhip::LinearSolverSettings bicg;
+  bicg.absoluteTolerance = 1e-10;
+  bicg.relativeTolerance = 1e-8;
+  bicg.maxIterations = 2000;
+  for (int s = 0; s < 3; ++s) {
+    const auto t = d.solveTimeStepResident(0.01, mom, bicg);
+    EXPECT(t.status == cfdhip::TransientStepStatus::Converged);
+    EXPECT(t.maxCFL > 0.0 && t.continuityResidual < 1e-6 && t.pressureIterations > 0);
+  }
+  const auto bad = d.solveTimeStepResident(-1.0, mom, bicg);
+  EXPECT(bad.status == cfdhip::TransientStepStatus::InvalidConfiguration);
 }
 
 }  // namespace
