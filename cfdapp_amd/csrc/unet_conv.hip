@@ -85,7 +85,11 @@ __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, co
   constexpr int C4 = CC / 4;
   constexpr int ITEMS = ROWS * COLS * C4;
   constexpr int PER = (ITEMS + 255) / 256;
+  static_assert(PER <= 32, "one sign bit per item");
   fx4 v[PER];
+  uint32_t neg = 0;  // items read through a Dirichlet side: negated at the store
+  // a window inside the image (uniform over the workgroup) skips the padding
+  const bool inside = y0 >= 0 && x0 >= 0 && y0 + ROWS <= h && x0 + COLS <= w;
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int it = threadIdx.x + k * 256;
@@ -95,8 +99,9 @@ __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, co
       const int px = it / C4;
       const int ty = px / COLS, tx = px - ty * COLS;
       int y = y0 + ty, x = x0 + tx;
-      const float sg = padSign(y, x, h, w, pad);
-      if (sg != 0.f) v[k] = sg * gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+      const float sg = inside ? 1.f : padSign(y, x, h, w, pad);
+      if (sg != 0.f) v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+      if (sg < 0.f) neg |= 1u << k;
     }
   }
 #pragma unroll
@@ -111,15 +116,17 @@ __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, co
       hb[q] = f2bf(v[k][q]);
       lb[q] = f2bf(v[k][q] - bf2f(hb[q]));
     }
+    // bf16(-x) = -bf16(x) and the split of -x is the negated split: flip the signs
+    const uint32_t flip = (neg >> k & 1u) ? 0x80008000u : 0u;
     u32x2 vh;
-    vh[0] = static_cast<uint32_t>(hb[0]) | (static_cast<uint32_t>(hb[1]) << 16);
-    vh[1] = static_cast<uint32_t>(hb[2]) | (static_cast<uint32_t>(hb[3]) << 16);
+    vh[0] = (static_cast<uint32_t>(hb[0]) | (static_cast<uint32_t>(hb[1]) << 16)) ^ flip;
+    vh[1] = (static_cast<uint32_t>(hb[2]) | (static_cast<uint32_t>(hb[3]) << 16)) ^ flip;
     const int o = px * cs + c4 * 4;
     *reinterpret_cast<u32x2*>(hi + o) = vh;
     if (lo != nullptr) {
       u32x2 vl;
-      vl[0] = static_cast<uint32_t>(lb[0]) | (static_cast<uint32_t>(lb[1]) << 16);
-      vl[1] = static_cast<uint32_t>(lb[2]) | (static_cast<uint32_t>(lb[3]) << 16);
+      vl[0] = (static_cast<uint32_t>(lb[0]) | (static_cast<uint32_t>(lb[1]) << 16)) ^ flip;
+      vl[1] = (static_cast<uint32_t>(lb[2]) | (static_cast<uint32_t>(lb[3]) << 16)) ^ flip;
       *reinterpret_cast<u32x2*>(lo + o) = vl;
     }
   }
@@ -127,6 +134,9 @@ __device__ inline void stageSplit(bf16* hi, bf16* lo, int cs, const float* A, co
 
 // The two halves of stageSplit for software pipelining: loadChunk issues a
 // chunk's loads into registers, storeChunk converts and writes them to LDS.
+// Pixels read through a Dirichlet side are negated afterwards in LDS by
+// flipSides, on the windows that touch such a side only, so that no sign
+// state is held in registers across the multiply.
 template <int C, int CA, int G, int CC, int ROWS, int COLS>
 struct ChunkStage {
   static constexpr int C4 = CC / 4;
@@ -144,8 +154,8 @@ struct ChunkStage {
         const int px = it / C4;
         const int ty = px / COLS, tx = px - ty * COLS;
         int y = y0 + ty, x = x0 + tx;
-        const float sg = padSign(y, x, h, w, pad);
-      if (sg != 0.f) v[k] = sg * gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
+        if (padSign(y, x, h, w, pad) != 0.f)
+          v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
       }
     }
   }
@@ -170,6 +180,38 @@ struct ChunkStage {
       const int o = px * cs + c4 * 4;
       *reinterpret_cast<u32x2*>(hi + o) = vh;
       *reinterpret_cast<u32x2*>(lo + o) = vl;
+    }
+  }
+  // Whether the window (y0, x0) of an h x w image reads through a Dirichlet
+  // side (uniform over the workgroup).
+  __device__ static inline bool hasSigns(int h, int w, int y0, int x0, int pad) {
+    return (pad & kPadNeumann) && (pad >> 1) != 0 &&
+           !(y0 >= 0 && x0 >= 0 && y0 + ROWS <= h && x0 + COLS <= w);
+  }
+  // Negates the staged items of this thread that were read through a
+  // Dirichlet side: bf16(-x) = -bf16(x) and the split of -x is the negated
+  // split, so the sign bits of both halves flip. Same item-to-thread map as
+  // store(), hence no barrier between the two.
+  __device__ static void flipSides(bf16* hi, bf16* lo, int cs, int h, int w, int y0, int x0,
+                                   int pad) {
+#pragma unroll 1
+    for (int it = threadIdx.x; it < ITEMS; it += 256) {
+      const int c4 = it % C4;
+      const int px = it / C4;
+      const int ty = px / COLS, tx = px - ty * COLS;
+      int y = y0 + ty, x = x0 + tx;
+      if (padSign(y, x, h, w, pad) < 0.f) {
+        const int o = px * cs + c4 * 4;
+        u32x2* ph = reinterpret_cast<u32x2*>(hi + o);
+        u32x2* pl = reinterpret_cast<u32x2*>(lo + o);
+        u32x2 th = *ph, tl = *pl;
+        th[0] ^= 0x80008000u;
+        th[1] ^= 0x80008000u;
+        tl[0] ^= 0x80008000u;
+        tl[1] ^= 0x80008000u;
+        *ph = th;
+        *pl = tl;
+      }
     }
   }
 };
@@ -361,8 +403,13 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
         stageFirst<float>(hiT, loT, CS, a, n, y0 - 1, x0 - 1);
     } else if (kConcat && ch < NCO) {
       coarseBuf.store(hiT, loT, CS);
+      const int ch2 = a.h >> 1, cw2 = a.w_ >> 1, cy = (y0 >> 1) - 1, cx = (x0 >> 1) - 1;
+      if (decltype(coarseBuf)::hasSigns(ch2, cw2, cy, cx, a.pad))
+        decltype(coarseBuf)::flipSides(hiT, loT, CS, ch2, cw2, cy, cx, a.pad);
     } else {
       stageBuf.store(hiT, loT, CS);
+      if (decltype(stageBuf)::hasSigns(a.h, a.w_, y0 - 1, x0 - 1, a.pad))
+        decltype(stageBuf)::flipSides(hiT, loT, CS, a.h, a.w_, y0 - 1, x0 - 1, a.pad);
     }
     __syncthreads();
     if constexpr (G != kGatherFirst) {

@@ -2,15 +2,15 @@
 // one-channel symmetric V(1,1) cycle below the network's bottom level, with
 // Neumann padding, so that the preconditioner reaches the coarsest grid.
 //
-// Three launches per application: a gather of the input channel of the
-// bottom tensor into a compact image (all pixels in parallel), the cycle
-// itself (one workgroup per image walks every tail level, a workgroup
-// barrier between dependent passes; the levels are at most H/16 x W/16 and
-// stay in L2), and a scatter of the prolonged correction K up(e_0) into the
-// output channel. The operator is symmetric (restriction 4 pool(K .) is the
-// transpose of prolongation K up(.), the same Jacobi sweep before and after
-// the coarse correction), so the backward pass runs the same three launches
-// from the output channel's gradient into the input channel's.
+// Launches per application: the first restriction of the input channel of
+// the bottom tensor (all coarse pixels in parallel, read in place); the
+// passes of the large levels, each over the whole chip; the small levels in
+// one workgroup per image (a workgroup barrier between dependent passes;
+// the levels stay in L2); and a scatter of the prolonged correction
+// K up(e_0) into the output channel. The operator is symmetric (restriction
+// 4 pool(K .) is the transpose of prolongation K up(.), the same Jacobi sweep
+// before and after the coarse correction), so the backward pass runs the
+// same launches from the output channel's gradient into the input channel's.
 #include "unet_net.hpp"
 
 namespace cfdhip {
@@ -18,18 +18,20 @@ namespace cfdhip {
 namespace {
 
 constexpr int kTailThreads = 1024;
+// a tail level with more pixels than this runs outside the one-workgroup cycle
+constexpr int kTailCyclePixels = 4096;
 
 dim3 tailGrid(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return dim3(static_cast<unsigned>(b < 1 ? 1 : (b > 8192 ? 8192 : b)));
 }
 
-// One image's tail workspace: the compact bottom channel, then per level t
-// the residual R_t, a work image S_t and the correction E_t.
+// One image's tail workspace: per level t the residual R_t, a work image
+// S_t and the correction E_t.
 struct TailShape {
   int h3, w3, levels;
   int64_t perImage;
-  int64_t off[3 * 16];  // R_t, S_t, E_t offsets (after the compact image)
+  int64_t off[3 * 16];  // R_t, S_t, E_t offsets
 };
 
 __host__ __device__ inline int tailH(const TailShape& s, int t) { return s.h3 >> (t + 1); }
@@ -46,7 +48,7 @@ TailShape tailShape(int h3, int w3, int maxLevels) {
     ++L;
   }
   s.levels = L;
-  int64_t o = static_cast<int64_t>(h3) * w3;
+  int64_t o = 0;
   for (int t = 0; t < L; ++t) {
     const int64_t m = static_cast<int64_t>(h3 >> (t + 1)) * (w3 >> (t + 1));
     for (int q = 0; q < 3; ++q) {
@@ -58,31 +60,70 @@ TailShape tailShape(int h3, int w3, int maxLevels) {
   return s;
 }
 
-// Pixel (y, x) of an h x w image with the network's padding (padSign).
-__device__ inline float at(const float* img, int h, int w, int y, int x, int pad) {
+// Pixel (y, x) of an h x w image (pixel stride cs) with the network's
+// padding (padSign).
+__device__ inline float at(const float* img, int h, int w, int y, int x, int pad, int cs = 1) {
   const float s = padSign(y, x, h, w, pad);
-  return s * img[static_cast<int64_t>(y) * w + x];
+  return s * img[(static_cast<int64_t>(y) * w + x) * cs];
 }
 
-// Symmetric 3 x 3 stencil (centre, edge, corner) at (y, x).
+// The symmetric 3 x 3 stencil (centre, edge, corner) over a 3 x 3 patch.
+__device__ inline float stencilPatch(const float (&q)[3][3], float c, float e, float k) {
+  const float edges = (q[0][1] + q[2][1]) + (q[1][0] + q[1][2]);
+  const float corners = (q[0][0] + q[0][2]) + (q[2][0] + q[2][2]);
+  return c * q[1][1] + e * edges + k * corners;
+}
+
+// Symmetric 3 x 3 stencil at (y, x); pixels off the image through the
+// padding, interior pixels by direct loads.
 __device__ inline float stencil3(const float* img, int h, int w, int y, int x, float c, float e,
                                  float k, int pad) {
-  const float edges = (at(img, h, w, y - 1, x, pad) + at(img, h, w, y + 1, x, pad)) +
-                      (at(img, h, w, y, x - 1, pad) + at(img, h, w, y, x + 1, pad));
-  const float corners = (at(img, h, w, y - 1, x - 1, pad) + at(img, h, w, y - 1, x + 1, pad)) +
-                        (at(img, h, w, y + 1, x - 1, pad) + at(img, h, w, y + 1, x + 1, pad));
-  return c * img[static_cast<int64_t>(y) * w + x] + e * edges + k * corners;
+  float q[3][3];
+  if (y > 0 && y + 1 < h && x > 0 && x + 1 < w) {
+    const float* r = img + static_cast<int64_t>(y - 1) * w + (x - 1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) q[a][b] = r[a * w + b];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) q[a][b] = at(img, h, w, y - 1 + a, x - 1 + b, pad);
+  }
+  return stencilPatch(q, c, e, k);
 }
 
-// 4 pool(K v) at coarse pixel (y, x) of v (fine h x w).
+// 4 pool(K v) at coarse pixel (y, x) of v (fine h x w, pixel stride cs):
+// the four stencils of the 2 x 2 block read from one 4 x 4 patch.
 __device__ inline float restrictK(const float* v, int h, int w, int y, int x, const TailParams& p,
-                                  int pad) {
+                                  int pad, int cs = 1) {
+  float t[4][4];
+  const int y0 = 2 * y - 1, x0 = 2 * x - 1;
+  if (y0 >= 0 && y0 + 3 < h && x0 >= 0 && x0 + 3 < w) {
+    const float* r = v + (static_cast<int64_t>(y0) * w + x0) * cs;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) t[a][b] = r[static_cast<int64_t>(a * w + b) * cs];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) t[a][b] = at(v, h, w, y0 + a, x0 + b, pad, cs);
+  }
   float s = 0.f;
 #pragma unroll
   for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
-    for (int dx = 0; dx < 2; ++dx)
-      s += stencil3(v, h, w, 2 * y + dy, 2 * x + dx, p.k[0], p.k[1], p.k[2], pad);
+    for (int dx = 0; dx < 2; ++dx) {
+      float q[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) q[a][b] = t[dy + a][dx + b];
+      s += stencilPatch(q, p.k[0], p.k[1], p.k[2]);
+    }
   return s;  // 4 x the 2 x 2 average
 }
 
@@ -90,75 +131,120 @@ __device__ inline float restrictK(const float* v, int h, int w, int y, int x, co
 __device__ inline float prolongK(const float* e, int h, int w, int y, int x, const TailParams& p,
                                  int pad) {
   const int wc = w >> 1;
-  auto up = [&](int yy, int xx) {  // the upsampled image with the padding
-    const float s = padSign(yy, xx, h, w, pad);
-    return s * e[static_cast<int64_t>(yy >> 1) * wc + (xx >> 1)];
-  };
-  const float edges = (up(y - 1, x) + up(y + 1, x)) + (up(y, x - 1) + up(y, x + 1));
-  const float corners = (up(y - 1, x - 1) + up(y - 1, x + 1)) + (up(y + 1, x - 1) + up(y + 1, x + 1));
-  return p.k[0] * up(y, x) + p.k[1] * edges + p.k[2] * corners;
+  float q[3][3];
+  if (y > 0 && y + 1 < h && x > 0 && x + 1 < w) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        q[a][b] = e[static_cast<int64_t>((y - 1 + a) >> 1) * wc + ((x - 1 + b) >> 1)];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        int yy = y - 1 + a, xx = x - 1 + b;  // the upsampled image with the padding
+        const float sg = padSign(yy, xx, h, w, pad);
+        q[a][b] = sg * e[static_cast<int64_t>(yy >> 1) * wc + (xx >> 1)];
+      }
+  }
+  return stencilPatch(q, p.k[0], p.k[1], p.k[2]);
 }
 
-__global__ __launch_bounds__(256) void tailGatherKernel(const float* __restrict__ t, int C, int ch,
-                                                        int64_t pix, int64_t perImage, int64_t hw,
-                                                        float* __restrict__ work) {
-  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < pix;
-       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t b = p / hw, q = p - b * hw;
-    work[b * perImage + q] = t[p * C + ch];
+// R_0 = 4 pool(K x) of the input channel, all coarse pixels of all images in
+// parallel, read in place from the C-channel bottom tensor.
+__global__ __launch_bounds__(256) void tailRestrictKernel(const float* __restrict__ t, int C, int ch,
+                                                          TailShape s, TailParams p, int pad,
+                                                          int64_t pix, float* __restrict__ work) {
+  const int h0 = tailH(s, 0), w0 = tailW(s, 0);
+  const int64_t hw0 = static_cast<int64_t>(h0) * w0, hw = static_cast<int64_t>(s.h3) * s.w3;
+  for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < pix;
+       q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t b = q / hw0, i = q - b * hw0;
+    const float* img = t + b * hw * C + ch;
+    work[b * s.perImage + s.off[0] + i] =
+        restrictK(img, s.h3, s.w3, static_cast<int>(i / w0), static_cast<int>(i % w0), p, pad, C);
   }
 }
 
+// The four passes of a tail level t, at pixel i of its output image:
+//   kPre:      S_t = R_t - omega A R_t (the pre-sweep from zero);
+//   kRestrict: R_{t+1} = 4 pool(K S_t) (i on level t+1);
+//   kProlong:  S_t = omega R_t + K up(E_{t+1});
+//   kPost:     E_t = S_t - omega A S_t + omega R_t (the post-sweep).
+enum TailPass { kPre = 0, kRestrict = 1, kProlong = 2, kPost = 3 };
+
+__device__ inline void tailPass(float* base, const TailShape& s, const TailParams& p, int pad,
+                                int t, int pass, int i) {
+  const float om = p.omega;
+  float* R = base + s.off[3 * t];
+  float* S = base + s.off[3 * t + 1];
+  float* E = base + s.off[3 * t + 2];
+  const int h = tailH(s, t), w = tailW(s, t);
+  if (pass == kRestrict) {
+    const int wc = tailW(s, t + 1);
+    base[s.off[3 * (t + 1)] + i] = restrictK(S, h, w, i / wc, i % wc, p, pad);
+    return;
+  }
+  const int y = i / w, x = i % w;
+  if (pass == kPre) {
+    S[i] = R[i] - om * stencil3(R, h, w, y, x, p.a[0], p.a[1], p.a[2], pad);
+  } else if (pass == kProlong) {
+    S[i] = om * R[i] + prolongK(base + s.off[3 * (t + 1) + 2], h, w, y, x, p, pad);
+  } else {
+    E[i] = S[i] - om * stencil3(S, h, w, y, x, p.a[0], p.a[1], p.a[2], pad) + om * R[i];
+  }
+}
+
+__host__ __device__ inline int tailPassPixels(const TailShape& s, int t, int pass) {
+  const int u = pass == kRestrict ? t + 1 : t;
+  return tailH(s, u) * tailW(s, u);
+}
+
+// One pass of a large level over all its pixels of all images.
+__global__ __launch_bounds__(256) void tailLevelKernel(TailShape s, TailParams p, int pad, int t,
+                                                       int pass, int64_t pix,
+                                                       float* __restrict__ work) {
+  const int64_t m = tailPassPixels(s, t, pass);
+  for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < pix;
+       q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t b = q / m;
+    tailPass(work + b * s.perImage, s, p, pad, t, pass, static_cast<int>(q - b * m));
+  }
+}
+
+// Levels t0 .. L-1 of one image per workgroup: down from R_t0, the coarsest
+// level, up to E_t0.
 __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, TailParams p, int pad,
-                                                               float* __restrict__ work) {
+                                                               int t0, float* __restrict__ work) {
   float* base = work + static_cast<int64_t>(blockIdx.x) * s.perImage;
   const int L = s.levels;
-  const float om = p.omega;
-  auto R = [&](int t) { return base + s.off[3 * t]; };
-  auto S = [&](int t) { return base + s.off[3 * t + 1]; };
-  auto E = [&](int t) { return base + s.off[3 * t + 2]; };
-  // level 0 residual: restriction of the bottom channel
-  {
-    const int h = tailH(s, 0), w = tailW(s, 0);
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x)
-      R(0)[i] = restrictK(base, s.h3, s.w3, i / w, i % w, p, pad);
-  }
-  __syncthreads();
-  // down: one Jacobi sweep from zero, restriction of the remaining residual
-  for (int t = 0; t + 1 < L; ++t) {
-    const int h = tailH(s, t), w = tailW(s, t);
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-      const int y = i / w, x = i % w;
-      S(t)[i] = R(t)[i] - om * stencil3(R(t), h, w, y, x, p.a[0], p.a[1], p.a[2], pad);
+  for (int t = t0; t + 1 < L; ++t) {
+    for (int pass = kPre; pass <= kRestrict; ++pass) {
+      const int m = tailPassPixels(s, t, pass);
+      for (int i = threadIdx.x; i < m; i += blockDim.x) tailPass(base, s, p, pad, t, pass, i);
+      __syncthreads();
     }
-    __syncthreads();
-    const int hc = tailH(s, t + 1), wc = tailW(s, t + 1);
-    for (int i = threadIdx.x; i < hc * wc; i += blockDim.x)
-      R(t + 1)[i] = restrictK(S(t), h, w, i / wc, i % wc, p, pad);
-    __syncthreads();
   }
   // coarsest level: the two sweeps without a coarse correction,
   // e = omega r + (I - omega A) omega r
   {
+    const float om = p.omega;
+    const float* R = base + s.off[3 * (L - 1)];
+    float* E = base + s.off[3 * (L - 1) + 2];
     const int h = tailH(s, L - 1), w = tailW(s, L - 1);
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-      const float ar = stencil3(R(L - 1), h, w, i / w, i % w, p.a[0], p.a[1], p.a[2], pad);
-      E(L - 1)[i] = 2.f * om * R(L - 1)[i] - om * om * ar;
+      const float ar = stencil3(R, h, w, i / w, i % w, p.a[0], p.a[1], p.a[2], pad);
+      E[i] = 2.f * om * R[i] - om * om * ar;
     }
   }
   __syncthreads();
-  // up: coarse correction, then the post-sweep
-  for (int t = L - 2; t >= 0; --t) {
-    const int h = tailH(s, t), w = tailW(s, t);
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x)
-      S(t)[i] = om * R(t)[i] + prolongK(E(t + 1), h, w, i / w, i % w, p, pad);
-    __syncthreads();
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-      const int y = i / w, x = i % w;
-      const float v = S(t)[i];
-      E(t)[i] = v - om * stencil3(S(t), h, w, y, x, p.a[0], p.a[1], p.a[2], pad) + om * R(t)[i];
+  for (int t = L - 2; t >= t0; --t) {
+    for (int pass = kProlong; pass <= kPost; ++pass) {
+      const int m = tailPassPixels(s, t, pass);
+      for (int i = threadIdx.x; i < m; i += blockDim.x) tailPass(base, s, p, pad, t, pass, i);
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -214,10 +300,28 @@ void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward) {
   const int outCh = backward ? tail.src : tail.dst;
   if (s.levels > 0) {
     tailWork.ensure(static_cast<int64_t>(n) * s.perImage);
-    hipLaunchKernelGGL(tailGatherKernel, tailGrid(pix), dim3(256), 0, stream, tensor, 64, inCh, pix,
-                       s.perImage, hw, tailWork.get());
+    const int64_t pix0 = static_cast<int64_t>(n) * tailH(s, 0) * tailW(s, 0);
+    hipLaunchKernelGGL(tailRestrictKernel, tailGrid(pix0), dim3(256), 0, stream, tensor, 64, inCh, s,
+                       tail, padBits(), pix0, tailWork.get());
+    // levels above kTailCyclePixels run pass by pass over the whole chip; the
+    // rest in the one-workgroup-per-image cycle
+    int t0 = 0;
+    while (t0 + 1 < s.levels && tailH(s, t0) * tailW(s, t0) > kTailCyclePixels) ++t0;
+    auto level = [&](int t, int pass) {
+      const int64_t m = static_cast<int64_t>(n) * tailPassPixels(s, t, pass);
+      hipLaunchKernelGGL(tailLevelKernel, tailGrid(m), dim3(256), 0, stream, s, tail, padBits(), t,
+                         pass, m, tailWork.get());
+    };
+    for (int t = 0; t < t0; ++t) {
+      level(t, kPre);
+      level(t, kRestrict);
+    }
     hipLaunchKernelGGL(tailCycleKernel, dim3(static_cast<unsigned>(n)), dim3(kTailThreads), 0,
-                       stream, s, tail, padBits(), tailWork.get());
+                       stream, s, tail, padBits(), t0, tailWork.get());
+    for (int t = t0 - 1; t >= 0; --t) {
+      level(t, kProlong);
+      level(t, kPost);
+    }
   }
   hipLaunchKernelGGL(tailScatterKernel, tailGrid(pix), dim3(256), 0, stream, tensor, 64, outCh, inCh,
                      backward ? 1 : 0, s, tail, padBits(), tailWork.get(), pix);
