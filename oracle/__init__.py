@@ -51,6 +51,7 @@ def lib():
         _LIB.orc_momentum_transient.restype = ctypes.c_int64
         _LIB.orc_piso.restype = ctypes.c_int64
         _LIB.orc_pressure_assemble.restype = ctypes.c_int64
+        _LIB.orc_pressure_compressible.restype = ctypes.c_int64
         _LIB.orc_case_set_bc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_double]
@@ -270,6 +271,24 @@ class Case:
                                           _p(f64(dv)), _p(dw), ctypes.c_double(rho),
                                           ctypes.c_int64(ref), _p(rp), _p(col), _p(val), _p(rhs),
                                           _p(coeff))
+        return Csr(rp, col[:nnz], val[:nnz]), rhs, coeff
+
+    def compressible_pressure_system(self, Fstar, du, face_density, p_abs, T, gas_constant,
+                                     pseudo_dt, ref=0):
+        """assembleCompressiblePressureCorrection (2D; du serves both components).
+        None where the reference throws InvalidArgumentError."""
+        n = self.n_cells
+        cap = 7 * n + 16
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(cap, np.int64)
+        val = np.zeros(cap)
+        rhs, coeff = np.zeros(n), np.zeros(self.n_faces)
+        nnz = lib().orc_pressure_compressible(
+            ctypes.c_void_p(self.h), _p(f64(Fstar)), _p(f64(du)), _p(f64(face_density)),
+            _p(f64(p_abs)), _p(f64(T)), ctypes.c_double(gas_constant), ctypes.c_double(pseudo_dt),
+            ctypes.c_int64(ref), _p(rp), _p(col), _p(val), _p(rhs), _p(coeff))
+        if nnz < 0:
+            return None
         return Csr(rp, col[:nnz], val[:nnz]), rhs, coeff
 
     def correct(self, ustar, du, dv, dw, pp, Fstar, coeff):

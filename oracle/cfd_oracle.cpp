@@ -1097,13 +1097,15 @@ struct PressureSystem {
   Csr A;
   std::vector<double> rhs, faceCoeff;
 };
-// assembleGeometricPressureCorrection (PressureCorrectionEquation.cpp:181-339) via
-// assemblePressureCorrection (:341-367): constant density, no additional diagonal,
-// no non-orthogonal pass.
+// assembleGeometricPressureCorrection (PressureCorrectionEquation.cpp:181-339), no
+// non-orthogonal pass: via assemblePressureCorrection (:341-367) with a constant
+// density, or with a per-face density (rhoF) and an additional diagonal added
+// after the face terms (extra, :304-310) for the compressible equation.
 PressureSystem assemblePressure(const Mesh& m, const std::vector<double>& Fstar,
                                 const std::vector<double>& dU, const std::vector<double>& dV,
                                 const std::vector<double>* dW, double rho, Index ref,
-                                const Boundaries& b) {
+                                const Boundaries& b, const std::vector<double>* rhoF = nullptr,
+                                const std::vector<double>* extra = nullptr) {
   const Index n = m.nCells();
   bool open = false;
   for (size_t p = 0; p < m.patches.size(); ++p)
@@ -1116,12 +1118,12 @@ PressureSystem assemblePressure(const Mesh& m, const std::vector<double>& Fstar,
     const Index o = m.owner[f];
     if (m.boundary(f)) {
       if (b.pkind[m.facePatch[f]] != kFixedValue) continue;
-      const double c = faceCoupling(m, f, rho, dU, dV, dW);
+      const double c = faceCoupling(m, f, rhoF ? (*rhoF)[f] : rho, dU, dV, dW);
       out.faceCoeff[f] = c;
       if (!pin || o != ref) B.add(o, o, c);
       continue;
     }
-    const double c = faceCoupling(m, f, rho, dU, dV, dW);
+    const double c = faceCoupling(m, f, rhoF ? (*rhoF)[f] : rho, dU, dV, dW);
     out.faceCoeff[f] = c;
     const Index nb = m.neigh[f];
     if (!pin || o != ref) {
@@ -1136,6 +1138,9 @@ PressureSystem assemblePressure(const Mesh& m, const std::vector<double>& Fstar,
   const Continuity cont = continuity(m, Fstar);
   out.rhs.resize(n);
   for (Index c = 0; c < n; ++c) out.rhs[c] = -cont.imbalance[c];
+  if (extra != nullptr)
+    for (Index c = 0; c < n; ++c)
+      if (!pin || c != ref) B.add(c, c, (*extra)[c]);
   if (pin) {
     B.add(ref, ref, 1.0);
     out.rhs[ref] = 0.0;
@@ -1689,6 +1694,35 @@ int64_t orc_pressure_assemble(void* h, const double* Fstar, const double* du, co
   if (m.dim == 3) dW.assign(dw, dw + n);
   const PressureSystem ps = assemblePressure(m, std::vector<double>(Fstar, Fstar + m.nFaces()), dU,
                                              dV, m.dim == 3 ? &dW : nullptr, rho, ref, c->bc);
+  std::copy(ps.rhs.begin(), ps.rhs.end(), rhs);
+  std::copy(ps.faceCoeff.begin(), ps.faceCoeff.end(), faceCoeff);
+  return csrOut(ps.A, rp, col, val);
+}
+// assembleCompressiblePressureCorrection
+// (src/compressible/CompressiblePressureCorrection.cpp:20-80): the compressibility
+// diagonal V / dtau * drho/dp per cell, with the ideal gas drho/dp = 1 / (R T)
+// (IdealGasEOS.cpp:35-39), then the geometric assembly with the per-face density.
+// Returns -1 where the reference throws InvalidArgumentError (p or T not finite
+// and > 0, R or dtau not finite and > 0).
+int64_t orc_pressure_compressible(void* h, const double* Fstar, const double* du,
+                                  const double* rhoF, const double* pAbs, const double* T,
+                                  double gasConstant, double dtau, int64_t ref, int64_t* rp,
+                                  int64_t* col, double* val, double* rhs, double* faceCoeff) {
+  auto* c = static_cast<Case*>(h);
+  const Mesh& m = c->mesh;
+  const Index n = m.nCells();
+  if (m.dim != 2) return -1;  // requireTwoDimensional
+  auto positive = [](double x) { return std::isfinite(x) && x > 0.0; };
+  if (!positive(gasConstant) || !positive(dtau)) return -1;
+  std::vector<double> extra(n);
+  for (Index k = 0; k < n; ++k) {
+    if (!positive(pAbs[k]) || !positive(T[k])) return -1;
+    const double dRhoDp = 1.0 / (gasConstant * T[k]);
+    extra[k] = m.vol[k] / dtau * dRhoDp;
+  }
+  const std::vector<double> dU(du, du + n), rf(rhoF, rhoF + m.nFaces());
+  const PressureSystem ps = assemblePressure(m, std::vector<double>(Fstar, Fstar + m.nFaces()), dU,
+                                             dU, nullptr, 1.0, ref, c->bc, &rf, &extra);
   std::copy(ps.rhs.begin(), ps.rhs.end(), rhs);
   std::copy(ps.faceCoeff.begin(), ps.faceCoeff.end(), faceCoeff);
   return csrOut(ps.A, rp, col, val);

@@ -17,6 +17,11 @@
 //                                                 (TransientMomentum.cpp) from that state, then
 //                                                 `steps` more PISO steps: per-step status, CFL,
 //                                                 continuity, and the fields after each step
+//   cpcorr  nx ny seed outlet                   -> assembleCompressiblePressureCorrection on
+//                                                 seeded flux, face density, d, absolute
+//                                                 pressure and temperature (cavity, or the
+//                                                 right patch FixedValue with outlet=1), and
+//                                                 its CG + Jacobi solution
 //   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
 //                                                 iterations on the seeded p' system
 //                                                 (CPU baseline; prints one JSON line)
@@ -34,6 +39,9 @@
 #include "cfd/algebra/CG.hpp"
 #include "cfd/algebra/Preconditioner.hpp"
 #include "cfd/boundary/FixedGradient.hpp"
+#include "cfd/boundary/FixedValue.hpp"
+#include "cfd/compressible/CompressiblePressureCorrection.hpp"
+#include "cfd/compressible/ThermodynamicProperties.hpp"
 #include "cfd/boundary/MovingWall.hpp"
 #include "cfd/boundary/Wall.hpp"
 #include "cfd/discretization/Gradient.hpp"
@@ -291,6 +299,43 @@ void cmdPcorr() {
   dumpResult("cg", r);
   const Vector Ax = asmb.system.matrix().multiply(r.solution);
   dumpF("Ax", toVec(Ax));
+}
+
+void cmdCpcorr() {
+  Case c = makeCavity();
+  const Mesh& m = c.mesh;
+  if (iarg("outlet", "0") != 0)
+    c.pbc.replace(m, "right", std::make_unique<boundary::FixedValue>(0.0));
+  const size_t n = m.numberOfCells(), nf = m.numberOfFaces();
+  SplitMix64 rng{static_cast<uint64_t>(iarg("seed", "7"))};
+  SurfaceField flux(nf, 0.0), rhoF(nf, 0.0);
+  for (size_t f = 0; f < nf; ++f) flux[f] = 2.0 * rng.uniform() - 1.0;
+  for (size_t f = 0; f < nf; ++f) rhoF[f] = 0.5 + rng.uniform();
+  ScalarField d(n), pAbs(n), T(n);
+  for (size_t k = 0; k < n; ++k) d[k] = 1e-3 * (0.5 + rng.uniform());
+  for (size_t k = 0; k < n; ++k) pAbs[k] = 1e5 * (1.0 + 0.1 * (2.0 * rng.uniform() - 1.0));
+  for (size_t k = 0; k < n; ++k) T[k] = 300.0 + 50.0 * (2.0 * rng.uniform() - 1.0);
+  const double R = farg("gas", "287.05"), dtau = farg("dtau", "1e-3");
+  const compressible::ThermodynamicProperties thermo(R, farg("cp", "1004.5"));
+  pressure_velocity::PressureCorrectionOptions opts;
+  const auto asmb = compressible::assembleCompressiblePressureCorrection(
+      m, flux, rhoF, d, d, pAbs, T, thermo, dtau, iarg("ref", "0"), c.pbc, opts);
+  dumpF("flux", toVec(flux));
+  dumpF("face_density", toVec(rhoF));
+  dumpF("d", toVec(d));
+  dumpF("p_abs", toVec(pAbs));
+  dumpF("T", toVec(T));
+  dumpMatrix("A", asmb.system.matrix());
+  dumpF("rhs", toVec(asmb.system.rhs()));
+  dumpF("face_coeff", toVec(asmb.faceCoefficient));
+  LinearSolverSettings s;
+  s.type = algebra::LinearSolverType::CG;
+  s.absoluteTolerance = 0.0;
+  s.relativeTolerance = farg("prel", "1e-12");
+  s.maxIterations = iarg("pmax", "20000");
+  algebra::CG cg(s, std::make_shared<algebra::JacobiPreconditioner>());
+  const auto r = cg.solve(asmb.system);
+  dumpResult("cg", r);
 }
 
 void cmdSimple() {
@@ -552,6 +597,7 @@ int main(int argc, char** argv) {
   else if (cmd == "stages") cmdStages();
   else if (cmd == "cgtime") cmdCgTime();
   else if (cmd == "piso") cmdPiso();
+  else if (cmd == "cpcorr") cmdCpcorr();
   else {
     std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
     return 2;
