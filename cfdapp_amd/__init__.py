@@ -371,6 +371,23 @@ class GpuSimpleDiscretization:
         # (PressureCorrectionEquation.cpp:221-228)
         self.pin = -1 if self._fixed_value else int(reference_cell)
 
+    def assemble_compressible_pressure_correction_resident(self, face_density, pressure_absolute,
+                                                           temperature, gas_constant,
+                                                           pseudo_time_step, reference_cell=0):
+        """assembleCompressiblePressureCorrection
+        (src/compressible/CompressiblePressureCorrection.cpp:20-80) from the
+        resident FLUX_STAR and RESPONSE: per-face density, ideal-gas
+        compressibility diagonal V / dtau / (R T)."""
+        rf, pa, t = _f64(face_density), _f64(pressure_absolute), _f64(temperature)
+        if rf.size != self.n_faces or pa.size != self.n_cells or t.size != self.n_cells:
+            raise ValueError("assembleCompressiblePressureCorrection: field size does not match "
+                             "the mesh")
+        P = ctypes.POINTER(ctypes.c_double)
+        self._ctx.call("cfdhip_assemble_compressible_pressure", int(reference_cell),
+                       rf.ctypes.data_as(P), pa.ctypes.data_as(P), t.ctypes.data_as(P),
+                       float(gas_constant), float(pseudo_time_step))
+        self.pin = -1 if self._fixed_value else int(reference_cell)
+
     def solve_pressure_correction_resident(self, settings: LinearSolverSettings) -> SolverResult:
         r = _lib.SolveResult()
         s = settings.c()

@@ -293,6 +293,20 @@ int cfdhip_assemble_pressure(cfdhip_ctx* ctx, int64_t reference_cell) {
   return guarded(ctx, [&](Context& c) { launchPressureAssembly(c, reference_cell); });
 }
 
+int cfdhip_assemble_compressible_pressure(cfdhip_ctx* ctx, int64_t reference_cell,
+                                          const double* face_density,
+                                          const double* pressure_absolute,
+                                          const double* temperature, double gas_constant,
+                                          double pseudo_time_step) {
+  return guarded(ctx, [&](Context& c) {
+    require(face_density != nullptr && pressure_absolute != nullptr && temperature != nullptr,
+            "assembleCompressiblePressureCorrection: face density, absolute pressure and "
+            "temperature are required");
+    launchCompressiblePressureAssembly(c, reference_cell, face_density, pressure_absolute,
+                                       temperature, gas_constant, pseudo_time_step);
+  });
+}
+
 int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r) {
   return guarded(ctx, [&](Context& c) {
     const cfdhip_solver_settings set = checkedSettings(s);

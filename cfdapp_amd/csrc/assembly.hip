@@ -389,8 +389,10 @@ __global__ void fluxKernel(View v, FluxArgs m) {
   m.F[f] = F;
 }
 
-// p' face coupling D_f (pressureCorrectionFaceCoupling, axis-aligned branch).
-__global__ void pressureFaceKernel(View v, const double* __restrict__ D, double* coeff) {
+// p' face coupling D_f (pressureCorrectionFaceCoupling, axis-aligned branch),
+// with the constant density or a per-face one (rhoF, the compressible equation).
+__global__ void pressureFaceKernel(View v, const double* __restrict__ D,
+                                   const double* __restrict__ rhoF, double* coeff) {
   const int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (f >= v.nf) return;
   int64_t idx[3];
@@ -404,13 +406,13 @@ __global__ void pressureFaceKernel(View v, const double* __restrict__ D, double*
     }
     idx[a] = fi == 0 ? 0 : fi - 1;
     const int64_t o = cellOf(v, idx);
-    coeff[f] = v.rho * PG.area * D[o] / PG.bdist;
+    coeff[f] = (rhoF != nullptr ? rhoF[f] : v.rho) * PG.area * D[o] / PG.bdist;
     return;
   }
   idx[a] = fi - 1;
   const int64_t o = cellOf(v, idx);
   const double dF = interpScalar(v, a, fi, D[o], D[o + v.stride[a]]);
-  coeff[f] = v.rho * v.area[a] * dF / tabv(v, a, kDPN, fi);
+  coeff[f] = (rhoF != nullptr ? rhoF[f] : v.rho) * v.area[a] * dF / tabv(v, a, kDPN, fi);
 }
 
 // Signed face-flux sum of a cell in face-list order (ContinuityEquation.cpp:21-29).
@@ -427,9 +429,13 @@ __device__ inline double cellImbalance(const View& v, const double* __restrict__
   return sum;
 }
 
-// Row of the p' system: diagonal and RHS (assembleGeometricPressureCorrection).
+// Row of the p' system: diagonal and RHS (assembleGeometricPressureCorrection);
+// an additional diagonal (extra, the compressibility term) is added after the
+// face terms, as the reference's builder sums it (PressureCorrectionEquation.cpp:
+// 304-310), except on the pinned row.
 __global__ void pressureRowKernel(View v, const double* __restrict__ coeff,
-                                  const double* __restrict__ Fstar, int64_t pin, double* diag,
+                                  const double* __restrict__ Fstar,
+                                  const double* __restrict__ extra, int64_t pin, double* diag,
                                   double* rhs, unsigned* nonFinite) {
   const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= v.n) return;
@@ -445,6 +451,7 @@ __global__ void pressureRowKernel(View v, const double* __restrict__ coeff,
         if (sl.boundary && v.patch[2 * a + side].pkind != CFDHIP_FIXED_VALUE) continue;
         acc.add(coeff[sl.face]);
       }
+    if (extra != nullptr) acc.add(extra[c]);
     d = acc.has ? acc.s : 0.0;
     b = -cellImbalance(v, Fstar, c, idx);
   }
@@ -765,7 +772,7 @@ void launchPredictedFlux(Context& ctx, bool rhieChow, double alpha) {
   CFDHIP_CHECK(hipGetLastError());
 }
 
-void launchPressureAssembly(Context& ctx, int64_t ref) {
+void launchPressureAssembly(Context& ctx, int64_t ref, const double* rhoF, const double* extra) {
   ctx.requireMesh();
   const View& v = ctx.view;
   require(ref >= 0 && ref < v.n, "assemblePressureCorrection: referenceCell out of range");
@@ -773,19 +780,70 @@ void launchPressureAssembly(Context& ctx, int64_t ref) {
   for (int p = 0; p < v.npatch; ++p) open = open || v.patch[p].pkind == CFDHIP_FIXED_VALUE;
   ctx.pin = open ? -1 : ref;
   hipLaunchKernelGGL(pressureFaceKernel, blocksFor(v.nf), dim3(kBlock), 0, ctx.stream, v,
-                     ctx.f(CFDHIP_FIELD_RESPONSE), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF));
+                     ctx.f(CFDHIP_FIELD_RESPONSE), rhoF, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF));
   CFDHIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(faceCoeffFiniteKernel, blocksFor(v.nf), dim3(kBlock), 0, ctx.stream, v.nf,
                      ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.flags.get());
   hipLaunchKernelGGL(pressureRowKernel, blocksFor(v.n), dim3(kBlock), 0, ctx.stream, v,
-                     ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.f(CFDHIP_FIELD_FLUX_STAR), ctx.pin,
-                     ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_RHS), ctx.flags.get());
+                     ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), ctx.f(CFDHIP_FIELD_FLUX_STAR), extra,
+                     ctx.pin, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_RHS),
+                     ctx.flags.get());
   CFDHIP_CHECK(hipGetLastError());
   const unsigned bad = readAndClear(ctx);
   ctx.pressureAssembled = bad == 0;
   require(bad == 0,
           "assembleGeometricPressureCorrection: assembled system contains a non-finite value",
           CFDHIP_ERR_NONFINITE);
+}
+
+// Compressibility diagonal of the compressible p' equation
+// (CompressiblePressureCorrection.cpp:52-62): V / dtau * drho/dp per cell, with
+// the ideal gas drho/dp = 1 / (R T) (IdealGasEOS.cpp:35-39); counts the cells
+// whose absolute pressure or temperature is not finite and > 0 (the EOS
+// rejects them).
+__global__ void compressibilityKernel(int64_t n, double vol, double dtau, double R,
+                                      const double* __restrict__ pAbs,
+                                      const double* __restrict__ T, double* extra,
+                                      unsigned* invalid) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double p = pAbs[c], t = T[c];
+  if (!(isfinite(p) && p > 0.0 && isfinite(t) && t > 0.0)) {
+    atomicAdd(invalid, 1u);
+    extra[c] = 0.0;
+    return;
+  }
+  const double dRhoDp = 1.0 / (R * t);
+  extra[c] = vol / dtau * dRhoDp;
+}
+
+void launchCompressiblePressureAssembly(Context& ctx, int64_t ref, const double* rhoF,
+                                        const double* pAbs, const double* T, double R,
+                                        double dtau) {
+  ctx.requireMesh();
+  const View& v = ctx.view;
+  require(v.dim == 2, "assembleCompressiblePressureCorrection: the reference's equation is 2D only",
+          CFDHIP_ERR_UNSUPPORTED);
+  require(std::isfinite(R) && R > 0.0, "IdealGasEOS: gasConstant must be finite and > 0");
+  require(std::isfinite(dtau) && dtau > 0.0,
+          "assembleCompressiblePressureCorrection: pseudoTimeStep must be finite and > 0");
+  require(ref >= 0 && ref < v.n, "assembleCompressiblePressureCorrection: referenceCell out of range");
+  CompressibleWork& w = ctx.comp;
+  w.rhoF.ensure(v.nf);
+  w.pAbs.ensure(v.n);
+  w.T.ensure(v.n);
+  w.extra.ensure(v.n);
+  CFDHIP_CHECK(hipMemcpyAsync(w.rhoF.get(), rhoF, v.nf * sizeof(double), hipMemcpyDefault,
+                              ctx.stream));
+  CFDHIP_CHECK(hipMemcpyAsync(w.pAbs.get(), pAbs, v.n * sizeof(double), hipMemcpyDefault,
+                              ctx.stream));
+  CFDHIP_CHECK(hipMemcpyAsync(w.T.get(), T, v.n * sizeof(double), hipMemcpyDefault, ctx.stream));
+  hipLaunchKernelGGL(compressibilityKernel, blocksFor(v.n), dim3(kBlock), 0, ctx.stream, v.n,
+                     v.vol, dtau, R, w.pAbs.get(), w.T.get(), w.extra.get(), ctx.flags.get());
+  CFDHIP_CHECK(hipGetLastError());
+  const unsigned bad = readAndClear(ctx);
+  require(bad == 0, "IdealGasEOS: pressure and temperature must be finite and > 0");
+  launchPressureAssembly(ctx, ref, w.rhoF.get(), w.extra.get());
 }
 
 void launchUpdatePressure(Context& ctx, double alpha) {

@@ -45,6 +45,11 @@ struct KrylovWork {
   KState* hostState = nullptr;  // pinned mirror
 };
 
+// inputs of the compressible p' assembly, kept on the device
+struct CompressibleWork {
+  DeviceArray<double> rhoF, pAbs, T, extra;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -64,6 +69,7 @@ struct Context {
   DeviceArray<unsigned> flags;  // [0] non-finite counter
 
   KrylovWork kw;
+  CompressibleWork comp;
   std::vector<double> lastHistory;
   std::unique_ptr<Unet> net;  // learned p' preconditioner (weights loaded by the caller)
   int64_t lastFallback = 0;
@@ -94,7 +100,15 @@ void launchMassFlux(Context& ctx, const double* u, const double* v, const double
 void launchMomentum(Context& ctx, double alpha, double dt = 0.0);
 void launchResponse(Context& ctx);
 void launchPredictedFlux(Context& ctx, bool rhieChow, double alpha);
-void launchPressureAssembly(Context& ctx, int64_t ref);
+// rhoF: per-face density (nullptr: the fluid's); extra: an additional diagonal
+// added after the face terms (nullptr: none)
+void launchPressureAssembly(Context& ctx, int64_t ref, const double* rhoF = nullptr,
+                            const double* extra = nullptr);
+// assembleCompressiblePressureCorrection: per-face density (faces), absolute
+// pressure and temperature (cells), host or device pointers
+void launchCompressiblePressureAssembly(Context& ctx, int64_t ref, const double* rhoF,
+                                        const double* pAbs, const double* T, double R,
+                                        double dtau);
 void launchUpdatePressure(Context& ctx, double alpha);
 void launchCorrect(Context& ctx);
 void launchContinuity(Context& ctx, double* out);

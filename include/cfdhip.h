@@ -190,6 +190,22 @@ int cfdhip_predicted_flux(cfdhip_ctx* ctx, int rhie_chow, double alpha);
 /* assemblePressureCorrectionResident (PressureCorrectionEquation.cpp:181-367):
  * PC_DIAG, PC_FACE_COEFF, PC_RHS; reference_cell pinned when no patch is FixedValue. */
 int cfdhip_assemble_pressure(cfdhip_ctx* ctx, int64_t reference_cell);
+/* assembleCompressiblePressureCorrection (src/compressible/
+ * CompressiblePressureCorrection.cpp:20-80, 2D): the same system with the
+ * per-face density face_density[n_faces] in the face coupling and the
+ * compressibility diagonal V / pseudo_time_step * drho/dp added to each
+ * unpinned row after the face terms, drho/dp = 1 / (gas_constant * T) of the
+ * ideal gas (IdealGasEOS.cpp:35-39) at pressure_absolute[n_cells] and
+ * temperature[n_cells]. Uses FLUX_STAR and RESPONSE (one response coefficient
+ * for both components, as on these meshes); host or device pointers.
+ * CFDHIP_ERR_INVALID where the reference throws InvalidArgumentError (a
+ * pressure, temperature, gas constant or pseudo time step not finite and > 0,
+ * reference_cell out of range); CFDHIP_ERR_UNSUPPORTED on a 3D mesh. */
+int cfdhip_assemble_compressible_pressure(cfdhip_ctx* ctx, int64_t reference_cell,
+                                          const double* face_density,
+                                          const double* pressure_absolute,
+                                          const double* temperature, double gas_constant,
+                                          double pseudo_time_step);
 /* solvePressureCorrectionResident: p' from x0 = 0 into PPRIME. */
 int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r);
 /* updatePressure: P += alpha * PPRIME (SIMPLE.cpp:939-942). */

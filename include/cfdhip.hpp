@@ -304,6 +304,22 @@ class GpuSimpleDiscretization {
   void assemblePressureCorrectionResident(std::size_t referenceCell) {
     detail::check(ctx(), cfdhip_assemble_pressure(ctx(), static_cast<int64_t>(referenceCell)));
   }
+  // assembleCompressiblePressureCorrection (2D): per-face density (faces),
+  // absolute pressure and temperature (cells) of an ideal gas with constant R.
+  void assembleCompressiblePressureCorrectionResident(std::span<const double> faceDensity,
+                                                      std::span<const double> pressureAbsolute,
+                                                      std::span<const double> temperature,
+                                                      double gasConstant, double pseudoTimeStep,
+                                                      std::size_t referenceCell) {
+    if (faceDensity.size() != faces_ || pressureAbsolute.size() != cells_ ||
+        temperature.size() != cells_)
+      throw InvalidArgumentError(
+          "assembleCompressiblePressureCorrection: field size does not match the mesh");
+    detail::check(ctx(), cfdhip_assemble_compressible_pressure(
+                             ctx(), static_cast<int64_t>(referenceCell), faceDensity.data(),
+                             pressureAbsolute.data(), temperature.data(), gasConstant,
+                             pseudoTimeStep));
+  }
   [[nodiscard]] SolverResult solvePressureCorrectionResident(const LinearSolverSettings& s) {
     const cfdhip_solver_settings c = s.c();
     cfdhip_solve_result r{};
