@@ -265,6 +265,13 @@ class GpuSimpleDiscretization:
             raise CfdHipError(_lib.ERR_UNSUPPORTED, why)
         return d
 
+    def cell_centroids(self):
+        """(n_cells, 3) centroids and (n_cells,) volumes in cell order."""
+        c = np.zeros(3 * self.n_cells)
+        vol = np.zeros(self.n_cells)
+        self._ctx.call("cfdhip_mesh_cells", _pf(c), _pf(vol))
+        return c.reshape(-1, 3), vol
+
     def set_boundary(self, patch, vkind, vvalue, pkind, pvalue):
         v = _f64(list(vvalue) + [0.0] * (3 - len(vvalue)))
         self._ctx.call("cfdhip_set_boundary", patch, vkind, _pf(v), pkind, float(pvalue))

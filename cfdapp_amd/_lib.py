@@ -97,6 +97,7 @@ SIGNATURES = {
     "cfdhip_response": (ctypes.c_int, [ctypes.c_void_p]),
     "cfdhip_predicted_flux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     "cfdhip_assemble_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "cfdhip_mesh_cells": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_F64]),
     "cfdhip_assemble_compressible_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64,
                                                              P_F64, P_F64, P_F64, ctypes.c_double,
                                                              ctypes.c_double]),

@@ -170,6 +170,25 @@ int cfdhip_mesh_faces(cfdhip_ctx* ctx, int64_t* owner, int64_t* neighbor) {
   });
 }
 
+int cfdhip_mesh_cells(cfdhip_ctx* ctx, double* centroid, double* volume) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(centroid != nullptr, "centroid buffer is required");
+    const View& v = c.view;
+    // MeshGeometry::createCartesian2D/3D: (i + 0.5) * delta per axis, the 3D
+    // coordinates offset by the zero origin (0.0 + x == x for these x >= 0)
+    for (int64_t k = 0; k < v.nz; ++k)
+      for (int64_t j = 0; j < v.ny; ++j)
+        for (int64_t i = 0; i < v.nx; ++i) {
+          const int64_t id = i + v.nx * (j + v.ny * k);
+          centroid[3 * id] = (static_cast<double>(i) + 0.5) * c.delta[0];
+          centroid[3 * id + 1] = (static_cast<double>(j) + 0.5) * c.delta[1];
+          centroid[3 * id + 2] = v.dim == 3 ? (static_cast<double>(k) + 0.5) * c.delta[2] : 0.0;
+          if (volume != nullptr) volume[id] = v.vol;
+        }
+  });
+}
+
 int cfdhip_set_boundary(cfdhip_ctx* ctx, int patch, int velocity_kind, const double* velocity3,
                         int pressure_kind, double pressure_value) {
   return guarded(ctx, [&](Context& c) {

@@ -58,6 +58,7 @@ struct Context {
   bool hasMesh = false;
   View view{};
   DeviceArray<double> tables[3];
+  double delta[3] = {0.0, 0.0, 0.0};  // uniform cell size per axis
   double rho = 1.0, mu = 0.0;
   bool fluidSet = false;
   bool bcSet[6] = {false, false, false, false, false, false};

@@ -346,6 +346,7 @@ void buildCartesianPlan(Context& ctx, int dim, int64_t nx, int64_t ny, int64_t n
   }
   v.transfer = transfer ? 1 : 0;
   ctx.view = v;
+  for (int a = 0; a < 3; ++a) ctx.delta[a] = m.delta[a];
   ctx.hasMesh = true;
   ctx.pressureAssembled = ctx.momentumAssembled = false;
   for (bool& b : ctx.bcSet) b = false;

@@ -142,6 +142,11 @@ int cfdhip_mesh_sizes(cfdhip_ctx* ctx, int64_t* out);
 /* Face owner/neighbour (neighbour = -1 on a boundary face), generated on the
  * device in the reference face order (Mesh face list / DeviceMeshCuda upload). */
 int cfdhip_mesh_faces(cfdhip_ctx* ctx, int64_t* owner, int64_t* neighbor);
+/* Cell centroids (x, y, z per cell, z = 0 in 2D) and volumes (may be NULL) in
+ * cell order, as Mesh::cell(id).centroid() / volume() of
+ * MeshGeometry::createCartesian2D/3D (src/mesh/MeshGeometry.cpp:428-553,
+ * 1047-1160); what CSVWriter::writeFields prints in its x, y (, z) columns. */
+int cfdhip_mesh_cells(cfdhip_ctx* ctx, double* centroid, double* volume);
 /* Boundary conditions of patch p (2D: left, right, bottom, top; 3D: xmin..zmax),
  * as the BoundaryConditionSet the reference passes to prepare(). */
 int cfdhip_set_boundary(cfdhip_ctx* ctx, int patch, int velocity_kind, const double* velocity3,

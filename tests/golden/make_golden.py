@@ -5,7 +5,8 @@ Runs ``oracle/_ref/ref_driver`` (the reference's own CPU sources compiled by
 container only) and packs each command's raw arrays into ``<name>.npz``. Also
 converts the reference's committed 20x20 cavity results
 (``cases/lid_driven_cavity/results/{fields,residuals}.csv``) into
-``reference_cavity20.npz``.
+``reference_cavity20.npz``, and copies the case files and results of the
+reference's lid_driven_cavity and poiseuille_flow cases into ``cases/``.
 
     python tests/golden/make_golden.py
 """
@@ -78,6 +79,28 @@ def reference_cavity20():
                 hist_g=col(res, "global_mass_imbalance"))
 
 
+CASES = ["lid_driven_cavity", "poiseuille_flow"]
+
+
+def reference_cases():
+    """The reference's case files (inputs) and committed results (fields.csv,
+    residuals.csv, gzipped) of CASES, copied as data into cases/<name>/."""
+    import gzip
+    import shutil
+    for name in CASES:
+        src = os.path.join(REF, "cases", name)
+        dst = os.path.join(HERE, "cases", name)
+        os.makedirs(dst, exist_ok=True)
+        for f in sorted(os.listdir(src)):
+            if f.endswith(".json"):
+                shutil.copyfile(os.path.join(src, f), os.path.join(dst, f))
+        for f in ("fields.csv", "residuals.csv"):
+            with open(os.path.join(src, "results", f), "rb") as i, \
+                    gzip.GzipFile(os.path.join(dst, f + ".gz"), "wb", mtime=0) as o:
+                o.write(i.read())
+        print("wrote cases/" + name)
+
+
 def main(names):
     if not os.path.exists(DRIVER):
         sys.exit("build oracle/_ref first: make -C oracle -f Makefile.ref")
@@ -88,6 +111,7 @@ def main(names):
     if not names:
         np.savez_compressed(os.path.join(HERE, "reference_cavity20.npz"), **reference_cavity20())
         print("wrote reference_cavity20")
+        reference_cases()
 
 
 if __name__ == "__main__":
