@@ -30,7 +30,8 @@ from ._lib import (BREAKDOWN, CONVERGED, FIELDS, FIXED_GRADIENT, FIXED_VALUE, FP
 
 __all__ = ["device_available", "LinearSolverSettings", "SolverResult", "make_gpu_cg",
            "make_gpu_bicgstab", "GpuSimpleDiscretization", "CfdHipError", "UnetNet",
-           "unet_param_count", "load_learned_weights", "load_learned_options", "WEIGHTS_PATH", "BatchSolver"]
+           "unet_param_count", "load_learned_weights", "load_learned_options", "WEIGHTS_PATH", "BatchSolver",
+           "FaceMeshPressure"]
 
 
 class CfdHipError(RuntimeError):
@@ -524,6 +525,76 @@ class GpuSimpleDiscretization:
                     global_imbalance=meta[5],
                     hist=dict(u=h[0], v=h[1], w=h[2] if self.dim == 3 else h[2][:0], p=h[3],
                               c=h[4]))
+
+
+class FaceMeshPressure:
+    """The p' system on a general face-list mesh (cfdhip_fmesh_*): non-orthogonal
+    and multiblock meshes, with the over-relaxed coefficient and the explicit
+    non-orthogonal flux (assembleGeometricPressureCorrection,
+    PressureCorrectionEquation.cpp:181-339), solved by CG / BiCGSTAB on the CSR
+    operator, and correctFaceMassFlux. The mesh arrays are those of the
+    reference's Mesh: owner, neighbour (-1 on boundary faces), face centroids
+    and area vectors (n_faces x 3), cell centroids (n_cells x 3), each cell's
+    face list (offsets + faces) and a FixedValue-pressure flag per face."""
+
+    def __init__(self, dim, owner, neighbor, face_centroid, area_vector, cell_centroid,
+                 cell_face_offsets, cell_faces, fixed_value, device=0):
+        self._ctx = _Context(device)
+        self.n_cells = len(cell_face_offsets) - 1
+        self.n_faces = len(owner)
+        self._keep = [_i64(owner), _i64(neighbor), _f64(face_centroid).reshape(-1),
+                      _f64(area_vector).reshape(-1), _f64(cell_centroid).reshape(-1),
+                      _i64(cell_face_offsets), _i64(cell_faces),
+                      np.ascontiguousarray(fixed_value, dtype=np.int32)]
+        k = self._keep
+        self._ctx.call("cfdhip_fmesh_set", int(dim), self.n_cells, self.n_faces, _pi(k[0]),
+                       _pi(k[1]), _pf(k[2]), _pf(k[3]), _pf(k[4]), _pi(k[5]), _pi(k[6]),
+                       k[7].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+
+    def close(self):
+        self._ctx.close()
+
+    def assemble(self, flux_star, du, dv, dw=None, density=1.0, face_density=None,
+                 non_orthogonal=True, grad_previous=None, extra_diagonal=None,
+                 reference_cell=0):
+        """grad_previous: (n_cells, 3) gradient of the previous p' (explicit term)."""
+        held = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = _f64(a).reshape(-1)
+            held.append(a)
+            return _pf(a)
+
+        self._ctx.call("cfdhip_fmesh_assemble_pressure", ptr(flux_star), ptr(face_density),
+                       float(density), ptr(du), ptr(dv), ptr(dw), 1 if non_orthogonal else 0,
+                       ptr(grad_previous), ptr(extra_diagonal), int(reference_cell))
+
+    def system(self):
+        nnz = ctypes.c_int64(0)
+        self._ctx.call("cfdhip_fmesh_pressure_system", None, None, None, 0, ctypes.byref(nnz),
+                       None, None, None)
+        rp = np.zeros(self.n_cells + 1, np.int64)
+        col = np.zeros(max(nnz.value, 1), np.int64)
+        val = np.zeros(max(nnz.value, 1))
+        rhs, coeff, expl = np.zeros(self.n_cells), np.zeros(self.n_faces), np.zeros(self.n_faces)
+        self._ctx.call("cfdhip_fmesh_pressure_system", _pi(rp), _pi(col), _pf(val), nnz.value,
+                       ctypes.byref(nnz), _pf(rhs), _pf(coeff), _pf(expl))
+        return (rp, col[:nnz.value], val[:nnz.value]), rhs, coeff, expl
+
+    def solve(self, settings: "LinearSolverSettings"):
+        x = np.zeros(self.n_cells)
+        r = _lib.SolveResult()
+        s = settings.c()
+        self._ctx.call("cfdhip_fmesh_solve_pressure", ctypes.byref(s), _pf(x), ctypes.byref(r))
+        return _result(r, x, self._ctx.history())
+
+    def correct_flux(self, pprime=None):
+        F = np.zeros(self.n_faces)
+        p = None if pprime is None else _f64(pprime)
+        self._ctx.call("cfdhip_fmesh_correct_flux", None if p is None else _pf(p), _pf(F))
+        return F
 
 
 FACE_FIELDS = _lib.FACE_FIELDS

@@ -115,6 +115,18 @@ SIGNATURES = {
                                            P_I64]),
     "cfdhip_csr_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, P_I64, P_I64, P_F64,
                                        P_F64, P_F64]),
+    "cfdhip_fmesh_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                        ctypes.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64,
+                                        P_I64, P_I64, ctypes.POINTER(ctypes.c_int32)]),
+    "cfdhip_fmesh_assemble_pressure": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_F64,
+                                                      ctypes.c_double, P_F64, P_F64, P_F64,
+                                                      ctypes.c_int, P_F64, P_F64,
+                                                      ctypes.c_int64]),
+    "cfdhip_fmesh_pressure_system": (ctypes.c_int, [ctypes.c_void_p, P_I64, P_I64, P_F64,
+                                                    ctypes.c_int64, P_I64, P_F64, P_F64, P_F64]),
+    "cfdhip_fmesh_solve_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverSettings),
+                                                   P_F64, ctypes.POINTER(SolveResult)]),
+    "cfdhip_fmesh_correct_flux": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_F64]),
     "cfdhip_csr_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverSettings),
                                         ctypes.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64, P_F64,
                                         ctypes.POINTER(SolveResult)]),

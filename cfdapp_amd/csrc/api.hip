@@ -326,6 +326,49 @@ int cfdhip_assemble_compressible_pressure(cfdhip_ctx* ctx, int64_t reference_cel
   });
 }
 
+int cfdhip_fmesh_set(cfdhip_ctx* ctx, int dim, int64_t n_cells, int64_t n_faces,
+                     const int64_t* owner, const int64_t* neighbor, const double* face_centroid,
+                     const double* area_vector, const double* cell_centroid,
+                     const int64_t* cell_face_offsets, const int64_t* cell_faces,
+                     const int32_t* fixed_value) {
+  return guarded(ctx, [&](Context& c) {
+    fmeshSet(c, dim, n_cells, n_faces, owner, neighbor, face_centroid, area_vector, cell_centroid,
+             cell_face_offsets, cell_faces, fixed_value);
+  });
+}
+
+int cfdhip_fmesh_assemble_pressure(cfdhip_ctx* ctx, const double* flux_star,
+                                   const double* face_density, double density, const double* du,
+                                   const double* dv, const double* dw, int non_orthogonal,
+                                   const double* grad_previous, const double* extra_diagonal,
+                                   int64_t reference_cell) {
+  return guarded(ctx, [&](Context& c) {
+    fmeshAssemblePressure(c, flux_star, face_density, density, du, dv, dw, non_orthogonal,
+                          grad_previous, extra_diagonal, reference_cell);
+  });
+}
+
+int cfdhip_fmesh_pressure_system(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
+                                 int64_t capacity, int64_t* nnz, double* rhs, double* face_coeff,
+                                 double* explicit_flux) {
+  return guarded(ctx, [&](Context& c) {
+    const int64_t k = fmeshSystem(c, rowptr, col, val, capacity, rhs, face_coeff, explicit_flux);
+    if (nnz) *nnz = k;
+  });
+}
+
+int cfdhip_fmesh_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, double* x,
+                                cfdhip_solve_result* r) {
+  return guarded(ctx, [&](Context& c) {
+    const cfdhip_solve_result res = fmeshSolve(c, checkedSettings(s), x);
+    if (r) *r = res;
+  });
+}
+
+int cfdhip_fmesh_correct_flux(cfdhip_ctx* ctx, const double* pprime, double* flux) {
+  return guarded(ctx, [&](Context& c) { fmeshCorrectFlux(c, pprime, flux); });
+}
+
 int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r) {
   return guarded(ctx, [&](Context& c) {
     const cfdhip_solver_settings set = checkedSettings(s);

@@ -45,6 +45,28 @@ struct KrylovWork {
   KState* hostState = nullptr;  // pinned mirror
 };
 
+struct CsrDevice {
+  DeviceArray<int64_t> rp, col;
+  DeviceArray<double> val;
+  int64_t n = 0;
+};
+
+// A general face-list mesh (fmesh.hip) and its p' system.
+struct FaceMesh {
+  int dim = 2;
+  int64_t n = 0, nf = 0;
+  bool open = false;  // a FixedValue pressure face exists: no pinned cell
+  std::vector<int64_t> hOwner, hNeigh, hRp, hCol;
+  std::vector<int32_t> hFixed;
+  DeviceArray<int64_t> owner, neigh, cfOff, cf, cfSorted, slotOff, slotCode;
+  DeviceArray<double> fc, sf, cc;
+  DeviceArray<int> fixed;
+  DeviceArray<double> flux, du, dv, dw, rhoF, grad, extra, coeff, expl, rhs, x;
+  CsrDevice A;
+  int64_t planPin = -2;
+  bool planExtra = false, assembled = false, hasExplicit = false, solved = false;
+};
+
 // inputs of the compressible p' assembly, kept on the device
 struct CompressibleWork {
   DeviceArray<double> rhoF, pAbs, T, extra;
@@ -71,6 +93,7 @@ struct Context {
 
   KrylovWork kw;
   CompressibleWork comp;
+  std::unique_ptr<FaceMesh> fmesh;
   std::vector<double> lastHistory;
   std::unique_ptr<Unet> net;  // learned p' preconditioner (weights loaded by the caller)
   int64_t lastFallback = 0;
@@ -118,14 +141,22 @@ double launchMaxCell(Context& ctx, bool cfl, double dt);
 int64_t exportPressureCsr(Context& ctx, int64_t* rp, int64_t* col, double* val, int64_t cap);
 int64_t exportMomentumCsr(Context& ctx, int64_t* rp, int64_t* col, double* val, int64_t cap);
 unsigned countNonFinite(Context& ctx, const double* const* arrays, const int64_t* counts, int k);
+// fmesh.hip: the p' system on a general face-list mesh
+void fmeshSet(Context& ctx, int dim, int64_t n, int64_t nf, const int64_t* owner,
+              const int64_t* neighbor, const double* faceCentroid, const double* areaVector,
+              const double* cellCentroid, const int64_t* cellFaceOffsets,
+              const int64_t* cellFaces, const int32_t* fixedValue);
+void fmeshAssemblePressure(Context& ctx, const double* fluxStar, const double* faceDensity,
+                           double rho, const double* du, const double* dv, const double* dw,
+                           int nonOrthogonal, const double* gradPrev, const double* extraDiag,
+                           int64_t ref);
+int64_t fmeshSystem(Context& ctx, int64_t* rowptr, int64_t* col, double* val, int64_t cap,
+                    double* rhs, double* faceCoeff, double* explicitFlux);
+cfdhip_solve_result fmeshSolve(Context& ctx, const cfdhip_solver_settings& s, double* x);
+void fmeshCorrectFlux(Context& ctx, const double* pprime, double* flux);
 
 // krylov.hip
 enum class OpKind { Pressure, Stencil, Csr };
-struct CsrDevice {
-  DeviceArray<int64_t> rp, col;
-  DeviceArray<double> val;
-  int64_t n = 0;
-};
 cfdhip_solve_result solvePressure(Context& ctx, const cfdhip_solver_settings& s, const double* b,
                                   double* x, bool zeroGuess);
 cfdhip_solve_result solveStencil(Context& ctx, const cfdhip_solver_settings& s, const double* b,

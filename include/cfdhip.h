@@ -211,6 +211,49 @@ int cfdhip_assemble_compressible_pressure(cfdhip_ctx* ctx, int64_t reference_cel
                                           const double* pressure_absolute,
                                           const double* temperature, double gas_constant,
                                           double pseudo_time_step);
+/* --- p' on a general face-list mesh (non-orthogonal, multiblock) ---------
+ * The reference's CSR route (cuda/kernels/DevicePressureCorrectionKernel.cu:
+ * 176-216) for meshes the structured path does not take: the mesh as its
+ * Mesh holds it -- face owner / neighbour (-1 on a boundary face), face
+ * centroid and area vector (x, y, z per face), cell centroid (x, y, z per
+ * cell), each cell's face list (Cell::faceIds() order, CSR offsets) and a
+ * FixedValue-pressure flag per face (boundary faces only). One mesh per
+ * context, replaced by the next call. */
+int cfdhip_fmesh_set(cfdhip_ctx* ctx, int dim, int64_t n_cells, int64_t n_faces,
+                     const int64_t* owner, const int64_t* neighbor, const double* face_centroid,
+                     const double* area_vector, const double* cell_centroid,
+                     const int64_t* cell_face_offsets, const int64_t* cell_faces,
+                     const int32_t* fixed_value);
+/* assembleGeometricPressureCorrection (PressureCorrectionEquation.cpp:181-339)
+ * on that mesh from host arrays: predictor flux, per-face density (NULL: the
+ * constant density), response coefficients du, dv (dw on a 3D mesh, else
+ * NULL); non_orthogonal = the over-relaxed implicit coefficient; with
+ * grad_previous (x, y, z per cell: the gradient of the previous p' under
+ * the p' conditions) the explicit -rho T . grad(p') flux moves to the RHS;
+ * extra_diagonal (NULL: none) is added to each unpinned row after the face
+ * terms (the compressible equation). Pins reference_cell when no face is
+ * FixedValue. CFDHIP_ERR_NONFINITE for degenerate geometry or a non-finite
+ * system, as the reference's NumericalError. */
+int cfdhip_fmesh_assemble_pressure(cfdhip_ctx* ctx, const double* flux_star,
+                                   const double* face_density, double density, const double* du,
+                                   const double* dv, const double* dw, int non_orthogonal,
+                                   const double* grad_previous, const double* extra_diagonal,
+                                   int64_t reference_cell);
+/* The assembled system as SparseMatrixBuilder builds it (columns ascending,
+ * exact-zero sums dropped): rowptr[n+1], col/val[capacity] (NULL rowptr: only
+ * *nnz), rhs[n], face_coeff[n_faces], explicit_flux[n_faces]; any may be NULL. */
+int cfdhip_fmesh_pressure_system(cfdhip_ctx* ctx, int64_t* rowptr, int64_t* col, double* val,
+                                 int64_t capacity, int64_t* nnz, double* rhs, double* face_coeff,
+                                 double* explicit_flux);
+/* p' from x0 = 0 by CG / BiCGSTAB (Jacobi or none) on the CSR operator; x[n]
+ * (host, may be NULL) receives it. The learned preconditioner needs the
+ * structured path (CFDHIP_ERR_UNSUPPORTED). */
+int cfdhip_fmesh_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, double* x,
+                                cfdhip_solve_result* r);
+/* correctFaceMassFlux (PressureCorrectionEquation.cpp:369-402): flux[n_faces]
+ * = F* + D_f (p'_P - p'_N) (+ the explicit flux when the assembly had one),
+ * from pprime[n] (host) or, when NULL, the last solve's p'. */
+int cfdhip_fmesh_correct_flux(cfdhip_ctx* ctx, const double* pprime, double* flux);
 /* solvePressureCorrectionResident: p' from x0 = 0 into PPRIME. */
 int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r);
 /* updatePressure: P += alpha * PPRIME (SIMPLE.cpp:939-942). */

@@ -22,6 +22,9 @@
 //                                                 pressure and temperature (cavity, or the
 //                                                 right patch FixedValue with outlet=1), and
 //                                                 its CG + Jacobi solution
+//   fpcorr  nx ny seed outlet amp               -> p' system on a distorted quad mesh with the
+//                                                 non-orthogonal correction and explicit term,
+//                                                 its CG + Jacobi solution, the corrected flux
 //   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
 //                                                 iterations on the seeded p' system
 //                                                 (CPU baseline; prints one JSON line)
@@ -214,9 +217,7 @@ pressure_velocity::SIMPLESettings simpleSettings() {
   return s;
 }
 
-void cmdMesh() {
-  const Case c = makeCavity();
-  const Mesh& m = c.mesh;
+void dumpMesh(const Mesh& m) {
   std::vector<int64_t> owner, neighbor, cellFaces, cellFaceOffsets{0};
   std::vector<double> fcx, fcy, fcz, sx, sy, sz, ccx, ccy, ccz, vol;
   for (const auto& f : m.faces()) {
@@ -258,6 +259,92 @@ void cmdMesh() {
   dumpF("cell_cy", ccy);
   dumpF("cell_cz", ccz);
   dumpF("cell_vol", vol);
+}
+
+void cmdMesh() {
+  const Case c = makeCavity();
+  dumpMesh(c.mesh);
+}
+
+// p' system on a distorted structured-quad mesh (createStructuredQuad2D, interior
+// vertices moved by a seeded fraction amp of the cell size) with the
+// non-orthogonal correction and the explicit term of a seeded previous p'
+// (its Green-Gauss gradient under the p' boundary conditions dumped as the
+// input the device takes); its CG + Jacobi solution and the corrected flux.
+void cmdFpcorr() {
+  const size_t nx = iarg("nx"), ny = iarg("ny");
+  const double amp = farg("amp", "0.2"), lx = farg("lx", "1.0"), ly = farg("ly", "1.0");
+  SplitMix64 rng{static_cast<uint64_t>(iarg("seed", "11"))};
+  std::vector<Vector2> verts;
+  for (size_t j = 0; j <= ny; ++j)
+    for (size_t i = 0; i <= nx; ++i) {
+      double x = lx * static_cast<double>(i) / static_cast<double>(nx);
+      double y = ly * static_cast<double>(j) / static_cast<double>(ny);
+      if (i > 0 && i < nx && j > 0 && j < ny) {
+        x += amp * (lx / static_cast<double>(nx)) * (2.0 * rng.uniform() - 1.0);
+        y += amp * (ly / static_cast<double>(ny)) * (2.0 * rng.uniform() - 1.0);
+      }
+      verts.push_back(Vector2{x, y});
+    }
+  const Mesh m = MeshGeometry::createStructuredQuad2D(nx, ny, verts);
+  boundary::BoundaryConditionSet pbc, gbc;
+  const bool outlet = iarg("outlet", "0") != 0;
+  for (const auto& patch : m.boundaryPatches()) {
+    const bool fv = outlet && patch.name() == "right";
+    if (fv) {
+      pbc.set(m, patch.name(), std::make_unique<boundary::FixedValue>(0.0));
+      gbc.set(m, patch.name(), std::make_unique<boundary::FixedValue>(0.0));
+    } else {
+      pbc.set(m, patch.name(), std::make_unique<boundary::FixedGradient>(0.0));
+      gbc.set(m, patch.name(), std::make_unique<boundary::FixedGradient>(0.0));
+    }
+  }
+  const size_t n = m.numberOfCells(), nf = m.numberOfFaces();
+  SurfaceField flux(nf, 0.0);
+  for (size_t f = 0; f < nf; ++f) flux[f] = 2.0 * rng.uniform() - 1.0;
+  ScalarField du(n), dv(n), prev(n);
+  for (size_t k = 0; k < n; ++k) du[k] = 0.5 + rng.uniform();
+  for (size_t k = 0; k < n; ++k) dv[k] = 0.5 + rng.uniform();
+  for (size_t k = 0; k < n; ++k) prev[k] = 2.0 * rng.uniform() - 1.0;
+  const VectorField grad = discretization::gradient(m, prev, gbc);
+  pressure_velocity::PressureCorrectionOptions opts;
+  opts.nonOrthogonal = iarg("nonorth", "1") != 0;
+  if (iarg("explicit", "1") != 0) opts.previousPressureCorrection = &prev;
+  const double rho = farg("rho", "1.0");
+  const auto asmb = pressure_velocity::assemblePressureCorrection(
+      m, flux, du, dv, rho, iarg("ref", "0"), pbc, opts);
+  dumpMesh(m);
+  std::vector<double> gx(n), gy(n);
+  for (size_t k = 0; k < n; ++k) {
+    gx[k] = grad[k].x;
+    gy[k] = grad[k].y;
+  }
+  dumpF("flux", toVec(flux));
+  dumpF("du", toVec(du));
+  dumpF("dv", toVec(dv));
+  dumpF("grad_x", gx);
+  dumpF("grad_y", gy);
+  dumpMatrix("A", asmb.system.matrix());
+  dumpF("rhs", toVec(asmb.system.rhs()));
+  dumpF("face_coeff", toVec(asmb.faceCoefficient));
+  dumpF("explicit_flux", toVec(asmb.explicitFaceFlux));
+  LinearSolverSettings s;
+  s.type = algebra::LinearSolverType::CG;
+  s.absoluteTolerance = 0.0;
+  s.relativeTolerance = farg("prel", "1e-12");
+  s.maxIterations = iarg("pmax", "20000");
+  algebra::CG cg(s, std::make_shared<algebra::JacobiPreconditioner>());
+  const auto r = cg.solve(asmb.system);
+  dumpResult("cg", r);
+  const ScalarField pp = [&] {
+    ScalarField f(n);
+    for (size_t k = 0; k < n; ++k) f[k] = r.solution[k];
+    return f;
+  }();
+  const SurfaceField corrected = pressure_velocity::correctFaceMassFlux(
+      m, flux, asmb.faceCoefficient, pp, opts.previousPressureCorrection ? &asmb.explicitFaceFlux
+                                                                         : nullptr);
+  dumpF("flux_corrected", toVec(corrected));
 }
 
 // p' operator with d == 1 (rho = 1, cell 0 pinned) and a seeded, mean-free RHS,
@@ -598,6 +685,7 @@ int main(int argc, char** argv) {
   else if (cmd == "cgtime") cmdCgTime();
   else if (cmd == "piso") cmdPiso();
   else if (cmd == "cpcorr") cmdCpcorr();
+  else if (cmd == "fpcorr") cmdFpcorr();
   else {
     std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
     return 2;
