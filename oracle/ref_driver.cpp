@@ -22,7 +22,8 @@
 //                                                 pressure and temperature (cavity, or the
 //                                                 right patch FixedValue with outlet=1), and
 //                                                 its CG + Jacobi solution
-//   fpcorr  nx ny seed outlet amp               -> p' system on a distorted quad mesh with the
+//   fpcorr  nx ny seed outlet amp [mesh=multiblock] -> p' system on a distorted quad mesh (or
+//                                                 two such blocks joined) with the
 //                                                 non-orthogonal correction and explicit term,
 //                                                 its CG + Jacobi solution, the corrected flux
 //   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
@@ -49,6 +50,7 @@
 #include "cfd/boundary/Wall.hpp"
 #include "cfd/discretization/Gradient.hpp"
 #include "cfd/mesh/MeshGeometry.hpp"
+#include "cfd/mesh/MultiBlockSpec.hpp"
 #include "cfd/physics/ContinuityEquation.hpp"
 #include "cfd/physics/FluidProperties.hpp"
 #include "cfd/physics/MassFlux.hpp"
@@ -275,18 +277,36 @@ void cmdFpcorr() {
   const size_t nx = iarg("nx"), ny = iarg("ny");
   const double amp = farg("amp", "0.2"), lx = farg("lx", "1.0"), ly = farg("ly", "1.0");
   SplitMix64 rng{static_cast<uint64_t>(iarg("seed", "11"))};
-  std::vector<Vector2> verts;
-  for (size_t j = 0; j <= ny; ++j)
-    for (size_t i = 0; i <= nx; ++i) {
-      double x = lx * static_cast<double>(i) / static_cast<double>(nx);
-      double y = ly * static_cast<double>(j) / static_cast<double>(ny);
-      if (i > 0 && i < nx && j > 0 && j < ny) {
-        x += amp * (lx / static_cast<double>(nx)) * (2.0 * rng.uniform() - 1.0);
-        y += amp * (ly / static_cast<double>(ny)) * (2.0 * rng.uniform() - 1.0);
+  // (nx + 1) x (ny + 1) vertices over [x0, x0 + lx] x [0, ly], interior ones moved
+  auto lattice = [&](double x0) {
+    std::vector<Vector2> verts;
+    for (size_t j = 0; j <= ny; ++j)
+      for (size_t i = 0; i <= nx; ++i) {
+        double x = x0 + lx * static_cast<double>(i) / static_cast<double>(nx);
+        double y = ly * static_cast<double>(j) / static_cast<double>(ny);
+        if (i > 0 && i < nx && j > 0 && j < ny) {
+          x += amp * (lx / static_cast<double>(nx)) * (2.0 * rng.uniform() - 1.0);
+          y += amp * (ly / static_cast<double>(ny)) * (2.0 * rng.uniform() - 1.0);
+        }
+        verts.push_back(Vector2{x, y});
       }
-      verts.push_back(Vector2{x, y});
-    }
-  const Mesh m = MeshGeometry::createStructuredQuad2D(nx, ny, verts);
+    return verts;
+  };
+  // mesh=multiblock: two such blocks side by side (the second over
+  // [lx, 2 lx]) joined conformally along x = lx (createMultiBlock2D)
+  const Mesh m = [&] {
+    if (arg("mesh", "quad") != "multiblock") return MeshGeometry::createStructuredQuad2D(nx, ny, lattice(0.0));
+    mesh::MultiBlockSpec spec;
+    spec.blocks.push_back(mesh::BlockSpec{"west", nx, ny, lattice(0.0)});
+    spec.blocks.push_back(mesh::BlockSpec{"east", nx, ny, lattice(lx)});
+    using S = mesh::BlockSide;
+    spec.interfaces.push_back(mesh::BlockInterfaceSpec{{0, S::Right}, {1, S::Left}, false});
+    spec.patches.push_back(mesh::BoundaryPatchSpec{"left", {{0, S::Left}}});
+    spec.patches.push_back(mesh::BoundaryPatchSpec{"right", {{1, S::Right}}});
+    spec.patches.push_back(mesh::BoundaryPatchSpec{"bottom", {{0, S::Bottom}, {1, S::Bottom}}});
+    spec.patches.push_back(mesh::BoundaryPatchSpec{"top", {{0, S::Top}, {1, S::Top}}});
+    return MeshGeometry::createMultiBlock2D(spec);
+  }();
   boundary::BoundaryConditionSet pbc, gbc;
   const bool outlet = iarg("outlet", "0") != 0;
   for (const auto& patch : m.boundaryPatches()) {

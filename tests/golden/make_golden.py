@@ -52,6 +52,8 @@ FIXTURES = {
     "cpcorr2d_20x12_outlet": ("cpcorr", dict(dim=2, nx=20, ny=12, seed=43, outlet=1, dtau=0.05)),
     "fpcorr2d_12x10": ("fpcorr", dict(nx=12, ny=10, seed=11, outlet=0, amp=0.2)),
     "fpcorr2d_16x8_outlet": ("fpcorr", dict(nx=16, ny=8, seed=13, outlet=1, amp=0.25, lx=2.0)),
+    "fpcorr2d_multiblock_outlet": ("fpcorr", dict(nx=7, ny=6, seed=17, outlet=1, amp=0.2,
+                                                  mesh="multiblock")),
     # tight linear solves: the step's fields then differ only by reduction order
     "piso2d_32_tight": ("piso", dict(dim=2, nx=32, ny=32, re=1000, dt=0.005, warm=3, steps=2,
                                      psolver="bicgstab", mabs=0, mrel=1e-13, pabs=0, prel=1e-13)),

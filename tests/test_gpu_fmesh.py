@@ -3,7 +3,8 @@
 Fixtures tests/golden/fpcorr*.npz come from the reference's own sources
 (oracle/ref_driver.cpp fpcorr): a createStructuredQuad2D mesh whose interior
 vertices are moved by a seeded fraction of the cell size (every internal and
-boundary face non-orthogonal or skewed), seeded predictor flux, response
+boundary face non-orthogonal or skewed), and two such blocks joined by
+createMultiBlock2D along a conformal interface; seeded predictor flux, response
 coefficients d_u != d_v and a previous p' whose Green-Gauss gradient feeds the
 explicit term; assemblePressureCorrection with the non-orthogonal correction,
 its CG + Jacobi solution and correctFaceMassFlux with the explicit flux. On
@@ -22,7 +23,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-NAMES = ["fpcorr2d_12x10", "fpcorr2d_16x8_outlet"]
+NAMES = ["fpcorr2d_12x10", "fpcorr2d_16x8_outlet", "fpcorr2d_multiblock_outlet"]
 
 
 def _bits(a):
