@@ -13,7 +13,7 @@
 // The convergence test is the reference's: recurrence residual relative to
 // |b - A x0|.
 //
-// Memory: one block per 62 x 30 tile (+1 halo): the tile's r, s, w, diag
+// Memory: one block per 62 x 30 (FP64) or 62 x 62 (FP32) tile (+1 halo): the tile's r, s, w, diag
 // are read in one burst, u = M r goes to LDS, the stencil reads LDS. r, s, w,
 // p are double-buffered (neighbouring tiles read the previous iterate), x is
 // updated in place. Per iteration the kernel reads p, s, x, r, w, diag and the
@@ -66,11 +66,16 @@ __device__ inline int64_t xcdTile(int64_t b, int64_t T) {
 }
 
 // Tile geometry: a block covers 62 interior columns (+1 halo each side, one
-// lane per column) and 30 interior rows (+1 halo each side); each of its 4
-// waves owns 8 consecutive tile rows.
-constexpr int kTileRows = 32;
-constexpr int kRowsPerWave = kTileRows / kWaves;
-constexpr int kInteriorRows = kTileRows - 2;
+// lane per column) and 30 (FP64) or 62 (FP32) interior rows (+1 halo each
+// side); each of its 4 waves owns a quarter of the tile rows. FP32 tiles are
+// twice as tall (half the halo share); at 64 rows the FP64 kernel would drop
+// to 3 waves per SIMD, which costs more than the halo saves.
+template <class T>
+struct TileShape {
+  static constexpr int kTileRows = sizeof(T) == 4 ? 64 : 32;
+  static constexpr int kRowsPerWave = kTileRows / kWaves;
+  static constexpr int kInteriorRows = kTileRows - 2;
+};
 
 // One CG iteration (start == 0) or the start step w0 = A M r0 (start == 1, r0
 // in rOld). Phase 1 loads the tile in one burst, forms the new residual and
@@ -83,6 +88,9 @@ __global__ __launch_bounds__(256) void cgIterKernel(
     T* __restrict__ sNew, const T* __restrict__ rOld, T* __restrict__ rNew,
     const T* __restrict__ wOld, T* __restrict__ wNew, T* __restrict__ x, int64_t tilesX,
     KState* st, double* partial, int start) {
+  constexpr int kTileRows = TileShape<T>::kTileRows;
+  constexpr int kRowsPerWave = TileShape<T>::kRowsPerWave;
+  constexpr int kInteriorRows = TileShape<T>::kInteriorRows;
   __shared__ T uTile[kTileRows][64];
   const bool skip = st->done;
   const T alpha = static_cast<T>(st->alpha);
@@ -195,6 +203,7 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   require(v.dim == 2, "the single-kernel PCG needs a 2D mesh");
   KrylovWork& w = ctx.kw;
   const int64_t tilesX = (v.nx + kOut - 1) / kOut;
+  constexpr int kInteriorRows = TileShape<T>::kInteriorRows;
   const int64_t tiles = tilesX * ((v.ny + kInteriorRows - 1) / kInteriorRows);
   const dim3 grid(static_cast<unsigned>(tiles));
   w.partial.ensure(std::max<int64_t>(kMaxBlocks * 4, static_cast<int64_t>(grid.x) * 3 + 8));
