@@ -297,7 +297,10 @@ constexpr int cmax() {
   return A > B ? A : B;
 }
 
-template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
+// SIDES: the padding may have Dirichlet sides (odd reflection), so staged
+// windows on such a side get the sign fix-up; without it that code is not
+// compiled in (it costs registers on the concatenating layers).
+template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT, bool SIDES>
 __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   constexpr int CC = kChunk(CIN);
   constexpr int NCH = CIN / CC;
@@ -404,11 +407,11 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
     } else if (kConcat && ch < NCO) {
       coarseBuf.store(hiT, loT, CS);
       const int ch2 = a.h >> 1, cw2 = a.w_ >> 1, cy = (y0 >> 1) - 1, cx = (x0 >> 1) - 1;
-      if (decltype(coarseBuf)::hasSigns(ch2, cw2, cy, cx, a.pad))
+      if (SIDES && decltype(coarseBuf)::hasSigns(ch2, cw2, cy, cx, a.pad))
         decltype(coarseBuf)::flipSides(hiT, loT, CS, ch2, cw2, cy, cx, a.pad);
     } else {
       stageBuf.store(hiT, loT, CS);
-      if (decltype(stageBuf)::hasSigns(a.h, a.w_, y0 - 1, x0 - 1, a.pad))
+      if (SIDES && decltype(stageBuf)::hasSigns(a.h, a.w_, y0 - 1, x0 - 1, a.pad))
         decltype(stageBuf)::flipSides(hiT, loT, CS, a.h, a.w_, y0 - 1, x0 - 1, a.pad);
     }
     __syncthreads();
@@ -599,15 +602,24 @@ int residentBlocks(K kernel) {
   return std::max(1, cus) * std::max(1, perCu);
 }
 
-template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
-void conv(hipStream_t s, const ConvArgs& a) {
-  static const int resident = residentBlocks(convKernel<CIN, CA, COUT, NBG, G, E, SPLIT>);
+template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT, bool SIDES>
+void convWith(hipStream_t s, const ConvArgs& a) {
+  static const int resident = residentBlocks(convKernel<CIN, CA, COUT, NBG, G, E, SPLIT, SIDES>);
   constexpr int groups = COUT / (16 * NBG);
   const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
   require(tiles < (int64_t{1} << 31), "the network batch has too many tiles");
   const int64_t perGroup = std::max(1, resident / groups);
   const dim3 grid(static_cast<unsigned>(std::min(tiles, perGroup)), static_cast<unsigned>(groups));
-  hipLaunchKernelGGL((convKernel<CIN, CA, COUT, NBG, G, E, SPLIT>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((convKernel<CIN, CA, COUT, NBG, G, E, SPLIT, SIDES>), grid, dim3(256), 0, s,
+                     a);
+}
+
+template <int CIN, int CA, int COUT, int NBG, int G, int E, int SPLIT>
+void conv(hipStream_t s, const ConvArgs& a) {
+  if ((a.pad & kPadNeumann) && (a.pad >> 1) != 0)
+    convWith<CIN, CA, COUT, NBG, G, E, SPLIT, true>(s, a);
+  else
+    convWith<CIN, CA, COUT, NBG, G, E, SPLIT, false>(s, a);
 }
 
 constexpr int kPpw = 2;
