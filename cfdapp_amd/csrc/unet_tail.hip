@@ -174,23 +174,25 @@ __global__ __launch_bounds__(256) void tailRestrictKernel(const float* __restric
 //   kPost:     E_t = S_t - omega A S_t + omega R_t (the post-sweep).
 enum TailPass { kPre = 0, kRestrict = 1, kProlong = 2, kPost = 3 };
 
+// base + s.off[k] - shift is level array k (shift: where base's level
+// arrays start in the workspace layout).
 __device__ inline void tailPass(float* base, const TailShape& s, const TailParams& p, int pad,
-                                int t, int pass, int i) {
+                                int t, int pass, int i, int64_t shift = 0) {
   const float om = p.omega;
-  float* R = base + s.off[3 * t];
-  float* S = base + s.off[3 * t + 1];
-  float* E = base + s.off[3 * t + 2];
+  float* R = base + (s.off[3 * t] - shift);
+  float* S = base + (s.off[3 * t + 1] - shift);
+  float* E = base + (s.off[3 * t + 2] - shift);
   const int h = tailH(s, t), w = tailW(s, t);
   if (pass == kRestrict) {
     const int wc = tailW(s, t + 1);
-    base[s.off[3 * (t + 1)] + i] = restrictK(S, h, w, i / wc, i % wc, p, pad);
+    base[s.off[3 * (t + 1)] - shift + i] = restrictK(S, h, w, i / wc, i % wc, p, pad);
     return;
   }
   const int y = i / w, x = i % w;
   if (pass == kPre) {
     S[i] = R[i] - om * stencil3(R, h, w, y, x, p.a[0], p.a[1], p.a[2], pad);
   } else if (pass == kProlong) {
-    S[i] = om * R[i] + prolongK(base + s.off[3 * (t + 1) + 2], h, w, y, x, p, pad);
+    S[i] = om * R[i] + prolongK(base + (s.off[3 * (t + 1) + 2] - shift), h, w, y, x, p, pad);
   } else {
     E[i] = S[i] - om * stencil3(S, h, w, y, x, p.a[0], p.a[1], p.a[2], pad) + om * R[i];
   }
@@ -213,16 +215,16 @@ __global__ __launch_bounds__(256) void tailLevelKernel(TailShape s, TailParams p
   }
 }
 
-// Levels t0 .. L-1 of one image per workgroup: down from R_t0, the coarsest
-// level, up to E_t0.
-__global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, TailParams p, int pad,
-                                                               int t0, float* __restrict__ work) {
-  float* base = work + static_cast<int64_t>(blockIdx.x) * s.perImage;
+// Levels t0 .. L-1 of one image: down from R_t0, the coarsest level, up to
+// E_t0; base holds the image's levels at the offsets of s.
+__device__ inline void cycleLevels(float* base, const TailShape& s, const TailParams& p, int pad,
+                                   int t0, int64_t shift) {
   const int L = s.levels;
   for (int t = t0; t + 1 < L; ++t) {
     for (int pass = kPre; pass <= kRestrict; ++pass) {
       const int m = tailPassPixels(s, t, pass);
-      for (int i = threadIdx.x; i < m; i += blockDim.x) tailPass(base, s, p, pad, t, pass, i);
+      for (int i = threadIdx.x; i < m; i += blockDim.x)
+        tailPass(base, s, p, pad, t, pass, i, shift);
       __syncthreads();
     }
   }
@@ -230,8 +232,8 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
   // e = omega r + (I - omega A) omega r
   {
     const float om = p.omega;
-    const float* R = base + s.off[3 * (L - 1)];
-    float* E = base + s.off[3 * (L - 1) + 2];
+    const float* R = base + (s.off[3 * (L - 1)] - shift);
+    float* E = base + (s.off[3 * (L - 1) + 2] - shift);
     const int h = tailH(s, L - 1), w = tailW(s, L - 1);
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
       const float ar = stencil3(R, h, w, i / w, i % w, p.a[0], p.a[1], p.a[2], pad);
@@ -242,10 +244,35 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
   for (int t = L - 2; t >= t0; --t) {
     for (int pass = kProlong; pass <= kPost; ++pass) {
       const int m = tailPassPixels(s, t, pass);
-      for (int i = threadIdx.x; i < m; i += blockDim.x) tailPass(base, s, p, pad, t, pass, i);
+      for (int i = threadIdx.x; i < m; i += blockDim.x)
+        tailPass(base, s, p, pad, t, pass, i, shift);
       __syncthreads();
     }
   }
+}
+
+// One image per workgroup, its levels in the global workspace.
+__global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, TailParams p, int pad,
+                                                               int t0, float* __restrict__ work) {
+  cycleLevels(work + static_cast<int64_t>(blockIdx.x) * s.perImage, s, p, pad, t0, 0);
+}
+
+// The same with levels t0 .. L-1 staged in LDS (R_t0 in, E_t0 out): every
+// pass then waits on LDS instead of L2. Levels t0.. hold at most
+// 3 x 4/3 x kTailCyclePixels floats.
+constexpr int kTailLdsFloats = 16384;
+__global__ __launch_bounds__(kTailThreads) void tailCycleLdsKernel(TailShape s, TailParams p,
+                                                                  int pad, int t0,
+                                                                  float* __restrict__ work) {
+  __shared__ float lds[kTailLdsFloats];
+  float* g = work + static_cast<int64_t>(blockIdx.x) * s.perImage;
+  const int64_t o0 = s.off[3 * t0];  // LDS holds the workspace from o0 on
+  const int m0 = tailH(s, t0) * tailW(s, t0);
+  for (int i = threadIdx.x; i < m0; i += blockDim.x) lds[i] = g[o0 + i];  // R_t0 at offset 0
+  __syncthreads();
+  cycleLevels(lds, s, p, pad, t0, o0);
+  const int64_t e0 = s.off[3 * t0 + 2];
+  for (int i = threadIdx.x; i < m0; i += blockDim.x) g[e0 + i] = lds[e0 - o0 + i];
 }
 
 // out channel = K up(E_0) (forward), or in channel += K up(E_0) and the
@@ -316,8 +343,12 @@ void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward) {
       level(t, kPre);
       level(t, kRestrict);
     }
-    hipLaunchKernelGGL(tailCycleKernel, dim3(static_cast<unsigned>(n)), dim3(kTailThreads), 0,
-                       stream, s, tail, padBits(), t0, tailWork.get());
+    if (s.perImage - s.off[3 * t0] <= kTailLdsFloats)
+      hipLaunchKernelGGL(tailCycleLdsKernel, dim3(static_cast<unsigned>(n)), dim3(kTailThreads), 0,
+                         stream, s, tail, padBits(), t0, tailWork.get());
+    else
+      hipLaunchKernelGGL(tailCycleKernel, dim3(static_cast<unsigned>(n)), dim3(kTailThreads), 0,
+                         stream, s, tail, padBits(), t0, tailWork.get());
     for (int t = t0 - 1; t >= 0; --t) {
       level(t, kProlong);
       level(t, kPost);
