@@ -67,30 +67,33 @@ __device__ inline int64_t xcdTile(int64_t b, int64_t T) {
 
 // Tile geometry: a block covers 62 interior columns (+1 halo each side, one
 // lane per column) and 30 (FP64) or 62 (FP32) interior rows (+1 halo each
-// side); each of its 4 waves owns a quarter of the tile rows. FP32 tiles are
-// twice as tall (half the halo share); at 64 rows the FP64 kernel would drop
-// to 3 waves per SIMD, which costs more than the halo saves.
-template <class T>
+// side); each of its 4 waves owns a quarter of the tile rows. On large grids
+// FP32 tiles are twice as tall (half the halo share); at 64 rows the FP64
+// kernel would drop to 3 waves per SIMD, which costs more than the halo
+// saves, and on small grids the taller tiles leave CUs idle.
+template <int ROWS>
 struct TileShape {
-  static constexpr int kTileRows = sizeof(T) == 4 ? 64 : 32;
+  static constexpr int kTileRows = ROWS;
   static constexpr int kRowsPerWave = kTileRows / kWaves;
   static constexpr int kInteriorRows = kTileRows - 2;
 };
+// 32-row tiles of a grid with at least this many use 64-row ones in FP32
+constexpr int64_t kTallTileMin = 2048;
 
 // One CG iteration (start == 0) or the start step w0 = A M r0 (start == 1, r0
 // in rOld). Phase 1 loads the tile in one burst, forms the new residual and
 // u = M r for every tile point (into LDS) and writes p, s, r, x of the
 // interior; phase 2 applies the 5-point stencil to u from LDS.
-template <class T>
+template <class T, int ROWS>
 __global__ __launch_bounds__(256) void cgIterKernel(
     Grid2 g, const T* __restrict__ diag, const T* __restrict__ coeff,
     const T* __restrict__ pOld, T* __restrict__ pNew, const T* __restrict__ sOld,
     T* __restrict__ sNew, const T* __restrict__ rOld, T* __restrict__ rNew,
     const T* __restrict__ wOld, T* __restrict__ wNew, T* __restrict__ x, int64_t tilesX,
     KState* st, double* partial, int start) {
-  constexpr int kTileRows = TileShape<T>::kTileRows;
-  constexpr int kRowsPerWave = TileShape<T>::kRowsPerWave;
-  constexpr int kInteriorRows = TileShape<T>::kInteriorRows;
+  constexpr int kTileRows = TileShape<ROWS>::kTileRows;
+  constexpr int kRowsPerWave = TileShape<ROWS>::kRowsPerWave;
+  constexpr int kInteriorRows = TileShape<ROWS>::kInteriorRows;
   __shared__ T uTile[kTileRows][64];
   const bool skip = st->done;
   const T alpha = static_cast<T>(st->alpha);
@@ -203,8 +206,12 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   require(v.dim == 2, "the single-kernel PCG needs a 2D mesh");
   KrylovWork& w = ctx.kw;
   const int64_t tilesX = (v.nx + kOut - 1) / kOut;
-  constexpr int kInteriorRows = TileShape<T>::kInteriorRows;
-  const int64_t tiles = tilesX * ((v.ny + kInteriorRows - 1) / kInteriorRows);
+  const int64_t tiles32 = tilesX * ((v.ny + TileShape<32>::kInteriorRows - 1) /
+                                    TileShape<32>::kInteriorRows);
+  const bool tall = sizeof(T) == 4 && tiles32 >= kTallTileMin;
+  const int64_t tiles = tall ? tilesX * ((v.ny + TileShape<64>::kInteriorRows - 1) /
+                                         TileShape<64>::kInteriorRows)
+                             : tiles32;
   const dim3 grid(static_cast<unsigned>(tiles));
   w.partial.ensure(std::max<int64_t>(kMaxBlocks * 4, static_cast<int64_t>(grid.x) * 3 + 8));
   w.state.ensure(1);
@@ -230,9 +237,14 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   };
   // the iteration kernel; its finalize step is launched by the caller
   auto launch = [&](int cur, int start) {
-    hipLaunchKernelGGL((cgIterKernel<T>), grid, dim3(256), 0, ctx.stream, g, diag, coeff, p[cur],
-                       p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur], wv[1 - cur],
-                       x, tilesX, w.state.get(), w.partial.get(), start);
+    if (tall)
+      hipLaunchKernelGGL((cgIterKernel<T, 64>), grid, dim3(256), 0, ctx.stream, g, diag, coeff,
+                         p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur],
+                         wv[1 - cur], x, tilesX, w.state.get(), w.partial.get(), start);
+    else
+      hipLaunchKernelGGL((cgIterKernel<T, 32>), grid, dim3(256), 0, ctx.stream, g, diag, coeff,
+                         p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur],
+                         wv[1 - cur], x, tilesX, w.state.get(), w.partial.get(), start);
   };
   hipEvent_t t0, t1;
   CFDHIP_CHECK(hipEventCreate(&t0));
