@@ -10,7 +10,8 @@
  *   cfd::gpu::makeGpuCG / makeGpuBiCGSTAB    (include/cfd/gpu/GpuLinearSolver.hpp:48,52)
  *   cfd::gpu::GpuSimpleDiscretization        (include/cfd/gpu/GpuSimpleDiscretization.hpp:56-262)
  *   cfd::gpu::cudaAvailable                   (include/cfd/gpu/GPUBackend.hpp:15)
- * and a maintainer binds them as shown in INTEGRATION.md.
+ * and a maintainer binds them behind those interfaces (a thin pimpl over the
+ * C++ adapter in cfdhip.hpp).
  *
  * Arithmetic: everything on the reference path is FP64. Assembly, gradient,
  * flux, correction and SpMV results are bitwise equal to the reference CPU

@@ -1,8 +1,8 @@
 // cfdhip.hpp -- C++20 host adapter over the C ABI of libcfdhip (cfdhip.h).
 //
 // Presents the shapes the reference's SIMPLE loop already calls, so a
-// maintainer can bind them behind the reference's own interfaces
-// (INTEGRATION.md shows the glue):
+// maintainer can bind them behind the reference's own interfaces with a few
+// lines of glue per seam:
 //   LinearSolverSettings / SolverResult / SolverStatus
 //       include/cfd/algebra/LinearSolver.hpp:16-166
 //   makeGpuCG / makeGpuBiCGSTAB (nullptr when unusable, never throw)

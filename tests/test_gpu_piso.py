@@ -12,7 +12,7 @@ PISO::solveTimeStep and assembleTransientMomentumComponent
     in the max norm: the Krylov reductions are ordered differently), its CFL
     exactly (the pre-step flux is the fixture's) and its continuity;
   * with tight solves (rel 1e-13, the piso2d_32_tight fixture) the fields
-    agree to 1e-10 relative, the north star's bar for the FP64 fields;
+    agree to 1e-10 relative, the bar for the FP64 fields;
   * p' solved by the learned FCG instead (the factory route the reference's
     PISO lacks) gives the same step to 1e-10 with rel 1e-12 solves, in about
     an eighth of BiCGSTAB's iterations; by Jacobi-PCG to 1e-6 with an absolute
