@@ -39,6 +39,8 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
+from cfdapp_amd.workloads import batch_cases, standalone_rhs  # noqa: E402
+
 N = 2048
 SEED = 20480001
 HBM_PEAK_GBS = 8000.0
@@ -74,34 +76,6 @@ def layer_bytes(p0):
     }
 
 
-def splitmix_uniform(seed, count):
-    """splitmix64 -> (x >> 11) * 2^-53, vectorised (same stream as the C driver)."""
-    out = np.empty(count, dtype=np.uint64)
-    s = np.uint64(seed)
-    golden = np.uint64(0x9E3779B97F4A7C15)
-    with np.errstate(over="ignore"):
-        idx = np.arange(1, count + 1, dtype=np.uint64)
-        z = s + golden * idx
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    out[:] = z
-    return (out >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
-
-
-def standalone_rhs(n_cells, n_faces=0, offset=0):
-    """b_i = 2 u_i - 1 from draws offset.., serial mean removed, b_0 = 0."""
-    u = splitmix_uniform(SEED, offset + n_cells)[offset:]
-    b = 2.0 * u - 1.0
-    s = 0.0
-    for chunk in np.array_split(b, 64):  # serial mean in index order
-        for v in chunk.tolist():
-            s += v
-    b -= s / n_cells
-    b[0] = 0.0
-    return b
-
-
 def cpu_baseline(iters_gpu):
     """The reference's own CG + Jacobi (compiled from its sources, OpenMP SpMV),
     bounded sample of 40 iterations at 2048x2048, scaled to the GPU's iteration
@@ -131,24 +105,6 @@ def cpu_baseline(iters_gpu):
             "sample": f"{k} CG+Jacobi iterations on the 2048x2048 p' system "
                       f"({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU solve's "
                       f"{iters_gpu} iterations"}
-
-
-def batch_cases(seed=5005, count=256):
-    """256 independent 2D cases: nx, ny uniform in 64..512 with aspect ratio at
-    most 4 (ny redrawn otherwise), cavity or channel, Re log-uniform in
-    100-3200, unit lid / inlet speed."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        nx = int(rng.integers(64, 513))
-        while True:
-            ny = int(rng.integers(64, 513))
-            if max(nx, ny) <= 4 * min(nx, ny):
-                break
-        kind = "cavity" if rng.random() < 0.5 else "channel"
-        re = float(np.exp(rng.uniform(np.log(100.0), np.log(3200.0))))
-        out.append((kind, nx, ny, re))
-    return out
 
 
 def batch_leg(cfd, log, cases, use_graph=True, sequential=True):

@@ -29,11 +29,16 @@
 //   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
 //                                                 iterations on the seeded p' system
 //                                                 (CPU baseline; prints one JSON line)
+//   bicgstab in=DIR abs rel max [jacobi=1]      -> the reference's BiCGSTAB::solve on the CSR
+//                                                 system of DIR (rowptr.i64 col.i64 val.f64
+//                                                 b.f64 x0.f64): solution, history, status,
+//                                                 iterations, restarts, residuals
 #include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <string>
@@ -650,6 +655,61 @@ void cmdStages() {
 
 // CPU baseline: the reference's CG + Jacobi on the stand-alone p' system
 // (d == 1, cell 0 pinned, seeded mean-free RHS), bounded to pmax iterations.
+std::vector<char> readRaw(const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) {
+    std::perror(path.c_str());
+    std::exit(1);
+  }
+  std::fseek(f, 0, SEEK_END);
+  const long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  std::vector<char> buf(static_cast<size_t>(size));
+  if (size > 0 && std::fread(buf.data(), 1, buf.size(), f) != buf.size()) {
+    std::perror(path.c_str());
+    std::exit(1);
+  }
+  std::fclose(f);
+  return buf;
+}
+template <class T>
+std::vector<T> readArray(const std::string& dir, const std::string& name) {
+  const std::vector<char> raw = readRaw(dir + "/" + name);
+  std::vector<T> v(raw.size() / sizeof(T));
+  if (!v.empty()) std::memcpy(v.data(), raw.data(), v.size() * sizeof(T));
+  return v;
+}
+
+// BiCGSTAB::solve (src/algebra/BiCGSTAB.cpp:48-275) on a stored CSR system.
+void cmdBicgstab() {
+  const std::string in = arg("in");
+  const auto rp = readArray<int64_t>(in, "rowptr.i64");
+  const auto col = readArray<int64_t>(in, "col.i64");
+  auto val = readArray<double>(in, "val.f64");
+  const auto b = readArray<double>(in, "b.f64");
+  const auto x0 = readArray<double>(in, "x0.f64");
+  const size_t n = rp.size() - 1;
+  std::vector<Index> offsets(rp.begin(), rp.end()), columns(col.begin(), col.end());
+  Vector rhs(n), guess(n);
+  for (size_t i = 0; i < n; ++i) {
+    rhs[i] = b[i];
+    guess[i] = x0[i];
+  }
+  const algebra::LinearSystem sys(SparseMatrix(n, n, std::move(val), std::move(columns),
+                                               std::move(offsets)),
+                                  rhs);
+  LinearSolverSettings s;
+  s.type = algebra::LinearSolverType::BiCGSTAB;
+  s.absoluteTolerance = farg("abs");
+  s.relativeTolerance = farg("rel");
+  s.maxIterations = iarg("max");
+  std::shared_ptr<algebra::Preconditioner> pre;
+  if (iarg("jacobi", "0")) pre = std::make_shared<algebra::JacobiPreconditioner>();
+  algebra::BiCGSTAB solver(s, pre);
+  const SolverResult r = solver.solve(sys, guess);
+  dumpResult("bcg", r);
+}
+
 void cmdCgTime() {
   const Case c = makeCavity();
   const Mesh& m = c.mesh;
@@ -706,6 +766,7 @@ int main(int argc, char** argv) {
   else if (cmd == "piso") cmdPiso();
   else if (cmd == "cpcorr") cmdCpcorr();
   else if (cmd == "fpcorr") cmdFpcorr();
+  else if (cmd == "bicgstab") cmdBicgstab();
   else {
     std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
     return 2;

@@ -6,7 +6,12 @@ container only) and packs each command's raw arrays into ``<name>.npz``. Also
 converts the reference's committed 20x20 cavity results
 (``cases/lid_driven_cavity/results/{fields,residuals}.csv``) into
 ``reference_cavity20.npz``, and copies the case files and results of the
-reference's lid_driven_cavity and poiseuille_flow cases into ``cases/``.
+reference's lid_driven_cavity and poiseuille_flow cases into ``cases/``, and
+writes ``bicgstab_*.npz``: the linear systems of the reference's BiCGSTAB
+breakdown tests (tests/unit/algebra/test_bicgstab_breakdown.cpp -- its two
+stored 1024-row systems from tests/data/linear_systems/, its hand-built 3x3
+systems and its scaled convection-diffusion systems) with the outcome of the
+reference's own ``BiCGSTAB::solve`` on each.
 
     python tests/golden/make_golden.py
 """
@@ -105,9 +110,131 @@ def reference_cases():
         print("wrote cases/" + name)
 
 
+# --- BiCGSTAB breakdown / restart systems (tests/unit/algebra/test_bicgstab_breakdown.cpp) ---
+
+STORED = {"smooth": "p12_mesh004_thermal_smooth_a005_n32.txt",
+          "rough": "p12_mesh004_thermal_rough_f03_n32.txt"}
+
+
+def read_stored(name):
+    """The bicg_capture text format of tests/data/linear_systems/: '#' comment
+    lines, then rows nonzeros, row offsets, columns, values, rhs, guess."""
+    toks = []
+    with open(os.path.join(REF, "tests", "data", "linear_systems", name)) as f:
+        for line in f:
+            if line and line[0] != "#":
+                toks.extend(line.split())
+    rows, nnz = int(toks[0]), int(toks[1])
+    k = 2
+    rp = np.array([int(t) for t in toks[k:k + rows + 1]], np.int64)
+    k += rows + 1
+    col = np.array([int(t) for t in toks[k:k + nnz]], np.int64)
+    k += nnz
+    val = np.array([float(t) for t in toks[k:k + nnz]])
+    k += nnz
+    b = np.array([float(t) for t in toks[k:k + rows]])
+    k += rows
+    x0 = np.array([float(t) for t in toks[k:k + rows]])
+    return rp, col, val, b, x0
+
+
+def dense_csr(a):
+    """SparseMatrixBuilder of the test's dense() helper: non-zero entries only."""
+    a = np.asarray(a, dtype=np.float64)
+    rp, col, val = [0], [], []
+    for row in a:
+        for j, v in enumerate(row):
+            if v != 0.0:
+                col.append(j)
+                val.append(v)
+        rp.append(len(col))
+    return np.array(rp, np.int64), np.array(col, np.int64), np.array(val)
+
+
+def convection_diffusion(n, scale):
+    """The test's 1D upwind convection-diffusion system (cell Peclet 0.5)."""
+    pe = 0.5
+    rp, col, val = [0], [], []
+    for i in range(n):
+        if i > 0:
+            col.append(i - 1)
+            val.append(-(1.0 + pe) * scale)
+        col.append(i)
+        val.append((2.0 + pe) * scale)
+        if i + 1 < n:
+            col.append(i + 1)
+            val.append(-1.0 * scale)
+        rp.append(len(col))
+    b = np.array([np.sin(0.1 * i) * scale for i in range(n)])
+    return np.array(rp, np.int64), np.array(col, np.int64), np.array(val), b
+
+
+def solve_reference(rp, col, val, b, x0, abs_tol, rel_tol, max_iter):
+    with tempfile.TemporaryDirectory() as d:
+        for name, a in (("rowptr.i64", rp), ("col.i64", col), ("val.f64", val), ("b.f64", b),
+                        ("x0.f64", x0)):
+            np.ascontiguousarray(a).tofile(os.path.join(d, name))
+        out = os.path.join(d, "out")
+        os.makedirs(out)
+        subprocess.run([DRIVER, "bicgstab", f"in={d}", f"abs={abs_tol!r}", f"rel={rel_tol!r}",
+                        f"max={max_iter}", f"out={out}"], check=True)
+        read = lambda n, t: np.fromfile(os.path.join(out, n), dtype=t)
+        return dict(x=read("bcg_x.f64", np.float64), hist=read("bcg_hist.f64", np.float64),
+                    meta=read("bcg_meta.i64", np.int64), res=read("bcg_res.f64", np.float64))
+
+
+def bicgstab_fixtures():
+    """bicgstab_stored.npz: the two stored systems at the default settings
+    (abs 1e-12, rel 1e-10, 1000 iterations). bicgstab_small.npz: the 3x3 /
+    2x2 / 1x1 systems at the test's strict settings (abs 0, rel 1e-14, 50) and
+    the convection-diffusion systems (n 60 at scales 1, 2^-100, 2^100 and
+    n 200 at 1, 2^-40; abs 0, rel 1e-10, 500 / 2000). Keys: <case>_{rowptr,
+    col, val, b, x0, x, hist, meta (status, iterations, restarts), res
+    (initial, final), settings (abs, rel, max)}."""
+    groups = {"stored": [], "small": []}
+    for key, fname in STORED.items():
+        rp, col, val, b, x0 = read_stored(fname)
+        groups["stored"].append((key, rp, col, val, b, x0, (1e-12, 1e-10, 1000)))
+    strict = (0.0, 1e-14, 50)
+    hand = {
+        "rotation": ([[0, 1], [-1, 0]], [1, 0]),
+        "omega": ([[-2, -2, -2], [-2, -2, -1], [-2, -1, 0]], [1, 1, 0]),
+        "rho_no_progress": ([[-2, -2, -2], [-2, -2, -1], [2, -2, -2]], [1, 0, 0]),
+        "restart": ([[-2, -2, -2], [-2, -2, 2], [2, -2, -2]], [1, 0, 1]),
+        "exact_1x1": ([[2.0]], [2.0]),
+        "zero_rhs": ([[4, 1], [2, 3]], [0.0, 0.0]),
+    }
+    for key, (a, b) in hand.items():
+        rp, col, val = dense_csr(a)
+        b = np.array(b, dtype=np.float64)
+        groups["small"].append((key, rp, col, val, b, np.zeros(len(b)), strict))
+    rp, col, val = dense_csr([[4, 1], [2, 3]])
+    groups["small"].append(("exact_guess", rp, col, val, np.array([7.0, 11.0]),
+                            np.array([1.0, 3.0]), strict))
+    for n, e, mx in ((60, 0, 500), (60, -100, 500), (60, 100, 500), (200, 0, 2000),
+                     (200, -40, 2000)):
+        rp, col, val, b = convection_diffusion(n, np.ldexp(1.0, e))
+        tag = f"cd{n}_{'m' if e < 0 else 'p'}{abs(e)}"
+        groups["small"].append((tag, rp, col, val, b, np.zeros(n), (0.0, 1e-10, mx)))
+    for group, items in groups.items():
+        arrays = {}
+        for key, rp, col, val, b, x0, (a, r, m) in items:
+            out = solve_reference(rp, col, val, b, x0, a, r, m)
+            arrays.update({f"{key}_rowptr": rp, f"{key}_col": col, f"{key}_val": val,
+                           f"{key}_b": b, f"{key}_x0": x0,
+                           f"{key}_settings": np.array([a, r, float(m)])})
+            arrays.update({f"{key}_{k}": v for k, v in out.items()})
+        arrays["cases"] = np.array([it[0] for it in items])
+        np.savez_compressed(os.path.join(HERE, f"bicgstab_{group}.npz"), **arrays)
+        print(f"wrote bicgstab_{group}")
+
+
 def main(names):
     if not os.path.exists(DRIVER):
         sys.exit("build oracle/_ref first: make -C oracle -f Makefile.ref")
+    if names == ["bicgstab"]:
+        bicgstab_fixtures()
+        return
     for name in names or FIXTURES:
         cmd, params = FIXTURES[name]
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **run(cmd, params))
@@ -116,6 +243,7 @@ def main(names):
         np.savez_compressed(os.path.join(HERE, "reference_cavity20.npz"), **reference_cavity20())
         print("wrote reference_cavity20")
         reference_cases()
+        bicgstab_fixtures()
 
 
 if __name__ == "__main__":

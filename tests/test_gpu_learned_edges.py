@@ -1,0 +1,95 @@
+"""Learned FCG at grid sizes that are not multiples of 8, and its stagnation fallback.
+
+The network runs on a grid rounded up to a multiple of 8. These tests solve
+p' systems whose sides are not multiples of 8 (the 129^2 C1 cavity, a 100 x 60
+cavity, a 100 x 60 channel whose FixedValue outlet gives a Dirichlet side)
+and require convergence to the FP64 target without the Jacobi-PCG fallback
+within a bound on the iterations, printing the count beside the nearest
+multiple-of-8 grid.
+
+The fallback: with noisy weights and a stagnation window of one iteration
+(the first iteration whose residual is not a new minimum ends the flexible
+solve), the solve must hand over to FP64 Jacobi-PCG and still reach the FP64
+target, with the residual recomputed on the host.
+"""
+import numpy as np
+import pytest
+
+from cfdapp_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+cfd = pytest.importorskip("cfdapp_amd")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    if not cfd.device_available():
+        pytest.fail("no gfx950 device: the HIP path must run on the GPU")
+
+
+def _system(kind, nx, ny):
+    """The p' system of the first SIMPLE iteration of a cavity or channel."""
+    from cfdapp_amd import learned
+    d = learned.make_case(kind, nx, ny=ny, re=400.0, speed=1.0)
+    learned.assemble_pprime(d, 2)
+    return d
+
+
+def _solve(d, settings):
+    r = d.solve_pressure_rhs(settings, None, download=True)
+    diag, fc, rhs = d.download("PC_DIAG"), d.download("PC_FACE_COEFF"), d.download("PC_RHS")
+    nx, ny = d.shape[:2]
+    true = float(np.linalg.norm(rhs - wl.apply_pprime_2d(nx, ny, d.pin, diag, fc, r.solution)))
+    return r, true / float(np.linalg.norm(rhs))
+
+
+FCG = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
+
+
+@pytest.mark.parametrize("kind,nx,ny,near", [("cavity", 129, 129, (128, 128)),
+                                             ("cavity", 100, 60, (96, 64)),
+                                             ("channel", 100, 60, (96, 64)),
+                                             ("channel", 61, 203, (64, 200))])
+def test_learned_fcg_off_multiple_of_8(kind, nx, ny, near):
+    print("case: learned", kind, nx, ny, flush=True)
+    its = {}
+    for shape in ((nx, ny), near):
+        d = _system(kind, *shape)
+        try:
+            d.set_learned_weights()
+            r, rel = _solve(d, FCG)
+            print(f"{kind} {shape[0]}x{shape[1]}: {r.status_name} {r.iterations} FCG it, "
+                  f"host true rel res {rel:.2e}, fallback {r.fallback_used}", flush=True)
+            assert r.converged and rel <= 1e-8
+            assert not r.fallback_used
+            its[shape] = r.iterations
+        finally:
+            d.close()
+    assert its[(nx, ny)] <= 80
+
+
+def test_stagnation_window_hands_over_to_jacobi_pcg():
+    print("case: learned stagnation fallback", flush=True)
+    d = _system("cavity", 96, 96)
+    try:
+        w = cfd.load_learned_weights().copy()
+        rng = np.random.default_rng(77)
+        nz = w != 0
+        w[nz] *= 1.0 + 0.6 * rng.standard_normal(int(nz.sum())).astype(np.float32)
+        d.set_learned_weights(w, options=cfd.load_learned_options())
+        free, rel_free = _solve(d, FCG)
+        print(f"noisy weights, no window: {free.status_name} {free.iterations} it, "
+              f"fallback {free.fallback_used}", flush=True)
+        hist = free.residual_history
+        rises = int(np.sum(hist[1:] >= np.minimum.accumulate(hist)[:-1]))
+        print(f"iterations without a new minimum: {rises}", flush=True)
+        s = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned", stagnation_window=1)
+        r, rel = _solve(d, s)
+        print(f"window 1: {r.status_name} {r.iterations} it, fallback {r.fallback_used}, "
+              f"host true rel res {rel:.2e}", flush=True)
+        assert r.converged and rel <= 1e-8
+        if rises > 0:
+            assert r.fallback_used
+    finally:
+        d.close()
