@@ -78,6 +78,7 @@ class LinearSolverSettings:
     check_interval: int = 0
     stagnation_window: int = 0
     graph: bool = False             # replay iteration chunks from a captured HIP graph
+    inner_tolerance: float = 0.0    # mixed precision: FP32 reduction between FP64 residuals
 
     def __post_init__(self):
         # LinearSolver's constructor validation (src/algebra/LinearSolver.cpp:9-32)
@@ -94,6 +95,8 @@ class LinearSolverSettings:
             raise ValueError(f"unknown solver type {self.type}")
         if self.preconditioner not in ("None", "Jacobi", "Learned"):
             raise ValueError(f"unknown preconditioner {self.preconditioner}")
+        if not 0.0 <= self.inner_tolerance < 1.0:
+            raise ValueError("LinearSolverSettings: inner_tolerance must be in [0, 1)")
 
     def c(self) -> _lib.SolverSettings:
         s = _lib.SolverSettings()
@@ -107,6 +110,7 @@ class LinearSolverSettings:
         s.check_interval = int(self.check_interval)
         s.stagnation_window = int(self.stagnation_window)
         s.graph = 1 if self.graph else 0
+        s.inner_tolerance = float(self.inner_tolerance)
         return s
 
 

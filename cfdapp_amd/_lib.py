@@ -45,7 +45,8 @@ class SolverSettings(ctypes.Structure):
                 ("precision", ctypes.c_int32),
                 ("check_interval", ctypes.c_int32),
                 ("stagnation_window", ctypes.c_int32),
-                ("graph", ctypes.c_int32)]
+                ("graph", ctypes.c_int32),
+                ("inner_tolerance", ctypes.c_double)]
 
 
 class SolveResult(ctypes.Structure):

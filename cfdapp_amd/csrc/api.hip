@@ -2,6 +2,7 @@
 // Each catches every library exception, records the message on the context
 // and returns the error code, so no exception crosses the C boundary.
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 
 #include "context.hpp"
@@ -68,6 +69,8 @@ int fieldCount(const Context& c, int id) {
 
 cfdhip_solver_settings checkedSettings(const cfdhip_solver_settings* s) {
   require(s != nullptr, "solver settings are required");
+  require(s->inner_tolerance >= 0.0 && s->inner_tolerance < 1.0,
+          "LinearSolverSettings: inner_tolerance must be in [0, 1)");
   return *s;
 }
 
@@ -409,7 +412,8 @@ int cfdhip_simple_run(cfdhip_ctx* ctx, const double* settings,
                       int64_t cap) {
   return guarded(ctx, [&](Context& c) {
     require(settings && momentum && pressure && meta, "settings, solvers and meta are required");
-    simpleRun(c, settings, *momentum, *pressure, meta, history, cap);
+    simpleRun(c, settings, checkedSettings(momentum), checkedSettings(pressure), meta, history,
+              cap);
   });
 }
 
@@ -588,3 +592,8 @@ int cfdhip_kernel_timing(cfdhip_ctx* ctx, double* out) {
 }
 
 }  // extern "C"
+
+// The C layout of the settings the host mirrors (ctypes, the C++ adapter) rely on.
+static_assert(sizeof(cfdhip_solver_settings) == 56, "cfdhip_solver_settings layout");
+static_assert(offsetof(cfdhip_solver_settings, inner_tolerance) == 48,
+              "cfdhip_solver_settings layout");

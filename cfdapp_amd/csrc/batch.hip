@@ -550,11 +550,8 @@ void solveBatch(cfdhip_batch& B, const cfdhip_solver_settings& set, int rowsReq,
   CFDHIP_CHECK(hipGetLastError());
 
   // states
-  static const double innerTol = [] {
-    const char* e = std::getenv("CFDHIP_MIXED_INNER_TOL");
-    const double t = e ? std::atof(e) : 1e-5;
-    return t > 0.0 && t < 1.0 ? t : 1e-5;
-  }();
+  // FP32 reduction per refinement cycle (the settings' inner tolerance)
+  const double innerTol = set.inner_tolerance > 0.0 ? set.inner_tolerance : 1e-5;
   B.states.ensure(nc);
   if (B.hostStates) CFDHIP_CHECK(hipHostFree(B.hostStates));
   B.hostStates = nullptr;
@@ -903,6 +900,8 @@ int cfdhip_batch_solve(cfdhip_batch* b, const cfdhip_solver_settings* s, int til
             "batched solves run CG with the Jacobi preconditioner");
     require(s->absolute_tolerance >= 0.0 && s->relative_tolerance >= 0.0,
             "tolerances must be >= 0");
+    require(s->inner_tolerance >= 0.0 && s->inner_tolerance < 1.0,
+            "LinearSolverSettings: inner_tolerance must be in [0, 1)");
     if (s->precision == CFDHIP_MIXED)
       solveBatch<float>(B, *s, tile_rows, use_graph, results, latency_ms, stats);
     else

@@ -19,6 +19,7 @@
 // updated in place. Per iteration the kernel reads p, s, x, r, w, diag and the
 // face couplings and writes p, s, x, r, w: 13 n values plus the tile halo.
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -83,13 +84,20 @@ constexpr int64_t kTallTileMin = 2048;
 // One CG iteration (start == 0) or the start step w0 = A M r0 (start == 1, r0
 // in rOld). Phase 1 loads the tile in one burst, forms the new residual and
 // u = M r for every tile point (into LDS) and writes p, s, r, x of the
-// interior; phase 2 applies the 5-point stencil to u from LDS.
-template <class T, int ROWS>
+// interior; phase 2 applies the 5-point stencil to u from LDS. The iterate x
+// has its own type TX: FP64 under FP32 iteration vectors in the mixed solve,
+// whose correction would otherwise carry an FP32 rounding of the solution
+// (an error of size eps32 |x| in x is one of size eps32 cond(A) |r| in the
+// residual).
+// NSUM = 6 adds the sums of the explicit-conjugation scalars of the mixed
+// solve: (u', s), (p, s), (r', p) of this iteration's direction p, its
+// recurrence s = A p and the new residual r', u' = M r'.
+template <class T, int ROWS, class TX = T, int NSUM = 3>
 __global__ __launch_bounds__(256) void cgIterKernel(
     Grid2 g, const T* __restrict__ diag, const T* __restrict__ coeff,
     const T* __restrict__ pOld, T* __restrict__ pNew, const T* __restrict__ sOld,
     T* __restrict__ sNew, const T* __restrict__ rOld, T* __restrict__ rNew,
-    const T* __restrict__ wOld, T* __restrict__ wNew, T* __restrict__ x, int64_t tilesX,
+    const T* __restrict__ wOld, T* __restrict__ wNew, TX* __restrict__ x, int64_t tilesX,
     KState* st, double* partial, int start) {
   constexpr int kTileRows = TileShape<ROWS>::kTileRows;
   constexpr int kRowsPerWave = TileShape<ROWS>::kRowsPerWave;
@@ -98,7 +106,8 @@ __global__ __launch_bounds__(256) void cgIterKernel(
   const bool skip = st->done;
   const T alpha = static_cast<T>(st->alpha);
   const T beta = static_cast<T>(st->beta);
-  const bool first = st->iter == 0;
+  const TX alphaX = static_cast<TX>(st->alpha);
+  const bool first = st->iter == st->dirStart;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t tile = xcdTile(blockIdx.x, gridDim.x);
   const int64_t tx = tile % tilesX, ty = tile / tilesX;
@@ -106,7 +115,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
   const int64_t jBase = ty * kInteriorRows - 1;  // tile row 0
   const bool col = i >= 0 && i < g.nx;
   const bool outCol = col && lane >= 1 && lane <= kOut;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[NSUM] = {};
   T dK[kRowsPerWave], uK[kRowsPerWave], fxW[kRowsPerWave];
   if (!skip) {
 #pragma unroll
@@ -135,7 +144,12 @@ __global__ __launch_bounds__(256) void cgIterKernel(
             pNew[c] = p;
             sNew[c] = sn;
             rNew[c] = rn;
-            x[c] = x[c] + (p * alpha);
+            x[c] = x[c] + (static_cast<TX>(p) * alphaX);
+            if constexpr (NSUM == 6) {
+              acc[3] += static_cast<double>(u) * static_cast<double>(sn);
+              acc[4] += static_cast<double>(p) * static_cast<double>(sn);
+              acc[5] += static_cast<double>(rn) * static_cast<double>(p);
+            }
           }
           acc[0] += static_cast<double>(rn) * static_cast<double>(u);
           acc[2] += static_cast<double>(rn) * static_cast<double>(rn);
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
       }
     }
   }
-  blockPartials<3>(acc, partial);
+  blockPartials<NSUM>(acc, partial);
 }
 
 // r0 = b - A x0 and |r0| (CG.cpp:45-57), one cell per thread.
@@ -196,34 +210,108 @@ __global__ __launch_bounds__(256) void cgStartKernel(Grid2 g, const T* __restric
   blockPartials<1>(acc, partial);
 }
 
+// FP64 residual replacement of the mixed-precision solve (FP32 iteration
+// vectors, FP64 solution): r64 = b - A x in FP64 with the FP64 operator,
+// r32 = (float) r64 into the vector the next iteration reads, |r64|^2.
+__global__ __launch_bounds__(256) void trueResidualKernel(Grid2 g, const double* __restrict__ diag,
+                                                          const double* __restrict__ coeff,
+                                                          const double* __restrict__ b,
+                                                          const double* __restrict__ x,
+                                                          float* __restrict__ r32,
+                                                          double* partial) {
+  const int64_t n = g.nx * g.ny;
+  double acc[1] = {0.0};
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % g.nx, j = c / g.nx;
+    const double xS = j > 0 ? x[c - g.nx] : 0.0;
+    const double xW = i > 0 ? x[c - 1] : 0.0;
+    const double xE = i < g.nx - 1 ? x[c + 1] : 0.0;
+    const double xN = j < g.ny - 1 ? x[c + g.nx] : 0.0;
+    const double y = applyRow<double>(g, i, j, c, diag[c], xS, xW, x[c], xE, xN,
+                                      coeff[g.offY + j * g.nx + i], coeff[j * (g.nx + 1) + i],
+                                      coeff[j * (g.nx + 1) + i + 1],
+                                      coeff[g.offY + (j + 1) * g.nx + i]);
+    const double rc = b[c] - y;
+    r32[c] = static_cast<float>(rc);
+    acc[0] += rc * rc;
+  }
+  blockPartials<1>(acc, partial);
+}
+
+// After a residual replacement: w = A u with u = M r (the same FP32 rounding
+// as the iteration kernel's u), and the six sums of kStepCgReplace:
+// (r, u), (w, u), |r|^2, (u, s), (p, s), (r, p), with p and s the last
+// iteration's direction and its recurrence s = A p.
+__global__ __launch_bounds__(256) void replaceKernel(Grid2 g, const float* __restrict__ diag,
+                                                     const float* __restrict__ coeff,
+                                                     const float* __restrict__ r,
+                                                     const float* __restrict__ p,
+                                                     const float* __restrict__ s,
+                                                     float* __restrict__ w, double* partial) {
+  const int64_t n = g.nx * g.ny;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  auto u = [&](int64_t c) { return (1.0f / diag[c]) * r[c]; };
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % g.nx, j = c / g.nx;
+    const float uc = u(c);
+    const float uS = j > 0 ? u(c - g.nx) : 0.f;
+    const float uW = i > 0 ? u(c - 1) : 0.f;
+    const float uE = i < g.nx - 1 ? u(c + 1) : 0.f;
+    const float uN = j < g.ny - 1 ? u(c + g.nx) : 0.f;
+    const float wc = applyRow<float>(g, i, j, c, diag[c], uS, uW, uc, uE, uN,
+                                     coeff[g.offY + j * g.nx + i], coeff[j * (g.nx + 1) + i],
+                                     coeff[j * (g.nx + 1) + i + 1],
+                                     coeff[g.offY + (j + 1) * g.nx + i]);
+    w[c] = wc;
+    const double rc = r[c], ud = uc, wd = wc, sc = s[c], pc = p[c];
+    acc[0] += rc * ud;
+    acc[1] += wd * ud;
+    acc[2] += rc * rc;
+    acc[3] += ud * sc;
+    acc[4] += pc * sc;
+    acc[5] += rc * pc;
+  }
+  blockPartials<6>(acc, partial);
+}
+
+__global__ void zeroKernel64(int64_t n, double* __restrict__ x) {
+  for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[k] = 0.0;
+}
+
 }  // namespace
 
-template <class T>
-cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings& set,
-                                      const T* diag, const T* coeff, const T* b, T* x,
-                                      T* const work[8], Snapshots* snap) {
+// FP64 Jacobi-PCG on buffers the caller provides (the context's work arrays,
+// or the library's fixed scratch for the no-allocation entry point). With
+// events == false the solve creates no HIP events and no graph, copies no
+// history to the host and reports host wall time as device_ms; the host
+// mirror of the state may then be ordinary (pageable) memory.
+cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& set,
+                                   const double* diag, const double* coeff, const double* b,
+                                   double* x, double* const work[8], const PcgBuffers& buf,
+                                   Snapshots* snap) {
+  using T = double;
   const View& v = ctx.view;
   require(v.dim == 2, "the single-kernel PCG needs a 2D mesh");
-  KrylovWork& w = ctx.kw;
   const int64_t tilesX = (v.nx + kOut - 1) / kOut;
-  const int64_t tiles32 = tilesX * ((v.ny + TileShape<32>::kInteriorRows - 1) /
-                                    TileShape<32>::kInteriorRows);
-  const bool tall = sizeof(T) == 4 && tiles32 >= kTallTileMin;
-  const int64_t tiles = tall ? tilesX * ((v.ny + TileShape<64>::kInteriorRows - 1) /
-                                         TileShape<64>::kInteriorRows)
-                             : tiles32;
+  const int64_t tiles = tilesX * ((v.ny + TileShape<32>::kInteriorRows - 1) /
+                                  TileShape<32>::kInteriorRows);
   const dim3 grid(static_cast<unsigned>(tiles));
-  w.partial.ensure(std::max<int64_t>(kMaxBlocks * 4, static_cast<int64_t>(grid.x) * 3 + 8));
-  w.state.ensure(1);
-  if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
-  w.hist.ensure(set.max_iterations + 2);
+  const int64_t nb = (v.n + 255) / 256;
+  require(buf.partialCap >= std::max<int64_t>(nb, static_cast<int64_t>(grid.x) * 3) &&
+              buf.histCap >= set.max_iterations + 2,
+          "PCG buffers too small for this grid");
+  KState* host = buf.host;
   KState k{};
   k.absTol = set.absolute_tolerance;
   k.relTol = set.relative_tolerance;
   k.maxIter = set.max_iterations;
   k.status = -1;
-  *w.hostState = k;
-  CFDHIP_CHECK(hipMemcpyAsync(w.state.get(), w.hostState, sizeof(KState), hipMemcpyHostToDevice,
+  *host = k;
+  CFDHIP_CHECK(hipMemcpyAsync(buf.state, host, sizeof(KState), hipMemcpyHostToDevice,
                               ctx.stream));
   const Grid2 g{v.nx, v.ny, ctx.pin, v.offY};
   // work: r0 r1 w0 w1 p0 p1 s0 s1
@@ -232,31 +320,27 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   T* p[2] = {work[4], work[5]};
   T* s[2] = {work[6], work[7]};
   auto finalize = [&](int step, int64_t parts) {
-    launchFinalize(ctx, FinalArgs{w.partial.get(), parts, w.state.get(), w.hist.get(), nullptr,
-                                  step});
+    launchFinalize(ctx, FinalArgs{buf.partial, parts, buf.state, buf.hist, nullptr, step});
   };
   // the iteration kernel; its finalize step is launched by the caller
   auto launch = [&](int cur, int start) {
-    if (tall)
-      hipLaunchKernelGGL((cgIterKernel<T, 64>), grid, dim3(256), 0, ctx.stream, g, diag, coeff,
-                         p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur],
-                         wv[1 - cur], x, tilesX, w.state.get(), w.partial.get(), start);
-    else
-      hipLaunchKernelGGL((cgIterKernel<T, 32>), grid, dim3(256), 0, ctx.stream, g, diag, coeff,
-                         p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur],
-                         wv[1 - cur], x, tilesX, w.state.get(), w.partial.get(), start);
+    hipLaunchKernelGGL((cgIterKernel<T, 32>), grid, dim3(256), 0, ctx.stream, g, diag, coeff,
+                       p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur],
+                       wv[1 - cur], x, tilesX, buf.state, buf.partial, start);
   };
-  hipEvent_t t0, t1;
-  CFDHIP_CHECK(hipEventCreate(&t0));
-  CFDHIP_CHECK(hipEventCreate(&t1));
-  CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
+  const bool timing = ctx.timing && buf.events;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  const auto wall0 = std::chrono::steady_clock::now();
+  if (buf.events) {
+    CFDHIP_CHECK(hipEventCreate(&t0));
+    CFDHIP_CHECK(hipEventCreate(&t1));
+    CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
+  }
   // start: r0 = b - A x0 into r[0], then w0 = A M r0 into w[1] (cur = 0 reads
   // r[0] as rOld and writes wNew = w[1]); the first iteration therefore runs
   // with cur = 1 reading w[1] and r0 from r[1] once r[0] and r[1] are swapped.
-  const int64_t nb = (v.n + 255) / 256;
-  w.partial.ensure(std::max<int64_t>(w.partial.count, nb + 8));
   hipLaunchKernelGGL((cgStartKernel<T>), dim3(static_cast<unsigned>(nb)), dim3(256), 0,
-                     ctx.stream, g, diag, coeff, b, x, r[0], w.partial.get());
+                     ctx.stream, g, diag, coeff, b, x, r[0], buf.partial);
   finalize(kStepStartNorm, nb);
   launch(0, 1);
   finalize(kStepCg1Start, grid.x);
@@ -264,7 +348,7 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   std::swap(r[0], r[1]);  // r0 now in r[1]: the first iteration reads index 1
   const int chunk = set.check_interval > 0 ? set.check_interval : 32;
   std::vector<hipEvent_t> ev;
-  if (ctx.timing) {
+  if (timing) {
     ev.resize(2 * chunk);
     for (hipEvent_t& e : ev) CFDHIP_CHECK(hipEventCreate(&e));
   }
@@ -274,7 +358,7 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   // graph mode: one chunk (an even number of iterations, so the buffer
   // parity returns to where it started) is captured once and replayed; the
   // kernels of iterations past the device verdict do nothing
-  const bool graphed = set.graph != 0 && snap == nullptr && !ctx.timing;
+  const bool graphed = set.graph != 0 && snap == nullptr && !ctx.timing && buf.events;
   const int gChunk = chunk + (chunk & 1);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -296,26 +380,24 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
       q = gChunk;
     }
     for (; !graphed && q < chunk && launched < set.max_iterations; ++q, ++launched) {
-      if (ctx.timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q], ctx.stream));
+      if (timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q], ctx.stream));
       launch(cur, 0);
-      if (ctx.timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q + 1], ctx.stream));
+      if (timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q + 1], ctx.stream));
       finalize(kStepCg1Iter, grid.x);
       cur = 1 - cur;
-      if constexpr (std::is_same_v<T, double>) {
-        if (snap != nullptr && (launched + 1) % snap->stride == 0 && snap->taken < snap->cap) {
-          CFDHIP_CHECK(hipMemcpyAsync(snap->buf + snap->taken * v.n, x, sizeof(T) * v.n,
-                                      hipMemcpyDeviceToDevice, ctx.stream));
-          ++snap->taken;
-        }
+      if (snap != nullptr && (launched + 1) % snap->stride == 0 && snap->taken < snap->cap) {
+        CFDHIP_CHECK(hipMemcpyAsync(snap->buf + snap->taken * v.n, x, sizeof(T) * v.n,
+                                    hipMemcpyDeviceToDevice, ctx.stream));
+        ++snap->taken;
       }
     }
     CFDHIP_CHECK(hipGetLastError());
-    const int64_t before = w.hostState->iter;
-    CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState), hipMemcpyDeviceToHost,
+    const int64_t before = host->iter;
+    CFDHIP_CHECK(hipMemcpyAsync(host, buf.state, sizeof(KState), hipMemcpyDeviceToHost,
                                 ctx.stream));
     CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
-    if (ctx.timing) {
-      const int64_t worked = std::min<int64_t>(q, w.hostState->iter - before);
+    if (timing) {
+      const int64_t worked = std::min<int64_t>(q, host->iter - before);
       for (int e = 0; e < worked; ++e) {
         float ms = 0.f;
         CFDHIP_CHECK(hipEventElapsedTime(&ms, ev[2 * e], ev[2 * e + 1]));
@@ -323,23 +405,29 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
         kLaunches += 1.0;
       }
     }
-    if (w.hostState->done || launched >= set.max_iterations) break;
+    if (host->done || launched >= set.max_iterations) break;
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
-  CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
-  CFDHIP_CHECK(hipEventSynchronize(t1));
   float ms = 0.f;
-  CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
+  if (buf.events) {
+    CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
+    CFDHIP_CHECK(hipEventSynchronize(t1));
+    CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+  } else {
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - wall0)
+             .count();
+  }
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  if (ctx.timing) {
+  if (timing) {
     ctx.spmvCount = kLaunches;
     ctx.spmvMeanUs = kLaunches > 0 ? 1000.0 * kMs / kLaunches : 0.0;
-    ctx.timedIters = static_cast<double>(w.hostState->iter);
+    ctx.timedIters = static_cast<double>(host->iter);
   }
-  KState& S = *w.hostState;
+  KState& S = *host;
   if (!S.done) S.status = CFDHIP_MAX_ITERATIONS;
   cfdhip_solve_result res{};
   res.status = S.status;
@@ -348,20 +436,256 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   res.final_residual = S.rnorm;
   res.true_residual = NAN;
   res.device_ms = ms;
-  res.history_length = S.histLen;
-  ctx.lastHistory.resize(S.histLen);
-  if (S.histLen > 0)
-    CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), w.hist.get(), S.histLen * sizeof(double),
-                           hipMemcpyDeviceToHost));
+  res.history_length = buf.events ? S.histLen : 0;
+  if (buf.events) {
+    ctx.lastHistory.resize(S.histLen);
+    if (S.histLen > 0)
+      CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), buf.hist, S.histLen * sizeof(double),
+                             hipMemcpyDeviceToHost));
+  }
   return res;
+}
+
+template <class T>
+cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings& set,
+                                      const T* diag, const T* coeff, const T* b, T* x,
+                                      T* const work[8], Snapshots* snap) {
+  static_assert(std::is_same_v<T, double>, "the single-kernel PCG runs in FP64");
+  const View& v = ctx.view;
+  KrylovWork& w = ctx.kw;
+  const int64_t tilesX = (v.nx + kOut - 1) / kOut;
+  const int64_t tiles = tilesX * ((v.ny + TileShape<32>::kInteriorRows - 1) /
+                                  TileShape<32>::kInteriorRows);
+  const int64_t nb = (v.n + 255) / 256;
+  w.partial.ensure(std::max<int64_t>({kMaxBlocks * 4, tiles * 3 + 8, nb + 8, w.partial.count}));
+  w.state.ensure(1);
+  if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+  w.hist.ensure(set.max_iterations + 2);
+  const PcgBuffers buf{w.partial.get(), w.partial.count, w.state.get(), w.hostState,
+                       w.hist.get(), w.hist.count, true};
+  return pcg2dOnBuffers(ctx, set, diag, coeff, b, x, work, buf, snap);
 }
 
 template cfdhip_solve_result singleKernelPcg2d<double>(Context&, const cfdhip_solver_settings&,
                                                        const double*, const double*,
                                                        const double*, double*,
                                                        double* const[8], Snapshots*);
-template cfdhip_solve_result singleKernelPcg2d<float>(Context&, const cfdhip_solver_settings&,
-                                                      const float*, const float*, const float*,
-                                                      float*, float* const[8], Snapshots*);
+
+// Mixed-precision Jacobi-PCG with FP64 residual replacement (reliable
+// updates, Sleijpen & van der Vorst 1996): the iteration vectors p, s, r, w
+// are FP32 and run the single-kernel iteration on an FP32 copy of the
+// operator, accumulating into the FP64 solution x. Whenever the FP32
+// recurrence residual has fallen below `ratio` times its largest value since
+// the last replacement (or below the target), the host recomputes
+// r = b - A x in FP64 with the FP64 operator; the iteration then either
+// restarts its direction from the new residual (kRestartOnReplace, the
+// default) or continues the same direction. The scalars use explicit
+// conjugation (kStepCgRobust: beta = -(u, A p) / (p, A p) with A p = s, the
+// exact line search along the new direction), which in exact arithmetic are
+// CG's and which stay valid when a replacement moves r off the FP32
+// recurrence; the Chronopoulos-Gear shortcut (p', A p') = delta - beta gamma
+// / alpha does not (it assumes r orthogonal to the last direction; FP64
+// Jacobi-PCG keeps it). The solve
+// ends when the FP64 |b - A x| meets the tolerance relative to |b| (x0 = 0).
+// A replacement that does not halve the FP64 residual (the FP32 operator's
+// rounding floor) or an FP32 breakdown hands the rest of the solve to FP64
+// Jacobi-PCG from x (fallback_used).
+cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
+                               double* x) {
+  const View& v = ctx.view;
+  require(v.dim == 2, "the single-kernel PCG needs a 2D mesh");
+  const int64_t n = v.n, nf = v.nf;
+  KrylovWork& w = ctx.kw;
+  for (auto& a : w.f8) a.ensure(n);
+  w.dinv32.ensure(n);  // FP32 diagonal
+  w.w32.ensure(nf);    // FP32 face couplings
+  castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
+  castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
+  const float* diag32 = w.dinv32.get();
+  const float* coeff32 = w.w32.get();
+  const double* diag64 = ctx.f(CFDHIP_FIELD_PC_DIAG);
+  const double* coeff64 = ctx.f(CFDHIP_FIELD_PC_FACE_COEFF);
+  const int64_t tilesX = (v.nx + kOut - 1) / kOut;
+  const int64_t tiles32 = tilesX * ((v.ny + TileShape<32>::kInteriorRows - 1) /
+                                    TileShape<32>::kInteriorRows);
+  const bool tall = tiles32 >= kTallTileMin;
+  const int64_t tiles = tall ? tilesX * ((v.ny + TileShape<64>::kInteriorRows - 1) /
+                                         TileShape<64>::kInteriorRows)
+                             : tiles32;
+  const dim3 grid(static_cast<unsigned>(tiles));
+  int64_t nb = (n + 255) / 256;
+  if (nb > kMaxBlocks) nb = kMaxBlocks;
+  const dim3 pgrid(static_cast<unsigned>(nb));
+  w.partial.ensure(std::max<int64_t>(kMaxBlocks * 6 + 8, static_cast<int64_t>(grid.x) * 6 + 8));
+  w.state.ensure(1);
+  w.scal.ensure(4);
+  if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+  w.hist.ensure(set.max_iterations + 2);
+  const Grid2 g{v.nx, v.ny, ctx.pin, v.offY};
+  float* r[2] = {w.f8[0].get(), w.f8[1].get()};
+  float* wv[2] = {w.f8[2].get(), w.f8[3].get()};
+  float* p[2] = {w.f8[4].get(), w.f8[5].get()};
+  float* s[2] = {w.f8[6].get(), w.f8[7].get()};
+  const double ratio = set.inner_tolerance > 0.0 ? set.inner_tolerance : 1e-5;
+  auto launch = [&](int cur, int start) {
+    if (tall)
+      hipLaunchKernelGGL((cgIterKernel<float, 64, double, 6>), grid, dim3(256), 0, ctx.stream, g,
+                         diag32, coeff32, p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur],
+                         r[1 - cur], wv[cur], wv[1 - cur], x, tilesX, w.state.get(),
+                         w.partial.get(), start);
+    else
+      hipLaunchKernelGGL((cgIterKernel<float, 32, double, 6>), grid, dim3(256), 0, ctx.stream, g,
+                         diag32, coeff32, p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur],
+                         r[1 - cur], wv[cur], wv[1 - cur], x, tilesX, w.state.get(),
+                         w.partial.get(), start);
+  };
+  auto finalize = [&](int step, int64_t parts) {
+    launchFinalize(ctx, FinalArgs{w.partial.get(), parts, w.state.get(), w.hist.get(),
+                                  w.scal.get(), step});
+  };
+  // FP64 b - A x into the FP32 residual the next iteration reads; |r64|
+  auto trueResidual = [&](float* rOut) {
+    hipLaunchKernelGGL(trueResidualKernel, pgrid, dim3(256), 0, ctx.stream, g, diag64, coeff64,
+                       b, x, rOut, w.partial.get());
+    finalize(kStepNorm, pgrid.x);
+    CFDHIP_CHECK(hipGetLastError());
+    double out = 0.0;
+    CFDHIP_CHECK(hipMemcpyAsync(&out, w.scal.get(), sizeof(double), hipMemcpyDeviceToHost,
+                                ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    return out;
+  };
+  hipEvent_t t0, t1;
+  CFDHIP_CHECK(hipEventCreate(&t0));
+  CFDHIP_CHECK(hipEventCreate(&t1));
+  CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
+  hipLaunchKernelGGL(zeroKernel64, pgrid, dim3(256), 0, ctx.stream, n, x);
+  const double b0 = trueResidual(r[0]);
+  double rTrue = b0;
+  auto converged = [&](double a) {
+    return a == 0.0 || a <= set.absolute_tolerance ||
+           (b0 > 0.0 && a / b0 <= set.relative_tolerance);
+  };
+  int status = CFDHIP_MAX_ITERATIONS;
+  int64_t replacements = 0, fallbackIters = 0;
+  bool fallback = false;
+  std::vector<double> history{b0};
+  KState k{};
+  k.b0 = b0;
+  k.absTol = set.absolute_tolerance;
+  k.relTol = set.relative_tolerance;
+  k.maxIter = set.max_iterations;
+  k.status = -1;
+  k.replaceRatio = ratio;
+  k.restartOnReplace = kRestartOnReplace;
+  k.rmax = b0;
+  *w.hostState = k;
+  if (!std::isfinite(b0)) {
+    status = CFDHIP_NONFINITE_RESIDUAL;
+  } else if (converged(b0)) {
+    status = CFDHIP_CONVERGED;
+  } else {
+    CFDHIP_CHECK(hipMemcpyAsync(w.state.get(), w.hostState, sizeof(KState),
+                                hipMemcpyHostToDevice, ctx.stream));
+    // start: w0 = A M r0 from r[0]; the iterations then read index 1
+    launch(0, 1);
+    finalize(kStepCgRobustStart, grid.x);
+    CFDHIP_CHECK(hipGetLastError());
+    std::swap(r[0], r[1]);
+    const int chunk = set.check_interval > 0 ? set.check_interval : 32;
+    double lastTrue = b0;
+    while (true) {
+      // the buffers an iteration reads follow from the completed count
+      // (iteration k reads index (k + 1) & 1), so launches past the device
+      // verdict, which do nothing, cannot shift them
+      const int64_t left = set.max_iterations - w.hostState->iter;
+      const int base = static_cast<int>((w.hostState->iter + 1) & 1);
+      for (int q = 0; q < chunk && q < left; ++q) {
+        launch(base ^ (q & 1), 0);
+        finalize(kStepCgRobust, grid.x);
+      }
+      CFDHIP_CHECK(hipGetLastError());
+      CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState),
+                                  hipMemcpyDeviceToHost, ctx.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+      KState& S = *w.hostState;
+      if (!S.done) continue;
+      // every stop: measure in FP64
+      const int cur = static_cast<int>(S.iter + 1) & 1;
+      rTrue = trueResidual(r[cur]);
+      history.push_back(rTrue);
+      if (!std::isfinite(rTrue)) {
+        status = CFDHIP_NONFINITE_RESIDUAL;
+        break;
+      }
+      if (converged(rTrue)) {
+        status = CFDHIP_CONVERGED;
+        break;
+      }
+      if (S.iter >= set.max_iterations) {
+        status = CFDHIP_MAX_ITERATIONS;
+        break;
+      }
+      if (!S.replace || (replacements > 0 && rTrue > 0.5 * lastTrue)) {
+        // an FP32 breakdown, or the FP32 operator's rounding floor: FP64
+        // Jacobi-PCG from x to the same target relative to |b|
+        cfdhip_solver_settings rest = set;
+        rest.precision = CFDHIP_FP64;
+        rest.relative_tolerance = set.relative_tolerance * b0 / rTrue;
+        rest.max_iterations = std::max<int64_t>(1, set.max_iterations - S.iter);
+        double* work64[8];
+        for (int q = 0; q < 8; ++q) {
+          w.d8[q].ensure(n);
+          work64[q] = w.d8[q].get();
+        }
+        const int64_t done = S.iter;
+        const cfdhip_solve_result fr =
+            singleKernelPcg2d<double>(ctx, rest, diag64, coeff64, b, x, work64);
+        fallback = true;
+        fallbackIters = fr.iterations;
+        w.hostState->iter = done;
+        history.insert(history.end(), ctx.lastHistory.begin(), ctx.lastHistory.end());
+        // the FP64 iteration reports its recurrence residual: measure the true one
+        rTrue = trueResidual(r[0]);
+        status = converged(rTrue) ? CFDHIP_CONVERGED
+                                  : (fr.status == CFDHIP_CONVERGED ? CFDHIP_MAX_ITERATIONS
+                                                                   : fr.status);
+        break;
+      }
+      lastTrue = rTrue;
+      ++replacements;
+      hipLaunchKernelGGL(replaceKernel, pgrid, dim3(256), 0, ctx.stream, g, diag32, coeff32,
+                         r[cur], p[cur], s[cur], wv[cur], w.partial.get());
+      finalize(kStepCgReplace, pgrid.x);
+      CFDHIP_CHECK(hipGetLastError());
+      CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState),
+                                  hipMemcpyDeviceToHost, ctx.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+      if (w.hostState->done) {  // breakdown on the replaced residual: finish in FP64
+        w.hostState->replace = 0;
+        continue;
+      }
+    }
+  }
+  const int64_t inner = w.hostState->iter;
+  CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
+  CFDHIP_CHECK(hipEventSynchronize(t1));
+  float ms = 0.f;
+  CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  cfdhip_solve_result res{};
+  res.status = status;
+  res.fallback_used = fallback ? 1 : 0;
+  res.iterations = inner + fallbackIters;
+  res.restarts = replacements;
+  res.initial_residual = b0;
+  res.final_residual = rTrue;
+  res.true_residual = rTrue;
+  res.device_ms = ms;
+  ctx.lastHistory = history;
+  res.history_length = static_cast<int64_t>(history.size());
+  return res;
+}
 
 }  // namespace cfdhip

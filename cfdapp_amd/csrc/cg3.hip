@@ -1,5 +1,5 @@
 // Jacobi-PCG for the 3D p' operator (7-point), in FP64 or FP32, and the
-// mixed-precision refinement around the FP32 variant.
+// mixed-precision solve (FP32 iteration with FP64 residual replacement).
 //
 // Same Chronopoulos-Gear recurrence and device-side scalar steps as the 2D
 // single-kernel iteration (cg1.hip; src/algebra/CG.cpp:77-142 in exact
@@ -71,16 +71,19 @@ __global__ __launch_bounds__(256) void start3Kernel(Op3<T> g, const T* __restric
   blockPartials<1>(acc, partial);
 }
 
-template <class T>
+// The iterate x has its own type TX (FP64 under the mixed solve's FP32
+// iteration vectors, as in the 2D iteration).
+template <class T, class TX = T>
 __global__ __launch_bounds__(256) void update3Kernel(int64_t n, const T* __restrict__ diag,
                                                      T* __restrict__ p, T* __restrict__ s,
                                                      T* __restrict__ r, const T* __restrict__ w,
-                                                     T* __restrict__ x, T* __restrict__ u,
+                                                     TX* __restrict__ x, T* __restrict__ u,
                                                      const KState* __restrict__ st, int start) {
   if (st->done) return;
   const T alpha = static_cast<T>(st->alpha);
   const T beta = static_cast<T>(st->beta);
-  const bool first = st->iter == 0;
+  const TX alphaX = static_cast<TX>(st->alpha);
+  const bool first = st->iter == st->dirStart;
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const T d = diag[c];
@@ -94,18 +97,22 @@ __global__ __launch_bounds__(256) void update3Kernel(int64_t n, const T* __restr
       s[c] = sn;
       p[c] = pn;
       r[c] = rn;
-      x[c] = x[c] + (pn * alpha);
+      x[c] = x[c] + (static_cast<TX>(pn) * alphaX);
     }
     u[c] = (T(1) / d) * rn;
   }
 }
 
-template <class T>
+// NSUM = 6 adds the explicit-conjugation sums of the mixed solve, (u, s),
+// (p, s), (r, p), with p, s the direction the update kernel just formed.
+template <class T, int NSUM = 3>
 __global__ __launch_bounds__(256) void apply3Kernel(Op3<T> g, const T* __restrict__ u,
                                                     const T* __restrict__ r, T* __restrict__ w,
+                                                    const T* __restrict__ p,
+                                                    const T* __restrict__ s,
                                                     const KState* __restrict__ st,
                                                     double* partial) {
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[NSUM] = {};
   if (!st->done) {
     const int64_t n = g.nx * g.ny * g.nz;
     for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
@@ -117,9 +124,15 @@ __global__ __launch_bounds__(256) void apply3Kernel(Op3<T> g, const T* __restric
       acc[0] += static_cast<double>(rc) * static_cast<double>(uc);
       acc[1] += static_cast<double>(wc) * static_cast<double>(uc);
       acc[2] += static_cast<double>(rc) * static_cast<double>(rc);
+      if constexpr (NSUM == 6) {
+        const double sc = s[c], pc = p[c];
+        acc[3] += static_cast<double>(uc) * sc;
+        acc[4] += pc * sc;
+        acc[5] += static_cast<double>(rc) * pc;
+      }
     }
   }
-  blockPartials<3>(acc, partial);
+  blockPartials<NSUM>(acc, partial);
 }
 
 // r64 = b - A x (FP64) -> r32, |r64|^2 partials (refinement step).
@@ -138,22 +151,43 @@ __global__ __launch_bounds__(256) void refine3Kernel(Op3<double> g, const double
   blockPartials<1>(acc, partial);
 }
 
-__global__ void zero3(int64_t n, float* a) {
+__global__ void zero3d(int64_t n, double* __restrict__ x) {
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    a[c] = 0.f;
+    x[c] = 0.0;
 }
 
-__global__ void zero3d(int64_t n, double* a) {
+// u = M r with the iteration's rounding (update3Kernel), whatever the verdict.
+__global__ void invDiag3(int64_t n, const float* __restrict__ diag, const float* __restrict__ r,
+                         float* __restrict__ u) {
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    a[c] = 0.0;
+    u[c] = (1.0f / diag[c]) * r[c];
 }
 
-__global__ void add3(int64_t n, const float* __restrict__ e, double* __restrict__ x) {
+// After a residual replacement: w = A u and the six sums of kStepCgReplace,
+// (r, u), (w, u), |r|^2, (u, s), (p, s), (r, p).
+__global__ __launch_bounds__(256) void replace3Kernel(Op3<float> g, const float* __restrict__ u,
+                                                      const float* __restrict__ r,
+                                                      const float* __restrict__ p,
+                                                      const float* __restrict__ s,
+                                                      float* __restrict__ w, double* partial) {
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int64_t n = g.nx * g.ny * g.nz;
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
-       c += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    x[c] += static_cast<double>(e[c]);
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = c % g.nx, t = c / g.nx, j = t % g.ny, k = t / g.ny;
+    const float wc = applyRow3<float>(g, i, j, k, c, u);
+    w[c] = wc;
+    const double uc = u[c], rc = r[c], sc = s[c], pc = p[c];
+    acc[0] += rc * uc;
+    acc[1] += static_cast<double>(wc) * uc;
+    acc[2] += rc * rc;
+    acc[3] += uc * sc;
+    acc[4] += pc * sc;
+    acc[5] += rc * pc;
+  }
+  blockPartials<6>(acc, partial);
 }
 
 }  // namespace
@@ -197,8 +231,8 @@ cfdhip_solve_result pcg3d(Context& ctx, const cfdhip_solver_settings& set, const
   fin(kStepStartNorm);
   hipLaunchKernelGGL((update3Kernel<T>), g, dim3(256), 0, ctx.stream, n, diag, p, s, r, wv, x, u,
                      w.state.get(), 1);
-  hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv, w.state.get(),
-                     w.partial.get());
+  hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv, p, s,
+                     w.state.get(), w.partial.get());
   fin(kStepCg1Start);
   CFDHIP_CHECK(hipGetLastError());
   const int chunk = set.check_interval > 0 ? set.check_interval : 32;
@@ -207,7 +241,7 @@ cfdhip_solve_result pcg3d(Context& ctx, const cfdhip_solver_settings& set, const
     for (int q = 0; q < chunk && launched < set.max_iterations; ++q, ++launched) {
       hipLaunchKernelGGL((update3Kernel<T>), g, dim3(256), 0, ctx.stream, n, diag, p, s, r, wv, x,
                          u, w.state.get(), 0);
-      hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv,
+      hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv, p, s,
                          w.state.get(), w.partial.get());
       fin(kStepCg1Iter);
     }
@@ -242,117 +276,162 @@ cfdhip_solve_result pcg3d(Context& ctx, const cfdhip_solver_settings& set, const
 template cfdhip_solve_result pcg3d<double>(Context&, const cfdhip_solver_settings&, const double*,
                                            const double*, const double*, double*,
                                            double* const[5]);
-template cfdhip_solve_result pcg3d<float>(Context&, const cfdhip_solver_settings&, const float*,
-                                          const float*, const float*, float*, float* const[5]);
 
-// Mixed precision in 3D: FP32 operator copy and FP32 inner PCG, FP64 true
-// residual and correction until the FP64 relative residual (against
-// |b - A x0|, x0 = 0) meets the tolerance.
+// Mixed precision in 3D with FP64 residual replacement: the scheme of the 2D
+// solve (cg1.hip, mixedPcg2d) on the two-kernel 3D iteration. FP32 p, s, r,
+// w, u on an FP32 copy of the operator, accumulating into the FP64 x; when
+// the FP32 recurrence residual is below `ratio` times its largest value
+// since the last replacement (or below the target) the host recomputes
+// r = b - A x in FP64 and continues with the same search
+// direction (kStepCgReplace). A replacement that does not halve the FP64
+// residual, or an FP32 breakdown, hands the rest to FP64 Jacobi-PCG from x.
 cfdhip_solve_result mixedPcg3d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                                double* x) {
   const View& v = ctx.view;
   const int64_t n = v.n, nf = v.nf;
   KrylovWork& w = ctx.kw;
-  w.r32.ensure(n);
-  w.x32.ensure(n);
   for (int q = 0; q < 5; ++q) w.f8[q].ensure(n);
   w.dinv32.ensure(n);
   w.w32.ensure(nf);
   castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
   castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
-  w.partial.ensure(std::max<int64_t>(w.partial.count, kMaxBlocks3 * 3 + 8));
+  w.partial.ensure(std::max<int64_t>(w.partial.count, kMaxBlocks3 * 6 + 8));
   w.state.ensure(1);
   w.scal.ensure(4);
+  if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+  w.hist.ensure(set.max_iterations + 2);
   const dim3 g = grid3(n);
-  const Op3<double> op64{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ, ctx.f(CFDHIP_FIELD_PC_DIAG),
-                         ctx.f(CFDHIP_FIELD_PC_FACE_COEFF)};
+  const double* diag64 = ctx.f(CFDHIP_FIELD_PC_DIAG);
+  const double* coeff64 = ctx.f(CFDHIP_FIELD_PC_FACE_COEFF);
+  const Op3<double> op64{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ, diag64, coeff64};
+  const Op3<float> op32{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ, w.dinv32.get(), w.w32.get()};
+  const float* diag32 = w.dinv32.get();
+  float* r = w.f8[0].get();
+  float* s = w.f8[1].get();
+  float* wv = w.f8[2].get();
+  float* p = w.f8[3].get();
+  float* u = w.f8[4].get();
+  const double ratio = set.inner_tolerance > 0.0 ? set.inner_tolerance : 1e-5;
+  auto fin = [&](int step) {
+    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+                                  w.hist.get(), w.scal.get(), step});
+  };
+  auto trueResidual = [&]() {
+    hipLaunchKernelGGL(refine3Kernel, g, dim3(256), 0, ctx.stream, op64, b, x, r,
+                       w.partial.get());
+    fin(kStepNorm);
+    CFDHIP_CHECK(hipGetLastError());
+    double out = 0.0;
+    CFDHIP_CHECK(hipMemcpyAsync(&out, w.scal.get(), sizeof(double), hipMemcpyDeviceToHost,
+                                ctx.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    return out;
+  };
   hipEvent_t t0, t1;
   CFDHIP_CHECK(hipEventCreate(&t0));
   CFDHIP_CHECK(hipEventCreate(&t1));
   CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
   hipLaunchKernelGGL(zero3d, g, dim3(256), 0, ctx.stream, n, x);
-  std::vector<double> history;
-  double b0 = 0.0, rTrue = 0.0;
-  int64_t total = 0, cycles = 0;
+  const double b0 = trueResidual();
+  double rTrue = b0;
+  auto converged = [&](double a) {
+    return a == 0.0 || a <= set.absolute_tolerance ||
+           (b0 > 0.0 && a / b0 <= set.relative_tolerance);
+  };
   int status = CFDHIP_MAX_ITERATIONS;
-  // inner FP32 target per refinement cycle (CFDHIP_MIXED_INNER_TOL overrides):
-  // on large 3D SIMPLE operators a refinement cycle gains at most about two
-  // decades before the FP32 operator's rounding dominates the correction
-  // (measured at 256^3: cycles asked for 1e-3 .. 1e-5 reached 1e-3 .. 4e-4),
-  // so a cycle asks for two
-  static const double innerTol = [] {
-    const char* e = std::getenv("CFDHIP_MIXED_INNER_TOL");
-    const double t = e ? std::atof(e) : 1e-2;
-    return t > 0.0 && t < 1.0 ? t : 1e-2;
-  }();
+  int64_t replacements = 0, fallbackIters = 0;
   bool fallback = false;
-  double prev = 0.0;
-  while (true) {
-    hipLaunchKernelGGL(refine3Kernel, g, dim3(256), 0, ctx.stream, op64, b, x, w.r32.get(),
-                       w.partial.get());
-    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
-                                  nullptr, w.scal.get(), kStepNorm});
+  std::vector<double> history{b0};
+  KState k{};
+  k.b0 = b0;
+  k.absTol = set.absolute_tolerance;
+  k.relTol = set.relative_tolerance;
+  k.maxIter = set.max_iterations;
+  k.status = -1;
+  k.replaceRatio = ratio;
+  k.restartOnReplace = kRestartOnReplace;
+  k.rmax = b0;
+  *w.hostState = k;
+  if (!std::isfinite(b0)) {
+    status = CFDHIP_NONFINITE_RESIDUAL;
+  } else if (converged(b0)) {
+    status = CFDHIP_CONVERGED;
+  } else {
+    CFDHIP_CHECK(hipMemcpyAsync(w.state.get(), w.hostState, sizeof(KState),
+                                hipMemcpyHostToDevice, ctx.stream));
+    hipLaunchKernelGGL((update3Kernel<float, double>), g, dim3(256), 0, ctx.stream, n, diag32, p,
+                       s, r, wv, x, u, w.state.get(), 1);
+    hipLaunchKernelGGL((apply3Kernel<float, 6>), g, dim3(256), 0, ctx.stream, op32, u, r, wv, p,
+                       s, w.state.get(), w.partial.get());
+    fin(kStepCgRobustStart);
     CFDHIP_CHECK(hipGetLastError());
-    CFDHIP_CHECK(hipMemcpyAsync(&rTrue, w.scal.get(), sizeof(double), hipMemcpyDeviceToHost,
-                                ctx.stream));
-    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
-    if (cycles == 0) b0 = rTrue;
-    history.push_back(rTrue);
-    if (!std::isfinite(rTrue)) {
-      status = CFDHIP_NONFINITE_RESIDUAL;
-      break;
-    }
-    if (rTrue == 0.0 || rTrue <= set.absolute_tolerance ||
-        (b0 > 0.0 && rTrue / b0 <= set.relative_tolerance)) {
-      status = CFDHIP_CONVERGED;
-      break;
-    }
-    if (total >= set.max_iterations) break;
-    if (cycles > 0 && rTrue > 0.5 * prev) {
-      // a cycle that did not halve the true residual: finish in FP64 from x
-      cfdhip_solver_settings rest = set;
-      rest.relative_tolerance = set.relative_tolerance * b0 / rTrue;
-      rest.max_iterations = set.max_iterations - total;
-      double* work64[5];
-      for (int q = 0; q < 5; ++q) {
-        w.d8[q].ensure(n);
-        work64[q] = w.d8[q].get();
+    const int chunk = set.check_interval > 0 ? set.check_interval : 32;
+    double lastTrue = b0;
+    while (true) {
+      const int64_t left = set.max_iterations - w.hostState->iter;
+      for (int q = 0; q < chunk && q < left; ++q) {
+        hipLaunchKernelGGL((update3Kernel<float, double>), g, dim3(256), 0, ctx.stream, n,
+                           diag32, p, s, r, wv, x, u, w.state.get(), 0);
+        hipLaunchKernelGGL((apply3Kernel<float, 6>), g, dim3(256), 0, ctx.stream, op32, u, r, wv,
+                           p, s, w.state.get(), w.partial.get());
+        fin(kStepCgRobust);
       }
-      const cfdhip_solve_result fr = pcg3d<double>(ctx, rest, ctx.f(CFDHIP_FIELD_PC_DIAG),
-                                                   ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
-                                                   work64);
-      total += fr.iterations;
-      fallback = true;
-      hipLaunchKernelGGL(refine3Kernel, g, dim3(256), 0, ctx.stream, op64, b, x, w.r32.get(),
-                         w.partial.get());
-      launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
-                                    nullptr, w.scal.get(), kStepNorm});
-      CFDHIP_CHECK(hipMemcpyAsync(&rTrue, w.scal.get(), sizeof(double), hipMemcpyDeviceToHost,
-                                  ctx.stream));
+      CFDHIP_CHECK(hipGetLastError());
+      CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState),
+                                  hipMemcpyDeviceToHost, ctx.stream));
       CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+      KState& S = *w.hostState;
+      if (!S.done) continue;
+      rTrue = trueResidual();
       history.push_back(rTrue);
-      status = fr.status;
-      break;
+      if (!std::isfinite(rTrue)) {
+        status = CFDHIP_NONFINITE_RESIDUAL;
+        break;
+      }
+      if (converged(rTrue)) {
+        status = CFDHIP_CONVERGED;
+        break;
+      }
+      if (S.iter >= set.max_iterations) {
+        status = CFDHIP_MAX_ITERATIONS;
+        break;
+      }
+      if (!S.replace || (replacements > 0 && rTrue > 0.5 * lastTrue)) {
+        cfdhip_solver_settings rest = set;
+        rest.precision = CFDHIP_FP64;
+        rest.relative_tolerance = set.relative_tolerance * b0 / rTrue;
+        rest.max_iterations = std::max<int64_t>(1, set.max_iterations - S.iter);
+        double* work64[5];
+        for (int q = 0; q < 5; ++q) {
+          w.d8[q].ensure(n);
+          work64[q] = w.d8[q].get();
+        }
+        const int64_t done = S.iter;
+        const cfdhip_solve_result fr = pcg3d<double>(ctx, rest, diag64, coeff64, b, x, work64);
+        fallback = true;
+        fallbackIters = fr.iterations;
+        w.hostState->iter = done;
+        history.insert(history.end(), ctx.lastHistory.begin(), ctx.lastHistory.end());
+        // pcg3d reports its recurrence residual: measure the FP64 one
+        rTrue = trueResidual();
+        status = converged(rTrue) ? CFDHIP_CONVERGED
+                                  : (fr.status == CFDHIP_CONVERGED ? CFDHIP_MAX_ITERATIONS
+                                                                   : fr.status);
+        break;
+      }
+      lastTrue = rTrue;
+      ++replacements;
+      hipLaunchKernelGGL(invDiag3, g, dim3(256), 0, ctx.stream, n, diag32, r, u);
+      hipLaunchKernelGGL(replace3Kernel, g, dim3(256), 0, ctx.stream, op32, u, r, p, s, wv,
+                         w.partial.get());
+      fin(kStepCgReplace);
+      CFDHIP_CHECK(hipGetLastError());
+      CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState),
+                                  hipMemcpyDeviceToHost, ctx.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
     }
-    prev = rTrue;
-    cfdhip_solver_settings inner = set;
-    inner.absolute_tolerance = 0.0;
-    inner.relative_tolerance = std::max(innerTol, set.relative_tolerance * b0 / rTrue);
-    inner.max_iterations = set.max_iterations - total;
-    hipLaunchKernelGGL(zero3, g, dim3(256), 0, ctx.stream, n, w.x32.get());
-    float* work[5];
-    for (int q = 0; q < 5; ++q) work[q] = w.f8[q].get();
-    const cfdhip_solve_result ir =
-        pcg3d<float>(ctx, inner, w.dinv32.get(), w.w32.get(), w.r32.get(), w.x32.get(), work);
-    total += ir.iterations;
-    ++cycles;
-    if (ir.iterations == 0 && ir.status != CFDHIP_CONVERGED) {
-      status = ir.status;
-      break;
-    }
-    hipLaunchKernelGGL(add3, g, dim3(256), 0, ctx.stream, n, w.x32.get(), x);
-    CFDHIP_CHECK(hipGetLastError());
   }
+  const int64_t inner = w.hostState->iter;
   CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
   CFDHIP_CHECK(hipEventSynchronize(t1));
   float ms = 0.f;
@@ -362,8 +441,8 @@ cfdhip_solve_result mixedPcg3d(Context& ctx, const cfdhip_solver_settings& set, 
   cfdhip_solve_result res{};
   res.status = status;
   res.fallback_used = fallback ? 1 : 0;
-  res.iterations = total;
-  res.restarts = cycles;
+  res.iterations = inner + fallbackIters;
+  res.restarts = replacements;
   res.initial_residual = b0;
   res.final_residual = rTrue;
   res.true_residual = rTrue;

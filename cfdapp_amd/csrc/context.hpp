@@ -28,7 +28,14 @@ struct KState {
   int64_t stagnant;    // iterations since it last decreased
   int64_t window;      // stagnant iterations that end the solve (0: never)
   int32_t stagnated;   // 1 when the window ran out
-  int32_t pad0;
+  // mixed-precision PCG with residual replacement: replaceRatio > 0 stops the
+  // FP32 recurrence (done, replace = 1) once its residual has fallen below
+  // replaceRatio times its largest value since the last FP64 replacement
+  // (rmax), or below the target; the host then recomputes the residual in FP64
+  int32_t replace;
+  int32_t restartOnReplace;  // 1: a replacement restarts the direction (p = u)
+  double replaceRatio, rmax;
+  int64_t dirStart;          // iteration count at which the current direction sequence began
 };
 
 // Work vectors (allocated once per size, reused across solves).
@@ -177,9 +184,26 @@ template <class T>
 cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings& set,
                                       const T* diag, const T* coeff, const T* b, T* x,
                                       T* const work[8], Snapshots* snap = nullptr);
-// pcg2d.hip: the mixed-precision solve
+// The same FP64 solve on caller-provided buffers (partials, device state, its
+// host mirror, history); events == false: no HIP events, graphs or history
+// copy (the no-allocation entry point).
+struct PcgBuffers {
+  double* partial;
+  int64_t partialCap;
+  KState* state;
+  KState* host;
+  double* hist;
+  int64_t histCap;
+  bool events;
+};
+cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& set,
+                                   const double* diag, const double* coeff, const double* b,
+                                   double* x, double* const work[8], const PcgBuffers& buf,
+                                   Snapshots* snap = nullptr);
+// cg1.hip: the mixed-precision solve (FP32 iteration, FP64 residual replacement)
 cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                                double* x);
+// pcg2d.hip
 void castToFloat(Context& ctx, int64_t n, const double* src, float* dst);
 // cg3.hip: 3D Jacobi-PCG (work = r s w p u) and its mixed-precision refinement
 template <class T>

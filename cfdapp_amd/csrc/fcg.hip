@@ -388,11 +388,8 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
   CFDHIP_CHECK(hipEventCreate(&t1));
   CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
   hipLaunchKernelGGL(zeroKernelD64, g, dim3(256), 0, ctx.stream, n, x);
-  static const double innerTol = [] {
-    const char* e = std::getenv("CFDHIP_FCG_INNER_TOL");
-    const double t = e ? std::atof(e) : 1e-4;
-    return t > 0.0 && t < 1.0 ? t : 1e-4;
-  }();
+  // FP32 vectors: reduction per restart cycle (the settings' inner tolerance)
+  const double innerTol = set.inner_tolerance > 0.0 ? set.inner_tolerance : 1e-4;
   auto outerResidual = [&]() {
     if (mixed)
       hipLaunchKernelGGL((outerResidualKernel<float>), g, dim3(256), 0, ctx.stream, v.nx, v.ny,

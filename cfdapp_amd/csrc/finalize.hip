@@ -266,6 +266,79 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
       s.restart = 2;  // the host zeroes v and p
       return;
     }
+    case kStepCgRobustStart: {  // p = u: alpha = (r, u) / (u, A u)
+      const double gamma = t[0], delta = t[1];
+      s.rz = gamma;
+      if (!isfinite(gamma) || !isfinite(delta) || fabs(delta) < 1e-30) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.alpha = gamma / delta;
+      s.beta = 0.0;
+      if (!isfinite(s.alpha)) finish(s, CFDHIP_BREAKDOWN);
+      return;
+    }
+    case kStepCgRobust:
+    case kStepCgReplace: {
+      // Explicit conjugation against the last direction p (s = A p by
+      // recurrence): beta = -(u, A p) / (p, A p), so the next direction is
+      // A-conjugate to p whatever rounding or an FP64 residual replacement
+      // did to r, (p', A p') = (u, A u) + beta (u, A p), and the exact line
+      // search alpha = (r, p') / (p', A p') with (r, p') = (r, u) + beta (r, p).
+      // In exact arithmetic these are CG's scalars (CG.cpp:77-142).
+      const double gamma = t[0], delta = t[1], us = t[3], ps = t[4], rp = t[5];
+      const double rnorm = sqrt(t[2]);
+      s.rnorm = rnorm;
+      if (stepId == kStepCgReplace) {  // a replacement completes no iteration
+        s.done = 0;
+        s.replace = 0;
+        s.status = -1;
+        s.rmax = rnorm;
+      } else {
+        s.iter += 1;
+        pushHist(s, hist, rnorm);
+        if (!isfinite(rnorm)) {
+          finish(s, CFDHIP_NONFINITE_RESIDUAL);
+          return;
+        }
+        // the FP32 recurrence only proposes; the host checks in FP64
+        s.rmax = fmax(s.rmax, rnorm);
+        if (convergedTest(s, rnorm) || rnorm <= s.replaceRatio * s.rmax ||
+            s.iter >= s.maxIter) {
+          s.replace = 1;
+          s.done = 1;
+          return;
+        }
+      }
+      if (stepId == kStepCgReplace && s.restartOnReplace) {
+        // restart from the replaced residual: p = u, (p, A p) = (u, A u)
+        if (!isfinite(gamma) || !isfinite(delta) || fabs(delta) < 1e-30) {
+          finish(s, CFDHIP_BREAKDOWN);
+          return;
+        }
+        s.alpha = gamma / delta;
+        s.beta = 0.0;
+        s.rz = gamma;
+        s.dirStart = s.iter;
+        if (!isfinite(s.alpha)) finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double beta = -us / ps;
+      const double pAp = delta + beta * us;
+      if (!isfinite(gamma) || !isfinite(beta) || !isfinite(pAp) || fabs(pAp) < 1e-30) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double alpha = (gamma + beta * rp) / pAp;
+      if (!isfinite(alpha)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.beta = beta;
+      s.alpha = alpha;
+      s.rz = gamma;
+      return;
+    }
     case kStepNorm:
       out[0] = sqrt(t[0]);
       return;
@@ -327,16 +400,21 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
 __global__ __launch_bounds__(kFinThreads) void finalizeKernel(FinalArgs a) {
   KState& s = *a.st;
   const bool bcg = a.step >= kStepBcgRho && a.step <= kStepBcgX;
-  const bool skip = a.step != kStepNorm && (s.done || (bcg && s.restart));
+  const bool skip =
+      a.step != kStepNorm && a.step != kStepCgReplace && (s.done || (bcg && s.restart));
   if (skip) return;
-  double t[3] = {0.0, 0.0, 0.0};
-  if (a.step == kStepCgUpdate || a.step == kStepBcgRho || a.step == kStepFcgAlpha) {
+  double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (a.step == kStepCgReplace || a.step == kStepCgRobust || a.step == kStepCgRobustStart) {
+    sumPartials<6>(a.partial, a.nParts, t);
+  } else if (a.step == kStepCgUpdate || a.step == kStepBcgRho || a.step == kStepFcgAlpha) {
     double v[2];
     sumPartials<2>(a.partial, a.nParts, v);
     t[0] = v[0], t[1] = v[1];
   } else if (a.step == kStepCg1Start || a.step == kStepCg1Iter || a.step == kStepBcgV ||
              a.step == kStepBcgT) {
-    sumPartials<3>(a.partial, a.nParts, t);
+    double v[3];
+    sumPartials<3>(a.partial, a.nParts, v);
+    t[0] = v[0], t[1] = v[1], t[2] = v[2];
   } else {
     double v[1];
     sumPartials<1>(a.partial, a.nParts, v);

@@ -1,12 +1,5 @@
-// Mixed-precision p' solve on the 2D operator: FP32 single-kernel Jacobi-PCG
-// inner solves (cg1.hip) on an FP32 copy of the operator, with the FP64 true
-// residual, the FP64 correction of x and the convergence test in FP64
-// (iterative refinement).
-#include <algorithm>
-#include <cfloat>
-#include <cmath>
-#include <cstdlib>
-
+// FP64 -> FP32 copies of the p' operator for the mixed-precision solves
+// (cg1.hip, cg3.hip, fcg.hip, batch.hip).
 #include "reduce.hpp"
 
 namespace cfdhip {
@@ -14,58 +7,6 @@ namespace cfdhip {
 namespace {
 
 constexpr int kMaxBlocks = 2048;
-
-struct Op2D {
-  int64_t nx, ny, pin;
-  int64_t offY;  // first y-face id
-};
-
-// r64 = b - A x (FP64), r32 = (float) r64, |r64|^2 -> out[0] (iterative refinement).
-__global__ __launch_bounds__(256) void refineResidualKernel(Op2D op, const double* __restrict__ diag,
-                                                            const double* __restrict__ coeff,
-                                                            const double* __restrict__ b,
-                                                            const double* __restrict__ x,
-                                                            float* r32, double* partial) {
-  const int64_t n = op.nx * op.ny;
-  double acc[1] = {0.0};
-  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < n;
-       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t i = c % op.nx, j = c / op.nx;
-    double y;
-    if (c == op.pin) {
-      y = 0.0 + diag[c] * x[c];
-    } else {
-      y = 0.0;
-      if (j > 0) y += (-coeff[op.offY + j * op.nx + i]) * x[c - op.nx];
-      if (i > 0) y += (-coeff[j * (op.nx + 1) + i]) * x[c - 1];
-      y += diag[c] * x[c];
-      if (i < op.nx - 1) y += (-coeff[j * (op.nx + 1) + i + 1]) * x[c + 1];
-      if (j < op.ny - 1) y += (-coeff[op.offY + (j + 1) * op.nx + i]) * x[c + op.nx];
-    }
-    const double ri = b[c] - y;
-    r32[c] = static_cast<float>(ri);
-    acc[0] += ri * ri;
-  }
-  blockPartials<1>(acc, partial);
-}
-
-__global__ void addCorrectionKernel(int64_t n, const float* __restrict__ e, double* x) {
-  for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
-       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    x[k] += static_cast<double>(e[k]);
-}
-
-__global__ void zeroKernelD(int64_t n, double* a) {
-  for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
-       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    a[k] = 0.0;
-}
-
-__global__ void zeroFloatKernel(int64_t n, float* a) {
-  for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
-       k += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    a[k] = 0.f;
-}
 
 template <class T>
 __global__ void castKernel(int64_t n, const double* __restrict__ src, T* dst) {
@@ -75,148 +16,6 @@ __global__ void castKernel(int64_t n, const double* __restrict__ src, T* dst) {
 }
 
 }  // namespace
-
-void castToFloat(Context& ctx, int64_t n, const double* src, float* dst);
-
-// Mixed precision: FP32 fused Jacobi-PCG inner solves on an FP32 copy of the
-// operator, FP64 true residual and correction (iterative refinement) until
-// |b - A x| <= max(abs, rel |b - A x0|) in FP64 (x0 = 0).
-cfdhip_solve_result mixedPcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
-                               double* x) {
-  const View& v = ctx.view;
-  const int64_t n = v.n, nf = v.nf;
-  KrylovWork& w = ctx.kw;
-  w.r32.ensure(n);
-  w.x32.ensure(n);
-  for (auto& a : w.f8) a.ensure(n);
-  w.dinv32.ensure(n);  // FP32 diagonal
-  w.w32.ensure(nf);    // FP32 face couplings
-  castToFloat(ctx, n, ctx.f(CFDHIP_FIELD_PC_DIAG), w.dinv32.get());
-  castToFloat(ctx, nf, ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.w32.get());
-  Op2D op{v.nx, v.ny, ctx.pin, v.offY};
-  int64_t gb = (n + 255) / 256;
-  if (gb > kMaxBlocks) gb = kMaxBlocks;
-  const dim3 g(static_cast<unsigned>(gb));
-  w.partial.ensure(kMaxBlocks * 4);
-  w.state.ensure(1);
-  w.scal.ensure(4);
-  double* normOut = w.scal.get();
-  hipEvent_t t0, t1;
-  CFDHIP_CHECK(hipEventCreate(&t0));
-  CFDHIP_CHECK(hipEventCreate(&t1));
-  CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
-  hipLaunchKernelGGL(zeroKernelD, g, dim3(256), 0, ctx.stream, n, x);
-  std::vector<double> history;
-  double b0 = 0.0, rTrue = 0.0;
-  int64_t total = 0, cycles = 0;
-  int status = CFDHIP_MAX_ITERATIONS;
-  // inner FP32 target per refinement cycle (CFDHIP_MIXED_INNER_TOL overrides)
-  static const double innerTol = [] {
-    const char* e = std::getenv("CFDHIP_MIXED_INNER_TOL");
-    const double t = e ? std::atof(e) : 1e-5;
-    return t > 0.0 && t < 1.0 ? t : 1e-5;
-  }();
-  double spmvMs = 0.0, spmvLaunches = 0.0;
-  bool fallback = false;
-  double prev = 0.0;
-  while (true) {
-    hipLaunchKernelGGL(refineResidualKernel, g, dim3(256), 0, ctx.stream, op,
-                       ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
-                       w.r32.get(), w.partial.get());
-    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
-                                  nullptr, normOut, kStepNorm});
-    CFDHIP_CHECK(hipGetLastError());
-    CFDHIP_CHECK(hipMemcpyAsync(&rTrue, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
-    CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
-    if (cycles == 0) b0 = rTrue;
-    history.push_back(rTrue);
-    if (!std::isfinite(rTrue)) {
-      status = CFDHIP_NONFINITE_RESIDUAL;
-      break;
-    }
-    if (rTrue == 0.0 || rTrue <= set.absolute_tolerance ||
-        (b0 > 0.0 && rTrue / b0 <= set.relative_tolerance)) {
-      status = CFDHIP_CONVERGED;
-      break;
-    }
-    if (total >= set.max_iterations) break;
-    if (cycles > 0 && rTrue > 0.5 * prev) {
-      // a cycle that did not halve the true residual: finish in FP64 from x
-      cfdhip_solver_settings rest = set;
-      rest.relative_tolerance = set.relative_tolerance * b0 / rTrue;
-      rest.max_iterations = set.max_iterations - total;
-      double* work64[8];
-      for (int q = 0; q < 8; ++q) {
-        w.d8[q].ensure(n);
-        work64[q] = w.d8[q].get();
-      }
-      const cfdhip_solve_result fr = singleKernelPcg2d<double>(
-          ctx, rest, ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
-          work64);
-      total += fr.iterations;
-      fallback = true;
-      hipLaunchKernelGGL(refineResidualKernel, g, dim3(256), 0, ctx.stream, op,
-                         ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), b, x,
-                         w.r32.get(), w.partial.get());
-      launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
-                                    nullptr, normOut, kStepNorm});
-      CFDHIP_CHECK(hipMemcpyAsync(&rTrue, normOut, sizeof(double), hipMemcpyDeviceToHost,
-                                  ctx.stream));
-      CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
-      history.push_back(rTrue);
-      status = fr.status;
-      break;
-    }
-    prev = rTrue;
-    cfdhip_solver_settings inner = set;
-    inner.absolute_tolerance = 0.0;
-    // the inner solve aims at the outer target when that is within FP32 reach
-    const double need = set.relative_tolerance * b0 / rTrue;
-    inner.relative_tolerance = std::max(innerTol, need);
-    inner.max_iterations = set.max_iterations - total;
-    hipLaunchKernelGGL(zeroFloatKernel, g, dim3(256), 0, ctx.stream, n, w.x32.get());
-    float* work[8];
-    for (int q = 0; q < 8; ++q) work[q] = w.f8[q].get();
-    const cfdhip_solve_result ir =
-        singleKernelPcg2d<float>(ctx, inner, w.dinv32.get(), w.w32.get(), w.r32.get(),
-                                 w.x32.get(), work);
-    total += ir.iterations;
-    ++cycles;
-    if (ctx.timing) {
-      spmvMs += ctx.spmvMeanUs * ctx.spmvCount / 1000.0;
-      spmvLaunches += ctx.spmvCount;
-    }
-    if (ir.iterations == 0 && ir.status != CFDHIP_CONVERGED) {
-      status = ir.status;
-      break;
-    }
-    hipLaunchKernelGGL(addCorrectionKernel, g, dim3(256), 0, ctx.stream, n, w.x32.get(), x);
-    CFDHIP_CHECK(hipGetLastError());
-  }
-  CFDHIP_CHECK(hipEventRecord(t1, ctx.stream));
-  CFDHIP_CHECK(hipEventSynchronize(t1));
-  float ms = 0.f;
-  CFDHIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
-  if (ctx.timing) {
-    ctx.spmvCount = spmvLaunches;
-    ctx.spmvMeanUs = spmvLaunches > 0 ? 1000.0 * spmvMs / spmvLaunches : 0.0;
-    ctx.timedIters = static_cast<double>(total);
-  }
-  cfdhip_solve_result res{};
-  res.status = status;
-  res.fallback_used = fallback ? 1 : 0;
-  res.iterations = total;
-  res.restarts = cycles;
-  res.initial_residual = b0;
-  res.final_residual = rTrue;
-  res.true_residual = rTrue;
-  res.device_ms = ms;
-  ctx.lastHistory = history;
-  res.history_length = static_cast<int64_t>(history.size());
-  return res;
-}
 
 void castToFloat(Context& ctx, int64_t n, const double* src, float* dst) {
   int64_t gb = (n + 255) / 256;

@@ -91,8 +91,21 @@ enum FinalStep : int {
   kStepNorm,            // sqrt of the sum into out                (1 sum)
   kStepFcgAlpha,        // (p, q), (p, r) -> alpha                 (2 sums)
   kStepFcgUpdate,       // |r| -> convergence, stagnation          (1 sum)
-  kStepFcgBeta          // (z, q) -> beta                          (1 sum)
+  kStepFcgBeta,         // (z, q) -> beta                          (1 sum)
+  // the mixed-precision iteration with explicit conjugation (6 sums each:
+  // (r,u), (w,u), |r|^2, (u,s), (p,s), (r,p) with p, s = A p the last direction)
+  kStepCgRobustStart,   // first direction p = u -> alpha
+  kStepCgRobust,        // |r| -> convergence / replacement; beta, alpha
+  kStepCgReplace        // the same scalars after an FP64 residual replacement
 };
+
+// Mixed-precision Jacobi-PCG: whether an FP64 residual replacement restarts
+// the search direction (1) or continues it with explicit conjugation (0).
+// Measured on the d = 1 p' operators (2048^2, 256^3): continuing the
+// direction across a replacement is stable at 512^2 but not at 2048^2 or
+// 256^3, where the residual jumps after the replacement; restarting every
+// 1e-5 of FP32 reduction is the fastest setting in 2D (0.82 x the FP64 time).
+constexpr int kRestartOnReplace = 1;
 
 struct FinalArgs {
   const double* partial;

@@ -77,6 +77,14 @@ typedef struct {
   int32_t stagnation_window; /* FCG: fall back to Jacobi-PCG after this many non-improving iterations (0 = 200) */
   int32_t graph;          /* 1: replay each chunk of check_interval iterations from a captured
                              HIP graph (2D Jacobi-PCG and the p' solves of cfdhip_simple_run) */
+  double inner_tolerance; /* precision CFDHIP_MIXED: the FP32 reduction between FP64 residual
+                             recomputations (0 = default). Jacobi-PCG replaces its residual by
+                             the FP64 b - A x (its iterate is FP64) and restarts its direction
+                             once the FP32 recurrence residual is below this fraction of its
+                             largest value since the last replacement (default 1e-5);
+                             flexible CG and the batched solver restart from the FP64
+                             residual after this much reduction (defaults 1e-4 and 1e-5).
+                             Must be in [0, 1). */
 } cfdhip_solver_settings;
 
 /* cfd::algebra::SolverResult (LinearSolver.hpp:103-134) without the solution

@@ -56,6 +56,8 @@ struct LinearSolverSettings {
   PreconditionerType preconditioner{PreconditionerType::None};
   Precision precision{Precision::FP64};
   bool graph{false};  // replay iteration chunks from a captured HIP graph
+  // mixed precision: FP32 reduction between FP64 residual recomputations (0: default)
+  double innerTolerance{0.0};
 
   // LinearSolver's constructor validation (src/algebra/LinearSolver.cpp:9-32).
   void validate() const {
@@ -68,6 +70,8 @@ struct LinearSolverSettings {
           "LinearSolverSettings: at least one of absoluteTolerance/relativeTolerance must be > 0");
     if (maxIterations == 0)
       throw InvalidArgumentError("LinearSolverSettings: maxIterations must be >= 1");
+    if (!(innerTolerance >= 0.0 && innerTolerance < 1.0))
+      throw InvalidArgumentError("LinearSolverSettings: innerTolerance must be in [0, 1)");
   }
   cfdhip_solver_settings c() const {
     cfdhip_solver_settings s{};
@@ -82,6 +86,7 @@ struct LinearSolverSettings {
                                                                       : CFDHIP_PREC_NONE;
     s.precision = precision == Precision::Mixed ? CFDHIP_MIXED : CFDHIP_FP64;
     s.graph = graph ? 1 : 0;
+    s.inner_tolerance = innerTolerance;
     return s;
   }
 };

@@ -58,6 +58,25 @@ def test_settings_validation_mirrors_reference():
         cfd.LinearSolverSettings(max_iterations=0)
     s = cfd.LinearSolverSettings(0.0, 1e-8, 100, "CG", "Jacobi").c()
     assert (s.type, s.preconditioner, s.max_iterations) == (0, 1, 100)
+    assert s.inner_tolerance == 0.0
+    for bad in (-0.1, 1.0, 2.0):
+        with pytest.raises(ValueError):
+            cfd.LinearSolverSettings(inner_tolerance=bad)
+    s = cfd.LinearSolverSettings(0.0, 1e-8, 100, "CG", "Jacobi", precision="mixed",
+                                 inner_tolerance=0.05).c()
+    assert (s.precision, s.inner_tolerance) == (1, 0.05)
+
+
+def test_settings_struct_layout_matches_header():
+    """cfdhip_solver_settings as the header declares it: the ctypes mirror's
+    size and the offset of the last field (a double after six int32)."""
+    import ctypes
+
+    from cfdapp_amd import _lib
+    S = _lib.SolverSettings
+    assert ctypes.sizeof(S) == 56
+    assert S.inner_tolerance.offset == 48
+    assert S.graph.offset == 44
 
 
 def test_no_device_factories_return_none():

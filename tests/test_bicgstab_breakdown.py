@@ -63,3 +63,64 @@ def test_oracle_bicgstab_bitwise(group, key):
     assert_bitwise(r.history, z[key + "_hist"], "history")
     assert_bitwise(r.x, z[key + "_x"], "solution")
     assert_bitwise([r.initial, r.final], z[key + "_res"], "residuals")
+
+
+def _bicgstab_order(z, key, dot):
+    """BiCGSTAB.cpp:111-274 without restarts, every inner product summed by
+    `dot`: how much the iteration count of a non-symmetric system depends on
+    the order of the sums alone."""
+    rp, col, val = z[key + "_rowptr"], z[key + "_col"], z[key + "_val"]
+    A = oracle.Csr(rp, col, val)
+    b = z[key + "_b"]
+    a, rel, m = z[key + "_settings"]
+    x = z[key + "_x0"].copy()
+    r = b - A.spmv(x)
+    rh = r.copy()
+    b0 = np.sqrt(dot(r, r))
+    conv = lambda q: q <= a or q / b0 <= rel  # noqa: E731
+    rho_old = alpha = omega = 1.0
+    v = np.zeros_like(b)
+    p = np.zeros_like(b)
+    for it in range(1, int(m) + 1):
+        rho = dot(rh, r)
+        beta = (rho / rho_old) * (alpha / omega)
+        p = r + (p - v * omega) * beta
+        v = A.spmv(p)
+        alpha = rho / dot(rh, v)
+        s = r - v * alpha
+        if conv(np.sqrt(dot(s, s))):
+            return it
+        t = A.spmv(s)
+        omega = dot(t, s) / dot(t, t)
+        x = x + p * alpha + s * omega
+        r = s - t * omega
+        if conv(np.sqrt(dot(r, r))):
+            return it
+        rho_old = rho
+    return -1
+
+
+def _serial(a, b):
+    s = 0.0
+    for u, w in zip(a.tolist(), b.tolist()):
+        s += u * w
+    return s
+
+
+def _pairwise(a, b):
+    v = a * b
+    while len(v) > 1:
+        if len(v) % 2:
+            v = np.append(v, 0.0)
+        v = v[0::2] + v[1::2]
+    return float(v[0])
+
+
+def test_iteration_counts_depend_on_summation_order():
+    """Serial sums reproduce the reference's counts; pairwise sums (the
+    device's kind of order) change them: 53 -> 47 on the rough stored system,
+    244 -> 258 on the n = 200 convection-diffusion system."""
+    assert _bicgstab_order(STORED, "rough", _serial) == 53
+    assert _bicgstab_order(STORED, "rough", _pairwise) == 47
+    assert _bicgstab_order(SMALL, "cd200_p0", _serial) == 244
+    assert _bicgstab_order(SMALL, "cd200_p0", _pairwise) == 258
