@@ -134,6 +134,10 @@ SIGNATURES = {
     "cfdhip_pressure_spmv": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_F64]),
     "cfdhip_pressure_solve_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverSettings),
                                                  P_F64, P_F64, ctypes.POINTER(SolveResult)]),
+    "cfdhip_pressure_solve_device_noalloc": (ctypes.c_int, [ctypes.c_void_p,
+                                                            ctypes.POINTER(SolverSettings),
+                                                            ctypes.c_void_p, ctypes.c_void_p,
+                                                            ctypes.POINTER(SolveResult)]),
     "cfdhip_solve_history": (ctypes.c_int, [ctypes.c_void_p, P_F64, ctypes.c_int64, P_I64]),
     "cfdhip_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "cfdhip_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, P_F64]),

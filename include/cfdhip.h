@@ -344,6 +344,23 @@ int cfdhip_pressure_spmv_device(cfdhip_ctx* ctx, const double* x, double* y, int
  * x0 = 0, solution into x (may be NULL to keep it on the device in PPRIME). */
 int cfdhip_pressure_solve_rhs(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
                               double* x, cfdhip_solve_result* r);
+/* No counterpart in the reference: cfdhip_pressure_solve_rhs for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation). FP64 Jacobi-PCG (type
+ * CFDHIP_SOLVER_CG, preconditioner CFDHIP_PREC_JACOBI, precision CFDHIP_FP64)
+ * on the resident 2D p' operator from x0 = 0, with the right-hand side b and
+ * the solution x in the caller's device memory (n doubles each); the same
+ * kernels and iterates as cfdhip_pressure_solve_rhs. Every work buffer is a
+ * fixed device array of the library (67.6 MB per GPU, held from the first
+ * use of the library on that GPU until the process ends); the call makes no
+ * device allocation, creates no HIP event or graph (settings.graph is
+ * ignored) and keeps no residual history (history_length 0; device_ms is the
+ * host wall time of the solve). Limits: at most 1048576 cells and 65536
+ * iterations (CFDHIP_ERR_INVALID beyond); CFDHIP_ERR_UNSUPPORTED for other
+ * solver settings or a 3D mesh. All contexts on one GPU share the array:
+ * calls that overlap on one GPU race. */
+int cfdhip_pressure_solve_device_noalloc(cfdhip_ctx* ctx, const cfdhip_solver_settings* s,
+                                         const double* b, double* x, cfdhip_solve_result* r);
 /* Residual history of the last solve on this context (up to cap entries). */
 int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t* length);
 
