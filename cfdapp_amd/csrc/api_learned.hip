@@ -212,12 +212,17 @@ int cfdhip_unet_forward(cfdhip_unet* h, const float* x, float* y, int64_t batch,
 int cfdhip_unet_apply(cfdhip_unet* h, const float* x, float* y, int64_t batch, int64_t hh,
                       int64_t w) {
   return guardedNet(h, [&](Unet& net) {
-    checkShape(batch, hh, w);
+    // any image size: the network runs on the grid rounded up to multiples
+    // of 8 and reflects its padding at the image's edges
+    require(hh >= 1 && w >= 1, "the image must have at least one pixel");
+    const int64_t h8 = (hh + 7) / 8 * 8, w8 = (w + 7) / 8 * 8;
+    checkShape(batch, h8, w8);
     require(x != nullptr && y != nullptr, "device buffers are required");
-    const int n = static_cast<int>(batch), H = static_cast<int>(hh), W = static_cast<int>(w);
+    const int n = static_cast<int>(batch), H = static_cast<int>(h8), W = static_cast<int>(w8);
     net.ensureShape(n, H, W, false);
-    net.forwardFrom(x, false, nullptr, H, W, hh * w, n, H, W, true);
-    net.outputTrain(n, H, W);
+    net.forwardFrom(x, false, nullptr, static_cast<int>(hh), static_cast<int>(w), hh * w, n, H, W,
+                    true);
+    net.outputTrain(n, H, W, static_cast<int>(hh), static_cast<int>(w));
     CFDHIP_CHECK(hipMemcpyAsync(y, net.output(), sizeof(float) * batch * hh * w,
                                 hipMemcpyDeviceToDevice, net.stream));
   });

@@ -91,7 +91,8 @@ __global__ __launch_bounds__(256) void firstConvKernel(const TS* __restrict__ sr
     for (int tap = 0; tap < 9; ++tap) {
       int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
       v[tap] = 0.f;
-      const float sg = padSign(yy, xx, h, wd, pad);
+      // the padding reflects at the source's edges (the physical image)
+      const float sg = padSign(yy, xx, srcH < h ? srcH : h, srcW < wd ? srcW : wd, pad);
       if (sg != 0.f && yy < srcH && xx < srcW) {
         const int64_t si = b * srcBatch + static_cast<int64_t>(yy) * srcW + xx;
         const float sv = static_cast<float>(src[si]);
@@ -122,8 +123,9 @@ struct LastTileLds {
   float s[9 * kLNP];
 };
 
+// h x wd: the d0 tensor; ph x pw: the physical image its padding reflects at
 __device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, int b, int y0, int x0,
-                                int h, int wd, int pad, float out[2]) {
+                                int h, int wd, int ph, int pw, int pad, float out[2]) {
   constexpr int PER = (kLNP + 255) / 256;
   fx4 v[PER][4];
 #pragma unroll
@@ -134,7 +136,7 @@ __device__ inline void lastTile(const float* __restrict__ d0, LastTileLds& L, in
     if (px < kLNP) {
       const int ty = px / kLC, tx = px - ty * kLC;
       int yy = y0 - 1 + ty, xx = x0 - 1 + tx;
-      const float sg = padSign(yy, xx, h, wd, pad);
+      const float sg = padSign(yy, xx, ph, pw, pad);
       if (sg != 0.f) {
         const fx4* src = reinterpret_cast<const fx4*>(
             d0 + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * 16);
@@ -177,26 +179,27 @@ __device__ inline void lastTileCoords(int64_t t, int tilesX, int tilesY, int& b,
   b = static_cast<int>(t / tilesY);
 }
 
-// Training output: y (FP32, n x h x w).
+// Network output y (FP32, n x ph x pw, compact) of the physical image
+// ph x pw inside the h x wd tensors (ph = h, pw = wd for training).
 __global__ __launch_bounds__(256) void lastConvKernel(const float* __restrict__ d0,
                                                       const float* __restrict__ w,
                                                       float* __restrict__ y, int n, int h, int wd,
-                                                      int pad) {
+                                                      int ph, int pw, int pad) {
   __shared__ LastTileLds L;
   if (threadIdx.x < 144) L.ws[threadIdx.x] = roundBf(w[threadIdx.x]);
-  const int tilesX = (wd + kLTW - 1) / kLTW, tilesY = (h + kLTH - 1) / kLTH;
+  const int tilesX = (pw + kLTW - 1) / kLTW, tilesY = (ph + kLTH - 1) / kLTH;
   const int64_t tiles = static_cast<int64_t>(tilesX) * tilesY * n;
   for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
     int b, y0, x0;
     lastTileCoords(t, tilesX, tilesY, b, y0, x0);
     __syncthreads();
     float out[2];
-    lastTile(d0, L, b, y0, x0, h, wd, pad, out);
+    lastTile(d0, L, b, y0, x0, h, wd, ph, pw, pad, out);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int o = threadIdx.x + j * 256;
       const int yy = y0 + o / kLTW, xx = x0 + o % kLTW;
-      if (yy < h && xx < wd) y[(static_cast<int64_t>(b) * h + yy) * wd + xx] = out[j];
+      if (yy < ph && xx < pw) y[(static_cast<int64_t>(b) * ph + yy) * pw + xx] = out[j];
     }
   }
 }
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(256) void lastConvApplyKernel(const float* __restri
     lastTileCoords(t, tilesX, tilesY, b, y0, x0);
     __syncthreads();
     float out[2];
-    lastTile(d0, L, 0, y0, x0, h, wd, pad, out);
+    lastTile(d0, L, 0, y0, x0, h, wd, ny, nx, pad, out);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int o = threadIdx.x + j * 256;
@@ -572,6 +575,12 @@ void Unet::conv(int layer, bool transposed, int gather, int epi, int split, cons
   args.tilesX = (w + 31) / 32;
   args.tilesY = (h + 7) / 8;
   args.pad = padBits();
+  // the physical image on this level: the level-0 extent halved (rounding
+  // up) once per level below the level-0 grid
+  args.ph = physLevelH > 0 ? physH : h;
+  args.pw = physLevelW > 0 ? physW : w;
+  for (int hh = physLevelH; hh > h; hh >>= 1) args.ph = (args.ph + 1) >> 1;
+  for (int ww = physLevelW; ww > w; ww >>= 1) args.pw = (args.pw + 1) >> 1;
   const int ci = transposed ? kLayers[layer].cout : kLayers[layer].cin;
   const int co = transposed ? kLayers[layer].cin : kLayers[layer].cout;
   args.w = transposed ? static_cast<const void*>(bwdPack.get() + bwdOff[layer])
@@ -631,6 +640,11 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
                        int64_t srcBatch, int n, int h, int w, bool fuseFirst) {
   if (!loaded) throw Error(CFDHIP_ERR_INVALID, "network weights are not loaded");
   const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+  // a source smaller than the padded grid is the physical image
+  physH = std::min(srcH, h);
+  physW = std::min(srcW, w);
+  physLevelH = h;
+  physLevelW = w;
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   if (fuseFirst) {
     // inference: e0a is computed inside e0b's tile staging and never stored
@@ -670,10 +684,12 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
   conv(kD0b, false, kGatherPlain, kEpiStore, 0, T(kTd0a), nullptr, T(kTd0), nullptr, n, h, w);
 }
 
-void Unet::outputTrain(int n, int h, int w) {
+void Unet::outputTrain(int n, int h, int w, int ph, int pw) {
+  if (ph <= 0) ph = h;
+  if (pw <= 0) pw = w;
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   hipLaunchKernelGGL(lastConvKernel, grid1(p0 / 2, 4096), dim3(256), 0, stream, T(kTd0),
-                     master.get() + layerOffset(kOut), y.get(), n, h, w, padBits());
+                     master.get() + layerOffset(kOut), y.get(), n, h, w, ph, pw, padBits());
   CFDHIP_CHECK(hipGetLastError());
 }
 
@@ -694,6 +710,9 @@ template void Unet::outputApply<double>(const float*, const double*, const doubl
                                         int, int64_t, int, int, double*, int);
 
 void Unet::backward(const float* x, const float* gyIn, int n, int h, int w) {
+  // training images fill the padded grid: the whole grid is the image
+  physH = physLevelH = h;
+  physW = physLevelW = w;
   require(padBits() == 0 || mirrorSymmetric,
           "Neumann padding: the backward pass needs kernels mirror-symmetric in both axes "
           "(W[ky][kx] == W[2-ky][kx] == W[ky][2-kx]); load such weights with "

@@ -97,6 +97,11 @@ struct ConvArgs {
   // 0) Dirichlet sides (left, right, bottom, top) where it reads the negated
   // nearest pixel (odd reflection: zero at the face)
   int pad;
+  // the image's physical extent on this level (ph <= h, pw <= w): a grid
+  // whose sides are not multiples of 8 sits in the top-left corner of the
+  // padded h x w tensors, and the padding reflects at its edges, not at the
+  // tensors' (pixels beyond it are computed but read by no live pixel)
+  int ph, pw;
 };
 
 struct DwArgs {

@@ -143,8 +143,9 @@ struct ChunkStage {
   static constexpr int ITEMS = ROWS * COLS * C4;
   static constexpr int PER = (ITEMS + 255) / 256;
   fx4 v[PER];
-  __device__ inline void load(const float* A, const float* B, int n, int h, int w, int y0, int x0,
-                              int c0, int pad) {
+  // h x w: the tensor (addressing); ph x pw: the physical image (padding)
+  __device__ inline void load(const float* A, const float* B, int n, int h, int w, int ph, int pw,
+                              int y0, int x0, int c0, int pad) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int it = threadIdx.x + k * 256;
@@ -154,7 +155,7 @@ struct ChunkStage {
         const int px = it / C4;
         const int ty = px / COLS, tx = px - ty * COLS;
         int y = y0 + ty, x = x0 + tx;
-        if (padSign(y, x, h, w, pad) != 0.f)
+        if (padSign(y, x, ph, pw, pad) != 0.f)
           v[k] = gather4<C, CA, G>(A, B, n, h, w, y, x, c0 + c4 * 4);
       }
     }
@@ -248,7 +249,7 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
     const int ty = i / XC, tx = i - ty * XC;
     int y = y0 - 1 + ty, x = x0 - 1 + tx;
     float v = 0.f;
-    const float sg = padSign(y, x, a.h, a.w_, a.pad);
+    const float sg = padSign(y, x, a.ph, a.pw, a.pad);
     if (sg != 0.f && y < a.srcH && x < a.srcW) {
       const int64_t si = static_cast<int64_t>(n) * a.srcH * a.srcW +
                          static_cast<int64_t>(y) * a.srcW + x;
@@ -272,7 +273,7 @@ __device__ inline void stageFirst(bf16* hi, bf16* lo, int cs, const ConvArgs& a,
     // (times the padding's sign), whose taps lie inside the staged window
     int y = y0 + px / kInCols, x = x0 + px % kInCols;
     fx4 acc = {0.f, 0.f, 0.f, 0.f};
-    const float sg = padSign(y, x, a.h, a.w_, a.pad);
+    const float sg = padSign(y, x, a.ph, a.pw, a.pad);
     if (sg != 0.f) {
       const int ty = y - y0, tx = x - x0;
 #pragma unroll
@@ -346,12 +347,13 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
   auto loadChunk = [&](int ch, int tn, int ty0, int tx0) {
     if constexpr (kConcat) {
       if (ch < NCO)
-        coarseBuf.load(a.a, nullptr, tn, a.h >> 1, a.w_ >> 1, (ty0 >> 1) - 1, (tx0 >> 1) - 1,
-                       ch * CC, a.pad);
+        coarseBuf.load(a.a, nullptr, tn, a.h >> 1, a.w_ >> 1, (a.ph + 1) >> 1, (a.pw + 1) >> 1,
+                       (ty0 >> 1) - 1, (tx0 >> 1) - 1, ch * CC, a.pad);
       else
-        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC - CA, a.pad);
+        stageBuf.load(a.b, nullptr, tn, a.h, a.w_, a.ph, a.pw, ty0 - 1, tx0 - 1, ch * CC - CA,
+                      a.pad);
     } else if constexpr (G != kGatherFirst) {
-      stageBuf.load(a.a, a.b, tn, a.h, a.w_, ty0 - 1, tx0 - 1, ch * CC, a.pad);
+      stageBuf.load(a.a, a.b, tn, a.h, a.w_, a.ph, a.pw, ty0 - 1, tx0 - 1, ch * CC, a.pad);
     }
   };
   loadChunk(0, n, y0, x0);
@@ -406,13 +408,14 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
         stageFirst<float>(hiT, loT, CS, a, n, y0 - 1, x0 - 1);
     } else if (kConcat && ch < NCO) {
       coarseBuf.store(hiT, loT, CS);
-      const int ch2 = a.h >> 1, cw2 = a.w_ >> 1, cy = (y0 >> 1) - 1, cx = (x0 >> 1) - 1;
+      const int ch2 = (a.ph + 1) >> 1, cw2 = (a.pw + 1) >> 1, cy = (y0 >> 1) - 1,
+                cx = (x0 >> 1) - 1;
       if (SIDES && decltype(coarseBuf)::hasSigns(ch2, cw2, cy, cx, a.pad))
         decltype(coarseBuf)::flipSides(hiT, loT, CS, ch2, cw2, cy, cx, a.pad);
     } else {
       stageBuf.store(hiT, loT, CS);
-      if (SIDES && decltype(stageBuf)::hasSigns(a.h, a.w_, y0 - 1, x0 - 1, a.pad))
-        decltype(stageBuf)::flipSides(hiT, loT, CS, a.h, a.w_, y0 - 1, x0 - 1, a.pad);
+      if (SIDES && decltype(stageBuf)::hasSigns(a.ph, a.pw, y0 - 1, x0 - 1, a.pad))
+        decltype(stageBuf)::flipSides(hiT, loT, CS, a.ph, a.pw, y0 - 1, x0 - 1, a.pad);
     }
     __syncthreads();
     if constexpr (G != kGatherFirst) {
@@ -422,6 +425,13 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
         loadChunk(0, nn, ny0, nx0);  // the next tile's first chunk
     }
     const bool coarse = kConcat && ch < NCO;
+    // Dirichlet top / right sides of odd fine length, and whether this tile
+    // reaches the fine ghost row / column beyond them (uniform per tile)
+    const bool oddTop = (a.pad & kPadTop) && (a.ph & 1);
+    const bool oddRight = (a.pad & kPadRight) && (a.pw & 1);
+    const bool oddEdgeTile = (oddTop && a.ph >= y0 - 1 && a.ph <= y0 + kTH) ||
+                             (oddRight && a.pw >= x0 - 1 && a.pw <= x0 + kTW);
+    (void)oddEdgeTile;
 #pragma unroll 2
     for (int m = 0; m < MSC; ++m) {
       const int kk = 32 * m + 8 * g;
@@ -438,6 +448,23 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
         for (int q = 0; q < 4; ++q) {
           ah[q] = *reinterpret_cast<const u32x4*>(hiT + base + q * qs);
           al[q] = *reinterpret_cast<const u32x4*>(loT + base + q * qs);
+        }
+        if constexpr (SIDES && kConcat) {
+          // the upsampled half one pixel beyond an odd fine edge maps to the
+          // last coarse pixel inside, which the coarse staging keeps as it is;
+          // across a Dirichlet side that fine ghost is its negation
+          if (coarse && oddEdgeTile) {
+            const int fy = y0 - 1 + ayl + ky;
+            const bool flipY = oddTop && fy == a.ph;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int fx = x0 - 1 + axl + kx + 8 * q;
+              if (flipY != (oddRight && fx == a.pw)) {
+                ah[q] ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+                al[q] ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+              }
+            }
+          }
         }
       } else {
 #pragma unroll
@@ -464,12 +491,22 @@ __global__ __launch_bounds__(256) void convKernel(ConvArgs a) {
     const int hh = a.h >> 1, hw = a.w_ >> 1;
     const int py = (y0 >> 1) + wv;
     constexpr int AUXC = E == kEpiStorePool ? COUT : SPLIT;
+    // a 2 x 2 block that straddles an odd physical edge: the pixels beyond it
+    // add nothing (the pool is then a quarter of the transposed cropped
+    // upsampling, as inside, which keeps a symmetric network symmetric)
+    const bool rowOut = y0 + 2 * wv + 1 >= a.ph;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int px = (x0 >> 1) + 4 * q + g;
+      const bool colOut = x0 + 8 * q + 2 * g + 1 >= a.pw;
 #pragma unroll
       for (int nb = 0; nb < NBG; ++nb) {
-        float s = (acc[q][nb][0] + acc[q][nb][1]) + (acc[q][nb][2] + acc[q][nb][3]);
+        f4v v = acc[q][nb];
+        if (E == kEpiStorePool) {
+          if (rowOut) v[2] = v[3] = 0.f;
+          if (colOut) v[1] = v[3] = 0.f;
+        }
+        float s = (v[0] + v[1]) + (v[2] + v[3]);
         if (E == kEpiStorePool) s *= 0.25f;
         if (py < hh && px < hw)
           a.aux[((static_cast<int64_t>(n) * hh + py) * hw + px) * AUXC + 16 * (nb0 + nb) + li] = s;

@@ -52,8 +52,9 @@ class Unet {
   // (inference only: the first layer's output is not stored).
   void forwardFrom(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
                    int64_t srcBatch, int n, int h, int w, bool fuseFirst = false);
-  // y = conv(d0) into the FP32 training output.
-  void outputTrain(int n, int h, int w);
+  // y = conv(d0) into the FP32 output, compact n x ph x pw for the physical
+  // image ph x pw of the last forward pass (0: the whole h x w grid).
+  void outputTrain(int n, int h, int w, int ph = 0, int pw = 0);
   // z = scale * conv(d0) on the ny x nx solver grid, z[pin] = r[pin] scale^2
   // (the inverse of the decoupled pinned row), and block partials of (z, q)
   // (parts blocks; none when partial is null).
@@ -113,6 +114,9 @@ class Unet {
     const float* scale = nullptr;
     int srcH = 0, srcW = 0, srcDouble = 0;
   } first;
+  // physical image of the last forward pass (its level-0 extent and the
+  // level-0 tensor size it sits in); ConvArgs::ph / pw of every conv
+  int physH = 0, physW = 0, physLevelH = 0, physLevelW = 0;
   int shapeN = 0, shapeH = 0, shapeW = 0;
   bool shapeTraining = false;
   bool loaded = false;

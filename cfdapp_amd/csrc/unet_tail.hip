@@ -27,30 +27,37 @@ dim3 tailGrid(int64_t n) {
 }
 
 // One image's tail workspace: per level t the residual R_t, a work image
-// S_t and the correction E_t.
+// S_t and the correction E_t. Level t is the (t + 1)-th halving of the bottom
+// grid, rounding up: a side of odd length keeps a last coarse pixel that
+// covers one fine row (column) only; the restriction then sums the fine
+// pixels it covers and the prolongation drops the copies beyond the fine
+// edge (restriction = prolongation transposed, as on even sides).
 struct TailShape {
   int h3, w3, levels;
+  int hs[16], ws[16];
   int64_t perImage;
   int64_t off[3 * 16];  // R_t, S_t, E_t offsets
 };
 
-__host__ __device__ inline int tailH(const TailShape& s, int t) { return s.h3 >> (t + 1); }
-__host__ __device__ inline int tailW(const TailShape& s, int t) { return s.w3 >> (t + 1); }
+__host__ __device__ inline int tailH(const TailShape& s, int t) { return s.hs[t]; }
+__host__ __device__ inline int tailW(const TailShape& s, int t) { return s.ws[t]; }
 
 TailShape tailShape(int h3, int w3, int maxLevels) {
   TailShape s{};
   s.h3 = h3;
   s.w3 = w3;
   int hh = h3, ww = w3, L = 0;
-  while (L < maxLevels && L < 16 && hh % 2 == 0 && ww % 2 == 0 && hh >= 2 && ww >= 2) {
-    hh /= 2;
-    ww /= 2;
+  while (L < maxLevels && L < 16 && hh >= 2 && ww >= 2) {
+    hh = (hh + 1) / 2;
+    ww = (ww + 1) / 2;
+    s.hs[L] = hh;
+    s.ws[L] = ww;
     ++L;
   }
   s.levels = L;
   int64_t o = 0;
   for (int t = 0; t < L; ++t) {
-    const int64_t m = static_cast<int64_t>(h3 >> (t + 1)) * (w3 >> (t + 1));
+    const int64_t m = static_cast<int64_t>(s.hs[t]) * s.ws[t];
     for (int q = 0; q < 3; ++q) {
       s.off[3 * t + q] = o;
       o += m;
@@ -117,6 +124,7 @@ __device__ inline float restrictK(const float* v, int h, int w, int y, int x, co
   for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
     for (int dx = 0; dx < 2; ++dx) {
+      if (2 * y + dy >= h || 2 * x + dx >= w) continue;  // beyond an odd fine edge
       float q[3][3];
 #pragma unroll
       for (int a = 0; a < 3; ++a)
@@ -127,10 +135,11 @@ __device__ inline float restrictK(const float* v, int h, int w, int y, int x, co
   return s;  // 4 x the 2 x 2 average
 }
 
-// K up(e) at fine pixel (y, x) of the h x w grid (e on h/2 x w/2).
+// K up(e) at fine pixel (y, x) of the h x w grid (e on the rounded-up
+// halves ceil(h / 2) x ceil(w / 2)).
 __device__ inline float prolongK(const float* e, int h, int w, int y, int x, const TailParams& p,
                                  int pad) {
-  const int wc = w >> 1;
+  const int wc = (w + 1) >> 1;
   float q[3][3];
   if (y > 0 && y + 1 < h && x > 0 && x + 1 < w) {
 #pragma unroll

@@ -458,7 +458,12 @@ int cfdhip_unet_get_weights(cfdhip_unet* net, float* weights, int64_t count);
 int cfdhip_unet_forward(cfdhip_unet* net, const float* x, float* y, int64_t batch, int64_t h,
                         int64_t w);
 /* y = N(x) along the preconditioner's inference path (the first layer fused
- * into the second; its output is not kept). Same values as cfdhip_unet_forward. */
+ * into the second; its output is not kept). Same values as cfdhip_unet_forward.
+ * Any h, w >= 1: the network runs on the grid rounded up to multiples of 8 with
+ * the image in its corner; each level's padding reflects at that level's
+ * edges (rounded-up halves), and a 2x2 pool that straddles an odd edge takes
+ * the padding's ghost of the pixel inside (what the learned p' solve does on
+ * grids whose sides are not multiples of 8). */
 int cfdhip_unet_apply(cfdhip_unet* net, const float* x, float* y, int64_t batch, int64_t h,
                       int64_t w);
 /* grad (host, param_count floats) = d/dW of sum(gy * N(x)). */

@@ -143,7 +143,16 @@ def conv3x3_weight_grad(x, g, neumann=False):
 
 
 def pool(x):
+    """2x2 average to the rounded-up halves. On a side of odd length the
+    coarse pixel that straddles the edge averages only the pixels inside
+    (a zero row or column beyond the last one): a quarter of the transposed
+    cropped upsampling ``up``, as everywhere inside, so the V-cycle network
+    stays symmetric (the device's rule for sides that are not multiples of 8)."""
+    x = np.asarray(x, dtype=np.float64)
     n, h, w, c = x.shape
+    if h % 2 or w % 2:
+        x = np.pad(x, ((0, 0), (0, h % 2), (0, w % 2), (0, 0)))
+        n, h, w, c = x.shape
     return x.reshape(n, h // 2, 2, w // 2, 2, c).mean(axis=(2, 4))
 
 
@@ -151,8 +160,10 @@ def pool_grad(g):
     return np.repeat(np.repeat(g, 2, axis=1), 2, axis=2) * 0.25
 
 
-def up(x):
-    return np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)
+def up(x, like=None):
+    """Nearest x2; cropped to the fine image `like` (odd sides)."""
+    u = np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)
+    return u if like is None else u[:, :like.shape[1], :like.shape[2]]
 
 
 def up_grad(g):
@@ -179,33 +190,38 @@ def _stencil(v, c, e, k, neumann=True):
 def tail_apply(s, tail, neumann=True):
     """The coarse tail on (N, H, W) images: K up(V(4 pool(K s))) with V the
     symmetric V(1,1) cycle of the device's TailParams (the network's padding;
-    levels H/2 .. the coarsest grid with even sides, at most tail["levels"])."""
+    levels: the rounded-up halves of H x W while both sides are at least 2, at
+    most tail["levels"] of them)."""
     s = np.asarray(s, dtype=np.float64)
     om, a, k = tail["omega"], tail["a"], tail["k"]
     st = lambda v, kern: _stencil(v, *kern, neumann)  # noqa: E731
     h, w = s.shape[1:]
     L = 0
-    while L < tail["levels"] and h % 2 == 0 and w % 2 == 0 and h >= 2 and w >= 2:
-        h, w, L = h // 2, w // 2, L + 1
+    while L < tail["levels"] and h >= 2 and w >= 2:  # rounded-up halves
+        h, w, L = (h + 1) // 2, (w + 1) // 2, L + 1
     if L == 0:
         return np.zeros_like(s)
 
     def restrict(v):
+        """4 pool(K v): the sum over the fine pixels each coarse pixel covers."""
         kv = st(v, k)
         n, hh, ww = kv.shape
-        return kv.reshape(n, hh // 2, 2, ww // 2, 2).sum(axis=(2, 4))
+        kv = np.pad(kv, ((0, 0), (0, hh % 2), (0, ww % 2)))
+        return kv.reshape(n, (hh + 1) // 2, 2, (ww + 1) // 2, 2).sum(axis=(2, 4))
 
-    def prolong(e):
-        return st(np.repeat(np.repeat(e, 2, axis=1), 2, axis=2), k)
+    def prolong(e, like):
+        """K up(e) on the fine grid of `like` (copies beyond an odd edge dropped)."""
+        u = np.repeat(np.repeat(e, 2, axis=1), 2, axis=2)[:, :like.shape[1], :like.shape[2]]
+        return st(u, k)
 
     r = [restrict(s)]
     for _ in range(L - 1):
         r.append(restrict(r[-1] - om * st(r[-1], a)))
     e = 2.0 * om * r[L - 1] - om * om * st(r[L - 1], a)
     for t in range(L - 2, -1, -1):
-        x2 = om * r[t] + prolong(e)
+        x2 = om * r[t] + prolong(e, r[t])
         e = x2 - om * st(x2, a) + om * r[t]
-    return prolong(e)
+    return prolong(e, s)
 
 
 def _tail_forward(b3a, opts):
@@ -228,7 +244,9 @@ def _tail_backward(g, opts):
 
 
 def forward(w, x, keep=False, opts=None):
-    """x: (N, H, W, 1) with H, W divisible by 8. Returns y (N, H, W, 1) [and the tape]."""
+    """x: (N, H, W, 1). Returns y (N, H, W, 1) [and the tape]. Sides not
+    divisible by 8 coarsen to the rounded-up half per level (pool's ghost
+    rule, up's crop); the padding reflects at each level's edges."""
     nm = _neumann(opts)
 
     def cv(a, name):
@@ -246,13 +264,13 @@ def forward(w, x, keep=False, opts=None):
     t["p2"] = pool(t["e2"])
     t["b3a"] = _tail_forward(cv(t["p2"], "b3a"), opts)
     t["b3"] = cv(t["b3a"], "b3b")
-    t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
+    t["c2"] = np.concatenate([up(t["b3"], t["e2"]), t["e2"]], axis=3)
     t["d2a"] = cv(t["c2"], "d2a")
     t["d2"] = cv(t["d2a"], "d2b")
-    t["c1"] = np.concatenate([up(t["d2"]), t["e1"]], axis=3)
+    t["c1"] = np.concatenate([up(t["d2"], t["e1"]), t["e1"]], axis=3)
     t["d1a"] = cv(t["c1"], "d1a")
     t["d1"] = cv(t["d1a"], "d1b")
-    t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
+    t["c0"] = np.concatenate([up(t["d1"], t["e0"]), t["e0"]], axis=3)
     t["d0a"] = cv(t["c0"], "d0a")
     t["d0"] = cv(t["d0a"], "d0b")
     t["y"] = cv(t["d0"], "out")
@@ -346,13 +364,13 @@ def forward_device(w, x, opts=None):
     t["e2"], t["p2"] = conv_pool("e2b", t["e2a"])
     t["b3a"] = f32(_tail_forward(f32(conv("b3a", t["p2"])), opts))
     t["b3"] = f32(conv("b3b", t["b3a"]))
-    t["c2"] = np.concatenate([up(t["b3"]), t["e2"]], axis=3)
+    t["c2"] = np.concatenate([up(t["b3"], t["e2"]), t["e2"]], axis=3)
     t["d2a"] = f32(conv("d2a", t["c2"]))
     t["d2"] = f32(conv("d2b", t["d2a"]))
-    t["c1"] = np.concatenate([up(t["d2"]), t["e1"]], axis=3)
+    t["c1"] = np.concatenate([up(t["d2"], t["e1"]), t["e1"]], axis=3)
     t["d1a"] = f32(conv("d1a", t["c1"]))
     t["d1"] = f32(conv("d1b", t["d1a"]))
-    t["c0"] = np.concatenate([up(t["d1"]), t["e0"]], axis=3)
+    t["c0"] = np.concatenate([up(t["d1"], t["e0"]), t["e0"]], axis=3)
     t["d0a"] = f32(conv("d0a", t["c0"]))
     t["d0"] = f32(conv("d0b", t["d0a"]))
     t["y"] = f32(conv3x3(t["d0"], wb["out"], nm))

@@ -1,11 +1,15 @@
 """Learned FCG at grid sizes that are not multiples of 8, and its stagnation fallback.
 
-The network runs on a grid rounded up to a multiple of 8. These tests solve
+The network's tensors are rounded up to multiples of 8 with the solver grid
+in their corner; every level's padding reflects at that level's own edges
+(the rounded-up halves), and the pools, upsamplings and the coarse tail treat
+odd sides as the oracle does (tests/test_vcycle_oracle.py). These tests solve
 p' systems whose sides are not multiples of 8 (the 129^2 C1 cavity, a 100 x 60
-cavity, a 100 x 60 channel whose FixedValue outlet gives a Dirichlet side)
-and require convergence to the FP64 target without the Jacobi-PCG fallback
-within a bound on the iterations, printing the count beside the nearest
-multiple-of-8 grid.
+cavity, 100 x 60 and 61 x 203 channels whose FixedValue outlet gives a
+Dirichlet side) and require convergence to the FP64 target without the
+Jacobi-PCG fallback in about as many iterations as the nearest multiple-of-8
+grid (measured: 25 vs 20, 23 vs 19, 14 vs 13, 16 vs 13; before the
+physical-edge padding the odd grids took 63, 45, 62 and 35).
 
 The fallback: with noisy weights and a stagnation window of one iteration
 (the first iteration whose residual is not a new minimum ends the flexible
@@ -66,7 +70,7 @@ def test_learned_fcg_off_multiple_of_8(kind, nx, ny, near):
             its[shape] = r.iterations
         finally:
             d.close()
-    assert its[(nx, ny)] <= 80
+    assert its[(nx, ny)] <= 1.5 * its[near] + 2
 
 
 def test_stagnation_window_hands_over_to_jacobi_pcg():

@@ -92,8 +92,12 @@ def test_tail_is_symmetric_positive_semidefinite():
     ta, tb = U.tail_apply(a, tail), U.tail_apply(b, tail)
     assert np.sum(ta * b) == pytest.approx(np.sum(a * tb), rel=1e-12)
     assert np.sum(ta * a) > 0.0
-    # an odd side has no tail level: the correction is zero
-    assert not np.any(U.tail_apply(rng.standard_normal((1, 5, 8)), tail))
+    # odd sides coarsen to the rounded-up halves; the tail stays symmetric and
+    # positive semi-definite there
+    a, b = rng.standard_normal((1, 17, 9)), rng.standard_normal((1, 17, 9))
+    ta, tb = U.tail_apply(a, tail), U.tail_apply(b, tail)
+    assert np.sum(ta * b) == pytest.approx(np.sum(a * tb), rel=1e-12)
+    assert np.sum(ta * a) > 0.0
 
 
 def _dense_stencil(h, w, c, e, k, sides=()):
@@ -116,11 +120,28 @@ def _dense_stencil(h, w, c, e, k, sides=()):
 
 
 def _dense_up(h, w):
-    """Nearest upsampling from (h/2, w/2) to (h, w) as a matrix."""
-    m = np.zeros((h * w, (h // 2) * (w // 2)))
+    """Nearest upsampling from the rounded-up halves (ceil(h/2), ceil(w/2)) to
+    (h, w) as a matrix (an odd side drops the last coarse pixel's second copy)."""
+    hc, wc = (h + 1) // 2, (w + 1) // 2
+    m = np.zeros((h * w, hc * wc))
     for y in range(h):
         for x in range(w):
-            m[y * w + x, (y // 2) * (w // 2) + x // 2] = 1.0
+            m[y * w + x, (y // 2) * wc + x // 2] = 1.0
+    return m
+
+
+def _dense_pool(h, w):
+    """2 x 2 average from (h, w) to (ceil(h/2), ceil(w/2)) as a matrix, each
+    coarse pixel over the fine pixels it covers inside the image (weight 1/4)."""
+    hc, wc = (h + 1) // 2, (w + 1) // 2
+    m = np.zeros((hc * wc, h * w))
+    for yc in range(hc):
+        for xc in range(wc):
+            for a in (0, 1):
+                for b in (0, 1):
+                    y, x = 2 * yc + a, 2 * xc + b
+                    if y < h and x < w:
+                        m[yc * wc + xc, y * w + x] += 0.25
     return m
 
 
@@ -131,14 +152,20 @@ def _dense_vcycle(theta, h, w, tail, sides=()):
     A = [tuple(theta[4 + 3 * l:7 + 3 * l]) for l in range(4)]
     K = [tuple(theta[16 + 3 * j:19 + 3 * j]) for j in range(3)]
     levels = 4
-    while tail and (h >> (levels - 1)) % 2 == 0 and (w >> (levels - 1)) % 2 == 0 \
-            and (h >> levels) >= 1 and (w >> levels) >= 1:
+    hl, wl = h, w
+    for _ in range(3):
+        hl, wl = (hl + 1) // 2, (wl + 1) // 2
+    while tail and hl >= 2 and wl >= 2 and levels - 4 < VCycle.TAIL["levels"]:
+        hl, wl = (hl + 1) // 2, (wl + 1) // 2
         om.append(VCycle.TAIL["omega"])
         A.append(VCycle.TAIL["a"])
         K.append(VCycle.TAIL["k"])
         levels += 1
-    # K between level 3 and the tail is the tail's kernel
-    sizes = [(h >> l, w >> l) for l in range(levels)]
+    # K between level 3 and the tail is the tail's kernel; sides that are
+    # not multiples of 8 coarsen to the rounded-up halves
+    sizes = [(h, w)]
+    for _ in range(levels - 1):
+        sizes.append(((sizes[-1][0] + 1) // 2, (sizes[-1][1] + 1) // 2))
     B = None
     for l in range(levels - 1, -1, -1):
         hh, ww = sizes[l]
@@ -150,8 +177,10 @@ def _dense_vcycle(theta, h, w, tail, sides=()):
         if B is None:
             B = base
             continue
-        P = _dense_stencil(hh, ww, *K[l], sides=sides) @ _dense_up(hh, ww)
-        B = base + S @ P @ B @ P.T @ S
+        Kl = _dense_stencil(hh, ww, *K[l], sides=sides)
+        P = Kl @ _dense_up(hh, ww)  # prolongation K up(.)
+        R = 4.0 * _dense_pool(hh, ww) @ Kl  # restriction 4 pool(K .) = P^T
+        B = base + S @ P @ B @ R @ S
     return B
 
 
@@ -167,6 +196,29 @@ def test_vcycle_network_equals_dense_cycle():
     np.testing.assert_allclose(y, B @ x[0, :, :, 0].ravel(), rtol=1e-10, atol=1e-10)
     assert np.abs(B - B.T).max() < 1e-10 * np.abs(B).max()
     assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > -1e-10
+
+
+@pytest.mark.parametrize("h,wd,tail,sides", [(61, 45, True, ()), (37, 21, False, ()),
+                                             (61, 45, True, ("right",)),
+                                             (29, 50, False, ("top", "left"))])
+def test_vcycle_network_on_odd_sides_equals_dense_cycle(h, wd, tail, sides):
+    """Sides that are not multiples of 8 (the p' solve's grid in the corner of
+    the padded tensors): the network equals the V(1,1) cycle on the rounded-up
+    coarse grids, with prolongation K up(.) dropping the copies beyond an odd
+    edge and restriction its transpose (the pool over the pixels inside), and
+    so stays symmetric positive semi-definite."""
+    rng = np.random.default_rng(h * wd)
+    th = VCycle.repair(VCycle.initial() + rng.normal(0.0, 0.03, VCycle.N_THETA))
+    th[25] = 0.0
+    w = _w64(th)
+    opts = dict(_opts(tail=tail), dirichlet_sides=list(sides))
+    B = _dense_vcycle(th, h, wd, tail=tail, sides=sides)
+    x = rng.standard_normal((1, h, wd, 1))
+    y = U.forward(w, x, opts=opts)[0, :, :, 0].ravel()
+    np.testing.assert_allclose(y, B @ x[0, :, :, 0].ravel(), rtol=1e-10, atol=1e-10)
+    # still a symmetric positive semi-definite preconditioner
+    assert np.abs(B - B.T).max() < 1e-10 * np.abs(B).max()
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > -1e-10 * np.abs(B).max()
 
 
 def test_backward_with_neumann_and_tail_matches_finite_differences():
