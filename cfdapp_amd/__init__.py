@@ -361,6 +361,17 @@ class GpuSimpleDiscretization:
     def assemble_momentum_resident(self, alpha):
         self._ctx.call("cfdhip_assemble_momentum", float(alpha))
 
+    def assemble_momentum_component(self, component, alpha, convection_scheme=0,
+                                    non_orthogonal=False, previous_component=None):
+        """assembleMomentum / assembleMomentumResident for one component
+        (include/cfd/gpu/GpuSimpleDiscretization.hpp:162-164,181-182); with
+        previous_component the relaxation's phi_old is the caller's field."""
+        prev = None if previous_component is None else _f64(previous_component)
+        if prev is not None and prev.size != self.n_cells:
+            raise ValueError("assembleMomentum: previousComponent size does not match the mesh")
+        self._ctx.call("cfdhip_assemble_momentum_component", int(component), float(alpha),
+                       int(convection_scheme), 1 if non_orthogonal else 0, _pf(prev))
+
     def assemble_transient_momentum(self, dt):
         """The PISO predictor's momentum system (TransientMomentum.cpp)."""
         self._ctx.call("cfdhip_assemble_transient_momentum", float(dt))
@@ -420,6 +431,42 @@ class GpuSimpleDiscretization:
 
     def correct(self):
         self._ctx.call("cfdhip_correct")
+
+    def correct_velocity(self, resident=True):
+        """correctVelocityResident (resident=True: into U/V/W) or
+        correctVelocity (returns the corrected components, U/V/W unchanged)."""
+        if resident:
+            self._ctx.call("cfdhip_correct_velocity", None)
+            return None
+        out = np.empty(3 * self.n_cells)
+        self._ctx.call("cfdhip_correct_velocity", _pf(out))
+        return [out[k * self.n_cells:(k + 1) * self.n_cells] for k in range(self.dim)]
+
+    def correct_flux(self, resident=True):
+        """correctFaceMassFluxResident (into FLUX) or correctFaceMassFlux
+        (returns the corrected flux, FLUX unchanged)."""
+        if resident:
+            self._ctx.call("cfdhip_correct_flux", None)
+            return None
+        out = np.empty(self.n_faces)
+        self._ctx.call("cfdhip_correct_flux", _pf(out))
+        return out
+
+    def all_finite(self, *names):
+        """SIMPLE's resident finiteness guards: True when no listed field
+        holds a NaN or an infinity."""
+        ids = (ctypes.c_int * max(1, len(names)))(*[FIELDS[n] for n in names])
+        ok = ctypes.c_int(0)
+        self._ctx.call("cfdhip_fields_all_finite", ids, len(names), ctypes.byref(ok))
+        return bool(ok.value)
+
+    def set_density(self, density):
+        self._ctx.call("cfdhip_set_density", float(density))
+
+    def resident_solve_bytes(self):
+        b = ctypes.c_int64(0)
+        self._ctx.call("cfdhip_resident_bytes", ctypes.byref(b))
+        return int(b.value)
 
     def continuity(self):
         out = np.zeros(2)

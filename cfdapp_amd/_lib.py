@@ -106,6 +106,15 @@ SIGNATURES = {
                                              ctypes.POINTER(SolveResult)]),
     "cfdhip_update_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "cfdhip_correct": (ctypes.c_int, [ctypes.c_void_p]),
+    "cfdhip_correct_velocity": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
+    "cfdhip_correct_flux": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
+    "cfdhip_fields_all_finite": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "cfdhip_set_density": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "cfdhip_resident_bytes": (ctypes.c_int, [ctypes.c_void_p, P_I64]),
+    "cfdhip_assemble_momentum_component": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                                          ctypes.c_double, ctypes.c_int,
+                                                          ctypes.c_int, P_F64]),
     "cfdhip_continuity": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
     "cfdhip_simple_run": (ctypes.c_int, [ctypes.c_void_p, P_F64, ctypes.POINTER(SolverSettings),
                                          ctypes.POINTER(SolverSettings), P_F64, P_F64,

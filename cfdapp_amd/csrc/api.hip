@@ -32,6 +32,21 @@ void Context::ensureFields() {
   CFDHIP_CHECK(hipStreamSynchronize(stream));
 }
 
+int64_t KrylovWork::bytes() const {
+  int64_t b = 0;
+  for (const DeviceArray<double>* a : {&r, &p, &Ap, &z, &x, &this->b, &dinv, &rhat, &v, &s, &t,
+                                       &phat, &shat, &partial, &hist, &scal})
+    b += a->count * static_cast<int64_t>(sizeof(double));
+  for (const DeviceArray<float>* a : {&r32, &x32, &dinv32, &w32, &dis32, &scaleTmp[0],
+                                      &scaleTmp[1]})
+    b += a->count * static_cast<int64_t>(sizeof(float));
+  for (int q = 0; q < 8; ++q)
+    b += d8[q].count * static_cast<int64_t>(sizeof(double)) +
+         f8[q].count * static_cast<int64_t>(sizeof(float));
+  b += state.count * static_cast<int64_t>(sizeof(KState));
+  return b;
+}
+
 }  // namespace cfdhip
 
 namespace {
@@ -272,6 +287,31 @@ int cfdhip_assemble_momentum(cfdhip_ctx* ctx, double alpha) {
   return guarded(ctx, [&](Context& c) { launchMomentum(c, alpha); });
 }
 
+int cfdhip_assemble_momentum_component(cfdhip_ctx* ctx, int component, double alpha,
+                                       int convection_scheme, int non_orthogonal,
+                                       const double* previous_component) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(component >= 0 && component < c.view.dim, "component out of range");
+    // cfd::discretization::ConvectionScheme: 0 = Upwind, the scheme these
+    // kernels implement (MomentumEquation.cpp:264-327)
+    require(convection_scheme == 0,
+            "the structured momentum kernels implement the Upwind convection scheme only",
+            CFDHIP_ERR_UNSUPPORTED);
+    require(non_orthogonal == 0,
+            "the explicit non-orthogonal momentum correction is not on the structured path",
+            CFDHIP_ERR_UNSUPPORTED);
+    const double* old = nullptr;
+    if (previous_component != nullptr) {
+      c.stage.ensure(3 * c.view.n);
+      CFDHIP_CHECK(hipMemcpyAsync(c.stage.get(), previous_component, c.view.n * sizeof(double),
+                                  hipMemcpyHostToDevice, c.stream));
+      old = c.stage.get();
+    }
+    launchMomentum(c, alpha, 0.0, component, old);
+  });
+}
+
 int cfdhip_assemble_transient_momentum(cfdhip_ctx* ctx, double dt) {
   return guarded(ctx, [&](Context& c) {
     require(std::isfinite(dt) && dt > 0.0,
@@ -392,6 +432,74 @@ int cfdhip_correct(cfdhip_ctx* ctx) {
   return guarded(ctx, [&](Context& c) {
     launchCorrect(c);
     CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_correct_velocity(cfdhip_ctx* ctx, double* out) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    const int64_t n = c.view.n;
+    if (out == nullptr) {
+      launchCorrectVelocity(c);
+    } else {
+      c.stage.ensure(3 * n);
+      double* dest[3] = {c.stage.get(), c.stage.get() + n, c.stage.get() + 2 * n};
+      // a 2D correction leaves W = 0, as the reference's Vector2 velocity
+      if (c.view.dim == 2) CFDHIP_CHECK(hipMemsetAsync(dest[2], 0, n * sizeof(double), c.stream));
+      launchCorrectVelocity(c, dest);
+      CFDHIP_CHECK(hipMemcpyAsync(out, c.stage.get(), 3 * n * sizeof(double),
+                                  hipMemcpyDeviceToHost, c.stream));
+    }
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_correct_flux(cfdhip_ctx* ctx, double* out) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    const int64_t nf = c.view.nf;
+    if (out == nullptr) {
+      launchCorrectFlux(c);
+    } else {
+      c.stage.ensure(nf > 3 * c.view.n ? nf : 3 * c.view.n);
+      launchCorrectFlux(c, c.stage.get());
+      CFDHIP_CHECK(hipMemcpyAsync(out, c.stage.get(), nf * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+    }
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_fields_all_finite(cfdhip_ctx* ctx, const int* fields, int count, int* all_finite) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(all_finite != nullptr && count >= 0 && (count == 0 || fields != nullptr),
+            "a field list and an output are required");
+    require(count <= CFDHIP_FIELD_COUNT, "too many fields");
+    const double* arrays[CFDHIP_FIELD_COUNT];
+    int64_t counts[CFDHIP_FIELD_COUNT];
+    for (int k = 0; k < count; ++k) {
+      require(fields[k] >= 0 && fields[k] < CFDHIP_FIELD_COUNT, "unknown field");
+      arrays[k] = c.f(fields[k]);
+      counts[k] = fieldCount(c, fields[k]);
+    }
+    *all_finite = countNonFinite(c, arrays, counts, count) == 0 ? 1 : 0;
+  });
+}
+
+int cfdhip_set_density(cfdhip_ctx* ctx, double density) {
+  return guarded(ctx, [&](Context& c) {
+    c.requireMesh();
+    require(std::isfinite(density) && density > 0.0, "density must be finite and > 0");
+    c.rho = density;
+    c.view.rho = density;
+  });
+}
+
+int cfdhip_resident_bytes(cfdhip_ctx* ctx, int64_t* bytes) {
+  return guarded(ctx, [&](Context& c) {
+    require(bytes != nullptr, "bytes is required");
+    *bytes = c.kw.bytes();
   });
 }
 

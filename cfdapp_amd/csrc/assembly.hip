@@ -209,6 +209,7 @@ struct MomentumArgs {
   // previous velocity uOld with density rho
   double rho, dt;
   const double* uOld[3];
+  const double* uRelax[3];  // the under-relaxation's phi_old (steady: the resident velocity)
 };
 
 __global__ void momentumKernel(View v, MomentumArgs m) {
@@ -307,7 +308,7 @@ __global__ void momentumKernel(View v, MomentumArgs m) {
     const double factor = (1.0 / m.alpha) - 1.0;
     const double extra = acc[P].s * factor;
     acc[P].add(extra);
-    for (int k = 0; k < D; ++k) rhs[k] += extra * m.u[k][c];
+    for (int k = 0; k < D; ++k) rhs[k] += extra * m.uRelax[k][c];
   }
 
   for (int q = 0; q < 2 * D + 1; ++q) {
@@ -706,7 +707,8 @@ void launchMassFlux(Context& ctx, const double* u, const double* v, const double
   CFDHIP_CHECK(hipGetLastError());
 }
 
-void launchMomentum(Context& ctx, double alpha, double dt) {
+void launchMomentum(Context& ctx, double alpha, double dt, int oldComponent,
+                    const double* oldValues) {
   ctx.requireMesh();
   require(std::isfinite(alpha) && alpha > 0.0 && alpha <= 1.0,
           "applyImplicitUnderRelaxation: alpha must be finite and in (0, 1]");
@@ -733,7 +735,9 @@ void launchMomentum(Context& ctx, double alpha, double dt) {
   m.alpha = alpha;
   m.rho = ctx.rho;
   m.dt = dt;
-  for (int k = 0; k < 3; ++k) m.uOld[k] = m.u[k];
+  for (int k = 0; k < 3; ++k) m.uOld[k] = m.uRelax[k] = m.u[k];
+  if (oldComponent >= 0 && oldComponent < D && oldValues != nullptr)
+    m.uRelax[oldComponent] = oldValues;
   hipLaunchKernelGGL(momentumKernel, blocksFor(v.n), dim3(kBlock), 0, ctx.stream, v, m);
   CFDHIP_CHECK(hipGetLastError());
   const unsigned bad = readAndClear(ctx);
@@ -853,22 +857,32 @@ void launchUpdatePressure(Context& ctx, double alpha) {
   CFDHIP_CHECK(hipGetLastError());
 }
 
-void launchCorrect(Context& ctx) {
+void launchCorrectVelocity(Context& ctx, double* const dest[3]) {
   ctx.requireMesh();
   const View& v = ctx.view;
   CorrectArgs m{};
   for (int k = 0; k < 3; ++k) {
     m.ustar[k] = k < v.dim ? ctx.f(CFDHIP_FIELD_USTAR + k) : nullptr;
-    m.u[k] = k < v.dim ? ctx.f(CFDHIP_FIELD_U + k) : nullptr;
+    m.u[k] = k < v.dim ? (dest != nullptr ? dest[k] : ctx.f(CFDHIP_FIELD_U + k)) : nullptr;
   }
   m.D = ctx.f(CFDHIP_FIELD_RESPONSE);
   m.pp = ctx.f(CFDHIP_FIELD_PPRIME);
   hipLaunchKernelGGL(correctVelocityKernel, blocksFor(v.n), dim3(kBlock), 0, ctx.stream, v, m);
   CFDHIP_CHECK(hipGetLastError());
+}
+
+void launchCorrectFlux(Context& ctx, double* dest) {
+  ctx.requireMesh();
+  const View& v = ctx.view;
   hipLaunchKernelGGL(correctFluxKernel, blocksFor(v.nf), dim3(kBlock), 0, ctx.stream, v,
                      ctx.f(CFDHIP_FIELD_FLUX_STAR), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
-                     ctx.f(CFDHIP_FIELD_PPRIME), ctx.f(CFDHIP_FIELD_FLUX));
+                     ctx.f(CFDHIP_FIELD_PPRIME), dest != nullptr ? dest : ctx.f(CFDHIP_FIELD_FLUX));
   CFDHIP_CHECK(hipGetLastError());
+}
+
+void launchCorrect(Context& ctx) {
+  launchCorrectVelocity(ctx);
+  launchCorrectFlux(ctx);
 }
 
 // Block maxima of v over a grid-stride range (0 when empty).

@@ -50,6 +50,8 @@ struct KrylovWork {
   DeviceArray<double> hist;
   DeviceArray<double> scal;  // scalar results of one-off reductions
   KState* hostState = nullptr;  // pinned mirror
+  // device bytes held by the work vectors and reduction buffers above
+  int64_t bytes() const;
 };
 
 struct CsrDevice {
@@ -97,6 +99,7 @@ struct Context {
   DeviceArray<double> field[CFDHIP_FIELD_COUNT];
   DeviceArray<double> momCoef;  // (2*dim+1) * n, CSR column order
   DeviceArray<unsigned> flags;  // [0] non-finite counter
+  DeviceArray<double> stage;    // a caller's phi_old / correction outputs on their way to the host
 
   KrylovWork kw;
   CompressibleWork comp;
@@ -128,7 +131,10 @@ void exportFaces(Context& ctx, int64_t* owner, int64_t* neighbor);
 void launchGradient(Context& ctx, const double* phi, int bcKind, double* gx, double* gy, double* gz);
 void launchMassFlux(Context& ctx, const double* u, const double* v, const double* w, double* F);
 // dt > 0: the transient momentum system (TransientMomentum.cpp), alpha 1
-void launchMomentum(Context& ctx, double alpha, double dt = 0.0);
+// oldComponent >= 0: oldValues (device, n) replaces that component's resident
+// velocity as the relaxation's phi_old
+void launchMomentum(Context& ctx, double alpha, double dt = 0.0, int oldComponent = -1,
+                    const double* oldValues = nullptr);
 void launchResponse(Context& ctx);
 void launchPredictedFlux(Context& ctx, bool rhieChow, double alpha);
 // rhoF: per-face density (nullptr: the fluid's); extra: an additional diagonal
@@ -142,6 +148,10 @@ void launchCompressiblePressureAssembly(Context& ctx, int64_t ref, const double*
                                         double dtau);
 void launchUpdatePressure(Context& ctx, double alpha);
 void launchCorrect(Context& ctx);
+// the two halves of launchCorrect; a non-null destination receives the result
+// (velocity: 3 component arrays of n) instead of U/V/W or FLUX
+void launchCorrectVelocity(Context& ctx, double* const dest[3] = nullptr);
+void launchCorrectFlux(Context& ctx, double* dest = nullptr);
 void launchContinuity(Context& ctx, double* out);
 // max over cells of the Courant number of FLUX (cfl) or of |IMBALANCE|
 double launchMaxCell(Context& ctx, bool cfl, double dt);

@@ -185,6 +185,22 @@ int cfdhip_mass_flux(cfdhip_ctx* ctx);
  * (src/pressure_velocity/RelaxedMomentum.cpp:30; the u, v, w matrices are
  * identical, the right-hand sides differ). assembleMomentumResident. */
 int cfdhip_assemble_momentum(cfdhip_ctx* ctx, double alpha);
+/* GpuSimpleDiscretization::assembleMomentum / assembleMomentumResident for one
+ * component (include/cfd/gpu/GpuSimpleDiscretization.hpp:162-164,181-182;
+ * cuda/kernels/GpuSimpleDiscretizationCuda.cpp:321-392): the relaxed momentum
+ * system of `component` from the resident U/V/W, P, FLUX and VISCOSITY. The
+ * matrix, the same for every component, is exported by cfdhip_momentum_csr;
+ * the right-hand side lands in MOM_RHS_U + component and the relaxed diagonal
+ * in MOM_DIAG. previous_component (host, n doubles, may be NULL) is the
+ * under-relaxation's phi_old; NULL takes the resident component, as
+ * assembleMomentumResident does. convection_scheme is
+ * cfd::discretization::ConvectionScheme (Convection.hpp:22-27): only 0
+ * (Upwind) is on the structured path, and non_orthogonal must be 0; anything
+ * else returns CFDHIP_ERR_UNSUPPORTED. CFDHIP_ERR_NONFINITE when the assembled
+ * system is not finite. */
+int cfdhip_assemble_momentum_component(cfdhip_ctx* ctx, int component, double alpha,
+                                       int convection_scheme, int non_orthogonal,
+                                       const double* previous_component);
 /* assembleTransientMomentumComponent (src/pressure_velocity/TransientMomentum.cpp,
  * effective-viscosity overload) for both components of a 2D case: the same
  * terms from (U, P, FLUX) plus the implicit-Euler time term
@@ -270,6 +286,34 @@ int cfdhip_update_pressure(cfdhip_ctx* ctx, double alpha);
 /* correctVelocityResident + correctFaceMassFluxResident
  * (PressureCorrectionEquation.cpp:369-438): U/V/W and FLUX. */
 int cfdhip_correct(cfdhip_ctx* ctx);
+/* correctVelocityResident / correctVelocity (GpuSimpleDiscretization.hpp:136,
+ * 252-253; correctVelocity, PressureCorrectionEquation.cpp:404-438): u = u* -
+ * d grad(p') from USTAR.., RESPONSE and PPRIME with the p' conditions (Green-
+ * Gauss). out NULL: the result replaces U/V/W (the resident form). Otherwise
+ * out (host, 3 x n doubles: x, then y, then z; z = 0 in 2D) receives it and
+ * U/V/W are left as they were. */
+int cfdhip_correct_velocity(cfdhip_ctx* ctx, double* out);
+/* correctFaceMassFluxResident / correctFaceMassFlux (GpuSimpleDiscretization.hpp:
+ * 137,257; PressureCorrectionEquation.cpp:369-402): F = F* + D_f (p'_P - p'_N)
+ * from FLUX_STAR, PC_FACE_COEFF and PPRIME. out NULL: into FLUX; otherwise into
+ * out (host, n_faces doubles) with FLUX unchanged. */
+int cfdhip_correct_flux(cfdhip_ctx* ctx, double* out);
+/* SIMPLE's finiteness guards on resident fields
+ * (residentMomentumPredictorAllFinite, residentVelocityAllFinite,
+ * residentPressureAllFinite, residentPressureCorrectionAllFinite:
+ * GpuSimpleDiscretization.hpp:143,200-201,241; SIMPLE.cpp:645-648,866-869,
+ * 1006-1012): *all_finite = 1 when no entry of the `count` listed fields
+ * (CFDHIP_FIELD_*) is a NaN or an infinity, else 0. One integer crosses the
+ * boundary. */
+int cfdhip_fields_all_finite(cfdhip_ctx* ctx, const int* fields, int count, int* all_finite);
+/* FluidProperties::density for the flux and p' stages, which the reference
+ * passes per call (computePredictedFaceFlux / assemblePressureCorrection*,
+ * GpuSimpleDiscretization.hpp:209-210,213-215,230-231); the viscosity field
+ * is left as it is. */
+int cfdhip_set_density(cfdhip_ctx* ctx, double density);
+/* residentSolveBytes (GpuSimpleDiscretization.hpp:245): device bytes held by
+ * this context's Krylov work vectors and reduction buffers. */
+int cfdhip_resident_bytes(cfdhip_ctx* ctx, int64_t* bytes);
 /* evaluateContinuity (src/physics/ContinuityEquation.cpp:14-55): IMBALANCE per
  * cell; out[0] = RMS imbalance, out[1] = global net boundary flux. */
 int cfdhip_continuity(cfdhip_ctx* ctx, double* out);

@@ -235,8 +235,23 @@ struct TransientStepResult {
   std::size_t momentumIterations{}, pressureIterations{};
 };
 
+// A system assembled on the device and brought back for a host solver: the
+// CSR arrays SparseMatrix takes (size_t indices, ascending columns, exact zeros
+// dropped, GpuSimpleDiscretizationCuda.cpp:44-57) and the right-hand side.
+struct HostSystem {
+  std::size_t rows{0};
+  std::vector<std::size_t> rowOffsets, columnIndices;
+  std::vector<double> values, rhs;
+};
+
 class GpuSimpleDiscretization {
  public:
+  // GpuSimpleDiscretization::FieldAuthority / PersistentField
+  // (include/cfd/gpu/GpuSimpleDiscretization.hpp:89-98): which side holds the
+  // authoritative copy of a persistent field. Host bookkeeping only.
+  enum class FieldAuthority { HostOnly, DeviceOwned, Synchronized };
+  enum class PersistentField { Velocity, Pressure, MassFlux, Viscosity };
+
   GpuSimpleDiscretization() = default;
   GpuSimpleDiscretization(const GpuSimpleDiscretization&) = delete;
   GpuSimpleDiscretization& operator=(const GpuSimpleDiscretization&) = delete;
@@ -245,9 +260,28 @@ class GpuSimpleDiscretization {
 
   [[nodiscard]] bool active() const noexcept { return ctx_ != nullptr; }
 
+  // prepare(): all or nothing, with the reason when the mesh or a condition
+  // is outside the structured path. This overload also sets a uniform
+  // laminar viscosity and the density.
   [[nodiscard]] bool prepare(const CartesianMesh& mesh, std::span<const PatchCondition> patches,
                              double density, double viscosity, std::string& reason) {
+    if (!prepare(mesh, patches, reason)) return false;
+    const int rc = cfdhip_set_fluid(ctx_.get(), density, viscosity);
+    if (rc != CFDHIP_OK) {
+      reason = cfdhip_last_error(ctx_.get());
+      ctx_.reset();
+      return false;
+    }
+    viscosityAuthority_ = FieldAuthority::Synchronized;
+    return true;
+  }
+  // The reference's prepare(mesh, velocityBoundaries, pressureBoundaries,
+  // reason): the fluid arrives later, the viscosity with uploadInitialState
+  // and the density with the flux and p' stages.
+  [[nodiscard]] bool prepare(const CartesianMesh& mesh, std::span<const PatchCondition> patches,
+                             std::string& reason) {
     ctx_.reset();
+    resetAuthority();
     if (!deviceAvailable()) {
       reason = "no usable gfx950 device";
       return false;
@@ -270,7 +304,6 @@ class GpuSimpleDiscretization {
       rc = cfdhip_set_boundary(ctx.get(), static_cast<int>(p), patches[p].velocityKind,
                                patches[p].velocity, patches[p].pressureKind,
                                patches[p].pressureValue);
-    if (rc == CFDHIP_OK) rc = cfdhip_set_fluid(ctx.get(), density, viscosity);
     if (rc != CFDHIP_OK) {
       reason = cfdhip_last_error(ctx.get());
       return false;
@@ -284,17 +317,115 @@ class GpuSimpleDiscretization {
 
   [[nodiscard]] std::size_t cells() const noexcept { return cells_; }
   [[nodiscard]] std::size_t faces() const noexcept { return faces_; }
+  [[nodiscard]] int dimension() const noexcept { return dim_; }
+  // The device's mesh: cell centroids (x, y, z per cell) and volumes, face
+  // owner / neighbour (-1 on a boundary face), in the reference's order.
+  void meshCells(std::span<double> centroid, std::span<double> volume) {
+    if (centroid.size() != 3 * cells_ || volume.size() != cells_)
+      throw InvalidArgumentError("meshCells: output sizes do not match the mesh");
+    detail::check(ctx(), cfdhip_mesh_cells(ctx(), centroid.data(), volume.data()));
+  }
+  void meshFaces(std::span<int64_t> owner, std::span<int64_t> neighbor) {
+    if (owner.size() != faces_ || neighbor.size() != faces_)
+      throw InvalidArgumentError("meshFaces: output sizes do not match the mesh");
+    detail::check(ctx(), cfdhip_mesh_faces(ctx(), owner.data(), neighbor.data()));
+  }
 
+  [[nodiscard]] FieldAuthority authority(PersistentField field) const noexcept {
+    switch (field) {
+      case PersistentField::Velocity: return velocityAuthority_;
+      case PersistentField::Pressure: return pressureAuthority_;
+      case PersistentField::MassFlux: return massFluxAuthority_;
+      case PersistentField::Viscosity: return viscosityAuthority_;
+    }
+    return FieldAuthority::HostOnly;
+  }
+
+  // Velocity components and pressure; the mass flux is computed on the device
+  // from the velocity (calculateMassFlux, as SIMPLE.cpp:229 does before the
+  // loop). An empty `w` leaves W at zero.
   void uploadInitialState(std::span<const double> u, std::span<const double> v,
                           std::span<const double> w, std::span<const double> p) {
     upload(CFDHIP_FIELD_U, u);
     upload(CFDHIP_FIELD_V, v);
-    if (dim_ == 3) upload(CFDHIP_FIELD_W, w);
+    if (!w.empty()) upload(CFDHIP_FIELD_W, w);
     upload(CFDHIP_FIELD_P, p);
     detail::check(ctx(), cfdhip_mass_flux(ctx()));
+    markUploaded(false);
   }
+  // uploadInitialState(velocity, pressure, massFlux, effectiveViscosity)
+  // (GpuSimpleDiscretization.hpp:104-107): the host state becomes
+  // authoritative for all four fields.
+  void uploadInitialState(std::span<const double> u, std::span<const double> v,
+                          std::span<const double> w, std::span<const double> p,
+                          std::span<const double> massFlux,
+                          std::span<const double> effectiveViscosity) {
+    upload(CFDHIP_FIELD_U, u);
+    upload(CFDHIP_FIELD_V, v);
+    if (!w.empty()) upload(CFDHIP_FIELD_W, w);
+    upload(CFDHIP_FIELD_P, p);
+    upload(CFDHIP_FIELD_FLUX, massFlux);
+    upload(CFDHIP_FIELD_VISCOSITY, effectiveViscosity);
+    markUploaded(true);
+  }
+  // beginIteration (GpuSimpleDiscretization.hpp:155-158) = uploadInitialState.
+  void beginIteration(std::span<const double> u, std::span<const double> v,
+                      std::span<const double> w, std::span<const double> p,
+                      std::span<const double> massFlux,
+                      std::span<const double> effectiveViscosity) {
+    uploadInitialState(u, v, w, p, massFlux, effectiveViscosity);
+  }
+  // beginIterationResident: no transfer; the device continues from its own
+  // corrected velocity and flux and updated pressure.
+  void beginIterationResident() {
+    if (!active()) return;
+    if (!stateUploaded_)
+      throw InvalidArgumentError(
+          "GpuSimpleDiscretization::beginIterationResident: uploadInitialState has not run, so "
+          "there is no resident state to continue from");
+  }
+  void setViscosity(std::span<const double> effectiveViscosity) {
+    if (!active()) return;
+    upload(CFDHIP_FIELD_VISCOSITY, effectiveViscosity);
+    viscosityAuthority_ = FieldAuthority::Synchronized;
+  }
+
+  // Relaxed momentum of all components in one pass (the matrix is shared).
   void assembleMomentumResident(double alpha) {
     detail::check(ctx(), cfdhip_assemble_momentum(ctx(), alpha));
+  }
+  // assembleMomentumResident(component, alpha, convectionScheme,
+  // applyNonOrthogonalCorrection): phi_old from the resident component.
+  void assembleMomentumResident(int component, double alpha, int convectionScheme,
+                                bool applyNonOrthogonalCorrection) {
+    detail::check(ctx(), cfdhip_assemble_momentum_component(ctx(), component, alpha,
+                                                            convectionScheme,
+                                                            applyNonOrthogonalCorrection ? 1 : 0,
+                                                            nullptr));
+  }
+  // assembleMomentum: the same assembly with the caller's phi_old, and the
+  // system brought back for a host solver.
+  [[nodiscard]] HostSystem assembleMomentum(int component,
+                                            std::span<const double> previousComponent,
+                                            double alpha, int convectionScheme,
+                                            bool applyNonOrthogonalCorrection) {
+    if (previousComponent.size() != cells_)
+      throw InvalidArgumentError("assembleMomentum: previousComponent size does not match the mesh");
+    detail::check(ctx(), cfdhip_assemble_momentum_component(ctx(), component, alpha,
+                                                            convectionScheme,
+                                                            applyNonOrthogonalCorrection ? 1 : 0,
+                                                            previousComponent.data()));
+    HostSystem sys = exportCsr(cfdhip_momentum_csr);
+    sys.rhs.resize(cells_);
+    download(CFDHIP_FIELD_MOM_RHS_U + component, sys.rhs);
+    return sys;
+  }
+  // setMomentumSolution: a host solver's u* back into the device predictor.
+  void setMomentumSolution(int component, std::span<const double> solution) {
+    if (!active()) return;
+    if (component < 0 || component >= dim_)
+      throw InvalidArgumentError("setMomentumSolution: component out of range");
+    upload(CFDHIP_FIELD_USTAR + component, solution);
   }
   [[nodiscard]] SolverResult solveMomentumResident(int component, const LinearSolverSettings& s) {
     const cfdhip_solver_settings c = s.c();
@@ -302,12 +433,66 @@ class GpuSimpleDiscretization {
     detail::check(ctx(), cfdhip_solve_momentum(ctx(), component, &c, &r));
     return detail::fromC(r, ctx());  // empty solution: it stays resident
   }
+  [[nodiscard]] bool residentMomentumPredictorAllFinite(bool threeDimensional) {
+    const int f[3] = {CFDHIP_FIELD_USTAR, CFDHIP_FIELD_VSTAR, CFDHIP_FIELD_WSTAR};
+    return allFinite(f, threeDimensional ? 3 : 2);
+  }
+  // U, V and W whatever the dimension (W holds zeros in 2D), as the host
+  // allFinite(VectorField) tests all three.
+  [[nodiscard]] bool residentVelocityAllFinite() {
+    const int f[3] = {CFDHIP_FIELD_U, CFDHIP_FIELD_V, CFDHIP_FIELD_W};
+    return allFinite(f, 3);
+  }
+  [[nodiscard]] bool residentPressureAllFinite() {
+    if (!active()) return true;
+    const int f[1] = {CFDHIP_FIELD_P};
+    return allFinite(f, 1);
+  }
+  [[nodiscard]] bool residentPressureCorrectionAllFinite() {
+    const int f[1] = {CFDHIP_FIELD_PPRIME};
+    return allFinite(f, 1);
+  }
+
   void computeResponseCoefficients() { detail::check(ctx(), cfdhip_response(ctx())); }
+  void computeResponseCoefficients(bool /*threeDimensional: the mesh's own dimension rules*/) {
+    computeResponseCoefficients();
+  }
   void computePredictedFaceFlux(bool rhieChow, double alpha) {
     detail::check(ctx(), cfdhip_predicted_flux(ctx(), rhieChow ? 1 : 0, alpha));
   }
+  // computePredictedFaceFlux(rhieChow, density, alpha, gradientScheme, 3D):
+  // the Rhie-Chow pressure gradient is Green-Gauss, as the reference's device
+  // path computes it whatever gradientScheme says
+  // (GpuSimpleDiscretizationCuda.cpp:493-499).
+  void computePredictedFaceFlux(bool rhieChow, double density, double alpha,
+                                int /*gradientScheme*/, bool /*threeDimensional*/) {
+    detail::check(ctx(), cfdhip_set_density(ctx(), density));
+    computePredictedFaceFlux(rhieChow, alpha);
+  }
+
   void assemblePressureCorrectionResident(std::size_t referenceCell) {
     detail::check(ctx(), cfdhip_assemble_pressure(ctx(), static_cast<int64_t>(referenceCell)));
+  }
+  void assemblePressureCorrectionResident(bool nonOrthogonal, std::size_t referenceCell,
+                                          double density, bool /*threeDimensional*/) {
+    requireOrthogonal(nonOrthogonal, "assemblePressureCorrectionResident");
+    detail::check(ctx(), cfdhip_set_density(ctx(), density));
+    assemblePressureCorrectionResident(referenceCell);
+  }
+  // assemblePressureCorrection: pass 1 only (previousPressureCorrection must be
+  // empty; SIMPLE keeps corrector passes on the CPU), brought back to the host.
+  [[nodiscard]] HostSystem assemblePressureCorrection(
+      bool nonOrthogonal, std::size_t referenceCell, double density,
+      std::span<const double> previousPressureCorrection, bool threeDimensional) {
+    if (!previousPressureCorrection.empty())
+      throw InvalidArgumentError(
+          "assemblePressureCorrection: corrector passes (a previous p') are not on the "
+          "structured path");
+    assemblePressureCorrectionResident(nonOrthogonal, referenceCell, density, threeDimensional);
+    HostSystem sys = exportCsr(cfdhip_pressure_csr);
+    sys.rhs.resize(cells_);
+    download(CFDHIP_FIELD_PC_RHS, sys.rhs);
+    return sys;
   }
   // assembleCompressiblePressureCorrection (2D): per-face density (faces),
   // absolute pressure and temperature (cells) of an ideal gas with constant R.
@@ -331,7 +516,20 @@ class GpuSimpleDiscretization {
     detail::check(ctx(), cfdhip_solve_pressure(ctx(), &c, &r));
     return detail::fromC(r, ctx());
   }
-  void updatePressure(double alpha) { detail::check(ctx(), cfdhip_update_pressure(ctx(), alpha)); }
+  // setPressureCorrection: a host solver's p' back to the device.
+  void setPressureCorrection(std::span<const double> pressureCorrection) {
+    if (!active()) return;
+    upload(CFDHIP_FIELD_PPRIME, pressureCorrection);
+  }
+  [[nodiscard]] std::size_t residentSolveBytes() const noexcept {
+    if (!ctx_) return 0;
+    int64_t b = 0;
+    return cfdhip_resident_bytes(ctx_.get(), &b) == CFDHIP_OK ? static_cast<std::size_t>(b) : 0;
+  }
+  void updatePressure(double alpha) {
+    detail::check(ctx(), cfdhip_update_pressure(ctx(), alpha));
+    pressureAuthority_ = FieldAuthority::DeviceOwned;
+  }
   // PISO::solveTimeStep (include/cfd/pressure_velocity/PISO.hpp:89-90) on the
   // resident state (U, V, P, FLUX), 2D; the solvers' types are honoured (the
   // reference always uses BiCGSTAB). A failed step leaves the previous state.
@@ -353,12 +551,60 @@ class GpuSimpleDiscretization {
     r.pressureIterations = static_cast<std::size_t>(meta[5]);
     return r;
   }
-  void correctVelocityAndFluxResident() { detail::check(ctx(), cfdhip_correct(ctx())); }
+  // correctVelocityResident / correctFaceMassFluxResident: the corrections
+  // replace the persistent velocity / mass flux on the device. Green-Gauss is
+  // the structured path's gradient; the explicit non-orthogonal flux term
+  // belongs to corrector passes, which SIMPLE keeps on the CPU.
+  void correctVelocityResident(int gradientScheme, bool /*threeDimensional*/) {
+    requireGreenGauss(gradientScheme, "correctVelocityResident");
+    detail::check(ctx(), cfdhip_correct_velocity(ctx(), nullptr));
+    velocityAuthority_ = FieldAuthority::DeviceOwned;
+  }
+  void correctFaceMassFluxResident(bool withExplicitTerm) {
+    requireOrthogonal(withExplicitTerm, "correctFaceMassFluxResident");
+    detail::check(ctx(), cfdhip_correct_flux(ctx(), nullptr));
+    massFluxAuthority_ = FieldAuthority::DeviceOwned;
+  }
+  void correctVelocityAndFluxResident() {
+    detail::check(ctx(), cfdhip_correct(ctx()));
+    velocityAuthority_ = massFluxAuthority_ = FieldAuthority::DeviceOwned;
+  }
+  // correctVelocity / correctFaceMassFlux: the corrected fields come back to
+  // the host (velocity: x, then y, then z, each of cells()); the persistent
+  // fields are left as they were.
+  void correctVelocity(int gradientScheme, bool /*threeDimensional*/, std::span<double> corrected) {
+    requireGreenGauss(gradientScheme, "correctVelocity");
+    if (corrected.size() != 3 * cells_)
+      throw InvalidArgumentError("correctVelocity: output must hold 3 x cells values");
+    detail::check(ctx(), cfdhip_correct_velocity(ctx(), corrected.data()));
+  }
+  void correctFaceMassFlux(bool withExplicitTerm, std::span<double> corrected) {
+    requireOrthogonal(withExplicitTerm, "correctFaceMassFlux");
+    if (corrected.size() != faces_)
+      throw InvalidArgumentError("correctFaceMassFlux: output must hold one value per face");
+    detail::check(ctx(), cfdhip_correct_flux(ctx(), corrected.data()));
+  }
   // out[0] = RMS imbalance, out[1] = global net boundary flux
   std::pair<double, double> continuity() {
     double out[2] = {0.0, 0.0};
     detail::check(ctx(), cfdhip_continuity(ctx(), out));
     return {out[0], out[1]};
+  }
+  // downloadVelocity / downloadPressure / downloadMassFlux: the host copy
+  // becomes current (Synchronized).
+  void downloadVelocity(std::span<double> u, std::span<double> v, std::span<double> w) {
+    download(CFDHIP_FIELD_U, u);
+    download(CFDHIP_FIELD_V, v);
+    download(CFDHIP_FIELD_W, w);
+    velocityAuthority_ = FieldAuthority::Synchronized;
+  }
+  void downloadPressure(std::span<double> p) {
+    download(CFDHIP_FIELD_P, p);
+    pressureAuthority_ = FieldAuthority::Synchronized;
+  }
+  void downloadMassFlux(std::span<double> flux) {
+    download(CFDHIP_FIELD_FLUX, flux);
+    massFluxAuthority_ = FieldAuthority::Synchronized;
   }
   void download(int field, std::span<double> out) {
     detail::check(ctx(), cfdhip_field_download(ctx(), field, out.data(),
@@ -374,9 +620,55 @@ class GpuSimpleDiscretization {
     if (!ctx_) throw InvalidArgumentError("GpuSimpleDiscretization: prepare() first");
     return ctx_.get();
   }
+  bool allFinite(const int* fields, int count) {
+    int ok = 0;
+    detail::check(ctx(), cfdhip_fields_all_finite(ctx(), fields, count, &ok));
+    return ok != 0;
+  }
+  static void requireGreenGauss(int gradientScheme, const char* what) {
+    if (gradientScheme != 0)  // cfd::discretization::GradientScheme::GreenGauss
+      throw InvalidArgumentError(std::string(what) +
+                                 ": the structured path's gradient is Green-Gauss");
+  }
+  static void requireOrthogonal(bool nonOrthogonal, const char* what) {
+    if (nonOrthogonal)
+      throw InvalidArgumentError(std::string(what) +
+                                 ": non-orthogonal correction is not on the structured path");
+  }
+  template <class Export>
+  HostSystem exportCsr(Export fn) {
+    HostSystem sys;
+    sys.rows = cells_;
+    sys.rowOffsets.resize(cells_ + 1);
+    const std::size_t cap = (2 * static_cast<std::size_t>(dim_) + 1) * cells_;
+    std::vector<int64_t> col(cap);
+    sys.values.resize(cap);
+    int64_t nnz = 0;
+    static_assert(sizeof(std::size_t) == sizeof(int64_t));
+    detail::check(ctx(), fn(ctx(), reinterpret_cast<int64_t*>(sys.rowOffsets.data()), col.data(),
+                            sys.values.data(), static_cast<int64_t>(cap), &nnz));
+    sys.values.resize(static_cast<std::size_t>(nnz));
+    sys.columnIndices.assign(col.begin(), col.begin() + nnz);
+    return sys;
+  }
+  void markUploaded(bool viscosity) {
+    velocityAuthority_ = pressureAuthority_ = massFluxAuthority_ = FieldAuthority::Synchronized;
+    if (viscosity) viscosityAuthority_ = FieldAuthority::Synchronized;
+    stateUploaded_ = true;
+  }
+  void resetAuthority() {
+    velocityAuthority_ = pressureAuthority_ = massFluxAuthority_ = viscosityAuthority_ =
+        FieldAuthority::HostOnly;
+    stateUploaded_ = false;
+  }
   detail::CtxPtr ctx_;
   std::size_t cells_{0}, faces_{0};
   int dim_{2};
+  FieldAuthority velocityAuthority_{FieldAuthority::HostOnly};
+  FieldAuthority pressureAuthority_{FieldAuthority::HostOnly};
+  FieldAuthority massFluxAuthority_{FieldAuthority::HostOnly};
+  FieldAuthority viscosityAuthority_{FieldAuthority::HostOnly};
+  bool stateUploaded_{false};
 };
 
 }  // namespace cfdhip
