@@ -764,12 +764,14 @@ SIDES = {"left": 1, "right": 2, "bottom": 4, "top": 8}
 
 def _unet_options(opts) -> "_lib.UnetOptions":
     """cfdhip_unet_options from a dict: padding "zero" / "neumann", an
-    optional tail {levels, omega, a, k}, and dirichlet_sides (names of the
-    sides with odd reflection: left, right, bottom, top)."""
+    optional tail {levels, omega, a, k}, dirichlet_sides (names of the
+    sides with odd reflection: left, right, bottom, top) and dense (True: the
+    MFMA layers over every channel, never the narrow kernels)."""
     o = _lib.UnetOptions()
     opts = opts or {}
     o.padding = 1 if opts.get("padding", "zero") == "neumann" else 0
     o.dirichlet_sides = sum(SIDES[s] for s in (opts.get("dirichlet_sides") or ()))
+    o.dense = 1 if opts.get("dense") else 0
     tail = opts.get("tail") or {}
     o.tail_levels = int(tail.get("levels", 0))
     o.tail_omega = float(tail.get("omega", 0.0))

@@ -67,7 +67,8 @@ class UnetOptions(ctypes.Structure):
                 ("tail_omega", ctypes.c_float),
                 ("tail_a", ctypes.c_float * 3),
                 ("tail_k", ctypes.c_float * 3),
-                ("dirichlet_sides", ctypes.c_int32)]
+                ("dirichlet_sides", ctypes.c_int32),
+                ("dense", ctypes.c_int32)]
 
 
 # Every exported symbol with its signature; the not-gpu tests check that the

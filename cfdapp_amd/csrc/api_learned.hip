@@ -108,6 +108,7 @@ int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* o) {
   return guardedCtx(ctx, [&](Context& c) {
     require(o != nullptr, "options are required");
     if (!c.net) c.net = std::make_unique<Unet>(c.stream);
+    c.net->setDense(o->dense != 0);
     c.net->setOptions(o->padding, 0, tailFrom(*o));
   });
 }
@@ -183,6 +184,7 @@ int cfdhip_unet_set_weights(cfdhip_unet* h, const float* weights, int64_t count)
 int cfdhip_unet_set_options(cfdhip_unet* h, const cfdhip_unet_options* o) {
   return guardedNet(h, [&](Unet& net) {
     require(o != nullptr, "options are required");
+    net.setDense(o->dense != 0);
     net.setOptions(o->padding, o->dirichlet_sides, tailFrom(*o));
   });
 }

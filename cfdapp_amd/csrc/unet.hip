@@ -453,6 +453,8 @@ void Unet::setWeights(const float* host) {
   CFDHIP_CHECK(hipMemcpyAsync(master.get(), host, kUnetParams * sizeof(float),
                               hipMemcpyHostToDevice, stream));
   repack();
+  hostMaster.assign(host, host + kUnetParams);
+  buildNarrow();
   loaded = true;
 }
 
@@ -645,6 +647,12 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
   physW = std::min(srcW, w);
   physLevelH = h;
   physLevelW = w;
+  lastNarrow = false;
+  if (fuseFirst && narrowActive()) {
+    // inference with unused channels dropped (unet_narrow.hip)
+    forwardNarrow(src, srcDouble, scale, srcH, srcW, srcBatch, n, h, w);
+    return;
+  }
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   if (fuseFirst) {
     // inference: e0a is computed inside e0b's tile staging and never stored
@@ -687,6 +695,10 @@ void Unet::forwardFrom(const void* src, bool srcDouble, const float* scale, int 
 void Unet::outputTrain(int n, int h, int w, int ph, int pw) {
   if (ph <= 0) ph = h;
   if (pw <= 0) pw = w;
+  if (lastNarrow) {
+    outputNarrow(n, h, w, ph, pw);
+    return;
+  }
   const int64_t p0 = static_cast<int64_t>(n) * h * w;
   hipLaunchKernelGGL(lastConvKernel, grid1(p0 / 2, 4096), dim3(256), 0, stream, T(kTd0),
                      master.get() + layerOffset(kOut), y.get(), n, h, w, ph, pw, padBits());
@@ -697,9 +709,12 @@ template <class TZ>
 void Unet::outputApply(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
                        int64_t pin, int h, int w, double* partial, int parts) {
   mark(kOut, true);
-  hipLaunchKernelGGL((lastConvApplyKernel<TZ>), dim3(static_cast<unsigned>(parts)), dim3(256), 0,
-                     stream, T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx, pin,
-                     h, w, padBits(), partial);
+  if (lastNarrow)
+    outputApplyNarrow<TZ>(scale, r, q, z, ny, nx, pin, h, w, partial, parts);
+  else
+    hipLaunchKernelGGL((lastConvApplyKernel<TZ>), dim3(static_cast<unsigned>(parts)), dim3(256), 0,
+                       stream, T(kTd0), master.get() + layerOffset(kOut), scale, r, q, z, ny, nx,
+                       pin, h, w, padBits(), partial);
   mark(kOut, false);
   CFDHIP_CHECK(hipGetLastError());
   collectTiming();

@@ -1,0 +1,612 @@
+// Channel-compacted inference of the learned preconditioner.
+//
+// A network whose weights leave most channels unused (the shipped weights
+// are a multigrid V-cycle written as a U-Net: at most two live channels per
+// tensor) wastes nearly all of the MFMA layers' HBM traffic on zeros: a
+// 16-channel FP32 pixel is a whole 64-byte line, so skipping zero channels
+// inside the full-width layout saves no bytes. buildNarrow() therefore finds,
+// from the weights and the tail options, the channels of every tensor that
+// some output depends on, and when no tensor keeps more than kNarrowMax of
+// them, inference stores only those (2 or 4 channels per pixel) and
+// runs these kernels:
+//  * narrowConvKernel: a 3x3 convolution over a 16 x 64 output tile, input
+//    staged once in LDS with its one-pixel halo (plain, the nearest-upsampled
+//    concatenation of the decoder, or the fused first layer computed from
+//    the scaled solver residual), FP32 activations, BF16-rounded weights,
+//    FP32 accumulation; one thread computes a 2 x 2 pixel block, so the
+//    encoder's 2 x 2 average pool is thread-local in the epilogue;
+//  * narrowLastKernel: the output layer, on the solver grid, with the
+//    preconditioner's scaling, pinned row and the (z, q) block partials.
+// Padding, physical extents, pooling at odd edges and the coarse tail are the
+// MFMA path's (unet_conv.hip, unet_tail.hip); the two paths compute the same
+// linear map and differ only in rounding (the MFMA layers see activations
+// as a hi + lo BF16 pair, these kernels in FP32).
+#include <algorithm>
+#include <cstring>
+
+#include "reduce.hpp"
+#include "unet_net.hpp"
+
+namespace cfdhip {
+
+namespace {
+
+constexpr int kNTH = 16, kNTW = 64;                  // output tile
+constexpr int kNHR = kNTH + 2, kNHC = kNTW + 2;      // its input with the halo
+constexpr int kNHP = kNHR * kNHC;
+constexpr int kNXR = kNTH + 4, kNXC = kNTW + 4;      // the fused first layer's scalar input
+constexpr int kNBX = kNTW / 2;                       // 2 x 2 blocks per tile row
+
+enum NarrowGather : int { kNPlain = 0, kNConcat = 1, kNFirst = 2 };
+
+struct NarrowArgs {
+  const float* a;   // plain: the input (h x w); concat: the coarse half (h/2 x w/2)
+  const float* b;   // concat: the skip half (h x w)
+  const float* w;   // this layer's weights [tap][slot in][slot out]
+  const float* w0;  // first layer: e0a's weights [tap][slot]
+  float* out;       // n x h x w x CO
+  float* aux;       // pool: n x h/2 x w/2 x CO
+  int n, h, w_, ph, pw, pad;
+  int tilesX, tilesY;
+  // first layer: x = src * scale on srcH x srcW (row stride srcW)
+  const void* src;
+  const float* scale;
+  int srcH, srcW;
+  int64_t srcBatch;
+};
+
+template <int CW>
+struct Vec;
+template <>
+struct Vec<2> {
+  using T = float2;
+  __device__ static inline void load(const float* p, float* v) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    v[0] = t.x;
+    v[1] = t.y;
+  }
+  __device__ static inline void store(float* p, const float* v) {
+    *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+  }
+};
+template <>
+struct Vec<4> {
+  using T = float4;
+  __device__ static inline void load(const float* p, float* v) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+  }
+  __device__ static inline void store(float* p, const float* v) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  }
+};
+
+// CA: stored channels of the input (concat: of its coarse half), CB: of the
+// skip half (concat only), CO: of the output. G = kNFirst: the input is e0a
+// (CA channels) computed in LDS from the scalar source.
+// padSign (unet.hpp) without branches: (y, x) moves into the h x w image
+// and the factor is returned (0 outside under zero padding), so a thread can
+// issue all its loads before it waits for the first.
+__device__ inline float padFactor(int& y, int& x, int h, int w, int pad) {
+  const bool below = y < 0, above = y >= h, left = x < 0, right = x >= w;
+  float s = 1.f;
+  if (pad & kPadNeumann) {
+    s = (below && (pad & kPadBottom)) ? -s : s;
+    s = (above && (pad & kPadTop)) ? -s : s;
+    s = (left && (pad & kPadLeft)) ? -s : s;
+    s = (right && (pad & kPadRight)) ? -s : s;
+  } else {
+    s = (below || above || left || right) ? 0.f : 1.f;
+  }
+  y = min(max(y, 0), h - 1);
+  x = min(max(x, 0), w - 1);
+  return s;
+}
+
+// s * v with the zero of zero padding exact (a non-finite value read from the
+// clamped pixel stays out of it)
+__device__ inline float signed_(float s, float v) { return s == 0.f ? 0.f : s * v; }
+
+template <int CA, int CB, int CO, int G, bool POOL, class TS>
+__global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
+  constexpr int CI = G == kNConcat ? CA + CB : CA;
+  __shared__ float tile[kNHP * CI];
+  __shared__ float ws[9 * CI * CO];
+  __shared__ float w0s[9 * CA];
+  __shared__ float xs[G == kNFirst ? kNXR * kNXC : 1];
+  for (int i = threadIdx.x; i < 9 * CI * CO; i += blockDim.x) ws[i] = a.w[i];
+  if (G == kNFirst)
+    for (int i = threadIdx.x; i < 9 * CA; i += blockDim.x) w0s[i] = a.w0[i];
+  const int h = a.h, wd = a.w_;
+  const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
+  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int x0 = static_cast<int>(t % a.tilesX) * kNTW;
+    const int64_t rest = t / a.tilesX;
+    const int y0 = static_cast<int>(rest % a.tilesY) * kNTH;
+    const int b = static_cast<int>(rest / a.tilesY);
+    __syncthreads();  // the weights are in; the previous tile's readers are done
+    if constexpr (G == kNFirst) {
+      // x = src * scale on rows y0 - 2 .. y0 + kNTH + 1, padded at the physical image
+      const TS* src = static_cast<const TS*>(a.src);
+      const int sh = a.srcH < h ? a.srcH : h, sw = a.srcW < wd ? a.srcW : wd;
+      constexpr int XPER = (kNXR * kNXC + 255) / 256;
+      float xv[XPER], xsg[XPER];
+#pragma unroll
+      for (int k = 0; k < XPER; ++k) {
+        const int i = threadIdx.x + k * 256;
+        int yy = y0 - 2 + i / kNXC, xx = x0 - 2 + i % kNXC;
+        xsg[k] = padFactor(yy, xx, sh, sw, a.pad);
+        xv[k] = 0.f;
+        if (i < kNXR * kNXC) {
+          const int64_t si = b * a.srcBatch + static_cast<int64_t>(yy) * a.srcW + xx;
+          const float sv = static_cast<float>(src[si]);
+          xv[k] = a.scale != nullptr ? sv * a.scale[si] : sv;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < XPER; ++k) {
+        const int i = threadIdx.x + k * 256;
+        if (i < kNXR * kNXC) xs[i] = signed_(xsg[k], xv[k]);
+      }
+      __syncthreads();
+      // e0a on the halo tile; a halo pixel outside the image takes e0a of the
+      // nearest pixel inside times the padding's sign (its taps lie inside xs:
+      // the tile starts inside the physical image, whose rounded-up tensor
+      // extends it by at most 7 pixels)
+      for (int i = threadIdx.x; i < kNHP; i += blockDim.x) {
+        int yy = y0 - 1 + i / kNHC, xx = x0 - 1 + i % kNHC;
+        float acc[CA];
+#pragma unroll
+        for (int c = 0; c < CA; ++c) acc[c] = 0.f;
+        const float sg = padSign(yy, xx, a.ph, a.pw, a.pad);
+        if (sg != 0.f) {
+          const int ry = yy - y0 + 1, rx = xx - x0 + 1;
+#pragma unroll
+          for (int tap = 0; tap < 9; ++tap) {
+            const float v = xs[(ry + tap / 3) * kNXC + rx + tap % 3];
+#pragma unroll
+            for (int c = 0; c < CA; ++c) acc[c] += v * w0s[tap * CA + c];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CA; ++c) tile[i * CI + c] = sg * acc[c];
+      }
+    } else {
+      constexpr int PER = (kNHP + 255) / 256;
+      float v[PER][CI], sg[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + k * 256;
+        int yy = y0 - 1 + i / kNHC, xx = x0 - 1 + i % kNHC;
+        sg[k] = padFactor(yy, xx, a.ph, a.pw, a.pad);
+#pragma unroll
+        for (int c = 0; c < CI; ++c) v[k][c] = 0.f;
+        if (i < kNHP) {
+          if constexpr (G == kNConcat) {
+            const int hh = h >> 1, hw = wd >> 1;
+            Vec<CA>::load(a.a + ((static_cast<int64_t>(b) * hh + (yy >> 1)) * hw + (xx >> 1)) * CA,
+                          v[k]);
+            Vec<CB>::load(a.b + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * CB, v[k] + CA);
+          } else {
+            Vec<CA>::load(a.a + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * CA, v[k]);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + k * 256;
+        if (i < kNHP) {
+#pragma unroll
+          for (int c = 0; c < CI; ++c) tile[i * CI + c] = signed_(sg[k], v[k][c]);
+        }
+      }
+    }
+    __syncthreads();
+    // this thread's 2 x 2 output block, tap by tap (one tap's weights live at a time)
+    const int bx = threadIdx.x % kNBX, by = threadIdx.x / kNBX;
+    float acc[4][CO];
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+#pragma unroll
+      for (int c = 0; c < CO; ++c) acc[o][c] = 0.f;
+    // not unrolled: an unrolled tap loop keeps every tap's weights in registers
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap % 3;
+      float wv[CI * CO];
+#pragma unroll
+      for (int i = 0; i < CI * CO; ++i) wv[i] = ws[tap * CI * CO + i];
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        const float* px = tile + ((2 * by + (o >> 1) + ky) * kNHC + 2 * bx + (o & 1) + kx) * CI;
+        float v[CI];
+#pragma unroll
+        for (int c = 0; c < CI; ++c) v[c] = px[c];
+#pragma unroll
+        for (int ci = 0; ci < CI; ++ci)
+#pragma unroll
+          for (int co = 0; co < CO; ++co) acc[o][co] += v[ci] * wv[ci * CO + co];
+      }
+    }
+    const int yb = y0 + 2 * by, xb = x0 + 2 * bx;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx)
+        if (yb + dy < h && xb + dx < wd)
+          Vec<CO>::store(a.out + ((static_cast<int64_t>(b) * h + yb + dy) * wd + xb + dx) * CO,
+                         acc[dy * 2 + dx]);
+    if constexpr (POOL) {
+      // pixels beyond an odd physical edge add nothing (as the MFMA epilogue)
+      const bool rowOut = yb + 1 >= a.ph, colOut = xb + 1 >= a.pw;
+      const int hh = h >> 1, hw = wd >> 1, py = yb >> 1, px = xb >> 1;
+      float s[CO];
+#pragma unroll
+      for (int c = 0; c < CO; ++c) {
+        const float v1 = colOut ? 0.f : acc[1][c];
+        const float v2 = rowOut ? 0.f : acc[2][c];
+        const float v3 = (rowOut || colOut) ? 0.f : acc[3][c];
+        s[c] = ((acc[0][c] + v1) + (v2 + v3)) * 0.25f;
+      }
+      if (py < hh && px < hw)
+        Vec<CO>::store(a.aux + ((static_cast<int64_t>(b) * hh + py) * hw + px) * CO, s);
+    }
+  }
+}
+
+// The output layer y = conv(d0, CW -> 1) at the n x ph x pw physical pixels
+// (CW: the stored channels of d0).
+// APPLY 0: y (compact n x ph x pw). APPLY 1 (n = 1, the solver grid): z =
+// scale * y, z[pin] = r[pin] scale[pin]^2 and block partials of (z, q).
+template <int CW, int APPLY, class TZ>
+__global__ __launch_bounds__(256) void narrowLastKernel(const float* __restrict__ d0,
+                                                        const float* __restrict__ w, int n, int h,
+                                                        int wd, int ph, int pw, int pad,
+                                                        float* __restrict__ y,
+                                                        const float* __restrict__ scale,
+                                                        const TZ* __restrict__ r,
+                                                        const TZ* __restrict__ q,
+                                                        TZ* __restrict__ z, int64_t pin,
+                                                        double* partial) {
+  __shared__ float ws[9 * CW];
+  if (threadIdx.x < 9 * CW) ws[threadIdx.x] = w[threadIdx.x];
+  __syncthreads();
+  double acc[1] = {0.0};
+  // a block walks whole rows (one division per row, none per pixel)
+  const int rows = n * ph;
+  for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int b = row / ph, yy = row - b * ph;
+    for (int xx = threadIdx.x; xx < pw; xx += blockDim.x) {
+      const int64_t p = static_cast<int64_t>(row) * pw + xx;
+      float v[9][CW], sg[9];
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        int ty = yy + tap / 3 - 1, tx = xx + tap % 3 - 1;
+        sg[tap] = padFactor(ty, tx, ph, pw, pad);
+        Vec<CW>::load(d0 + ((static_cast<int64_t>(b) * h + ty) * wd + tx) * CW, v[tap]);
+      }
+      float o = 0.f;
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        float dot = 0.f;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) dot += signed_(sg[tap], v[tap][c]) * ws[tap * CW + c];
+        o += dot;
+      }
+      if constexpr (APPLY == 0) {
+        y[p] = o;
+      } else {
+        TZ v = static_cast<TZ>(o * scale[p]);
+        if (p == pin) v = r[p] * static_cast<TZ>(scale[p]) * static_cast<TZ>(scale[p]);
+        z[p] = v;
+        if (q != nullptr) acc[0] += static_cast<double>(v) * static_cast<double>(q[p]);
+      }
+    }
+  }
+  if constexpr (APPLY == 1) {
+    if (partial != nullptr) blockPartials<1>(acc, partial);
+  }
+}
+
+// BF16 round to nearest even, as the device's f2bf (NaN stays NaN).
+float roundBfHost(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof(u));
+  if ((u & 0x7f800000u) == 0x7f800000u) {
+    u = ((u >> 16) | ((u & 0xffffu) != 0 ? 0x40u : 0u)) << 16;
+  } else {
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+  }
+  float r;
+  std::memcpy(&r, &u, sizeof(r));
+  return r;
+}
+
+// channels of each forward tensor in the full network
+constexpr int kTensorChannels[kNumForward] = {16, 16, 16, 32, 32, 32, 64, 64, 64,
+                                              64, 64, 64, 64, 32, 32, 16, 16};
+
+template <int CA, int CB, int CO>
+void launchCombo(hipStream_t s, int gather, bool pool, bool srcDouble, dim3 grid,
+                 const NarrowArgs& a) {
+  if (gather == kNFirst) {
+    if (srcDouble)
+      hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNFirst, true, double>), grid, dim3(256), 0,
+                         s, a);
+    else
+      hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNFirst, true, float>), grid, dim3(256), 0,
+                         s, a);
+  } else if (gather == kNConcat) {
+    hipLaunchKernelGGL((narrowConvKernel<CA, CB, CO, kNConcat, false, float>), grid, dim3(256), 0,
+                       s, a);
+  } else if (pool) {
+    hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNPlain, true, float>), grid, dim3(256), 0, s,
+                       a);
+  } else {
+    hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNPlain, false, float>), grid, dim3(256), 0, s,
+                       a);
+  }
+}
+
+// ca, cb, co: stored widths (2 or 4; cb only for the concatenation)
+void launchNarrowConv(hipStream_t s, int ca, int cb, int co, int gather, bool pool,
+                      bool srcDouble, const NarrowArgs& a) {
+  const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
+  const dim3 grid(static_cast<unsigned>(std::min<int64_t>(tiles, 1 << 20)));
+  const int key = (ca == 4 ? 4 : 0) | (cb == 4 ? 2 : 0) | (co == 4 ? 1 : 0);
+  switch (key) {
+    case 0: launchCombo<2, 2, 2>(s, gather, pool, srcDouble, grid, a); break;
+    case 1: launchCombo<2, 2, 4>(s, gather, pool, srcDouble, grid, a); break;
+    case 2: launchCombo<2, 4, 2>(s, gather, pool, srcDouble, grid, a); break;
+    case 3: launchCombo<2, 4, 4>(s, gather, pool, srcDouble, grid, a); break;
+    case 4: launchCombo<4, 2, 2>(s, gather, pool, srcDouble, grid, a); break;
+    case 5: launchCombo<4, 2, 4>(s, gather, pool, srcDouble, grid, a); break;
+    case 6: launchCombo<4, 4, 2>(s, gather, pool, srcDouble, grid, a); break;
+    default: launchCombo<4, 4, 4>(s, gather, pool, srcDouble, grid, a); break;
+  }
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void Unet::setDense(bool dense) { dense_ = dense; }
+
+void Unet::levelExtent(int h, int w, int& ph, int& pw) const {
+  ph = physLevelH > 0 ? physH : h;
+  pw = physLevelW > 0 ? physW : w;
+  for (int hh = physLevelH; hh > h; hh >>= 1) ph = (ph + 1) >> 1;
+  for (int ww = physLevelW; ww > w; ww >>= 1) pw = (pw + 1) >> 1;
+}
+
+void Unet::buildNarrow() {
+  narrow = NarrowPlan{};
+  if (static_cast<int64_t>(hostMaster.size()) != kUnetParams) return;
+  auto W = [&](int l, int tap, int ci, int co) {
+    return hostMaster[layerOffset(l) +
+                      (static_cast<int64_t>(tap) * kLayers[l].cin + ci) * kLayers[l].cout + co];
+  };
+  std::vector<std::vector<char>> need(kNumForward);
+  for (int t = 0; t < kNumForward; ++t) need[t].assign(kTensorChannels[t], 0);
+  // layer l produces the channels `wanted` of its output from input A (and,
+  // concatenated after A's ca channels, input B): mark what it reads
+  auto back = [&](int l, const std::vector<char>& wanted, int A, int B, int ca) {
+    for (int co = 0; co < kLayers[l].cout; ++co) {
+      if (!wanted[co]) continue;
+      for (int tap = 0; tap < 9; ++tap)
+        for (int ci = 0; ci < kLayers[l].cin; ++ci) {
+          if (W(l, tap, ci, co) == 0.f) continue;
+          if (B < 0 || ci < ca) need[A][ci] = 1;
+          else need[B][ci - ca] = 1;
+        }
+    }
+  };
+  const std::vector<char> yWanted(1, 1);
+  back(kOut, yWanted, kTd0, -1, 0);
+  back(kD0b, need[kTd0], kTd0a, -1, 0);
+  back(kD0a, need[kTd0a], kTd1, kTe0, 32);
+  back(kD1b, need[kTd1], kTd1a, -1, 0);
+  back(kD1a, need[kTd1a], kTd2, kTe1, 64);
+  back(kD2b, need[kTd2], kTd2a, -1, 0);
+  back(kD2a, need[kTd2a], kTb3, kTe2, 64);
+  back(kB3b, need[kTb3], kTb3a, -1, 0);
+  // the tail overwrites its destination channel from its source channel
+  const bool tailLive = tail.levels > 0 && need[kTb3a][tail.dst];
+  std::vector<char> convWanted = need[kTb3a];
+  if (tailLive) {
+    need[kTb3a][tail.src] = 1;
+    convWanted = need[kTb3a];
+    convWanted[tail.dst] = 0;
+  }
+  back(kB3a, convWanted, kTp2, -1, 0);
+  // a pooled tensor keeps its source's channels
+  auto pooled = [&](int e, int p) {
+    for (int c = 0; c < kTensorChannels[e]; ++c) need[e][c] = need[e][c] || need[p][c];
+    need[p] = need[e];
+  };
+  pooled(kTe2, kTp2);
+  back(kE2b, need[kTe2], kTe2a, -1, 0);
+  back(kE2a, need[kTe2a], kTp1, -1, 0);
+  pooled(kTe1, kTp1);
+  back(kE1b, need[kTe1], kTe1a, -1, 0);
+  back(kE1a, need[kTe1a], kTp0, -1, 0);
+  pooled(kTe0, kTp0);
+  back(kE0b, need[kTe0], kTe0a, -1, 0);
+  for (int t = 0; t < kNumForward; ++t) {
+    int k = 0;
+    for (int c = 0; c < kTensorChannels[t]; ++c)
+      if (need[t][c]) {
+        if (k == kNarrowMax) return;  // too many live channels: the MFMA layers
+        narrow.chan[t][k++] = c;
+      }
+    if (k == 0) narrow.chan[t][k++] = 0;  // nothing reads it: one placeholder slot
+    narrow.width[t] = k;
+    narrow.sw[t] = k <= 2 ? 2 : 4;
+  }
+  if (tailLive) {
+    for (int k = 0; k < narrow.width[kTb3a]; ++k) {
+      if (narrow.chan[kTb3a][k] == tail.src) narrow.tailSrc = k;
+      if (narrow.chan[kTb3a][k] == tail.dst) narrow.tailDst = k;
+    }
+  }
+  // compacted weights, BF16-rounded, [tap][slot in][slot out] over the stored
+  // widths; slots beyond a tensor's live width get zeros
+  auto slotChannel = [&](int t, int k) { return k < narrow.width[t] ? narrow.chan[t][k] : -1; };
+  std::vector<float>& wc = narrow.w;
+  auto layer = [&](int l, int out, int A, int B, int ca) {
+    narrow.wOff[l] = static_cast<int64_t>(wc.size());
+    const int sa = narrow.sw[A], sb = B < 0 ? 0 : narrow.sw[B];
+    const int co = out < 0 ? 1 : narrow.sw[out];  // out < 0: the one-channel output layer
+    for (int tap = 0; tap < 9; ++tap)
+      for (int i = 0; i < sa + sb; ++i)
+        for (int o = 0; o < co; ++o) {
+          int src = -1;
+          if (i < sa) {
+            src = slotChannel(A, i);
+          } else if (slotChannel(B, i - sa) >= 0) {
+            src = ca + slotChannel(B, i - sa);
+          }
+          const int dst = out < 0 ? 0 : slotChannel(out, o);
+          wc.push_back(src < 0 || dst < 0 ? 0.f : roundBfHost(W(l, tap, src, dst)));
+        }
+  };
+  // e0a takes the scalar input: [tap][slot]
+  narrow.wOff[kE0a] = 0;
+  for (int tap = 0; tap < 9; ++tap)
+    for (int o = 0; o < narrow.sw[kTe0a]; ++o) {
+      const int dst = slotChannel(kTe0a, o);
+      wc.push_back(dst < 0 ? 0.f : roundBfHost(W(kE0a, tap, 0, dst)));
+    }
+  layer(kE0b, kTe0, kTe0a, -1, 0);
+  layer(kE1a, kTe1a, kTp0, -1, 0);
+  layer(kE1b, kTe1, kTe1a, -1, 0);
+  layer(kE2a, kTe2a, kTp1, -1, 0);
+  layer(kE2b, kTe2, kTe2a, -1, 0);
+  layer(kB3a, kTb3a, kTp2, -1, 0);
+  layer(kB3b, kTb3, kTb3a, -1, 0);
+  layer(kD2a, kTd2a, kTb3, kTe2, 64);
+  layer(kD2b, kTd2, kTd2a, -1, 0);
+  layer(kD1a, kTd1a, kTd2, kTe1, 64);
+  layer(kD1b, kTd1, kTd1a, -1, 0);
+  layer(kD0a, kTd0a, kTd1, kTe0, 32);
+  layer(kD0b, kTd0, kTd0a, -1, 0);
+  layer(kOut, -1, kTd0, -1, 0);  // [tap][slot]
+  nW.ensure(static_cast<int64_t>(wc.size()));
+  CFDHIP_CHECK(hipMemcpyAsync(nW.get(), wc.data(), wc.size() * sizeof(float),
+                              hipMemcpyHostToDevice, stream));
+  CFDHIP_CHECK(hipStreamSynchronize(stream));
+  narrowN = 0;  // the stored widths may have changed: size the tensors again
+  narrow.usable = true;
+}
+
+void Unet::ensureNarrow(int n, int h, int w) {
+  if (n == narrowN && h == narrowH && w == narrowW) return;
+  const int64_t p0 = static_cast<int64_t>(n) * h * w;
+  const int64_t px[4] = {p0, static_cast<int64_t>(n) * (h / 2) * (w / 2),
+                         static_cast<int64_t>(n) * (h / 4) * (w / 4),
+                         static_cast<int64_t>(n) * (h / 8) * (w / 8)};
+  // level of each forward tensor (e0a is fused into e0b and not stored)
+  constexpr int kLevel[kNumForward] = {-1, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 2, 2, 1, 1, 0, 0};
+  int64_t off = 0;
+  for (int t = 0; t < kNumForward; ++t) {
+    nOff[t] = off;
+    if (kLevel[t] >= 0) off += (px[kLevel[t]] * narrow.sw[t] + 63) / 64 * 64;
+  }
+  nact.ensure(off);
+  narrowBytes = off * static_cast<int64_t>(sizeof(float));
+  narrowN = n;
+  narrowH = h;
+  narrowW = w;
+}
+
+void Unet::forwardNarrow(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
+                         int64_t srcBatch, int n, int h, int w) {
+  require(h - std::min(srcH, h) < 8 && w - std::min(srcW, w) < 8,
+          "the network grid rounds the image up to a multiple of 8");
+  ensureNarrow(n, h, w);
+  const float* wts = nW.get();
+  const int* sw = narrow.sw;
+  auto run = [&](int layer, int gather, bool pool, int A, int B, int out, int aux, int hh,
+                 int ww) {
+    NarrowArgs a{};
+    a.a = A < 0 ? nullptr : NT(A);
+    a.b = B < 0 ? nullptr : NT(B);
+    a.w = wts + narrow.wOff[layer];
+    a.w0 = wts + narrow.wOff[kE0a];
+    a.out = NT(out);
+    a.aux = aux < 0 ? nullptr : NT(aux);
+    a.n = n;
+    a.h = hh;
+    a.w_ = ww;
+    levelExtent(hh, ww, a.ph, a.pw);
+    a.pad = padBits();
+    a.tilesX = (ww + kNTW - 1) / kNTW;
+    a.tilesY = (hh + kNTH - 1) / kNTH;
+    a.src = src;
+    a.scale = scale;
+    a.srcH = srcH;
+    a.srcW = srcW;
+    a.srcBatch = srcBatch;
+    const int ca = gather == kNFirst ? sw[kTe0a] : sw[A];
+    mark(layer, true);
+    launchNarrowConv(stream, ca, B < 0 ? 2 : sw[B], sw[out], gather, pool, srcDouble, a);
+    mark(layer, false);
+  };
+  const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+  run(kE0b, kNFirst, true, -1, -1, kTe0, kTp0, h, w);
+  run(kE1a, kNPlain, false, kTp0, -1, kTe1a, -1, h1, w1);
+  run(kE1b, kNPlain, true, kTe1a, -1, kTe1, kTp1, h1, w1);
+  run(kE2a, kNPlain, false, kTp1, -1, kTe2a, -1, h2, w2);
+  run(kE2b, kNPlain, true, kTe2a, -1, kTe2, kTp2, h2, w2);
+  run(kB3a, kNPlain, false, kTp2, -1, kTb3a, -1, h3, w3);
+  if (narrow.tailSrc >= 0 && narrow.tailDst >= 0) {
+    mark(kNumLayers, true);
+    tailApply(NT(kTb3a), n, h3, w3, false, sw[kTb3a], narrow.tailSrc, narrow.tailDst);
+    mark(kNumLayers, false);
+  }
+  run(kB3b, kNPlain, false, kTb3a, -1, kTb3, -1, h3, w3);
+  run(kD2a, kNConcat, false, kTb3, kTe2, kTd2a, -1, h2, w2);
+  run(kD2b, kNPlain, false, kTd2a, -1, kTd2, -1, h2, w2);
+  run(kD1a, kNConcat, false, kTd2, kTe1, kTd1a, -1, h1, w1);
+  run(kD1b, kNPlain, false, kTd1a, -1, kTd1, -1, h1, w1);
+  run(kD0a, kNConcat, false, kTd1, kTe0, kTd0a, -1, h, w);
+  run(kD0b, kNPlain, false, kTd0a, -1, kTd0, -1, h, w);
+  lastNarrow = true;
+}
+
+void Unet::outputNarrow(int n, int h, int w, int ph, int pw) {
+  const dim3 grid(static_cast<unsigned>(std::max(1, std::min(n * ph, 8192))));
+  const float* wo = nW.get() + narrow.wOff[kOut];
+  if (narrow.sw[kTd0] == 2)
+    hipLaunchKernelGGL((narrowLastKernel<2, 0, float>), grid, dim3(256), 0, stream, NT(kTd0), wo, n,
+                       h, w, ph, pw, padBits(), y.get(), nullptr, nullptr, nullptr, nullptr,
+                       int64_t(-1), nullptr);
+  else
+    hipLaunchKernelGGL((narrowLastKernel<4, 0, float>), grid, dim3(256), 0, stream, NT(kTd0), wo, n,
+                       h, w, ph, pw, padBits(), y.get(), nullptr, nullptr, nullptr, nullptr,
+                       int64_t(-1), nullptr);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
+template <class TZ>
+void Unet::outputApplyNarrow(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
+                             int64_t pin, int h, int w, double* partial, int parts) {
+  const float* wo = nW.get() + narrow.wOff[kOut];
+  const dim3 grid(static_cast<unsigned>(parts));
+  if (narrow.sw[kTd0] == 2)
+    hipLaunchKernelGGL((narrowLastKernel<2, 1, TZ>), grid, dim3(256), 0, stream, NT(kTd0), wo, 1, h,
+                       w, ny, nx, padBits(), nullptr, scale, r, q, z, pin, partial);
+  else
+    hipLaunchKernelGGL((narrowLastKernel<4, 1, TZ>), grid, dim3(256), 0, stream, NT(kTd0), wo, 1, h,
+                       w, ny, nx, padBits(), nullptr, scale, r, q, z, pin, partial);
+  CFDHIP_CHECK(hipGetLastError());
+}
+template void Unet::outputApplyNarrow<float>(const float*, const float*, const float*, float*, int,
+                                             int, int64_t, int, int, double*, int);
+template void Unet::outputApplyNarrow<double>(const float*, const double*, const double*, double*,
+                                              int, int, int64_t, int, int, double*, int);
+
+}  // namespace cfdhip

@@ -18,6 +18,24 @@ enum UnetTensor : int {
   kNumTensors
 };
 
+// The network with the channels no output depends on dropped, built from
+// the weights and the tail options: per forward tensor the channels that
+// some later weight reads (slots; a tensor stores only those, 2 or 4 per pixel),
+// per layer the weights between the kept channels (BF16-rounded, as every
+// layer applies them). When every tensor keeps at most kNarrowMax channels,
+// inference runs on the narrow kernels of unet_narrow.hip instead of the
+// MFMA layers; the result is the same linear map.
+constexpr int kNarrowMax = 4;
+struct NarrowPlan {
+  bool usable = false;
+  int sw[kNumForward] = {};                 // stored channels per tensor (2 or 4)
+  int width[kNumForward] = {};              // kept channels per tensor (<= sw)
+  int chan[kNumForward][kNarrowMax] = {};   // the original channel of each slot
+  int tailSrc = -1, tailDst = -1;           // slots of the tail's channels in b3a
+  std::vector<float> w;                     // per layer [tap][slot in][slot out]
+  int64_t wOff[kNumLayers] = {};
+};
+
 class Unet {
  public:
   explicit Unet(hipStream_t s);
@@ -29,6 +47,12 @@ class Unet {
   // none; needs Neumann padding).
   void setOptions(int neumannPadding, int dirichletSides, const TailParams& t);
   void setSides(int dirichletSides);
+  // dense: run inference through the MFMA layers over every channel even
+  // where the weights leave channels unused
+  void setDense(bool dense);
+  // whether inference takes the narrow kernels
+  bool narrowActive() const { return narrow.usable && !dense_; }
+  const NarrowPlan& narrowPlan() const { return narrow; }
   int padding() const { return neumann; }
   // ConvArgs::pad of the current options
   int padBits() const { return neumann ? (kPadNeumann | (sides << 1)) : 0; }
@@ -85,6 +109,25 @@ class Unet {
 
  private:
   void repack();
+  void buildNarrow();
+  void ensureNarrow(int n, int h, int w);
+  void forwardNarrow(const void* src, bool srcDouble, const float* scale, int srcH, int srcW,
+                     int64_t srcBatch, int n, int h, int w);
+  void outputNarrow(int n, int h, int w, int ph, int pw);
+  template <class TZ>
+  void outputApplyNarrow(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
+                         int64_t pin, int h, int w, double* partial, int parts);
+  float* NT(int t) { return nact.get() + nOff[t]; }
+  // the physical extent of the level whose tensors are h x w
+  void levelExtent(int h, int w, int& ph, int& pw) const;
+  NarrowPlan narrow;
+  bool dense_ = false;
+  bool lastNarrow = false;  // the last forward pass ran on the narrow kernels
+  std::vector<float> hostMaster;
+  DeviceArray<float> nact, nW;
+  int64_t nOff[kNumForward] = {};
+  int narrowN = 0, narrowH = 0, narrowW = 0;
+  int64_t narrowBytes = 0;
   float* T(int t) { return act.get() + tensorOff[t]; }
   void conv(int layer, bool transposed, int gather, int epi, int split, const float* a,
             const float* b, float* out, float* aux, int n, int h, int w);
@@ -93,7 +136,9 @@ class Unet {
   void singleDw(int mode, const float* s1, const float* m16, int n, int h, int w, float* out);
   // The coarse tail on the bottom tensor (h3 x w3 per image): forward writes
   // channel tail.dst from tail.src; backward is its adjoint on the gradient.
-  void tailApply(float* tensor, int n, int h3, int w3, bool backward);
+  // C: channels of the tensor; srcCh / dstCh (-1: the options' channels)
+  void tailApply(float* tensor, int n, int h3, int w3, bool backward, int C = 64, int srcCh = -1,
+                 int dstCh = -1);
   int neumann = 0;
   int sides = 0;
   TailParams tail{};

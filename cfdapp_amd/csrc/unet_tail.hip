@@ -326,18 +326,22 @@ void Unet::setOptions(int neumannPadding, int dirichletSides, const TailParams& 
   require(t.levels == 0 || neumannPadding == 1, "the coarse tail needs Neumann padding");
   neumann = neumannPadding;
   tail = t;
+  buildNarrow();  // the tail's channels take part in which channels are live
 }
 
-void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward) {
+void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward, int C, int srcCh,
+                     int dstCh) {
   if (tail.levels == 0) return;
   const TailShape s = tailShape(h3, w3, tail.levels);
   const int64_t hw = static_cast<int64_t>(h3) * w3, pix = n * hw;
-  const int inCh = backward ? tail.dst : tail.src;
-  const int outCh = backward ? tail.src : tail.dst;
+  if (srcCh < 0) srcCh = tail.src;
+  if (dstCh < 0) dstCh = tail.dst;
+  const int inCh = backward ? dstCh : srcCh;
+  const int outCh = backward ? srcCh : dstCh;
   if (s.levels > 0) {
     tailWork.ensure(static_cast<int64_t>(n) * s.perImage);
     const int64_t pix0 = static_cast<int64_t>(n) * tailH(s, 0) * tailW(s, 0);
-    hipLaunchKernelGGL(tailRestrictKernel, tailGrid(pix0), dim3(256), 0, stream, tensor, 64, inCh, s,
+    hipLaunchKernelGGL(tailRestrictKernel, tailGrid(pix0), dim3(256), 0, stream, tensor, C, inCh, s,
                        tail, padBits(), pix0, tailWork.get());
     // levels above kTailCyclePixels run pass by pass over the whole chip; the
     // rest in the one-workgroup-per-image cycle
@@ -363,7 +367,7 @@ void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward) {
       level(t, kPost);
     }
   }
-  hipLaunchKernelGGL(tailScatterKernel, tailGrid(pix), dim3(256), 0, stream, tensor, 64, outCh, inCh,
+  hipLaunchKernelGGL(tailScatterKernel, tailGrid(pix), dim3(256), 0, stream, tensor, C, outCh, inCh,
                      backward ? 1 : 0, s, tail, padBits(), tailWork.get(), pix);
   CFDHIP_CHECK(hipGetLastError());
 }

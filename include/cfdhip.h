@@ -455,7 +455,12 @@ int cfdhip_set_learned_weights(cfdhip_ctx* ctx, const float* weights, int64_t co
  * dirichlet_sides (padding 1): sides where the padding is the odd reflection
  *   (zero value at the face) instead of the even one, bits 1 left, 2 right,
  *   4 bottom, 8 top. On a solver context the sides are the case's FixedValue
- *   pressure patches and this field is ignored. */
+ *   pressure patches and this field is ignored.
+ * dense 0 (default): inference drops the channels no output depends on, and
+ *   a network whose every tensor keeps at most 4 live channels (the shipped
+ *   V-cycle weights keep 2) runs on narrow FP32 kernels over only those;
+ *   dense 1: inference runs every channel through the BF16 MFMA layers.
+ *   Both compute the same linear map (they differ in rounding only). */
 typedef struct {
   int32_t padding;
   int32_t tail_levels;
@@ -463,6 +468,7 @@ typedef struct {
   float tail_a[3];
   float tail_k[3];
   int32_t dirichlet_sides;
+  int32_t dense;
 } cfdhip_unet_options;
 int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* options);
 
