@@ -49,62 +49,137 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_BYTES = 451.0e6
 
 
-def layer_bytes(p0):
-    """Algorithmic HBM bytes of each preconditioner layer on the solve path
-    (FP32 activations, each tensor read and written once; p0 = fine pixels)."""
-    p1, p2, p3 = p0 / 4, p0 / 16, p0 / 64
-    f = 4.0
+def layer_bytes(p0, ch):
+    """Algorithmic HBM bytes of each preconditioner layer on the inference
+    path: every stored tensor read and written once at the channels it stores
+    per pixel (ch: tensor name -> channels, FP32; the shipped weights keep 2,
+    the full network 16 .. 64), the FP64 residual and the FP32 scaling into
+    the first layer, and the FP64 q and z of the output layer (p0 = fine
+    pixels)."""
+    px = [p0, p0 / 4, p0 / 16, p0 / 64]
+    level = {"e0": 0, "p0": 1, "e1a": 1, "e1": 1, "p1": 2, "e2a": 2, "e2": 2, "p2": 3,
+             "b3a": 3, "b3": 3, "d2a": 2, "d2": 2, "d1a": 1, "d1": 1, "d0a": 0, "d0": 0}
+
+    def t(name):
+        return 4.0 * ch[name] * px[level[name]]
+
     return {
-        "e0b": p0 * (8 + 4) + f * (16 * p0 + 16 * p1),   # r (FP64) + D^-1/2 -> e0, p0
-        "e1a": f * (16 * p1 + 32 * p1),
-        "e1b": f * (32 * p1 + 32 * p1 + 32 * p2),
-        "e2a": f * (32 * p2 + 64 * p2),
-        "e2b": f * (64 * p2 + 64 * p2 + 64 * p3),
-        "b3a": f * (64 * p3 + 64 * p3),
-        "b3b": f * (64 * p3 + 64 * p3),
-        "d2a": f * (64 * p3 + 64 * p2 + 64 * p2),
-        "d2b": f * (64 * p2 + 64 * p2),
-        "d1a": f * (64 * p2 + 32 * p1 + 32 * p1),
-        "d1b": f * (32 * p1 + 32 * p1),
-        "d0a": f * (32 * p1 + 16 * p0 + 16 * p0),
-        "d0b": f * (16 * p0 + 16 * p0),
-        "out": f * 16 * p0 + p0 * (4 + 8 + 8),         # d0, D^-1/2, q -> z (FP64)
+        "e0b": p0 * (8 + 4) + t("e0") + t("p0"),
+        "e1a": t("p0") + t("e1a"),
+        "e1b": t("e1a") + t("e1") + t("p1"),
+        "e2a": t("p1") + t("e2a"),
+        "e2b": t("e2a") + t("e2") + t("p2"),
+        "b3a": t("p2") + t("b3a"),
+        "b3b": t("b3a") + t("b3"),
+        "d2a": t("b3") + t("e2") + t("d2a"),
+        "d2b": t("d2a") + t("d2"),
+        "d1a": t("d2") + t("e1") + t("d1a"),
+        "d1b": t("d1a") + t("d1"),
+        "d0a": t("d1") + t("e0") + t("d0a"),
+        "d0b": t("d0a") + t("d0"),
+        "out": t("d0") + p0 * (4 + 8 + 8),
         "e0a": 0.0,
         # coarse tail: one channel of b3a in, one out, and its levels (H/16 ..)
         # each read and written a few times
-        "tail": f * (2 * p3 + 6 * p3 / 3),
+        "tail": 4.0 * (2 * px[3] + 6 * px[3] / 3),
     }
 
 
-def cpu_baseline(iters_gpu):
-    """The reference's own CG + Jacobi (compiled from its sources, OpenMP SpMV),
-    bounded sample of 40 iterations at 2048x2048, scaled to the GPU's iteration
-    count; falls back to the oracle restatement when the reference build is absent."""
+def _lscpu():
+    """Model name and physical core count of the host (lscpu)."""
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {}
+    info = {}
+    for line in text.splitlines():
+        key, _, val = line.partition(":")
+        info[key.strip()] = val.strip()
+    try:
+        cores = int(info.get("Core(s) per socket", "0")) * int(info.get("Socket(s)", "1"))
+    except ValueError:
+        cores = 0
+    return {"model": info.get("Model name", ""), "physical_cores": cores,
+            "logical_cpus": int(info.get("CPU(s)", "0") or 0)}
+
+
+def _ref(driver, args, threads):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    out = subprocess.run([driver, *args], capture_output=True, text=True, env=env, timeout=600,
+                         check=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def cpu_baseline(iters_gpu, pprime_its_per_outer):
+    """The reference's own CPU path, compiled from its sources (oracle/Makefile.ref:
+    -O3 -fopenmp -ffp-contract=off; OpenMP CSR SpMV rows, SparseMatrix.cpp:91), on
+    this host, median of 3 runs each, with OMP_NUM_THREADS = the physical cores
+    (at most the 16 cores this GPU's share holds) and with 1 thread:
+      * C2 stand-alone p' solve: CG + Jacobi iterations on the 2048^2 system
+        (10 per run at full width, 3 at one thread), scaled to the GPU solve's
+        iteration count -> `value`;
+      * C1 129^2 cavity, Re 100, CG + Jacobi p' to 1e-8: 5 SIMPLE iterations
+        per run -> SIMPLE iter/s and p' seconds per outer iteration;
+      * C2 SIMPLE outer iteration at 2048^2, Re 1000: the stages before the p'
+        solve timed at 512^2 (one outer iteration stopped at its p' solve) and
+        scaled by the cell count, plus the p' solve at the CG per-iteration cost
+        times the GPU run's p' iterations per outer iteration (a full CPU
+        iteration at 2048^2 takes several minutes).
+    Falls back to the oracle restatement (one thread, 10 iterations) when the
+    reference build is absent."""
+    import statistics
     driver = os.path.join(REPO, "oracle", "_ref", "ref_driver")
-    cores = min(os.cpu_count() or 1, 16)
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
-    k = 40
-    if os.path.exists(driver):
-        out = subprocess.run([driver, "cgtime", "dim=2", f"nx={N}", f"ny={N}", f"seed={SEED}",
-                              f"pmax={k}"], capture_output=True, text=True, env=env,
-                             timeout=600, check=True).stdout
-        rec = json.loads(out.strip().splitlines()[-1])
-        per_iter = rec["seconds"] / max(rec["iterations"], 1)
-        kind = "reference"
-    else:
+    cpu = _lscpu()
+    threads = max(1, min(cpu.get("physical_cores") or (os.cpu_count() or 1), 16))
+    if not os.path.exists(driver):
         import oracle
         c = oracle.Case.cavity(2, N, N)
         ones = np.ones(c.n_cells)
         A, _, _ = c.pressure_system(np.zeros(c.n_faces), ones, ones)
         b = standalone_rhs(c.n_cells)
         t0 = time.perf_counter()
-        r = A.cg(b, max_iter=k)
+        r = A.cg(b, max_iter=10)
         per_iter = (time.perf_counter() - t0) / max(r.iterations, 1)
-        kind = "port"
-    return {"value": per_iter * iters_gpu, "unit": "s", "cores": cores, "kind": kind,
-            "sample": f"{k} CG+Jacobi iterations on the 2048x2048 p' system "
-                      f"({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU solve's "
-                      f"{iters_gpu} iterations"}
+        return {"value": per_iter * iters_gpu, "unit": "s", "cores": 1, "kind": "port",
+                "sample": f"10 CG+Jacobi iterations of the oracle on the 2048x2048 p' system "
+                          f"({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU solve's "
+                          f"{iters_gpu} iterations", "host": cpu}
+    med = statistics.median
+    per_it = {}
+    for t, k in ((threads, 10), (1, 3)):
+        runs = [_ref(driver, ["cgtime", "dim=2", f"nx={N}", f"ny={N}", f"seed={SEED}",
+                              f"pmax={k}"], t) for _ in range(3)]
+        per_it[t] = med(r["seconds"] / max(r["iterations"], 1) for r in runs)
+    c1 = {}
+    for t in (threads, 1):
+        runs = [_ref(driver, ["simpletime", "dim=2", "nx=129", "ny=129", "re=100", "iters=5",
+                              "flux=rhie_chow", "psolver=cg_jacobi", "pabs=0", "prel=1e-8"], t)
+                for _ in range(3)]
+        c1[f"threads_{t}"] = {
+            "simple_iter_per_s": med(r["iterations"] / r["seconds"] for r in runs),
+            "pprime_s_per_outer_iteration": med(r["pressure_solve"] / r["iterations"]
+                                                for r in runs)}
+    r = _ref(driver, ["simpletime", "dim=2", "nx=512", "ny=512", "re=1000", "iters=1",
+                      "flux=rhie_chow", "psolver=cg_jacobi", "pabs=0", "prel=1e-8", "pmax=1"],
+             threads)
+    before_pprime = (r["setup"] + r["momentum_solve"] + r["discretization"]) * (N * N) / 512 ** 2
+    c2_outer = before_pprime + per_it[threads] * pprime_its_per_outer
+    return {"value": per_it[threads] * iters_gpu, "unit": "s", "cores": threads,
+            "kind": "reference",
+            "sample": f"median of 3 runs of 10 CG+Jacobi iterations (the reference's CG, "
+                      f"OpenMP SpMV, {threads} threads) on the 2048x2048 p' system "
+                      f"({per_it[threads] * 1e3:.1f} ms/iteration), scaled to the GPU solve's "
+                      f"{iters_gpu} iterations",
+            "build": "g++ -O3 -fopenmp -ffp-contract=off (oracle/Makefile.ref)",
+            "host": cpu, "omp": "OMP_PROC_BIND=close OMP_PLACES=cores",
+            "one_thread_value": per_it[1] * iters_gpu,
+            "one_thread_ms_per_iteration": per_it[1] * 1e3,
+            "c1_cavity129_re100": c1,
+            "c2_simple_s_per_outer_iteration": c2_outer,
+            "c2_simple_sample": f"stages before the p' solve of one outer iteration at 512^2 "
+                                f"({r['seconds']:.2f} s) scaled by 16, plus "
+                                f"{pprime_its_per_outer:.0f} p' CG iterations"}
 
 
 def batch_leg(cfd, log, cases, use_graph=True, sequential=True):
@@ -281,12 +356,78 @@ def mixed3d_leg(cfd, log, n, iters):
     return out
 
 
+def _check(r, bnorm, label):
+    """A timed solve counts only if it converged to the FP64 target."""
+    rel = r.true_residual / bnorm
+    if r.status != 0 or not rel <= 1.05e-8:
+        raise RuntimeError(f"{label}: {r.status_name} after {r.iterations} iterations, true "
+                           f"relative residual {rel:.3e}: not a valid measurement")
+    return rel
+
+
+def pprime_compare(cfd, log, d, label):
+    """The p' system assembled on `d` solved by learned FCG and by Jacobi-PCG
+    to the FP64 target (x0 = 0); their times, iterations and agreement."""
+    b = d.download("PC_RHS")
+    bnorm = float(np.linalg.norm(b))
+    fcg = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
+    jac = cfd.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi", check_interval=64)
+    out, x = {}, {}
+    for name, st in (("learned_fcg", fcg), ("jacobi_pcg", jac)):
+        d.solve_pressure_rhs(st, None, download=False)  # warm-up
+        d._ctx.call("cfdhip_synchronize")
+        t0 = time.perf_counter()
+        r = d.solve_pressure_rhs(st, None, download=True)
+        dt = time.perf_counter() - t0
+        rel = _check(r, bnorm, f"{label} {name}")
+        x[name] = r.solution
+        out.update({f"{name}_s": dt, f"{name}_iterations": r.iterations,
+                    f"{name}_true_relative_residual": rel, f"{name}_status": r.status_name})
+        if name == "learned_fcg":
+            out["learned_fcg_fallback"] = bool(r.fallback_used)
+    out["speedup_learned_vs_jacobi"] = out["jacobi_pcg_s"] / out["learned_fcg_s"]
+    xl, xj = x["learned_fcg"], x["jacobi_pcg"]
+    out["solution_rel_diff_inf"] = float(np.max(np.abs(xl - xj)) / np.max(np.abs(xj)))
+    log(f"{label}: learned FCG {out['learned_fcg_iterations']} it {out['learned_fcg_s'] * 1e3:.1f} "
+        f"ms, Jacobi-PCG {out['jacobi_pcg_iterations']} it {out['jacobi_pcg_s'] * 1e3:.1f} ms "
+        f"({out['speedup_learned_vs_jacobi']:.1f}x), solutions agree to "
+        f"{out['solution_rel_diff_inf']:.1e}")
+    return out
+
+
+def c4_leg(cfd, log, n=2048, warm=3):
+    """The learned preconditioner on a cavity it was not trained on: n^2, Re
+    3200 (lid 1, mu = 1/3200); `warm` SIMPLE iterations from rest, then the next
+    iteration's p' system solved by learned FCG and by Jacobi-PCG."""
+    from cfdapp_amd import learned
+    d = learned.make_case("cavity", n, re=3200.0, speed=1.0)
+    try:
+        d.set_learned_weights()
+        fcg = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
+        mom = cfd.LinearSolverSettings(1e-12, 1e-8, 2000, "BiCGSTAB")
+        res = d.simple_run(warm, pressure=fcg, momentum=mom)
+        if res["iterations"] != warm:
+            raise RuntimeError(f"C4 warm-up SIMPLE stopped: status {res['status']}")
+        learned.assemble_pprime(d, 0)
+        out = {"cells": n * n, "re": 3200.0, "warm_simple_iterations": warm,
+               **pprime_compare(cfd, log, d, f"C4 {n}^2 Re 3200 p' after {warm} iterations")}
+    finally:
+        d.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--simple-iters", type=int, default=2)
+    ap.add_argument("--simple-iters", type=int, default=3,
+                    help="2048^2 SIMPLE iterations from rest with each p' solver (0 skips)")
+    ap.add_argument("--c2iii-outer", type=int, default=10,
+                    help="the learned SIMPLE run continues to this outer iteration and solves "
+                         "the next p' system with both methods (<= --simple-iters skips)")
+    ap.add_argument("--no-c4", dest="c4", action="store_false",
+                    help="skip the 2048^2 Re 3200 learned vs Jacobi p' leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c129-iters", type=int, default=200,
                     help="SIMPLE iterations of the 129^2 cavity leg (0 skips it)")
@@ -386,6 +527,8 @@ def main():
             rs.append(d.solve_pressure_rhs(settings, None, download=False))
         d._ctx.call("cfdhip_synchronize")
         sec = (time.perf_counter() - t0) / args.steps
+        for r in rs:
+            _check(r, bnorm, label)
         r = rs[-1]
         log(f"{label}: {args.steps} solves, {sec * 1e3:.1f} ms/solve, {r.status_name}, "
             f"{r.iterations} it, true rel res {r.true_residual / bnorm:.3e}, "
@@ -417,12 +560,13 @@ def main():
     names = ["e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "b3a", "b3b", "d2a", "d2b", "d1a", "d1b",
              "d0a", "d0b", "out", "tail"]
     p0 = float(n)
-    lb = layer_bytes(p0)
+    channels = d.learned_channels()
+    lb = layer_bytes(p0, channels)
     layers = {}
     for i, nm in enumerate(names):
         if layer_calls[i] > 0:
             us = 1e3 * layer_ms[i] / layer_calls[i]
-            layers[nm] = {"us": us, "gbs": lb[nm] / (us * 1e-6) / 1e9}
+            layers[nm] = {"us": us, "bytes": lb[nm], "gbs": lb[nm] / (us * 1e-6) / 1e9}
     per_apply_us = sum(v["us"] for v in layers.values())
     log("preconditioner layers (us, GB/s): " +
         ", ".join(f"{k} {v['us']:.0f} {v['gbs']:.0f}" for k, v in layers.items()) +
@@ -437,6 +581,7 @@ def main():
     t2 = time.perf_counter()
     rm = d.solve_pressure_rhs(mx, None, download=False)
     dtm = time.perf_counter() - t2
+    _check(rm, bnorm, "mixed Jacobi-PCG")
     mixed = {"mixed_jacobi_pcg_s": dtm, "mixed_iterations": rm.iterations,
              "mixed_refinements": rm.restarts,
              "mixed_true_relative_residual": rm.true_residual / bnorm}
@@ -450,6 +595,7 @@ def main():
     t2 = time.perf_counter()
     rfm = d.solve_pressure_rhs(fcg_mx, None, download=False)
     dtfm = time.perf_counter() - t2
+    _check(rfm, bnorm, "mixed learned FCG")
     mixed_fcg = {"learned_fcg_mixed_s": dtfm, "learned_fcg_mixed_iterations": rfm.iterations,
                  "learned_fcg_mixed_refinements": rfm.restarts,
                  "learned_fcg_mixed_fallback": bool(rfm.fallback_used),
@@ -458,26 +604,46 @@ def main():
         f"{dtfm * 1e3:.1f} ms, true rel res {rfm.true_residual / bnorm:.3e}")
 
     # --- SIMPLE outer iterations at Re = 1000: Jacobi-PCG vs learned FCG p' solves ------
-    simple = {}
+    # Both from rest for --simple-iters iterations (the same iterations, so
+    # iter/s compare like for like); the learned run then continues to outer
+    # iteration 10 and its next p' system (C2(iii): the operator of a developed
+    # flow, not the d = 1 Laplacian) is solved by both methods.
+    simple, c2iii = {}, {}
     if args.simple_iters > 0:
         runs = {}
+        mom = cfd.LinearSolverSettings(1e-12, 1e-8, 2000, "BiCGSTAB")
         for label, ps in (("jacobi", cfd.LinearSolverSettings(0.0, 1e-8, 40000, "CG", "Jacobi",
                                                               check_interval=64)),
-                          ("learned", cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG",
-                                                               "Learned"))):
+                          ("learned", fcg)):
             ds = cfd.GpuSimpleDiscretization.cavity(2, N, N, viscosity=1e-3)
             try:
                 if label == "learned":
                     ds.set_learned_weights()
                 ds._ctx.call("cfdhip_synchronize")
                 t1 = time.perf_counter()
-                res = ds.simple_run(args.simple_iters, pressure=ps)
+                res = ds.simple_run(args.simple_iters, pressure=ps, momentum=mom)
                 dt = time.perf_counter() - t1
+                if res["iterations"] != args.simple_iters:
+                    raise RuntimeError(f"SIMPLE ({label} p') stopped: status {res['status']}")
                 runs[label] = (res, dt, ds.download("P"), ds.download_velocity())
+                log(f"SIMPLE ({label} p'): {res['iterations']} outer it in {dt:.2f} s, status "
+                    f"{res['status']}, p' iterations {res['pressure_iterations']}")
+                if label == "learned" and args.c2iii_outer > args.simple_iters:
+                    more = args.c2iii_outer - args.simple_iters
+                    t1 = time.perf_counter()
+                    res2 = ds.simple_run(more, pressure=ps, momentum=mom)
+                    dt2 = time.perf_counter() - t1
+                    if res2["iterations"] != more:
+                        raise RuntimeError(f"SIMPLE (learned p') stopped: status {res2['status']}")
+                    from cfdapp_amd import learned as _learned
+                    _learned.assemble_pprime(ds, 0)
+                    c2iii = {"outer_iteration": args.c2iii_outer + 1,
+                             "learned_simple_iter_per_s_to_it10": more / dt2,
+                             **pprime_compare(cfd, log, ds,
+                                              f"C2(iii) 2048^2 Re 1000 p' at outer iteration "
+                                              f"{args.c2iii_outer + 1}")}
             finally:
                 ds.close()
-            log(f"SIMPLE ({label} p'): {res['iterations']} outer it in {dt:.2f} s, status "
-                f"{res['status']}, p' iterations {res['pressure_iterations']}")
         (rj_, tj, pj, uj), (rl_, tl, pl, ul) = runs["jacobi"], runs["learned"]
         umax = max(float(np.max(np.abs(c))) for c in uj)
         simple = {"simple_iter_per_s": rj_["iterations"] / tj,
@@ -495,6 +661,10 @@ def main():
         log(f"SIMPLE learned vs Jacobi p': {tj / tl:.1f}x iter/s; |dp|/|p| "
             f"{simple['simple_learned_vs_jacobi_p_rel_err_inf']:.2e}, |du|/|u| "
             f"{simple['simple_learned_vs_jacobi_u_rel_err_inf']:.2e}")
+
+    c4 = {}
+    if args.c4:
+        c4 = c4_leg(cfd, log)
 
     c129 = {}
     if args.c129_iters > 0:
@@ -514,7 +684,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rj.iterations)
+        per_outer = (simple["simple_pressure_iterations"] / simple["simple_iterations"]
+                     if simple else float(rj.iterations))
+        cpu = cpu_baseline(rj.iterations, per_outer)
         log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['kind']}, {cpu['cores']} cores)")
 
     line = {
@@ -535,18 +707,24 @@ def main():
                                "Jacobi-PCG",
                    "cells": n,
                    "learned_fcg_s": fcg_s, "learned_fcg_iterations": rf.iterations,
+                   "learned_fcg_status": rf.status_name,
+                   "learned_channels_stored": channels,
                    "learned_fcg_fallback": bool(rf.fallback_used),
                    "learned_fcg_true_relative_residual": rf.true_residual / bnorm,
                    "learned_weights": weights_kind,
-                   "learned_precision": "FCG FP64 vectors; U-Net BF16 weights, FP32 activations "
-                                        "(hi/lo BF16 pairs at the MFMA), FP32 coarse tail",
+                   "learned_precision": "FCG FP64 vectors; U-Net BF16-rounded weights, FP32 "
+                                        "activations and accumulation (unused channels "
+                                        "dropped: narrow FP32 kernels), FP32 coarse tail",
                    "preconditioner_us_per_application": per_apply_us,
                    "preconditioner_layers": layers,
                    "jacobi_pcg_s": jac_s, "jacobi_pcg_iterations": rj.iterations,
+                   "jacobi_pcg_status": rj.status_name,
                    "jacobi_pcg_true_relative_residual": rj.true_residual / bnorm,
                    "speedup_learned_vs_jacobi": jac_s / fcg_s,
                    "plain_spmv": spmv,
                    **mixed, **mixed_fcg, **simple,
+                   "c2iii_developed_flow_pprime": c2iii,
+                   "c4_unseen_cavity_re3200": c4,
                    "cavity129_simple": c129,
                    "piso": piso,
                    "mixed_precision_3d": cube,

@@ -521,6 +521,13 @@ class GpuSimpleDiscretization:
         w = np.ascontiguousarray(weights, dtype=np.float32)
         self._ctx.call("cfdhip_set_learned_weights", w.ctypes.data_as(_lib.P_F32), w.size)
 
+    def learned_channels(self):
+        """Channels stored per pixel by each forward tensor of the learned
+        preconditioner's inference path (dict by tensor name; 0: not stored)."""
+        out = (ctypes.c_int32 * 17)()
+        self._ctx.call("cfdhip_learned_channels", out, 17)
+        return dict(zip(UnetNet.TENSORS[:17], [int(v) for v in out]))
+
     def set_learned_options(self, options=None):
         """Padding and coarse tail of the learned preconditioner (a dict, see
         _unet_options; None: zero padding, no tail)."""

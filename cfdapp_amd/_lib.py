@@ -156,6 +156,8 @@ SIGNATURES = {
     "cfdhip_unet_param_count": (ctypes.c_int64, []),
     "cfdhip_set_learned_weights": (ctypes.c_int, [VP, P_F32, ctypes.c_int64]),
     "cfdhip_set_learned_options": (ctypes.c_int, [VP, ctypes.POINTER(UnetOptions)]),
+    "cfdhip_learned_channels": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int32),
+                                               ctypes.c_int32]),
     "cfdhip_pressure_harvest": (ctypes.c_int, [VP, ctypes.POINTER(SolverSettings), P_F64,
                                                ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                ctypes.c_int64, VP, VP, VP, VP, P_F32, P_I64,

@@ -113,6 +113,18 @@ int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* o) {
   });
 }
 
+int cfdhip_learned_channels(cfdhip_ctx* ctx, int32_t* stored, int32_t count) {
+  return guardedCtx(ctx, [&](Context& c) {
+    require(stored != nullptr && count == kNumForward, "stored must hold 17 entries");
+    require(c.net && c.net->ready(), "no learned weights loaded on this context");
+    constexpr int kFull[kNumForward] = {16, 16, 16, 32, 32, 32, 64, 64, 64,
+                                        64, 64, 64, 64, 32, 32, 16, 16};
+    const bool narrow = c.net->narrowActive();
+    for (int t = 0; t < kNumForward; ++t)
+      stored[t] = narrow ? (t == kTe0a ? 0 : c.net->narrowPlan().sw[t]) : kFull[t];
+  });
+}
+
 int cfdhip_pressure_harvest(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, const double* b,
                             int64_t stride, int64_t max_pairs, int64_t h, int64_t w, float* r_out,
                             float* e_out, float* ce_out, float* cn_out, float* den_out,

@@ -471,6 +471,12 @@ typedef struct {
   int32_t dense;
 } cfdhip_unet_options;
 int cfdhip_set_learned_options(cfdhip_ctx* ctx, const cfdhip_unet_options* options);
+/* Channels stored per pixel by each of the network's 17 forward tensors on
+ * this context's inference path (e0a e0 p0 e1a e1 p1 e2a e2 p2 b3a b3 d2a d2
+ * d1a d1 d0a d0; the full widths 16 .. 64 on the MFMA layers, 2 or 4 with the
+ * unused channels dropped, 0 for a tensor that is not stored), after
+ * cfdhip_set_learned_weights. No counterpart in the reference. */
+int cfdhip_learned_channels(cfdhip_ctx* ctx, int32_t* stored, int32_t count);
 
 /* Training pairs from the assembled p' system (b: host RHS of n doubles, or
  * NULL for the assembled one): an FP64 Jacobi-PCG solve (settings s) keeps its

@@ -27,6 +27,7 @@
 //                                                 non-orthogonal correction and explicit term,
 //                                                 its CG + Jacobi solution, the corrected flux
 //   cgtime  nx ny seed pmax                     -> seconds of pmax reference CG+Jacobi
+//   simpletime dim nx ny re iters flux psolver  -> seconds of SIMPLE::solve and its stage times
 //                                                 iterations on the seeded p' system
 //                                                 (CPU baseline; prints one JSON line)
 //   bicgstab in=DIR abs rel max [jacobi=1]      -> the reference's BiCGSTAB::solve on the CSR
@@ -743,6 +744,30 @@ void cmdCgTime() {
               r.finalResidual);
 }
 
+// Wall time of SIMPLE::solve on the cavity (the CPU path, OpenMP SpMV), with
+// the reference's own per-stage seconds (SIMPLEResult::stageSeconds).
+void cmdSimpleTime() {
+  const Case c = makeCavity();
+  const double re = farg("re", "100");
+  const physics::FluidProperties fluid(1.0, farg("lid", "1.0") / re);
+  pressure_velocity::SIMPLE simple(simpleSettings(), 0);
+  const size_t n = c.mesh.numberOfCells();
+  const auto t0 = std::chrono::steady_clock::now();
+  const auto res = simple.solve(c.mesh, fluid, c.vbc, c.pbc, VectorField(n, Vector3{}),
+                                ScalarField(n, 0.0));
+  const double sec =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const auto& st = res.stageSeconds;
+  std::printf("{\"seconds\": %.9g, \"status\": %d, \"iterations\": %lld, "
+              "\"momentum_iterations\": %lld, \"pressure_iterations\": %lld, "
+              "\"setup\": %.9g, \"momentum_solve\": %.9g, \"pressure_solve\": %.9g, "
+              "\"discretization\": %.9g, \"bookkeeping\": %.9g}\n",
+              sec, static_cast<int>(res.status), static_cast<long long>(res.iterations),
+              static_cast<long long>(res.momentumLinearIterations),
+              static_cast<long long>(res.pressureLinearIterations), st.setup, st.momentumSolve,
+              st.pressureSolve, st.discretization(), st.bookkeeping);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -763,6 +788,7 @@ int main(int argc, char** argv) {
   else if (cmd == "simple") cmdSimple();
   else if (cmd == "stages") cmdStages();
   else if (cmd == "cgtime") cmdCgTime();
+  else if (cmd == "simpletime") cmdSimpleTime();
   else if (cmd == "piso") cmdPiso();
   else if (cmd == "cpcorr") cmdCpcorr();
   else if (cmd == "fpcorr") cmdFpcorr();

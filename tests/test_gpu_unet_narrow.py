@@ -144,6 +144,12 @@ def test_fcg_with_either_path(n):
         try:
             learned.assemble_pprime(d, 2)
             d.set_learned_weights(options=dict(ca.load_learned_options(), dense=dense))
+            ch = d.learned_channels()
+            print("stored channels:", ch, flush=True)
+            if dense:
+                assert ch["e0"] == 16 and ch["b3a"] == 64
+            else:  # two live channels everywhere but b3a (its own, the tail's source and target)
+                assert ch["e0a"] == 0 and ch["e0"] == 2 and ch["d0"] == 2 and ch["b3a"] == 4
             r = d.solve_pressure_rhs(ca.LinearSolverSettings(0.0, 1e-8, 2000, "FCG", "Learned"),
                                      None, download=False)
             print(f"dense={dense}: {r.status_name} {r.iterations} it, {r.device_ms:.2f} ms, "
