@@ -443,6 +443,11 @@ def main():
                     help="only run a plain 512 MiB device copy 20 times (counter calibration)")
     ap.add_argument("--jacobi-only", action="store_true",
                     help="only the Jacobi-PCG solves (for kernel counters)")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only the learned FCG solves (for kernel profiles and counters)")
+    ap.add_argument("--learned-dense", action="store_true",
+                    help="run the learned preconditioner through the MFMA layers over every "
+                         "channel (for comparison)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched-solve leg (for kernel profiles)")
     ap.add_argument("--batch-no-graph", action="store_true",
@@ -492,7 +497,7 @@ def main():
     weights_kind = ("V-cycle parametrisation, Neumann padding, coarse tail "
                     f"({opts['tail']['levels']} levels max), model selected by the training run"
                     if opts.get("padding") == "neumann" else "zero-padded network")
-    d.set_learned_weights()
+    d.set_learned_weights(options=dict(opts, dense=args.learned_dense))
     fcg = cfd.LinearSolverSettings(0.0, 1e-8, 20000, "FCG", "Learned")
     log(f"setup done (learned weights: {weights_kind})")
 
@@ -537,6 +542,9 @@ def main():
 
     if args.jacobi_only:
         timed(pcg, "Jacobi-PCG")
+        return
+    if args.learned_only:
+        timed(fcg, "learned FCG")
         return
     fcg_s, fcg_rs = timed(fcg, "learned FCG")
     jac_s, jac_rs = timed(pcg, "Jacobi-PCG")

@@ -106,6 +106,16 @@ __device__ inline float padFactor(int& y, int& x, int h, int w, int pad) {
   return s;
 }
 
+// Workgroup b runs on XCD b % 8 (round-robin dispatch over the 8 XCDs, each
+// with its own L2). With one workgroup per tile and a multiple of 8 tiles,
+// XCD x takes the x-th contiguous eighth of the row-major tile order, so a
+// tile's halo neighbours are mostly on its own XCD and the halo is fetched
+// from HBM once per L2, not once per neighbour.
+__device__ inline int64_t xcdTile(int64_t b, int64_t tiles) {
+  if (tiles % 8 != 0 || static_cast<int64_t>(gridDim.x) != tiles) return b;
+  return (b % 8) * (tiles / 8) + b / 8;
+}
+
 // s * v with the zero of zero padding exact (a non-finite value read from the
 // clamped pixel stays out of it)
 __device__ inline float signed_(float s, float v) { return s == 0.f ? 0.f : s * v; }
@@ -122,7 +132,8 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
     for (int i = threadIdx.x; i < 9 * CA; i += blockDim.x) w0s[i] = a.w0[i];
   const int h = a.h, wd = a.w_;
   const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
-  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+  for (int64_t tb = blockIdx.x; tb < tiles; tb += gridDim.x) {
+    const int64_t t = xcdTile(tb, tiles);
     const int x0 = static_cast<int>(t % a.tilesX) * kNTW;
     const int64_t rest = t / a.tilesX;
     const int y0 = static_cast<int>(rest % a.tilesY) * kNTH;
@@ -275,9 +286,17 @@ __global__ __launch_bounds__(256) void narrowLastKernel(const float* __restrict_
   if (threadIdx.x < 9 * CW) ws[threadIdx.x] = w[threadIdx.x];
   __syncthreads();
   double acc[1] = {0.0};
-  // a block walks whole rows (one division per row, none per pixel)
+  // a block walks whole rows (one division per row, none per pixel); the
+  // blocks of XCD x (b % 8 == x) take the x-th contiguous eighth of the rows,
+  // so the three rows a pixel reads are mostly in that XCD's L2
   const int rows = n * ph;
-  for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+  const bool byXcd = gridDim.x % 8 == 0;
+  const int per = byXcd ? (rows + 7) / 8 : rows;
+  const int x8 = byXcd ? static_cast<int>(blockIdx.x % 8) : 0;
+  const int g8 = byXcd ? static_cast<int>(gridDim.x / 8) : static_cast<int>(gridDim.x);
+  const int lo = x8 * per, hi = min(rows, lo + per);
+  for (int row = lo + static_cast<int>(byXcd ? blockIdx.x / 8 : blockIdx.x); row < hi;
+       row += g8) {
     const int b = row / ph, yy = row - b * ph;
     for (int xx = threadIdx.x; xx < pw; xx += blockDim.x) {
       const int64_t p = static_cast<int64_t>(row) * pw + xx;
