@@ -888,6 +888,18 @@ class UnetNet:
                    g.ctypes.data_as(_lib.P_F32))
         return loss.value, g
 
+    def spectral_loss_grad(self, z, ce, cn, sys_offset, batch, h, w, steps, cd=None,
+                           grad=True):
+        """Multi-step spectral loss of x <- x + N(b - A x) on the error images z
+        (cfdhip_unet_spectral_loss_grad) and, with grad, its gradient in the
+        master weights; returns (loss, gradient or None)."""
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(unet_param_count(), dtype=np.float32) if grad else None
+        self._call("cfdhip_unet_spectral_loss_grad", _dev(z), _dev(ce), _dev(cn), _dev(cd),
+                   _dev(sys_offset), int(batch), int(h), int(w), int(steps), ctypes.byref(loss),
+                   g.ctypes.data_as(_lib.P_F32) if grad else None)
+        return loss.value, g
+
     def train_step(self, x, e, ce, cn, sys_offset, den, batch, h, w, lr, update=True,
                    cd=None) -> float:
         loss = ctypes.c_double(0.0)

@@ -184,6 +184,9 @@ SIGNATURES = {
     "cfdhip_unet_train_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP, ctypes.c_int64,
                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                               ctypes.c_int, P_F64]),
+    "cfdhip_unet_spectral_loss_grad": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, ctypes.c_int64,
+                                                      ctypes.c_int64, ctypes.c_int64,
+                                                      ctypes.c_int32, P_F64, P_F32]),
     "cfdhip_pressure_spmv_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int]),
     "cfdhip_field_device": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(VP)]),
     "cfdhip_assemble_transient_momentum": (ctypes.c_int, [VP, ctypes.c_double]),

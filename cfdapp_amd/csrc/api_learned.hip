@@ -289,6 +289,20 @@ int cfdhip_unet_loss_grad(cfdhip_unet* h, const float* x, const float* e, const 
   });
 }
 
+int cfdhip_unet_spectral_loss_grad(cfdhip_unet* h, const float* z, const float* ce,
+                                   const float* cn, const float* cd, const int64_t* sys_offset,
+                                   int64_t batch, int64_t hh, int64_t w, int32_t steps,
+                                   double* loss, float* grad) {
+  return guardedNet(h, [&](Unet& net) {
+    checkShape(batch, hh, w);
+    require(z && ce && cn && sys_offset && loss, "buffers are required");
+    require(steps >= 1 && steps <= 32, "steps must be in 1..32");
+    const int n = static_cast<int>(batch), H = static_cast<int>(hh), W = static_cast<int>(w);
+    net.ensureShape(n, H, W, grad != nullptr);
+    *loss = net.spectral(z, ce, cn, cd, sys_offset, n, H, W, steps, grad);
+  });
+}
+
 int cfdhip_unet_train_step(cfdhip_unet* h, const float* x, const float* e, const float* ce,
                            const float* cn, const float* cd, const int64_t* sys_offset,
                            const float* den, int64_t batch, int64_t hh, int64_t w, float lr,

@@ -92,17 +92,66 @@ constexpr int64_t kTallTileMin = 2048;
 // NSUM = 6 adds the sums of the explicit-conjugation scalars of the mixed
 // solve: (u', s), (p, s), (r', p) of this iteration's direction p, its
 // recurrence s = A p and the new residual r', u' = M r'.
-template <class T, int ROWS, class TX = T, int NSUM = 3>
+// FUSED: the iteration also runs the previous iteration's scalar step
+// (kStepCg1Iter) itself, so no finalize launch sits between two iterations:
+// every block sums the previous launch's partials (fuse.partIn, a fixed order,
+// the same in every block) and steps a copy of the state read from st; block
+// 0 writes the stepped state to fuse.stOut (a second state buffer, so no
+// block reads a state another block of the same launch writes) and the
+// history. doStep == 0 takes the state as it is (the first launch after a
+// finalize). Meant for grids of few tiles, where the partials are few.
+struct FuseArgs {
+  KState* stOut;
+  const double* partIn;
+  int64_t nPartsIn;
+  double* hist;
+  int doStep;
+};
+constexpr int64_t kFusedMaxTiles = 256;
+
+template <class T, int ROWS, class TX = T, int NSUM = 3, bool FUSED = false>
 __global__ __launch_bounds__(256) void cgIterKernel(
     Grid2 g, const T* __restrict__ diag, const T* __restrict__ coeff,
     const T* __restrict__ pOld, T* __restrict__ pNew, const T* __restrict__ sOld,
     T* __restrict__ sNew, const T* __restrict__ rOld, T* __restrict__ rNew,
     const T* __restrict__ wOld, T* __restrict__ wNew, TX* __restrict__ x, int64_t tilesX,
-    KState* st, double* partial, int start) {
+    KState* st, double* partial, int start, FuseArgs fuse = {}) {
   constexpr int kTileRows = TileShape<ROWS>::kTileRows;
   constexpr int kRowsPerWave = TileShape<ROWS>::kRowsPerWave;
   constexpr int kInteriorRows = TileShape<ROWS>::kInteriorRows;
   __shared__ T uTile[kTileRows][64];
+  if constexpr (FUSED) {
+    __shared__ KState ks;
+    __shared__ double sums[3][kRedThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double t3[3] = {0.0, 0.0, 0.0};
+    if (fuse.doStep)
+      for (int64_t b = threadIdx.x; b < fuse.nPartsIn; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) t3[q] += fuse.partIn[b * 3 + q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double y = t3[q];
+      for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+      if (lane == 0) sums[q][wid] = y;
+    }
+    if (threadIdx.x == 0) ks = *st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (fuse.doStep && !ks.done) {
+        double t[3];
+        for (int q = 0; q < 3; ++q) {
+          double y = 0.0;
+          for (int i = 0; i < kRedThreads / 64; ++i) y += sums[q][i];
+          t[q] = y;
+        }
+        cg1IterStep(ks, blockIdx.x == 0 ? fuse.hist : nullptr, t);
+      }
+      if (blockIdx.x == 0) *fuse.stOut = ks;
+    }
+    __syncthreads();
+    st = &ks;
+  }
   const bool skip = st->done;
   const T alpha = static_cast<T>(st->alpha);
   const T beta = static_cast<T>(st->beta);
@@ -328,6 +377,23 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
                        p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur], wv[cur],
                        wv[1 - cur], x, tilesX, buf.state, buf.partial, start);
   };
+  // small grids: the fused iteration kernel steps the previous iteration's
+  // scalars itself. Launch k of a chunk (an even number of launches) reads
+  // state S[k % 2] and writes S[(k + 1) % 2], writes partials P[k % 2] and
+  // reads P[(k - 1) % 2]; launch 0 takes S[0] as the finalize left it, and one
+  // finalize after the chunk steps S[0] in place from the last partials.
+  const bool fused = buf.state2 != nullptr && snap == nullptr &&
+                     static_cast<int64_t>(grid.x) <= kFusedMaxTiles &&
+                     buf.partialCap >= 6 * kFusedMaxTiles;
+  KState* Sb[2] = {buf.state, buf.state2};
+  double* P[2] = {buf.partial, buf.partial + 3 * kFusedMaxTiles};
+  auto launchFused = [&](int cur, int k) {
+    const FuseArgs f{Sb[(k + 1) & 1], P[(k + 1) & 1], static_cast<int64_t>(grid.x), buf.hist,
+                     k > 0 ? 1 : 0};
+    hipLaunchKernelGGL((cgIterKernel<T, 32, T, 3, true>), grid, dim3(256), 0, ctx.stream, g, diag,
+                       coeff, p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur],
+                       wv[cur], wv[1 - cur], x, tilesX, Sb[k & 1], P[k & 1], 0, f);
+  };
   const bool timing = ctx.timing && buf.events;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   const auto wall0 = std::chrono::steady_clock::now();
@@ -362,15 +428,35 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
   const int gChunk = chunk + (chunk & 1);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // one chunk of fused launches (gChunk of them, an even count) and its finalize
+  auto fusedChunk = [&]() {
+    for (int k = 0; k < gChunk; ++k) {
+      launchFused(cur, k);
+      cur = 1 - cur;
+    }
+    launchFinalize(ctx, FinalArgs{P[(gChunk - 1) & 1], static_cast<int64_t>(grid.x), Sb[0],
+                                  buf.hist, nullptr, kStepCg1Iter});
+  };
   while (true) {
     int q = 0;
+    if (fused && !graphed) {
+      if (timing) CFDHIP_CHECK(hipEventRecord(ev[0], ctx.stream));
+      fusedChunk();
+      if (timing) CFDHIP_CHECK(hipEventRecord(ev[1], ctx.stream));
+      launched += gChunk;
+      q = gChunk;
+    }
     if (graphed) {
       if (exec == nullptr) {
         CFDHIP_CHECK(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < gChunk; ++k) {
-          launch(cur, 0);
-          finalize(kStepCg1Iter, grid.x);
-          cur = 1 - cur;
+        if (fused) {
+          fusedChunk();
+        } else {
+          for (int k = 0; k < gChunk; ++k) {
+            launch(cur, 0);
+            finalize(kStepCg1Iter, grid.x);
+            cur = 1 - cur;
+          }
         }
         CFDHIP_CHECK(hipStreamEndCapture(ctx.stream, &graph));
         CFDHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -379,7 +465,7 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
       launched += gChunk;
       q = gChunk;
     }
-    for (; !graphed && q < chunk && launched < set.max_iterations; ++q, ++launched) {
+    for (; !graphed && !fused && q < chunk && launched < set.max_iterations; ++q, ++launched) {
       if (timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q], ctx.stream));
       launch(cur, 0);
       if (timing) CFDHIP_CHECK(hipEventRecord(ev[2 * q + 1], ctx.stream));
@@ -396,7 +482,16 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
     CFDHIP_CHECK(hipMemcpyAsync(host, buf.state, sizeof(KState), hipMemcpyDeviceToHost,
                                 ctx.stream));
     CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
-    if (timing) {
+    if (timing && fused) {
+      // the chunk as a whole: launches per iteration that did work
+      const int64_t worked = std::min<int64_t>(q, host->iter - before);
+      float ms = 0.f;
+      CFDHIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      if (worked == q) {
+        kMs += ms;
+        kLaunches += static_cast<double>(q);
+      }
+    } else if (timing) {
       const int64_t worked = std::min<int64_t>(q, host->iter - before);
       for (int e = 0; e < worked; ++e) {
         float ms = 0.f;
@@ -458,11 +553,11 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
                                   TileShape<32>::kInteriorRows);
   const int64_t nb = (v.n + 255) / 256;
   w.partial.ensure(std::max<int64_t>({kMaxBlocks * 4, tiles * 3 + 8, nb + 8, w.partial.count}));
-  w.state.ensure(1);
+  w.state.ensure(2);
   if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
   w.hist.ensure(set.max_iterations + 2);
   const PcgBuffers buf{w.partial.get(), w.partial.count, w.state.get(), w.hostState,
-                       w.hist.get(), w.hist.count, true};
+                       w.hist.get(), w.hist.count, true, w.state.get() + 1};
   return pcg2dOnBuffers(ctx, set, diag, coeff, b, x, work, buf, snap);
 }
 

@@ -205,6 +205,10 @@ struct PcgBuffers {
   double* hist;
   int64_t histCap;
   bool events;
+  // a second device state: lets small grids run the fused iteration kernel
+  // (no finalize launch between iterations); null keeps one finalize per
+  // iteration
+  KState* state2 = nullptr;
 };
 cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& set,
                                    const double* diag, const double* coeff, const double* b,

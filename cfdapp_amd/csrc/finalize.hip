@@ -9,17 +9,6 @@ namespace cfdhip {
 
 namespace {
 
-__device__ inline bool convergedTest(const KState& s, double a) {
-  return a <= s.absTol || ((s.b0 > 0.0) && (a / s.b0 <= s.relTol));
-}
-__device__ inline void finish(KState& s, int status) {
-  s.status = status;
-  s.done = 1;
-}
-__device__ inline void pushHist(KState& s, double* hist, double value) {
-  hist[s.histLen] = value;
-  s.histLen += 1;
-}
 // cancelledToRoundingLevel (BiCGSTAB.cpp:34-41) with sum_i |x_i y_i| at hand.
 __device__ inline bool cancelled(double ip, double absSum, double nx, double ny) {
   if (fabs(ip) > DBL_EPSILON * nx * ny) return false;
@@ -108,41 +97,9 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
       s.beta = 0.0;
       return;
     }
-    case kStepCg1Iter: {
-      const double gamma = t[0], delta = t[1];
-      const double rnorm = sqrt(t[2]);
-      s.iter += 1;
-      pushHist(s, hist, rnorm);
-      s.rnorm = rnorm;
-      if (!isfinite(rnorm)) {
-        finish(s, CFDHIP_NONFINITE_RESIDUAL);
-        return;
-      }
-      if (convergedTest(s, rnorm)) {
-        finish(s, CFDHIP_CONVERGED);
-        return;
-      }
-      if (!isfinite(gamma)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      const double beta = gamma / s.rz;
-      const double pAp = delta - beta * gamma / s.alpha;  // (p', A p') of the next step
-      if (!isfinite(beta) || !isfinite(pAp) || fabs(pAp) < 1e-30) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      const double alpha = gamma / pAp;
-      if (!isfinite(alpha)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      s.beta = beta;
-      s.alpha = alpha;
-      s.rz = gamma;
-      if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+    case kStepCg1Iter:
+      cg1IterStep(s, hist, t);
       return;
-    }
     case kStepBcgRho: {  // BiCGSTAB.cpp:149-163
       const double rho = t[0];
       if (!isfinite(rho)) {

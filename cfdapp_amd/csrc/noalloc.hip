@@ -33,7 +33,7 @@ constexpr int64_t kNaMaxIter = int64_t(1) << 16;
 // for n cells) and start partials (one per 256 cells)
 constexpr int64_t kNaPartials = 131072;
 constexpr int64_t kNaHist = kNaMaxIter + 2;
-constexpr int64_t kNaStateDoubles = (sizeof(KState) + 7) / 8 + 8;
+constexpr int64_t kNaStateDoubles = 2 * ((sizeof(KState) + 7) / 8) + 8;  // two states
 // layout in doubles: 8 work vectors | partials | history | state | counters + norm
 constexpr int64_t kNaWork = 8 * kNaMaxCells;
 constexpr int64_t kNaOffPartial = kNaWork;
@@ -185,7 +185,7 @@ cfdhip_solve_result solveNoAlloc(Context& ctx, const cfdhip_solver_settings& set
   KState host{};
   cfdhip_solver_settings run = set;
   run.graph = 0;
-  const PcgBuffers buf{partial, kNaPartials, state, &host, hist, kNaHist, false};
+  const PcgBuffers buf{partial, kNaPartials, state, &host, hist, kNaHist, false, state + 1};
   res = pcg2dOnBuffers(ctx, run, diag, coeff, b, x, work, buf);
   // the FP64 true residual |b - A x|
   const dim3 g = strideGrid(n);

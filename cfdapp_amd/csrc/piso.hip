@@ -165,7 +165,7 @@ int64_t pisoRun(Context& ctx, double dt, int64_t steps, int64_t ref,
   int64_t k = 0;
   for (; k < steps; ++k) {
     pisoStep(ctx, dt, ref, mom, pres, meta + 6 * k, save);
-    if (meta[6 * k] != kPisoConverged) {
+    if (meta[6 * k] != static_cast<double>(kPisoConverged)) {
       ++k;
       break;
     }

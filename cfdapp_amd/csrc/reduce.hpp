@@ -99,6 +99,59 @@ enum FinalStep : int {
   kStepCgReplace        // the same scalars after an FP64 residual replacement
 };
 
+__device__ inline bool convergedTest(const KState& s, double a) {
+  return a <= s.absTol || ((s.b0 > 0.0) && (a / s.b0 <= s.relTol));
+}
+__device__ inline void finish(KState& s, int status) {
+  s.status = status;
+  s.done = 1;
+}
+// Appends to the residual history (hist null: counts the entry only).
+__device__ inline void pushHist(KState& s, double* hist, double value) {
+  if (hist != nullptr) hist[s.histLen] = value;
+  s.histLen += 1;
+}
+
+// kStepCg1Iter (CG.cpp:97-140 in the single-reduction form): t = (r', u'),
+// (w', u'), |r'|^2 of the iteration just done -> the recurrence residual, the
+// convergence test, and beta, alpha of the next iteration. Run by the
+// finalize kernel and, redundantly in every block, by the fused small-grid
+// iteration kernel.
+__device__ inline void cg1IterStep(KState& s, double* hist, const double* t) {
+  const double gamma = t[0], delta = t[1];
+  const double rnorm = sqrt(t[2]);
+  s.iter += 1;
+  pushHist(s, hist, rnorm);
+  s.rnorm = rnorm;
+  if (!isfinite(rnorm)) {
+    finish(s, CFDHIP_NONFINITE_RESIDUAL);
+    return;
+  }
+  if (convergedTest(s, rnorm)) {
+    finish(s, CFDHIP_CONVERGED);
+    return;
+  }
+  if (!isfinite(gamma)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  const double beta = gamma / s.rz;
+  const double pAp = delta - beta * gamma / s.alpha;  // (p', A p') of the next step
+  if (!isfinite(beta) || !isfinite(pAp) || fabs(pAp) < 1e-30) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  const double alpha = gamma / pAp;
+  if (!isfinite(alpha)) {
+    finish(s, CFDHIP_BREAKDOWN);
+    return;
+  }
+  s.beta = beta;
+  s.alpha = alpha;
+  s.rz = gamma;
+  if (s.iter >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+}
+
 // Mixed-precision Jacobi-PCG: whether an FP64 residual replacement restarts
 // the search direction (1) or continues it with explicit conjugation (0).
 // Measured on the d = 1 p' operators (2048^2, 256^3): continuing the

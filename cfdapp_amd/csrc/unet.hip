@@ -401,6 +401,67 @@ __global__ void sumDoubleKernel(const double* __restrict__ partial, int parts, d
   if (threadIdx.x == 0) out[0] = v[0];
 }
 
+// y = B_s x on every image s of the batch: diagonal cD (unit when null),
+// couplings cE to x + 1 and cN to y + 1, all at sysOff[s] (the loss operator
+// of anormLossKernel).
+__global__ __launch_bounds__(256) void opApplyKernel(
+    const float* __restrict__ x, const float* __restrict__ cE, const float* __restrict__ cN,
+    const float* __restrict__ cD, const int64_t* __restrict__ sysOff, int n, int h, int wd,
+    float* __restrict__ y) {
+  const int64_t hw = static_cast<int64_t>(h) * wd, total = n * hw;
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int s = static_cast<int>(p / hw);
+    const int64_t c = p - s * hw;
+    const int xx = static_cast<int>(c % wd), yy = static_cast<int>(c / wd);
+    const float* ce = cE + sysOff[s];
+    const float* cn = cN + sysOff[s];
+    float a = cD != nullptr ? cD[sysOff[s] + c] * x[p] : x[p];
+    if (xx + 1 < wd) a -= ce[c] * x[p + 1];
+    if (xx > 0) a -= ce[c - 1] * x[p - 1];
+    if (yy + 1 < h) a -= cn[c] * x[p + wd];
+    if (yy > 0) a -= cn[c - wd] * x[p - wd];
+    y[p] = a;
+  }
+}
+
+// out = a - f[s] b per image s (f null: 1).
+__global__ __launch_bounds__(256) void subScaledKernel(const float* __restrict__ a,
+                                                       const float* __restrict__ b,
+                                                       const float* __restrict__ f, int64_t hw,
+                                                       int64_t total, float* __restrict__ out) {
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[p] = a[p] - (f != nullptr ? f[p / hw] : 1.f) * b[p];
+}
+
+// x = f[s] x per image s.
+__global__ __launch_bounds__(256) void scaleImagesKernel(float* __restrict__ x,
+                                                         const float* __restrict__ f, int64_t hw,
+                                                         int64_t total) {
+  for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < total;
+       p += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[p] *= f[p / hw];
+}
+
+// out[s] = (a_s, b_s) in FP64, one block per image, a fixed summation order.
+__global__ __launch_bounds__(256) void imageDotKernel(const float* __restrict__ a,
+                                                      const float* __restrict__ b, int64_t hw,
+                                                      double* __restrict__ out) {
+  const int64_t base = blockIdx.x * hw;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < hw; i += 256)
+    acc += static_cast<double>(a[base + i]) * static_cast<double>(b[base + i]);
+  __shared__ double lds[256];
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = lds[0];
+}
+
 __global__ void adamKernel(int64_t count, float* __restrict__ w, float* __restrict__ m,
                            float* __restrict__ v, const float* __restrict__ g, float lr, float b1,
                            float b2, float eps, float bc1, float bc2) {
@@ -802,6 +863,101 @@ double Unet::loss(const float* e, const float* cE, const float* cN, const float*
   CFDHIP_CHECK(hipMemcpyAsync(&v, out, sizeof(double), hipMemcpyDeviceToHost, stream));
   CFDHIP_CHECK(hipStreamSynchronize(stream));
   return v;
+}
+
+double Unet::spectral(const float* z, const float* cE, const float* cN, const float* cD,
+                      const int64_t* sysOffDev, int n, int h, int w, int steps, float* gradHost) {
+  const int64_t hw = static_cast<int64_t>(h) * w, P = n * hw;
+  const int k = steps;
+  spec.ensure((2LL * (k + 1) + 2) * P);
+  specF.ensure(n);
+  specDots.ensure(n);
+  float* V = spec.get();                          // v_0 .. v_k
+  float* BV = V + static_cast<int64_t>(k + 1) * P;  // B v_0 .. B v_k
+  float* G = BV + static_cast<int64_t>(k + 1) * P;  // dL/dv_j
+  float* Tm = G + P;
+  const dim3 g = grid1(P);
+  auto op = [&](const float* x, float* out) {
+    hipLaunchKernelGGL(opApplyKernel, g, dim3(256), 0, stream, x, cE, cN, cD, sysOffDev, n, h, w,
+                       out);
+    CFDHIP_CHECK(hipGetLastError());
+  };
+  std::vector<double> dots(n);
+  auto imageDots = [&](const float* a, const float* b) {
+    hipLaunchKernelGGL(imageDotKernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, stream, a, b,
+                       hw, specDots.get());
+    CFDHIP_CHECK(hipGetLastError());
+    CFDHIP_CHECK(hipMemcpyAsync(dots.data(), specDots.get(), sizeof(double) * n,
+                                hipMemcpyDeviceToHost, stream));
+    CFDHIP_CHECK(hipStreamSynchronize(stream));
+  };
+  std::vector<float> f(n);
+  auto setFactors = [&]() {
+    CFDHIP_CHECK(hipMemcpyAsync(specF.get(), f.data(), sizeof(float) * n, hipMemcpyHostToDevice,
+                                stream));
+    CFDHIP_CHECK(hipStreamSynchronize(stream));
+  };
+  // y = N(x) on the training path (every layer stored, nothing fused)
+  auto apply = [&](const float* x) {
+    forward(x, nullptr, h, w, hw, n, h, w);
+    outputTrain(n, h, w);
+  };
+  // v_0 = z / |z|_B
+  CFDHIP_CHECK(hipMemcpyAsync(V, z, sizeof(float) * P, hipMemcpyDeviceToDevice, stream));
+  op(V, BV);
+  imageDots(V, BV);
+  for (int s = 0; s < n; ++s) {
+    if (!(dots[s] > 0.0) || !std::isfinite(dots[s]))
+      throw Error(CFDHIP_ERR_INVALID,
+                  "spectral loss: an error image has no positive energy in its operator");
+    f[s] = static_cast<float>(1.0 / std::sqrt(dots[s]));
+  }
+  setFactors();
+  hipLaunchKernelGGL(scaleImagesKernel, g, dim3(256), 0, stream, V, specF.get(), hw, P);
+  hipLaunchKernelGGL(scaleImagesKernel, g, dim3(256), 0, stream, BV, specF.get(), hw, P);
+  CFDHIP_CHECK(hipGetLastError());
+  // v_j = v_{j-1} - N(B v_{j-1})
+  for (int j = 1; j <= k; ++j) {
+    apply(BV + (j - 1) * P);
+    hipLaunchKernelGGL(subScaledKernel, g, dim3(256), 0, stream, V + (j - 1) * P, y.get(),
+                       nullptr, hw, P, V + j * P);
+    CFDHIP_CHECK(hipGetLastError());
+    op(V + j * P, BV + j * P);
+  }
+  imageDots(V + k * P, BV + k * P);
+  std::vector<double> raw(dots);
+  double loss = 0.0;
+  for (int s = 0; s < n; ++s) {
+    if (!(raw[s] > 0.0) || !std::isfinite(raw[s]))
+      throw Error(CFDHIP_ERR_NONFINITE, "spectral loss: the iterated error vanished or overflowed");
+    loss += std::log(raw[s]);
+  }
+  loss /= 2.0 * k * n;
+  if (gradHost == nullptr) return loss;
+  // L = mean_s log(|v_k|_B^2) / 2k: dL/dv_k = 2 B v_k / (2k n |v_k|_B^2)
+  for (int s = 0; s < n; ++s) f[s] = static_cast<float>(1.0 / (k * n * raw[s]));
+  setFactors();
+  CFDHIP_CHECK(hipMemcpyAsync(G, BV + k * P, sizeof(float) * P, hipMemcpyDeviceToDevice, stream));
+  hipLaunchKernelGGL(scaleImagesKernel, g, dim3(256), 0, stream, G, specF.get(), hw, P);
+  CFDHIP_CHECK(hipGetLastError());
+  std::vector<double> acc(kUnetParams, 0.0);
+  std::vector<float> part(kUnetParams);
+  for (int j = k; j >= 1; --j) {
+    // dL/dW -= d/dW (G, N(B v_{j-1}))
+    forward(BV + (j - 1) * P, nullptr, h, w, hw, n, h, w);
+    backward(BV + (j - 1) * P, G, n, h, w);
+    getGrads(part.data());
+    for (int64_t i = 0; i < kUnetParams; ++i) acc[i] -= part[i];
+    if (j > 1) {
+      // dL/dv_{j-1} = G - B N^T G, with N^T = N (a self-adjoint network)
+      apply(G);
+      op(y.get(), Tm);
+      hipLaunchKernelGGL(subScaledKernel, g, dim3(256), 0, stream, G, Tm, nullptr, hw, P, G);
+      CFDHIP_CHECK(hipGetLastError());
+    }
+  }
+  for (int64_t i = 0; i < kUnetParams; ++i) gradHost[i] = static_cast<float>(acc[i]);
+  return loss;
 }
 
 void Unet::adam(float lr, float b1, float b2, float eps) {

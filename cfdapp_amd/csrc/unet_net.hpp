@@ -92,6 +92,14 @@ class Unet {
               const int64_t* sysOffDev,
               const float* denDev, int n, int h, int w);
   void adam(float lr, float b1, float b2, float eps);
+  // Multi-step spectral loss of the stationary iteration x <- x + N(b - B x)
+  // on n error images z (h x w, multiples of 8) with the operators of the
+  // A-norm loss: v_0 = z / |z|_B, v_j = v_{j-1} - N(B v_{j-1}) (j = 1..steps),
+  // L = mean_s log(|v_steps|_B^2) / (2 steps). gradHost (param count floats)
+  // receives dL/dW by reverse accumulation through the steps with N^T = N,
+  // or nothing when null.
+  double spectral(const float* z, const float* cE, const float* cN, const float* cD,
+                  const int64_t* sysOffDev, int n, int h, int w, int steps, float* gradHost);
 
   float* output() { return y.get(); }
   // Tensor t of the current shape, copied to the host.
@@ -149,6 +157,8 @@ class Unet {
 
   DeviceArray<float> master, grad, adamM, adamV, y, gy, dwPartial;
   DeviceArray<double> lossPartial;
+  DeviceArray<float> spec, specF;  // spectral loss: iterates, their B images, per-image factors
+  DeviceArray<double> specDots;
   DeviceArray<bf16> fwdPack, bwdPack;
   DeviceArray<float> act;
   int64_t fwdOff[kNumLayers] = {};

@@ -407,6 +407,149 @@ def train_symmetric(torch, bank, theta, steps, lr, seed=0, deadline=None, log_ev
     return theta, losses, evals
 
 
+TAIL_THETA0 = np.array([0.75, 1.0, -0.25, 0.0, 0.25, 0.125, 0.0625])
+
+
+def tail_options(tail_theta, levels=16):
+    """The coarse tail of the options from its 7 parameters: omega, the level
+    operator (centre, edge, corner) and the transfer kernel (centre, edge,
+    corner)."""
+    t = np.asarray(tail_theta, np.float64)
+    return {"padding": "neumann",
+            "tail": {"levels": levels, "omega": float(t[0]), "a": [float(v) for v in t[1:4]],
+                     "k": [float(v) for v in t[4:7]]}}
+
+
+def repair_tail(tail_theta, margin=0.02):
+    """0 < omega <= (2 - margin) / max symbol of the tail's level operator."""
+    t = np.array(tail_theta, np.float64)
+    c, e, k = t[1:4]
+    top = max(c + 4 * e + 4 * k, c - 4 * e + 4 * k, c - 4 * k, 1e-6)
+    t[0] = float(np.clip(t[0], 0.05, (2.0 - margin) / top))
+    return t
+
+
+def train_spectral(torch, bank, theta, tail_theta, steps, lr, k=6, seed=0, deadline=None,
+                   log_every=25, eval_fn=None, eval_every=0, train_tail=True, fd_step=5e-3):
+    """Adam on the V-cycle parameters (and the coarse tail's) with the
+    multi-step spectral loss (UnetNet.spectral_loss_grad): the mean log
+    convergence factor of the stationary iteration x <- x + N(b - A x) over
+    k steps from the harvested errors, whose gradient follows the factor
+    of the slowest error modes -- the ones that set the FCG iteration count --
+    where the one-application A-norm loss follows the average error. The
+    network weights' gradient comes from the device's backward pass (chain
+    rule to theta on the host); the tail's 7 shared coefficients are not
+    weights, and their gradient is the central difference of the same loss on
+    the same batch. With eval_fn (weights, options -> score, lower is better)
+    the model is scored before training and every eval_every steps, and the
+    best-scoring parameters are returned (model selection on held-out
+    systems): (theta, tail_theta, losses, evals)."""
+    net = UnetNet(VCycle.weights(theta), options=tail_options(tail_theta))
+    rng = np.random.default_rng(seed)
+    sizes = [key for key in bank.count if bank.count[key] > 0]
+    probs = np.array([bank.count[key] * key[0] * key[0] for key in sizes], np.float64)
+    probs /= probs.sum()
+    th = np.concatenate([np.asarray(theta, np.float64), np.asarray(tail_theta, np.float64)])
+    nv = len(theta)
+    m = np.zeros(len(th))
+    v = np.zeros(len(th))
+    b1, b2 = 0.9, 0.999
+    losses, evals = [], []
+    t0 = time.time()
+    best = (None, th.copy(), 0)
+
+    def opts_of(t, key):
+        return dict(tail_options(t[nv:]), dirichlet_sides=list(key[1]))
+
+    # the output scale does not change the FCG iterations but sets the
+    # stationary iteration's convergence: start it at the scale that minimises
+    # the loss on a few batches, so that the first steps do not spend the
+    # other parameters on a diverging iteration
+    batches = []
+    for _ in range(4):
+        key = sizes[int(rng.choice(len(sizes), p=probs))]
+        n = key[0]
+        b = min(BATCH[n], bank.count[key])
+        idx = np.sort(rng.choice(bank.count[key], size=b, replace=False))
+        batches.append((key, n, b, idx))
+
+    def batch_loss(t):
+        tot = 0.0
+        for key, n, b, idx in batches:
+            z = bank.e[key].index_select(0, torch.from_numpy(idx).cuda()).contiguous()
+            off = torch.from_numpy(bank.sys[key][idx] * n * n).cuda()
+            net.set_options(opts_of(t, key))
+            net.set_weights(VCycle.weights(t[:nv]))
+            tot += net.spectral_loss_grad(z, bank.ce[key], bank.cn[key], off, b, n, n, k,
+                                          cd=bank.cd[key], grad=False)[0]
+        return tot / len(batches)
+
+    scales = np.linspace(-2.5, 0.5, 13)
+    trial = []
+    for sc in scales:
+        t = th.copy()
+        t[25] = sc
+        trial.append(batch_loss(t))
+    th[25] = float(scales[int(np.argmin(trial))])
+    _log(f"output scale from the line search: exp({th[25]:.2f}), loss {min(trial):.4f}")
+    if eval_fn is not None:
+        sc = eval_fn(VCycle.weights(th[:nv]), tail_options(th[nv:]))
+        evals.append((0, sc))
+        best = (sc[0], th.copy(), 0)
+        _log(f"eval at 0: {sc}")
+    for step in range(steps):
+        if deadline is not None and time.time() > deadline:
+            _log(f"train: time budget reached at step {step}")
+            break
+        key = sizes[int(rng.choice(len(sizes), p=probs))]
+        n = key[0]
+        b = min(BATCH[n], bank.count[key])
+        idx = np.sort(rng.choice(bank.count[key], size=b, replace=False))
+        z = bank.e[key].index_select(0, torch.from_numpy(idx).cuda()).contiguous()
+        off = torch.from_numpy(bank.sys[key][idx] * n * n).cuda()
+        cd = bank.cd[key]
+        net.set_options(opts_of(th, key))
+        net.set_weights(VCycle.weights(th[:nv]))
+        loss, gw = net.spectral_loss_grad(z, bank.ce[key], bank.cn[key], off, b, n, n, k, cd=cd)
+        g = np.zeros(len(th))
+        g[:nv] = VCycle.theta_grad(th[:nv], gw)
+        if train_tail:
+            for i in range(nv, len(th)):
+                tp, tm = th.copy(), th.copy()
+                tp[i] += fd_step
+                tm[i] -= fd_step
+                net.set_options(opts_of(tp, key))
+                lp, _ = net.spectral_loss_grad(z, bank.ce[key], bank.cn[key], off, b, n, n, k,
+                                               cd=cd, grad=False)
+                net.set_options(opts_of(tm, key))
+                lm, _ = net.spectral_loss_grad(z, bank.ce[key], bank.cn[key], off, b, n, n, k,
+                                               cd=cd, grad=False)
+                g[i] = (lp - lm) / (2 * fd_step)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        f = 0.5 * (1.0 + math.cos(math.pi * step / steps))
+        th -= lr * f * (m / (1 - b1 ** (step + 1))) / (np.sqrt(v / (1 - b2 ** (step + 1))) + 1e-8)
+        th[:nv] = VCycle.repair(th[:nv])
+        th[nv:] = repair_tail(th[nv:])
+        losses.append(loss)
+        if step % log_every == 0 or step == steps - 1:
+            _log(f"step {step}: spectral loss {np.mean(losses[-log_every:]):.4f} ({n}^2 x{b}, "
+                 f"{time.time() - t0:.0f} s) omega {np.round(th[0:4], 3).tolist()} "
+                 f"tail {np.round(th[nv:], 3).tolist()} |g| {np.abs(g[:nv]).max():.2e} "
+                 f"{np.abs(g[nv:]).max():.2e}")
+        if eval_fn is not None and eval_every and ((step + 1) % eval_every == 0 or
+                                                   step == steps - 1):
+            sc = eval_fn(VCycle.weights(th[:nv]), tail_options(th[nv:]))
+            evals.append((step + 1, sc))
+            _log(f"eval at {step + 1}: {sc}")
+            if sc[0] < best[0]:
+                best = (sc[0], th.copy(), step + 1)
+    if eval_fn is not None:
+        _log(f"selected the model of step {best[2]} (score {best[0]:.4f})")
+        th = best[1]
+    return th[:nv], th[nv:], losses, evals
+
+
 class Validation:
     """Held-out p' systems for model selection (none is among the training
     cases, which use seed-drawn Re and speeds at 256^2-1024^2): a 256^2 cavity at
@@ -754,6 +897,12 @@ def main(argv=None):
     ap.add_argument("--checkpoints", type=int, default=10,
                     help="validation scores during training (model selection)")
     ap.add_argument("--summary", default=None)
+    ap.add_argument("--loss", choices=["spectral", "anorm"], default="spectral",
+                    help="spectral: the k-step convergence factor of the stationary iteration "
+                         "(V-cycle only); anorm: the one-application A-norm error")
+    ap.add_argument("--k", type=int, default=6, help="steps of the spectral loss")
+    ap.add_argument("--fixed-tail", action="store_true",
+                    help="keep the coarse tail's coefficients at their start")
     args = ap.parse_args(argv)
     if args.time_forward:
         ms = time_forward(args.time_forward, args.passes)
@@ -782,6 +931,24 @@ def main(argv=None):
     if args.free:
         w, losses = train(torch, bank, w0, args.steps, args.lr, args.seed,
                           time.time() + args.train_seconds)
+    elif vcycle and args.loss == "spectral":
+        val = Validation()
+        try:
+            theta, tail, losses, evals = train_spectral(
+                torch, bank, theta0, TAIL_THETA0, args.steps, args.lr, k=args.k, seed=args.seed,
+                deadline=time.time() + args.train_seconds,
+                eval_fn=lambda wts, o: val.score(wts, o),
+                eval_every=max(1, args.steps // args.checkpoints),
+                train_tail=not args.fixed_tail)
+        finally:
+            val.close()
+        summary["theta"] = theta.tolist()
+        summary["tail_theta"] = tail.tolist()
+        summary["evals"] = evals
+        summary["parametrisation"] = "VCycle"
+        summary["loss"] = f"spectral, {args.k} steps"
+        opts = tail_options(tail)
+        w = param.weights(theta)
     else:
         val = Validation()
         try:

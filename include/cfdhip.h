@@ -539,6 +539,22 @@ int cfdhip_unet_loss_grad(cfdhip_unet* net, const float* x, const float* e, cons
                           const float* cn, const float* cd, const int64_t* sys_offset,
                           const float* den, int64_t batch, int64_t h, int64_t w, double* loss,
                           float* grad);
+/* Multi-step spectral loss of the stationary iteration x <- x + N(b - A x) (no
+ * counterpart in the reference): per sample s, with error image z_s and the
+ * operator A_s of cfdhip_unet_loss_grad, v_0 = z_s / |z_s|_A,
+ * v_j = v_{j-1} - N(A_s v_{j-1}) for j = 1..steps, and
+ * L = mean_s log(|v_steps|_A^2) / (2 steps): by Gelfand's formula an estimate of
+ * the log of the iteration's spectral radius, which bounds the convergence of
+ * the preconditioned Krylov solve from above, where the one-application
+ * loss only weighs the average error. grad (host, param_count floats; NULL:
+ * the loss only) is dL/dW, accumulated backwards through the steps with
+ * N^T = N (exact for a self-adjoint network such as the symmetric V-cycle
+ * weights; the Neumann-padded backward pass requires mirror-symmetric
+ * kernels). steps in 1..32. */
+int cfdhip_unet_spectral_loss_grad(cfdhip_unet* net, const float* z, const float* ce,
+                                   const float* cn, const float* cd, const int64_t* sys_offset,
+                                   int64_t batch, int64_t h, int64_t w, int32_t steps,
+                                   double* loss, float* grad);
 /* The same loss; with update != 0 one Adam step (lr, betas 0.9 / 0.999). */
 int cfdhip_unet_train_step(cfdhip_unet* net, const float* x, const float* e, const float* ce,
                            const float* cn, const float* cd, const int64_t* sys_offset,

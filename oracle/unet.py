@@ -444,6 +444,62 @@ def anorm_loss(y, e, ce, cn, den, cd=None):
     return float(per.mean()), 2.0 * ad / (den[:, None, None] * n)
 
 
+def operator_apply(v, ce, cn, cd=None):
+    """B v per image for (N, H, W) arrays: diagonal cd (None: unit), couplings
+    ce (to x + 1) and cn (to y + 1) -- the operator of anorm_loss."""
+    v = np.asarray(v, np.float64)
+    out = v.copy() if cd is None else np.asarray(cd, np.float64) * v
+    out[:, :, :-1] -= ce[:, :, :-1] * v[:, :, 1:]
+    out[:, :, 1:] -= ce[:, :, :-1] * v[:, :, :-1]
+    out[:, :-1, :] -= cn[:, :-1, :] * v[:, 1:, :]
+    out[:, 1:, :] -= cn[:, :-1, :] * v[:, :-1, :]
+    return out
+
+
+def spectral_loss(w, z, ce, cn, steps, cd=None, opts=None, grad=True, device=False):
+    """Multi-step spectral loss of x <- x + N(b - B x): v_0 = z / |z|_B,
+    v_j = v_{j-1} - N(B v_{j-1}), L = mean_s log(|v_steps|_B^2) / (2 steps);
+    with grad, dL/dW by reverse accumulation through the steps (N^T = N, a
+    self-adjoint network). z, ce, cn, cd: (N, H, W). device: the network's
+    passes with the device's rounding points (forward_device,
+    backward_device). Returns (L, dW dict or None)."""
+    z = np.asarray(z, np.float64)
+    n = z.shape[0]
+    if device:
+        def net(x):
+            return forward_device(w, x[..., None], opts=opts)[0][..., 0]
+
+        def grad_of(x, g):
+            _, tape = forward_device(w, x[..., None], opts=opts)
+            return backward_device(w, tape, g[..., None], opts=opts)
+    else:
+        def net(x):
+            return forward(w, x[..., None], opts=opts)[..., 0]
+
+        def grad_of(x, g):
+            _, tape = forward(w, x[..., None], keep=True, opts=opts)
+            return backward(w, tape, g[..., None], opts=opts)[0]
+    bz = operator_apply(z, ce, cn, cd)
+    nz = np.sqrt((z * bz).sum(axis=(1, 2)))[:, None, None]
+    v, bv = [z / nz], [bz / nz]
+    for _ in range(steps):
+        v.append(v[-1] - net(bv[-1]))
+        bv.append(operator_apply(v[-1], ce, cn, cd))
+    raw = (v[-1] * bv[-1]).sum(axis=(1, 2))
+    loss = float(np.mean(np.log(raw)) / (2 * steps))
+    if not grad:
+        return loss, None
+    g = bv[-1] / (steps * n * raw[:, None, None])
+    dw = {name: np.zeros((3, 3, ci, co)) for name, ci, co in LAYERS}
+    for j in range(steps, 0, -1):
+        gj = grad_of(bv[j - 1], g)
+        for name in dw:
+            dw[name] -= gj[name]
+        if j > 1:
+            g = g - operator_apply(net(g), ce, cn, cd)
+    return loss, dw
+
+
 def mse_loss(y, target):
     """L = mean((y - target)^2); returns (L, dL/dy)."""
     d = y - target
