@@ -382,17 +382,37 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
   // state S[k % 2] and writes S[(k + 1) % 2], writes partials P[k % 2] and
   // reads P[(k - 1) % 2]; launch 0 takes S[0] as the finalize left it, and one
   // finalize after the chunk steps S[0] in place from the last partials.
-  const bool fused = buf.state2 != nullptr && snap == nullptr &&
-                     static_cast<int64_t>(grid.x) <= kFusedMaxTiles &&
+  // The fused kernel takes the shortest tiles (4, 8, 16 or 32 rows) that keep
+  // the grid within kFusedMaxTiles blocks: on a small grid the iteration is
+  // latency-bound, and fewer rows per wave shorten each block's chain of loads.
+  int fusedRows = 0;
+  int64_t fusedTiles = 0;
+  for (int rows : {4, 8, 16, 32}) {
+    const int64_t t = tilesX * ((v.ny + rows - 3) / (rows - 2));
+    if (t <= kFusedMaxTiles) {
+      fusedRows = rows;
+      fusedTiles = t;
+      break;
+    }
+  }
+  const bool fused = buf.state2 != nullptr && snap == nullptr && fusedRows > 0 &&
                      buf.partialCap >= 6 * kFusedMaxTiles;
+  const dim3 gridF(static_cast<unsigned>(fused ? fusedTiles : 1));
   KState* Sb[2] = {buf.state, buf.state2};
   double* P[2] = {buf.partial, buf.partial + 3 * kFusedMaxTiles};
   auto launchFused = [&](int cur, int k) {
-    const FuseArgs f{Sb[(k + 1) & 1], P[(k + 1) & 1], static_cast<int64_t>(grid.x), buf.hist,
-                     k > 0 ? 1 : 0};
-    hipLaunchKernelGGL((cgIterKernel<T, 32, T, 3, true>), grid, dim3(256), 0, ctx.stream, g, diag,
-                       coeff, p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur],
-                       wv[cur], wv[1 - cur], x, tilesX, Sb[k & 1], P[k & 1], 0, f);
+    const FuseArgs f{Sb[(k + 1) & 1], P[(k + 1) & 1], fusedTiles, buf.hist, k > 0 ? 1 : 0};
+#define CFDHIP_FUSED(ROWS)                                                                       \
+  hipLaunchKernelGGL((cgIterKernel<T, ROWS, T, 3, true>), gridF, dim3(256), 0, ctx.stream, g,    \
+                     diag, coeff, p[cur], p[1 - cur], s[cur], s[1 - cur], r[cur], r[1 - cur],    \
+                     wv[cur], wv[1 - cur], x, tilesX, Sb[k & 1], P[k & 1], 0, f)
+    switch (fusedRows) {
+      case 4: CFDHIP_FUSED(4); break;
+      case 8: CFDHIP_FUSED(8); break;
+      case 16: CFDHIP_FUSED(16); break;
+      default: CFDHIP_FUSED(32); break;
+    }
+#undef CFDHIP_FUSED
   };
   const bool timing = ctx.timing && buf.events;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -434,8 +454,8 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
       launchFused(cur, k);
       cur = 1 - cur;
     }
-    launchFinalize(ctx, FinalArgs{P[(gChunk - 1) & 1], static_cast<int64_t>(grid.x), Sb[0],
-                                  buf.hist, nullptr, kStepCg1Iter});
+    launchFinalize(ctx, FinalArgs{P[(gChunk - 1) & 1], fusedTiles, Sb[0], buf.hist, nullptr,
+                                  kStepCg1Iter});
   };
   while (true) {
     int q = 0;
