@@ -67,6 +67,21 @@ TailShape tailShape(int h3, int w3, int maxLevels) {
   return s;
 }
 
+// The shape copied from the kernel arguments into LDS by the whole block.
+// Read with a run-time level index, a kernel-argument struct is fetched
+// through the scalar cache from the argument buffer, a cache line at a
+// time and again for every level a pass visits (measured: about 1.6 us per
+// pass of the one-workgroup cycle even on a 1 x 1 level); staged once, every
+// later read is an LDS read.
+__device__ inline void stageShape(const TailShape& s, TailShape& sh) {
+  static_assert(sizeof(TailShape) % 4 == 0, "the shape is copied in 4-byte words");
+  constexpr int kWords = static_cast<int>(sizeof(TailShape) / 4);
+  const int* src = reinterpret_cast<const int*>(&s);
+  int* dst = reinterpret_cast<int*>(&sh);
+  for (int i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+
 // Pixel (y, x) of an h x w image (pixel stride cs) with the network's
 // padding (padSign).
 __device__ inline float at(const float* img, int h, int w, int y, int x, int pad, int cs = 1) {
@@ -213,9 +228,11 @@ __host__ __device__ inline int tailPassPixels(const TailShape& s, int t, int pas
 }
 
 // One pass of a large level over all its pixels of all images.
-__global__ __launch_bounds__(256) void tailLevelKernel(TailShape s, TailParams p, int pad, int t,
+__global__ __launch_bounds__(256) void tailLevelKernel(TailShape sArg, TailParams p, int pad, int t,
                                                        int pass, int64_t pix,
                                                        float* __restrict__ work) {
+  __shared__ TailShape s;
+  stageShape(sArg, s);
   const int64_t m = tailPassPixels(s, t, pass);
   for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < pix;
        q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -226,43 +243,146 @@ __global__ __launch_bounds__(256) void tailLevelKernel(TailShape s, TailParams p
 
 // Levels t0 .. L-1 of one image: down from R_t0, the coarsest level, up to
 // E_t0; base holds the image's levels at the offsets of s.
+// (y, x) moved into the h x w image through the Neumann padding and the
+// factor the pixel there takes (-1 per crossed Dirichlet side): padSign
+// without branches, so a thread issues all of a stencil's reads at once.
+__device__ inline float reflect(int& y, int& x, int h, int w, int pad) {
+  float s = 1.f;
+  s = (y < 0 && (pad & kPadBottom)) ? -s : s;
+  s = (y >= h && (pad & kPadTop)) ? -s : s;
+  s = (x < 0 && (pad & kPadLeft)) ? -s : s;
+  s = (x >= w && (pad & kPadRight)) ? -s : s;
+  y = min(max(y, 0), h - 1);
+  x = min(max(x, 0), w - 1);
+  return s;
+}
+
+// The 3 x 3 patch of img (h x w) around (y, x), padded.
+__device__ inline void patch3(const float* img, int h, int w, int y, int x, int pad,
+                              float (&q)[3][3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      int yy = y - 1 + a, xx = x - 1 + b;
+      const float sg = reflect(yy, xx, h, w, pad);
+      q[a][b] = sg * img[yy * w + xx];
+    }
+}
+
+// Every pixel of an h x w level once: wave v takes rows v, v + waves, ...,
+// lane l columns l, l + 64, ... (no division per pixel).
+template <class F>
+__device__ inline void forPixels(int h, int w, F&& f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  for (int y = wave; y < h; y += waves)
+    for (int x = lane; x < w; x += 64) f(y, x);
+}
+
+// Levels t0 .. L-1 of one image: down from R_t0, the coarsest level, up to
+// E_t0; base holds the image's levels at the offsets of s (minus shift).
+// The same arithmetic, in the same order, as tailPass.
 __device__ inline void cycleLevels(float* base, const TailShape& s, const TailParams& p, int pad,
                                    int t0, int64_t shift) {
   const int L = s.levels;
+  const float om = p.omega;
+  const float a0 = p.a[0], a1 = p.a[1], a2 = p.a[2];
+  const float k0 = p.k[0], k1 = p.k[1], k2 = p.k[2];
+  auto level = [&](int t, int q) { return base + (s.off[3 * t + q] - shift); };
   for (int t = t0; t + 1 < L; ++t) {
-    for (int pass = kPre; pass <= kRestrict; ++pass) {
-      const int m = tailPassPixels(s, t, pass);
-      for (int i = threadIdx.x; i < m; i += blockDim.x)
-        tailPass(base, s, p, pad, t, pass, i, shift);
-      __syncthreads();
-    }
+    const int h = tailH(s, t), w = tailW(s, t);
+    const float* R = level(t, 0);
+    float* S = level(t, 1);
+    // S_t = R_t - omega A R_t
+    forPixels(h, w, [&](int y, int x) {
+      float q[3][3];
+      patch3(R, h, w, y, x, pad, q);
+      S[y * w + x] = R[y * w + x] - om * stencilPatch(q, a0, a1, a2);
+    });
+    __syncthreads();
+    // R_{t+1} = 4 pool(K S_t): the K stencils of the 2 x 2 fine block from
+    // one 4 x 4 patch; fine pixels beyond an odd edge add nothing
+    const int hc = tailH(s, t + 1), wc = tailW(s, t + 1);
+    float* Rc = level(t + 1, 0);
+    forPixels(hc, wc, [&](int y, int x) {
+      float v[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          int yy = 2 * y - 1 + a, xx = 2 * x - 1 + b;
+          const float sg = reflect(yy, xx, h, w, pad);
+          v[a][b] = sg * S[yy * w + xx];
+        }
+      float acc = 0.f;
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          if (2 * y + dy >= h || 2 * x + dx >= w) continue;
+          float q[3][3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) q[a][b] = v[dy + a][dx + b];
+          acc += stencilPatch(q, k0, k1, k2);
+        }
+      Rc[y * wc + x] = acc;
+    });
+    __syncthreads();
   }
   // coarsest level: the two sweeps without a coarse correction,
   // e = omega r + (I - omega A) omega r
   {
-    const float om = p.omega;
-    const float* R = base + (s.off[3 * (L - 1)] - shift);
-    float* E = base + (s.off[3 * (L - 1) + 2] - shift);
     const int h = tailH(s, L - 1), w = tailW(s, L - 1);
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-      const float ar = stencil3(R, h, w, i / w, i % w, p.a[0], p.a[1], p.a[2], pad);
-      E[i] = 2.f * om * R[i] - om * om * ar;
-    }
+    const float* R = level(L - 1, 0);
+    float* E = level(L - 1, 2);
+    forPixels(h, w, [&](int y, int x) {
+      float q[3][3];
+      patch3(R, h, w, y, x, pad, q);
+      const float ar = stencilPatch(q, a0, a1, a2);
+      E[y * w + x] = 2.f * om * R[y * w + x] - om * om * ar;
+    });
   }
   __syncthreads();
   for (int t = L - 2; t >= t0; --t) {
-    for (int pass = kProlong; pass <= kPost; ++pass) {
-      const int m = tailPassPixels(s, t, pass);
-      for (int i = threadIdx.x; i < m; i += blockDim.x)
-        tailPass(base, s, p, pad, t, pass, i, shift);
-      __syncthreads();
-    }
+    const int h = tailH(s, t), w = tailW(s, t);
+    const int wc = tailW(s, t + 1);
+    const float* R = level(t, 0);
+    float* S = level(t, 1);
+    float* E = level(t, 2);
+    const float* Ec = level(t + 1, 2);
+    // S_t = omega R_t + K up(E_{t+1})
+    forPixels(h, w, [&](int y, int x) {
+      float q[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          int yy = y - 1 + a, xx = x - 1 + b;
+          const float sg = reflect(yy, xx, h, w, pad);
+          q[a][b] = sg * Ec[(yy >> 1) * wc + (xx >> 1)];
+        }
+      S[y * w + x] = om * R[y * w + x] + stencilPatch(q, k0, k1, k2);
+    });
+    __syncthreads();
+    // E_t = S_t - omega A S_t + omega R_t
+    forPixels(h, w, [&](int y, int x) {
+      float q[3][3];
+      patch3(S, h, w, y, x, pad, q);
+      E[y * w + x] = S[y * w + x] - om * stencilPatch(q, a0, a1, a2) + om * R[y * w + x];
+    });
+    __syncthreads();
   }
 }
 
 // One image per workgroup, its levels in the global workspace.
-__global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, TailParams p, int pad,
-                                                               int t0, float* __restrict__ work) {
+__global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape sArg, TailParams p,
+                                                               int pad, int t0,
+                                                               float* __restrict__ work) {
+  __shared__ TailShape s;
+  stageShape(sArg, s);
   cycleLevels(work + static_cast<int64_t>(blockIdx.x) * s.perImage, s, p, pad, t0, 0);
 }
 
@@ -270,10 +390,12 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleKernel(TailShape s, Tai
 // pass then waits on LDS instead of L2. Levels t0.. hold at most
 // 3 x 4/3 x kTailCyclePixels floats.
 constexpr int kTailLdsFloats = 16384;
-__global__ __launch_bounds__(kTailThreads) void tailCycleLdsKernel(TailShape s, TailParams p,
+__global__ __launch_bounds__(kTailThreads) void tailCycleLdsKernel(TailShape sArg, TailParams p,
                                                                   int pad, int t0,
                                                                   float* __restrict__ work) {
-  __shared__ float lds[kTailLdsFloats];
+  __shared__ float lds[kTailLdsFloats];  // with the shape: 64.5 KiB (gfx950: 160 KiB per block)
+  __shared__ TailShape s;
+  stageShape(sArg, s);
   float* g = work + static_cast<int64_t>(blockIdx.x) * s.perImage;
   const int64_t o0 = s.off[3 * t0];  // LDS holds the workspace from o0 on
   const int m0 = tailH(s, t0) * tailW(s, t0);
