@@ -667,9 +667,12 @@ WEIGHTS_PATH = __import__("os").path.join(_lib.HERE, "weights", "pprime_unet.f32
 class BatchSolver:
     """Many independent 2D p' systems of uneven size solved together
     (cfdhip_batch_*): Jacobi-PCG, one launch per step for the whole batch,
-    FP32 inner iterations inside FP64 refinement (precision "mixed") or FP64.
-    No reference counterpart; each case meets the same FP64 true-residual
-    test as a single solve (src/algebra/CG.cpp:14-148)."""
+    FP32 inner iterations inside FP64 refinement (precision "mixed") or FP64;
+    or, after set_learned(), flexible CG with the learned preconditioner
+    (settings type "FCG", preconditioner "Learned", FP64 vectors), each step
+    one launch over all active cases. No reference counterpart; each case
+    meets the same FP64 true-residual test as a single solve
+    (src/algebra/CG.cpp:14-148)."""
 
     def __init__(self, device=0):
         self.lib = _lib.load()
@@ -714,6 +717,18 @@ class BatchSolver:
         self._call("cfdhip_batch_add_pressure", d._ctx.h, ctypes.byref(idx))
         self.shapes.append(tuple(d.shape[:2]))
         return idx.value
+
+    def set_learned(self, weights=None, options=None):
+        """The learned preconditioner of the batched FCG solve (the shipped
+        weights and their options by default)."""
+        if weights is None:
+            weights = load_learned_weights()
+            if options is None:
+                options = load_learned_options()
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        o = _unet_options(options)
+        self._call("cfdhip_batch_set_learned", w.ctypes.data_as(_lib.P_F32), w.size,
+                   ctypes.byref(o))
 
     def solve(self, settings: LinearSolverSettings, tile_rows=0, use_graph=True):
         """Returns a list of SolverResult (solution None; device_ms = the case's

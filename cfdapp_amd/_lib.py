@@ -204,6 +204,8 @@ SIGNATURES = {
                                           ctypes.c_int, ctypes.POINTER(SolveResult), P_F64,
                                           P_F64]),
     "cfdhip_batch_solution": (ctypes.c_int, [VP, ctypes.c_int64, P_F64, ctypes.c_int64]),
+    "cfdhip_batch_set_learned": (ctypes.c_int, [VP, P_F32, ctypes.c_int64,
+                                                ctypes.POINTER(UnetOptions)]),
 }
 
 _lib = None

@@ -71,7 +71,16 @@ int guardedNet(cfdhip_unet* h, F&& fn) {
   }
 }
 
-TailParams tailFrom(const cfdhip_unet_options& o) {
+
+void checkShape(int64_t batch, int64_t h, int64_t w) {
+  require(batch > 0 && h >= 8 && w >= 8 && h % 8 == 0 && w % 8 == 0,
+          "the network grid must be a positive multiple of 8 in each direction");
+  require(batch * h * w < (int64_t{1} << 31), "the network batch is too large");
+}
+
+}  // namespace
+
+TailParams cfdhip::tailFrom(const cfdhip_unet_options& o) {
   TailParams t{};
   t.levels = o.tail_levels;
   t.omega = o.tail_omega;
@@ -83,14 +92,6 @@ TailParams tailFrom(const cfdhip_unet_options& o) {
   t.dst = 2;
   return t;
 }
-
-void checkShape(int64_t batch, int64_t h, int64_t w) {
-  require(batch > 0 && h >= 8 && w >= 8 && h % 8 == 0 && w % 8 == 0,
-          "the network grid must be a positive multiple of 8 in each direction");
-  require(batch * h * w < (int64_t{1} << 31), "the network batch is too large");
-}
-
-}  // namespace
 
 extern "C" {
 

@@ -31,7 +31,7 @@
 #include <type_traits>
 #include <vector>
 
-#include "context.hpp"
+#include "batch.hpp"
 
 using namespace cfdhip;
 
@@ -56,13 +56,6 @@ struct CaseT {
   double* x;
   const T *diagT, *coeffT;
   T *r[2], *w[2], *p[2], *s[2], *xT;
-};
-
-struct CaseState {
-  double b0, rTrue, absTol, relTol, innerFloor, innerTol;
-  double ib0, rz, alpha, beta, rnorm;
-  int64_t iter, total, cycles, maxIter, steps, doneStep;
-  int32_t phase, cur, status, done;
 };
 
 // Row of A applied at (i, j), ascending column order from 0.
@@ -352,37 +345,6 @@ constexpr int kRowChoices[3] = {16, 32, 64};
 
 }  // namespace
 }  // namespace cfdhip
-
-// The batch: systems added one by one (FP64, device-resident), solved together.
-struct cfdhip_batch {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string lastError;
-  struct Sys {
-    int64_t nx, ny, pin;
-    DeviceArray<double> diag, coeff, b;
-  };
-  std::vector<std::unique_ptr<Sys>> sys;
-  // solve-time buffers
-  DeviceArray<double> arena64;
-  DeviceArray<float> arena32;
-  DeviceArray<unsigned char> caseBuf;
-  DeviceArray<CaseState> states;
-  DeviceArray<double> partial;
-  DeviceArray<int32_t> blockCase, blockTile, active;
-  DeviceArray<int64_t> tile0, tiles;
-  CaseState* hostStates = nullptr;
-  int32_t* hostTable = nullptr;  // pinned staging of the block table and active list
-  int64_t hostTableCap = 0;
-  int lastRows = 0;
-  std::vector<int64_t> xOff;  // solution of each case in arena64 (after a solve)
-
-  ~cfdhip_batch() {
-    if (hostStates) (void)hipHostFree(hostStates);
-    if (hostTable) (void)hipHostFree(hostTable);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
 
 namespace cfdhip {
 namespace {
@@ -794,7 +756,47 @@ int guardedBatch(cfdhip_batch* b, F&& fn) {
   }
 }
 
+// Sides whose every cell (but a pinned one) has a diagonal larger than the sum
+// of its internal-face couplings: a FixedValue pressure patch there adds its
+// face coupling to the diagonal, a zero-gradient wall adds nothing (bits 1
+// left, 2 right, 4 bottom, 8 top, the 2D patch order).
+int inferFixedValueSides(int64_t nx, int64_t ny, int64_t pin, const double* diag,
+                         const double* coeff) {
+  const int64_t offY = (nx + 1) * ny;
+  auto excess = [&](int64_t i, int64_t j) {
+    const int64_t c = j * nx + i;
+    double s = 0.0;
+    if (i > 0) s += coeff[j * (nx + 1) + i];
+    if (i < nx - 1) s += coeff[j * (nx + 1) + i + 1];
+    if (j > 0) s += coeff[offY + j * nx + i];
+    if (j < ny - 1) s += coeff[offY + (j + 1) * nx + i];
+    return c == pin || diag[c] - s > 1e-12 * std::fabs(diag[c]);
+  };
+  int sides = 0;
+  bool all[4] = {true, true, true, true};
+  for (int64_t j = 0; j < ny; ++j) {
+    all[0] = all[0] && excess(0, j);
+    all[1] = all[1] && excess(nx - 1, j);
+  }
+  for (int64_t i = 0; i < nx; ++i) {
+    all[2] = all[2] && excess(i, 0);
+    all[3] = all[3] && excess(i, ny - 1);
+  }
+  for (int q = 0; q < 4; ++q)
+    if (all[q]) sides |= 1 << q;
+  return sides;
+}
+
 }  // namespace
+
+void solveBatchJacobi(cfdhip_batch& B, const cfdhip_solver_settings& set, int rows, int useGraph,
+                      cfdhip_solve_result* results, double* latencyMs, double* stats) {
+  if (set.precision == CFDHIP_MIXED)
+    solveBatch<float>(B, set, rows, useGraph, results, latencyMs, stats);
+  else
+    solveBatch<double>(B, set, rows, useGraph, results, latencyMs, stats);
+}
+
 }  // namespace cfdhip
 
 extern "C" {
@@ -855,6 +857,7 @@ int cfdhip_batch_add_system(cfdhip_batch* b, int64_t nx, int64_t ny, int64_t pin
                                 hipMemcpyHostToDevice, B.stream));
     CFDHIP_CHECK(hipMemcpyAsync(s->b.get(), rhs, n * sizeof(double), hipMemcpyHostToDevice,
                                 B.stream));
+    s->sides = inferFixedValueSides(nx, ny, pin, diag, face_coeff);
     CFDHIP_CHECK(hipStreamSynchronize(B.stream));
     B.sys.push_back(std::move(s));
     if (index) *index = static_cast<int64_t>(B.sys.size()) - 1;
@@ -875,6 +878,8 @@ int cfdhip_batch_add_pressure(cfdhip_batch* b, cfdhip_ctx* ctx, int64_t* index) 
     s->nx = nx;
     s->ny = ny;
     s->pin = c.pin;
+    for (int p = 0; p < 4 && p < c.view.npatch; ++p)
+      if (c.view.patch[p].pkind == CFDHIP_FIXED_VALUE) s->sides |= 1 << p;
     s->diag.ensure(n);
     s->coeff.ensure(nf);
     s->b.ensure(n);
@@ -896,16 +901,36 @@ int cfdhip_batch_solve(cfdhip_batch* b, const cfdhip_solver_settings* s, int til
                        double* stats) {
   if (s == nullptr || results == nullptr) return CFDHIP_ERR_INVALID;
   return guardedBatch(b, [&](cfdhip_batch& B) {
-    require(s->type == CFDHIP_SOLVER_CG && s->preconditioner == CFDHIP_PREC_JACOBI,
-            "batched solves run CG with the Jacobi preconditioner");
+    const bool learned = s->type == CFDHIP_SOLVER_FCG && s->preconditioner == CFDHIP_PREC_LEARNED;
+    require(learned || (s->type == CFDHIP_SOLVER_CG && s->preconditioner == CFDHIP_PREC_JACOBI),
+            "batched solves run CG with the Jacobi preconditioner or flexible CG with the "
+            "learned one");
     require(s->absolute_tolerance >= 0.0 && s->relative_tolerance >= 0.0,
             "tolerances must be >= 0");
     require(s->inner_tolerance >= 0.0 && s->inner_tolerance < 1.0,
             "LinearSolverSettings: inner_tolerance must be in [0, 1)");
-    if (s->precision == CFDHIP_MIXED)
-      solveBatch<float>(B, *s, tile_rows, use_graph, results, latency_ms, stats);
+    if (learned)
+      solveBatchLearned(B, *s, use_graph, results, latency_ms, stats);
     else
-      solveBatch<double>(B, *s, tile_rows, use_graph, results, latency_ms, stats);
+      solveBatchJacobi(B, *s, tile_rows, use_graph, results, latency_ms, stats);
+  });
+}
+
+int cfdhip_batch_set_learned(cfdhip_batch* b, const float* weights, int64_t count,
+                             const cfdhip_unet_options* options) {
+  return guardedBatch(b, [&](cfdhip_batch& B) {
+    require(weights != nullptr && count == kUnetParams,
+            "learned weights: expected cfdhip_unet_param_count() floats");
+    require(options != nullptr && options->padding == 1,
+            "the batched learned solve needs options with Neumann padding (padding 1)");
+    if (!B.net) B.net = std::make_unique<Unet>(B.stream);
+    B.net->setDense(false);
+    B.net->setOptions(options->padding, 0, tailFrom(*options));
+    B.net->setWeights(weights);
+    CFDHIP_CHECK(hipStreamSynchronize(B.stream));
+    require(B.net->narrowActive(),
+            "the batched learned solve runs the narrow kernels: the weights must leave at most "
+            "4 live channels per tensor");
   });
 }
 

@@ -230,6 +230,13 @@ cfdhip_solve_result mixedPcg3d(Context& ctx, const cfdhip_solver_settings& set, 
 cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set, const double* b,
                                  double* x);
 void loadLearnedWeights(Context& ctx, const float* weights);
+// the learned preconditioner's scaling S of a Neumann-padded network on a 2D
+// p' system (nx x ny, face couplings at x-faces then y-faces from offY):
+// S^-2 the interior-equivalent diagonal geometric-mean filtered, D^-1/2 at
+// the pinned cell; tmp0, tmp1: two work images
+void learnedScaling(hipStream_t stream, int64_t nx, int64_t ny, int64_t offY, int64_t pin,
+                    const double* diag, const double* coeff, float* tmp0, float* tmp1,
+                    float* out);
 
 // simple.hip
 // piso.hip: PISO time steps on the resident state; meta 6 per step

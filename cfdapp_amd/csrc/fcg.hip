@@ -314,6 +314,17 @@ InnerResult innerFcg(Context& ctx, const OpT<T>& op, double relTol, int64_t maxI
 
 }  // namespace
 
+void learnedScaling(hipStream_t stream, int64_t nx, int64_t ny, int64_t offY, int64_t pin,
+                    const double* diag, const double* coeff, float* tmp0, float* tmp1,
+                    float* out) {
+  const dim3 g = stride(nx * ny);
+  hipLaunchKernelGGL(logInteriorDiagKernel, g, dim3(256), 0, stream, nx, ny, offY, diag, coeff,
+                     tmp0);
+  hipLaunchKernelGGL(boxMeanKernel, g, dim3(256), 0, stream, nx, ny, 0, tmp0, tmp1, 0, pin, diag);
+  hipLaunchKernelGGL(boxMeanKernel, g, dim3(256), 0, stream, nx, ny, 1, tmp1, out, 1, pin, diag);
+  CFDHIP_CHECK(hipGetLastError());
+}
+
 void loadLearnedWeights(Context& ctx, const float* weights) {
   if (!ctx.net) ctx.net = std::make_unique<Unet>(ctx.stream);
   ctx.net->setWeights(weights);
@@ -365,15 +376,9 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
     ctx.net->setSides(sides);
     w.scaleTmp[0].ensure(n);
     w.scaleTmp[1].ensure(n);
-    hipLaunchKernelGGL(logInteriorDiagKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, v.offY,
-                       ctx.f(CFDHIP_FIELD_PC_DIAG), ctx.f(CFDHIP_FIELD_PC_FACE_COEFF),
-                       w.scaleTmp[0].get());
-    hipLaunchKernelGGL(boxMeanKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, 0,
-                       w.scaleTmp[0].get(), w.scaleTmp[1].get(), 0, ctx.pin,
-                       ctx.f(CFDHIP_FIELD_PC_DIAG));
-    hipLaunchKernelGGL(boxMeanKernel, g, dim3(256), 0, ctx.stream, v.nx, v.ny, 1,
-                       w.scaleTmp[1].get(), w.dis32.get(), 1, ctx.pin,
-                       ctx.f(CFDHIP_FIELD_PC_DIAG));
+    learnedScaling(ctx.stream, v.nx, v.ny, v.offY, ctx.pin, ctx.f(CFDHIP_FIELD_PC_DIAG),
+                   ctx.f(CFDHIP_FIELD_PC_FACE_COEFF), w.scaleTmp[0].get(), w.scaleTmp[1].get(),
+                   w.dis32.get());
   } else
     hipLaunchKernelGGL(invSqrtKernel, g, dim3(256), 0, ctx.stream, n, ctx.f(CFDHIP_FIELD_PC_DIAG),
                        w.dis32.get());

@@ -382,8 +382,50 @@ __global__ __launch_bounds__(kFinThreads) void finalizeKernel(FinalArgs a) {
 
 }  // namespace
 
+// The scalar steps of many independent solves at once (a batch of cases):
+// block b runs case active[b]: it sums that case's W partial streams over its
+// slots slot0 .. slot0 + slots - 1 in slot order and steps its state (no
+// history). kStepNorm writes out[case].
+__global__ __launch_bounds__(256) void caseFinalizeKernel(const double* __restrict__ partial,
+                                                          const int64_t* __restrict__ slot0,
+                                                          const int64_t* __restrict__ slots,
+                                                          const int32_t* __restrict__ active,
+                                                          KState* states, int W, int stepId,
+                                                          double* out) {
+  __shared__ double lds[6][256 / 64];
+  const int c = active[blockIdx.x];
+  KState& s = states[c];
+  if (stepId != kStepNorm && s.done) return;  // uniform: one case per block
+  double x[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int64_t b0 = slot0[c], nb = slots[c];
+  for (int64_t k = threadIdx.x; k < nb; k += blockDim.x)
+    for (int w = 0; w < W; ++w) x[w] += partial[(b0 + k) * W + w];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int w = 0; w < W; ++w) {
+    double y = x[w];
+    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+    if (lane == 0) lds[w][wid] = y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < W; ++w)
+      for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t[w] += lds[w][i];
+    step(s, nullptr, out != nullptr ? out + c : nullptr, stepId, t);
+  }
+}
+
 void launchFinalize(Context& ctx, const FinalArgs& a) {
   hipLaunchKernelGGL(finalizeKernel, dim3(1), dim3(kFinThreads), 0, ctx.stream, a);
+}
+
+void launchCaseFinalize(hipStream_t stream, const double* partial, const int64_t* slot0,
+                        const int64_t* slots, const int32_t* active, int64_t nActive,
+                        KState* states, int W, int stepId, double* out) {
+  require(W >= 1 && W <= 6, "case finalize: 1 to 6 sums");
+  if (nActive <= 0) return;
+  hipLaunchKernelGGL(caseFinalizeKernel, dim3(static_cast<unsigned>(nActive)), dim3(256), 0,
+                     stream, partial, slot0, slots, active, states, W, stepId, out);
 }
 
 }  // namespace cfdhip

@@ -170,5 +170,10 @@ struct FinalArgs {
 };
 
 void launchFinalize(Context& ctx, const FinalArgs& a);
+// The finalize of many independent solves (one block per active case, its
+// partials in slots slot0[case] .. slot0[case] + slots[case] - 1).
+void launchCaseFinalize(hipStream_t stream, const double* partial, const int64_t* slot0,
+                        const int64_t* slots, const int32_t* active, int64_t nActive,
+                        KState* states, int W, int stepId, double* out);
 
 }  // namespace cfdhip

@@ -31,7 +31,7 @@ namespace cfdhip {
 
 namespace {
 
-constexpr int kNTH = 16, kNTW = 64;                  // output tile
+constexpr int kNTH = kNarrowTileH, kNTW = kNarrowTileW;  // output tile
 constexpr int kNHR = kNTH + 2, kNHC = kNTW + 2;      // its input with the halo
 constexpr int kNHP = kNHR * kNHC;
 constexpr int kNXR = kNTH + 4, kNXC = kNTW + 4;      // the fused first layer's scalar input
@@ -53,7 +53,77 @@ struct NarrowArgs {
   const float* scale;
   int srcH, srcW;
   int64_t srcBatch;
+  // batched cases of uneven size (cases != null): block entry i of tab is
+  // (case, tile of the case at this level); a, b, out, aux are the
+  // activation arena and tA, tB, tOut, tAux the tensors read and written
+  // (their per-case offsets in NarrowCase::tOff); src and scale are the
+  // batch's solver vectors (per-case offset NarrowCase::cells)
+  const NarrowCase* cases;
+  const int2* tab;
+  int64_t tabCount;
+  int level, tA, tB, tOut, tAux;
 };
+
+// One tile's image: its tensors, extents and padding.
+struct NarrowTile {
+  const float* a;
+  const float* b;
+  float* out;
+  float* aux;
+  const void* src;
+  const float* scale;
+  int h, w, ph, pw, pad, srcH, srcW, x0, y0;
+  int64_t srcRow0;  // src / scale index of the image's first pixel
+};
+
+// Tile tb of the launch (after the XCD-aware reordering): the image it
+// belongs to and where the tile sits in it.
+template <int CA, int CB, int CO, int G>
+__device__ inline NarrowTile narrowTile(const NarrowArgs& a, int64_t t) {
+  NarrowTile q;
+  if (a.cases != nullptr) {
+    const int2 e = a.tab[t];
+    const NarrowCase& c = a.cases[e.x];
+    const int L = a.level;
+    q.h = c.h[L];
+    q.w = c.w[L];
+    q.ph = c.ph[L];
+    q.pw = c.pw[L];
+    q.pad = c.pad;
+    q.x0 = (e.y % c.tilesX[L]) * kNTW;
+    q.y0 = (e.y / c.tilesX[L]) * kNTH;
+    q.a = a.a != nullptr && a.tA >= 0 ? a.a + c.act + c.tOff[a.tA] : nullptr;
+    q.b = a.b != nullptr && a.tB >= 0 ? a.b + c.act + c.tOff[a.tB] : nullptr;
+    q.out = a.out + c.act + c.tOff[a.tOut];
+    q.aux = a.aux != nullptr && a.tAux >= 0 ? a.aux + c.act + c.tOff[a.tAux] : nullptr;
+    q.src = a.src;
+    q.scale = a.scale;
+    q.srcH = c.ny;
+    q.srcW = c.nx;
+    q.srcRow0 = c.cells;
+  } else {
+    q.h = a.h;
+    q.w = a.w_;
+    q.ph = a.ph;
+    q.pw = a.pw;
+    q.pad = a.pad;
+    q.x0 = static_cast<int>(t % a.tilesX) * kNTW;
+    const int64_t rest = t / a.tilesX;
+    q.y0 = static_cast<int>(rest % a.tilesY) * kNTH;
+    const int64_t b = rest / a.tilesY;
+    const int64_t hw = static_cast<int64_t>(a.h) * a.w_, hw2 = hw / 4;
+    q.a = a.a == nullptr ? nullptr : a.a + b * (G == kNConcat ? hw2 : hw) * CA;
+    q.b = a.b == nullptr ? nullptr : a.b + b * hw * CB;
+    q.out = a.out + b * hw * CO;
+    q.aux = a.aux == nullptr ? nullptr : a.aux + b * hw2 * CO;
+    q.src = a.src;
+    q.scale = a.scale;
+    q.srcH = a.srcH;
+    q.srcW = a.srcW;
+    q.srcRow0 = b * a.srcBatch;
+  }
+  return q;
+}
 
 template <int CW>
 struct Vec;
@@ -130,31 +200,28 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
   for (int i = threadIdx.x; i < 9 * CI * CO; i += blockDim.x) ws[i] = a.w[i];
   if (G == kNFirst)
     for (int i = threadIdx.x; i < 9 * CA; i += blockDim.x) w0s[i] = a.w0[i];
-  const int h = a.h, wd = a.w_;
-  const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
+  const int64_t tiles =
+      a.cases != nullptr ? a.tabCount : static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
   for (int64_t tb = blockIdx.x; tb < tiles; tb += gridDim.x) {
-    const int64_t t = xcdTile(tb, tiles);
-    const int x0 = static_cast<int>(t % a.tilesX) * kNTW;
-    const int64_t rest = t / a.tilesX;
-    const int y0 = static_cast<int>(rest % a.tilesY) * kNTH;
-    const int b = static_cast<int>(rest / a.tilesY);
+    const NarrowTile q = narrowTile<CA, CB, CO, G>(a, xcdTile(tb, tiles));
+    const int h = q.h, wd = q.w, x0 = q.x0, y0 = q.y0;
     __syncthreads();  // the weights are in; the previous tile's readers are done
     if constexpr (G == kNFirst) {
       // x = src * scale on rows y0 - 2 .. y0 + kNTH + 1, padded at the physical image
-      const TS* src = static_cast<const TS*>(a.src);
-      const int sh = a.srcH < h ? a.srcH : h, sw = a.srcW < wd ? a.srcW : wd;
+      const TS* src = static_cast<const TS*>(q.src);
+      const int sh = q.srcH < h ? q.srcH : h, sw = q.srcW < wd ? q.srcW : wd;
       constexpr int XPER = (kNXR * kNXC + 255) / 256;
       float xv[XPER], xsg[XPER];
 #pragma unroll
       for (int k = 0; k < XPER; ++k) {
         const int i = threadIdx.x + k * 256;
         int yy = y0 - 2 + i / kNXC, xx = x0 - 2 + i % kNXC;
-        xsg[k] = padFactor(yy, xx, sh, sw, a.pad);
+        xsg[k] = padFactor(yy, xx, sh, sw, q.pad);
         xv[k] = 0.f;
         if (i < kNXR * kNXC) {
-          const int64_t si = b * a.srcBatch + static_cast<int64_t>(yy) * a.srcW + xx;
+          const int64_t si = q.srcRow0 + static_cast<int64_t>(yy) * q.srcW + xx;
           const float sv = static_cast<float>(src[si]);
-          xv[k] = a.scale != nullptr ? sv * a.scale[si] : sv;
+          xv[k] = q.scale != nullptr ? sv * q.scale[si] : sv;
         }
       }
 #pragma unroll
@@ -172,7 +239,7 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
         float acc[CA];
 #pragma unroll
         for (int c = 0; c < CA; ++c) acc[c] = 0.f;
-        const float sg = padSign(yy, xx, a.ph, a.pw, a.pad);
+        const float sg = padSign(yy, xx, q.ph, q.pw, q.pad);
         if (sg != 0.f) {
           const int ry = yy - y0 + 1, rx = xx - x0 + 1;
 #pragma unroll
@@ -192,17 +259,16 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
       for (int k = 0; k < PER; ++k) {
         const int i = threadIdx.x + k * 256;
         int yy = y0 - 1 + i / kNHC, xx = x0 - 1 + i % kNHC;
-        sg[k] = padFactor(yy, xx, a.ph, a.pw, a.pad);
+        sg[k] = padFactor(yy, xx, q.ph, q.pw, q.pad);
 #pragma unroll
         for (int c = 0; c < CI; ++c) v[k][c] = 0.f;
         if (i < kNHP) {
           if constexpr (G == kNConcat) {
-            const int hh = h >> 1, hw = wd >> 1;
-            Vec<CA>::load(a.a + ((static_cast<int64_t>(b) * hh + (yy >> 1)) * hw + (xx >> 1)) * CA,
-                          v[k]);
-            Vec<CB>::load(a.b + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * CB, v[k] + CA);
+            const int hw = wd >> 1;
+            Vec<CA>::load(q.a + (static_cast<int64_t>(yy >> 1) * hw + (xx >> 1)) * CA, v[k]);
+            Vec<CB>::load(q.b + (static_cast<int64_t>(yy) * wd + xx) * CB, v[k] + CA);
           } else {
-            Vec<CA>::load(a.a + ((static_cast<int64_t>(b) * h + yy) * wd + xx) * CA, v[k]);
+            Vec<CA>::load(q.a + (static_cast<int64_t>(yy) * wd + xx) * CA, v[k]);
           }
         }
       }
@@ -248,11 +314,11 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
 #pragma unroll
       for (int dx = 0; dx < 2; ++dx)
         if (yb + dy < h && xb + dx < wd)
-          Vec<CO>::store(a.out + ((static_cast<int64_t>(b) * h + yb + dy) * wd + xb + dx) * CO,
+          Vec<CO>::store(q.out + (static_cast<int64_t>(yb + dy) * wd + xb + dx) * CO,
                          acc[dy * 2 + dx]);
     if constexpr (POOL) {
       // pixels beyond an odd physical edge add nothing (as the MFMA epilogue)
-      const bool rowOut = yb + 1 >= a.ph, colOut = xb + 1 >= a.pw;
+      const bool rowOut = yb + 1 >= q.ph, colOut = xb + 1 >= q.pw;
       const int hh = h >> 1, hw = wd >> 1, py = yb >> 1, px = xb >> 1;
       float s[CO];
 #pragma unroll
@@ -263,7 +329,7 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
         s[c] = ((acc[0][c] + v1) + (v2 + v3)) * 0.25f;
       }
       if (py < hh && px < hw)
-        Vec<CO>::store(a.aux + ((static_cast<int64_t>(b) * hh + py) * hw + px) * CO, s);
+        Vec<CO>::store(q.aux + (static_cast<int64_t>(py) * hw + px) * CO, s);
     }
   }
 }
@@ -330,6 +396,61 @@ __global__ __launch_bounds__(256) void narrowLastKernel(const float* __restrict_
   }
 }
 
+// The output layer of a batched application: block entry e = (case, first
+// row, rows, partial slot) computes z on those rows of the case's solver
+// grid, z = scale conv(d0) and z = r scale^2 at the pinned cell (as
+// narrowLastKernel with APPLY 1), and the block's partial of (z, q) into
+// partial[slot] (q null: none).
+template <int CW>
+__global__ __launch_bounds__(256) void narrowLastBatchKernel(
+    const float* __restrict__ act, const NarrowCase* __restrict__ cases,
+    const int4* __restrict__ blocks, const float* __restrict__ w, const float* __restrict__ scale,
+    const double* __restrict__ r, const double* __restrict__ q, double* __restrict__ z,
+    double* partial) {
+  __shared__ float ws[9 * CW];
+  __shared__ double red[256 / 64];
+  if (threadIdx.x < 9 * CW) ws[threadIdx.x] = w[threadIdx.x];
+  __syncthreads();
+  const int4 e = blocks[blockIdx.x];
+  const NarrowCase& c = cases[e.x];
+  const float* d0 = act + c.act + c.tOff[kTd0];
+  const int wd = c.w[0], ph = c.ny, pw = c.nx, pad = c.pad;
+  double acc = 0.0;
+  const int rowEnd = min(e.y + e.z, ph);
+  for (int yy = e.y; yy < rowEnd; ++yy)
+    for (int xx = threadIdx.x; xx < pw; xx += blockDim.x) {
+      const int64_t lp = static_cast<int64_t>(yy) * pw + xx, p = c.cells + lp;
+      float v[9][CW], sg[9];
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        int ty = yy + tap / 3 - 1, tx = xx + tap % 3 - 1;
+        sg[tap] = padFactor(ty, tx, ph, pw, pad);
+        Vec<CW>::load(d0 + (static_cast<int64_t>(ty) * wd + tx) * CW, v[tap]);
+      }
+      float o = 0.f;
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        float dot = 0.f;
+#pragma unroll
+        for (int k = 0; k < CW; ++k) dot += signed_(sg[tap], v[tap][k]) * ws[tap * CW + k];
+        o += dot;
+      }
+      double zv = static_cast<double>(o * scale[p]);
+      if (lp == c.pin) zv = r[p] * static_cast<double>(scale[p]) * static_cast<double>(scale[p]);
+      z[p] = zv;
+      if (q != nullptr) acc += zv * q[p];
+    }
+  if (q == nullptr || partial == nullptr) return;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t += red[i];
+    partial[e.w] = t;
+  }
+}
+
 // BF16 round to nearest even, as the device's f2bf (NaN stays NaN).
 float roundBfHost(float f) {
   uint32_t u;
@@ -374,7 +495,8 @@ void launchCombo(hipStream_t s, int gather, bool pool, bool srcDouble, dim3 grid
 // ca, cb, co: stored widths (2 or 4; cb only for the concatenation)
 void launchNarrowConv(hipStream_t s, int ca, int cb, int co, int gather, bool pool,
                       bool srcDouble, const NarrowArgs& a) {
-  const int64_t tiles = static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
+  const int64_t tiles =
+      a.cases != nullptr ? a.tabCount : static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
   const dim3 grid(static_cast<unsigned>(std::min<int64_t>(tiles, 1 << 20)));
   const int key = (ca == 4 ? 4 : 0) | (cb == 4 ? 2 : 0) | (co == 4 ? 1 : 0);
   switch (key) {
@@ -627,5 +749,95 @@ template void Unet::outputApplyNarrow<float>(const float*, const float*, const f
                                              int, int64_t, int, int, double*, int);
 template void Unet::outputApplyNarrow<double>(const float*, const double*, const double*, double*,
                                               int, int, int64_t, int, int, double*, int);
+
+int64_t Unet::narrowBatchLayout(std::vector<NarrowCase>& cases) const {
+  require(narrow.usable && !dense_,
+          "batched inference runs the narrow kernels: the weights must leave at most 4 live "
+          "channels per tensor");
+  // level of each forward tensor (e0a is fused into e0b and not stored)
+  constexpr int kLevel[kNumForward] = {-1, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 2, 2, 1, 1, 0, 0};
+  int64_t off = 0;
+  for (NarrowCase& c : cases) {
+    c.h[0] = (c.ny + 7) / 8 * 8;
+    c.w[0] = (c.nx + 7) / 8 * 8;
+    c.ph[0] = c.ny;
+    c.pw[0] = c.nx;
+    for (int l = 1; l < 4; ++l) {
+      c.h[l] = c.h[l - 1] / 2;
+      c.w[l] = c.w[l - 1] / 2;
+      c.ph[l] = (c.ph[l - 1] + 1) / 2;
+      c.pw[l] = (c.pw[l - 1] + 1) / 2;
+    }
+    for (int l = 0; l < 4; ++l) c.tilesX[l] = (c.w[l] + kNTW - 1) / kNTW;
+    c.act = off;
+    int64_t o = 0;
+    for (int t = 0; t < kNumForward; ++t) {
+      c.tOff[t] = o;
+      if (kLevel[t] >= 0)
+        o += (static_cast<int64_t>(c.h[kLevel[t]]) * c.w[kLevel[t]] * narrow.sw[t] + 15) / 16 * 16;
+    }
+    off += o;
+  }
+  return off;
+}
+
+void Unet::forwardNarrowBatch(const NarrowBatch& nb, const double* r, const float* scale,
+                              const double* q, double* z, double* partial) {
+  const float* wts = nW.get();
+  const int* sw = narrow.sw;
+  auto run = [&](int layer, int gather, bool pool, int A, int B, int out, int aux, int L) {
+    NarrowArgs a{};
+    a.a = nb.act;
+    a.b = B < 0 ? nullptr : nb.act;
+    a.out = nb.act;
+    a.aux = aux < 0 ? nullptr : nb.act;
+    a.w = wts + narrow.wOff[layer];
+    a.w0 = wts + narrow.wOff[kE0a];
+    a.src = r;
+    a.scale = scale;
+    a.cases = nb.cases;
+    a.tab = nb.tab[L];
+    a.tabCount = nb.tabCount[L];
+    a.level = L;
+    a.tA = A;
+    a.tB = B;
+    a.tOut = out;
+    a.tAux = aux;
+    const int ca = gather == kNFirst ? sw[kTe0a] : sw[A];
+    mark(layer, true);
+    launchNarrowConv(stream, ca, B < 0 ? 2 : sw[B], sw[out], gather, pool, true, a);
+    mark(layer, false);
+  };
+  run(kE0b, kNFirst, true, -1, -1, kTe0, kTp0, 0);
+  run(kE1a, kNPlain, false, kTp0, -1, kTe1a, -1, 1);
+  run(kE1b, kNPlain, true, kTe1a, -1, kTe1, kTp1, 1);
+  run(kE2a, kNPlain, false, kTp1, -1, kTe2a, -1, 2);
+  run(kE2b, kNPlain, true, kTe2a, -1, kTe2, kTp2, 2);
+  run(kB3a, kNPlain, false, kTp2, -1, kTb3a, -1, 3);
+  if (narrow.tailSrc >= 0 && narrow.tailDst >= 0) {
+    mark(kNumLayers, true);
+    tailBatch(nb, sw[kTb3a], narrow.tailSrc, narrow.tailDst);
+    mark(kNumLayers, false);
+  }
+  run(kB3b, kNPlain, false, kTb3a, -1, kTb3, -1, 3);
+  run(kD2a, kNConcat, false, kTb3, kTe2, kTd2a, -1, 2);
+  run(kD2b, kNPlain, false, kTd2a, -1, kTd2, -1, 2);
+  run(kD1a, kNConcat, false, kTd2, kTe1, kTd1a, -1, 1);
+  run(kD1b, kNPlain, false, kTd1a, -1, kTd1, -1, 1);
+  run(kD0a, kNConcat, false, kTd1, kTe0, kTd0a, -1, 0);
+  run(kD0b, kNPlain, false, kTd0a, -1, kTd0, -1, 0);
+  const float* wo = wts + narrow.wOff[kOut];
+  const dim3 grid(static_cast<unsigned>(std::max<int64_t>(1, nb.lastCount)));
+  mark(kOut, true);
+  if (sw[kTd0] == 2)
+    hipLaunchKernelGGL((narrowLastBatchKernel<2>), grid, dim3(256), 0, stream, nb.act, nb.cases,
+                       nb.last, wo, scale, r, q, z, partial);
+  else
+    hipLaunchKernelGGL((narrowLastBatchKernel<4>), grid, dim3(256), 0, stream, nb.act, nb.cases,
+                       nb.last, wo, scale, r, q, z, partial);
+  mark(kOut, false);
+  CFDHIP_CHECK(hipGetLastError());
+  collectTiming();
+}
 
 }  // namespace cfdhip

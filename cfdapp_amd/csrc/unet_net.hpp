@@ -26,6 +26,9 @@ enum UnetTensor : int {
 // inference runs on the narrow kernels of unet_narrow.hip instead of the
 // MFMA layers; the result is the same linear map.
 constexpr int kNarrowMax = 4;
+// the narrow convolutions' output tile (rows x columns) and the rows of one
+// output-layer block of a batched application
+constexpr int kNarrowTileH = 16, kNarrowTileW = 64, kNarrowLastRows = 8;
 struct NarrowPlan {
   bool usable = false;
   int sw[kNumForward] = {};                 // stored channels per tensor (2 or 4)
@@ -35,6 +38,37 @@ struct NarrowPlan {
   std::vector<float> w;                     // per layer [tap][slot in][slot out]
   int64_t wOff[kNumLayers] = {};
 };
+
+// One case of a batched application of the narrow network (many 2D cases
+// of uneven size in one launch per layer): per network level its tensor size
+// (level 0 rounded up to multiples of 8), physical image (the rounded-up
+// halves of the solver grid) and conv tiles per tensor row; its padding bits;
+// the offsets of its solver vectors (cells) and of its activations in the
+// batch's arena (act + tOff[tensor], floats).
+struct NarrowCase {
+  int h[4], w[4], ph[4], pw[4], tilesX[4];
+  int pad, ny, nx;
+  int64_t cells, pin, act;
+  int64_t tOff[kNumForward];
+};
+
+// The device tables of a batched application: the case table, per level the
+// (case, tile) entries of the conv launches, the (case, first row, rows,
+// partial slot) entries of the output layer, and the coarse tail's shape of
+// each case (an opaque device array written by Unet::narrowBatchTables).
+struct NarrowBatch {
+  const NarrowCase* cases = nullptr;
+  int64_t nCases = 0;
+  const int2* tab[4] = {};
+  int64_t tabCount[4] = {};
+  const int4* last = nullptr;
+  int64_t lastCount = 0;
+  const void* tailShapes = nullptr;
+  float* act = nullptr;
+};
+
+// The coarse tail of the C options (channels 1 -> 2 of the bottom tensor).
+TailParams tailFrom(const cfdhip_unet_options& o);
 
 class Unet {
  public:
@@ -111,6 +145,22 @@ class Unet {
   // Per-layer kernel timing (HIP events on the stream around each layer's
   // launch; the host waits at the end of every forward pass while enabled).
   void setTiming(bool on);
+  // Batched inference over cases of uneven size (narrow kernels only: the
+  // weights must leave at most kNarrowMax live channels per tensor). The
+  // layout fills each case's level sizes, tiles and activation offsets
+  // (cases[c].ny, nx, pad, cells, pin set by the caller) and returns the
+  // arena size in floats; the tables are then built on the host (tabs,
+  // last) with kNarrowTile* and the tail shapes uploaded by narrowBatchTail.
+  int64_t narrowBatchLayout(std::vector<NarrowCase>& cases) const;
+  // the per-case coarse tail shapes, as a device array of nCases entries
+  // (bytes of one entry: tailShapeBytes())
+  static int64_t tailShapeBytes();
+  void narrowBatchTail(const std::vector<NarrowCase>& cases, void* hostStaging, void* device);
+  // z = S N(S r) on every case (r, q, z, scale: the batch's FP64 vectors and
+  // FP32 scaling at NarrowCase::cells), z = r scale^2 at pinned cells, and the
+  // block partials of (z, q) in each output-layer entry's slot (q null: none)
+  void forwardNarrowBatch(const NarrowBatch& nb, const double* r, const float* scale,
+                          const double* q, double* z, double* partial);
   // index kNumLayers: the coarse tail
   double layerMs[kNumLayers + 1] = {};
   int64_t layerCalls[kNumLayers + 1] = {};
@@ -145,6 +195,10 @@ class Unet {
   // The coarse tail on the bottom tensor (h3 x w3 per image): forward writes
   // channel tail.dst from tail.src; backward is its adjoint on the gradient.
   // C: channels of the tensor; srcCh / dstCh (-1: the options' channels)
+  // The coarse tail of every case of a batched application, one workgroup
+  // per case (cases whose bottom level is at most 128 x 128: the whole tail in
+  // LDS); C, srcCh, dstCh: the stored channels of b3a and the tail's slots.
+  void tailBatch(const NarrowBatch& nb, int C, int srcCh, int dstCh);
   void tailApply(float* tensor, int n, int h3, int w3, bool backward, int C = 64, int srcCh = -1,
                  int dstCh = -1);
   int neumann = 0;

@@ -406,6 +406,35 @@ __global__ __launch_bounds__(kTailThreads) void tailCycleLdsKernel(TailShape sAr
   for (int i = threadIdx.x; i < m0; i += blockDim.x) g[e0 + i] = lds[e0 - o0 + i];
 }
 
+// The whole tail of one case of a batch per workgroup: R_0 = 4 pool(K x) of
+// the source channel of the case's bottom tensor into LDS, the cycle over
+// every level in LDS, and the destination channel = K up(E_0) -- the
+// launches of tailApply for one image, in one kernel (its levels fit LDS).
+__global__ __launch_bounds__(kTailThreads) void tailBatchKernel(const TailShape* __restrict__ shapes,
+                                                               const NarrowCase* __restrict__ cases,
+                                                               float* __restrict__ act, int C,
+                                                               int srcCh, int dstCh, TailParams p) {
+  __shared__ float lds[kTailLdsFloats];
+  __shared__ TailShape s;
+  stageShape(shapes[blockIdx.x], s);
+  const NarrowCase& c = cases[blockIdx.x];
+  float* b3a = act + c.act + c.tOff[kTb3a];
+  const int pad = c.pad;
+  if (s.levels > 0) {
+    const int h0 = tailH(s, 0), w0 = tailW(s, 0);
+    forPixels(h0, w0, [&](int y, int x) {
+      lds[s.off[0] + y * w0 + x] = restrictK(b3a + srcCh, s.h3, s.w3, y, x, p, pad, C);
+    });
+    __syncthreads();
+    cycleLevels(lds, s, p, pad, 0, 0);
+  }
+  const float* e0 = lds + s.off[2];
+  forPixels(s.h3, s.w3, [&](int y, int x) {
+    b3a[(static_cast<int64_t>(y) * s.w3 + x) * C + dstCh] =
+        s.levels > 0 ? prolongK(e0, s.h3, s.w3, y, x, p, pad) : 0.f;
+  });
+}
+
 // out channel = K up(E_0) (forward), or in channel += K up(E_0) and the
 // consumed channel cleared (backward).
 __global__ __launch_bounds__(256) void tailScatterKernel(float* __restrict__ t, int C, int outCh,
@@ -449,6 +478,27 @@ void Unet::setOptions(int neumannPadding, int dirichletSides, const TailParams& 
   neumann = neumannPadding;
   tail = t;
   buildNarrow();  // the tail's channels take part in which channels are live
+}
+
+int64_t Unet::tailShapeBytes() { return static_cast<int64_t>(sizeof(TailShape)); }
+
+void Unet::narrowBatchTail(const std::vector<NarrowCase>& cases, void* hostStaging, void* device) {
+  TailShape* h = static_cast<TailShape*>(hostStaging);
+  for (size_t c = 0; c < cases.size(); ++c) {
+    h[c] = tailShape(cases[c].h[3], cases[c].w[3], tail.levels);
+    require(h[c].perImage <= kTailLdsFloats,
+            "a case of the batch is too large for the batched coarse tail (sides up to 1024)");
+  }
+  CFDHIP_CHECK(hipMemcpyAsync(device, hostStaging, sizeof(TailShape) * cases.size(),
+                              hipMemcpyHostToDevice, stream));
+}
+
+void Unet::tailBatch(const NarrowBatch& nb, int C, int srcCh, int dstCh) {
+  if (tail.levels == 0 || nb.nCases == 0) return;
+  hipLaunchKernelGGL(tailBatchKernel, dim3(static_cast<unsigned>(nb.nCases)), dim3(kTailThreads),
+                     0, stream, static_cast<const TailShape*>(nb.tailShapes), nb.cases, nb.act, C,
+                     srcCh, dstCh, tail);
+  CFDHIP_CHECK(hipGetLastError());
 }
 
 void Unet::tailApply(float* tensor, int n, int h3, int w3, bool backward, int C, int srcCh,

@@ -595,6 +595,27 @@ int cfdhip_batch_add_pressure(cfdhip_batch* batch, cfdhip_ctx* ctx, int64_t* ind
 int cfdhip_batch_solve(cfdhip_batch* batch, const cfdhip_solver_settings* s, int tile_rows,
                        int use_graph, cfdhip_solve_result* results, double* latency_ms,
                        double* stats);
+/* The learned preconditioner of the batch's flexible-CG solve (no counterpart
+ * in the reference): weights (cfdhip_unet_param_count() floats) and options
+ * as for cfdhip_set_learned_weights / cfdhip_set_learned_options, Neumann
+ * padding required; each case's Dirichlet sides are its FixedValue pressure
+ * patches (from the context for cfdhip_batch_add_pressure; for
+ * cfdhip_batch_add_system the sides whose every cell has a diagonal larger
+ * than the sum of its internal-face couplings). The weights must leave at
+ * most 4 live channels per tensor (the narrow kernels; the shipped V-cycle
+ * weights keep 2). With them, cfdhip_batch_solve with type
+ * CFDHIP_SOLVER_FCG, preconditioner CFDHIP_PREC_LEARNED and precision
+ * CFDHIP_FP64 solves every case (sides up to 1024 cells) by flexible CG with
+ * FP64 Krylov vectors, each step one launch over all active cases: the
+ * network's layers over (case, tile) tables, the coarse tail one workgroup
+ * per case, the vector kernels over (case, block) tables, one-block-per-case
+ * scalar steps. A case whose recurrence residual drifted from its FP64 true
+ * residual restarts from b - A x; one that stagnated (stagnation_window) or
+ * broke down is solved by the batched FP64 Jacobi-PCG (fallback_used = 1).
+ * tile_rows is ignored; stats: device ms, rounds, sum of cells x iterations,
+ * 0, graph captures. */
+int cfdhip_batch_set_learned(cfdhip_batch* batch, const float* weights, int64_t count,
+                             const cfdhip_unet_options* options);
 /* Copies the solution of case `index` of the last solve to the host. */
 int cfdhip_batch_solution(cfdhip_batch* batch, int64_t index, double* host, int64_t count);
 
