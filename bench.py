@@ -16,13 +16,14 @@ Jacobi-PCG, SIMPLE outer iterations per second at 2048x2048, Re = 1000, the
 per-layer kernel times of the preconditioner, the plain iteration kernel's
 HBM roofline, and the batched leg: the p' systems of the first SIMPLE
 iteration of 256 cavities and channels of uneven size (seed 5005) solved
-together by the batched mixed-precision Jacobi-PCG (aggregate cells x
-iterations per second, per-case p50 / p99 completion time) and one after
-another by the single-case solve; the reference's own 129^2 cavity (Re 100)
-SIMPLE iterations per second with stream-launched and graph-replayed p'
-iteration chunks; the PISO leg: time steps of a 1024^2 cavity at Re 1000 with
-the reference's BiCGSTAB p' solves and with the learned FCG; and the 3D leg: SIMPLE iterations on the
-256^3 cavity at Re = 400 with FP64 and with mixed-precision p' solves, and the
+together by the batched learned FCG and by the batched mixed-precision
+Jacobi-PCG (aggregate cells x iterations per second, per-case p50 / p99
+completion time) and one after another by the single-case learned FCG;
+the reference's own 129^2 cavity (Re 100) SIMPLE iterations per second with
+stream-launched and graph-replayed p' iteration chunks; the PISO leg: time
+steps of a 1024^2 cavity at Re 1000 with the reference's BiCGSTAB p' solves
+and with the learned FCG; and the 3D leg: SIMPLE iterations on the 256^3
+cavity at Re = 400 with FP64 and with mixed-precision p' solves, and the
 mixed run's pressure / velocity error against the FP64 run.
 
 Prints one JSON line (see the repository's bench contract).
@@ -183,69 +184,86 @@ def cpu_baseline(iters_gpu, pprime_its_per_outer):
 
 
 def batch_leg(cfd, log, cases, use_graph=True, sequential=True):
-    """p' systems of the first SIMPLE iteration of each case, solved (a) by the
-    batched mixed-precision Jacobi-PCG in one launch per step and (b) one case
-    after another by the single-case mixed-precision solve, both to 1e-8 FP64
-    true relative residual."""
+    """p' systems of the first SIMPLE iteration of each case, every one to
+    1e-8 FP64 true relative residual: (a) the batched flexible CG with the
+    learned preconditioner (FP64 vectors), one launch per step over all active
+    cases; (b) the batched mixed-precision Jacobi-PCG, one launch per step;
+    (c) one case after another by the single-case learned FCG."""
     from cfdapp_amd import learned
     t0 = time.perf_counter()
     batch = cfd.BatchSolver()
-    seq_s, seq_lat, seq_iters = 0.0, [], 0
-    mx = cfd.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi", precision="mixed",
-                                  check_interval=64)
+    batch.set_learned()
+    fcg = cfd.LinearSolverSettings(0.0, 1e-8, 2000, "FCG", "Learned", check_interval=4)
+    seq_s, seq_lat = 0.0, []
     cells = 0
+    sizes = []
     for kind, nx, ny, re in cases:
         d = learned.make_case(kind, nx, ny=ny, re=re, speed=1.0)
         try:
             learned.assemble_pprime(d, 0)
             batch.add_pressure(d)
             cells += nx * ny
+            sizes.append(nx * ny)
             if not sequential:
                 continue
+            d.set_learned_weights()
             d._ctx.call("cfdhip_synchronize")
             t1 = time.perf_counter()
-            r = d.solve_pressure_rhs(mx, None, download=False)
+            r = d.solve_pressure_rhs(fcg, None, download=False)
             dt = time.perf_counter() - t1
             if r.status != 0:
                 raise RuntimeError(f"sequential {kind} {nx}x{ny}: {r.status_name}")
             seq_s += dt
             seq_lat.append(dt * 1e3)
-            seq_iters += r.iterations
         finally:
             d.close()
     log(f"batch: {len(cases)} cases, {cells} cells, set up in {time.perf_counter() - t0:.1f} s")
+    sizes = np.array(sizes, np.float64)
+
+    def timed(settings, label):
+        batch.solve(settings, use_graph=use_graph)  # warm-up (the Jacobi leg tunes its tiles)
+        t1 = time.perf_counter()
+        res = batch.solve(settings, use_graph=use_graph)
+        wall = time.perf_counter() - t1
+        bad = [k for k, r in enumerate(res) if r.status != 0 or r.fallback_used]
+        if bad:
+            raise RuntimeError(f"batched {label}: {len(bad)} cases did not converge on their own")
+        return res, wall, dict(batch.stats)
+
+    res, wall, st = timed(fcg, "learned FCG")
+    lat = np.array([r.device_ms for r in res])
+    its = np.array([r.iterations for r in res], np.float64)
+    out = {"cases": len(cases), "cells": cells,
+           "learned_batched_s": wall, "learned_batched_device_ms": st["device_ms"],
+           "learned_cell_iter_per_s": float((its * sizes).sum()) / wall,
+           "learned_p50_ms": float(np.percentile(lat, 50)),
+           "learned_p99_ms": float(np.percentile(lat, 99)),
+           "learned_iterations_min": int(its.min()), "learned_iterations_max": int(its.max()),
+           "learned_rounds": st["steps"], "learned_graph_captures": st["graph_captures"],
+           "learned_max_true_rel_res": float(max(r.true_residual / r.initial_residual
+                                                 for r in res if r.initial_residual > 0))}
     bs = cfd.LinearSolverSettings(0.0, 1e-8, 100000, "CG", "Jacobi", precision="mixed",
                                   check_interval=32)
-    batch.solve(bs, use_graph=use_graph)  # warm-up (also tunes the tile height)
-    t1 = time.perf_counter()
-    res = batch.solve(bs, use_graph=use_graph)
-    wall = time.perf_counter() - t1
-    st = batch.stats
-    bad = [k for k, r in enumerate(res) if r.status != 0]
-    if bad:
-        raise RuntimeError(f"batched solve: {len(bad)} cases did not converge")
-    lat = np.array([r.device_ms for r in res])
-    out = {"cases": len(cases), "cells": cells, "batched_s": wall,
-           "batched_device_ms": st["device_ms"],
-           "batched_cell_iter_per_s": st["cell_iterations"] / wall,
-           "batched_p50_ms": float(np.percentile(lat, 50)),
-           "batched_p99_ms": float(np.percentile(lat, 99)),
-           "batched_tile_rows": st["tile_rows"], "batched_steps": st["steps"],
-           "batched_graph_captures": st["graph_captures"],
-           "tile_tuning_us": st["tune_us"],
-           }
+    res_j, wall_j, st_j = timed(bs, "Jacobi-PCG")
+    lat_j = np.array([r.device_ms for r in res_j])
+    out.update({"jacobi_batched_s": wall_j, "jacobi_batched_device_ms": st_j["device_ms"],
+                "jacobi_cell_iter_per_s": st_j["cell_iterations"] / wall_j,
+                "jacobi_p50_ms": float(np.percentile(lat_j, 50)),
+                "jacobi_p99_ms": float(np.percentile(lat_j, 99)),
+                "jacobi_tile_rows": st_j["tile_rows"], "jacobi_steps": st_j["steps"],
+                "jacobi_graph_captures": st_j["graph_captures"], "tile_tuning_us": st_j["tune_us"],
+                "speedup_learned_vs_jacobi_batched": wall_j / wall})
     if sequential:
-        out.update({"sequential_s": seq_s,
-                    "sequential_cell_iter_per_s": sum(r.iterations * nx * ny for r, (_, nx, ny, _)
-                                                      in zip(res, cases)) / seq_s,
-                    "sequential_p50_ms": float(np.percentile(seq_lat, 50)),
-                    "sequential_p99_ms": float(np.percentile(seq_lat, 99)),
-                    "speedup_batched_vs_sequential": seq_s / wall})
-    log(f"batch: batched {wall * 1e3:.1f} ms ({out['batched_cell_iter_per_s']:.3e} cell-it/s, "
-        f"p50 {out['batched_p50_ms']:.1f} ms, p99 {out['batched_p99_ms']:.1f} ms, "
-        f"{st['tile_rows']}-row tiles, {st['graph_captures']} graph captures)" +
-        (f"; sequential {seq_s * 1e3:.1f} ms (p50 {out['sequential_p50_ms']:.1f} ms, "
-         f"p99 {out['sequential_p99_ms']:.1f} ms)" if sequential else ""))
+        out.update({"sequential_learned_s": seq_s,
+                    "sequential_learned_p50_ms": float(np.percentile(seq_lat, 50)),
+                    "sequential_learned_p99_ms": float(np.percentile(seq_lat, 99)),
+                    "speedup_batched_vs_sequential_learned": seq_s / wall})
+    log(f"batch: learned FCG batched {wall * 1e3:.1f} ms ({out['learned_cell_iter_per_s']:.3e} "
+        f"cell-it/s, {its.min():.0f}..{its.max():.0f} it, p50 {out['learned_p50_ms']:.1f} ms, "
+        f"p99 {out['learned_p99_ms']:.1f} ms); Jacobi-PCG batched {wall_j * 1e3:.1f} ms "
+        f"(p50 {out['jacobi_p50_ms']:.1f} ms, p99 {out['jacobi_p99_ms']:.1f} ms)" +
+        (f"; learned FCG one case at a time {seq_s * 1e3:.1f} ms (p50 "
+         f"{out['sequential_learned_p50_ms']:.1f} ms)" if sequential else ""))
     batch.close()
     return out
 

@@ -12,8 +12,8 @@ rounding points, FP64 sums) to 2 % of its largest component (a BF16 operand
 that lands on the other side of a rounding boundary differs by one BF16 unit,
 2^-8 relative, in either direction), and with the exact oracle to 8 % (the
 rounding of a sum with cancellation across the steps); a wrong term in the
-reverse accumulation moves components by their own size. A third check is the device's own gradient against central
-differences of its own loss.
+reverse accumulation moves components by their own size. A third check is
+the device's own gradient against central differences of its own loss.
 """
 import numpy as np
 import pytest
