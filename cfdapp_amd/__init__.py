@@ -176,12 +176,51 @@ def _result(r: _lib.SolveResult, x, hist) -> SolverResult:
                         residual_history=hist)
 
 
-class GpuLinearSolver:
-    """LinearSolver::solve(system, x0) on the device for a host CSR system."""
+class _HostCall:
+    """The C callback of a host preconditioner: z = M r into the library's host
+    arrays. No exception may cross the C boundary, so one is kept here and the
+    library told to stop (nonzero return); solve() raises it."""
 
-    def __init__(self, settings: LinearSolverSettings, ctx: _Context):
+    def __init__(self, preconditioner):
+        self.preconditioner = preconditioner
+        self.error = None
+        self.callback = _lib.HOST_PRECONDITIONER(self._apply)
+
+    def _apply(self, r, z, n, _user):
+        try:
+            self.preconditioner.apply(np.ctypeslib.as_array(r, (n,)),
+                                      np.ctypeslib.as_array(z, (n,)))
+            return 0
+        except BaseException as e:  # noqa: BLE001 - raised again by solve()
+            self.error = e
+            return 1
+
+
+class GpuLinearSolver:
+    """LinearSolver::solve(system, x0) on the device for a host CSR system.
+
+    With a host ``preconditioner`` (an object with ``build(rowptr, col, val)``
+    and ``apply(r, z)`` writing z = M r, the shape of
+    cfd::algebra::Preconditioner, include/cfd/algebra/Preconditioner.hpp:11-17)
+    the device CG / BiCGSTAB applies it through a host round trip per
+    application (cuda/kernels/GpuLinearSolverCuda.cpp:103-118), in place of the
+    settings' preconditioner. build() runs on the system before the solve; a
+    ValueError from it makes the result INVALID_SYSTEM (the reference's
+    try/catch of InvalidArgumentError around Preconditioner::build,
+    cuda/kernels/GpuLinearSolverCuda.cpp:130-143), and an exception from apply()
+    leaves solve()."""
+
+    def __init__(self, settings: LinearSolverSettings, ctx: _Context, preconditioner=None):
         self.settings = settings
         self._ctx = ctx
+        self.preconditioner = preconditioner
+        self._host = None
+        if preconditioner is not None:
+            # the callback holds the preconditioner, not this solver: no cycle
+            # keeps the context alive after the solver is dropped
+            self._host = _HostCall(preconditioner)
+            self._ctx.call("cfdhip_set_host_preconditioner",
+                           ctypes.cast(self._host.callback, ctypes.c_void_p), None)
 
     def solve(self, rowptr, col, val, b, x0=None) -> SolverResult:
         rowptr, col, val, b = _i64(rowptr), _i64(col), _f64(val), _f64(b)
@@ -189,8 +228,25 @@ class GpuLinearSolver:
         x = np.empty(n)
         r = _lib.SolveResult()
         s = self.settings.c()
-        self._ctx.call("cfdhip_csr_solve", ctypes.byref(s), n, _pi(rowptr), _pi(col), _pf(val),
-                       _pf(b), _pf(None if x0 is None else _f64(x0)), _pf(x), ctypes.byref(r))
+        if self.preconditioner is not None:
+            try:
+                self.preconditioner.build(rowptr, col, val)
+            except ValueError:  # InvalidArgumentError in the reference: InvalidSystem
+                x = np.array(_f64(x0)) if x0 is not None else np.zeros(n)
+                return SolverResult(solution=x, status=INVALID_SYSTEM, iterations=0,
+                                    initial_residual=0.0, final_residual=0.0,
+                                    true_residual=float("nan"))
+            s.preconditioner = _lib.PREC_HOST
+            self._host.error = None
+        try:
+            self._ctx.call("cfdhip_csr_solve", ctypes.byref(s), n, _pi(rowptr), _pi(col),
+                           _pf(val), _pf(b), _pf(None if x0 is None else _f64(x0)), _pf(x),
+                           ctypes.byref(r))
+        except CfdHipError:
+            if self._host is not None and self._host.error is not None:
+                err, self._host.error = self._host.error, None
+                raise err
+            raise
         return _result(r, x, self._ctx.history())
 
     def spmv(self, rowptr, col, val, x):
@@ -201,21 +257,23 @@ class GpuLinearSolver:
         return y
 
 
-def _make(settings, kind):
+def _make(settings, kind, preconditioner):
     if not device_available():
         return None
     s = LinearSolverSettings(**{**settings.__dict__, "type": kind})
-    return GpuLinearSolver(s, _Context())
+    return GpuLinearSolver(s, _Context(), preconditioner)
 
 
-def make_gpu_cg(settings: LinearSolverSettings):
-    """cfd::gpu::makeGpuCG: a device CG solver, or None when no device."""
-    return _make(settings, "CG")
+def make_gpu_cg(settings: LinearSolverSettings, preconditioner=None):
+    """cfd::gpu::makeGpuCG: a device CG solver, or None when no device. A host
+    preconditioner object is applied through the host (GpuLinearSolver)."""
+    return _make(settings, "CG", preconditioner)
 
 
-def make_gpu_bicgstab(settings: LinearSolverSettings):
-    """cfd::gpu::makeGpuBiCGSTAB: a device BiCGSTAB solver, or None when no device."""
-    return _make(settings, "BiCGSTAB")
+def make_gpu_bicgstab(settings: LinearSolverSettings, preconditioner=None):
+    """cfd::gpu::makeGpuBiCGSTAB: a device BiCGSTAB solver, or None when no
+    device. A host preconditioner object is applied through the host."""
+    return _make(settings, "BiCGSTAB", preconditioner)
 
 
 class GpuSimpleDiscretization:

@@ -21,7 +21,7 @@ VP = ctypes.c_void_p
 OK, ERR_INVALID, ERR_RUNTIME, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_NONFINITE = range(6)
 CONVERGED, MAX_ITERATIONS, BREAKDOWN, NONFINITE_INPUT, NONFINITE_RESIDUAL, INVALID_SYSTEM = range(6)
 SOLVER_CG, SOLVER_BICGSTAB, SOLVER_FCG = 0, 1, 3
-PREC_NONE, PREC_JACOBI, PREC_LEARNED = 0, 1, 2
+PREC_NONE, PREC_JACOBI, PREC_LEARNED, PREC_HOST = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 WALL, MOVING_WALL, INLET, OUTLET = 0, 1, 2, 3
 FIXED_GRADIENT, FIXED_VALUE = 0, 1
@@ -73,6 +73,9 @@ class UnetOptions(ctypes.Structure):
 
 # Every exported symbol with its signature; the not-gpu tests check that the
 # built library exports exactly these names.
+# int (*)(const double* r, double* z, int64_t n, void* user)
+HOST_PRECONDITIONER = ctypes.CFUNCTYPE(ctypes.c_int, P_F64, P_F64, ctypes.c_int64, ctypes.c_void_p)
+
 SIGNATURES = {
     "cfdhip_device_available": (ctypes.c_int, []),
     "cfdhip_create": (ctypes.c_void_p, [ctypes.c_int]),
@@ -150,6 +153,7 @@ SIGNATURES = {
                                                             ctypes.POINTER(SolveResult)]),
     "cfdhip_solve_history": (ctypes.c_int, [ctypes.c_void_p, P_F64, ctypes.c_int64, P_I64]),
     "cfdhip_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "cfdhip_set_host_preconditioner": (ctypes.c_int, [ctypes.c_void_p, VP, VP]),
     "cfdhip_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
     "cfdhip_preconditioner_timing": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_I64]),
     # learned preconditioner (device buffers are passed as void*)
@@ -219,6 +223,15 @@ def load():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with cfdapp_amd/build.sh "
                 "(or __graft_entry__.build()); there is no CPU fallback")
+        # One HIP runtime per process. torch ships its own libamdhip64 /
+        # libhsa-runtime64 under the same sonames as ROCm's: imported first,
+        # its copies also serve libcfdhip.so; loaded after libcfdhip.so has
+        # pulled in ROCm's, torch maps a second runtime that finds no device.
+        # So torch, when it is installed, is loaded first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -673,6 +673,13 @@ int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t*
   });
 }
 
+int cfdhip_set_host_preconditioner(cfdhip_ctx* ctx, cfdhip_host_preconditioner fn, void* user) {
+  return guarded(ctx, [&](Context& c) {
+    c.hostPc = fn;
+    c.hostPcUser = fn != nullptr ? user : nullptr;
+  });
+}
+
 int cfdhip_set_timing(cfdhip_ctx* ctx, int enabled) {
   return guarded(ctx, [&](Context& c) {
     c.timing = enabled != 0;

@@ -106,6 +106,10 @@ struct Context {
   std::unique_ptr<FaceMesh> fmesh;
   std::vector<double> lastHistory;
   std::unique_ptr<Unet> net;  // learned p' preconditioner (weights loaded by the caller)
+  // CFDHIP_PREC_HOST: the caller's host preconditioner and its staging vectors
+  cfdhip_host_preconditioner hostPc = nullptr;
+  void* hostPcUser = nullptr;
+  std::vector<double> hostPcIn, hostPcOut;
   int64_t lastFallback = 0;
 
   // timing of the pressure SpMV kernel

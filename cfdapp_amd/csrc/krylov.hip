@@ -216,6 +216,56 @@ __global__ void cgDirectionKernel(int64_t n, const double* __restrict__ r,
   }
 }
 
+// The caller's host preconditioner (CFDHIP_PREC_HOST): z arrives from the
+// host, so the kernels that form z = M r inline take it as a vector.
+// p = z; r . z
+__global__ void cgStartZKernel(int64_t n, const double* __restrict__ r,
+                               const double* __restrict__ z, double* p, Red R) {
+  double acc[1] = {0.0};
+  if (!R.st->done)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      p[i] = z[i];
+      acc[0] += r[i] * z[i];
+    }
+  blockPartials<1>(acc, R.partial);
+}
+
+// x += p alpha; r = r - Ap alpha (no sums: z = M r comes from the host next)
+__global__ void cgUpdateXRKernel(int64_t n, double* x, double* r, const double* __restrict__ p,
+                                 const double* __restrict__ Ap, const KState* st) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    x[i] += p[i] * alpha;
+    r[i] = r[i] - (Ap[i] * alpha);
+  }
+}
+
+// |r|^2, r . z (the sums of kStepCgUpdate)
+__global__ void rzKernel(int64_t n, const double* __restrict__ r, const double* __restrict__ z,
+                         Red R) {
+  double acc[2] = {0.0, 0.0};
+  if (!R.st->done)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      acc[0] += r[i] * r[i];
+      acc[1] += r[i] * z[i];
+    }
+  blockPartials<2>(acc, R.partial);
+}
+
+// p = z + p beta
+__global__ void cgDirectionZKernel(int64_t n, const double* __restrict__ z, double* p,
+                                   const KState* st) {
+  if (st->done) return;
+  const double beta = st->beta;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = z[i] + (p[i] * beta);
+}
+
 // ---------------------------------------------------------------------------
 // BiCGSTAB (BiCGSTAB.cpp:111-274).
 
@@ -664,12 +714,185 @@ cfdhip_solve_result runBiCGSTAB(Context& ctx, Op A, int64_t n, const cfdhip_solv
   return S.result(tr, ms);
 }
 
+// out = M in through the caller's host preconditioner: the one vector crosses
+// to the host and back per application, as in the reference's GPU solvers
+// (cuda/kernels/GpuLinearSolverCuda.cpp:103-118).
+void hostApply(Context& ctx, const double* in, double* out, int64_t n) {
+  ctx.hostPcIn.resize(n);
+  ctx.hostPcOut.assign(n, 0.0);
+  CFDHIP_CHECK(hipMemcpyAsync(ctx.hostPcIn.data(), in, n * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+  if (ctx.hostPc(ctx.hostPcIn.data(), ctx.hostPcOut.data(), n, ctx.hostPcUser) != 0)
+    throw Error(CFDHIP_ERR_RUNTIME, "the host preconditioner failed");
+  CFDHIP_CHECK(hipMemcpyAsync(out, ctx.hostPcOut.data(), n * sizeof(double),
+                              hipMemcpyHostToDevice, ctx.stream));
+  CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));  // the host buffer is reused
+}
+
+// CG (CG.cpp:59-147) with the caller's host preconditioner: the state is read
+// after every iteration (the host is in the loop anyway), so M is not applied
+// once the solve is done.
+template <class Op>
+cfdhip_solve_result runCGHost(Context& ctx, Op A, int64_t n, const cfdhip_solver_settings& set,
+                              const double* b, double* x,
+                              const std::vector<std::pair<const double*, int64_t>>& opArrays) {
+  require(ctx.hostPc != nullptr,
+          "CFDHIP_PREC_HOST: no host preconditioner set (cfdhip_set_host_preconditioner)");
+  KrylovWork& w = ctx.kw;
+  w.r.ensure(n);
+  w.p.ensure(n);
+  w.Ap.ensure(n);
+  w.z.ensure(n);
+  Solve S(ctx, n);
+  S.begin(set);
+  auto arrays = opArrays;
+  arrays.push_back({b, n});
+  arrays.push_back({x, n});
+  if (!inputsFinite(ctx, A, arrays)) {
+    S.hs->status = CFDHIP_NONFINITE_INPUT;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  Timer timer(ctx.stream);
+  const dim3 g = S.grid();
+  hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x, w.r.get(),
+                     nullptr, S.R, 0);
+  S.fin(kStepStartNorm, g);
+  S.pull();
+  if (!S.hs->done) {
+    hostApply(ctx, w.r.get(), w.z.get(), n);
+    hipLaunchKernelGGL(cgStartZKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), w.z.get(),
+                       w.p.get(), S.R);
+    S.fin(kStepCgStartRz, g);
+    for (int64_t it = 0; it < set.max_iterations; ++it) {
+      hipLaunchKernelGGL((cgSpmvKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.p.get(),
+                         w.Ap.get(), S.R);
+      S.fin(kStepCgPAp, g);
+      hipLaunchKernelGGL(cgUpdateXRKernel, g, dim3(kBlock), 0, ctx.stream, n, x, w.r.get(),
+                         w.p.get(), w.Ap.get(), S.R.st);
+      CFDHIP_CHECK(hipGetLastError());
+      S.pull();
+      if (S.hs->done) break;
+      hostApply(ctx, w.r.get(), w.z.get(), n);
+      hipLaunchKernelGGL(rzKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), w.z.get(), S.R);
+      S.fin(kStepCgUpdate, g);
+      hipLaunchKernelGGL(cgDirectionZKernel, g, dim3(kBlock), 0, ctx.stream, n, w.z.get(),
+                         w.p.get(), S.R.st);
+      CFDHIP_CHECK(hipGetLastError());
+    }
+    S.pull();
+  }
+  const double ms = timer.stop();
+  if (!S.hs->done) {
+    S.hs->status = CFDHIP_MAX_ITERATIONS;
+    S.hs->done = 1;
+  }
+  const double tr = trueResidual(ctx, A, n, b, x);
+  return S.result(tr, ms);
+}
+
+// BiCGSTAB (BiCGSTAB.cpp:48-275) with the caller's host preconditioner:
+// phat = M p and shat = M s cross to the host, the rest is the device
+// recurrence of runBiCGSTAB (the kernels form phat = p, shat = s, which the
+// host result then replaces).
+template <class Op>
+cfdhip_solve_result runBiCGSTABHost(Context& ctx, Op A, int64_t n,
+                                    const cfdhip_solver_settings& set, const double* b, double* x,
+                                    const std::vector<std::pair<const double*, int64_t>>& opArrays) {
+  require(ctx.hostPc != nullptr,
+          "CFDHIP_PREC_HOST: no host preconditioner set (cfdhip_set_host_preconditioner)");
+  KrylovWork& w = ctx.kw;
+  for (DeviceArray<double>* a : {&w.r, &w.p, &w.rhat, &w.v, &w.s, &w.t, &w.phat, &w.shat})
+    a->ensure(n);
+  Solve S(ctx, n);
+  S.begin(set);
+  auto arrays = opArrays;
+  arrays.push_back({b, n});
+  arrays.push_back({x, n});
+  if (!inputsFinite(ctx, A, arrays)) {
+    S.hs->status = CFDHIP_NONFINITE_INPUT;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  Timer timer(ctx.stream);
+  const dim3 g = S.grid();
+  hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x, w.r.get(),
+                     w.rhat.get(), S.R, 0);
+  S.fin(kStepStartNorm, g);
+  hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, w.v.get(), w.p.get());
+  CFDHIP_CHECK(hipGetLastError());
+  const int64_t launchCap = 4 * set.max_iterations + 64;
+  for (int64_t launched = 0; launched < launchCap; ++launched) {
+    S.pull();
+    if (S.hs->done) break;
+    if (S.hs->restart == 1) {
+      // BiCGSTAB.cpp:128-130: restart only after progress in this sequence.
+      if (!(S.hs->rnorm < S.hs->restartResidual)) {
+        S.hs->status = CFDHIP_BREAKDOWN;
+        S.hs->done = 1;
+        S.hs->iter = S.hs->completed;
+        break;
+      }
+      hipLaunchKernelGGL((residualKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x,
+                         w.r.get(), w.rhat.get(), S.R, 1);
+      S.fin(kStepBcgRestart, g);
+      hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, w.v.get(), w.p.get());
+      CFDHIP_CHECK(hipGetLastError());
+      S.pull();
+      if (S.hs->done) {
+        S.hs->iter = S.hs->completed;
+        break;
+      }
+      S.hs->restart = 0;
+      S.push();
+    }
+    hipLaunchKernelGGL(bcgRhoKernel, g, dim3(kBlock), 0, ctx.stream, n, w.rhat.get(), w.r.get(),
+                       S.R);
+    S.fin(kStepBcgRho, g);
+    hipLaunchKernelGGL(bcgDirectionKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(),
+                       w.v.get(), nullptr, w.p.get(), w.phat.get(), S.R.st);
+    S.pull();
+    if (S.hs->done || S.hs->restart == 1) continue;
+    hostApply(ctx, w.p.get(), w.phat.get(), n);
+    hipLaunchKernelGGL((bcgVKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.phat.get(),
+                       w.rhat.get(), w.v.get(), S.R);
+    S.fin(kStepBcgV, g);
+    hipLaunchKernelGGL(bcgSKernel, g, dim3(kBlock), 0, ctx.stream, n, w.r.get(), w.v.get(),
+                       nullptr, w.s.get(), w.shat.get(), S.R);
+    S.fin(kStepBcgS, g);
+    S.pull();
+    if (S.hs->done || S.hs->restart == 1) continue;
+    hostApply(ctx, w.s.get(), w.shat.get(), n);
+    hipLaunchKernelGGL((bcgTKernel<Op>), g, dim3(kBlock), 0, ctx.stream, A, n, w.shat.get(),
+                       w.s.get(), w.phat.get(), w.t.get(), x, S.R);
+    S.fin(kStepBcgT, g);
+    hipLaunchKernelGGL(bcgXKernel, g, dim3(kBlock), 0, ctx.stream, n, x, w.r.get(), w.phat.get(),
+                       w.shat.get(), w.s.get(), w.t.get(), S.R);
+    S.fin(kStepBcgX, g);
+    CFDHIP_CHECK(hipGetLastError());
+  }
+  S.pull();
+  const double ms = timer.stop();
+  if (!S.hs->done) {
+    S.hs->status = CFDHIP_MAX_ITERATIONS;
+    S.hs->done = 1;
+  }
+  if (S.hs->status == CFDHIP_BREAKDOWN) S.hs->iter = S.hs->completed;
+  const double tr = trueResidual(ctx, A, n, b, x);
+  return S.result(tr, ms);
+}
+
 template <class Op>
 cfdhip_solve_result dispatch(Context& ctx, Op A, int64_t n, const cfdhip_solver_settings& s,
                              const double* b, double* x,
                              const std::vector<std::pair<const double*, int64_t>>& arrays) {
-  if (s.type == CFDHIP_SOLVER_CG) return runCG(ctx, A, n, s, b, x, arrays);
-  if (s.type == CFDHIP_SOLVER_BICGSTAB) return runBiCGSTAB(ctx, A, n, s, b, x, arrays);
+  const bool host = s.preconditioner == CFDHIP_PREC_HOST;
+  if (s.type == CFDHIP_SOLVER_CG)
+    return host ? runCGHost(ctx, A, n, s, b, x, arrays) : runCG(ctx, A, n, s, b, x, arrays);
+  if (s.type == CFDHIP_SOLVER_BICGSTAB)
+    return host ? runBiCGSTABHost(ctx, A, n, s, b, x, arrays)
+                : runBiCGSTAB(ctx, A, n, s, b, x, arrays);
   throw Error(CFDHIP_ERR_INVALID, "unsupported linear solver type for this operator");
 }
 

@@ -56,8 +56,11 @@ enum {
 
 /* cfd::algebra::LinearSolverType (LinearSolver.hpp:31-39) plus flexible CG. */
 enum { CFDHIP_SOLVER_CG = 0, CFDHIP_SOLVER_BICGSTAB = 1, CFDHIP_SOLVER_FCG = 3 };
-/* cfd::algebra::PreconditionerType (LinearSolver.hpp:62-65) plus the learned one. */
-enum { CFDHIP_PREC_NONE = 0, CFDHIP_PREC_JACOBI = 1, CFDHIP_PREC_LEARNED = 2 };
+/* cfd::algebra::PreconditionerType (LinearSolver.hpp:62-65) plus the learned one, and
+ * CFDHIP_PREC_HOST: the caller's own host preconditioner (cfdhip_set_host_preconditioner),
+ * the Preconditioner object makeGpuCG / makeGpuBiCGSTAB take (GpuLinearSolver.hpp:48-52). */
+enum { CFDHIP_PREC_NONE = 0, CFDHIP_PREC_JACOBI = 1, CFDHIP_PREC_LEARNED = 2,
+       CFDHIP_PREC_HOST = 3 };
 /* Arithmetic of the inner iteration. */
 enum { CFDHIP_FP64 = 0, CFDHIP_MIXED = 1 };
 
@@ -407,6 +410,18 @@ int cfdhip_pressure_solve_device_noalloc(cfdhip_ctx* ctx, const cfdhip_solver_se
                                          const double* b, double* x, cfdhip_solve_result* r);
 /* Residual history of the last solve on this context (up to cap entries). */
 int cfdhip_solve_history(cfdhip_ctx* ctx, double* history, int64_t cap, int64_t* length);
+
+/* A caller's preconditioner z = M r applied on the host (cfd::algebra::Preconditioner::apply,
+ * include/cfd/algebra/Preconditioner.hpp:16), for CG and BiCGSTAB solves with preconditioner
+ * CFDHIP_PREC_HOST on this context (cfdhip_solve_pressure*, cfdhip_solve_momentum,
+ * cfdhip_csr_solve): each application copies the one vector to the host and z back, as the
+ * reference's GPU solvers do for such a preconditioner
+ * (cuda/kernels/GpuLinearSolverCuda.cpp:103-118); the recurrence stays on the device. r and z
+ * are host arrays of n doubles, valid during the call. The callback returns 0, or nonzero when
+ * it failed (the reference's Preconditioner::apply throwing): the solve then stops and returns
+ * CFDHIP_ERR_RUNTIME. fn null clears it. */
+typedef int (*cfdhip_host_preconditioner)(const double* r, double* z, int64_t n, void* user);
+int cfdhip_set_host_preconditioner(cfdhip_ctx* ctx, cfdhip_host_preconditioner fn, void* user);
 
 /* ---- timing -------------------------------------------------------------------- */
 /* Kernel timing of the last solve: out[0] = SpMV kernel launches,

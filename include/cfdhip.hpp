@@ -16,6 +16,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <functional>
 #include <memory>
 #include <span>
 #include <stdexcept>
@@ -144,11 +146,23 @@ inline SolverResult fromC(const cfdhip_solve_result& r, cfdhip_ctx* ctx) {
 }
 }  // namespace detail
 
+// A caller's host preconditioner, z = M r on n values
+// (cfd::algebra::Preconditioner::apply, include/cfd/algebra/Preconditioner.hpp:16).
+using HostPreconditioner = std::function<void(const double* r, double* z, std::size_t n)>;
+
 // LinearSolver::solve(system, x0) on the device.
 class GpuLinearSolver {
  public:
   GpuLinearSolver(LinearSolverSettings settings, detail::CtxPtr ctx)
       : settings_(settings), ctx_(std::move(ctx)) {}
+  // With a host preconditioner (the reference's makeGpuCG(settings, preconditioner) with a
+  // Preconditioner it has no device form of): every application crosses to the host and back.
+  GpuLinearSolver(LinearSolverSettings settings, detail::CtxPtr ctx, HostPreconditioner pc)
+      : settings_(settings), ctx_(std::move(ctx)),
+        pc_(std::make_unique<HostCall>(HostCall{std::move(pc), nullptr})) {
+    detail::check(ctx_.get(), cfdhip_set_host_preconditioner(ctx_.get(), &trampoline,
+                                                             pc_.get()));
+  }
 
   [[nodiscard]] const LinearSolverSettings& settings() const noexcept { return settings_; }
 
@@ -158,13 +172,17 @@ class GpuLinearSolver {
       throw InvalidArgumentError("GpuLinearSolver::solve: size mismatch");
     static_assert(sizeof(std::size_t) == sizeof(int64_t));
     std::vector<double> x(A.rows);
-    const cfdhip_solver_settings s = settings_.c();
+    cfdhip_solver_settings s = settings_.c();
+    if (pc_) s.preconditioner = CFDHIP_PREC_HOST;
     cfdhip_solve_result r{};
-    detail::check(ctx_.get(),
-                  cfdhip_csr_solve(ctx_.get(), &s, static_cast<int64_t>(A.rows),
-                                   reinterpret_cast<const int64_t*>(A.rowOffsets),
-                                   reinterpret_cast<const int64_t*>(A.columnIndices), A.values,
-                                   b.data(), x0.data(), x.data(), &r));
+    if (pc_) pc_->error = nullptr;
+    const int rc = cfdhip_csr_solve(ctx_.get(), &s, static_cast<int64_t>(A.rows),
+                                    reinterpret_cast<const int64_t*>(A.rowOffsets),
+                                    reinterpret_cast<const int64_t*>(A.columnIndices), A.values,
+                                    b.data(), x0.data(), x.data(), &r);
+    // what the host preconditioner threw leaves solve() as it would the reference's
+    if (pc_ && pc_->error) std::rethrow_exception(std::exchange(pc_->error, nullptr));
+    detail::check(ctx_.get(), rc);
     SolverResult out = detail::fromC(r, ctx_.get());
     out.solution = std::move(x);
     return out;
@@ -177,27 +195,54 @@ class GpuLinearSolver {
   }
 
  private:
+  struct HostCall {
+    HostPreconditioner fn;
+    std::exception_ptr error;  // what fn threw, rethrown by solve()
+  };
+  // No exception crosses the C boundary: it is kept and the library told to stop.
+  static int trampoline(const double* r, double* z, int64_t n, void* user) {
+    auto* call = static_cast<HostCall*>(user);
+    try {
+      call->fn(r, z, static_cast<std::size_t>(n));
+      return 0;
+    } catch (...) {
+      call->error = std::current_exception();
+      return 1;
+    }
+  }
   LinearSolverSettings settings_;
   detail::CtxPtr ctx_;
+  std::unique_ptr<HostCall> pc_;  // stable address for the C callback
 };
 
 namespace detail {
-inline std::unique_ptr<GpuLinearSolver> make(LinearSolverSettings s, LinearSolverType type) {
+inline std::unique_ptr<GpuLinearSolver> make(LinearSolverSettings s, LinearSolverType type,
+                                             HostPreconditioner pc = nullptr) {
   s.type = type;
   s.validate();  // the reference's constructor validation throws the same way
   if (!deviceAvailable()) return nullptr;
   CtxPtr ctx(cfdhip_create(0));
   if (!ctx) return nullptr;
+  if (pc) return std::make_unique<GpuLinearSolver>(s, std::move(ctx), std::move(pc));
   return std::make_unique<GpuLinearSolver>(s, std::move(ctx));
 }
 }  // namespace detail
 
-// makeGpuCG / makeGpuBiCGSTAB: nullptr iff the device is unusable.
+// makeGpuCG / makeGpuBiCGSTAB: nullptr iff the device is unusable. The overloads with
+// a host preconditioner apply it in place of the settings' preconditioner.
 [[nodiscard]] inline std::unique_ptr<GpuLinearSolver> makeGpuCG(LinearSolverSettings s) {
   return detail::make(s, LinearSolverType::CG);
 }
 [[nodiscard]] inline std::unique_ptr<GpuLinearSolver> makeGpuBiCGSTAB(LinearSolverSettings s) {
   return detail::make(s, LinearSolverType::BiCGSTAB);
+}
+[[nodiscard]] inline std::unique_ptr<GpuLinearSolver> makeGpuCG(LinearSolverSettings s,
+                                                                HostPreconditioner pc) {
+  return detail::make(s, LinearSolverType::CG, std::move(pc));
+}
+[[nodiscard]] inline std::unique_ptr<GpuLinearSolver> makeGpuBiCGSTAB(LinearSolverSettings s,
+                                                                      HostPreconditioner pc) {
+  return detail::make(s, LinearSolverType::BiCGSTAB, std::move(pc));
 }
 
 // Boundary condition of one patch as prepare() receives it.

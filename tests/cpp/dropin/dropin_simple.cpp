@@ -22,6 +22,12 @@
 //                      (assemblePressureCorrection / setPressureCorrection)
 //   steady      the resident loop and the CPU run converged to outer
 //               tolerances of 1e-12.
+// Host preconditioners: makeGpuCG / makeGpuBiCGSTAB with a Preconditioner
+// object the device has no form of (symmetric Gauss-Seidel, a Jacobi the
+// factory cannot recognise, one that throws) on a 2D Poisson system, against
+// the reference's CG / BiCGSTAB with the same object: status, iterations
+// within one, solutions within the tolerance bound, one apply() per
+// iteration, and the exception of a failing apply() leaving solve().
 // Every device run must agree with its CPU run within 1e-10 relative (max
 // over cells / faces, relative to the field's max) on u, v, p and the mass
 // flux, with matching status and iteration count. A lifecycle section checks
@@ -35,12 +41,18 @@
 #include <string>
 #include <vector>
 
+#include "cfd/algebra/BiCGSTAB.hpp"
+#include "cfd/algebra/CG.hpp"
+#include "cfd/algebra/LinearSystem.hpp"
+#include "cfd/algebra/Preconditioner.hpp"
+#include "cfd/algebra/SparseMatrix.hpp"
 #include "cfd/boundary/FixedGradient.hpp"
 #include "cfd/core/Exception.hpp"
 #include "cfd/boundary/MovingWall.hpp"
 #include "cfd/boundary/Symmetry.hpp"
 #include "cfd/boundary/Wall.hpp"
 #include "cfd/gpu/GPUBackend.hpp"
+#include "cfd/gpu/GpuLinearSolver.hpp"
 #include "cfd/gpu/GpuSimpleDiscretization.hpp"
 #include "cfd/mesh/MeshGeometry.hpp"
 #include "cfd/mesh/MeshGrading.hpp"
@@ -265,6 +277,161 @@ void lifecycle(const Cavity& c) {
          "lifecycle: a Symmetry condition is declined (" + reason + ")");
 }
 
+// z = (D + U)^-1 D (D + L)^-1 r: symmetric Gauss-Seidel, SPD for an SPD A
+class SymmetricGaussSeidel final : public cfd::algebra::Preconditioner {
+ public:
+  void build(const cfd::algebra::SparseMatrix& m) override {
+    A_ = std::make_unique<cfd::algebra::SparseMatrix>(m);
+  }
+  void apply(const cfd::algebra::Vector& r, cfd::algebra::Vector& z) const override {
+    ++calls;
+    const std::size_t n = A_->rows();
+    const double* v = A_->valuesData();
+    const std::size_t* col = A_->columnIndicesData();
+    const std::size_t* rp = A_->rowOffsetsData();
+    std::vector<double> y(n);
+    for (std::size_t i = 0; i < n; ++i) {  // (D + L) y = r
+      double acc = r[i];
+      for (std::size_t k = rp[i]; k < rp[i + 1]; ++k)
+        if (col[k] < i) acc -= v[k] * y[col[k]];
+      y[i] = acc / A_->diagonal(i);
+    }
+    z = cfd::algebra::Vector(n);
+    for (std::size_t i = n; i-- > 0;) {  // (D + U) z = D y
+      double acc = A_->diagonal(i) * y[i];
+      for (std::size_t k = rp[i]; k < rp[i + 1]; ++k)
+        if (col[k] > i) acc -= v[k] * z[col[k]];
+      z[i] = acc / A_->diagonal(i);
+    }
+  }
+  mutable int calls = 0;
+
+ private:
+  std::unique_ptr<cfd::algebra::SparseMatrix> A_;
+};
+
+// diag(A)^-1 behind a type the factory does not know
+class WrappedJacobi final : public cfd::algebra::Preconditioner {
+ public:
+  void build(const cfd::algebra::SparseMatrix& m) override { inner_.build(m); }
+  void apply(const cfd::algebra::Vector& r, cfd::algebra::Vector& z) const override {
+    ++calls;
+    inner_.apply(r, z);
+  }
+  mutable int calls = 0;
+
+ private:
+  cfd::algebra::JacobiPreconditioner inner_;
+};
+
+// throws on its fifth application
+class FailingPreconditioner final : public cfd::algebra::Preconditioner {
+ public:
+  void build(const cfd::algebra::SparseMatrix&) override {}
+  void apply(const cfd::algebra::Vector& r, cfd::algebra::Vector& z) const override {
+    if (++calls == 5) throw NumericalError("FailingPreconditioner: fifth application");
+    z = r;
+  }
+  mutable int calls = 0;
+};
+
+// 5-point Poisson on an m x m grid with Dirichlet sides (SPD), sorted columns
+cfd::algebra::LinearSystem poisson(std::size_t m) {
+  std::vector<double> val;
+  std::vector<std::size_t> col, rp{0};
+  const std::size_t n = m * m;
+  for (std::size_t j = 0; j < m; ++j)
+    for (std::size_t i = 0; i < m; ++i) {
+      const std::size_t c = j * m + i;
+      auto put = [&](std::size_t k, double a) {
+        col.push_back(k);
+        val.push_back(a);
+      };
+      if (j > 0) put(c - m, -1.0);
+      if (i > 0) put(c - 1, -1.0);
+      put(c, 4.0 + 0.01 * static_cast<double>((c * 7) % 5));
+      if (i + 1 < m) put(c + 1, -1.0);
+      if (j + 1 < m) put(c + m, -1.0);
+      rp.push_back(col.size());
+    }
+  cfd::algebra::Vector b(n);
+  for (std::size_t c = 0; c < n; ++c) b[c] = std::sin(0.37 * static_cast<double>(c)) + 0.25;
+  return {cfd::algebra::SparseMatrix(n, n, std::move(val), std::move(col), std::move(rp)),
+          std::move(b)};
+}
+
+void hostPreconditioners() {
+  using cfd::algebra::SolverStatus;
+  const cfd::algebra::LinearSystem sys = poisson(48);
+  const std::size_t n = sys.size();
+  const cfd::algebra::Vector x0(n, 0.0);
+  auto set = [](LinearSolverBackend b) {
+    return linearSolver(LinearSolverType::CG, b, PreconditionerType::None, 0.0, 1e-10, 5000);
+  };
+  auto same = [&](const std::string& name, const cfd::algebra::SolverResult& g,
+                  const cfd::algebra::SolverResult& c) {
+    const double e = maxRel(n, [&](std::size_t i) { return g.solution[i]; },
+                            [&](std::size_t i) { return c.solution[i]; });
+    char buf[240];
+    std::snprintf(buf, sizeof(buf),
+                  "%s: status %d (cpu %d), %zu it (cpu %zu), backend GPU, max rel x %.2e", name.c_str(),
+                  int(g.status), int(c.status), std::size_t(g.iterations),
+                  std::size_t(c.iterations), e);
+    const long d = long(g.iterations) - long(c.iterations);
+    expect(g.status == SolverStatus::Converged && c.status == SolverStatus::Converged &&
+               d >= -1 && d <= 1 && e <= 1e-7 &&
+               g.backendUsed == LinearSolverBackend::GPU,
+           buf);
+  };
+
+  auto sgsC = std::make_shared<SymmetricGaussSeidel>();
+  auto sgsG = std::make_shared<SymmetricGaussSeidel>();
+  const auto cpuCg = cfd::algebra::CG(set(LinearSolverBackend::CPU), sgsC).solve(sys, x0);
+  const auto gpuCgSolver = cfd::gpu::makeGpuCG(set(LinearSolverBackend::GPU), sgsG);
+  expect(gpuCgSolver != nullptr, "host pc: makeGpuCG with symmetric Gauss-Seidel is a device solver");
+  if (gpuCgSolver == nullptr) return;
+  const auto gpuCg = gpuCgSolver->solve(sys, x0);
+  same("host pc cg+sgs", gpuCg, cpuCg);
+  expect(sgsG->calls >= int(gpuCg.iterations) && sgsG->calls <= int(gpuCg.iterations) + 1,
+         "host pc cg+sgs: " + std::to_string(sgsG->calls) + " host applications for " +
+             std::to_string(gpuCg.iterations) + " iterations");
+
+  auto wjC = std::make_shared<WrappedJacobi>();
+  auto wjG = std::make_shared<WrappedJacobi>();
+  const auto cpuJ = cfd::algebra::CG(set(LinearSolverBackend::CPU), wjC).solve(sys, x0);
+  const auto gpuJ = cfd::gpu::makeGpuCG(set(LinearSolverBackend::GPU), wjG)->solve(sys, x0);
+  same("host pc cg+wrapped jacobi", gpuJ, cpuJ);
+  expect(gpuCg.iterations < gpuJ.iterations,
+         "host pc: symmetric Gauss-Seidel (" + std::to_string(gpuCg.iterations) +
+             " it) beats Jacobi (" + std::to_string(gpuJ.iterations) + " it)");
+  LinearSolverSettings jac = set(LinearSolverBackend::GPU);
+  jac.preconditioner = PreconditionerType::Jacobi;
+  const auto devJ = cfd::gpu::makeGpuCG(jac)->solve(sys, x0);
+  same("host pc: wrapped jacobi vs the device's own jacobi", gpuJ, devJ);
+
+  LinearSolverSettings bs = set(LinearSolverBackend::CPU);
+  bs.type = LinearSolverType::BiCGSTAB;
+  const auto cpuB =
+      cfd::algebra::BiCGSTAB(bs, std::make_shared<SymmetricGaussSeidel>()).solve(sys, x0);
+  bs.backend = LinearSolverBackend::GPU;
+  const auto gpuB =
+      cfd::gpu::makeGpuBiCGSTAB(bs, std::make_shared<SymmetricGaussSeidel>())->solve(sys, x0);
+  same("host pc bicgstab+sgs", gpuB, cpuB);
+
+  const auto bad = cfd::gpu::makeGpuCG(set(LinearSolverBackend::GPU),
+                                       std::make_shared<FailingPreconditioner>());
+  std::string what = "no exception";
+  try {
+    (void)bad->solve(sys, x0);
+  } catch (const NumericalError& e) {
+    what = e.what();
+  } catch (const std::exception& e) {
+    what = std::string("other exception: ") + e.what();
+  }
+  expect(what == "FailingPreconditioner: fifth application",
+         "host pc: a throwing apply() leaves solve() with its exception (" + what + ")");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -275,6 +442,8 @@ int main(int argc, char** argv) {
     return 2;
   }
   const Cavity c = cavity(MeshGeometry::createCartesian2D(n, n, 1.0, 1.0));
+  std::printf("case: host preconditioners, 48 x 48 Poisson\n");
+  hostPreconditioners();
   std::printf("case: %zu x %zu cavity, Re 100, Rhie-Chow\n", n, n);
   lifecycle(c);
 

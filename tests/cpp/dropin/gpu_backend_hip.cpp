@@ -34,13 +34,28 @@ using cfd::algebra::Vector;
 
 class HipKrylovSolver final : public LinearSolver {
  public:
-  HipKrylovSolver(LinearSolverSettings settings, std::unique_ptr<cfdhip::GpuLinearSolver> device)
-      : LinearSolver(settings), device_(std::move(device)) {}
+  HipKrylovSolver(LinearSolverSettings settings, std::unique_ptr<cfdhip::GpuLinearSolver> device,
+                  std::shared_ptr<Preconditioner> host = nullptr)
+      : LinearSolver(settings), device_(std::move(device)), host_(std::move(host)) {}
 
   SolverResult solve(const LinearSystem& system, const Vector& initialGuess) const override {
     const auto& A = system.matrix();
     if (initialGuess.size() != system.size())
       throw cfd::InvalidArgumentError("LinearSolver::solve: initial guess size mismatch");
+    // a host preconditioner is built on this system's matrix, as the reference's
+    // solvers do before they iterate; one that rejects it makes the result
+    // InvalidSystem (cuda/kernels/GpuLinearSolverCuda.cpp:130-143)
+    if (host_) {
+      try {
+        host_->build(A);
+      } catch (const cfd::InvalidArgumentError&) {
+        SolverResult bad;
+        bad.solution = initialGuess;
+        bad.status = cfd::algebra::SolverStatus::InvalidSystem;
+        bad.backendUsed = cfd::algebra::LinearSolverBackend::GPU;
+        return bad;
+      }
+    }
     const cfdhip::CsrView view{A.rows(), A.rowOffsetsData(), A.columnIndicesData(),
                                A.valuesData()};
     cfdhip::SolverResult r;
@@ -67,6 +82,7 @@ class HipKrylovSolver final : public LinearSolver {
 
  private:
   std::unique_ptr<cfdhip::GpuLinearSolver> device_;
+  std::shared_ptr<Preconditioner> host_;  // applied by the device solver's host callback
 };
 
 cfdhip::LinearSolverSettings toDevice(const LinearSolverSettings& s, bool jacobi) {
@@ -82,8 +98,9 @@ cfdhip::LinearSolverSettings toDevice(const LinearSolverSettings& s, bool jacobi
 // The caller's preconditioner: a JacobiPreconditioner is diag(A)^-1, which
 // the device builds from the same matrix (the settings' Jacobi path);
 // IdentityPreconditioner is no preconditioner. Any other host preconditioner
-// has no device form here, and nullptr sends the factory to its CPU solver
-// (src/algebra/LinearSolverFactory.cpp:12-31).
+// has no device form: the device solver applies it through a host round trip
+// per application, as the reference's GPU solvers do
+// (cuda/kernels/GpuLinearSolverCuda.cpp:103-118).
 bool deviceForm(const LinearSolverSettings& s, const std::shared_ptr<Preconditioner>& pc,
                 bool& jacobi) {
   jacobi = s.preconditioner == PreconditionerType::Jacobi;
@@ -102,7 +119,20 @@ bool deviceForm(const LinearSolverSettings& s, const std::shared_ptr<Preconditio
 std::unique_ptr<LinearSolver> make(LinearSolverSettings settings,
                                    const std::shared_ptr<Preconditioner>& pc, bool cg) {
   bool jacobi = false;
-  if (!deviceForm(settings, pc, jacobi)) return nullptr;
+  if (!deviceForm(settings, pc, jacobi)) {
+    const cfdhip::LinearSolverSettings d = toDevice(settings, false);
+    Preconditioner* raw = pc.get();  // kept alive by the solver's host_ member
+    cfdhip::HostPreconditioner apply = [raw](const double* r, double* z, std::size_t n) {
+      Vector in(n), out(n);
+      for (std::size_t i = 0; i < n; ++i) in[i] = r[i];
+      raw->apply(in, out);
+      for (std::size_t i = 0; i < n; ++i) z[i] = out[i];
+    };
+    std::unique_ptr<cfdhip::GpuLinearSolver> device =
+        cg ? cfdhip::makeGpuCG(d, std::move(apply)) : cfdhip::makeGpuBiCGSTAB(d, std::move(apply));
+    if (device == nullptr) return nullptr;
+    return std::make_unique<HipKrylovSolver>(settings, std::move(device), pc);
+  }
   const cfdhip::LinearSolverSettings d = toDevice(settings, jacobi);
   std::unique_ptr<cfdhip::GpuLinearSolver> device =
       cg ? cfdhip::makeGpuCG(d) : cfdhip::makeGpuBiCGSTAB(d);
