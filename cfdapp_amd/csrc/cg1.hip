@@ -120,10 +120,65 @@ __global__ __launch_bounds__(256) void cgIterKernel(
   constexpr int kRowsPerWave = TileShape<ROWS>::kRowsPerWave;
   constexpr int kInteriorRows = TileShape<ROWS>::kInteriorRows;
   __shared__ T uTile[kTileRows][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t tile = xcdTile(blockIdx.x, gridDim.x);
+  const int64_t tx = tile % tilesX, ty = tile / tilesX;
+  const int64_t i = tx * kOut - 1 + lane;
+  const int64_t jBase = ty * kInteriorRows - 1;  // tile row 0
+  const bool col = i >= 0 && i < g.nx;
+  const bool outCol = col && lane >= 1 && lane <= kOut;
+  // The tile's operands in registers: diag, r, w, s and the x-face coupling
+  // of every tile point (row part 1), p and x of the interior (part 1), the
+  // y-face couplings of the interior (part 2). Points outside the grid keep
+  // d = 1 and zeros. The fused kernel loads all of them up front; the other
+  // forms load a row just before its use, which keeps their register count
+  // (and occupancy) where a bandwidth-bound grid needs it.
+  T dK[kRowsPerWave], rK[kRowsPerWave], wK[kRowsPerWave], sK[kRowsPerWave];
+  T fxK[kRowsPerWave], pK[kRowsPerWave], fySK[kRowsPerWave], fyNK[kRowsPerWave];
+  TX xK[kRowsPerWave];
+  auto load1 = [&](int k) {
+    const int tr = wave * kRowsPerWave + k;
+    const int64_t j = jBase + tr;
+    const bool valid = col && j >= 0 && j < g.ny;
+    const bool center = valid && outCol && tr >= 1 && tr <= kInteriorRows;
+    dK[k] = T(1);
+    rK[k] = wK[k] = sK[k] = fxK[k] = pK[k] = T(0);
+    xK[k] = TX(0);
+    if (valid) {
+      const int64_t c = j * g.nx + i;
+      dK[k] = diag[c];
+      rK[k] = rOld[c];
+      fxK[k] = coeff[j * (g.nx + 1) + i];
+      if (!start) {
+        wK[k] = wOld[c];
+        sK[k] = sOld[c];
+        if (center) {
+          pK[k] = pOld[c];
+          xK[k] = x[c];
+        }
+      }
+    }
+  };
+  auto load2 = [&](int k) {
+    const int tr = wave * kRowsPerWave + k;
+    const int64_t j = jBase + tr;
+    const bool center = outCol && j >= 0 && j < g.ny && tr >= 1 && tr <= kInteriorRows;
+    fySK[k] = fyNK[k] = T(0);
+    if (center) {
+      fySK[k] = coeff[g.offY + j * g.nx + i];
+      fyNK[k] = coeff[g.offY + (j + 1) * g.nx + i];
+    }
+  };
   if constexpr (FUSED) {
+    // the tile's loads are issued before the previous iteration's partials
+    // are summed, so the two memory round trips overlap
+#pragma unroll
+    for (int k = 0; k < kRowsPerWave; ++k) {
+      load1(k);
+      load2(k);
+    }
     __shared__ KState ks;
     __shared__ double sums[3][kRedThreads / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double t3[3] = {0.0, 0.0, 0.0};
     if (fuse.doStep)
       for (int64_t b = threadIdx.x; b < fuse.nPartsIn; b += blockDim.x)
@@ -133,7 +188,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
     for (int q = 0; q < 3; ++q) {
       double y = t3[q];
       for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
-      if (lane == 0) sums[q][wid] = y;
+      if (lane == 0) sums[q][wave] = y;
     }
     if (threadIdx.x == 0) ks = *st;
     __syncthreads();
@@ -142,7 +197,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
         double t[3];
         for (int q = 0; q < 3; ++q) {
           double y = 0.0;
-          for (int i = 0; i < kRedThreads / 64; ++i) y += sums[q][i];
+          for (int w = 0; w < kRedThreads / 64; ++w) y += sums[q][w];
           t[q] = y;
         }
         cg1IterStep(ks, blockIdx.x == 0 ? fuse.hist : nullptr, t);
@@ -157,15 +212,8 @@ __global__ __launch_bounds__(256) void cgIterKernel(
   const T beta = static_cast<T>(st->beta);
   const TX alphaX = static_cast<TX>(st->alpha);
   const bool first = st->iter == st->dirStart;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t tile = xcdTile(blockIdx.x, gridDim.x);
-  const int64_t tx = tile % tilesX, ty = tile / tilesX;
-  const int64_t i = tx * kOut - 1 + lane;
-  const int64_t jBase = ty * kInteriorRows - 1;  // tile row 0
-  const bool col = i >= 0 && i < g.nx;
-  const bool outCol = col && lane >= 1 && lane <= kOut;
   double acc[NSUM] = {};
-  T dK[kRowsPerWave], uK[kRowsPerWave], fxW[kRowsPerWave];
+  T uK[kRowsPerWave];
   if (!skip) {
 #pragma unroll
     for (int k = 0; k < kRowsPerWave; ++k) {
@@ -173,27 +221,27 @@ __global__ __launch_bounds__(256) void cgIterKernel(
       const int64_t j = jBase + tr;
       const bool valid = col && j >= 0 && j < g.ny;
       const bool center = valid && outCol && tr >= 1 && tr <= kInteriorRows;
-      T u = T(0), d = T(1), fx = T(0);
+      if constexpr (!FUSED) load1(k);
+      T u = T(0);
       if (valid) {
         const int64_t c = j * g.nx + i;
-        d = diag[c];
-        const T r0 = rOld[c];
+        const T d = dK[k];
+        const T r0 = rK[k];
         T sn = T(0), rn = r0;
         if (!start) {
-          const T w0 = wOld[c];
-          sn = first ? w0 : w0 + (sOld[c] * beta);
+          const T w0 = wK[k];
+          sn = first ? w0 : w0 + (sK[k] * beta);
           rn = r0 - (sn * alpha);
         }
         u = (T(1) / d) * rn;
-        fx = coeff[j * (g.nx + 1) + i];
         if (center) {
           if (!start) {
             const T uOld = (T(1) / d) * r0;
-            const T p = first ? uOld : uOld + (pOld[c] * beta);
+            const T p = first ? uOld : uOld + (pK[k] * beta);
             pNew[c] = p;
             sNew[c] = sn;
             rNew[c] = rn;
-            x[c] = x[c] + (static_cast<TX>(p) * alphaX);
+            x[c] = xK[k] + (static_cast<TX>(p) * alphaX);
             if constexpr (NSUM == 6) {
               acc[3] += static_cast<double>(u) * static_cast<double>(sn);
               acc[4] += static_cast<double>(p) * static_cast<double>(sn);
@@ -205,9 +253,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
         }
       }
       uTile[tr][lane] = u;
-      dK[k] = d;
       uK[k] = u;
-      fxW[k] = fx;
     }
   }
   __syncthreads();
@@ -216,15 +262,14 @@ __global__ __launch_bounds__(256) void cgIterKernel(
     for (int k = 0; k < kRowsPerWave; ++k) {
       const int tr = wave * kRowsPerWave + k;
       const int64_t j = jBase + tr;
-      const T fxE = __shfl_down(fxW[k], 1, 64);
+      if constexpr (!FUSED) load2(k);
+      const T fxE = __shfl_down(fxK[k], 1, 64);
       const T uW = __shfl_up(uK[k], 1, 64), uE = __shfl_down(uK[k], 1, 64);
       const bool center = outCol && j >= 0 && j < g.ny && tr >= 1 && tr <= kInteriorRows;
       if (center) {
         const int64_t c = j * g.nx + i;
-        const T fyS = coeff[g.offY + j * g.nx + i];
-        const T fyN = coeff[g.offY + (j + 1) * g.nx + i];
         const T w = applyRow<T>(g, i, j, c, dK[k], uTile[tr - 1][lane], uW, uK[k], uE,
-                                uTile[tr + 1][lane], fyS, fxW[k], fxE, fyN);
+                                uTile[tr + 1][lane], fySK[k], fxK[k], fxE, fyNK[k]);
         wNew[c] = w;
         acc[1] += static_cast<double>(w) * static_cast<double>(uK[k]);
       }
