@@ -54,18 +54,6 @@ __device__ inline T applyRow(const Grid2& g, int64_t i, int64_t j, int64_t c, T 
   return y;
 }
 
-// XCD-aware tile order: workgroups are dispatched to the 8 XCDs round-robin
-// by workgroup id, so workgroup b runs on XCD b % 8. Giving each XCD one
-// contiguous range of tiles (row-major) keeps the halo a tile shares with its
-// left/right and upper/lower neighbours in the same L2 instead of fetching it
-// once per XCD. A bijection of [0, T) for any tile count T.
-constexpr int kXcds = 8;
-__device__ inline int64_t xcdTile(int64_t b, int64_t T) {
-  const int64_t q = T / kXcds, r = T % kXcds;
-  const int64_t x = b % kXcds, k = b / kXcds;
-  return x * q + (x < r ? x : r) + k;
-}
-
 // Tile geometry: a block covers 62 interior columns (+1 halo each side, one
 // lane per column) and 30 (FP64) or 62 (FP32) interior rows (+1 halo each
 // side); each of its 4 waves owns a quarter of the tile rows. On large grids
@@ -121,7 +109,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
   constexpr int kInteriorRows = TileShape<ROWS>::kInteriorRows;
   __shared__ T uTile[kTileRows][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t tile = xcdTile(blockIdx.x, gridDim.x);
+  const int64_t tile = xcdTileOrder(blockIdx.x, gridDim.x);
   const int64_t tx = tile % tilesX, ty = tile / tilesX;
   const int64_t i = tx * kOut - 1 + lane;
   const int64_t jBase = ty * kInteriorRows - 1;  // tile row 0

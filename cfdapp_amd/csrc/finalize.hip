@@ -9,12 +9,6 @@ namespace cfdhip {
 
 namespace {
 
-// cancelledToRoundingLevel (BiCGSTAB.cpp:34-41) with sum_i |x_i y_i| at hand.
-__device__ inline bool cancelled(double ip, double absSum, double nx, double ny) {
-  if (fabs(ip) > DBL_EPSILON * nx * ny) return false;
-  return fabs(ip) <= DBL_EPSILON * absSum;
-}
-
 __device__ void step(KState& s, double* hist, double* out, int stepId, const double* t) {
   switch (stepId) {
     case kStepStartNorm: {  // r0 = b - A x0 (CG.cpp:45-57, BiCGSTAB.cpp:82-93)
@@ -100,108 +94,13 @@ __device__ void step(KState& s, double* hist, double* out, int stepId, const dou
     case kStepCg1Iter:
       cg1IterStep(s, hist, t);
       return;
-    case kStepBcgRho: {  // BiCGSTAB.cpp:149-163
-      const double rho = t[0];
-      if (!isfinite(rho)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      if (cancelled(rho, t[1], s.rHatNorm, s.rnorm)) {
-        s.restart = 1;
-        return;
-      }
-      const double beta = (rho / s.rhoOld) * (s.alpha / s.omega);
-      if (!isfinite(beta)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      s.rho = rho;
-      s.beta = beta;
+    case kStepBcgRho:
+    case kStepBcgV:
+    case kStepBcgS:
+    case kStepBcgT:
+    case kStepBcgX:
+      bcgStep(s, hist, stepId, t);
       return;
-    }
-    case kStepBcgV: {  // BiCGSTAB.cpp:171-185
-      const double rHatDotV = t[0];
-      if (!isfinite(rHatDotV)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      if (cancelled(rHatDotV, t[1], s.rHatNorm, sqrt(t[2]))) {
-        s.restart = 1;
-        return;
-      }
-      const double alpha = s.rho / rHatDotV;
-      if (!isfinite(alpha)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      s.alpha = alpha;
-      return;
-    }
-    case kStepBcgS: {  // BiCGSTAB.cpp:187-207
-      const double sNorm = sqrt(t[0]);
-      s.snorm = sNorm;
-      const int64_t iter = s.completed + 1;
-      if (!isfinite(sNorm)) {
-        s.iter = iter;
-        pushHist(s, hist, sNorm);
-        s.rnorm = sNorm;
-        finish(s, CFDHIP_NONFINITE_RESIDUAL);
-        return;
-      }
-      if (convergedTest(s, sNorm)) {
-        s.iter = iter;
-        pushHist(s, hist, sNorm);
-        s.rnorm = sNorm;
-        s.sConverged = 1;  // x += pHat alpha in the next worker
-      }
-      return;
-    }
-    case kStepBcgT: {  // BiCGSTAB.cpp:209-238
-      if (s.sConverged) {
-        finish(s, CFDHIP_CONVERGED);
-        return;
-      }
-      const double tDotT = t[0];
-      if (!isfinite(tDotT) || tDotT < DBL_MIN) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      const double tDotS = t[1];
-      if (!isfinite(tDotS)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      if (cancelled(tDotS, t[2], sqrt(tDotT), s.snorm)) {
-        s.restart = 1;
-        return;
-      }
-      const double omega = tDotS / tDotT;
-      if (!isfinite(omega)) {
-        finish(s, CFDHIP_BREAKDOWN);
-        return;
-      }
-      s.omega = omega;
-      return;
-    }
-    case kStepBcgX: {  // BiCGSTAB.cpp:240-268
-      const double rn = sqrt(t[0]);
-      const int64_t iter = s.completed + 1;
-      s.rnorm = rn;
-      s.iter = iter;
-      pushHist(s, hist, rn);
-      if (!isfinite(rn)) {
-        finish(s, CFDHIP_NONFINITE_RESIDUAL);
-        return;
-      }
-      s.completed = iter;
-      if (convergedTest(s, rn)) {
-        finish(s, CFDHIP_CONVERGED);
-        return;
-      }
-      s.rhoOld = s.rho;
-      if (s.completed >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
-      return;
-    }
     case kStepBcgRestart: {  // BiCGSTAB.cpp:128-144, after the rNorm < restartResidual test
       const double norm = sqrt(t[0]);
       s.rnorm = norm;

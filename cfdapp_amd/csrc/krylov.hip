@@ -387,6 +387,301 @@ __global__ void bcgXKernel(int64_t n, double* x, double* r, const double* __rest
   blockPartials<1>(acc, R.partial);
 }
 
+// ---------------------------------------------------------------------------
+// Fused BiCGSTAB for the 2D momentum stencil on small grids. The worker
+// kernels above and their five finalize launches make eleven dependent
+// launches per iteration; here three kernels do an iteration, each running
+// the scalar steps that precede its work itself (bcgStep, redundantly in
+// every block, on the previous kernel's partials summed in one fixed order):
+//   dirV   [X, Rho]  p = r + (p - v omega) beta, pHat = M p, v = A pHat
+//   sT     [V]       s = r - v alpha, sHat = M s, t = A sHat
+//   xr     [S, T]    x += pHat alpha (+ sHat omega), r = s - t omega
+// dirV and sT form the preconditioned vector of their tile's halo ring
+// themselves (the same expression on the same inputs, so the same bits as
+// the ring's own tile) and apply the stencil to it from LDS. p and v are
+// double-buffered, since a halo reads the previous iterate. Tiles as in the
+// single-kernel PCG: 62 interior columns (one lane each, +1 halo each side)
+// by ROWS - 2 interior rows, four waves.
+constexpr int kBcgOut = 62;
+constexpr int64_t kBcgMaxTiles = 256;
+
+struct Bcg2 {
+  int64_t nx, ny, n, tilesX;
+  const double* coef;  // 5 n: the S, W, C, E, N coefficients of each row
+  const double* dinv;  // Jacobi inverse diagonal, or null
+  const double* rhat;
+  double* r;
+  double* x;
+  double* p[2];
+  double* v[2];
+  double* phat;
+  double* s;
+  double* shat;
+  double* t;
+  double* hist;
+};
+
+struct Bcg2Launch {
+  int cur;               // p[cur], v[cur]: the previous iterate
+  const KState* stIn;    // the state the steps start from
+  KState* stOut;         // block 0 writes the stepped state here
+  const double* partIn;  // the previous kernel's partials (one W-tuple per block)
+  double* partOut;
+  int pending;           // dirV: run the X and Rho steps first (0 after a chunk boundary)
+};
+
+// Sums of W partial streams over n blocks, the same order in every block;
+// valid in thread 0.
+template <int W>
+__device__ inline void bcgSumParts(const double* __restrict__ part, int64_t n, double (&t)[W]) {
+  __shared__ double lds[W][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double a[W];
+#pragma unroll
+  for (int q = 0; q < W; ++q) a[q] = 0.0;
+  for (int64_t b = threadIdx.x; b < n; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < W; ++q) a[q] += part[b * W + q];
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    double y = a[q];
+    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+    if (lane == 0) lds[q][wid] = y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < W; ++q) {
+      double y = 0.0;
+      for (int i = 0; i < kRedThreads / 64; ++i) y += lds[q][i];
+      t[q] = y;
+    }
+}
+
+// a BiCGSTAB step unless the solve is done or waits for a restart (the
+// finalize kernel's rule)
+__device__ inline void bcgStepIfRunning(KState& s, double* hist, int stepId, const double* t) {
+  if (!s.done && !s.restart) bcgStep(s, hist, stepId, t);
+}
+
+template <int ROWS>
+struct BcgTile {
+  static constexpr int kRowsPerWave = ROWS / 4;
+  static constexpr int kInterior = ROWS - 2;
+  int64_t i, jBase;
+  bool col, outCol;
+  int lane, wave;
+  __device__ BcgTile(const Bcg2& a) {
+    lane = threadIdx.x & 63;
+    wave = threadIdx.x >> 6;
+    const int64_t tile = xcdTileOrder(blockIdx.x, gridDim.x);
+    const int64_t tx = tile % a.tilesX, ty = tile / a.tilesX;
+    i = tx * kBcgOut - 1 + lane;
+    jBase = ty * kInterior - 1;
+    col = i >= 0 && i < a.nx;
+    outCol = col && lane >= 1 && lane <= kBcgOut;
+  }
+  __device__ int row(int k) const { return wave * kRowsPerWave + k; }
+  __device__ bool valid(const Bcg2& a, int tr) const {
+    const int64_t j = jBase + tr;
+    return col && j >= 0 && j < a.ny;
+  }
+  __device__ bool center(const Bcg2& a, int tr) const {
+    return valid(a, tr) && outCol && tr >= 1 && tr <= kInterior;
+  }
+};
+
+// Row c of the momentum stencil (StencilOp::apply in 2D: S, W, C, E, N, the
+// order of SparseMatrix::multiply) on the tile values h (LDS) and the row's
+// own value hc and its west/east neighbours.
+template <int ROWS>
+__device__ inline double bcgApply(const Bcg2& a, int64_t i, int64_t j, int64_t c,
+                                  const double (&h)[ROWS][64], int tr, int lane, double hW,
+                                  double hC, double hE) {
+  double y = 0.0;
+  if (j > 0) y += a.coef[c] * h[tr - 1][lane];
+  if (i > 0) y += a.coef[a.n + c] * hW;
+  y += a.coef[2 * a.n + c] * hC;
+  if (i < a.nx - 1) y += a.coef[3 * a.n + c] * hE;
+  if (j < a.ny - 1) y += a.coef[4 * a.n + c] * h[tr + 1][lane];
+  return y;
+}
+
+// the scalar steps of a launch, run by thread 0 of every block on its own
+// copy; block 0 publishes the result
+template <int W, class F>
+__device__ inline void bcgSteps(const Bcg2& a, const Bcg2Launch& L, KState& ks, bool sum, F run) {
+  double t[W > 0 ? W : 1];
+  if constexpr (W > 0) {
+    if (sum) bcgSumParts<W>(L.partIn, gridDim.x, t);
+  }
+  if (threadIdx.x == 0) {
+    ks = *L.stIn;
+    if (sum) run(ks, blockIdx.x == 0 ? a.hist : nullptr, t);
+    if (blockIdx.x == 0) *L.stOut = ks;
+  }
+  __syncthreads();
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void bcgDirVKernel(Bcg2 a, Bcg2Launch L) {
+  __shared__ KState ks;
+  __shared__ double h[ROWS][64];
+  bcgSteps<3>(a, L, ks, L.pending != 0, [](KState& s, double* hist, const double* t) {
+    bcgStepIfRunning(s, hist, kStepBcgX, t);
+    bcgStepIfRunning(s, hist, kStepBcgRho, t + 1);
+  });
+  double acc[3] = {0.0, 0.0, 0.0};
+  const BcgTile<ROWS> T(a);
+  if (!ks.done && !ks.restart) {
+    const double beta = ks.beta, omega = ks.omega;
+    const double* pOld = a.p[L.cur];
+    const double* vOld = a.v[L.cur];
+    double* pNew = a.p[1 - L.cur];
+    double* vNew = a.v[1 - L.cur];
+    double hK[BcgTile<ROWS>::kRowsPerWave];
+#pragma unroll
+    for (int k = 0; k < BcgTile<ROWS>::kRowsPerWave; ++k) {
+      const int tr = T.row(k);
+      double ph = 0.0;
+      if (T.valid(a, tr)) {
+        const int64_t c = (T.jBase + tr) * a.nx + T.i;
+        const double pn = a.r[c] + ((pOld[c] - (vOld[c] * omega)) * beta);
+        ph = a.dinv ? a.dinv[c] * pn : pn;
+        if (T.center(a, tr)) {
+          pNew[c] = pn;
+          a.phat[c] = ph;
+        }
+      }
+      h[tr][T.lane] = ph;
+      hK[k] = ph;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BcgTile<ROWS>::kRowsPerWave; ++k) {
+      const int tr = T.row(k);
+      const double hW = __shfl_up(hK[k], 1, 64), hE = __shfl_down(hK[k], 1, 64);
+      if (T.center(a, tr)) {
+        const int64_t j = T.jBase + tr, c = j * a.nx + T.i;
+        const double y = bcgApply<ROWS>(a, T.i, j, c, h, tr, T.lane, hW, hK[k], hE);
+        vNew[c] = y;
+        const double q = a.rhat[c] * y;
+        acc[0] += q;
+        acc[1] += fabs(q);
+        acc[2] += y * y;
+      }
+    }
+  }
+  blockPartials<3>(acc, L.partOut);
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void bcgSTKernel(Bcg2 a, Bcg2Launch L) {
+  __shared__ KState ks;
+  __shared__ double h[ROWS][64];
+  bcgSteps<3>(a, L, ks, true, [](KState& s, double* hist, const double* t) {
+    bcgStepIfRunning(s, hist, kStepBcgV, t);
+  });
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const BcgTile<ROWS> T(a);
+  if (!ks.done && !ks.restart) {
+    const double alpha = ks.alpha;
+    const double* v = a.v[1 - L.cur];
+    double hK[BcgTile<ROWS>::kRowsPerWave], sK[BcgTile<ROWS>::kRowsPerWave];
+#pragma unroll
+    for (int k = 0; k < BcgTile<ROWS>::kRowsPerWave; ++k) {
+      const int tr = T.row(k);
+      double sh = 0.0, si = 0.0;
+      if (T.valid(a, tr)) {
+        const int64_t c = (T.jBase + tr) * a.nx + T.i;
+        si = a.r[c] - (v[c] * alpha);
+        sh = a.dinv ? a.dinv[c] * si : si;
+        if (T.center(a, tr)) {
+          a.s[c] = si;
+          a.shat[c] = sh;
+          acc[0] += si * si;
+        }
+      }
+      h[tr][T.lane] = sh;
+      hK[k] = sh;
+      sK[k] = si;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BcgTile<ROWS>::kRowsPerWave; ++k) {
+      const int tr = T.row(k);
+      const double hW = __shfl_up(hK[k], 1, 64), hE = __shfl_down(hK[k], 1, 64);
+      if (T.center(a, tr)) {
+        const int64_t j = T.jBase + tr, c = j * a.nx + T.i;
+        const double y = bcgApply<ROWS>(a, T.i, j, c, h, tr, T.lane, hW, hK[k], hE);
+        a.t[c] = y;
+        const double ts = y * sK[k];
+        acc[1] += y * y;
+        acc[2] += ts;
+        acc[3] += fabs(ts);
+      }
+    }
+  }
+  blockPartials<4>(acc, L.partOut);
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void bcgXRKernel(Bcg2 a, Bcg2Launch L) {
+  __shared__ KState ks;
+  __shared__ int mode;  // 0: nothing, 1: the early x update (s converged), 2: full update
+  double t[4];
+  bcgSumParts<4>(L.partIn, gridDim.x, t);
+  if (threadIdx.x == 0) {
+    ks = *L.stIn;
+    int m = 0;
+    if (!ks.done && !ks.restart) {
+      double* hist = blockIdx.x == 0 ? a.hist : nullptr;
+      bcgStep(ks, hist, kStepBcgS, t);
+      if (!ks.done) {
+        // the early update needs only the S step; the T step then finishes
+        m = ks.sConverged ? 1 : 2;
+        bcgStep(ks, hist, kStepBcgT, t + 1);
+        if (m == 2 && (ks.done || ks.restart)) m = 0;
+      }
+    }
+    mode = m;
+    if (blockIdx.x == 0) *L.stOut = ks;
+  }
+  __syncthreads();
+  double acc[3] = {0.0, 0.0, 0.0};
+  const BcgTile<ROWS> T(a);
+  if (mode != 0) {
+    const double alpha = ks.alpha, omega = ks.omega;
+#pragma unroll
+    for (int k = 0; k < BcgTile<ROWS>::kRowsPerWave; ++k) {
+      const int tr = T.row(k);
+      if (!T.center(a, tr)) continue;
+      const int64_t c = (T.jBase + tr) * a.nx + T.i;
+      a.x[c] += a.phat[c] * alpha;
+      if (mode == 1) continue;
+      a.x[c] += a.shat[c] * omega;
+      const double ri = a.s[c] - (a.t[c] * omega);
+      a.r[c] = ri;
+      acc[0] += ri * ri;
+      const double q = a.rhat[c] * ri;
+      acc[1] += q;
+      acc[2] += fabs(q);
+    }
+  }
+  blockPartials<3>(acc, L.partOut);
+}
+
+// The X and Rho steps of the last xr kernel of a chunk (one block).
+__global__ __launch_bounds__(256) void bcgPendingKernel(const double* part, int64_t nParts,
+                                                        KState* st, double* hist) {
+  double t[3];
+  bcgSumParts<3>(part, nParts, t);
+  if (threadIdx.x == 0) {
+    KState& s = *st;
+    bcgStepIfRunning(s, hist, kStepBcgX, t);
+    bcgStepIfRunning(s, hist, kStepBcgRho, t + 1);
+  }
+}
+
 __global__ void zeroKernel(int64_t n, double* a, double* b) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -714,6 +1009,155 @@ cfdhip_solve_result runBiCGSTAB(Context& ctx, Op A, int64_t n, const cfdhip_solv
   return S.result(tr, ms);
 }
 
+// BiCGSTAB (BiCGSTAB.cpp:48-275) on the 2D momentum stencil of a small grid
+// with the fused kernels: the same recurrence, statuses, restarts and
+// history as runBiCGSTAB, three launches per iteration. A chunk of
+// check_interval iterations starts from a state whose Rho step is done
+// (bcgRhoKernel + finalize at the start and after a restart), runs its
+// iterations, and ends with the pending X and Rho steps in one block; the
+// state then lives in S[0] and the host reads it once per chunk.
+cfdhip_solve_result runBiCGSTABFused2d(Context& ctx, const StencilOp& A, int rows, int64_t tiles,
+                                       const cfdhip_solver_settings& set, const double* b,
+                                       double* x,
+                                       const std::vector<std::pair<const double*, int64_t>>& opArrays) {
+  const View& vw = ctx.view;
+  const int64_t n = vw.n;
+  KrylovWork& w = ctx.kw;
+  for (DeviceArray<double>* a : {&w.r, &w.p, &w.rhat, &w.v, &w.s, &w.t, &w.phat, &w.shat,
+                                 &w.dinv, &w.Ap, &w.z})
+    a->ensure(n);
+  w.state.ensure(2);  // before Solve takes the address of state 0
+  w.partial.ensure(kMaxBlocks * 4);
+  Solve S(ctx, n);
+  S.begin(set);
+  auto arrays = opArrays;
+  arrays.push_back({b, n});
+  arrays.push_back({x, n});
+  if (!inputsFinite(ctx, A, arrays)) {
+    S.hs->status = CFDHIP_NONFINITE_INPUT;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  const bool jacobi = set.preconditioner == CFDHIP_PREC_JACOBI;
+  if (jacobi && !buildJacobi(ctx, A, n, w.dinv.get())) {
+    S.hs->status = CFDHIP_INVALID_SYSTEM;
+    S.hs->done = 1;
+    return S.result(NAN, 0.0);
+  }
+  KState* St[2] = {S.R.st, S.R.st + 1};
+  double* P[2] = {w.partial.get(), w.partial.get() + 4 * kBcgMaxTiles};
+  // p and v alternate between (p, Ap) and (v, z)
+  Bcg2 a{vw.nx,     vw.ny,  n,         (vw.nx + kBcgOut - 1) / kBcgOut,
+         A.coef,    jacobi ? w.dinv.get() : nullptr,   w.rhat.get(),
+         w.r.get(), x,      {w.p.get(), w.Ap.get()},   {w.v.get(), w.z.get()},
+         w.phat.get(), w.s.get(), w.shat.get(), w.t.get(), ctx.kw.hist.get()};
+  Timer timer(ctx.stream);
+  const dim3 g = S.grid();
+  const dim3 gt(static_cast<unsigned>(tiles));
+  int cur = 0;
+  auto zeroPV = [&]() {
+    hipLaunchKernelGGL(zeroKernel, g, dim3(kBlock), 0, ctx.stream, n, a.v[cur], a.p[cur]);
+  };
+  auto rho = [&]() {
+    hipLaunchKernelGGL(bcgRhoKernel, g, dim3(kBlock), 0, ctx.stream, n, w.rhat.get(), w.r.get(),
+                       S.R);
+    S.fin(kStepBcgRho, g);
+  };
+  hipLaunchKernelGGL((residualKernel<StencilOp>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x,
+                     w.r.get(), w.rhat.get(), S.R, 0);
+  S.fin(kStepStartNorm, g);
+  zeroPV();
+  rho();
+  CFDHIP_CHECK(hipGetLastError());
+  const int chunk = set.check_interval > 0 ? set.check_interval : 8;
+  int64_t launched = 0;
+  const int64_t launchCap = 4 * set.max_iterations + 64;
+#define CFDHIP_BCG_LAUNCH(KERNEL, LA)                                                        \
+  switch (rows) {                                                                             \
+    case 4: hipLaunchKernelGGL((KERNEL<4>), gt, dim3(256), 0, ctx.stream, a, LA); break;      \
+    case 8: hipLaunchKernelGGL((KERNEL<8>), gt, dim3(256), 0, ctx.stream, a, LA); break;      \
+    case 16: hipLaunchKernelGGL((KERNEL<16>), gt, dim3(256), 0, ctx.stream, a, LA); break;    \
+    default: hipLaunchKernelGGL((KERNEL<32>), gt, dim3(256), 0, ctx.stream, a, LA); break;    \
+  }
+  while (true) {
+    // launch q of the chunk reads state St[q & 1] and writes St[(q + 1) & 1],
+    // reads partials P[(q - 1) & 1] and writes P[q & 1]
+    int q = 0;
+    for (int k = 0; k < chunk && launched < launchCap; ++k, ++launched) {
+      const Bcg2Launch l1{cur, St[q & 1], St[(q + 1) & 1], P[(q + 1) & 1], P[q & 1], k > 0};
+      CFDHIP_BCG_LAUNCH(bcgDirVKernel, l1);
+      ++q;
+      const Bcg2Launch l2{cur, St[q & 1], St[(q + 1) & 1], P[(q + 1) & 1], P[q & 1], 0};
+      CFDHIP_BCG_LAUNCH(bcgSTKernel, l2);
+      ++q;
+      const Bcg2Launch l3{cur, St[q & 1], St[(q + 1) & 1], P[(q + 1) & 1], P[q & 1], 0};
+      CFDHIP_BCG_LAUNCH(bcgXRKernel, l3);
+      ++q;
+      cur = 1 - cur;
+    }
+    // the last xr's steps, into St[0]
+    if (q > 0) {
+      if ((q & 1) != 0)
+        CFDHIP_CHECK(hipMemcpyAsync(St[0], St[1], sizeof(KState), hipMemcpyDeviceToDevice,
+                                    ctx.stream));
+      hipLaunchKernelGGL(bcgPendingKernel, dim3(1), dim3(256), 0, ctx.stream, P[(q - 1) & 1],
+                         static_cast<int64_t>(tiles), St[0], ctx.kw.hist.get());
+    }
+    CFDHIP_CHECK(hipGetLastError());
+    S.pull();
+    if (S.hs->done) break;
+    if (S.hs->restart == 1) {
+      // BiCGSTAB.cpp:128-130: restart only after progress in this sequence.
+      if (!(S.hs->rnorm < S.hs->restartResidual)) {
+        S.hs->status = CFDHIP_BREAKDOWN;
+        S.hs->done = 1;
+        S.hs->iter = S.hs->completed;
+        break;
+      }
+      hipLaunchKernelGGL((residualKernel<StencilOp>), g, dim3(kBlock), 0, ctx.stream, A, n, b, x,
+                         w.r.get(), w.rhat.get(), S.R, 1);
+      S.fin(kStepBcgRestart, g);
+      zeroPV();
+      CFDHIP_CHECK(hipGetLastError());
+      S.pull();
+      if (S.hs->done) {
+        S.hs->iter = S.hs->completed;
+        break;
+      }
+      S.hs->restart = 0;
+      S.push();
+      rho();
+    }
+    if (launched >= launchCap) break;
+  }
+#undef CFDHIP_BCG_LAUNCH
+  const double ms = timer.stop();
+  if (!S.hs->done) {
+    S.hs->status = CFDHIP_MAX_ITERATIONS;
+    S.hs->done = 1;
+  }
+  if (S.hs->status == CFDHIP_BREAKDOWN) S.hs->iter = S.hs->completed;
+  const double tr = trueResidual(ctx, A, n, b, x);
+  return S.result(tr, ms);
+}
+
+// The fused form's tiles for a 2D grid: the shortest rows (4, 8, 16 or 32)
+// whose tile count fits kBcgMaxTiles; rows = 0 when none does.
+void bcgFusedTiles(const View& v, int& rows, int64_t& tiles) {
+  rows = 0;
+  tiles = 0;
+  if (v.dim != 2) return;
+  const int64_t tilesX = (v.nx + kBcgOut - 1) / kBcgOut;
+  for (int r : {4, 8, 16, 32}) {
+    const int64_t t = tilesX * ((v.ny + r - 3) / (r - 2));
+    if (t <= kBcgMaxTiles) {
+      rows = r;
+      tiles = t;
+      return;
+    }
+  }
+}
+
 // out = M in through the caller's host preconditioner: the one vector crosses
 // to the host and back per application, as in the reference's GPU solvers
 // (cuda/kernels/GpuLinearSolverCuda.cpp:103-118).
@@ -984,7 +1428,16 @@ cfdhip_solve_result solveStencil(Context& ctx, const cfdhip_solver_settings& s, 
   require(ctx.momentumAssembled, "no momentum system assembled");
   const int64_t n = ctx.view.n;
   StencilOp A{ctx.view, ctx.momCoef.get()};
-  return dispatch(ctx, A, n, s, b, x, {{ctx.momCoef.get(), (2 * ctx.view.dim + 1) * n}});
+  const std::vector<std::pair<const double*, int64_t>> arrays = {
+      {ctx.momCoef.get(), (2 * ctx.view.dim + 1) * n}};
+  // small 2D grids: the fused BiCGSTAB (three launches per iteration)
+  int rows = 0;
+  int64_t tiles = 0;
+  bcgFusedTiles(ctx.view, rows, tiles);
+  if (s.type == CFDHIP_SOLVER_BICGSTAB && rows > 0 &&
+      (s.preconditioner == CFDHIP_PREC_NONE || s.preconditioner == CFDHIP_PREC_JACOBI))
+    return runBiCGSTABFused2d(ctx, A, rows, tiles, s, b, x, arrays);
+  return dispatch(ctx, A, n, s, b, x, arrays);
 }
 
 cfdhip_solve_result solveCsr(Context& ctx, const cfdhip_solver_settings& s, const CsrDevice& M,

@@ -9,11 +9,25 @@
 // per block while the kernel is streaming its stores.
 #pragma once
 
+#include <cfloat>
+
 #include "context.hpp"
 
 namespace cfdhip {
 
 constexpr int kRedThreads = 256;
+
+// XCD-aware tile order: workgroups are dispatched to the 8 XCDs round-robin
+// by workgroup id, so workgroup b runs on XCD b % 8. Giving each XCD one
+// contiguous range of tiles (row-major) keeps the halo a tile shares with its
+// left/right and upper/lower neighbours in the same L2 instead of fetching it
+// once per XCD. A bijection of [0, T) for any tile count T.
+constexpr int kXcds = 8;
+__device__ inline int64_t xcdTileOrder(int64_t b, int64_t T) {
+  const int64_t q = T / kXcds, r = T % kXcds;
+  const int64_t x = b % kXcds, k = b / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
 
 template <int W>
 __device__ inline void blockPartials(double (&val)[W], double* partial) {
@@ -175,5 +189,124 @@ void launchFinalize(Context& ctx, const FinalArgs& a);
 void launchCaseFinalize(hipStream_t stream, const double* partial, const int64_t* slot0,
                         const int64_t* slots, const int32_t* active, int64_t nActive,
                         KState* states, int W, int stepId, double* out);
+
+
+// cancelledToRoundingLevel (BiCGSTAB.cpp:34-41) with sum_i |x_i y_i| at hand.
+__device__ inline bool cancelled(double ip, double absSum, double nx, double ny) {
+  if (fabs(ip) > DBL_EPSILON * nx * ny) return false;
+  return fabs(ip) <= DBL_EPSILON * absSum;
+}
+
+// The scalar steps of BiCGSTAB (BiCGSTAB.cpp:149-268) on the sums t of the
+// worker that precedes each: run by the finalize kernel and, redundantly in
+// every block, by the fused 2D stencil BiCGSTAB kernels.
+__device__ inline void bcgStep(KState& s, double* hist, int stepId, const double* t) {
+  switch (stepId) {
+    case kStepBcgRho: {  // BiCGSTAB.cpp:149-163
+      const double rho = t[0];
+      if (!isfinite(rho)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (cancelled(rho, t[1], s.rHatNorm, s.rnorm)) {
+        s.restart = 1;
+        return;
+      }
+      const double beta = (rho / s.rhoOld) * (s.alpha / s.omega);
+      if (!isfinite(beta)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.rho = rho;
+      s.beta = beta;
+      return;
+    }
+    case kStepBcgV: {  // BiCGSTAB.cpp:171-185
+      const double rHatDotV = t[0];
+      if (!isfinite(rHatDotV)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (cancelled(rHatDotV, t[1], s.rHatNorm, sqrt(t[2]))) {
+        s.restart = 1;
+        return;
+      }
+      const double alpha = s.rho / rHatDotV;
+      if (!isfinite(alpha)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.alpha = alpha;
+      return;
+    }
+    case kStepBcgS: {  // BiCGSTAB.cpp:187-207
+      const double sNorm = sqrt(t[0]);
+      s.snorm = sNorm;
+      const int64_t iter = s.completed + 1;
+      if (!isfinite(sNorm)) {
+        s.iter = iter;
+        pushHist(s, hist, sNorm);
+        s.rnorm = sNorm;
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      if (convergedTest(s, sNorm)) {
+        s.iter = iter;
+        pushHist(s, hist, sNorm);
+        s.rnorm = sNorm;
+        s.sConverged = 1;  // x += pHat alpha in the next worker
+      }
+      return;
+    }
+    case kStepBcgT: {  // BiCGSTAB.cpp:209-238
+      if (s.sConverged) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      const double tDotT = t[0];
+      if (!isfinite(tDotT) || tDotT < DBL_MIN) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      const double tDotS = t[1];
+      if (!isfinite(tDotS)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      if (cancelled(tDotS, t[2], sqrt(tDotT), s.snorm)) {
+        s.restart = 1;
+        return;
+      }
+      const double omega = tDotS / tDotT;
+      if (!isfinite(omega)) {
+        finish(s, CFDHIP_BREAKDOWN);
+        return;
+      }
+      s.omega = omega;
+      return;
+    }
+    case kStepBcgX: {  // BiCGSTAB.cpp:240-268
+      const double rn = sqrt(t[0]);
+      const int64_t iter = s.completed + 1;
+      s.rnorm = rn;
+      s.iter = iter;
+      pushHist(s, hist, rn);
+      if (!isfinite(rn)) {
+        finish(s, CFDHIP_NONFINITE_RESIDUAL);
+        return;
+      }
+      s.completed = iter;
+      if (convergedTest(s, rn)) {
+        finish(s, CFDHIP_CONVERGED);
+        return;
+      }
+      s.rhoOld = s.rho;
+      if (s.completed >= s.maxIter) finish(s, CFDHIP_MAX_ITERATIONS);
+      return;
+    }
+    default:
+      return;
+  }
+}
 
 }  // namespace cfdhip
