@@ -18,6 +18,7 @@ namespace cfdhip {
 
 Context::~Context() {
   if (kw.hostState) (void)hipHostFree(kw.hostState);
+  if (kw.hostState2) (void)hipHostFree(kw.hostState2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

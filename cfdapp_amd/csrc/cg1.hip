@@ -490,7 +490,41 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
     launchFinalize(ctx, FinalArgs{P[(gChunk - 1) & 1], fusedTiles, Sb[0], buf.hist, nullptr,
                                   kStepCg1Iter});
   };
-  while (true) {
+  // Pipelined chunks (fused kernels, stream launches): chunk c + 1 is queued
+  // before the host waits for chunk c's state, so the read-back and the
+  // host's launch work hide behind the device's work instead of idling it;
+  // a chunk queued after the verdict finds the state done and does nothing.
+  // Two pinned state mirrors, one per chunk in flight.
+  const bool pipelined = fused && !graphed && !timing && buf.events && buf.host2 != nullptr;
+  if (pipelined) {
+    KState* hb[2] = {host, buf.host2};
+    hipEvent_t ready[2] = {nullptr, nullptr};
+    for (hipEvent_t& e : ready)
+      CFDHIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    auto enqueue = [&](int c) {
+      fusedChunk();
+      launched += gChunk;
+      CFDHIP_CHECK(hipMemcpyAsync(hb[c & 1], Sb[0], sizeof(KState), hipMemcpyDeviceToHost,
+                                  ctx.stream));
+      CFDHIP_CHECK(hipEventRecord(ready[c & 1], ctx.stream));
+    };
+    enqueue(0);
+    for (int c = 0;; ++c) {
+      const bool more = launched < set.max_iterations;
+      if (more) enqueue(c + 1);
+      CFDHIP_CHECK(hipGetLastError());
+      CFDHIP_CHECK(hipEventSynchronize(ready[c & 1]));
+      if (hb[c & 1]->done || !more) {
+        // the chunk queued behind it (if any) has seen the final state
+        if (more) CFDHIP_CHECK(hipEventSynchronize(ready[(c + 1) & 1]));
+        const KState fin = *hb[(more ? c + 1 : c) & 1];
+        *host = fin;
+        break;
+      }
+    }
+    for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+  }
+  while (!pipelined) {
     int q = 0;
     if (fused && !graphed) {
       if (timing) CFDHIP_CHECK(hipEventRecord(ev[0], ctx.stream));
@@ -585,7 +619,8 @@ cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& s
   res.true_residual = NAN;
   res.device_ms = ms;
   res.history_length = buf.events ? S.histLen : 0;
-  if (buf.events) {
+  if (ctx.inLoop) ctx.lastHistory.clear();
+  if (buf.events && !ctx.inLoop) {
     ctx.lastHistory.resize(S.histLen);
     if (S.histLen > 0)
       CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), buf.hist, S.histLen * sizeof(double),
@@ -608,9 +643,11 @@ cfdhip_solve_result singleKernelPcg2d(Context& ctx, const cfdhip_solver_settings
   w.partial.ensure(std::max<int64_t>({kMaxBlocks * 4, tiles * 3 + 8, nb + 8, w.partial.count}));
   w.state.ensure(2);
   if (!w.hostState) CFDHIP_CHECK(hipHostMalloc(&w.hostState, sizeof(KState)));
+  if (!w.hostState2) CFDHIP_CHECK(hipHostMalloc(&w.hostState2, sizeof(KState)));
   w.hist.ensure(set.max_iterations + 2);
   const PcgBuffers buf{w.partial.get(), w.partial.count, w.state.get(), w.hostState,
-                       w.hist.get(), w.hist.count, true, w.state.get() + 1};
+                       w.hist.get(),    w.hist.count,    true,          w.state.get() + 1,
+                       w.hostState2};
   return pcg2dOnBuffers(ctx, set, diag, coeff, b, x, work, buf, snap);
 }
 

@@ -50,6 +50,7 @@ struct KrylovWork {
   DeviceArray<double> hist;
   DeviceArray<double> scal;  // scalar results of one-off reductions
   KState* hostState = nullptr;  // pinned mirror
+  KState* hostState2 = nullptr; // a second one: two chunks of a solve in flight
   // device bytes held by the work vectors and reduction buffers above
   int64_t bytes() const;
 };
@@ -111,6 +112,11 @@ struct Context {
   void* hostPcUser = nullptr;
   std::vector<double> hostPcIn, hostPcOut;
   int64_t lastFallback = 0;
+
+  // inside the resident SIMPLE loop: its assembly checks already cover the
+  // inputs of its solves (no input scan), and it reads neither the true
+  // residual nor the history of a solve (neither is computed or copied)
+  bool inLoop = false;
 
   // timing of the pressure SpMV kernel
   bool timing = false;
@@ -213,6 +219,9 @@ struct PcgBuffers {
   // (no finalize launch between iterations); null keeps one finalize per
   // iteration
   KState* state2 = nullptr;
+  // a second pinned host mirror: lets the fused chunks run pipelined (the
+  // next chunk queued before the host reads the last one's state)
+  KState* host2 = nullptr;
 };
 cfdhip_solve_result pcg2dOnBuffers(Context& ctx, const cfdhip_solver_settings& set,
                                    const double* diag, const double* coeff, const double* b,

@@ -779,6 +779,10 @@ struct Solve {
     r.true_residual = trueRes;
     r.device_ms = ms;
     r.history_length = hs->histLen;
+    if (ctx.inLoop) {
+      ctx.lastHistory.clear();
+      return r;
+    }
     ctx.lastHistory.resize(hs->histLen);
     if (hs->histLen > 0)
       CFDHIP_CHECK(hipMemcpy(ctx.lastHistory.data(), ctx.kw.hist.get(),
@@ -789,6 +793,7 @@ struct Solve {
 
 template <class Op>
 bool inputsFinite(Context& ctx, Op&, const std::vector<std::pair<const double*, int64_t>>& arrays) {
+  if (ctx.inLoop) return true;
   for (const auto& a : arrays) {
     if (a.second <= 0 || a.first == nullptr) continue;
     hipLaunchKernelGGL(nonFiniteKernel, gridStride(a.second), dim3(kBlock), 0, ctx.stream,
@@ -838,6 +843,7 @@ struct Timer {
 
 template <class Op>
 double trueResidual(Context& ctx, Op A, int64_t n, const double* b, const double* x) {
+  if (ctx.inLoop) return NAN;
   KrylovWork& w = ctx.kw;
   w.scal.ensure(4);
   w.partial.ensure(kMaxBlocks * 4);

@@ -48,6 +48,12 @@ void simpleRun(Context& ctx, const double* s, const cfdhip_solver_settings& mom,
   if (s[8] != 0.0) presRun.graph = 1;
 
   for (int k = 0; k < 6; ++k) meta[k] = 0.0;
+  // the loop's own checks cover its solves' inputs (see Context::inLoop)
+  struct InLoop {
+    Context& c;
+    explicit InLoop(Context& cc) : c(cc) { c.inLoop = true; }
+    ~InLoop() { c.inLoop = false; }
+  } inLoop(ctx);
   int status = kMaxIterations;
   int64_t it = 0, momIters = 0, presIters = 0;
   double lastCont = 0.0, lastGlobal = 0.0;
