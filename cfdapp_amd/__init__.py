@@ -712,6 +712,48 @@ class FaceMeshPressure:
         self._ctx.call("cfdhip_fmesh_correct_flux", None if p is None else _pf(p), _pf(F))
         return F
 
+    def set_cell_volumes(self, volume):
+        """Cell::volume() of every cell: the gradients need it."""
+        v = _f64(volume)
+        if v.size != self.n_cells:
+            raise ValueError("set_cell_volumes: one volume per cell")
+        self._ctx.call("cfdhip_fmesh_set_cell_volumes", _pf(v))
+
+    GRADIENT_SCHEMES = {"green_gauss": 0, "least_squares": 1}
+
+    def gradient(self, field, bc_kind=None, bc_value=None, scheme="green_gauss"):
+        """cfd::discretization::gradient of a cell field (n_cells, 3): bc_kind per
+        face (1 FixedValue, 0 FixedGradient; boundary faces) with bc_value (the
+        value / the gradient); None: the p' conditions of the explicit term."""
+        phi = _f64(field)
+        g = np.zeros((self.n_cells, 3))
+        kind = None if bc_kind is None else np.ascontiguousarray(bc_kind, dtype=np.int32)
+        val = None if bc_value is None else _f64(bc_value)
+        self._ctx.call("cfdhip_fmesh_gradient", _pf(phi),
+                       None if kind is None else kind.ctypes.data_as(
+                           ctypes.POINTER(ctypes.c_int32)),
+                       None if val is None else _pf(val), self.GRADIENT_SCHEMES[scheme],
+                       _pf(g))
+        return g
+
+    def assemble_pass(self, previous=None, scheme="green_gauss"):
+        """A corrector pass's assembly: the last assemble()'s inputs with the
+        explicit flux of the device gradient of `previous` (None: the last
+        solve's p')."""
+        p = None if previous is None else _f64(previous)
+        self._ctx.call("cfdhip_fmesh_assemble_pressure_pass", None if p is None else _pf(p),
+                       self.GRADIENT_SCHEMES[scheme])
+
+    def passes(self, settings: "LinearSolverSettings", passes, scheme="green_gauss"):
+        """The p' corrector passes (SIMPLE.cpp:876-924) from the first-pass
+        assembly: the last p' and one SolverResult per pass."""
+        x = np.zeros(self.n_cells)
+        rs = (_lib.SolveResult * passes)()
+        s = settings.c()
+        self._ctx.call("cfdhip_fmesh_pressure_passes", ctypes.byref(s), int(passes),
+                       self.GRADIENT_SCHEMES[scheme], _pf(x), rs)
+        return x, [_result(r, None, np.zeros(0)) for r in rs]
+
 
 FACE_FIELDS = _lib.FACE_FIELDS
 

@@ -141,6 +141,15 @@ SIGNATURES = {
     "cfdhip_fmesh_solve_pressure": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverSettings),
                                                    P_F64, ctypes.POINTER(SolveResult)]),
     "cfdhip_fmesh_correct_flux": (ctypes.c_int, [ctypes.c_void_p, P_F64, P_F64]),
+    "cfdhip_fmesh_set_cell_volumes": (ctypes.c_int, [ctypes.c_void_p, P_F64]),
+    "cfdhip_fmesh_gradient": (ctypes.c_int, [ctypes.c_void_p, P_F64,
+                                             ctypes.POINTER(ctypes.c_int32), P_F64, ctypes.c_int,
+                                             P_F64]),
+    "cfdhip_fmesh_assemble_pressure_pass": (ctypes.c_int, [ctypes.c_void_p, P_F64, ctypes.c_int]),
+    "cfdhip_fmesh_pressure_passes": (ctypes.c_int, [ctypes.c_void_p,
+                                                    ctypes.POINTER(SolverSettings), ctypes.c_int,
+                                                    ctypes.c_int, P_F64,
+                                                    ctypes.POINTER(SolveResult)]),
     "cfdhip_csr_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverSettings),
                                         ctypes.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64, P_F64,
                                         ctypes.POINTER(SolveResult)]),

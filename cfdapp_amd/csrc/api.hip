@@ -413,6 +413,93 @@ int cfdhip_fmesh_correct_flux(cfdhip_ctx* ctx, const double* pprime, double* flu
   return guarded(ctx, [&](Context& c) { fmeshCorrectFlux(c, pprime, flux); });
 }
 
+int cfdhip_fmesh_set_cell_volumes(cfdhip_ctx* ctx, const double* cell_volume) {
+  return guarded(ctx, [&](Context& c) { fmeshSetVolumes(c, cell_volume); });
+}
+
+int cfdhip_fmesh_gradient(cfdhip_ctx* ctx, const double* field, const int32_t* bc_kind,
+                          const double* bc_value, int scheme, double* grad) {
+  return guarded(ctx, [&](Context& c) {
+    require(c.fmesh != nullptr, "no face-list mesh: call cfdhip_fmesh_set first");
+    FaceMesh& m = *c.fmesh;
+    require(field != nullptr && grad != nullptr, "gradient: field and output are required");
+    require((bc_kind == nullptr) == (bc_value == nullptr),
+            "gradient: condition kinds and values come together");
+    DeviceArray<double> phi, out;
+    phi.ensure(m.n);
+    out.ensure(3 * m.n);
+    CFDHIP_CHECK(hipMemcpyAsync(phi.get(), field, m.n * sizeof(double), hipMemcpyHostToDevice,
+                                c.stream));
+    if (bc_kind == nullptr) {
+      fmeshPprimeConditions(c, m);
+    } else {
+      for (int64_t f = 0; f < m.nf; ++f)
+        require(m.hNeigh[f] >= 0 || bc_kind[f] == 0 || bc_kind[f] == 1,
+                "gradient: a boundary condition kind is 0 (FixedGradient) or 1 (FixedValue)");
+      m.bcKind.ensure(m.nf);
+      m.bcVal.ensure(m.nf);
+      CFDHIP_CHECK(hipMemcpyAsync(m.bcKind.get(), bc_kind, m.nf * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, c.stream));
+      CFDHIP_CHECK(hipMemcpyAsync(m.bcVal.get(), bc_value, m.nf * sizeof(double),
+                                  hipMemcpyHostToDevice, c.stream));
+    }
+    fmeshGradient(c, phi.get(), m.bcKind.get(), m.bcVal.get(), scheme, out.get());
+    CFDHIP_CHECK(hipMemcpyAsync(grad, out.get(), 3 * m.n * sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_fmesh_assemble_pressure_pass(cfdhip_ctx* ctx, const double* previous_pprime,
+                                        int gradient_scheme) {
+  return guarded(ctx, [&](Context& c) {
+    require(c.fmesh != nullptr, "no face-list mesh: call cfdhip_fmesh_set first");
+    FaceMesh& m = *c.fmesh;
+    DeviceArray<double> prev;
+    const double* p = m.x.get();
+    if (previous_pprime != nullptr) {
+      prev.ensure(m.n);
+      CFDHIP_CHECK(hipMemcpyAsync(prev.get(), previous_pprime, m.n * sizeof(double),
+                                  hipMemcpyHostToDevice, c.stream));
+      p = prev.get();
+    } else {
+      require(m.solved, "face-list p': no previous p' given and none solved");
+    }
+    fmeshAssemblePass(c, p, gradient_scheme);
+    CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int cfdhip_fmesh_pressure_passes(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, int passes,
+                                 int gradient_scheme, double* x, cfdhip_solve_result* results) {
+  return guarded(ctx, [&](Context& c) {
+    require(c.fmesh != nullptr, "no face-list mesh: call cfdhip_fmesh_set first");
+    FaceMesh& m = *c.fmesh;
+    require(passes >= 1, "nonOrthogonalCorrections: at least one pass");
+    require(m.assembled && !m.hasExplicit,
+            "face-list p': the passes start from a first-pass assembly (no explicit term)");
+    const cfdhip_solver_settings set = checkedSettings(s);
+    for (int k = 0; k < passes; ++k) {
+      if (k > 0) fmeshAssemblePass(c, m.x.get(), gradient_scheme);
+      const cfdhip_solve_result r = fmeshSolve(c, set, nullptr);
+      if (results) results[k] = r;
+      // SIMPLE.cpp:904-911: a pass that does not converge ends the loop
+      if (r.status != CFDHIP_CONVERGED) {
+        for (int q = k + 1; results && q < passes; ++q) {
+          results[q] = cfdhip_solve_result{};
+          results[q].status = -1;
+        }
+        break;
+      }
+    }
+    if (x) {
+      CFDHIP_CHECK(hipMemcpyAsync(x, m.x.get(), m.n * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+      CFDHIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+  });
+}
+
 int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r) {
   return guarded(ctx, [&](Context& c) {
     const cfdhip_solver_settings set = checkedSettings(s);

@@ -61,20 +61,51 @@ struct CsrDevice {
   int64_t n = 0;
 };
 
+// Cell gradients on a face-list mesh (fgrad.hip): what depends only on the
+// geometry is computed once on the host, in the reference's expressions; the
+// field arithmetic runs on the device.
+struct FaceGradPlan {
+  bool built = false;
+  bool transfer = false;  // some boundary fit line meets the boundary off the face centroid
+  int64_t nSkew = 0, nClaims = 0;
+  DeviceArray<double> wts;          // 2 per face: owner / neighbour centroid to face distance
+  DeviceArray<int64_t> skewFace;    // internal faces with a non-zero skew vector
+  DeviceArray<double> skewGeo;      // 4 per skewed face: t, skew vector
+  DeviceArray<int32_t> slotClaim;   // per cell-face slot: the claim on that face, or -1
+  DeviceArray<int64_t> claimFaces;  // 3 per claim: boundary face, opposite face, far cell
+  DeviceArray<double> claimGeo;     // 6 per claim: far distance, w, back distance, offset
+  // oblique Neumann-type boundary faces, for the conditions last used
+  std::vector<int32_t> oblKinds;
+  int64_t nObl = 0;
+  DeviceArray<int64_t> oblFaces;
+  DeviceArray<double> oblGeo;       // 8 per face: oblique (0/1), dn, unit normal, tangential offset
+};
+
 // A general face-list mesh (fmesh.hip) and its p' system.
 struct FaceMesh {
   int dim = 2;
   int64_t n = 0, nf = 0;
   bool open = false;  // a FixedValue pressure face exists: no pinned cell
-  std::vector<int64_t> hOwner, hNeigh, hRp, hCol;
+  std::vector<int64_t> hOwner, hNeigh, hRp, hCol, hCfOff, hCf;
   std::vector<int32_t> hFixed;
+  std::vector<double> hFc, hSf, hCc;
   DeviceArray<int64_t> owner, neigh, cfOff, cf, cfSorted, slotOff, slotCode;
-  DeviceArray<double> fc, sf, cc;
+  DeviceArray<double> fc, sf, cc, vol;
   DeviceArray<int> fixed;
   DeviceArray<double> flux, du, dv, dw, rhoF, grad, extra, coeff, expl, rhs, x;
   CsrDevice A;
   int64_t planPin = -2;
   bool planExtra = false, assembled = false, hasExplicit = false, solved = false;
+  bool hasVolumes = false;
+  // the last assembly's inputs beyond the uploaded arrays (corrector passes)
+  double rho = 1.0;
+  int nonOrth = 1;
+  bool hasRhoF = false, hasDw = false, hasExtra = false;
+  int64_t ref = 0;
+  // gradients
+  FaceGradPlan gp;
+  DeviceArray<double> fv, gradA, gradB, field, bcVal, gg;
+  DeviceArray<int32_t> bcKind;
 };
 
 // inputs of the compressible p' assembly, kept on the device
@@ -181,6 +212,20 @@ int64_t fmeshSystem(Context& ctx, int64_t* rowptr, int64_t* col, double* val, in
                     double* rhs, double* faceCoeff, double* explicitFlux);
 cfdhip_solve_result fmeshSolve(Context& ctx, const cfdhip_solver_settings& s, double* x);
 void fmeshCorrectFlux(Context& ctx, const double* pprime, double* flux);
+// the assembly again, on the last call's inputs, with the explicit term of the
+// device gradient `gradDev` (3 per cell) or none
+void fmeshReassemble(Context& ctx, const double* gradDev);
+// fgrad.hip: cell gradients on the face-list mesh (device arrays): field[n],
+// per-face condition kind (1 FixedValue, 0 FixedGradient) and value, scheme
+// 0 Green-Gauss / 1 least squares; out[3 n]
+void fmeshSetVolumes(Context& ctx, const double* cellVolume);
+void fmeshGradient(Context& ctx, const double* field, const int32_t* kind, const double* value,
+                   int scheme, double* out);
+// the p' conditions of the explicit term into FaceMesh::bcKind / bcVal
+void fmeshPprimeConditions(Context& ctx, FaceMesh& m);
+// a corrector pass's assembly: the explicit term from the gradient (scheme)
+// of the previous p' (device, n values)
+void fmeshAssemblePass(Context& ctx, const double* previousDev, int scheme);
 
 // krylov.hip
 enum class OpKind { Pressure, Stencil, Csr };

@@ -316,6 +316,10 @@ void fmeshSet(Context& ctx, int dim, int64_t n, int64_t nf, const int64_t* owner
   m->hOwner.assign(owner, owner + nf);
   m->hNeigh.assign(neighbor, neighbor + nf);
   m->hFixed.assign(fixedValue, fixedValue + nf);
+  m->hFc.assign(faceCentroid, faceCentroid + 3 * nf);
+  m->hSf.assign(areaVector, areaVector + 3 * nf);
+  m->hCc.assign(cellCentroid, cellCentroid + 3 * n);
+  m->hCfOff.assign(cellFaceOffsets, cellFaceOffsets + n + 1);
   m->open = false;
   for (int64_t f = 0; f < nf; ++f) {
     require(owner[f] >= 0 && owner[f] < n && neighbor[f] >= -1 && neighbor[f] < n &&
@@ -325,6 +329,7 @@ void fmeshSet(Context& ctx, int dim, int64_t n, int64_t nf, const int64_t* owner
             "face-list mesh: only boundary faces carry a FixedValue pressure");
     m->open = m->open || fixedValue[f] != 0;
   }
+  m->hCf.assign(cellFaces, cellFaces + ncf);
   std::vector<int64_t> sorted(cellFaces, cellFaces + ncf);
   for (int64_t c = 0; c < n; ++c) {
     for (int64_t k = cellFaceOffsets[c]; k < cellFaceOffsets[c + 1]; ++k) {
@@ -347,6 +352,45 @@ void fmeshSet(Context& ctx, int dim, int64_t n, int64_t nf, const int64_t* owner
   ctx.fmesh = std::move(m);
 }
 
+namespace {
+
+// The assembly on the inputs held in m; gradDev: the gradient of the previous
+// p' (3 per cell, device) for the explicit term, or null.
+void assembleOnDevice(Context& ctx, FaceMesh& m, const double* gradDev) {
+  const int64_t pin = m.open ? -1 : m.ref;
+  m.assembled = false;
+  m.coeff.ensure(m.nf);
+  m.expl.ensure(m.nf);
+  m.rhs.ensure(m.n);
+  buildPlan(ctx, m, pin, m.hasExtra);
+  FaceArgs a{m.nf, m.owner.get(), m.neigh.get(), m.fc.get(), m.sf.get(), m.cc.get(), m.fixed.get(),
+             m.hasRhoF ? m.rhoF.get() : nullptr, m.rho, m.du.get(), m.dv.get(),
+             m.hasDw ? m.dw.get() : nullptr, m.nonOrth, gradDev, m.coeff.get(), m.expl.get(),
+             ctx.flags.get()};
+  hipLaunchKernelGGL(faceCouplingKernel, fmBlocks(m.nf), dim3(kFmBlock), 0, ctx.stream, a);
+  CFDHIP_CHECK(hipGetLastError());
+  require(takeNonFinite(ctx) == 0,
+          "pressureCorrectionFaceCoupling: non-finite face coupling (degenerate face geometry)",
+          CFDHIP_ERR_NONFINITE);
+  const int64_t nslots = static_cast<int64_t>(m.hCol.size());
+  hipLaunchKernelGGL(slotKernel, fmBlocks(nslots), dim3(kFmBlock), 0, ctx.stream, nslots,
+                     m.slotOff.get(), m.slotCode.get(), m.coeff.get(),
+                     m.hasExtra ? m.extra.get() : nullptr, m.A.val.get());
+  hipLaunchKernelGGL(fmRhsKernel, fmBlocks(m.n), dim3(kFmBlock), 0, ctx.stream, m.n, m.cfOff.get(),
+                     m.cf.get(), m.cfSorted.get(), m.owner.get(), m.flux.get(), m.expl.get(),
+                     gradDev ? 1 : 0, pin, m.rhs.get());
+  CFDHIP_CHECK(hipGetLastError());
+  const double* arrays[2] = {m.A.val.get(), m.rhs.get()};
+  const int64_t counts[2] = {nslots, m.n};
+  require(countNonFinite(ctx, arrays, counts, 2) == 0,
+          "assembleGeometricPressureCorrection: assembled system contains a non-finite value",
+          CFDHIP_ERR_NONFINITE);
+  m.hasExplicit = gradDev != nullptr;
+  m.assembled = true;
+}
+
+}  // namespace
+
 void fmeshAssemblePressure(Context& ctx, const double* fluxStar, const double* faceDensity,
                            double rho, const double* du, const double* dv, const double* dw,
                            int nonOrthogonal, const double* gradPrev, const double* extraDiag,
@@ -357,7 +401,6 @@ void fmeshAssemblePressure(Context& ctx, const double* fluxStar, const double* f
   require(m.dim == 2 || dw != nullptr, "face-list p': a 3D mesh needs the w response coefficient",
           CFDHIP_ERR_INVALID);
   require(ref >= 0 && ref < m.n, "assembleGeometricPressureCorrection: referenceCell out of range");
-  const int64_t pin = m.open ? -1 : ref;
   m.assembled = false;
   upload(ctx, m.flux, fluxStar, m.nf);
   upload(ctx, m.du, du, m.n);
@@ -366,34 +409,20 @@ void fmeshAssemblePressure(Context& ctx, const double* fluxStar, const double* f
   if (faceDensity) upload(ctx, m.rhoF, faceDensity, m.nf);
   if (gradPrev) upload(ctx, m.grad, gradPrev, 3 * m.n);
   if (extraDiag) upload(ctx, m.extra, extraDiag, m.n);
-  m.coeff.ensure(m.nf);
-  m.expl.ensure(m.nf);
-  m.rhs.ensure(m.n);
-  buildPlan(ctx, m, pin, extraDiag != nullptr);
-  FaceArgs a{m.nf, m.owner.get(), m.neigh.get(), m.fc.get(), m.sf.get(), m.cc.get(), m.fixed.get(),
-             faceDensity ? m.rhoF.get() : nullptr, rho, m.du.get(), m.dv.get(),
-             dw ? m.dw.get() : nullptr, nonOrthogonal, gradPrev ? m.grad.get() : nullptr,
-             m.coeff.get(), m.expl.get(), ctx.flags.get()};
-  hipLaunchKernelGGL(faceCouplingKernel, fmBlocks(m.nf), dim3(kFmBlock), 0, ctx.stream, a);
-  CFDHIP_CHECK(hipGetLastError());
-  require(takeNonFinite(ctx) == 0,
-          "pressureCorrectionFaceCoupling: non-finite face coupling (degenerate face geometry)",
-          CFDHIP_ERR_NONFINITE);
-  const int64_t nslots = static_cast<int64_t>(m.hCol.size());
-  hipLaunchKernelGGL(slotKernel, fmBlocks(nslots), dim3(kFmBlock), 0, ctx.stream, nslots,
-                     m.slotOff.get(), m.slotCode.get(), m.coeff.get(),
-                     extraDiag ? m.extra.get() : nullptr, m.A.val.get());
-  hipLaunchKernelGGL(fmRhsKernel, fmBlocks(m.n), dim3(kFmBlock), 0, ctx.stream, m.n, m.cfOff.get(),
-                     m.cf.get(), m.cfSorted.get(), m.owner.get(), m.flux.get(), m.expl.get(),
-                     gradPrev ? 1 : 0, pin, m.rhs.get());
-  CFDHIP_CHECK(hipGetLastError());
-  const double* arrays[2] = {m.A.val.get(), m.rhs.get()};
-  const int64_t counts[2] = {nslots, m.n};
-  require(countNonFinite(ctx, arrays, counts, 2) == 0,
-          "assembleGeometricPressureCorrection: assembled system contains a non-finite value",
-          CFDHIP_ERR_NONFINITE);
-  m.hasExplicit = gradPrev != nullptr;
-  m.assembled = true;
+  m.rho = rho;
+  m.nonOrth = nonOrthogonal;
+  m.hasRhoF = faceDensity != nullptr;
+  m.hasDw = dw != nullptr;
+  m.hasExtra = extraDiag != nullptr;
+  m.ref = ref;
+  assembleOnDevice(ctx, m, gradPrev ? m.grad.get() : nullptr);
+}
+
+void fmeshReassemble(Context& ctx, const double* gradDev) {
+  FaceMesh& m = need(ctx);
+  require(m.flux.get() != nullptr && m.du.get() != nullptr,
+          "face-list p': no assembly to repeat (call cfdhip_fmesh_assemble_pressure first)");
+  assembleOnDevice(ctx, m, gradDev);
 }
 
 int64_t fmeshSystem(Context& ctx, int64_t* rowptr, int64_t* col, double* val, int64_t cap,

@@ -282,6 +282,39 @@ int cfdhip_fmesh_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s
  * = F* + D_f (p'_P - p'_N) (+ the explicit flux when the assembly had one),
  * from pprime[n] (host) or, when NULL, the last solve's p'. */
 int cfdhip_fmesh_correct_flux(cfdhip_ctx* ctx, const double* pprime, double* flux);
+/* Cell volumes (Cell::volume(), n_cells values) of the face-list mesh: the
+ * gradients below need them. */
+int cfdhip_fmesh_set_cell_volumes(cfdhip_ctx* ctx, const double* cell_volume);
+/* cfd::discretization::gradient (src/discretization/Gradient.cpp:598-607) of
+ * a cell field on the face-list mesh (2D): scheme 0 Green-Gauss with the
+ * boundary-consistent face values, skewness correction and oblique Neumann
+ * faces (greenGaussGradient, :534-596), 1 weighted least squares with the
+ * Green-Gauss fallback (leastSquaresGradient, :361-447). field[n] and
+ * grad[3 n] (x, y, z per cell) are host arrays; bc_kind / bc_value give each
+ * boundary face's scalar condition (1 FixedValue: the value, 0 FixedGradient:
+ * the gradient; internal faces ignored); both NULL: the p' conditions of the
+ * explicit term (FixedValue 0 on the mesh's FixedValue faces, FixedGradient 0
+ * elsewhere, makeGradientBoundaries, PressureCorrectionEquation.cpp:43-54).
+ * CFDHIP_ERR_UNSUPPORTED on a 3D mesh. */
+int cfdhip_fmesh_gradient(cfdhip_ctx* ctx, const double* field, const int32_t* bc_kind,
+                          const double* bc_value, int scheme, double* grad);
+/* A corrector pass's assembly (SIMPLE.cpp:880-891): the last
+ * cfdhip_fmesh_assemble_pressure's inputs again, with the explicit
+ * non-orthogonal flux from the gradient (gradient_scheme as above, the p'
+ * conditions) of previous_pprime (host, n values; NULL: the last solve's p'),
+ * computed on the device. Needs non_orthogonal = 1. */
+int cfdhip_fmesh_assemble_pressure_pass(cfdhip_ctx* ctx, const double* previous_pprime,
+                                        int gradient_scheme);
+/* The p' corrector passes of SIMPLE.cpp:876-924 (nonOrthogonalCorrections =
+ * passes) on the device: the first-pass system (the last
+ * cfdhip_fmesh_assemble_pressure, without an explicit term) solved from
+ * x0 = 0, then passes 2.. each reassembled with the explicit flux of the
+ * previous pass's p' and solved from x0 = 0. results[passes] (may be NULL)
+ * receive each solve; a pass that does not converge ends the loop (later
+ * results: status -1). x[n] (host, may be NULL) receives the last p';
+ * cfdhip_fmesh_correct_flux(NULL) then applies the last pass's explicit flux. */
+int cfdhip_fmesh_pressure_passes(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, int passes,
+                                 int gradient_scheme, double* x, cfdhip_solve_result* results);
 /* solvePressureCorrectionResident: p' from x0 = 0 into PPRIME. */
 int cfdhip_solve_pressure(cfdhip_ctx* ctx, const cfdhip_solver_settings* s, cfdhip_solve_result* r);
 /* updatePressure: P += alpha * PPRIME (SIMPLE.cpp:939-942). */

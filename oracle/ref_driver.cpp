@@ -42,6 +42,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <optional>
 #include <string>
 #include <vector>
 
@@ -371,6 +372,55 @@ void cmdFpcorr() {
       m, flux, asmb.faceCoefficient, pp, opts.previousPressureCorrection ? &asmb.explicitFaceFlux
                                                                          : nullptr);
   dumpF("flux_corrected", toVec(corrected));
+  // the gradient's input and the other gradient forms of it: least squares
+  // under the same (p') conditions, and both schemes under conditions with
+  // non-zero values (FixedValue 0.3 on "right", FixedGradient -0.7 elsewhere)
+  dumpF("prev", toVec(prev));
+  auto dumpGrad = [&](const std::string& name, const VectorField& g) {
+    std::vector<double> x(n), y(n);
+    for (size_t k = 0; k < n; ++k) {
+      x[k] = g[k].x;
+      y[k] = g[k].y;
+    }
+    dumpF(name + "_x", x);
+    dumpF(name + "_y", y);
+  };
+  using discretization::GradientScheme;
+  dumpGrad("grad_ls", discretization::gradient(m, prev, gbc, GradientScheme::LeastSquares));
+  boundary::BoundaryConditionSet vbc;
+  for (const auto& patch : m.boundaryPatches()) {
+    if (patch.name() == "right")
+      vbc.set(m, patch.name(), std::make_unique<boundary::FixedValue>(0.3));
+    else
+      vbc.set(m, patch.name(), std::make_unique<boundary::FixedGradient>(-0.7));
+  }
+  dumpGrad("grad_v", discretization::gradient(m, prev, vbc, GradientScheme::GreenGauss));
+  dumpGrad("grad_vls", discretization::gradient(m, prev, vbc, GradientScheme::LeastSquares));
+  // the corrector passes of SIMPLE.cpp:876-924 on this system: pass 1 without
+  // the explicit term, passes 2.. with grad(p' of the pass before) (scheme
+  // `gscheme`), each solved by CG + Jacobi from zero; per pass its RHS,
+  // explicit flux and solution, then the flux corrected with the last one
+  const int passes = iarg("passes", "3");
+  const GradientScheme gs =
+      iarg("gscheme", "0") != 0 ? GradientScheme::LeastSquares : GradientScheme::GreenGauss;
+  pressure_velocity::PressureCorrectionOptions popts;
+  popts.nonOrthogonal = true;
+  popts.gradientScheme = gs;
+  ScalarField pPrev(n, 0.0);
+  std::optional<pressure_velocity::PressureCorrectionAssembly> last;
+  for (int k = 1; k <= passes; ++k) {
+    popts.previousPressureCorrection = k > 1 ? &pPrev : nullptr;
+    last = pressure_velocity::assemblePressureCorrection(m, flux, du, dv, rho, iarg("ref", "0"),
+                                                         pbc, popts);
+    const auto pr = cg.solve(last->system);
+    dumpF("pass" + std::to_string(k) + "_rhs", toVec(last->system.rhs()));
+    dumpF("pass" + std::to_string(k) + "_explicit", toVec(last->explicitFaceFlux));
+    dumpResult("pass" + std::to_string(k), pr);
+    for (size_t q = 0; q < n; ++q) pPrev[q] = pr.solution[q];
+  }
+  dumpF("passes_flux", toVec(pressure_velocity::correctFaceMassFlux(
+                           m, flux, last->faceCoefficient, pPrev,
+                           passes > 1 ? &last->explicitFaceFlux : nullptr)));
 }
 
 // p' operator with d == 1 (rho = 1, cell 0 pinned) and a seeded, mean-free RHS,

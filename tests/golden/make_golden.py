@@ -59,6 +59,9 @@ FIXTURES = {
     "fpcorr2d_16x8_outlet": ("fpcorr", dict(nx=16, ny=8, seed=13, outlet=1, amp=0.25, lx=2.0)),
     "fpcorr2d_multiblock_outlet": ("fpcorr", dict(nx=7, ny=6, seed=17, outlet=1, amp=0.2,
                                                   mesh="multiblock")),
+    # corrector passes with the least-squares gradient of the previous p'
+    "fpcorr2d_14x9_lsq": ("fpcorr", dict(nx=14, ny=9, seed=19, outlet=1, amp=0.22, passes=4,
+                                         gscheme=1)),
     # tight linear solves: the step's fields then differ only by reduction order
     "piso2d_32_tight": ("piso", dict(dim=2, nx=32, ny=32, re=1000, dt=0.005, warm=3, steps=2,
                                      psolver="bicgstab", mabs=0, mrel=1e-13, pabs=0, prel=1e-13)),
