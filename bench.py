@@ -47,7 +47,7 @@ SEED = 20480001
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one cgIterKernel<double> launch at 2048^2 from rocprofv3 counters
 # (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 512 MiB copy kernel; see profiles/)
-TRAFFIC_BYTES = 451.0e6
+TRAFFIC_BYTES = 458.0e6
 
 
 def layer_bytes(p0, ch):
