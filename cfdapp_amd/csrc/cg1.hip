@@ -97,10 +97,12 @@ struct FuseArgs {
 };
 constexpr int64_t kFusedMaxTiles = 256;
 
-// A large grid streams every vector through once per iteration: its stores
-// bypass the caches (nontemporal), which leaves the L2 to the tiles' shared
-// halo rows (2048^2 FP64 iteration 100 -> 89 us). The fused small-grid kernel
-// stores normally: its next launch reads the vectors back from the L2.
+// A large grid streams every vector through once per iteration: its FP64
+// stores bypass the caches (nontemporal), which leaves the L2 to the tiles'
+// shared halo rows (2048^2 FP64 iteration 100 -> 89 us). The FP32 iteration
+// of the mixed solve stores normally (nontemporal stores slowed it from 40 to
+// 61 us at 2048^2), and so does the fused small-grid kernel, whose next launch
+// reads the vectors back from the L2.
 template <bool NT, class V>
 __device__ inline void storeV(V* p, V v) {
   if constexpr (NT)
@@ -208,6 +210,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
     st = &ks;
   }
   const bool skip = st->done;
+  constexpr bool kNT = !FUSED && std::is_same_v<T, double>;
   const T alpha = static_cast<T>(st->alpha);
   const T beta = static_cast<T>(st->beta);
   const TX alphaX = static_cast<TX>(st->alpha);
@@ -238,10 +241,10 @@ __global__ __launch_bounds__(256) void cgIterKernel(
           if (!start) {
             const T uOld = (T(1) / d) * r0;
             const T p = first ? uOld : uOld + (pK[k] * beta);
-            storeV<!FUSED>(&pNew[c], p);
-            storeV<!FUSED>(&sNew[c], sn);
-            storeV<!FUSED>(&rNew[c], rn);
-            storeV<!FUSED>(&x[c], static_cast<TX>(xK[k] + (static_cast<TX>(p) * alphaX)));
+            storeV<kNT>(&pNew[c], p);
+            storeV<kNT>(&sNew[c], sn);
+            storeV<kNT>(&rNew[c], rn);
+            storeV<kNT>(&x[c], static_cast<TX>(xK[k] + (static_cast<TX>(p) * alphaX)));
             if constexpr (NSUM == 6) {
               acc[3] += static_cast<double>(u) * static_cast<double>(sn);
               acc[4] += static_cast<double>(p) * static_cast<double>(sn);
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(256) void cgIterKernel(
         const int64_t c = j * g.nx + i;
         const T w = applyRow<T>(g, i, j, c, dK[k], uTile[tr - 1][lane], uW, uK[k], uE,
                                 uTile[tr + 1][lane], fySK[k], fxK[k], fxE, fyNK[k]);
-        storeV<!FUSED>(&wNew[c], w);
+        storeV<kNT>(&wNew[c], w);
         acc[1] += static_cast<double>(w) * static_cast<double>(uK[k]);
       }
     }
