@@ -3,8 +3,8 @@
 //
 // Same Chronopoulos-Gear recurrence and device-side scalar steps as the 2D
 // single-kernel iteration (cg1.hip; src/algebra/CG.cpp:77-142 in exact
-// arithmetic), split in two kernels per iteration because the 3D halo of
-// u = M r is not worth an LDS tile here:
+// arithmetic). FP64 runs one kernel per iteration on z-marching tiles
+// (cg3IterKernel below); the FP32 iteration of the mixed solve keeps two:
 //   update: s = w + beta s, r -= alpha s, p = M r_old + beta p, x += alpha p,
 //           u = M r                                  (pointwise, in place)
 //   apply:  w = A u; (r, u), (w, u), |r|^2           (7-point stencil)
@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "reduce.hpp"
@@ -190,6 +192,135 @@ __global__ __launch_bounds__(256) void replace3Kernel(Op3<float> g, const float*
   blockPartials<6>(acc, partial);
 }
 
+// One FP64 Jacobi-PCG iteration in a single kernel (the Chronopoulos-Gear
+// step of cg1.hip's 2D iteration) on z-marching tiles: a block owns 62 x
+// (ROWS - 2) interior cells of an x-y tile (one lane per column, +1 halo on
+// every side) over KZ planes, and walks its planes upwards. The walk is
+// sequential (two barriers per plane), so short walks over many blocks win. For each plane it
+// forms the new residual and u = M r of every tile point (the halo ring and
+// the planes just below and above its range too, from the previous iterate)
+// into a three-plane LDS ring, writes p, s, r, x of its own cells, and applies
+// the 7-point stencil to the plane below from the ring. Per iteration a cell
+// is read once (d, r, w, s, p, x, its three lower faces) and written once
+// (p, s, r, x, w): 14 values against the 19 of the two-kernel form. r, s, w
+// alternate between two buffers (a halo reads the previous iterate); p and x
+// are only read by their own block and stay in place. start == 1: the start
+// step w0 = A M r0 (rOld = r0, only wNew written).
+struct Grid3 {
+  int64_t nx, ny, nz, pin, offY, offZ;
+};
+constexpr int kOut3 = 62;
+
+template <int ROWS, int KZ>
+__global__ __launch_bounds__(256) void cg3IterKernel(
+    Grid3 g, const double* __restrict__ diag, const double* __restrict__ coeff,
+    double* __restrict__ p, const double* __restrict__ sOld, double* __restrict__ sNew,
+    const double* __restrict__ rOld, double* __restrict__ rNew, const double* __restrict__ wOld,
+    double* __restrict__ wNew, double* __restrict__ x, int64_t tilesX, int64_t tilesY,
+    const KState* st, double* partial, int start) {
+  constexpr int kRW = ROWS / 4;  // rows per wave
+  constexpr int kIR = ROWS - 2;  // interior rows
+  __shared__ double U[3][ROWS][64];
+  const bool skip = st->done;
+  const double alpha = st->alpha, beta = st->beta;
+  const bool first = st->iter == st->dirStart;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t tile = xcdTileOrder(blockIdx.x, gridDim.x);
+  const int64_t perPlane = tilesX * tilesY;
+  const int64_t tz = tile / perPlane, txy = tile % perPlane;
+  const int64_t tx = txy % tilesX, ty = txy / tilesX;
+  const int64_t i = tx * kOut3 - 1 + lane;
+  const int64_t jBase = ty * kIR - 1;
+  const int64_t kz0 = tz * KZ;
+  const bool col = i >= 0 && i < g.nx;
+  const bool outCol = col && lane >= 1 && lane <= kOut3;
+  const int64_t sxy = g.nx * g.ny;
+  double acc[3] = {0.0, 0.0, 0.0};
+  auto slot = [](int kk) { return (kk + 3) % 3; };
+  for (int kk = -1; kk <= KZ && !skip; ++kk) {
+    const int64_t k = kz0 + kk;
+    const bool planeValid = k >= 0 && k < g.nz;
+    const bool planeOwn = kk >= 0 && kk < KZ && planeValid;
+    const int sl = slot(kk);
+#pragma unroll
+    for (int q = 0; q < kRW; ++q) {
+      const int tr = wave * kRW + q;
+      const int64_t j = jBase + tr;
+      const bool valid = planeValid && col && j >= 0 && j < g.ny;
+      const bool center = valid && planeOwn && outCol && tr >= 1 && tr <= kIR;
+      double u = 0.0;
+      if (valid) {
+        const int64_t c = k * sxy + j * g.nx + i;
+        const double d = diag[c];
+        const double r0 = rOld[c];
+        double sn = 0.0, rn = r0;
+        if (!start) {
+          const double w0 = wOld[c];
+          sn = first ? w0 : w0 + (sOld[c] * beta);
+          rn = r0 - (sn * alpha);
+        }
+        u = (1.0 / d) * rn;
+        if (center) {
+          if (!start) {
+            const double uOld = (1.0 / d) * r0;
+            const double pn = first ? uOld : uOld + (p[c] * beta);
+            p[c] = pn;
+            sNew[c] = sn;
+            rNew[c] = rn;
+            x[c] = x[c] + (pn * alpha);
+          }
+          acc[0] += rn * u;
+          acc[2] += rn * rn;
+        }
+      }
+      U[sl][tr][lane] = u;
+    }
+    __syncthreads();
+    // the stencil on the plane below (its neighbours above are now in the ring)
+    const int kb = kk - 1;
+    const int64_t k1 = kz0 + kb;
+    if (kb >= 0 && k1 < g.nz) {
+      const int sm = slot(kb), sd = slot(kb - 1), su = slot(kk);
+#pragma unroll
+      for (int q = 0; q < kRW; ++q) {
+        const int tr = wave * kRW + q;
+        const int64_t j = jBase + tr;
+        const bool center = outCol && j >= 0 && j < g.ny && tr >= 1 && tr <= kIR;
+        const bool inGrid = col && j >= 0 && j < g.ny;
+        const double uC = U[sm][tr][lane];
+        const double uW = __shfl_up(uC, 1, 64), uE = __shfl_down(uC, 1, 64);
+        const int64_t c = k1 * sxy + j * g.nx + i;
+        // every tile point loads its west face, so the east face of the last
+        // interior column comes from the halo lane
+        const double fxW = inGrid ? coeff[(k1 * g.ny + j) * (g.nx + 1) + i] : 0.0;
+        const double fxE = __shfl_down(fxW, 1, 64);
+        if (center) {
+          const double d = diag[c];
+          double y;
+          if (c == g.pin) {
+            y = 0.0 + d * uC;
+          } else {
+            y = 0.0;
+            if (k1 > 0) y += (-coeff[g.offZ + (k1 * g.ny + j) * g.nx + i]) * U[sd][tr][lane];
+            if (j > 0) y += (-coeff[g.offY + (k1 * (g.ny + 1) + j) * g.nx + i]) * U[sm][tr - 1][lane];
+            if (i > 0) y += (-fxW) * uW;
+            y += d * uC;
+            if (i < g.nx - 1) y += (-fxE) * uE;
+            if (j < g.ny - 1)
+              y += (-coeff[g.offY + (k1 * (g.ny + 1) + j + 1) * g.nx + i]) * U[sm][tr + 1][lane];
+            if (k1 < g.nz - 1)
+              y += (-coeff[g.offZ + ((k1 + 1) * g.ny + j) * g.nx + i]) * U[su][tr][lane];
+          }
+          wNew[c] = y;
+          acc[1] += y * uC;
+        }
+      }
+    }
+    __syncthreads();  // the next plane overwrites the ring slot just read
+  }
+  blockPartials<3>(acc, partial);
+}
+
 }  // namespace
 
 // work: r s w p u (n each)
@@ -214,36 +345,53 @@ cfdhip_solve_result pcg3d(Context& ctx, const cfdhip_solver_settings& set, const
   CFDHIP_CHECK(hipMemcpyAsync(w.state.get(), w.hostState, sizeof(KState), hipMemcpyHostToDevice,
                               ctx.stream));
   const Op3<T> op{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ, diag, coeff};
-  T* r = work[0];
-  T* s = work[1];
-  T* wv = work[2];
+  // the single-kernel iteration's buffers: r, s, w twice (work 0-2 and two
+  // more of the context's), p in place (work 3)
+  for (int q = 5; q < 8; ++q) w.d8[q].ensure(n);
+  static_assert(std::is_same_v<T, double>, "the single-kernel 3D iteration runs in FP64");
+  T* Rv[2] = {work[0], w.d8[5].get()};
+  T* Sv[2] = {work[1], w.d8[6].get()};
+  T* Wv[2] = {work[2], w.d8[7].get()};
   T* p = work[3];
-  T* u = work[4];
-  auto fin = [&](int step) {
-    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(g.x), w.state.get(),
+  const Grid3 g3{v.nx, v.ny, v.nz, ctx.pin, v.offY, v.offZ};
+  // tile shape measured at 256^3 (ms per iteration): 16 x 4 planes 0.538, 12 x 6 0.540,
+  // 12 x 8 0.550, 8 x 8 0.579, 16 x 32 0.746, 32 x 8 1.65; the two-kernel form 0.633
+  constexpr int kRows3 = 16, kPlanes3 = 4;
+  const int64_t tilesX = (v.nx + kOut3 - 1) / kOut3;
+  const int64_t tilesY = (v.ny + (kRows3 - 2) - 1) / (kRows3 - 2);
+  const int64_t zChunks = (v.nz + kPlanes3 - 1) / kPlanes3;
+  const dim3 gt(static_cast<unsigned>(tilesX * tilesY * zChunks));
+  w.partial.ensure(std::max<int64_t>(w.partial.count, static_cast<int64_t>(gt.x) * 3 + 8));
+  auto fin = [&](int step, dim3 grid) {
+    launchFinalize(ctx, FinalArgs{w.partial.get(), static_cast<int64_t>(grid.x), w.state.get(),
                                   w.hist.get(), nullptr, step});
+  };
+  auto iterate = [&](int cur, int start) {
+    hipLaunchKernelGGL((cg3IterKernel<kRows3, kPlanes3>), gt, dim3(256), 0, ctx.stream, g3, diag,
+                       coeff, p, Sv[cur], Sv[1 - cur], Rv[cur], Rv[1 - cur], Wv[cur], Wv[1 - cur], x,
+                       tilesX, tilesY, w.state.get(), w.partial.get(), start);
   };
   hipEvent_t t0, t1;
   CFDHIP_CHECK(hipEventCreate(&t0));
   CFDHIP_CHECK(hipEventCreate(&t1));
   CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
-  hipLaunchKernelGGL((start3Kernel<T>), g, dim3(256), 0, ctx.stream, op, b, x, r, w.partial.get());
-  fin(kStepStartNorm);
-  hipLaunchKernelGGL((update3Kernel<T>), g, dim3(256), 0, ctx.stream, n, diag, p, s, r, wv, x, u,
-                     w.state.get(), 1);
-  hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv, p, s,
-                     w.state.get(), w.partial.get());
-  fin(kStepCg1Start);
+  hipLaunchKernelGGL((start3Kernel<T>), g, dim3(256), 0, ctx.stream, op, b, x, Rv[0],
+                     w.partial.get());
+  fin(kStepStartNorm, g);
+  // start step: w0 = A M r0 from R[0] into W[1]; r0 then moves to R[1], so
+  // the first iteration reads index 1 (R, W; S is not read on a first step)
+  iterate(0, 1);
+  fin(kStepCg1Start, gt);
+  std::swap(Rv[0], Rv[1]);
   CFDHIP_CHECK(hipGetLastError());
+  int cur = 1;
   const int chunk = set.check_interval > 0 ? set.check_interval : 32;
   int64_t launched = 0;
   while (true) {
     for (int q = 0; q < chunk && launched < set.max_iterations; ++q, ++launched) {
-      hipLaunchKernelGGL((update3Kernel<T>), g, dim3(256), 0, ctx.stream, n, diag, p, s, r, wv, x,
-                         u, w.state.get(), 0);
-      hipLaunchKernelGGL((apply3Kernel<T>), g, dim3(256), 0, ctx.stream, op, u, r, wv, p, s,
-                         w.state.get(), w.partial.get());
-      fin(kStepCg1Iter);
+      iterate(cur, 0);
+      fin(kStepCg1Iter, gt);
+      cur = 1 - cur;
     }
     CFDHIP_CHECK(hipGetLastError());
     CFDHIP_CHECK(hipMemcpyAsync(w.hostState, w.state.get(), sizeof(KState), hipMemcpyDeviceToHost,
