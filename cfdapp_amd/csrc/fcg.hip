@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <functional>
 #include <vector>
 
 #include "reduce.hpp"
@@ -233,6 +234,8 @@ struct InnerResult {
   int64_t iterations = 0;
   int status = CFDHIP_MAX_ITERATIONS;
   bool stagnated = false;
+  int64_t replacements = 0;  // reliable updates made
+  bool replacedOut = false;  // ended by a reliable update (the caller's FP64 test decides)
 };
 
 // The preconditioner: z = M r on the solver grid (ny x nx), network grid h x w.
@@ -246,9 +249,18 @@ void applyM(Context& ctx, const T* r, const T* q, T* z, double* partial, int par
                      ctx.pin, h, w, partial, parts);
 }
 
+// FCG(1) on r (T vectors) into the correction d, to relTol of |r| at the
+// start. With `replace` (FP32 vectors), a reliable update whenever the
+// recurrence residual has fallen by replaceRatio since the last one: replace()
+// moves d into the FP64 solution, recomputes r = b - A x in FP64 and returns
+// its norm, and the iteration continues with the same search direction (a
+// flexible method takes the perturbed residual as it comes), instead of
+// restarting the Krylov space at every FP64 correction.
 template <class T>
 InnerResult innerFcg(Context& ctx, const OpT<T>& op, double relTol, int64_t maxIter,
-                     int64_t window, T* r, T* d, T* const work[4], std::vector<double>& history) {
+                     int64_t window, T* r, T* d, T* const work[4], std::vector<double>& history,
+                     const std::function<double()>* replace = nullptr,
+                     double replaceRatio = 0.0) {
   const View& v = ctx.view;
   const int64_t n = v.n;
   KrylovWork& w = ctx.kw;
@@ -280,6 +292,7 @@ InnerResult innerFcg(Context& ctx, const OpT<T>& op, double relTol, int64_t maxI
   int64_t launched = 0;
   int cur = 0;
   InnerResult res;
+  double lastTrue = 0.0;  // the FP64 residual norm at the last reliable update
   while (true) {
     for (int c = 0; c < chunk && launched < maxIter; ++c, ++launched) {
       const bool first = launched == 0;
@@ -299,6 +312,18 @@ InnerResult innerFcg(Context& ctx, const OpT<T>& op, double relTol, int64_t maxI
                                 ctx.stream));
     CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
     if (w.hostState->done || launched >= maxIter) break;
+    KState& hs = *w.hostState;
+    if (replace != nullptr && hs.rnorm <= replaceRatio * (lastTrue > 0.0 ? lastTrue : hs.b0)) {
+      const double prev = lastTrue > 0.0 ? lastTrue : hs.b0;
+      lastTrue = (*replace)();
+      ++res.replacements;
+      // the FP64 residual meets the target, or no longer falls (the FP32
+      // floor): the caller's FP64 test decides
+      if (lastTrue <= relTol * hs.b0 || !(lastTrue < 0.5 * prev)) {
+        res.replacedOut = true;
+        break;
+      }
+    }
   }
   const KState& s = *w.hostState;
   res.iterations = s.iter;
@@ -393,8 +418,11 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
   CFDHIP_CHECK(hipEventCreate(&t1));
   CFDHIP_CHECK(hipEventRecord(t0, ctx.stream));
   hipLaunchKernelGGL(zeroKernelD64, g, dim3(256), 0, ctx.stream, n, x);
-  // FP32 vectors: reduction per restart cycle (the settings' inner tolerance)
-  const double innerTol = set.inner_tolerance > 0.0 ? set.inner_tolerance : 1e-4;
+  // FP32 vectors: recurrence reduction between two reliable updates (the
+  // settings' inner tolerance). At 2048^2 the iteration count does not depend
+  // on it from 0.5 to 0.1 (20 against 16 with FP64 vectors) and grows below
+  // (22 at 0.05, 27 at 1e-4); the updates cost an FP64 residual each.
+  const double innerTol = set.inner_tolerance > 0.0 ? set.inner_tolerance : 0.1;
   auto outerResidual = [&]() {
     if (mixed)
       hipLaunchKernelGGL((outerResidualKernel<float>), g, dim3(256), 0, ctx.stream, v.nx, v.ny,
@@ -411,6 +439,13 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
     CFDHIP_CHECK(hipMemcpyAsync(&v0, normOut, sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
     CFDHIP_CHECK(hipStreamSynchronize(ctx.stream));
     return v0;
+  };
+  // a reliable update of the FP32 inner solve: d into x, r = b - A x in FP64
+  const std::function<double()> replace = [&]() {
+    hipLaunchKernelGGL((addCorrection<float>), g, dim3(256), 0, ctx.stream, n, d32, x);
+    hipLaunchKernelGGL((zeroT<float>), g, dim3(256), 0, ctx.stream, n, d32);
+    CFDHIP_CHECK(hipGetLastError());
+    return outerResidual();
   };
   std::vector<double> history;
   double b0 = 0.0, rTrue = 0.0;
@@ -441,12 +476,13 @@ cfdhip_solve_result learnedFcg2d(Context& ctx, const cfdhip_solver_settings& set
     const double need = std::max(set.relative_tolerance * b0, set.absolute_tolerance) / rTrue;
     const int64_t window = set.stagnation_window == 0 ? 200 : std::max(0, set.stagnation_window);
     // FP64 vectors aim at the final target (with a small margin for the
-    // recurrence drift); FP32 vectors restart every innerTol of reduction
-    const double tol = mixed ? std::max(innerTol, need) : std::max(0.5 * need, 1e-15);
+    // recurrence drift); FP32 vectors too, with a reliable update every
+    // innerTol of reduction
+    const double tol = std::max(0.5 * need, 1e-15);
     InnerResult ir;
     if (mixed)
       ir = innerFcg<float>(ctx, op32, tol, set.max_iterations - total, window, r32, d32, work32,
-                           history);
+                           history, &replace, innerTol);
     else
       ir = innerFcg<double>(ctx, op64, tol, set.max_iterations - total, window, r64, d64, work64,
                             history);
