@@ -470,6 +470,8 @@ def main():
                     help="only the batched-solve leg (for kernel profiles)")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="batched leg with stream launches instead of graph replay")
+    ap.add_argument("--c129-only", action="store_true",
+                    help="only the 129^2 cavity SIMPLE leg (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -493,6 +495,9 @@ def main():
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
+    if args.c129_only:
+        print(json.dumps(cavity129_leg(cfd, log, args.c129_iters)), flush=True)
+        return
     if args.batch_only:
         print(json.dumps(batch_leg(cfd, log, batch_cases(count=args.batch_cases),
                                    use_graph=not args.batch_no_graph, sequential=False)),
