@@ -13,8 +13,9 @@
 //    staged once in LDS with its one-pixel halo (plain, the nearest-upsampled
 //    concatenation of the decoder, or the fused first layer computed from
 //    the scaled solver residual), FP32 activations, BF16-rounded weights,
-//    FP32 accumulation; one thread computes a 2 x 2 pixel block, so the
-//    encoder's 2 x 2 average pool is thread-local in the epilogue;
+//    FP32 fused multiply-add accumulation; one thread computes a 2 x 2
+//    pixel block, so the encoder's 2 x 2 average pool is thread-local in the
+//    epilogue;
 //  * narrowLastKernel: the output layer, on the solver grid, with the
 //    preconditioner's scaling, pinned row and the (z, q) block partials.
 // Padding, physical extents, pooling at odd edges and the coarse tail are the
@@ -107,10 +108,14 @@ __device__ inline NarrowTile narrowTile(const NarrowArgs& a, int64_t t) {
     q.ph = a.ph;
     q.pw = a.pw;
     q.pad = a.pad;
-    q.x0 = static_cast<int>(t % a.tilesX) * kNTW;
-    const int64_t rest = t / a.tilesX;
-    q.y0 = static_cast<int>(rest % a.tilesY) * kNTH;
-    const int64_t b = rest / a.tilesY;
+    // 32-bit decoding (the launch checks tiles < 2^31): a 64-bit division is
+    // over a hundred scalar instructions per wave and tile
+    const uint32_t tt = static_cast<uint32_t>(t), tx = static_cast<uint32_t>(a.tilesX),
+                   ty = static_cast<uint32_t>(a.tilesY);
+    const uint32_t rest = tt / tx;
+    q.x0 = static_cast<int>(tt - rest * tx) * kNTW;
+    const uint32_t b = rest / ty;
+    q.y0 = static_cast<int>(rest - b * ty) * kNTH;
     const int64_t hw = static_cast<int64_t>(a.h) * a.w_, hw2 = hw / 4;
     q.a = a.a == nullptr ? nullptr : a.a + b * (G == kNConcat ? hw2 : hw) * CA;
     q.b = a.b == nullptr ? nullptr : a.b + b * hw * CB;
@@ -177,12 +182,14 @@ __device__ inline float padFactor(int& y, int& x, int h, int w, int pad) {
 }
 
 // Workgroup b runs on XCD b % 8 (round-robin dispatch over the 8 XCDs, each
-// with its own L2). With one workgroup per tile and a multiple of 8 tiles,
-// XCD x takes the x-th contiguous eighth of the row-major tile order, so a
-// tile's halo neighbours are mostly on its own XCD and the halo is fetched
-// from HBM once per L2, not once per neighbour.
+// with its own L2). Workgroup b takes tiles b, b + G, b + 2G, ... of a grid of
+// G workgroups; with G and the tile count multiples of 8, tile t then runs on
+// XCD t % 8, and mapping it to (t % 8) * (tiles / 8) + t / 8 gives XCD x the
+// x-th contiguous eighth of the row-major tile order, so a tile's halo
+// neighbours are mostly on its own XCD and the halo is fetched from HBM once
+// per L2, not once per neighbour.
 __device__ inline int64_t xcdTile(int64_t b, int64_t tiles) {
-  if (tiles % 8 != 0 || static_cast<int64_t>(gridDim.x) != tiles) return b;
+  if (tiles % 8 != 0 || gridDim.x % 8 != 0) return b;
   return (b % 8) * (tiles / 8) + b / 8;
 }
 
@@ -202,63 +209,36 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
     for (int i = threadIdx.x; i < 9 * CA; i += blockDim.x) w0s[i] = a.w0[i];
   const int64_t tiles =
       a.cases != nullptr ? a.tabCount : static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
-  for (int64_t tb = blockIdx.x; tb < tiles; tb += gridDim.x) {
-    const NarrowTile q = narrowTile<CA, CB, CO, G>(a, xcdTile(tb, tiles));
-    const int h = q.h, wd = q.w, x0 = q.x0, y0 = q.y0;
-    __syncthreads();  // the weights are in; the previous tile's readers are done
+  // A block walks its tiles with the next tile's global loads in flight
+  // while it computes the current one: fetch() issues a tile's loads into
+  // registers (the first layer: x = src * scale on rows y0 - 2 .. y0 + kNTH + 1
+  // padded at the physical image; otherwise the input tile with its halo),
+  // stage() writes them to LDS.
+  constexpr int XPER = (kNXR * kNXC + 255) / 256, PER = (kNHP + 255) / 256;
+  constexpr int NL = G == kNFirst ? XPER : PER, NC = G == kNFirst ? 1 : CI;
+  float v[NL][NC], sg[NL];
+  auto fetch = [&](const NarrowTile& q) {
     if constexpr (G == kNFirst) {
-      // x = src * scale on rows y0 - 2 .. y0 + kNTH + 1, padded at the physical image
       const TS* src = static_cast<const TS*>(q.src);
-      const int sh = q.srcH < h ? q.srcH : h, sw = q.srcW < wd ? q.srcW : wd;
-      constexpr int XPER = (kNXR * kNXC + 255) / 256;
-      float xv[XPER], xsg[XPER];
+      const int sh = q.srcH < q.h ? q.srcH : q.h, sw = q.srcW < q.w ? q.srcW : q.w;
 #pragma unroll
       for (int k = 0; k < XPER; ++k) {
         const int i = threadIdx.x + k * 256;
-        int yy = y0 - 2 + i / kNXC, xx = x0 - 2 + i % kNXC;
-        xsg[k] = padFactor(yy, xx, sh, sw, q.pad);
-        xv[k] = 0.f;
+        int yy = q.y0 - 2 + i / kNXC, xx = q.x0 - 2 + i % kNXC;
+        sg[k] = padFactor(yy, xx, sh, sw, q.pad);
+        v[k][0] = 0.f;
         if (i < kNXR * kNXC) {
           const int64_t si = q.srcRow0 + static_cast<int64_t>(yy) * q.srcW + xx;
           const float sv = static_cast<float>(src[si]);
-          xv[k] = q.scale != nullptr ? sv * q.scale[si] : sv;
+          v[k][0] = q.scale != nullptr ? sv * q.scale[si] : sv;
         }
-      }
-#pragma unroll
-      for (int k = 0; k < XPER; ++k) {
-        const int i = threadIdx.x + k * 256;
-        if (i < kNXR * kNXC) xs[i] = signed_(xsg[k], xv[k]);
-      }
-      __syncthreads();
-      // e0a on the halo tile; a halo pixel outside the image takes e0a of the
-      // nearest pixel inside times the padding's sign (its taps lie inside xs:
-      // the tile starts inside the physical image, whose rounded-up tensor
-      // extends it by at most 7 pixels)
-      for (int i = threadIdx.x; i < kNHP; i += blockDim.x) {
-        int yy = y0 - 1 + i / kNHC, xx = x0 - 1 + i % kNHC;
-        float acc[CA];
-#pragma unroll
-        for (int c = 0; c < CA; ++c) acc[c] = 0.f;
-        const float sg = padSign(yy, xx, q.ph, q.pw, q.pad);
-        if (sg != 0.f) {
-          const int ry = yy - y0 + 1, rx = xx - x0 + 1;
-#pragma unroll
-          for (int tap = 0; tap < 9; ++tap) {
-            const float v = xs[(ry + tap / 3) * kNXC + rx + tap % 3];
-#pragma unroll
-            for (int c = 0; c < CA; ++c) acc[c] += v * w0s[tap * CA + c];
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < CA; ++c) tile[i * CI + c] = sg * acc[c];
       }
     } else {
-      constexpr int PER = (kNHP + 255) / 256;
-      float v[PER][CI], sg[PER];
+      const int wd = q.w;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int i = threadIdx.x + k * 256;
-        int yy = y0 - 1 + i / kNHC, xx = x0 - 1 + i % kNHC;
+        int yy = q.y0 - 1 + i / kNHC, xx = q.x0 - 1 + i % kNHC;
         sg[k] = padFactor(yy, xx, q.ph, q.pw, q.pad);
 #pragma unroll
         for (int c = 0; c < CI; ++c) v[k][c] = 0.f;
@@ -272,40 +252,94 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
           }
         }
       }
+    }
+  };
+  auto stage = [&]() {
 #pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int i = threadIdx.x + k * 256;
-        if (i < kNHP) {
+    for (int k = 0; k < NL; ++k) {
+      const int i = threadIdx.x + k * 256;
+      if constexpr (G == kNFirst) {
+        if (i < kNXR * kNXC) xs[i] = signed_(sg[k], v[k][0]);
+      } else if (i < kNHP) {
 #pragma unroll
-          for (int c = 0; c < CI; ++c) tile[i * CI + c] = signed_(sg[k], v[k][c]);
+        for (int c = 0; c < CI; ++c) tile[i * CI + c] = signed_(sg[k], v[k][c]);
+      }
+    }
+  };
+  NarrowTile next;
+  if (blockIdx.x < tiles) {
+    next = narrowTile<CA, CB, CO, G>(a, xcdTile(blockIdx.x, tiles));
+    fetch(next);
+  }
+  for (int64_t tb = blockIdx.x; tb < tiles; tb += gridDim.x) {
+    const NarrowTile q = next;
+    const int h = q.h, wd = q.w, x0 = q.x0, y0 = q.y0;
+    __syncthreads();  // the weights are in; the previous tile's readers are done
+    stage();
+    if (tb + gridDim.x < tiles) {
+      next = narrowTile<CA, CB, CO, G>(a, xcdTile(tb + gridDim.x, tiles));
+      fetch(next);
+    }
+    if constexpr (G == kNFirst) {
+      __syncthreads();
+      // e0a on the halo tile; a halo pixel outside the image takes e0a of the
+      // nearest pixel inside times the padding's sign (its taps lie inside xs:
+      // the tile starts inside the physical image, whose rounded-up tensor
+      // extends it by at most 7 pixels)
+      for (int i = threadIdx.x; i < kNHP; i += blockDim.x) {
+        int yy = y0 - 1 + i / kNHC, xx = x0 - 1 + i % kNHC;
+        float acc[CA];
+#pragma unroll
+        for (int c = 0; c < CA; ++c) acc[c] = 0.f;
+        const float sgn = padSign(yy, xx, q.ph, q.pw, q.pad);
+        if (sgn != 0.f) {
+          const int ry = yy - y0 + 1, rx = xx - x0 + 1;
+#pragma unroll
+          for (int tap = 0; tap < 9; ++tap) {
+            const float xv = xs[(ry + tap / 3) * kNXC + rx + tap % 3];
+#pragma unroll
+            for (int c = 0; c < CA; ++c) acc[c] = fmaf(xv, w0s[tap * CA + c], acc[c]);
+          }
         }
+#pragma unroll
+        for (int c = 0; c < CA; ++c) tile[i * CI + c] = sgn * acc[c];
       }
     }
     __syncthreads();
-    // this thread's 2 x 2 output block, tap by tap (one tap's weights live at a time)
+    // this thread's 2 x 2 output block from its 4 x 4 input patch, read from
+    // LDS once (16 reads, not 36), one patch row at a time: row py feeds
+    // output row 0 with kernel row py and output row 1 with kernel row py - 1,
+    // so each output still adds its taps in (ky, kx, ci) order
     const int bx = threadIdx.x % kNBX, by = threadIdx.x / kNBX;
     float acc[4][CO];
 #pragma unroll
     for (int o = 0; o < 4; ++o)
 #pragma unroll
       for (int c = 0; c < CO; ++c) acc[o][c] = 0.f;
-    // not unrolled: an unrolled tap loop keeps every tap's weights in registers
+    // not unrolled: an unrolled row loop keeps every tap's weights in registers
 #pragma unroll 1
-    for (int tap = 0; tap < 9; ++tap) {
-      const int ky = tap / 3, kx = tap % 3;
-      float wv[CI * CO];
+    for (int py = 0; py < 4; ++py) {
+      float rv[4][CI];
+      const float* s = tile + ((2 * by + py) * kNHC + 2 * bx) * CI;
 #pragma unroll
-      for (int i = 0; i < CI * CO; ++i) wv[i] = ws[tap * CI * CO + i];
+      for (int px = 0; px < 4; ++px)
 #pragma unroll
-      for (int o = 0; o < 4; ++o) {
-        const float* px = tile + ((2 * by + (o >> 1) + ky) * kNHC + 2 * bx + (o & 1) + kx) * CI;
-        float v[CI];
+        for (int c = 0; c < CI; ++c) rv[px][c] = s[px * CI + c];
 #pragma unroll
-        for (int c = 0; c < CI; ++c) v[c] = px[c];
+      for (int oy = 0; oy < 2; ++oy) {
+        const int ky = py - oy;
+        if (ky < 0 || ky > 2) continue;
+        const float* wr = ws + ky * 3 * CI * CO;
 #pragma unroll
-        for (int ci = 0; ci < CI; ++ci)
+        for (int kx = 0; kx < 3; ++kx)
 #pragma unroll
-          for (int co = 0; co < CO; ++co) acc[o][co] += v[ci] * wv[ci * CO + co];
+          for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int ci = 0; ci < CI; ++ci)
+#pragma unroll
+              for (int co = 0; co < CO; ++co)
+                acc[oy * 2 + ox][co] =
+                    fmaf(rv[ox + kx][ci], wr[(kx * CI + ci) * CO + co], acc[oy * 2 + ox][co]);
       }
     }
     const int yb = y0 + 2 * by, xb = x0 + 2 * bx;
@@ -334,11 +368,39 @@ __global__ __launch_bounds__(256) void narrowConvKernel(NarrowArgs a) {
   }
 }
 
+// padFactor along one axis: the factor of (y, x) is the product of the
+// row's and the column's (both +-1 under Neumann padding, 0 or 1 under zero
+// padding), and c moves into [0, n).
+__device__ inline float padAxis(int& c, int n, int pad, int flipLo, int flipHi) {
+  const bool below = c < 0, above = c >= n;
+  float s;
+  if (pad & kPadNeumann)
+    s = (below && (pad & flipLo)) || (above && (pad & flipHi)) ? -1.f : 1.f;
+  else
+    s = below || above ? 0.f : 1.f;
+  c = min(max(c, 0), n - 1);
+  return s;
+}
+
+// Output rows per thread in narrowLastKernel (RB): the largest of these that
+// still gives every block of the launch a strip.
+inline int lastRows(int n, int ph, int pw, int64_t blocks) {
+  const int64_t segs = (pw + 255) / 256;
+  for (int rb : {8, 2})
+    if (static_cast<int64_t>(n) * ((ph + rb - 1) / rb) * segs >= blocks) return rb;
+  return 1;
+}
+
 // The output layer y = conv(d0, CW -> 1) at the n x ph x pw physical pixels
 // (CW: the stored channels of d0).
 // APPLY 0: y (compact n x ph x pw). APPLY 1 (n = 1, the solver grid): z =
 // scale * y, z[pin] = r[pin] scale[pin]^2 and block partials of (z, q).
-template <int CW, int APPLY, class TZ>
+// A unit of work is a strip of RB rows by 256 columns of one image, a thread
+// one column of it: the RB + 2 rows of d0 it reads are loaded at once (three
+// taps each), so each d0 row serves three output rows and every load is in
+// flight before the first is used. Units go XCD-contiguously to the blocks
+// (xcdTile), so the rows a strip shares with its neighbours meet in one L2.
+template <int CW, int APPLY, int RB, class TZ>
 __global__ __launch_bounds__(256) void narrowLastKernel(const float* __restrict__ d0,
                                                         const float* __restrict__ w, int n, int h,
                                                         int wd, int ph, int pw, int pad,
@@ -352,42 +414,56 @@ __global__ __launch_bounds__(256) void narrowLastKernel(const float* __restrict_
   if (threadIdx.x < 9 * CW) ws[threadIdx.x] = w[threadIdx.x];
   __syncthreads();
   double acc[1] = {0.0};
-  // a block walks whole rows (one division per row, none per pixel); the
-  // blocks of XCD x (b % 8 == x) take the x-th contiguous eighth of the rows,
-  // so the three rows a pixel reads are mostly in that XCD's L2
-  const int rows = n * ph;
-  const bool byXcd = gridDim.x % 8 == 0;
-  const int per = byXcd ? (rows + 7) / 8 : rows;
-  const int x8 = byXcd ? static_cast<int>(blockIdx.x % 8) : 0;
-  const int g8 = byXcd ? static_cast<int>(gridDim.x / 8) : static_cast<int>(gridDim.x);
-  const int lo = x8 * per, hi = min(rows, lo + per);
-  for (int row = lo + static_cast<int>(byXcd ? blockIdx.x / 8 : blockIdx.x); row < hi;
-       row += g8) {
-    const int b = row / ph, yy = row - b * ph;
-    for (int xx = threadIdx.x; xx < pw; xx += blockDim.x) {
-      const int64_t p = static_cast<int64_t>(row) * pw + xx;
-      float v[9][CW], sg[9];
+  const int strips = (ph + RB - 1) / RB, segs = (pw + 255) / 256;
+  const int64_t units = static_cast<int64_t>(n) * strips * segs;
+  for (int64_t ub = blockIdx.x; ub < units; ub += gridDim.x) {
+    const int64_t u = xcdTile(ub, units);
+    const int seg = static_cast<int>(u % segs);
+    const int64_t rest = u / segs;
+    const int y0 = static_cast<int>(rest % strips) * RB, b = static_cast<int>(rest / strips);
+    const int xx = seg * 256 + static_cast<int>(threadIdx.x);
+    if (xx >= pw) continue;
+    int cx[3];
+    float sx[3];
 #pragma unroll
-      for (int tap = 0; tap < 9; ++tap) {
-        int ty = yy + tap / 3 - 1, tx = xx + tap % 3 - 1;
-        sg[tap] = padFactor(ty, tx, ph, pw, pad);
-        Vec<CW>::load(d0 + ((static_cast<int64_t>(b) * h + ty) * wd + tx) * CW, v[tap]);
+    for (int k = 0; k < 3; ++k) {
+      cx[k] = xx + k - 1;
+      sx[k] = padAxis(cx[k], pw, pad, kPadLeft, kPadRight);
+    }
+    float v[RB + 2][3][CW], sg[RB + 2][3];
+#pragma unroll
+    for (int i = 0; i < RB + 2; ++i) {
+      int ty = y0 - 1 + i;
+      const float sy = padAxis(ty, ph, pad, kPadBottom, kPadTop);
+      const float* row = d0 + (static_cast<int64_t>(b) * h + ty) * wd * CW;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sg[i][k] = sy * sx[k];
+        Vec<CW>::load(row + cx[k] * CW, v[i][k]);
       }
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int yy = y0 + i;
+      if (yy >= ph) break;
       float o = 0.f;
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3;
         float dot = 0.f;
 #pragma unroll
-        for (int c = 0; c < CW; ++c) dot += signed_(sg[tap], v[tap][c]) * ws[tap * CW + c];
+        for (int c = 0; c < CW; ++c)
+          dot += signed_(sg[i + ky][kx], v[i + ky][kx][c]) * ws[tap * CW + c];
         o += dot;
       }
+      const int64_t p = (static_cast<int64_t>(b) * ph + yy) * pw + xx;
       if constexpr (APPLY == 0) {
         y[p] = o;
       } else {
-        TZ v = static_cast<TZ>(o * scale[p]);
-        if (p == pin) v = r[p] * static_cast<TZ>(scale[p]) * static_cast<TZ>(scale[p]);
-        z[p] = v;
-        if (q != nullptr) acc[0] += static_cast<double>(v) * static_cast<double>(q[p]);
+        TZ zv = static_cast<TZ>(o * scale[p]);
+        if (p == pin) zv = r[p] * static_cast<TZ>(scale[p]) * static_cast<TZ>(scale[p]);
+        z[p] = zv;
+        if (q != nullptr) acc[0] += static_cast<double>(zv) * static_cast<double>(q[p]);
       }
     }
   }
@@ -470,25 +546,46 @@ float roundBfHost(float f) {
 constexpr int kTensorChannels[kNumForward] = {16, 16, 16, 32, 32, 32, 64, 64, 64,
                                               64, 64, 64, 64, 32, 32, 16, 16};
 
+// Resident workgroups per CU the convolution launches use (at most what the
+// kernel's occupancy allows; 0: one workgroup per tile): each walks several
+// tiles, the next tile's loads in flight behind the current tile's
+// arithmetic. Measured at 2048^2 (learned FCG, 16 iterations): 8 (the
+// occupancy, 7 for the first layer) 5.68 ms per solve, one per tile 5.73,
+// 4 6.06, 2 6.73.
+constexpr int kConvBlocksPerCU = 8;
+
+// A convolution's launch: one workgroup per tile up to kConvBlocksPerCU per
+// CU, the grid a multiple of 8 (xcdTile).
+template <int CA, int CB, int CO, int G, bool POOL, class TS>
+void launchConv(hipStream_t s, int64_t tiles, const NarrowArgs& a) {
+  static const int64_t resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    CFDHIP_CHECK(hipGetDevice(&dev));
+    CFDHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CFDHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, narrowConvKernel<CA, CB, CO, G, POOL, TS>, 256, 0));
+    return static_cast<int64_t>(std::max(1, std::min(per, kConvBlocksPerCU))) * std::max(1, cus);
+  }();
+  int64_t g = kConvBlocksPerCU == 0 ? tiles : std::min<int64_t>(tiles, resident);
+  if (g < tiles && g >= 8) g -= g % 8;
+  hipLaunchKernelGGL((narrowConvKernel<CA, CB, CO, G, POOL, TS>),
+                     dim3(static_cast<unsigned>(std::max<int64_t>(1, g))), dim3(256), 0, s, a);
+}
+
 template <int CA, int CB, int CO>
-void launchCombo(hipStream_t s, int gather, bool pool, bool srcDouble, dim3 grid,
+void launchCombo(hipStream_t s, int gather, bool pool, bool srcDouble, int64_t tiles,
                  const NarrowArgs& a) {
   if (gather == kNFirst) {
     if (srcDouble)
-      hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNFirst, true, double>), grid, dim3(256), 0,
-                         s, a);
+      launchConv<CA, 0, CO, kNFirst, true, double>(s, tiles, a);
     else
-      hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNFirst, true, float>), grid, dim3(256), 0,
-                         s, a);
+      launchConv<CA, 0, CO, kNFirst, true, float>(s, tiles, a);
   } else if (gather == kNConcat) {
-    hipLaunchKernelGGL((narrowConvKernel<CA, CB, CO, kNConcat, false, float>), grid, dim3(256), 0,
-                       s, a);
+    launchConv<CA, CB, CO, kNConcat, false, float>(s, tiles, a);
   } else if (pool) {
-    hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNPlain, true, float>), grid, dim3(256), 0, s,
-                       a);
+    launchConv<CA, 0, CO, kNPlain, true, float>(s, tiles, a);
   } else {
-    hipLaunchKernelGGL((narrowConvKernel<CA, 0, CO, kNPlain, false, float>), grid, dim3(256), 0, s,
-                       a);
+    launchConv<CA, 0, CO, kNPlain, false, float>(s, tiles, a);
   }
 }
 
@@ -497,17 +594,17 @@ void launchNarrowConv(hipStream_t s, int ca, int cb, int co, int gather, bool po
                       bool srcDouble, const NarrowArgs& a) {
   const int64_t tiles =
       a.cases != nullptr ? a.tabCount : static_cast<int64_t>(a.tilesX) * a.tilesY * a.n;
-  const dim3 grid(static_cast<unsigned>(std::min<int64_t>(tiles, 1 << 20)));
+  require(tiles < (int64_t(1) << 31), "narrow convolution: too many tiles for one launch");
   const int key = (ca == 4 ? 4 : 0) | (cb == 4 ? 2 : 0) | (co == 4 ? 1 : 0);
   switch (key) {
-    case 0: launchCombo<2, 2, 2>(s, gather, pool, srcDouble, grid, a); break;
-    case 1: launchCombo<2, 2, 4>(s, gather, pool, srcDouble, grid, a); break;
-    case 2: launchCombo<2, 4, 2>(s, gather, pool, srcDouble, grid, a); break;
-    case 3: launchCombo<2, 4, 4>(s, gather, pool, srcDouble, grid, a); break;
-    case 4: launchCombo<4, 2, 2>(s, gather, pool, srcDouble, grid, a); break;
-    case 5: launchCombo<4, 2, 4>(s, gather, pool, srcDouble, grid, a); break;
-    case 6: launchCombo<4, 4, 2>(s, gather, pool, srcDouble, grid, a); break;
-    default: launchCombo<4, 4, 4>(s, gather, pool, srcDouble, grid, a); break;
+    case 0: launchCombo<2, 2, 2>(s, gather, pool, srcDouble, tiles, a); break;
+    case 1: launchCombo<2, 2, 4>(s, gather, pool, srcDouble, tiles, a); break;
+    case 2: launchCombo<2, 4, 2>(s, gather, pool, srcDouble, tiles, a); break;
+    case 3: launchCombo<2, 4, 4>(s, gather, pool, srcDouble, tiles, a); break;
+    case 4: launchCombo<4, 2, 2>(s, gather, pool, srcDouble, tiles, a); break;
+    case 5: launchCombo<4, 2, 4>(s, gather, pool, srcDouble, tiles, a); break;
+    case 6: launchCombo<4, 4, 2>(s, gather, pool, srcDouble, tiles, a); break;
+    default: launchCombo<4, 4, 4>(s, gather, pool, srcDouble, tiles, a); break;
   }
   CFDHIP_CHECK(hipGetLastError());
 }
@@ -718,32 +815,45 @@ void Unet::forwardNarrow(const void* src, bool srcDouble, const float* scale, in
   lastNarrow = true;
 }
 
-void Unet::outputNarrow(int n, int h, int w, int ph, int pw) {
-  const dim3 grid(static_cast<unsigned>(std::max(1, std::min(n * ph, 8192))));
-  const float* wo = nW.get() + narrow.wOff[kOut];
-  if (narrow.sw[kTd0] == 2)
-    hipLaunchKernelGGL((narrowLastKernel<2, 0, float>), grid, dim3(256), 0, stream, NT(kTd0), wo, n,
-                       h, w, ph, pw, padBits(), y.get(), nullptr, nullptr, nullptr, nullptr,
-                       int64_t(-1), nullptr);
-  else
-    hipLaunchKernelGGL((narrowLastKernel<4, 0, float>), grid, dim3(256), 0, stream, NT(kTd0), wo, n,
-                       h, w, ph, pw, padBits(), y.get(), nullptr, nullptr, nullptr, nullptr,
-                       int64_t(-1), nullptr);
+// The output layer's launch: RB from lastRows, CW from the stored channels of d0.
+template <int APPLY, class TZ>
+void launchLast(hipStream_t stream, dim3 grid, int rb, int cw, const float* d0, const float* w,
+                int n, int h, int wd, int ph, int pw, int pad, float* y, const float* scale,
+                const TZ* r, const TZ* q, TZ* z, int64_t pin, double* partial) {
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, d0, w, n, h, wd, ph, pw, pad, y, scale,
+                       r, q, z, pin, partial);
+  };
+  if (cw == 2) {
+    if (rb == 8) go(narrowLastKernel<2, APPLY, 8, TZ>);
+    else if (rb == 2) go(narrowLastKernel<2, APPLY, 2, TZ>);
+    else go(narrowLastKernel<2, APPLY, 1, TZ>);
+  } else {
+    if (rb == 8) go(narrowLastKernel<4, APPLY, 8, TZ>);
+    else if (rb == 2) go(narrowLastKernel<4, APPLY, 2, TZ>);
+    else go(narrowLastKernel<4, APPLY, 1, TZ>);
+  }
   CFDHIP_CHECK(hipGetLastError());
+}
+
+void Unet::outputNarrow(int n, int h, int w, int ph, int pw) {
+  // at most 8192 blocks, enough for 8 per CU; each block walks its units
+  const int rb = lastRows(n, ph, pw, 2048);
+  const int64_t units =
+      static_cast<int64_t>(n) * ((ph + rb - 1) / rb) * ((pw + 255) / 256);
+  const dim3 grid(static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(units, 8192))));
+  launchLast<0, float>(stream, grid, rb, narrow.sw[kTd0], NT(kTd0), nW.get() + narrow.wOff[kOut],
+                       n, h, w, ph, pw, padBits(), y.get(), nullptr, nullptr, nullptr, nullptr,
+                       int64_t(-1), nullptr);
 }
 
 template <class TZ>
 void Unet::outputApplyNarrow(const float* scale, const TZ* r, const TZ* q, TZ* z, int ny, int nx,
                              int64_t pin, int h, int w, double* partial, int parts) {
-  const float* wo = nW.get() + narrow.wOff[kOut];
-  const dim3 grid(static_cast<unsigned>(parts));
-  if (narrow.sw[kTd0] == 2)
-    hipLaunchKernelGGL((narrowLastKernel<2, 1, TZ>), grid, dim3(256), 0, stream, NT(kTd0), wo, 1, h,
-                       w, ny, nx, padBits(), nullptr, scale, r, q, z, pin, partial);
-  else
-    hipLaunchKernelGGL((narrowLastKernel<4, 1, TZ>), grid, dim3(256), 0, stream, NT(kTd0), wo, 1, h,
-                       w, ny, nx, padBits(), nullptr, scale, r, q, z, pin, partial);
-  CFDHIP_CHECK(hipGetLastError());
+  // one block per partial slot (the caller sums parts partials)
+  launchLast<1, TZ>(stream, dim3(static_cast<unsigned>(parts)), lastRows(1, ny, nx, parts),
+                    narrow.sw[kTd0], NT(kTd0), nW.get() + narrow.wOff[kOut], 1, h, w, ny, nx,
+                    padBits(), nullptr, scale, r, q, z, pin, partial);
 }
 template void Unet::outputApplyNarrow<float>(const float*, const float*, const float*, float*, int,
                                              int, int64_t, int, int, double*, int);
