@@ -100,9 +100,9 @@ constexpr int64_t kFusedMaxTiles = 256;
 // A large grid streams every vector through once per iteration: its FP64
 // stores bypass the caches (nontemporal), which leaves the L2 to the tiles'
 // shared halo rows (2048^2 FP64 iteration 100 -> 89 us). The FP32 iteration
-// of the mixed solve stores normally (nontemporal stores slowed it from 40 to
-// 61 us at 2048^2), and so does the fused small-grid kernel, whose next launch
-// reads the vectors back from the L2.
+// of the mixed solve stores normally (nontemporal stores made it slower at
+// 2048^2), and so does the fused small-grid kernel, whose next launch reads
+// the vectors back from the L2.
 template <bool NT, class V>
 __device__ inline void storeV(V* p, V v) {
   if constexpr (NT)
