@@ -11,7 +11,8 @@ with the oracle's network (oracle/unet.py forward_device) within the
 network tests' 1e-4 relative bound, and must not equal the MFMA output bit
 for bit (which would mean the narrow path did not run). Covered: the
 shipped weights with the coarse tail (two live channels: 2 stored per
-pixel), a three-channel network (4 stored per pixel), zero and Neumann
+pixel; at 1100 x 1100 the tail's first level runs pass by pass before the
+one-workgroup cycle), a three-channel network (4 stored per pixel), zero and Neumann
 padding, Dirichlet sides, odd image sides, a batch of two, and the FCG solve
 with either path.
 """
@@ -66,8 +67,10 @@ def _three_channel_weights(seed, mirror):
     return w
 
 
+# 1100 x 1100: the coarse tail's first level (69 x 69) is above the LDS cycle's
+# 4096 pixels and runs pass by pass over the chip before the cycle
 SHIPPED = [((1, 64, 96), ()), ((2, 32, 64), ()), ((1, 61, 45), ()), ((1, 129, 129), ()),
-           ((1, 100, 60), ("right",)), ((1, 37, 21), ("left", "bottom"))]
+           ((1, 100, 60), ("right",)), ((1, 37, 21), ("left", "bottom")), ((1, 1100, 1100), ())]
 
 
 @pytest.mark.parametrize("shape,sides", SHIPPED)
