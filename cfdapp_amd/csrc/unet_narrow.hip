@@ -23,6 +23,7 @@
 // linear map and differ only in rounding (the MFMA layers see activations
 // as a hi + lo BF16 pair, these kernels in FP32).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 
 #include "reduce.hpp"
@@ -555,18 +556,29 @@ constexpr int kTensorChannels[kNumForward] = {16, 16, 16, 32, 32, 32, 64, 64, 64
 constexpr int kConvBlocksPerCU = 8;
 
 // A convolution's launch: one workgroup per tile up to kConvBlocksPerCU per
-// CU, the grid a multiple of 8 (xcdTile).
+// CU, the grid a multiple of 8 (xcdTile). The resident count is taken from
+// the occupancy query once, outside any stream capture; a launch recorded
+// into a graph before that uses one workgroup per tile (the grid changes no
+// result: each tile's arithmetic is the same).
 template <int CA, int CB, int CO, int G, bool POOL, class TS>
 void launchConv(hipStream_t s, int64_t tiles, const NarrowArgs& a) {
-  static const int64_t resident = [] {
-    int dev = 0, cus = 0, per = 0;
-    CFDHIP_CHECK(hipGetDevice(&dev));
-    CFDHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    CFDHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per, narrowConvKernel<CA, CB, CO, G, POOL, TS>, 256, 0));
-    return static_cast<int64_t>(std::max(1, std::min(per, kConvBlocksPerCU))) * std::max(1, cus);
-  }();
-  int64_t g = kConvBlocksPerCU == 0 ? tiles : std::min<int64_t>(tiles, resident);
+  static std::atomic<int64_t> resident{0};
+  if (kConvBlocksPerCU > 0 && resident.load(std::memory_order_relaxed) == 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    CFDHIP_CHECK(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+      int dev = 0, cus = 0, per = 0;
+      CFDHIP_CHECK(hipGetDevice(&dev));
+      CFDHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      CFDHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per, narrowConvKernel<CA, CB, CO, G, POOL, TS>, 256, 0));
+      resident.store(static_cast<int64_t>(std::max(1, std::min(per, kConvBlocksPerCU))) *
+                         std::max(1, cus),
+                     std::memory_order_relaxed);
+    }
+  }
+  const int64_t r = resident.load(std::memory_order_relaxed);
+  int64_t g = r > 0 ? std::min<int64_t>(tiles, r) : tiles;
   if (g < tiles && g >= 8) g -= g % 8;
   hipLaunchKernelGGL((narrowConvKernel<CA, CB, CO, G, POOL, TS>),
                      dim3(static_cast<unsigned>(std::max<int64_t>(1, g))), dim3(256), 0, s, a);
