@@ -472,6 +472,8 @@ def main():
                     help="batched leg with stream launches instead of graph replay")
     ap.add_argument("--c129-only", action="store_true",
                     help="only the 129^2 cavity SIMPLE leg (for kernel profiles)")
+    ap.add_argument("--cube-only", action="store_true",
+                    help="only the 3D cavity leg, FP64 and mixed (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -497,6 +499,9 @@ def main():
 
     if args.c129_only:
         print(json.dumps(cavity129_leg(cfd, log, args.c129_iters)), flush=True)
+        return
+    if args.cube_only:
+        print(json.dumps(mixed3d_leg(cfd, log, args.cube, args.cube_iters)), flush=True)
         return
     if args.batch_only:
         print(json.dumps(batch_leg(cfd, log, batch_cases(count=args.batch_cases),
