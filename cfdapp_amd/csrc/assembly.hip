@@ -541,8 +541,8 @@ __global__ void imbalanceKernel(View v, const double* __restrict__ F, double* im
 // the sum of the imbalance partials in block order: the reference's serial
 // summation order, kept by thread 0 adding in sequence. The block first
 // loads each chunk of values into LDS (in parallel), so the additions wait on
-// LDS instead of on one memory round trip each (256^3: 94 ms -> well under
-// a millisecond).
+// LDS instead of on one memory round trip each (256^3: 94 ms per call
+// before).
 constexpr int kFluxThreads = 256, kFluxChunk = 4096;
 __global__ __launch_bounds__(kFluxThreads) void globalFluxKernel(View v,
                                                                 const double* __restrict__ F,
@@ -556,8 +556,18 @@ __global__ __launch_bounds__(kFluxThreads) void globalFluxKernel(View v,
       const int len = static_cast<int>(m - c0 < kFluxChunk ? m - c0 : kFluxChunk);
       for (int t = threadIdx.x; t < len; t += blockDim.x) buf[t] = at(c0 + t);
       __syncthreads();
-      if (threadIdx.x == 0)
-        for (int t = 0; t < len; ++t) sum += buf[t];
+      if (threadIdx.x == 0) {
+        // eight LDS reads in flight ahead of the (in-order) additions
+        int t = 0;
+        for (; t + 8 <= len; t += 8) {
+          double x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[u] = buf[t + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) sum += x[u];
+        }
+        for (; t < len; ++t) sum += buf[t];
+      }
       __syncthreads();
     }
   };
