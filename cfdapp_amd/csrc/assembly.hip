@@ -948,11 +948,21 @@ __global__ void absMaxKernel(int64_t n, const double* __restrict__ x, double* pa
   blockMax(c < n ? fabs(x[c]) : 0.0, partial);
 }
 
-__global__ void maxFinalKernel(const double* __restrict__ partial, int64_t parts, double* out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// max(0, partials) by one block of kBlock threads (a max is exact in any
+// order; fmax drops a NaN operand as the serial form does).
+__global__ __launch_bounds__(kBlock) void maxFinalKernel(const double* __restrict__ partial,
+                                                         int64_t parts, double* out) {
+  __shared__ double lds[kBlock / 64];
   double t = 0.0;
-  for (int64_t b = 0; b < parts; ++b) t = fmax(t, partial[b]);
-  out[0] = t;
+  for (int64_t b = threadIdx.x; b < parts; b += blockDim.x) t = fmax(t, partial[b]);
+  for (int o = 32; o > 0; o >>= 1) t = fmax(t, __shfl_xor(t, o, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) m = fmax(m, lds[w]);
+    out[0] = m;
+  }
 }
 
 // max over cells of the Courant number (FLUX) or of |IMBALANCE| (exact: a max
@@ -969,7 +979,7 @@ double launchMaxCell(Context& ctx, bool cfl, double dt) {
   else
     hipLaunchKernelGGL(absMaxKernel, grid, dim3(kBlock), 0, ctx.stream, v.n,
                        ctx.f(CFDHIP_FIELD_IMBALANCE), part.get());
-  hipLaunchKernelGGL(maxFinalKernel, dim3(1), dim3(64), 0, ctx.stream, part.get(),
+  hipLaunchKernelGGL(maxFinalKernel, dim3(1), dim3(kBlock), 0, ctx.stream, part.get(),
                      static_cast<int64_t>(grid.x), part.get() + grid.x);
   CFDHIP_CHECK(hipGetLastError());
   double out = 0.0;
